@@ -1,0 +1,230 @@
+/*
+ * tnalg_hip.h - C ABI of the MI355X (gfx950) finite-size DMRG kernels.
+ *
+ * The reference (ranshiju/T-Nalg) is pure Python: it has no FFI.  Each entry
+ * point below replaces a numpy/scipy routine on the path of
+ * dmrg_finite_size (algorithms/DMRG_anyH.py:18-101); the comment over it cites
+ * the reference function (file:line) whose arithmetic it takes over.  A
+ * Python ctypes binding of exactly these symbols lives in
+ * t-nalg_amd/tnalg_amd/_lib.py.
+ *
+ * Conventions
+ *  - All pointers are DEVICE pointers unless a parameter says "host".  The
+ *    library never allocates persistent memory: every buffer (operands,
+ *    descriptors, scratch) belongs to the caller.
+ *  - Matrices are row-major.  MPS tensors are (Dl, d, Dr) row-major; an
+ *    environment matrix is indexed [bra, ket].
+ *  - Every call is ordered on the HIP stream it is given (NULL = the default
+ *    stream) and returns 0 on success or a negative tnx_status; the message of
+ *    the last failure on the calling thread is tnx_last_error().
+ *  - Descriptor arrays are plain structs (layouts below are part of the ABI;
+ *    sizes are checked by tnx_abi_sizes()).
+ */
+#ifndef TNALG_HIP_H
+#define TNALG_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* tnx_stream_t; /* a hipStream_t */
+
+enum tnx_status {
+  TNX_OK = 0,
+  TNX_EINVAL = -1,   /* invalid argument (shape, dtype, count) */
+  TNX_EHIP = -2,     /* a HIP runtime call failed */
+  TNX_ENOCONV = -3,  /* eigensolver hit its restart limit (result still written) */
+};
+
+/* element types of operands */
+enum tnx_dtype { TNX_F64 = 0, TNX_F32 = 1, TNX_BF16 = 2 };
+
+/* GEMM precision modes: (operand type -> accumulate -> output type)       */
+enum tnx_gemm_mode {
+  TNX_GEMM_F64 = 0,       /* f64 x f64 -> MFMA f64 -> f64                   */
+  TNX_GEMM_F32 = 1,       /* f32 x f32 -> MFMA f32 -> f32                   */
+  TNX_GEMM_BF16_F32 = 2,  /* bf16 x bf16 -> MFMA bf16 (f32 acc) -> f32      */
+  TNX_GEMM_BF16_BF16 = 3, /* bf16 x bf16 -> MFMA bf16 (f32 acc) -> bf16     */
+};
+
+/* One K-segment of a grouped GEMM problem: C += alpha * op(A) * op(B).
+ * op(A) is M x K: A[i*lda+k] (transA=0) or A[k*lda+i] (transA=1).
+ * op(B) is K x N: B[k*ldb+j] (transB=0) or B[j*ldb+k] (transB=1).           */
+typedef struct tnx_gemm_seg {
+  uint64_t a, b;
+  int32_t lda, ldb;
+  int32_t transA, transB;
+  int32_t K, reserved;
+  double alpha;
+} tnx_gemm_seg; /* 48 bytes */
+
+/* One output matrix: C = sum_seg alpha*op(A)op(B) + beta*C0 (C0 may be 0). */
+typedef struct tnx_gemm_problem {
+  uint64_t c, c0;
+  int32_t M, N;
+  int32_t ldc, ldc0;
+  int32_t seg_begin, seg_count;
+  int32_t tile_begin;  /* prefix sum of 64x64 output tiles over problems */
+  int32_t active;      /* index into the active-flag array, or -1 */
+  double beta;
+} tnx_gemm_problem; /* 64 bytes */
+
+/* One output tensor of the elementwise physical-index mixer:
+ *   dst[a,s,b] (=|+=) sum_terms coef * sum_t O[s,t] src[a,t,b]
+ * over an (Dl, d, Dr) tensor; a D x D matrix is (D, 1, D); op < 0 means the
+ * identity.                                                                  */
+typedef struct tnx_mix_job {
+  uint64_t dst;
+  int32_t Dl, d, Dr;
+  int32_t dst_dtype;   /* tnx_dtype */
+  int32_t term_begin, term_count;
+  int32_t accumulate;  /* 1: add into dst */
+  int32_t active;      /* index into the active-flag array, or -1 */
+  int32_t block_begin; /* prefix sum of 1024-element blocks over jobs */
+  int32_t reserved;
+} tnx_mix_job; /* 48 bytes */
+
+typedef struct tnx_mix_term {
+  uint64_t src;
+  double coef;
+  int32_t op;          /* index into the d x d f64 operator table, or -1 */
+  int32_t src_dtype;   /* tnx_dtype */
+} tnx_mix_term; /* 24 bytes */
+
+/* The effective-Hamiltonian matvec of one local step of a batch of solves,
+ * expressed as   T_k = F_k . x   (GEMM 1, grouped),
+ *                U   = mix(T, x) (elementwise),
+ *                y   = U_0 + sum U_r . R_r^T  (GEMM 2, K-concatenated).
+ * The arrays are device descriptor arrays built by the host planner.       */
+typedef struct tnx_matvec_plan {
+  const tnx_gemm_problem* g1_prob; const tnx_gemm_seg* g1_seg;
+  int32_t g1_nprob, g1_ntiles;
+  const tnx_mix_job* mix_job; const tnx_mix_term* mix_term;
+  int32_t mix_njob, mix_nblocks;
+  const tnx_gemm_problem* g2_prob; const tnx_gemm_seg* g2_seg;
+  int32_t g2_nprob, g2_ntiles;
+  const double* ops;   /* operator table for the mixer */
+  int32_t mode;        /* tnx_gemm_mode of GEMM 1 and GEMM 2 (see below)   */
+  int32_t mode2;
+} tnx_matvec_plan;
+
+/* One eigenproblem of a batched Lanczos solve.                              */
+typedef struct tnx_lanczos_solve {
+  uint64_t V;          /* f64 Krylov basis, ncv rows of stride ldv          */
+  uint64_t w;          /* matvec output (f64 or f32, see w_dtype)           */
+  uint64_t x;          /* matvec input copy (tnx_dtype x_dtype)             */
+  uint64_t init;       /* f64 start vector (read at start)                  */
+  uint64_t out;        /* f64 result vector (written at the end)            */
+  int32_t n, ldv;      /* dimension, basis stride (elements)                */
+  int32_t m;           /* Krylov dimension used (<= ncv, <= n)              */
+  int32_t chunk_begin; /* prefix sum of 2048-element chunks over solves     */
+  double tol_scale;    /* converged when |r| <= tol_scale*|1 - tau*theta|   */
+  double tau;          /*   (ARPACK's test on I - tau*H, MPSClass.py:886)   */
+  double floor_rel;    /*   or |r| <= floor_rel*|theta| (precision floor)   */
+  double reserved;
+} tnx_lanczos_solve; /* 88 bytes */
+
+/* Per-solve results of tnx_lanczos_run (host-visible after the call).      */
+typedef struct tnx_lanczos_result {
+  double theta;        /* Ritz value (lowest eigenvalue of H)               */
+  double resid;        /* |beta_j y_j| at exit                              */
+  int32_t iters;       /* matvecs spent                                     */
+  int32_t converged;
+} tnx_lanczos_result; /* 24 bytes */
+
+/* ------------------------------------------------------------------------ */
+/* version / errors                                                          */
+int tnx_version(void);
+const char* tnx_last_error(void);
+/* writes sizeof of each ABI struct in declaration order into sizes[0..n)    */
+int tnx_abi_sizes(int32_t* sizes, int n);
+
+/* Grouped GEMM over descriptor arrays (device).  Replaces the numpy
+ * contractions of library/TensorBasicModule.py:489-613 (absorb_matrix2tensor,
+ * absorb_matrices2tensor[_full_fast]) and :632-721 (bound_vec_operator_left2right
+ * / right2left) as used by MPSClass.py:837-858 and MPSClass.py:555-611.
+ * ntiles = total 64x64 tiles; active = device int flags or NULL.            */
+int tnx_gemm_grouped(int mode, const tnx_gemm_problem* prob, int nprob,
+                     const tnx_gemm_seg* seg, int ntiles, const int32_t* active,
+                     tnx_stream_t stream);
+
+/* Elementwise physical-index mixer / linear combination / dtype cast.
+ * Replaces absorb_matrix2tensor(tensor, op.T, 1) (TensorBasicModule.py:489-526
+ * with bond=1) and the opt_env accumulation of classify_and_update_env
+ * (MPSClass.py:766-815).                                                     */
+int tnx_mix(const tnx_mix_job* job, int njob, const tnx_mix_term* term, int nblocks,
+            const double* ops, const int32_t* active, tnx_stream_t stream);
+
+/* One effective-Hamiltonian matvec (GEMM 1, mix, GEMM 2) for a batch.
+ * Replaces update_tensor_eigs_f_handle_optimized (MPSClass.py:837-858) without
+ * the "psi - tau*" shift.                                                    */
+int tnx_heff_matvec(const tnx_matvec_plan* plan /* host struct */, const int32_t* active,
+                    tnx_stream_t stream);
+
+/* Batched Lanczos eigensolver for the lowest eigenpair of each solve's Heff,
+ * with f64 basis, two-pass classical Gram-Schmidt reorthogonalisation,
+ * on-device tridiagonal eigensolve and restart with the Ritz vector; the
+ * matvec is `plan` (its x / w buffers are the solves' x / w).  Replaces
+ * scipy eigsh(LinearOperator(I - tau*Heff), k=1, which='LM', v0, tol) at
+ * MPSClass.py:883-889 (semantics of Eigs_Module_sjr.eigs_fh, EIG:18-58).
+ * solves: device array; state: device scratch of tnx_lanczos_state_bytes();
+ * results: host array of nsolve entries.                                    */
+size_t tnx_lanczos_state_bytes(int nsolve, int nchunks, int ncv);
+int tnx_lanczos_run(const tnx_matvec_plan* plan, const tnx_lanczos_solve* solves, int nsolve,
+                    int nchunks, int ncv, int max_restarts, int x_dtype, int w_dtype,
+                    void* state, tnx_lanczos_result* results, tnx_stream_t stream);
+/* As tnx_lanczos_run; the convergence test is applied only from Krylov step
+ * min_iters on (min_iters = ncv reproduces ARPACK's habit of completing its
+ * first ncv-step factorisation before testing).  The basis V of every solve
+ * needs ncv + 1 rows (row ncv is scratch).                                  */
+int tnx_lanczos_run_ex(const tnx_matvec_plan* plan, const tnx_lanczos_solve* solves, int nsolve,
+                       int nchunks, int ncv, int max_restarts, int min_iters, int x_dtype,
+                       int w_dtype, void* state, tnx_lanczos_result* results,
+                       tnx_stream_t stream);
+
+/* Batched f64 Householder QR gauge move.  For each matrix i, the logical
+ * matrix M (m x n) is src[r*ld+c] (trans=0) or src[c*ld+r] (trans=1); on
+ * return q holds the k=min(m,n) orthonormal columns (row-major m x k when
+ * q_trans=0, i.e. q[r*k+c]; column-major k x m rows when q_trans=1, i.e.
+ * q[c*m+r]) and r holds R (k x n row-major).  work: m*n f64 per matrix at
+ * work + i*work_stride.  Replaces left2right_decompose_tensor /
+ * right2left_decompose_tensor with way='qr' (TensorBasicModule.py:416-486).  */
+typedef struct tnx_qr_job {
+  uint64_t src, q, r, work;
+  int32_t m, n, ld, trans;
+  int32_t q_trans, reserved;
+} tnx_qr_job; /* 48 bytes */
+int tnx_qr_f64(const tnx_qr_job* jobs, int njob, tnx_stream_t stream);
+
+/* Batched f64 one-sided Jacobi (Hestenes) SVD of M (m x n, m >= n given by
+ * the same addressing as tnx_qr_job): u (m x n, layout as q above) holds the
+ * left singular vectors, s the singular values in descending order, vt
+ * (n x n row-major) the right ones scaled by s (i.e. diag(s) V^T, the matrix
+ * the reference absorbs).  Replaces np.linalg.svd in
+ * left2right_decompose_tensor / right2left_decompose_tensor with way='svd'
+ * (TensorBasicModule.py:438-444,474-480) as used by
+ * calculate_entanglement_spectrum (MPSClass.py:965-985).                    */
+typedef struct tnx_svd_job {
+  uint64_t src, u, s, svt, work;
+  int32_t m, n, ld, trans;
+  int32_t u_trans, reserved;
+} tnx_svd_job; /* 56 bytes */
+int tnx_svd_jacobi_f64(const tnx_svd_job* jobs, int njob, int max_sweeps, tnx_stream_t stream);
+
+/* Batched traces / Frobenius inner products: out[i] = sum_ab X_i[a,b] Y_i[a,b]
+ * (Y = NULL: trace of X).  Replaces the np.trace closings of observation_s1
+ * and observation_s1_s2 (MPSClass.py:1105-1157).                            */
+typedef struct tnx_dot_job {
+  uint64_t x, y;
+  int32_t rows, cols;
+  int32_t ldx, ldy;
+} tnx_dot_job; /* 32 bytes */
+int tnx_trace_dot(const tnx_dot_job* jobs, int njob, double* out, tnx_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TNALG_HIP_H */

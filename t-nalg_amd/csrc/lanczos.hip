@@ -1,0 +1,646 @@
+// Batched Lanczos eigensolver for the lowest eigenpair of every solve's
+// effective Hamiltonian (the local problem of one-site DMRG).
+//
+// All solves of a batch advance in lockstep over Krylov index j; a solve
+// that has converged is masked out (its active flag is 0) so the matvec and
+// the kernels below skip it.  Vectors are cut into 2048-element chunks, one
+// workgroup per chunk; per-solve reductions use the "last block reduces"
+// pattern (partials + fence + atomic ticket), so a Lanczos step is
+//   matvec (3 launches) + dots + update/dots + update/norm/tridiag + scale.
+// The Krylov basis and all reductions are f64 (two-pass classical
+// Gram-Schmidt against the full basis); the matvec may run in f32 or bf16.
+// The (j+1)x(j+1) tridiagonal eigenproblem is solved on the device by 65-way
+// multisection of the Sturm count (one wavefront) followed by inverse
+// iteration with partial pivoting.
+#include "tnx_common.h"
+
+namespace {
+
+constexpr int NCV_MAX = 32;
+constexpr int CHUNK = 2048;  // 256 lanes x 8 elements
+constexpr int NT = 256;
+
+struct LzState {
+  double alpha[NCV_MAX];
+  double beta[NCV_MAX];
+  double h1[NCV_MAX + 1];
+  double h2[NCV_MAX + 1];
+  double y[NCV_MAX];
+  double theta, resid, scale, spare;
+  int32_t converged, iters, jlast, counter;
+  int32_t restarts, pad[3];
+};
+
+struct LzLayout {
+  LzState* st;
+  int32_t* act;
+  double* part;
+  int32_t* nact;
+};
+
+__host__ __device__ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+__host__ __device__ inline LzLayout lz_layout(void* state, int nsolve, int nchunks) {
+  char* p = (char*)state;
+  LzLayout L;
+  L.st = (LzState*)p;
+  p += align256(sizeof(LzState) * (size_t)nsolve);
+  L.act = (int32_t*)p;
+  p += align256(sizeof(int32_t) * (size_t)nsolve);
+  L.part = (double*)p;
+  p += align256(sizeof(double) * (size_t)nchunks * (NCV_MAX + 1));
+  L.nact = (int32_t*)p;
+  return L;
+}
+
+struct ChunkInfo {
+  int s;        // solve index
+  int c;        // chunk within solve
+  int nch;      // chunks of this solve
+  int i0, i1;   // element range
+};
+
+__device__ __forceinline__ ChunkInfo chunk_of(const tnx_lanczos_solve* solves, int nsolve) {
+  ChunkInfo ci;
+  ci.s = tnx_find_owner(solves, sizeof(tnx_lanczos_solve), offsetof(tnx_lanczos_solve, chunk_begin),
+                        nsolve, blockIdx.x);
+  const int n = solves[ci.s].n;
+  ci.c = blockIdx.x - solves[ci.s].chunk_begin;
+  ci.nch = (n + CHUNK - 1) / CHUNK;
+  ci.i0 = ci.c * CHUNK;
+  ci.i1 = min(n, ci.i0 + CHUNK);
+  return ci;
+}
+
+// Wavefront sum of s into red[wave][k] (a block sum is red[0..3][k] after a
+// barrier).
+__device__ __forceinline__ void wave_to_lds(double s, int k, double (*red)[NCV_MAX + 1]) {
+  s = tnx_wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = s;
+}
+
+// Block-wide sum of one value per thread into red[0..3][0].
+__device__ __forceinline__ void block_sums(const double* v, int nv, double (*red)[NCV_MAX + 1]) {
+  (void)nv;
+  wave_to_lds(v[0], 0, red);
+  __syncthreads();
+}
+
+// Ticket: returns true in exactly one block per solve (the last to finish).
+__device__ __forceinline__ bool last_block(int32_t* counter, int nch) {
+  __shared__ int is_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int prev = atomicAdd(counter, 1);
+    is_last = (prev == nch - 1);
+  }
+  __syncthreads();
+  if (is_last) __threadfence();
+  return is_last != 0;
+}
+
+__device__ __forceinline__ double ld_part(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Reduce the partials of solve s for k in [0, nk) into dst (last block only).
+__device__ __forceinline__ void reduce_partials(const double* part, int chunk_begin, int nch,
+                                                int nk, double* dst) {
+  for (int k = threadIdx.x; k < nk; k += NT) {
+    double s = 0.0;
+    for (int c = 0; c < nch; ++c) s += ld_part(part + (int64_t)(chunk_begin + c) * (NCV_MAX + 1) + k);
+    dst[k] = s;
+  }
+}
+
+// Number of eigenvalues of the symmetric tridiagonal (a, b) below x.
+__device__ inline int sturm_count(const double* a, const double* b, int k, double x) {
+  const double tiny = 1e-300;
+  int cnt = 0;
+  double q = a[0] - x;
+  if (q < 0) cnt++;
+  for (int i = 1; i < k; ++i) {
+    if (fabs(q) < tiny) q = -tiny;
+    q = (a[i] - x) - b[i - 1] * b[i - 1] / q;
+    if (q < 0) cnt++;
+  }
+  return cnt;
+}
+
+// Lowest eigenpair of the k x k tridiagonal (a, b); called by wave 0 only;
+// lane 0 writes y[0..k) and *theta.
+__device__ void tridiag_lowest(const double* a, const double* b, int k, double* y,
+                               double* theta) {
+  const int lane = threadIdx.x & 63;
+  if (k == 1) {
+    if (lane == 0) { y[0] = 1.0; *theta = a[0]; }
+    return;
+  }
+  double lo = a[0], hi = a[0];
+  for (int i = 0; i < k; ++i) {
+    double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < k - 1 ? fabs(b[i]) : 0.0);
+    lo = fmin(lo, a[i] - r);
+    hi = fmax(hi, a[i] + r);
+  }
+  const double span = fmax(hi - lo, 1e-300);
+  lo -= 1e-12 * span;
+  hi += 1e-12 * span;
+  for (int round = 0; round < 16; ++round) {
+    const double w = hi - lo;
+    if (w <= 4.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)) + 1e-300) break;
+    const double x = lo + w * (double)(lane + 1) / 65.0;
+    const int cnt = sturm_count(a, b, k, x);
+    const unsigned long long m = __ballot(cnt >= 1);
+    if (m == 0ull) {
+      lo = lo + w * 64.0 / 65.0;
+    } else {
+      const int f = __ffsll((long long)m) - 1;
+      const double xf = lo + w * (double)(f + 1) / 65.0;
+      const double xp = lo + w * (double)f / 65.0;
+      hi = xf;
+      lo = xp;
+    }
+  }
+  if (lane != 0) return;
+  const double th = 0.5 * (lo + hi);
+  *theta = th;
+  // inverse iteration with partial pivoting on (T - th I)
+  double z[NCV_MAX], rhs[NCV_MAX];
+  double u0[NCV_MAX], u1[NCV_MAX], u2[NCV_MAX];
+  const double tiny = 2.220446049250313e-16 * span;
+  for (int i = 0; i < k; ++i) z[i] = 1.0 / sqrt((double)k);
+  for (int it = 0; it < 3; ++it) {
+    for (int i = 0; i < k; ++i) rhs[i] = z[i];
+    // current pivot-row candidate (cols i, i+1, i+2)
+    double p0 = a[0] - th, p1 = b[0], p2 = 0.0, pr = rhs[0];
+    for (int i = 0; i < k - 1; ++i) {
+      const double n0 = b[i], n1 = a[i + 1] - th, n2 = (i + 1 < k - 1) ? b[i + 1] : 0.0;
+      const double nr = rhs[i + 1];
+      if (fabs(p0) >= fabs(n0)) {
+        if (fabs(p0) < tiny) p0 = (p0 < 0 ? -tiny : tiny);
+        const double f = n0 / p0;
+        u0[i] = p0; u1[i] = p1; u2[i] = p2; rhs[i] = pr;
+        p0 = n1 - f * p1; p1 = n2 - f * p2; p2 = 0.0; pr = nr - f * pr;
+      } else {
+        const double f = p0 / n0;
+        u0[i] = n0; u1[i] = n1; u2[i] = n2; rhs[i] = nr;
+        const double q0 = p1 - f * n1, q1 = p2 - f * n2;
+        p0 = q0; p1 = q1; p2 = 0.0; pr = pr - f * nr;
+      }
+    }
+    if (fabs(p0) < tiny) p0 = (p0 < 0 ? -tiny : tiny);
+    u0[k - 1] = p0; u1[k - 1] = 0.0; u2[k - 1] = 0.0; rhs[k - 1] = pr;
+    for (int i = k - 1; i >= 0; --i) {
+      double s = rhs[i];
+      if (i + 1 < k) s -= u1[i] * z[i + 1];
+      if (i + 2 < k) s -= u2[i] * z[i + 2];
+      z[i] = s / u0[i];
+    }
+    double nrm = 0.0;
+    for (int i = 0; i < k; ++i) nrm += z[i] * z[i];
+    nrm = 1.0 / sqrt(nrm);
+    for (int i = 0; i < k; ++i) z[i] *= nrm;
+  }
+  for (int i = 0; i < k; ++i) y[i] = z[i];
+}
+
+template <typename T> __device__ __forceinline__ void store_x(void* x, int dtype, int64_t i, double v) {
+  tnx_store(x, dtype, i, v);
+}
+
+// ---------------------------------------------------------------- kernels
+
+// ||init||^2 -> scale; state reset; act = 1
+__global__ __launch_bounds__(NT) void k_init_norm(const tnx_lanczos_solve* __restrict__ S, int nsolve,
+                                                  void* state, int nchunks) {
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  ChunkInfo ci = chunk_of(S, nsolve);
+  const tnx_lanczos_solve D = S[ci.s];
+  const double* v = (const double*)D.init;
+  double acc = 0.0;
+  for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) acc += v[i] * v[i];
+  __shared__ double red[4][NCV_MAX + 1];
+  block_sums(&acc, 1, red);
+  if (threadIdx.x == 0) L.part[(int64_t)blockIdx.x * (NCV_MAX + 1)] = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+  if (last_block(&L.st[ci.s].counter, ci.nch)) {
+    __shared__ double tot;
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int c = 0; c < ci.nch; ++c) s += ld_part(L.part + (int64_t)(D.chunk_begin + c) * (NCV_MAX + 1));
+      tot = s;
+      LzState& st = L.st[ci.s];
+      st.scale = s > 0.0 ? 1.0 / sqrt(s) : -1.0;
+      st.converged = 0;
+      st.iters = 0;
+      st.jlast = 0;
+      st.restarts = 0;
+      st.theta = 0.0;
+      st.resid = 0.0;
+      st.y[0] = 1.0;
+      for (int k = 1; k < NCV_MAX; ++k) st.y[k] = 0.0;
+      L.act[ci.s] = 1;
+      st.counter = 0;
+    }
+  }
+}
+
+// V[0] = init * scale (or a flat vector when init == 0); x = cast(V[0])
+__global__ __launch_bounds__(NT) void k_init_scale(const tnx_lanczos_solve* __restrict__ S, int nsolve,
+                                                   void* state, int nchunks, int x_dtype) {
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  ChunkInfo ci = chunk_of(S, nsolve);
+  const tnx_lanczos_solve D = S[ci.s];
+  const double sc = L.st[ci.s].scale;
+  const double* v = (const double*)D.init;
+  double* V0 = (double*)D.V;
+  const double flat = 1.0 / sqrt((double)D.n);
+  for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
+    double val = sc > 0.0 ? v[i] * sc : flat;
+    V0[i] = val;
+    tnx_store((void*)D.x, x_dtype, i, val);
+  }
+}
+
+// pass 1: h1[k] = V[k] . w, k <= j
+__global__ __launch_bounds__(NT) void k_dots(const tnx_lanczos_solve* __restrict__ S, int nsolve,
+                                             void* state, int nchunks, int j, int w_dtype) {
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  ChunkInfo ci = chunk_of(S, nsolve);
+  if (L.act[ci.s] == 0) return;
+  const tnx_lanczos_solve D = S[ci.s];
+  double w[CHUNK / NT];
+#pragma unroll
+  for (int q = 0; q < CHUNK / NT; ++q) {
+    int i = ci.i0 + q * NT + threadIdx.x;
+    w[q] = (i < ci.i1) ? tnx_load((const void*)D.w, w_dtype, i) : 0.0;
+  }
+  const double* V = (const double*)D.V;
+  __shared__ double red[4][NCV_MAX + 1];
+  for (int k = 0; k <= j; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < CHUNK / NT; ++q) {
+      int i = ci.i0 + q * NT + threadIdx.x;
+      if (i < ci.i1) s += V[(int64_t)k * D.ldv + i] * w[q];
+    }
+    wave_to_lds(s, k, red);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k <= j; k += NT)
+    L.part[(int64_t)blockIdx.x * (NCV_MAX + 1) + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+  if (last_block(&L.st[ci.s].counter, ci.nch)) {
+    reduce_partials(L.part, D.chunk_begin, ci.nch, j + 1, L.st[ci.s].h1);
+    __syncthreads();
+    if (threadIdx.x == 0) L.st[ci.s].counter = 0;
+  }
+}
+
+// r = w - sum h1[k] V[k] -> V[j+1]; pass 2: h2[k] = V[k] . r
+__global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __restrict__ S, int nsolve,
+                                                    void* state, int nchunks, int j, int w_dtype) {
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  ChunkInfo ci = chunk_of(S, nsolve);
+  if (L.act[ci.s] == 0) return;
+  const tnx_lanczos_solve D = S[ci.s];
+  __shared__ double h[NCV_MAX + 1];
+  for (int k = threadIdx.x; k <= j; k += NT) h[k] = L.st[ci.s].h1[k];
+  __syncthreads();
+  const double* V = (const double*)D.V;
+  double r[CHUNK / NT];
+#pragma unroll
+  for (int q = 0; q < CHUNK / NT; ++q) {
+    int i = ci.i0 + q * NT + threadIdx.x;
+    r[q] = (i < ci.i1) ? tnx_load((const void*)D.w, w_dtype, i) : 0.0;
+  }
+  for (int k = 0; k <= j; ++k) {
+    const double hk = h[k];
+#pragma unroll
+    for (int q = 0; q < CHUNK / NT; ++q) {
+      int i = ci.i0 + q * NT + threadIdx.x;
+      if (i < ci.i1) r[q] -= hk * V[(int64_t)k * D.ldv + i];
+    }
+  }
+  double* Vn = (double*)D.V + (int64_t)(j + 1) * D.ldv;
+#pragma unroll
+  for (int q = 0; q < CHUNK / NT; ++q) {
+    int i = ci.i0 + q * NT + threadIdx.x;
+    if (i < ci.i1) Vn[i] = r[q];
+  }
+  __shared__ double red[4][NCV_MAX + 1];
+  for (int k = 0; k <= j; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < CHUNK / NT; ++q) {
+      int i = ci.i0 + q * NT + threadIdx.x;
+      if (i < ci.i1) s += V[(int64_t)k * D.ldv + i] * r[q];
+    }
+    wave_to_lds(s, k, red);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k <= j; k += NT)
+    L.part[(int64_t)blockIdx.x * (NCV_MAX + 1) + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+  if (last_block(&L.st[ci.s].counter, ci.nch)) {
+    reduce_partials(L.part, D.chunk_begin, ci.nch, j + 1, L.st[ci.s].h2);
+    __syncthreads();
+    if (threadIdx.x == 0) L.st[ci.s].counter = 0;
+  }
+}
+
+// r -= sum h2[k] V[k]; beta = ||r||; alpha_j = h1[j] + h2[j]; tridiagonal
+// eigenpair; convergence test (last block)
+__global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __restrict__ S, int nsolve,
+                                                    void* state, int nchunks, int j, int min_iters) {
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  ChunkInfo ci = chunk_of(S, nsolve);
+  if (L.act[ci.s] == 0) return;
+  const tnx_lanczos_solve D = S[ci.s];
+  __shared__ double h[NCV_MAX + 1];
+  for (int k = threadIdx.x; k <= j; k += NT) h[k] = L.st[ci.s].h2[k];
+  __syncthreads();
+  const double* V = (const double*)D.V;
+  double* Vn = (double*)D.V + (int64_t)(j + 1) * D.ldv;
+  double r[CHUNK / NT];
+#pragma unroll
+  for (int q = 0; q < CHUNK / NT; ++q) {
+    int i = ci.i0 + q * NT + threadIdx.x;
+    r[q] = (i < ci.i1) ? Vn[i] : 0.0;
+  }
+  for (int k = 0; k <= j; ++k) {
+    const double hk = h[k];
+#pragma unroll
+    for (int q = 0; q < CHUNK / NT; ++q) {
+      int i = ci.i0 + q * NT + threadIdx.x;
+      if (i < ci.i1) r[q] -= hk * V[(int64_t)k * D.ldv + i];
+    }
+  }
+  double nn = 0.0;
+#pragma unroll
+  for (int q = 0; q < CHUNK / NT; ++q) {
+    int i = ci.i0 + q * NT + threadIdx.x;
+    if (i < ci.i1) {
+      Vn[i] = r[q];
+      nn += r[q] * r[q];
+    }
+  }
+  __shared__ double red[4][NCV_MAX + 1];
+  block_sums(&nn, 1, red);
+  if (threadIdx.x == 0)
+    L.part[(int64_t)blockIdx.x * (NCV_MAX + 1)] = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+  if (last_block(&L.st[ci.s].counter, ci.nch)) {
+    LzState& st = L.st[ci.s];
+    __shared__ double a_s[NCV_MAX], b_s[NCV_MAX], y_s[NCV_MAX], th_s;
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int c = 0; c < ci.nch; ++c) s += ld_part(L.part + (int64_t)(D.chunk_begin + c) * (NCV_MAX + 1));
+      const double beta = sqrt(s);
+      st.alpha[j] = st.h1[j] + st.h2[j];
+      st.beta[j] = beta;
+      st.iters += 1;
+      for (int k = 0; k <= j; ++k) {
+        a_s[k] = st.alpha[k];
+        b_s[k] = st.beta[k];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) tridiag_lowest(a_s, b_s, j + 1, y_s, &th_s);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double beta = b_s[j];
+      const double th = th_s;
+      const double res = fabs(beta * y_s[j]);
+      for (int k = 0; k <= j; ++k) st.y[k] = y_s[k];
+      st.theta = th;
+      st.resid = res;
+      st.jlast = j;
+      const double thr = fmax(D.tol_scale * fabs(1.0 - D.tau * th), D.floor_rel * fabs(th));
+      double anorm = 0.0;
+      for (int k = 0; k <= j; ++k) anorm = fmax(anorm, fabs(a_s[k]) + (k > 0 ? fabs(b_s[k - 1]) : 0.0));
+      const bool breakdown = beta <= 1e-14 * fmax(anorm, 1e-300);
+      const bool full = (j + 1 >= D.n);
+      if (full || breakdown || ((j + 1 >= min_iters) && res <= thr)) {
+        st.converged = 1;
+        L.act[ci.s] = 0;
+        atomicSub(L.nact, 1);
+      }
+      st.counter = 0;
+    }
+  }
+}
+
+// V[j+1] /= beta_j; x = cast(V[j+1])
+__global__ __launch_bounds__(NT) void k_normalize(const tnx_lanczos_solve* __restrict__ S, int nsolve,
+                                                  void* state, int nchunks, int j, int x_dtype) {
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  ChunkInfo ci = chunk_of(S, nsolve);
+  if (L.act[ci.s] == 0) return;
+  const tnx_lanczos_solve D = S[ci.s];
+  const double inv = 1.0 / L.st[ci.s].beta[j];
+  double* Vn = (double*)D.V + (int64_t)(j + 1) * D.ldv;
+  for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
+    double v = Vn[i] * inv;
+    Vn[i] = v;
+    tnx_store((void*)D.x, x_dtype, i, v);
+  }
+}
+
+// Ritz vector of the active solves into the spare row V[m]; its norm
+__global__ __launch_bounds__(NT) void k_ritz(const tnx_lanczos_solve* __restrict__ S, int nsolve,
+                                             void* state, int nchunks, int m_rows) {
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  ChunkInfo ci = chunk_of(S, nsolve);
+  if (L.act[ci.s] == 0) return;
+  const tnx_lanczos_solve D = S[ci.s];
+  const LzState& st = L.st[ci.s];
+  const int kk = st.jlast + 1;
+  const double* V = (const double*)D.V;
+  double* X = (double*)D.V + (int64_t)m_rows * D.ldv;
+  double nn = 0.0;
+  for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
+    double s = 0.0;
+    for (int k = 0; k < kk; ++k) s += st.y[k] * V[(int64_t)k * D.ldv + i];
+    X[i] = s;
+    nn += s * s;
+  }
+  __shared__ double red[4][NCV_MAX + 1];
+  block_sums(&nn, 1, red);
+  if (threadIdx.x == 0)
+    L.part[(int64_t)blockIdx.x * (NCV_MAX + 1)] = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+  if (last_block(&L.st[ci.s].counter, ci.nch)) {
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int c = 0; c < ci.nch; ++c) s += ld_part(L.part + (int64_t)(D.chunk_begin + c) * (NCV_MAX + 1));
+      L.st[ci.s].scale = 1.0 / sqrt(s);
+      L.st[ci.s].restarts += 1;
+      L.st[ci.s].counter = 0;
+    }
+  }
+}
+
+// restart: V[0] = V[m] * scale; x = cast(V[0])
+__global__ __launch_bounds__(NT) void k_restart(const tnx_lanczos_solve* __restrict__ S, int nsolve,
+                                                void* state, int nchunks, int m_rows, int x_dtype) {
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  ChunkInfo ci = chunk_of(S, nsolve);
+  if (L.act[ci.s] == 0) return;
+  const tnx_lanczos_solve D = S[ci.s];
+  const double sc = L.st[ci.s].scale;
+  const double* X = (const double*)D.V + (int64_t)m_rows * D.ldv;
+  double* V0 = (double*)D.V;
+  for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
+    double v = X[i] * sc;
+    V0[i] = v;
+    tnx_store((void*)D.x, x_dtype, i, v);
+  }
+}
+
+// out = sum_k y[k] V[k] for every solve
+__global__ __launch_bounds__(NT) void k_final(const tnx_lanczos_solve* __restrict__ S, int nsolve,
+                                              void* state, int nchunks) {
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  ChunkInfo ci = chunk_of(S, nsolve);
+  const tnx_lanczos_solve D = S[ci.s];
+  const LzState& st = L.st[ci.s];
+  const int kk = st.jlast + 1;
+  const double* V = (const double*)D.V;
+  double* out = (double*)D.out;
+  for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
+    double s = 0.0;
+    for (int k = 0; k < kk; ++k) s += st.y[k] * V[(int64_t)k * D.ldv + i];
+    out[i] = s;
+  }
+}
+
+__global__ void k_set_nact(void* state, int nsolve, int nchunks) {
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  if (threadIdx.x == 0 && blockIdx.x == 0) *L.nact = nsolve;
+}
+
+__global__ void k_results(void* state, int nsolve, int nchunks, tnx_lanczos_result* res) {
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nsolve) {
+    res[s].theta = L.st[s].theta;
+    res[s].resid = L.st[s].resid;
+    res[s].iters = L.st[s].iters;
+    res[s].converged = L.st[s].converged;
+  }
+}
+
+// pinned staging for the polled active count and the results
+struct Pinned {
+  int32_t* nact = nullptr;
+  tnx_lanczos_result* res = nullptr;
+  int res_cap = 0;
+};
+thread_local Pinned g_pin;
+
+int ensure_pinned(int nsolve) {
+  if (g_pin.nact == nullptr) {
+    TNX_CHECK_HIP(hipHostMalloc((void**)&g_pin.nact, 64 * sizeof(int32_t), hipHostMallocDefault));
+  }
+  if (g_pin.res_cap < nsolve) {
+    if (g_pin.res) (void)hipHostFree(g_pin.res);
+    g_pin.res = nullptr;
+    int cap = nsolve < 256 ? 256 : nsolve;
+    TNX_CHECK_HIP(hipHostMalloc((void**)&g_pin.res, sizeof(tnx_lanczos_result) * cap, hipHostMallocDefault));
+    g_pin.res_cap = cap;
+  }
+  return TNX_OK;
+}
+
+}  // namespace
+
+extern "C" size_t tnx_lanczos_state_bytes(int nsolve, int nchunks, int ncv) {
+  (void)ncv;
+  return align256(sizeof(LzState) * (size_t)nsolve) + align256(sizeof(int32_t) * (size_t)nsolve) +
+         align256(sizeof(double) * (size_t)nchunks * (NCV_MAX + 1)) + 256;
+}
+
+extern "C" int tnx_lanczos_run_ex(const tnx_matvec_plan* plan, const tnx_lanczos_solve* solves,
+                                  int nsolve, int nchunks, int ncv, int max_restarts,
+                                  int min_iters, int x_dtype, int w_dtype, void* state,
+                                  tnx_lanczos_result* results, tnx_stream_t stream) {
+  if (plan == nullptr || solves == nullptr || state == nullptr || nsolve <= 0 || nchunks <= 0 ||
+      ncv < 2 || ncv > NCV_MAX || max_restarts < 0) {
+    tnx_set_error("tnx_lanczos_run: bad arguments (nsolve=%d nchunks=%d ncv=%d)", nsolve, nchunks,
+                  ncv);
+    return TNX_EINVAL;
+  }
+  if (int rc = ensure_pinned(nsolve)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  const dim3 grid(nchunks), block(NT);
+  TNX_CHECK_HIP(hipMemsetAsync(L.st, 0, sizeof(LzState) * (size_t)nsolve, s));
+  hipLaunchKernelGGL(k_set_nact, dim3(1), dim3(64), 0, s, state, nsolve, nchunks);
+  hipLaunchKernelGGL(k_init_norm, grid, block, 0, s, solves, nsolve, state, nchunks);
+  hipLaunchKernelGGL(k_init_scale, grid, block, 0, s, solves, nsolve, state, nchunks, x_dtype);
+  TNX_CHECK_LAUNCH();
+
+  hipEvent_t ev[2];
+  TNX_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  TNX_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  volatile int32_t* pin = g_pin.nact;
+  pin[0] = nsolve;
+  pin[1] = nsolve;
+  bool done = false;
+  int rc = TNX_OK;
+  int issued = 0;  // events recorded so far
+  for (int r = 0; r <= max_restarts && !done; ++r) {
+    if (r > 0) {
+      hipLaunchKernelGGL(k_ritz, grid, block, 0, s, solves, nsolve, state, nchunks, ncv);
+      hipLaunchKernelGGL(k_restart, grid, block, 0, s, solves, nsolve, state, nchunks, ncv, x_dtype);
+    }
+    for (int j = 0; j < ncv && !done; ++j) {
+      if ((rc = tnx_heff_matvec(plan, L.act, stream))) break;
+      hipLaunchKernelGGL(k_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
+      hipLaunchKernelGGL(k_update_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
+      hipLaunchKernelGGL(k_update_norm, grid, block, 0, s, solves, nsolve, state, nchunks, j,
+                         min_iters);
+      if (j + 1 < ncv)
+        hipLaunchKernelGGL(k_normalize, grid, block, 0, s, solves, nsolve, state, nchunks, j, x_dtype);
+      TNX_CHECK_LAUNCH();
+      // lagged poll: copy the active count of this step, then wait for the
+      // previous step's copy (the device keeps one step of work queued)
+      const int slot = issued & 1;
+      TNX_CHECK_HIP(hipMemcpyAsync((void*)(pin + slot), L.nact, sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, s));
+      TNX_CHECK_HIP(hipEventRecord(ev[slot], s));
+      ++issued;
+      if (issued >= 2) {
+        const int prev = (issued - 2) & 1;
+        TNX_CHECK_HIP(hipEventSynchronize(ev[prev]));
+        if (pin[prev] <= 0) done = true;
+      }
+    }
+    if (rc) break;
+    if (!done) {
+      // end of a Krylov cycle: make the restart decision on fresh data
+      const int slot = (issued - 1) & 1;
+      TNX_CHECK_HIP(hipEventSynchronize(ev[slot]));
+      if (pin[slot] <= 0) done = true;
+    }
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_final, grid, block, 0, s, solves, nsolve, state, nchunks);
+  hipLaunchKernelGGL(k_results, dim3((nsolve + 63) / 64), dim3(64), 0, s, state, nsolve, nchunks,
+                     g_pin.res);
+  TNX_CHECK_LAUNCH();
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  int nconv = 0;
+  for (int i = 0; i < nsolve; ++i) {
+    if (results) results[i] = g_pin.res[i];
+    nconv += g_pin.res[i].converged;
+  }
+  return nconv == nsolve ? TNX_OK : TNX_ENOCONV;
+}
+
+extern "C" int tnx_lanczos_run(const tnx_matvec_plan* plan, const tnx_lanczos_solve* solves,
+                               int nsolve, int nchunks, int ncv, int max_restarts, int x_dtype,
+                               int w_dtype, void* state, tnx_lanczos_result* results,
+                               tnx_stream_t stream) {
+  return tnx_lanczos_run_ex(plan, solves, nsolve, nchunks, ncv, max_restarts, 1, x_dtype, w_dtype,
+                            state, results, stream);
+}

@@ -1,0 +1,143 @@
+// Elementwise kernels of the DMRG step: the physical-index mixer
+//   dst[a,s,b] (=|+=) sum_terms coef * sum_t O[s,t] src[a,t,b]
+// (which is also the linear combination of environment matrices and the
+// dtype cast), batched traces / Frobenius products, the matvec composition
+// and the library's error/version entry points.
+#include <stdarg.h>
+
+#include "tnx_common.h"
+
+static thread_local std::string g_last_error;
+
+void tnx_set_error(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+namespace {
+
+constexpr int MIX_BLOCK = 1024;  // elements per workgroup (256 lanes x 4)
+
+__global__ __launch_bounds__(256) void mix_kernel(const tnx_mix_job* __restrict__ jobs, int njob,
+                                                  const tnx_mix_term* __restrict__ terms,
+                                                  const double* __restrict__ ops,
+                                                  const int32_t* __restrict__ active) {
+  const int bid = blockIdx.x;
+  const int ji = tnx_find_owner(jobs, sizeof(tnx_mix_job), offsetof(tnx_mix_job, block_begin),
+                                njob, bid);
+  const tnx_mix_job J = jobs[ji];
+  if (J.active >= 0 && active != nullptr && active[J.active] == 0) return;
+  const int64_t total = (int64_t)J.Dl * J.d * J.Dr;
+  const int64_t dr = J.Dr, ddr = (int64_t)J.d * J.Dr;
+  const int64_t base = (int64_t)(bid - J.block_begin) * MIX_BLOCK;
+  for (int q = 0; q < 4; ++q) {
+    const int64_t e = base + q * 256 + threadIdx.x;
+    if (e >= total) break;
+    const int64_t a = e / ddr;
+    const int64_t rem = e - a * ddr;
+    const int s = (int)(rem / dr);
+    const int64_t b = rem - (int64_t)s * dr;
+    double acc = 0.0;
+    for (int ti = J.term_begin; ti < J.term_begin + J.term_count; ++ti) {
+      const tnx_mix_term T = terms[ti];
+      if (T.op < 0) {
+        acc += T.coef * tnx_load((const void*)T.src, T.src_dtype, e);
+      } else {
+        const double* O = ops + (int64_t)T.op * J.d * J.d + (int64_t)s * J.d;
+        double v = 0.0;
+        for (int t = 0; t < J.d; ++t) {
+          const double o = O[t];
+          if (o != 0.0) v += o * tnx_load((const void*)T.src, T.src_dtype, a * ddr + t * dr + b);
+        }
+        acc += T.coef * v;
+      }
+    }
+    if (J.accumulate) acc += tnx_load((const void*)J.dst, J.dst_dtype, e);
+    tnx_store((void*)J.dst, J.dst_dtype, e, acc);
+  }
+}
+
+__global__ __launch_bounds__(256) void trace_dot_kernel(const tnx_dot_job* __restrict__ jobs,
+                                                        double* __restrict__ out) {
+  const tnx_dot_job J = jobs[blockIdx.x];
+  const double* X = (const double*)J.x;
+  const double* Y = (const double*)J.y;
+  double acc = 0.0;
+  if (Y == nullptr) {
+    const int nd = J.rows < J.cols ? J.rows : J.cols;
+    for (int i = threadIdx.x; i < nd; i += 256) acc += X[(int64_t)i * J.ldx + i];
+  } else {
+    const int64_t tot = (int64_t)J.rows * J.cols;
+    for (int64_t e = threadIdx.x; e < tot; e += 256) {
+      int64_t r = e / J.cols, c = e - r * J.cols;
+      acc += X[r * J.ldx + c] * Y[r * J.ldy + c];
+    }
+  }
+  __shared__ double red[4];
+  acc = tnx_wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+}  // namespace
+
+extern "C" int tnx_version(void) { return TNX_VERSION; }
+
+extern "C" const char* tnx_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int tnx_abi_sizes(int32_t* sizes, int n) {
+  const int32_t all[] = {(int32_t)sizeof(tnx_gemm_seg),      (int32_t)sizeof(tnx_gemm_problem),
+                         (int32_t)sizeof(tnx_mix_job),       (int32_t)sizeof(tnx_mix_term),
+                         (int32_t)sizeof(tnx_matvec_plan),   (int32_t)sizeof(tnx_lanczos_solve),
+                         (int32_t)sizeof(tnx_lanczos_result), (int32_t)sizeof(tnx_qr_job),
+                         (int32_t)sizeof(tnx_svd_job),       (int32_t)sizeof(tnx_dot_job)};
+  const int na = (int)(sizeof(all) / sizeof(all[0]));
+  if (sizes == nullptr) return na;
+  for (int i = 0; i < n && i < na; ++i) sizes[i] = all[i];
+  return na;
+}
+
+extern "C" int tnx_mix(const tnx_mix_job* job, int njob, const tnx_mix_term* term, int nblocks,
+                       const double* ops, const int32_t* active, tnx_stream_t stream) {
+  if (nblocks == 0 || njob == 0) return TNX_OK;
+  if (job == nullptr || term == nullptr || nblocks < 0 || njob < 0) {
+    tnx_set_error("tnx_mix: bad arguments (njob=%d nblocks=%d)", njob, nblocks);
+    return TNX_EINVAL;
+  }
+  hipLaunchKernelGGL(mix_kernel, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, job, njob, term,
+                     ops, active);
+  TNX_CHECK_LAUNCH();
+  return TNX_OK;
+}
+
+extern "C" int tnx_trace_dot(const tnx_dot_job* jobs, int njob, double* out, tnx_stream_t stream) {
+  if (njob == 0) return TNX_OK;
+  if (jobs == nullptr || out == nullptr || njob < 0) {
+    tnx_set_error("tnx_trace_dot: bad arguments");
+    return TNX_EINVAL;
+  }
+  hipLaunchKernelGGL(trace_dot_kernel, dim3(njob), dim3(256), 0, (hipStream_t)stream, jobs, out);
+  TNX_CHECK_LAUNCH();
+  return TNX_OK;
+}
+
+extern "C" int tnx_heff_matvec(const tnx_matvec_plan* plan, const int32_t* active,
+                               tnx_stream_t stream) {
+  if (plan == nullptr) {
+    tnx_set_error("tnx_heff_matvec: null plan");
+    return TNX_EINVAL;
+  }
+  int rc = tnx_gemm_grouped(plan->mode, plan->g1_prob, plan->g1_nprob, plan->g1_seg,
+                            plan->g1_ntiles, active, stream);
+  if (rc) return rc;
+  rc = tnx_mix(plan->mix_job, plan->mix_njob, plan->mix_term, plan->mix_nblocks, plan->ops, active,
+               stream);
+  if (rc) return rc;
+  return tnx_gemm_grouped(plan->mode2, plan->g2_prob, plan->g2_nprob, plan->g2_seg,
+                          plan->g2_ntiles, active, stream);
+}

@@ -1,0 +1,297 @@
+// f64 orthogonalisations of MPS tensors: Householder QR (gauge moves) and
+// one-sided Jacobi (Hestenes) SVD (entanglement spectrum moves).
+//
+// One workgroup (256 lanes) per matrix; the working copy lives column-major
+// in the job's work buffer so every reflector / rotation touches contiguous
+// columns.  Columns are distributed over the 4 wavefronts, rows over the 64
+// lanes of a wavefront, and column dot products are wavefront reductions.
+#include "tnx_common.h"
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int KMAX = 4096;  // max columns of a gauge-move matrix
+
+__device__ __forceinline__ double src_at(const double* src, int ld, int trans, int r, int c) {
+  return trans ? src[(int64_t)c * ld + r] : src[(int64_t)r * ld + c];
+}
+
+// column-major m x n work matrix W (ld = m); Q scratch m x k after it.
+__global__ __launch_bounds__(NT) void qr_kernel(const tnx_qr_job* __restrict__ jobs) {
+  const tnx_qr_job J = jobs[blockIdx.x];
+  const int m = J.m, n = J.n, k = m < n ? m : n;
+  if (k > KMAX || m <= 0 || n <= 0) return;  // rejected on the host
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double* W = (double*)J.work;
+  double* Q = W + (int64_t)m * n;
+  const double* src = (const double*)J.src;
+  __shared__ double tau[KMAX];
+  __shared__ double bcast[2];
+
+  for (int64_t e = threadIdx.x; e < (int64_t)m * n; e += NT) {
+    int r, c;
+    if (J.trans) { c = (int)(e / m); r = (int)(e - (int64_t)c * m); }   // src[c*ld+r] contiguous in r
+    else { r = (int)(e / n); c = (int)(e - (int64_t)r * n); }           // src[r*ld+c] contiguous in c
+    W[(int64_t)c * m + r] = src_at(src, J.ld, J.trans, r, c);
+  }
+  __syncthreads();
+
+  for (int j = 0; j < k; ++j) {
+    double* wj = W + (int64_t)j * m;
+    if (wave == 0) {
+      double s = 0.0;
+      for (int r = j + 1 + lane; r < m; r += 64) s += wj[r] * wj[r];
+      s = tnx_wave_sum(s);
+      const double x0 = wj[j];
+      double beta, t, scal;
+      if (s == 0.0) {
+        beta = x0; t = 0.0; scal = 0.0;
+      } else {
+        double nrm = sqrt(x0 * x0 + s);
+        beta = x0 >= 0.0 ? -nrm : nrm;
+        t = (beta - x0) / beta;
+        scal = 1.0 / (x0 - beta);
+      }
+      for (int r = j + 1 + lane; r < m; r += 64) wj[r] *= scal;
+      if (lane == 0) {
+        wj[j] = beta;
+        tau[j] = t;
+      }
+    }
+    __syncthreads();
+    const double tj = tau[j];
+    if (tj != 0.0) {
+      for (int c = j + 1 + wave; c < n; c += 4) {
+        double* wc = W + (int64_t)c * m;
+        double s = (lane == 0) ? wc[j] : 0.0;
+        for (int r = j + 1 + lane; r < m; r += 64) s += wj[r] * wc[r];
+        s = tnx_wave_sum(s) * tj;
+        if (lane == 0) wc[j] -= s;
+        for (int r = j + 1 + lane; r < m; r += 64) wc[r] -= s * wj[r];
+      }
+    }
+    __syncthreads();
+  }
+
+  // R (k x n, row-major): upper triangle of W
+  double* R = (double*)J.r;
+  for (int64_t e = threadIdx.x; e < (int64_t)k * n; e += NT) {
+    int i = (int)(e / n), c = (int)(e - (int64_t)i * n);
+    R[e] = (i <= c) ? W[(int64_t)c * m + i] : 0.0;
+  }
+  // Q = H_0 ... H_{k-1} [I_k; 0], column-major m x k, backward accumulation
+  for (int64_t e = threadIdx.x; e < (int64_t)m * k; e += NT) {
+    int c = (int)(e / m), r = (int)(e - (int64_t)c * m);
+    Q[e] = (r == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int j = k - 1; j >= 0; --j) {
+    const double tj = tau[j];
+    if (tj != 0.0) {
+      const double* v = W + (int64_t)j * m;  // v[j] = 1 implicit, v[r>j] stored
+      for (int c = j + wave; c < k; c += 4) {
+        double* qc = Q + (int64_t)c * m;
+        double s = (lane == 0) ? qc[j] : 0.0;
+        for (int r = j + 1 + lane; r < m; r += 64) s += v[r] * qc[r];
+        s = tnx_wave_sum(s) * tj;
+        if (lane == 0) qc[j] -= s;
+        for (int r = j + 1 + lane; r < m; r += 64) qc[r] -= s * v[r];
+      }
+    }
+    __syncthreads();
+  }
+  double* q = (double*)J.q;
+  if (J.q_trans) {
+    for (int64_t e = threadIdx.x; e < (int64_t)m * k; e += NT) q[e] = Q[e];
+  } else {
+    for (int64_t e = threadIdx.x; e < (int64_t)m * k; e += NT) {
+      int r = (int)(e / k), c = (int)(e - (int64_t)r * k);
+      q[e] = Q[(int64_t)c * m + r];
+    }
+  }
+  (void)bcast;
+}
+
+// One-sided Jacobi SVD; work: W (m x n col-major), V (n x n col-major).
+__global__ __launch_bounds__(NT) void svd_kernel(const tnx_svd_job* __restrict__ jobs, int max_sweeps) {
+  const tnx_svd_job J = jobs[blockIdx.x];
+  const int m = J.m, n = J.n;
+  if (n > KMAX || m < n || n <= 0) return;  // rejected on the host
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double* W = (double*)J.work;
+  double* V = W + (int64_t)m * n;
+  const double* src = (const double*)J.src;
+  __shared__ int rotated;
+  __shared__ double sv[KMAX];
+  __shared__ int order[KMAX];
+
+  for (int64_t e = threadIdx.x; e < (int64_t)m * n; e += NT) {
+    int r, c;
+    if (J.trans) { c = (int)(e / m); r = (int)(e - (int64_t)c * m); }
+    else { r = (int)(e / n); c = (int)(e - (int64_t)r * n); }
+    W[(int64_t)c * m + r] = src_at(src, J.ld, J.trans, r, c);
+  }
+  for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
+    int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+    V[e] = (r == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+
+  const int np = (n + 1) & ~1;  // players in the round-robin tournament
+  // rotate while |a.b| > m*eps*|a||b| (columns orthogonal to working accuracy)
+  const double eps = 2.220446049250313e-16 * (double)(m > 8 ? m : 8);
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    if (threadIdx.x == 0) rotated = 0;
+    __syncthreads();
+    for (int round = 0; round < np - 1; ++round) {
+      for (int p = wave; p < np / 2; p += 4) {
+        // circle method: player np-1 fixed, others rotate
+        int a = (p == 0) ? np - 1 : (round + p) % (np - 1);
+        int b = (round + np - 1 - p) % (np - 1);
+        if (p == 0) b = round % (np - 1);
+        if (a >= n || b >= n || a == b) continue;
+        if (a > b) { int t = a; a = b; b = t; }
+        double* wa = W + (int64_t)a * m;
+        double* wb = W + (int64_t)b * m;
+        double al = 0.0, be = 0.0, ga = 0.0;
+        for (int r = lane; r < m; r += 64) {
+          const double x = wa[r], y = wb[r];
+          al += x * x;
+          be += y * y;
+          ga += x * y;
+        }
+        al = tnx_wave_sum(al);
+        be = tnx_wave_sum(be);
+        ga = tnx_wave_sum(ga);
+        if (ga != 0.0 && fabs(ga) > eps * sqrt(al * be)) {
+          if (lane == 0) rotated = 1;
+          const double zeta = (be - al) / (2.0 * ga);
+          const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+          for (int r = lane; r < m; r += 64) {
+            const double x = wa[r], y = wb[r];
+            wa[r] = cs * x - sn * y;
+            wb[r] = sn * x + cs * y;
+          }
+          double* va = V + (int64_t)a * n;
+          double* vb = V + (int64_t)b * n;
+          for (int r = lane; r < n; r += 64) {
+            const double x = va[r], y = vb[r];
+            va[r] = cs * x - sn * y;
+            vb[r] = sn * x + cs * y;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (rotated == 0) break;
+    __syncthreads();
+  }
+  // singular values and descending order
+  for (int c = wave; c < n; c += 4) {
+    const double* wc = W + (int64_t)c * m;
+    double s = 0.0;
+    for (int r = lane; r < m; r += 64) s += wc[r] * wc[r];
+    s = tnx_wave_sum(s);
+    if (lane == 0) sv[c] = sqrt(s);
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < n; c += NT) {
+    int rank = 0;
+    const double x = sv[c];
+    for (int o = 0; o < n; ++o) rank += (sv[o] > x) || (sv[o] == x && o < c);
+    order[rank] = c;
+  }
+  __syncthreads();
+  double* s_out = (double*)J.s;
+  double* u = (double*)J.u;
+  double* svt = (double*)J.svt;
+  for (int i = threadIdx.x; i < n; i += NT) s_out[i] = sv[order[i]];
+  // U columns (normalised), sorted; zero columns are completed afterwards
+  for (int i = wave; i < n; i += 4) {
+    const int c = order[i];
+    const double sc = sv[c] > 0.0 ? 1.0 / sv[c] : 0.0;
+    const double* wc = W + (int64_t)c * m;
+    for (int r = lane; r < m; r += 64) {
+      const double v = wc[r] * sc;
+      if (J.u_trans) u[(int64_t)i * m + r] = v;
+      else u[(int64_t)r * n + i] = v;
+    }
+  }
+  // svt[i][col] = s_i * V[col][order i]
+  for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
+    int i = (int)(e / n), col = (int)(e - (int64_t)i * n);
+    const int c = order[i];
+    svt[e] = sv[c] * V[(int64_t)c * n + col];
+  }
+  __syncthreads();
+  // complete null columns of U (singular value exactly 0) by Gram-Schmidt
+  if (wave == 0) {
+    int next_unit = 0;
+    for (int i = 0; i < n; ++i) {
+      if (sv[order[i]] > 0.0) continue;
+      for (; next_unit < m; ++next_unit) {
+        // candidate e_{next_unit}, orthogonalised twice against columns != i
+        for (int r = lane; r < m; r += 64) {
+          const double v = (r == next_unit) ? 1.0 : 0.0;
+          if (J.u_trans) u[(int64_t)i * m + r] = v; else u[(int64_t)r * n + i] = v;
+        }
+        for (int pass = 0; pass < 2; ++pass) {
+          for (int o = 0; o < n; ++o) {
+            if (o == i || (sv[order[o]] == 0.0 && o > i)) continue;
+            double d = 0.0;
+            for (int r = lane; r < m; r += 64) {
+              const double uo = J.u_trans ? u[(int64_t)o * m + r] : u[(int64_t)r * n + o];
+              const double ui = J.u_trans ? u[(int64_t)i * m + r] : u[(int64_t)r * n + i];
+              d += uo * ui;
+            }
+            d = tnx_wave_sum(d);
+            for (int r = lane; r < m; r += 64) {
+              const double uo = J.u_trans ? u[(int64_t)o * m + r] : u[(int64_t)r * n + o];
+              if (J.u_trans) u[(int64_t)i * m + r] -= d * uo; else u[(int64_t)r * n + i] -= d * uo;
+            }
+          }
+        }
+        double nn = 0.0;
+        for (int r = lane; r < m; r += 64) {
+          const double ui = J.u_trans ? u[(int64_t)i * m + r] : u[(int64_t)r * n + i];
+          nn += ui * ui;
+        }
+        nn = tnx_wave_sum(nn);
+        if (nn > 0.25) {
+          const double sc = 1.0 / sqrt(nn);
+          for (int r = lane; r < m; r += 64) {
+            if (J.u_trans) u[(int64_t)i * m + r] *= sc; else u[(int64_t)r * n + i] *= sc;
+          }
+          ++next_unit;
+          break;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int tnx_qr_f64(const tnx_qr_job* jobs, int njob, tnx_stream_t stream) {
+  if (njob == 0) return TNX_OK;
+  if (jobs == nullptr || njob < 0) {
+    tnx_set_error("tnx_qr_f64: bad arguments");
+    return TNX_EINVAL;
+  }
+  hipLaunchKernelGGL(qr_kernel, dim3(njob), dim3(NT), 0, (hipStream_t)stream, jobs);
+  TNX_CHECK_LAUNCH();
+  return TNX_OK;
+}
+
+extern "C" int tnx_svd_jacobi_f64(const tnx_svd_job* jobs, int njob, int max_sweeps,
+                                  tnx_stream_t stream) {
+  if (njob == 0) return TNX_OK;
+  if (jobs == nullptr || njob < 0 || max_sweeps < 1) {
+    tnx_set_error("tnx_svd_jacobi_f64: bad arguments");
+    return TNX_EINVAL;
+  }
+  hipLaunchKernelGGL(svd_kernel, dim3(njob), dim3(NT), 0, (hipStream_t)stream, jobs, max_sweeps);
+  TNX_CHECK_LAUNCH();
+  return TNX_OK;
+}

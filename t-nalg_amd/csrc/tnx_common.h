@@ -1,0 +1,82 @@
+// Shared helpers of the gfx950 DMRG kernels: error reporting, dtype access,
+// XCD-aware block remapping and wavefront reductions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string>
+
+#include "../../include/tnalg_hip.h"
+
+#define TNX_VERSION 10000
+
+// thread-local message of the last failure (tnx_last_error)
+void tnx_set_error(const char* fmt, ...);
+
+#define TNX_CHECK_HIP(call)                                                      \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      tnx_set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),       \
+                    __FILE__, __LINE__);                                         \
+      return TNX_EHIP;                                                           \
+    }                                                                            \
+  } while (0)
+
+#define TNX_CHECK_LAUNCH()                                                       \
+  do {                                                                           \
+    hipError_t e_ = hipGetLastError();                                           \
+    if (e_ != hipSuccess) {                                                      \
+      tnx_set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(e_),   \
+                    __FILE__, __LINE__);                                         \
+      return TNX_EHIP;                                                           \
+    }                                                                            \
+  } while (0)
+
+typedef double tnx_d4 __attribute__((ext_vector_type(4)));
+typedef float tnx_f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 tnx_bf8 __attribute__((ext_vector_type(8)));
+
+// Load element idx of a buffer of runtime dtype as double.
+__device__ __forceinline__ double tnx_load(const void* p, int dtype, int64_t idx) {
+  if (dtype == TNX_F64) return ((const double*)p)[idx];
+  if (dtype == TNX_F32) return (double)((const float*)p)[idx];
+  return (double)(float)((const __bf16*)p)[idx];
+}
+
+__device__ __forceinline__ void tnx_store(void* p, int dtype, int64_t idx, double v) {
+  if (dtype == TNX_F64) ((double*)p)[idx] = v;
+  else if (dtype == TNX_F32) ((float*)p)[idx] = (float)v;
+  else ((__bf16*)p)[idx] = (__bf16)(float)v;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs of MI355X, each with
+// its own L2.  Remap the hardware block id so that consecutive logical tiles
+// (which share operands) land on the same XCD.
+__device__ __forceinline__ int tnx_xcd_remap(int bid, int nblocks) {
+  const int nx = 8;
+  if (nblocks < nx) return bid;
+  int x = bid % nx, i = bid / nx;
+  int q = nblocks / nx, r = nblocks % nx;
+  return x * q + (x < r ? x : r) + i;
+}
+
+__device__ __forceinline__ double tnx_wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Largest index i in [0, n) whose 32-bit field at byte offset `off` of the
+// i-th record (records of `rec` bytes) is <= key.  The fields are prefix
+// sums (non-decreasing, first one 0), so i is the record owning block `key`.
+__device__ __forceinline__ int tnx_find_owner(const void* base, int rec, int off, int n,
+                                              int key) {
+  const char* b = (const char*)base + off;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (*(const int32_t*)(b + (int64_t)mid * rec) <= key) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}

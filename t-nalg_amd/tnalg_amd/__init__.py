@@ -1,0 +1,27 @@
+"""tnalg_amd - MI355X-native finite-size DMRG, a drop-in for T-Nalg's
+``dmrg_finite_size(para)`` path (``algorithms/DMRG_anyH.py:18-101``).
+
+Public API (mirrors the reference's names):
+    dmrg_finite_size(para=None, ...) -> (ob, A, info, para)
+    parameters.generate_parameters_dmrg / make_consistent_parameter_dmrg / ...
+    MpsOpenBoundaryClass  (the returned ``A``)
+    Engine / Solve        (batched device engine)
+    serving.SolveServer   (continuous-batching request loop)
+The compute runs in ``libtnalg_hip.so`` (HIP, gfx950); there is no CPU path.
+"""
+from . import parameters  # noqa: F401
+from .parameters import generate_parameters_dmrg, make_consistent_parameter_dmrg  # noqa: F401
+
+__all__ = ['parameters', 'generate_parameters_dmrg', 'make_consistent_parameter_dmrg',
+           'dmrg_finite_size', 'Engine', 'MpsOpenBoundaryClass']
+
+
+def __getattr__(name):
+    # the engine imports torch and loads the HIP library lazily
+    if name in ('Engine', 'Solve'):
+        from . import engine
+        return getattr(engine, name)
+    if name in ('dmrg_finite_size', 'MpsOpenBoundaryClass', 'run_dmrg'):
+        from . import dmrg
+        return getattr(dmrg, name)
+    raise AttributeError(name)

@@ -1,0 +1,279 @@
+"""``dmrg_finite_size(para)`` - the drop-in entry of the finite-size DMRG path.
+
+Same call, keys and outputs as the reference's
+``algorithms/DMRG_anyH.py:18-101``: ``ob, A, info, para =
+dmrg_finite_size(para)`` with ``ob`` holding ``eb_full, mx, mz, e_per_site,
+eb, corr_x, corr_z``, ``A`` an MPS object with the attributes and observation
+methods of ``MpsOpenBoundaryClass`` (``library/MPSClass.py:28-108``), and
+``info`` with ``convergence`` and ``t_cost``.  The sweep schedule, the
+convergence test every ``dt_ob`` sweeps and the end-of-run entanglement
+spectrum follow the reference; the arithmetic runs on the GPU.
+
+Extra keyword arguments (never read from ``para``): ``seed`` (``None`` draws
+the initial MPS from ``numpy.random``'s global state in the reference's
+order, like the reference), ``precision`` (``'fp64'`` plain, or a mixed
+schedule, see :func:`mixed_schedule`), ``ncv`` (Krylov size, scipy's default
+20), ``engine`` (an existing :class:`Engine`).
+"""
+import copy
+import time
+
+import numpy as np
+
+from .parameters import generate_parameters_dmrg, positions_fully_connected
+
+
+def get_bond_energies(eb_full, positions, index2):
+    """Sum the per-row energies per bond (``algorithms/DMRG_anyH.py:338-346``)."""
+    eb_full = np.asarray(eb_full).reshape(-1)
+    positions = np.asarray(positions)
+    eb = np.zeros((positions.shape[0], 1))
+    for i in range(eb_full.size):
+        hit = (index2[i, 0] == positions[:, 0]) & (index2[i, 1] == positions[:, 1])
+        eb[np.nonzero(hit)] += eb_full[i]
+    return eb
+
+
+def entanglement_entropy(lm, tol=1e-20):
+    """``-2 sum lam^2 ln lam`` of the normalised spectrum (TBM:888-904)."""
+    lm = np.sort(np.asarray(lm, dtype=float).reshape(-1))[::-1]
+    lm = lm[lm > tol]
+    lm = lm / np.linalg.norm(lm)
+    return float(-2.0 * np.dot(lm ** 2, np.log(lm)))
+
+
+def mixed_schedule(low='bf16', mid='fp32', n_fp64=1, n_mid=2):
+    """Precision per sweep: ``low`` first, then ``mid`` for ``n_mid`` sweeps,
+    the last ``n_fp64`` sweeps in f64."""
+    def sched(sweep, nsweeps):
+        if sweep >= nsweeps - n_fp64:
+            return 'fp64'
+        if sweep >= nsweeps - n_fp64 - n_mid:
+            return mid
+        return low
+    return sched
+
+
+def _schedule_from(precision):
+    if precision in (None, 'fp64'):
+        return lambda s, n: 'fp64'
+    if precision in ('fp32', 'bf16'):
+        return mixed_schedule(low=precision, mid='fp32', n_fp64=1, n_mid=1)
+    if precision == 'mixed':
+        return mixed_schedule()
+    if callable(precision):
+        return precision
+    raise ValueError('unknown precision %r' % (precision,))
+
+
+class MpsOpenBoundaryClass:
+    """Result MPS of the drop-in (the reference's ``MpsOpenBoundaryClass``,
+    ``library/MPSClass.py:28-108``): host copies of the tensors plus a handle
+    to the device solve, which the observation methods use."""
+
+    def __init__(self, solve, engine, spin='half'):
+        self._solve = solve
+        self._engine = engine
+        self.spin = spin
+        self.decomp_way = 'qr'
+        self.refresh_from_device()
+
+    def refresh_from_device(self):
+        s = self._solve
+        self.length = s.n
+        self.phys_dim = s.d
+        self.mps = s.host_mps()
+        self.virtual_dim = list(s.dims)
+        self.center = s.center
+        self.orthogonality = np.zeros((s.n, 1))
+        self.orthogonality[:s.center] = -1
+        self.orthogonality[s.center + 1:] = 1
+        self.operators = [np.asarray(o) for o in s.para['op']]
+        self.lm = [np.asarray(x) for x in s.lm]
+        self.ent = np.zeros((s.n - 1, 1))
+
+    # -- observation methods (MPSClass.py:1105-1212) --------------------------
+    def _measure(self, one, two):
+        from .observe import measure
+        if self._solve is None:
+            raise RuntimeError('the device state of this MPS was released (clean_to_save)')
+        return measure(self._engine, [self._solve], [(one, two)])[0]
+
+    def observation_s1(self, inputs):
+        sn, position = inputs
+        return self._measure([(position, sn)], [])[0][0]
+
+    def observation_s1_s2(self, inputs):
+        ssn, positions = inputs
+        return self._measure([], [(positions[0], positions[1], ssn[0], ssn[1])])[1][0]
+
+    def observe_magnetization(self, sn):
+        v, _, _ = self._measure([(i, sn) for i in range(self.length)], [])
+        return v.reshape(-1, 1)
+
+    def observe_bond_energy(self, index2, coeff2):
+        index2 = np.asarray(index2)
+        _, v, _ = self._measure([], [tuple(r) for r in index2[:, [0, 1, 2, 3]]])
+        return (np.asarray(coeff2, dtype=float).reshape(-1) * v).reshape(-1, 1)
+
+    def observe_bond_energy_from_jxyz(self, pos2, jx, jy, jz, tol=1e-20):
+        """``jx<SxSx> + jy<SySy> + jz<SzSz>`` per pair; ``SySy`` is evaluated
+        as ``(S+S- + S-S+ - S+S+ - S-S-)/4`` with the real operators 4, 5."""
+        pos2 = np.asarray(pos2)
+        rows, coefs = [], []
+        for n in range(pos2.shape[0]):
+            i, j = int(pos2[n, 0]), int(pos2[n, 1])
+            if abs(jx) > tol:
+                rows.append((i, j, 1, 1)); coefs.append((n, jx))
+            if abs(jy) > tol:
+                for a, b, c in ((4, 5, 0.25), (5, 4, 0.25), (4, 4, -0.25), (5, 5, -0.25)):
+                    rows.append((i, j, a, b)); coefs.append((n, jy * c))
+            if abs(jz) > tol:
+                rows.append((i, j, 3, 3)); coefs.append((n, jz))
+        eb = np.zeros((pos2.shape[0], 1))
+        if rows:
+            _, v, _ = self._measure([], rows)
+            for (n, c), x in zip(coefs, v):
+                eb[n] += c * x
+        return eb
+
+    def observe_correlators_from_middle(self, op1, op2, ob_len=None):
+        pairs = correlator_pairs(self.length, ob_len)
+        if not pairs:
+            return np.zeros(0)
+        _, v, _ = self._measure([], [(a, b, op1, op2) for a, b in pairs])
+        return np.asarray(v)
+
+    def norm_mps(self):
+        return float(np.sqrt(self._measure([], [])[2]))
+
+    def full_coefficients(self):
+        """Dense state vector (small chains only)."""
+        v = self.mps[0].reshape(-1, self.mps[0].shape[2])
+        for a in self.mps[1:]:
+            v = (v @ a.reshape(a.shape[0], -1)).reshape(-1, a.shape[2])
+        return v.reshape(-1)
+
+    def clean_to_save(self):
+        """Release the device state (``MPSClass.py:1396-1405`` drops caches)."""
+        if self._solve is not None and self._engine is not None:
+            self._engine.remove_solve(self._solve)
+        self._solve = None
+
+    def __getstate__(self):
+        st = dict(self.__dict__)
+        st['_solve'] = None
+        st['_engine'] = None
+        return st
+
+
+def correlator_pairs(length, ob_len=None):
+    """Site pairs of ``observe_correlators_from_middle`` (MPSClass.py:1196-1212)."""
+    if ob_len is None:
+        ob_len = length
+    out = []
+    pos1 = int(round(length / 2))
+    pos2 = pos1 + 1
+    k = 0
+    while pos1 > -0.1 and pos2 < length and ob_len > -0.1:
+        out.append((pos1, pos2))
+        if k % 2 == 0:
+            pos1 -= 1
+        else:
+            pos2 += 1
+        k += 1
+        ob_len -= 1
+    return out
+
+
+def observe_energy(engine, solve, para):
+    """``eb_full, mx, mz, e_per_site`` as in ``DMRG_anyH.py:66-70``."""
+    from .observe import measure
+    idx2 = np.asarray(para['index2'])
+    n = solve.n
+    one = [(i, 1) for i in range(n)] + [(i, 3) for i in range(n)]
+    (vone, vtwo, norm), = measure(engine, [solve], [(one, [tuple(r) for r in idx2])])
+    ob = {}
+    ob['eb_full'] = (np.asarray(para['coeff2'], dtype=float).reshape(-1) * vtwo).reshape(-1, 1)
+    ob['mx'] = vone[:n].reshape(-1, 1)
+    ob['mz'] = vone[n:].reshape(-1, 1)
+    ob['e_per_site'] = (sum(ob['eb_full']) - para['hx'] * sum(ob['mx'])
+                        - para['hz'] * sum(ob['mz'])) / n
+    return ob
+
+
+def schedule_spectrum(solve):
+    """``calculate_entanglement_spectrum`` (MPSClass.py:965-985): SVD centre
+    moves from the centre to the first bond without a spectrum, to the last
+    one, and back to the centre."""
+    from .engine import Step
+    c = solve.center
+    empty = [b for b in range(solve.n - 1) if solve.lm[b].size == 0]
+    if not empty:
+        return
+    p0, p1 = min(empty), max(empty)
+    path = [p0, p1 + 1, c]
+    cur = c
+    for tgt in path:
+        while cur < tgt:
+            solve.queue.append(Step('svdmove', cur, +1))
+            cur += 1
+        while cur > tgt:
+            solve.queue.append(Step('svdmove', cur, -1))
+            cur -= 1
+
+
+def dmrg_finite_size(para=None, seed=None, precision='fp64', ncv=20, engine=None, verbose=False):
+    """``algorithms/DMRG_anyH.py:18-101`` on the GPU engine."""
+    from .engine import Engine
+    t_start = time.time()
+    info = {}
+    if para is None:
+        para = generate_parameters_dmrg()
+    eng = engine or Engine(precision_schedule=_schedule_from(precision), ncv=ncv,
+                           lowp=precision not in (None, 'fp64'))
+    rng = np.random if seed is None else np.random.RandomState(seed)
+    s = eng.add_solve(para, seed=rng)
+    s.schedule_canonicalise()
+    eng.run_queues([s])
+    e0 = 0.0
+    info['convergence'] = 1
+    info['energies'] = []
+    ob = {}
+    for t in range(para['sweep_time']):
+        if_ob = ((t + 1) % para['dt_ob'] == 0) or t == para['sweep_time'] - 1
+        s.schedule_sweep(t)
+        eng.run_queues([s])
+        if if_ob:
+            ob = observe_energy(eng, s, para)
+            e = float(np.ravel(ob['e_per_site'])[0])
+            info['energies'].append(e * s.n)
+            info['convergence'] = abs(ob['e_per_site'] - e0)
+            if verbose:
+                print('sweep %d: e_per_site = %.15g, convergence %g' % (t + 1, e, info['convergence']))
+            if info['convergence'] < para['break_tol']:
+                break
+            e0 = ob['e_per_site']
+    info['sweeps'] = t + 1
+    ob['eb'] = get_bond_energies(ob['eb_full'], para['positions_h2'], np.asarray(para['index2']))
+    schedule_spectrum(s)
+    eng.run_queues([s])
+    A = MpsOpenBoundaryClass(s, eng, spin=para.get('spin', 'half'))
+    for i in range(s.n - 1):
+        A.ent[i] = -1 if A.lm[i].size == 0 else entanglement_entropy(A.lm[i])
+    ob['corr_x'] = A.observe_correlators_from_middle(1, 1)
+    ob['corr_z'] = A.observe_correlators_from_middle(3, 3)
+    info['t_cost'] = time.time() - t_start
+    info['stats'] = dict(s.stats)
+    return ob, A, info, para
+
+
+def all_pair_spin_correlations(A):
+    """``<S_i . S_j>`` for every pair ``i < j`` (``observe_bond_energy_from_jxyz``
+    over ``positions_fully_connected``), as an ``(N, N)`` symmetric matrix."""
+    pos = positions_fully_connected(A.length)
+    v = A.observe_bond_energy_from_jxyz(pos, 1.0, 1.0, 1.0).reshape(-1)
+    out = np.zeros((A.length, A.length))
+    out[pos[:, 0], pos[:, 1]] = v
+    out[pos[:, 1], pos[:, 0]] = v
+    return out

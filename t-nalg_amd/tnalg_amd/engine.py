@@ -1,0 +1,726 @@
+"""Batched one-site DMRG engine on the gfx950 HIP library.
+
+Every solve keeps its MPS, its block environments (``H`` plus open
+operators per bond, see ``terms.py``) and its step scratch resident in HBM
+(torch-ROCm allocations).  A *global step* advances every active solve of
+the batch by one entry of its own schedule; each phase of the step
+(QR gauge move, absorption, environment update, linear combinations,
+Lanczos with the grouped-GEMM matvec) is ONE launch over all solves that
+take part, built by concatenating the solves' descriptor records.
+
+Reference correspondence (``library/MPSClass.py``):
+  gauge move      correct_orthogonal_center / orthogonalize_mps   :186-279
+  env update      update_effect_op_l0_to_l1 / bound_vec_operator_* :555-611, TBM:632-721
+  Heff / matvec   all_environments_optimized + f_handle_optimized :715-858
+  local solve     eigsh(I - tau Heff, k=1, 'LM', v0=A[p])          :883-889
+Precision: ``'fp64'`` (plain) runs every GEMM in f64 MFMA; ``'fp32'`` and
+``'bf16'`` run the matvec GEMMs in f32 / bf16 MFMA (f32 accumulation) on
+low-precision copies of the environments, while gauge moves, environment
+updates, the Krylov basis, reorthogonalisation and the tridiagonal solve stay
+f64.
+"""
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import records as R
+from .device import ITEMSIZE, TORCH_DTYPE, Uploader, ctypes_void, require_gpu
+from .terms import compile_terms
+
+PREC = {
+    # name: (TI dtype, TC dtype, GEMM1 mode, GEMM2 mode, floor_rel)
+    'fp64': (L.F64, L.F64, L.GEMM_F64, L.GEMM_F64, 0.0),
+    'fp32': (L.F32, L.F32, L.GEMM_F32, L.GEMM_F32, 2e-6),
+    'bf16': (L.BF16, L.F32, L.GEMM_BF16_BF16, L.GEMM_BF16_F32, 3e-3),
+}
+
+
+def _p(t, off_elems=0, itemsize=8):
+    return t.data_ptr() + off_elems * itemsize
+
+
+class Step:
+    """One schedule entry of a solve: kind in {'move', 'solve', 'svdmove'},
+    site, direction (+1: centre moves right, -1: left)."""
+    __slots__ = ('kind', 'site', 'dir', 'sweep')
+
+    def __init__(self, kind, site, direction, sweep=-1):
+        self.kind, self.site, self.dir, self.sweep = kind, site, direction, sweep
+
+    def __repr__(self):
+        return 'Step(%s,%d,%+d,sw=%d)' % (self.kind, self.site, self.dir, self.sweep)
+
+
+class Solve:
+    """Device state of one ground-state solve."""
+
+    def __init__(self, engine, para, seed=0, mps=None, tag=None):
+        self.engine = engine
+        self.dev = engine.device
+        self.tag = tag
+        self.para = para
+        self.T = compile_terms(para)
+        self.n = self.T.n
+        self.d = self.T.d
+        self.chi = int(para['chi'])
+        self.tau = float(para.get('tau', 1e-4))
+        self.eigs_tol = float(para.get('eigs_tol', 1e-5))
+        self.ob = int(para.get('ob_position', 0))
+        n, d, chi = self.n, self.d, self.chi
+        if n < 2:
+            raise ValueError('the chain needs at least 2 sites')
+        if not 0 <= self.ob < n:
+            raise ValueError('ob_position must be in [0, l)')
+        # initial MPS: reference draw order (TensorBasicModule.py:179-211)
+        if mps is None:
+            rng = seed if hasattr(seed, 'randn') else np.random.RandomState(seed)
+            host = [None] * n
+            host[0] = rng.randn(1, d, chi)
+            host[n - 1] = rng.randn(chi, d, 1)
+            for s in range(1, n - 1):
+                host[s] = rng.randn(chi, d, chi)
+        else:
+            host = [np.asarray(a, dtype=np.float64) for a in mps]
+        dims = [host[0].shape[0]] + [a.shape[2] for a in host]
+        self.dims = list(dims)
+        # dims after the initial canonicalisation (bonds only shrink later)
+        cd = list(dims)
+        for s in range(0, self.ob):
+            cd[s + 1] = min(cd[s] * d, cd[s + 1])
+        for s in range(n - 1, self.ob, -1):
+            cd[s] = min(cd[s], d * cd[s + 1])
+        self.dmax = [max(a, b) for a, b in zip(dims, cd)]
+        dm = self.dmax
+        f64 = torch.float64
+        self.mps = [torch.from_numpy(np.ascontiguousarray(host[s]).reshape(-1)).to(self.dev)
+                    for s in range(n)]
+        for s in range(n):  # room for the largest shape the site takes
+            need = dm[s] * d * dm[s + 1]
+            if self.mps[s].numel() < need:
+                t = torch.zeros(need, dtype=f64, device=self.dev)
+                t[:self.mps[s].numel()] = self.mps[s]
+                self.mps[s] = t
+        T = self.T
+        self.nL = [len(T.left_opens[b]) for b in range(n + 1)]
+        self.nR = [len(T.right_opens[b]) for b in range(n + 1)]
+        self.Lenv = [torch.zeros((1 + self.nL[b]) * dm[b] * dm[b], dtype=f64, device=self.dev)
+                     for b in range(n + 1)]
+        self.Renv = [torch.zeros((1 + self.nR[b]) * dm[b] * dm[b], dtype=f64, device=self.dev)
+                     for b in range(n + 1)]
+        self.lp_bytes = 4 if engine.uses_lowp else 0
+        if self.lp_bytes:
+            self.Lenv_lp = [torch.zeros((1 + self.nL[b]) * dm[b] * dm[b] * self.lp_bytes,
+                                        dtype=torch.uint8, device=self.dev) for b in range(n + 1)]
+            self.Renv_lp = [torch.zeros((1 + self.nR[b]) * dm[b] * dm[b] * self.lp_bytes,
+                                        dtype=torch.uint8, device=self.dev) for b in range(n + 1)]
+        # scratch sizes
+        psi = max(dm[s] * d * dm[s + 1] for s in range(n))
+        dsq = max(x * x for x in dm)
+        st = T.sites
+        n_lm = max(len(s.lm) for s in st)
+        n_mr = max(len(s.mr) for s in st)
+        n_sd = max(len(s.strad) for s in st)
+        n_lc = max(len(s.lm) + len(s.mr) + len(s.strad) for s in st)
+        n_t = 1 + n_lm + n_sd
+        n_y = 1 + max(n_lm, n_mr) + max(max(self.nL), max(self.nR))
+        n_w = 1 + max(max(self.nL), max(self.nR))
+        ncv = engine.ncv
+        self.psi_max, self.dsq_max = psi, dsq
+        tb = 4 if engine.uses_lowp else 8
+        self.lc64 = torch.zeros(max(1, max(n_lm, n_mr)) * dsq, dtype=f64, device=self.dev)
+        self.lc_lp = torch.zeros(max(1, n_lc) * dsq * 8, dtype=torch.uint8, device=self.dev)
+        self.Tb = torch.zeros(n_t * psi * 8, dtype=torch.uint8, device=self.dev)
+        self.Ub = torch.zeros((1 + n_mr) * psi * 8, dtype=torch.uint8, device=self.dev)
+        self.Yb = torch.zeros(n_y * psi, dtype=f64, device=self.dev)
+        self.Wb = torch.zeros(n_w * psi, dtype=f64, device=self.dev)
+        self.V = torch.zeros((ncv + 1) * psi, dtype=f64, device=self.dev)
+        self.w = torch.zeros(psi, dtype=f64, device=self.dev)
+        self.x = torch.zeros(psi * 8, dtype=torch.uint8, device=self.dev)
+        self.init = torch.zeros(psi, dtype=f64, device=self.dev)
+        self.Rq = torch.zeros(dsq + max(dm) + 64, dtype=f64, device=self.dev)
+        self.qrw = torch.zeros(2 * psi + dsq, dtype=f64, device=self.dev)
+        del tb
+        self.ops_slot = engine.alloc_ops(T.ops)
+        self.d2 = d * d
+        # bookkeeping
+        self.center = -1
+        self.queue = []
+        self.sweeps_done = 0
+        self.lowp_dtype = None   # dtype currently held by the *_lp copies
+        self.last_theta = float('nan')
+        self.thetas = []
+        self.stats = {'matvecs': 0, 'steps': 0, 'unconverged': 0, 'restarts': 0}
+        self.lm = [np.zeros(0) for _ in range(n - 1)]
+        self.failed = None
+        self.cache = {}
+
+    # ----------------------------------------------------------- addresses
+    def op_index(self, k):
+        return self.ops_slot + k
+
+    def L(self, b, m):
+        return _p(self.Lenv[b], m * self.dmax[b] ** 2)
+
+    def Rr(self, b, m):
+        return _p(self.Renv[b], m * self.dmax[b] ** 2)
+
+    def L_lp(self, b, m, ti):
+        if ti == L.F64:
+            return self.L(b, m)
+        return self.Lenv_lp[b].data_ptr() + m * self.dmax[b] ** 2 * ITEMSIZE[ti]
+
+    def R_lp(self, b, m, ti):
+        if ti == L.F64:
+            return self.Rr(b, m)
+        return self.Renv_lp[b].data_ptr() + m * self.dmax[b] ** 2 * ITEMSIZE[ti]
+
+    def free(self):
+        self.engine.free_ops(self.ops_slot)
+        for name in ('mps', 'Lenv', 'Renv', 'Lenv_lp', 'Renv_lp'):
+            if hasattr(self, name):
+                setattr(self, name, None)
+        for name in ('lc64', 'lc_lp', 'Tb', 'Ub', 'Yb', 'Wb', 'V', 'w', 'x', 'init', 'Rq', 'qrw'):
+            setattr(self, name, None)
+
+    # ------------------------------------------------------------ schedule
+    def schedule_canonicalise(self):
+        """``correct_orthogonal_center(ob)`` on a centre-less MPS
+        (``MPSClass.py:250-279``): left moves 0..ob-1, right moves N-1..ob+1,
+        each followed by the environment update through the moved site."""
+        q = [Step('move', s, +1) for s in range(0, self.ob)]
+        q += [Step('move', s, -1) for s in range(self.n - 1, self.ob, -1)]
+        self.queue.extend(q)
+        self._sched_center = self.ob
+
+    def schedule_sweep(self, sweep):
+        """One sweep of ``DMRG_anyH.py:47-64``: ob+1..N-1, N-2..0, 1..ob-1."""
+        n, ob = self.n, self.ob
+        sites = [(s, +1) for s in range(ob + 1, n)] + [(s, -1) for s in range(n - 2, -1, -1)] + \
+            [(s, +1) for s in range(1, ob)]
+        c = self._sched_center
+        for s, _ in sites:
+            while abs(c - s) > 1:   # multi-hop centre moves (MPSClass.py:275-277)
+                dirn = 1 if s > c else -1
+                self.queue.append(Step('move', c, dirn, sweep))
+                c += dirn
+            dirn = 1 if s > c else -1
+            self.queue.append(Step('solve', s, dirn, sweep))
+            c = s
+        self._sched_center = c
+
+    # ------------------------------------------------------------- planning
+    def plan(self, st, prec):
+        """Descriptor records of one step (dict phase -> arrays)."""
+        key = (st.kind, st.site, st.dir, tuple(self.dims), prec)
+        hit = self.cache.get(key)
+        if hit is not None:
+            self.dims = list(hit['dims_after'])
+            return hit
+        ph = {}
+        if st.kind in ('move', 'solve'):
+            q = st.site if st.kind == 'move' else st.site - st.dir
+            self._plan_gauge(ph, q, st.dir, copy_back=(st.kind == 'move'))
+            self._plan_env(ph, q, st.dir)
+            if self.engine.uses_lowp and prec != 'fp64':
+                self._plan_cast_block(ph, q + 1 if st.dir > 0 else q, st.dir > 0, PREC[prec][0])
+            if st.kind == 'solve':
+                self._plan_local(ph, st.site, prec)
+        elif st.kind == 'svdmove':
+            self._plan_svd(ph, st.site, st.dir)
+        ph['dims_after'] = list(self.dims)
+        self.cache[key] = ph
+        return ph
+
+    def _plan_gauge(self, ph, q, dirn, copy_back):
+        d, dims = self.d, self.dims
+        if dirn > 0:
+            Dl, Dm, Dn = dims[q], dims[q + 1], dims[q + 2]
+            m, nn = Dl * d, Dm
+            k = min(m, nn)
+            if k > L.QR_KMAX:
+                raise ValueError('bond dimension %d exceeds the QR kernel limit %d' % (k, L.QR_KMAX))
+            ph['qr'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(), self.Rq.data_ptr(),
+                                  self.qrw.data_ptr(), m, nn, nn, 0, 0, 0)], dtype=L.QR_JOB)
+            g = R.GemmList()
+            g.add(self.init.data_ptr(), k, d * Dn, d * Dn,
+                  [(self.Rq.data_ptr(), nn, 0, self.mps[q + 1].data_ptr(), d * Dn, 0, Dm, 1.0)])
+            ph['absorb'] = g.arrays()
+            if copy_back:
+                mx = R.MixList()
+                mx.add(self.mps[q + 1].data_ptr(), L.F64, 1, 1, k * d * Dn,
+                       [(self.init.data_ptr(), L.F64, 1.0, -1)])
+                ph['copy'] = mx.arrays()
+            self.dims[q + 1] = k
+        else:
+            Dp, Dm, Dr = dims[q - 1], dims[q], dims[q + 1]
+            m, nn = d * Dr, Dm
+            k = min(m, nn)
+            if k > L.QR_KMAX:
+                raise ValueError('bond dimension %d exceeds the QR kernel limit %d' % (k, L.QR_KMAX))
+            ph['qr'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(), self.Rq.data_ptr(),
+                                  self.qrw.data_ptr(), m, nn, m, 1, 1, 0)], dtype=L.QR_JOB)
+            g = R.GemmList()
+            g.add(self.init.data_ptr(), Dp * d, k, k,
+                  [(self.mps[q - 1].data_ptr(), Dm, 0, self.Rq.data_ptr(), Dm, 1, Dm, 1.0)])
+            ph['absorb'] = g.arrays()
+            if copy_back:
+                mx = R.MixList()
+                mx.add(self.mps[q - 1].data_ptr(), L.F64, 1, 1, Dp * d * k,
+                       [(self.init.data_ptr(), L.F64, 1.0, -1)])
+                ph['copy'] = mx.arrays()
+            self.dims[q] = k
+
+    def _plan_env(self, ph, q, dirn):
+        """Block environment through the (new) isometry at site q."""
+        T, d, dims = self.T, self.d, self.dims
+        site = T.sites[q]
+        Qp = self.mps[q].data_ptr()
+        lc, g1, mx, gT = R.MixList(), R.GemmList(), R.MixList(), R.GemmList()
+        psi = self.psi_max
+        Y = lambda t: _p(self.Yb, t * psi)
+        W = lambda t: _p(self.Wb, t * psi)
+        LC = lambda t: _p(self.lc64, t * self.dsq_max)
+        if dirn > 0:
+            Dl, k = dims[q], dims[q + 1]
+            groups = site.lm
+            src_opens = T.left_opens[q]
+            dst_opens = T.left_opens[q + 1]
+            sidx = {o: t for t, o in enumerate(src_opens)}
+            for g, (opb, grp) in enumerate(groups):
+                lc.add(LC(g), L.F64, Dl, 1, Dl, [(self.L(q, 1 + l), L.F64, c, -1) for l, c in grp])
+            ny = 0
+            yH = None
+            if q > 0:
+                yH = ny
+                g1.add(Y(ny), Dl, d * k, d * k, [(self.L(q, 0), Dl, 0, Qp, d * k, 0, Dl, 1.0)])
+                ny += 1
+            yG = []
+            for g in range(len(groups)):
+                g1.add(Y(ny), Dl, d * k, d * k, [(LC(g), Dl, 0, Qp, d * k, 0, Dl, 1.0)])
+                yG.append(ny)
+                ny += 1
+            terms = []
+            if yH is not None:
+                terms.append((Y(yH), L.F64, 1.0, -1))
+            for g, (opb, _) in enumerate(groups):
+                terms.append((Y(yG[g]), L.F64, 1.0, self.op_index(opb)))
+            for (kop, c) in site.field:
+                terms.append((Qp, L.F64, c, self.op_index(kop)))
+            mx.add(W(0), L.F64, Dl, d, k, terms)
+            gT.add(self.L(q + 1, 0), k, k, k, [(Qp, k, 1, W(0), k, 0, Dl * d, 1.0)])
+            nw = 1
+            for t, (i, a) in enumerate(dst_opens):
+                if i < q:
+                    g1.add(Y(ny), Dl, d * k, d * k,
+                           [(self.L(q, 1 + sidx[(i, a)]), Dl, 0, Qp, d * k, 0, Dl, 1.0)])
+                    gT.add(self.L(q + 1, 1 + t), k, k, k, [(Qp, k, 1, Y(ny), k, 0, Dl * d, 1.0)])
+                    ny += 1
+                else:
+                    mx.add(W(nw), L.F64, Dl, d, k, [(Qp, L.F64, 1.0, self.op_index(a))])
+                    gT.add(self.L(q + 1, 1 + t), k, k, k, [(Qp, k, 1, W(nw), k, 0, Dl * d, 1.0)])
+                    nw += 1
+        else:
+            k, Dr = dims[q], dims[q + 1]
+            groups = site.mr
+            src_opens = T.right_opens[q + 1]
+            dst_opens = T.right_opens[q]
+            sidx = {o: t for t, o in enumerate(src_opens)}
+            for g, (opa, grp) in enumerate(groups):
+                lc.add(LC(g), L.F64, Dr, 1, Dr, [(self.Rr(q + 1, 1 + r), L.F64, c, -1) for r, c in grp])
+            ny = 0
+            yH = None
+            if q < self.n - 1:
+                yH = ny
+                g1.add(Y(ny), k * d, Dr, Dr, [(Qp, Dr, 0, self.Rr(q + 1, 0), Dr, 1, Dr, 1.0)])
+                ny += 1
+            yG = []
+            for g in range(len(groups)):
+                g1.add(Y(ny), k * d, Dr, Dr, [(Qp, Dr, 0, LC(g), Dr, 1, Dr, 1.0)])
+                yG.append(ny)
+                ny += 1
+            terms = []
+            if yH is not None:
+                terms.append((Y(yH), L.F64, 1.0, -1))
+            for g, (opa, _) in enumerate(groups):
+                terms.append((Y(yG[g]), L.F64, 1.0, self.op_index(opa)))
+            for (kop, c) in site.field:
+                terms.append((Qp, L.F64, c, self.op_index(kop)))
+            mx.add(W(0), L.F64, k, d, Dr, terms)
+            gT.add(self.Rr(q, 0), k, k, k, [(Qp, d * Dr, 0, W(0), d * Dr, 1, d * Dr, 1.0)])
+            nw = 1
+            for t, (j, b) in enumerate(dst_opens):
+                if j > q:
+                    g1.add(Y(ny), k * d, Dr, Dr,
+                           [(Qp, Dr, 0, self.Rr(q + 1, 1 + sidx[(j, b)]), Dr, 1, Dr, 1.0)])
+                    gT.add(self.Rr(q, 1 + t), k, k, k, [(Qp, d * Dr, 0, Y(ny), d * Dr, 1, d * Dr, 1.0)])
+                    ny += 1
+                else:
+                    mx.add(W(nw), L.F64, k, d, Dr, [(Qp, L.F64, 1.0, self.op_index(b))])
+                    gT.add(self.Rr(q, 1 + t), k, k, k, [(Qp, d * Dr, 0, W(nw), d * Dr, 1, d * Dr, 1.0)])
+                    nw += 1
+        ph['env_lc'] = lc.arrays()
+        ph['env_g1'] = g1.arrays()
+        ph['env_mix'] = mx.arrays()
+        ph['env_gT'] = gT.arrays()
+
+    def _plan_cast_block(self, ph, b, left, ti):
+        D = self.dims[b]
+        cnt = 1 + (self.nL[b] if left else self.nR[b])
+        mx = R.MixList()
+        for m in range(cnt):
+            src = self.L(b, m) if left else self.Rr(b, m)
+            dst = self.L_lp(b, m, ti) if left else self.R_lp(b, m, ti)
+            mx.add(dst, ti, D, 1, D, [(src, L.F64, 1.0, -1)])
+        ph['cast'] = mx.arrays()
+
+    def cast_all_records(self, ti):
+        mx = R.MixList()
+        for b in range(self.n + 1):
+            D = self.dims[b]
+            for m in range(1 + self.nL[b]):
+                mx.add(self.L_lp(b, m, ti), ti, D, 1, D, [(self.L(b, m), L.F64, 1.0, -1)])
+            for m in range(1 + self.nR[b]):
+                mx.add(self.R_lp(b, m, ti), ti, D, 1, D, [(self.Rr(b, m), L.F64, 1.0, -1)])
+        return mx.arrays()
+
+    def _plan_local(self, ph, p, prec):
+        ti, tc, mode1, mode2, floor_rel = PREC[prec]
+        T, d, dims = self.T, self.d, self.dims
+        site = T.sites[p]
+        Dl, Dr = dims[p], dims[p + 1]
+        n_el = Dl * d * Dr
+        psi, dsq = self.psi_max, self.dsq_max
+        isz = ITEMSIZE[ti]
+        Xp = self.x.data_ptr()
+        LCp = lambda t: self.lc_lp.data_ptr() + t * dsq * isz
+        Tp = lambda t: self.Tb.data_ptr() + t * psi * isz
+        base = self.Ub.data_ptr()
+        Up = lambda t: self.Ub.data_ptr() + psi * 8 + t * psi * isz
+        lc = R.MixList()
+        nlm, nmr = len(site.lm), len(site.mr)
+        for g, (_, grp) in enumerate(site.lm):
+            lc.add(LCp(g), ti, Dl, 1, Dl, [(self.L(p, 1 + l), L.F64, c, -1) for l, c in grp])
+        for g, (_, grp) in enumerate(site.mr):
+            lc.add(LCp(nlm + g), ti, Dr, 1, Dr, [(self.Rr(p + 1, 1 + r), L.F64, c, -1) for r, c in grp])
+        for u, (raw, grp) in enumerate(site.strad):
+            if site.strad_left:
+                lc.add(LCp(nlm + nmr + u), ti, Dr, 1, Dr,
+                       [(self.Rr(p + 1, 1 + r), L.F64, c, -1) for r, c in grp])
+            else:
+                lc.add(LCp(nlm + nmr + u), ti, Dl, 1, Dl,
+                       [(self.L(p, 1 + l), L.F64, c, -1) for l, c in grp])
+        g1, mx, g2 = R.GemmList(), R.MixList(), R.GemmList()
+        nt = 0
+        tH = None
+        if p > 0:
+            tH = nt
+            g1.add(Tp(nt), Dl, d * Dr, d * Dr, [(self.L_lp(p, 0, ti), Dl, 0, Xp, d * Dr, 0, Dl, 1.0)])
+            nt += 1
+        tG = []
+        for g in range(nlm):
+            g1.add(Tp(nt), Dl, d * Dr, d * Dr, [(LCp(g), Dl, 0, Xp, d * Dr, 0, Dl, 1.0)])
+            tG.append(nt)
+            nt += 1
+        tS = []
+        for u, (raw, _) in enumerate(site.strad):
+            F = self.L_lp(p, 1 + raw, ti) if site.strad_left else LCp(nlm + nmr + u)
+            g1.add(Tp(nt), Dl, d * Dr, d * Dr, [(F, Dl, 0, Xp, d * Dr, 0, Dl, 1.0)])
+            tS.append(nt)
+            nt += 1
+        terms = []
+        if tH is not None:
+            terms.append((Tp(tH), ti, 1.0, -1))
+        for g, (opb, _) in enumerate(site.lm):
+            terms.append((Tp(tG[g]), ti, 1.0, self.op_index(opb)))
+        for (kop, c) in site.field:
+            terms.append((Xp, ti, c, self.op_index(kop)))
+        mx.add(base, tc, Dl, d, Dr, terms)
+        for g, (opa, _) in enumerate(site.mr):
+            mx.add(Up(g), ti, Dl, d, Dr, [(Xp, ti, 1.0, self.op_index(opa))])
+        segs = []
+        if p < self.n - 1:
+            segs.append((Xp, Dr, 0, self.R_lp(p + 1, 0, ti), Dr, 1, Dr, 1.0))
+        for g in range(nmr):
+            segs.append((Up(g), Dr, 0, LCp(nlm + g), Dr, 1, Dr, 1.0))
+        for u, (raw, _) in enumerate(site.strad):
+            B = LCp(nlm + nmr + u) if site.strad_left else self.R_lp(p + 1, 1 + raw, ti)
+            segs.append((Tp(tS[u]), Dr, 0, B, Dr, 1, Dr, 1.0))
+        g2.add(self.w.data_ptr(), Dl * d, Dr, Dr, segs, c0=base, ldc0=Dr, beta=1.0)
+        ph['mv_lc'] = lc.arrays()
+        ph['mv_g1'] = g1.arrays()
+        ph['mv_mix'] = mx.arrays()
+        ph['mv_g2'] = g2.arrays()
+        tol = self.eigs_tol if self.eigs_tol > 0 else 2.220446049250313e-16
+        ph['lz'] = np.array([(self.V.data_ptr(), self.w.data_ptr(), Xp, self.init.data_ptr(),
+                              self.mps[p].data_ptr(), n_el, psi, min(self.engine.ncv, n_el), 0,
+                              tol / self.tau, self.tau, floor_rel, 0.0)], dtype=L.LANCZOS_SOLVE)
+        ph['prec'] = prec
+        ph['mv_flops'] = self._matvec_flops(p)
+
+    def _matvec_flops(self, p):
+        """Algorithmic flops of one matvec at site p (GEMM terms executed)."""
+        s = self.T.sites[p]
+        Dl, Dr, d = self.dims[p], self.dims[p + 1], self.d
+        n1 = (p > 0) + len(s.lm) + len(s.strad)
+        n2 = (p < self.n - 1) + len(s.mr) + len(s.strad)
+        return 2.0 * d * Dl * Dr * (Dl * n1 + Dr * n2)
+
+    def _plan_svd(self, ph, q, dirn):
+        d, dims = self.d, self.dims
+        if dirn > 0:
+            Dl, Dm, Dn = dims[q], dims[q + 1], dims[q + 2]
+            m, nn = Dl * d, Dm
+            if m < nn:
+                raise ValueError('SVD move needs Dl*d >= Dr (site %d)' % q)
+            ph['svd'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(), self.Rq.data_ptr(),
+                                   _p(self.Rq, nn), self.qrw.data_ptr(), m, nn, nn, 0, 0, 0)],
+                                 dtype=L.SVD_JOB)
+            g = R.GemmList()
+            g.add(self.init.data_ptr(), nn, d * Dn, d * Dn,
+                  [(_p(self.Rq, nn), nn, 0, self.mps[q + 1].data_ptr(), d * Dn, 0, Dm, 1.0)])
+            mx = R.MixList()
+            mx.add(self.mps[q + 1].data_ptr(), L.F64, 1, 1, nn * d * Dn,
+                   [(self.init.data_ptr(), L.F64, 1.0, -1)])
+            ph['lm_bond'] = q
+        else:
+            Dp, Dm, Dr = dims[q - 1], dims[q], dims[q + 1]
+            m, nn = d * Dr, Dm
+            if m < nn:
+                raise ValueError('SVD move needs d*Dr >= Dl (site %d)' % q)
+            ph['svd'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(), self.Rq.data_ptr(),
+                                   _p(self.Rq, nn), self.qrw.data_ptr(), m, nn, m, 1, 1, 0)],
+                                 dtype=L.SVD_JOB)
+            g = R.GemmList()
+            g.add(self.init.data_ptr(), Dp * d, nn, nn,
+                  [(self.mps[q - 1].data_ptr(), Dm, 0, _p(self.Rq, nn), nn, 1, Dm, 1.0)])
+            mx = R.MixList()
+            mx.add(self.mps[q - 1].data_ptr(), L.F64, 1, 1, Dp * d * nn,
+                   [(self.init.data_ptr(), L.F64, 1.0, -1)])
+            ph['lm_bond'] = q - 1
+        ph['absorb'] = g.arrays()
+        ph['copy'] = mx.arrays()
+        ph['svd_n'] = nn
+
+    # ----------------------------------------------------------- host view
+    def host_mps(self):
+        out = []
+        for s in range(self.n):
+            a = self.mps[s][:self.dims[s] * self.d * self.dims[s + 1]].cpu().numpy()
+            out.append(a.reshape(self.dims[s], self.d, self.dims[s + 1]).copy())
+        return out
+
+
+class Engine:
+    """Owns the device, the stream, the operator table and the launch loop."""
+
+    def __init__(self, precision_schedule=None, ncv=20, max_restarts=20, min_iters=1,
+                 device='cuda:0', lowp=True, ops_capacity=4096):
+        require_gpu()
+        self.lib = L.load()
+        self.device = torch.device(device)
+        self.ncv = int(ncv)
+        if not 2 <= self.ncv <= L.LZ_NCV_MAX:
+            raise ValueError('ncv must be in [2, %d]' % L.LZ_NCV_MAX)
+        self.max_restarts = int(max_restarts)
+        self.min_iters = int(min_iters)
+        self.precision_schedule = precision_schedule or (lambda sweep, nsweeps: 'fp64')
+        self.uses_lowp = bool(lowp)
+        self.up = Uploader(self.device)
+        self.ops_capacity = ops_capacity
+        self.ops = torch.zeros(ops_capacity * 16, dtype=torch.float64, device=self.device)
+        self.ops_free = [(0, ops_capacity)]
+        self.ops_d2 = None
+        self.solves = []
+        self.lz_state = None
+        self.counters = {'matvec_flops': 0.0, 'matvecs': 0, 'lanczos_calls': 0, 'steps': 0,
+                         'lz_time': 0.0}
+
+    # ------------------------------------------------------------ op table
+    def alloc_ops(self, ops):
+        d2 = ops.shape[1] * ops.shape[2]
+        if self.ops_d2 is None:
+            self.ops_d2 = d2
+        if d2 != self.ops_d2:
+            raise ValueError('all solves of an engine must share the physical dimension')
+        cnt = ops.shape[0]
+        for t, (s, c) in enumerate(self.ops_free):
+            if c >= cnt:
+                self.ops_free[t] = (s + cnt, c - cnt)
+                flat = torch.from_numpy(np.ascontiguousarray(ops).reshape(-1)).to(self.device)
+                self.ops[s * d2:(s + cnt) * d2].copy_(flat)
+                self._ops_owner = getattr(self, '_ops_owner', {})
+                self._ops_owner[s] = cnt
+                return s
+        raise RuntimeError('operator table full')
+
+    def free_ops(self, slot):
+        cnt = self._ops_owner.pop(slot, 0)
+        if cnt:
+            self.ops_free.append((slot, cnt))
+
+    def stream(self):
+        return ctypes_void(torch.cuda.current_stream().cuda_stream)
+
+    # ------------------------------------------------------------- solves
+    def add_solve(self, para, seed=0, mps=None, tag=None):
+        s = Solve(self, para, seed=seed, mps=mps, tag=tag)
+        self.solves.append(s)
+        return s
+
+    def remove_solve(self, s):
+        if s in self.solves:
+            self.solves.remove(s)
+        s.free()
+
+    # ------------------------------------------------------------- launch
+    def _gemm(self, mode, parts, actives=None):
+        p, sg, nt = R.concat_gemm(parts, actives)
+        if nt == 0:
+            return
+        dp, ds = self.up.upload([p, sg])
+        L.check(self.lib.tnx_gemm_grouped(mode, ctypes_void(dp), len(p), ctypes_void(ds), nt, None,
+                                          self.stream()), 'tnx_gemm_grouped')
+
+    def _mix(self, parts):
+        j, t, nb = R.concat_mix(parts)
+        if nb == 0:
+            return
+        dj, dt = self.up.upload([j, t])
+        L.check(self.lib.tnx_mix(ctypes_void(dj), len(j), ctypes_void(dt), nb,
+                                 ctypes_void(self.ops.data_ptr()), None, self.stream()), 'tnx_mix')
+
+    def _qr(self, jobs):
+        if len(jobs) == 0:
+            return
+        (dj,) = self.up.upload([jobs])
+        L.check(self.lib.tnx_qr_f64(ctypes_void(dj), len(jobs), self.stream()), 'tnx_qr_f64')
+
+    def _svd(self, jobs, max_sweeps=60):
+        if len(jobs) == 0:
+            return
+        (dj,) = self.up.upload([jobs])
+        L.check(self.lib.tnx_svd_jacobi_f64(ctypes_void(dj), len(jobs), max_sweeps, self.stream()),
+                'tnx_svd_jacobi_f64')
+
+    def _lanczos(self, group):
+        """group: list of (solve, plan) sharing one precision."""
+        prec = group[0][1]['prec']
+        ti, tc, mode1, mode2, _ = PREC[prec]
+        actives = list(range(len(group)))
+        lc_j, lc_t, lc_nb = R.concat_mix([ph['mv_lc'] for _, ph in group])
+        g1p, g1s, g1t = R.concat_gemm([ph['mv_g1'] for _, ph in group], actives)
+        mxj, mxt, mxb = R.concat_mix([ph['mv_mix'] for _, ph in group], actives)
+        g2p, g2s, g2t = R.concat_gemm([ph['mv_g2'] for _, ph in group], actives)
+        lz = np.concatenate([ph['lz'] for _, ph in group])
+        chunks = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
+        lz['chunk_begin'] = np.concatenate([[0], np.cumsum(chunks)[:-1]])
+        nchunks = int(chunks.sum())
+        ptrs = self.up.upload([lc_j, lc_t, g1p, g1s, mxj, mxt, g2p, g2s, lz])
+        if lc_nb:
+            L.check(self.lib.tnx_mix(ctypes_void(ptrs[0]), len(lc_j), ctypes_void(ptrs[1]), lc_nb,
+                                     ctypes_void(self.ops.data_ptr()), None, self.stream()),
+                    'tnx_mix')
+        plan = L.MatvecPlan(ptrs[2], ptrs[3], len(g1p), g1t, ptrs[4], ptrs[5], len(mxj), mxb,
+                            ptrs[6], ptrs[7], len(g2p), g2t, self.ops.data_ptr(), mode1, mode2)
+        need = int(self.lib.tnx_lanczos_state_bytes(len(group), nchunks, self.ncv))
+        if self.lz_state is None or self.lz_state.numel() < need:
+            self.lz_state = torch.empty(max(need, 1 << 20), dtype=torch.uint8, device=self.device)
+        res = np.zeros(len(group), dtype=L.LANCZOS_RESULT)
+        t0 = time.perf_counter()
+        rc = self.lib.tnx_lanczos_run_ex(plan, ctypes_void(ptrs[8]), len(group), nchunks, self.ncv,
+                                         self.max_restarts, self.min_iters, ti, tc,
+                                         ctypes_void(self.lz_state.data_ptr()),
+                                         ctypes_void(res.ctypes.data), self.stream())
+        self.counters['lz_time'] += time.perf_counter() - t0
+        if rc not in (0, -3):
+            L.check(rc, 'tnx_lanczos_run')
+        self.counters['lanczos_calls'] += 1
+        for (s, ph), r in zip(group, res):
+            s.last_theta = float(r['theta'])
+            s.thetas.append(s.last_theta)
+            s.stats['matvecs'] += int(r['iters'])
+            if not r['converged']:
+                s.stats['unconverged'] += 1
+            self.counters['matvecs'] += int(r['iters'])
+            self.counters['matvec_flops'] += ph['mv_flops'] * int(r['iters'])
+
+    def step(self, solves=None):
+        """Advance every solve with a queued entry by one step."""
+        solves = [s for s in (solves or self.solves) if s.queue and s.failed is None]
+        if not solves:
+            return 0
+        plans = []
+        for s in solves:
+            st = s.queue.pop(0)
+            nsw = s.para.get('sweep_time', 1)
+            prec = self.precision_schedule(max(st.sweep, 0), nsw) if st.kind == 'solve' else 'fp64'
+            if self.uses_lowp and prec != 'fp64' and s.lowp_dtype != PREC[prec][0]:
+                self._mix([s.cast_all_records(PREC[prec][0])])
+                s.lowp_dtype = PREC[prec][0]
+            ph = s.plan(st, prec)
+            plans.append((s, st, ph))
+
+        def sel(key):
+            return [ph[key] for _, _, ph in plans if key in ph]
+
+        # one upload for every descriptor of the step's pre-Lanczos phases
+        qr = R.concat_plain(sel('qr'), L.QR_JOB)
+        svd = R.concat_plain(sel('svd'), L.SVD_JOB)
+        gemms = [R.concat_gemm(sel(k)) for k in ('absorb', 'env_g1', 'env_gT')]
+        mixes = [R.concat_mix(sel(k)) for k in ('copy', 'env_lc', 'env_mix', 'cast')]
+        arrs = [qr, svd]
+        for p, sg, _ in gemms:
+            arrs += [p, sg]
+        for j, t, _ in mixes:
+            arrs += [j, t]
+        ptr = self.up.upload(arrs)
+        st_ = self.stream()
+        lib = self.lib
+        opsp = ctypes_void(self.ops.data_ptr())
+
+        def run_gemm(i):
+            p, sg, nt = gemms[i]
+            if nt:
+                L.check(lib.tnx_gemm_grouped(L.GEMM_F64, ctypes_void(ptr[2 + 2 * i]), len(p),
+                                             ctypes_void(ptr[3 + 2 * i]), nt, None, st_), 'gemm')
+
+        def run_mix(i):
+            j, t, nb = mixes[i]
+            if nb:
+                L.check(lib.tnx_mix(ctypes_void(ptr[8 + 2 * i]), len(j), ctypes_void(ptr[9 + 2 * i]),
+                                    nb, opsp, None, st_), 'mix')
+
+        if len(qr):
+            L.check(lib.tnx_qr_f64(ctypes_void(ptr[0]), len(qr), st_), 'tnx_qr_f64')
+        if len(svd):
+            L.check(lib.tnx_svd_jacobi_f64(ctypes_void(ptr[1]), len(svd), 60, st_), 'tnx_svd')
+        run_gemm(0)      # absorb R into the neighbour
+        run_mix(0)       # copy back (moves)
+        run_mix(1)       # linear combinations for the environment update
+        run_gemm(1)      # X . Q
+        run_mix(2)       # physical mixing
+        run_gemm(2)      # Q^T . W
+        run_mix(3)       # low-precision copies of the new block
+        groups = {}
+        for s, st, ph in plans:
+            if 'lz' in ph:
+                groups.setdefault(ph['prec'], []).append((s, ph))
+        for prec in sorted(groups):
+            self._lanczos(groups[prec])
+        for s, st, ph in plans:
+            s.stats['steps'] += 1
+            if st.kind == 'solve':
+                s.center = st.site
+            elif st.kind in ('move', 'svdmove'):
+                s.center = st.site + st.dir
+            if 'lm_bond' in ph:
+                s.lm[ph['lm_bond']] = s.Rq[:ph['svd_n']].cpu().numpy().copy()
+        self.counters['steps'] += 1
+        return len(plans)
+
+    def run_queues(self, solves=None):
+        while self.step(solves):
+            pass

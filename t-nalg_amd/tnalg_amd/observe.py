@@ -1,0 +1,196 @@
+"""Device expectation values of MPS states (one- and two-point functions).
+
+Restates the observation routines of ``library/MPSClass.py:1105-1212``
+(``observation_s1``, ``observation_s1_s2``, ``observe_magnetization``,
+``observe_bond_energy``, ``observe_bond_energy_from_jxyz``,
+``observe_correlators_from_middle``) as one gauge-independent pass:
+
+* a right-to-left pass builds the right identity environments ``F_b``
+  (``F_N = 1``, ``F_b = sum_s A_s F_{b+1} A_s^T``);
+* a left-to-right pass carries the left identity environment ``E_b`` and one
+  "string" matrix per open left operator ``(i, a)`` needed by the requested
+  pairs; at site ``s`` every carried matrix ``X`` is transferred through
+  ``A_s`` once (``Z = X A_s``), mixed with the operators that close there,
+  contracted back (``A_s^T (O Z)``) and closed against ``F_{s+1}`` with a
+  Frobenius product:  ``<O_a(i) O_b(j)> = < T(X_(i,a), A_j, O_b), F_{j+1} >``.
+
+All sites' transfers of all solves are batched into one grouped-GEMM /
+mixer / GEMM / dot launch per site.  Values are divided by the norm
+``<psi|psi> = Tr E_N``.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import records as R
+from .device import ctypes_void
+
+
+def _p(t, off=0):
+    return t.data_ptr() + 8 * off
+
+
+class _MeasurePlan:
+    def __init__(self, solve, one, two):
+        self.s = solve
+        n = solve.n
+        self.one = [(int(i), int(k)) for i, k in one]
+        rows = []
+        for (i, j, a, b) in two:
+            i, j, a, b = int(i), int(j), int(a), int(b)
+            if i == j:
+                raise ValueError('two-point rows need two different sites (got %d, %d)' % (i, j))
+            if i > j:
+                i, j, a, b = j, i, b, a
+            rows.append((i, j, a, b))
+        self.two = rows
+        # chains: (i, a) -> last site needed
+        self.chain_end = {}
+        for (i, j, a, b) in rows:
+            self.chain_end[(i, a)] = max(self.chain_end.get((i, a), -1), j)
+        self.chains = sorted(self.chain_end)
+        self.cidx = {c: t for t, c in enumerate(self.chains)}
+        dm = solve.dmax
+        self.dsq = max(x * x for x in dm)
+        self.psi = solve.psi_max
+        nc = max(1, len(self.chains))
+        dev = solve.dev
+        f64 = torch.float64
+        # close counts per site
+        self.close_at = [[] for _ in range(n)]   # (kind, ref, op, out_index)
+        self.values = np.zeros(len(self.one) + len(rows))
+        for t, (i, k) in enumerate(self.one):
+            self.close_at[i].append(('E', None, k, t))
+        for t, (i, j, a, b) in enumerate(rows):
+            self.close_at[j].append(('X', self.cidx[(i, a)], b, len(self.one) + t))
+        nclose = max(1, max(len(c) for c in self.close_at))
+        self.E = torch.zeros((n + 1) * self.dsq, dtype=f64, device=dev)
+        self.F = torch.zeros((n + 1) * self.dsq, dtype=f64, device=dev)
+        self.X = [torch.zeros(nc * self.dsq, dtype=f64, device=dev) for _ in range(2)]
+        self.Z = torch.zeros((nc + 1) * self.psi, dtype=f64, device=dev)
+        self.M = torch.zeros((nclose + nc) * self.psi, dtype=f64, device=dev)
+        self.Y = torch.zeros(nclose * self.dsq, dtype=f64, device=dev)
+        self.out = torch.zeros(len(self.values) + 1, dtype=f64, device=dev)
+        self.E[0] = 1.0
+        self.F[n * self.dsq] = 1.0
+
+    def F_at(self, b):
+        return _p(self.F, b * self.dsq)
+
+    def E_at(self, b):
+        return _p(self.E, b * self.dsq)
+
+
+def measure(engine, solves, requests):
+    """``requests[k] = (one, two)`` for ``solves[k]``: ``one`` = [(site, op)],
+    ``two`` = [(i, j, op_i, op_j)].  Returns, per solve, ``(one_vals, two_vals,
+    norm)`` as numpy arrays (real parts)."""
+    plans = [_MeasurePlan(s, one, two) for s, (one, two) in zip(solves, requests)]
+    lib = engine.lib
+    stream = engine.stream()
+    nmax = max(s.n for s in solves)
+    up = engine.up
+
+    def launch_gemm(g):
+        p, sg = g.arrays()
+        nt = R.gemm_tiles(p)
+        if nt == 0:
+            return
+        dp, ds = up.upload([p, sg])
+        L.check(lib.tnx_gemm_grouped(L.GEMM_F64, ctypes_void(dp), len(p), ctypes_void(ds), nt, None,
+                                     stream), 'gemm')
+
+    def launch_mix(mx):
+        j, t = mx.arrays()
+        nb = R.mix_blocks(j)
+        if nb == 0:
+            return
+        dj, dt = up.upload([j, t])
+        L.check(lib.tnx_mix(ctypes_void(dj), len(j), ctypes_void(dt), nb,
+                            ctypes_void(engine.ops.data_ptr()), None, stream), 'mix')
+
+    # right identity environments F_b, b = N-1 .. 0
+    for b in range(nmax - 1, -1, -1):
+        g1, g2 = R.GemmList(), R.GemmList()
+        for P in plans:
+            s = P.s
+            if b >= s.n:
+                continue
+            d, Dl, Dr = s.d, s.dims[b], s.dims[b + 1]
+            A = s.mps[b].data_ptr()
+            g1.add(_p(P.Z), Dl * d, Dr, Dr, [(A, Dr, 0, P.F_at(b + 1), Dr, 1, Dr, 1.0)])
+            g2.add(P.F_at(b), Dl, Dl, Dl, [(A, d * Dr, 0, _p(P.Z), d * Dr, 1, d * Dr, 1.0)])
+        launch_gemm(g1)
+        launch_gemm(g2)
+
+    # left pass with strings
+    dot_out = torch.zeros(1024, dtype=torch.float64, device=solves[0].dev)
+    dot_used = 0
+    dot_map = []
+    for site in range(nmax):
+        g1, mx, g2 = R.GemmList(), R.MixList(), R.GemmList()
+        dots = []
+        for P in plans:
+            s = P.s
+            if site >= s.n:
+                continue
+            d, Dl, Dr = s.d, s.dims[site], s.dims[site + 1]
+            A = s.mps[site].data_ptr()
+            cur, nxt = P.X[site % 2], P.X[(site + 1) % 2]
+            psi = P.psi
+            Zp = lambda t: _p(P.Z, t * psi)
+            Mp = lambda t: _p(P.M, t * psi)
+            # Z_E and Z_X for chains alive at this site (started before it)
+            g1.add(Zp(0), Dl, d * Dr, d * Dr, [(P.E_at(site), Dl, 0, A, d * Dr, 0, Dl, 1.0)])
+            g2.add(P.E_at(site + 1), Dr, Dr, Dr, [(A, Dr, 1, Zp(0), Dr, 0, Dl * d, 1.0)])
+            alive = [c for c in P.chains if c[0] < site <= P.chain_end[c]]
+            for (i, a) in alive:
+                t = P.cidx[(i, a)]
+                g1.add(Zp(1 + t), Dl, d * Dr, d * Dr,
+                       [(_p(cur, t * P.dsq), Dl, 0, A, d * Dr, 0, Dl, 1.0)])
+                if P.chain_end[(i, a)] > site:
+                    g2.add(_p(nxt, t * P.dsq), Dr, Dr, Dr, [(A, Dr, 1, Zp(1 + t), Dr, 0, Dl * d, 1.0)])
+            nm = 0
+            for (kind, ref, op, out) in P.close_at[site]:
+                src = Zp(0) if kind == 'E' else Zp(1 + ref)
+                mx.add(Mp(nm), L.F64, Dl, d, Dr, [(src, L.F64, 1.0, s.op_index(op))])
+                g2.add(_p(P.Y, nm * P.dsq), Dr, Dr, Dr, [(A, Dr, 1, Mp(nm), Dr, 0, Dl * d, 1.0)])
+                dots.append((_p(P.Y, nm * P.dsq), P.F_at(site + 1), Dr, Dr, Dr, Dr, P, out))
+                nm += 1
+            # strings that start here: X_(site, a) = A^T O_a (E A)
+            for (i, a) in P.chains:
+                if i != site:
+                    continue
+                t = P.cidx[(i, a)]
+                mx.add(Mp(nm), L.F64, Dl, d, Dr, [(Zp(0), L.F64, 1.0, s.op_index(a))])
+                g2.add(_p(nxt, t * P.dsq), Dr, Dr, Dr, [(A, Dr, 1, Mp(nm), Dr, 0, Dl * d, 1.0)])
+                nm += 1
+        launch_gemm(g1)
+        launch_mix(mx)
+        launch_gemm(g2)
+        if dots:
+            jobs = np.array([(x, y, r, c, lx, ly) for (x, y, r, c, lx, ly, _, _) in dots],
+                            dtype=L.DOT_JOB)
+            if dot_used + len(dots) > dot_out.numel():
+                grow = torch.zeros(2 * (dot_used + len(dots)), dtype=torch.float64,
+                                   device=solves[0].dev)
+                grow[:dot_used] = dot_out[:dot_used]
+                dot_out = grow
+            (dj,) = up.upload([jobs])
+            L.check(lib.tnx_trace_dot(ctypes_void(dj), len(jobs),
+                                      ctypes_void(_p(dot_out, dot_used)), stream), 'trace_dot')
+            for k, (_, _, _, _, _, _, P, out) in enumerate(dots):
+                dot_map.append((dot_used + k, P, out))
+            dot_used += len(dots)
+    vals = dot_out[:dot_used].cpu().numpy()
+    for (k, P, out) in dot_map:
+        P.values[out] = vals[k]
+    # norms = E_N (1 x 1)
+    results = []
+    for P in plans:
+        norm = float(P.E[P.s.n * P.dsq].item())
+        vals = P.values / norm
+        results.append((vals[:len(P.one)], vals[len(P.one):], norm))
+    return results

@@ -1,0 +1,171 @@
+"""Compile ``para`` coupling lists into block-environment term tables.
+
+The reference keeps one effective operator per coupling row and bond
+(``library/MPSClass.py:284-611``) and sums them into ``opt_env`` classes at
+every step (``MPSClass.py:715-815``).  The engine instead carries, at every
+bond ``b``, a left block (sites ``< b``) made of ``H_L`` plus one matrix per
+*open* operator ``(i, a)`` (site ``i < b`` whose operator ``a`` still has a
+partner ``j >= b``), and the mirror-image right block.  The effective
+Hamiltonian at site ``p`` is then (``x`` = the ``(Dl, d, Dr)`` site tensor)
+
+    Heff x = H_L x + x H_R^T + h_p x
+             + sum_b O_b (Lt_b x)              Lt_b = sum_{rows (i,p,a,b)} c L[(i,a)]
+             + sum_a O_a (x Rt_a^T)            Rt_a = sum_{rows (p,j,a,b)} c R[(j,b)]
+             + sum_{rows i<p<j} c L[(i,a)] x R[(j,b)]^T
+
+which is the reference's ``Heff`` term for term (same rows, same ``|c| > tol``
+cut of ``MPSClass.py:725-734``).  The straddling double sum is regrouped
+over the smaller side: ``sum_l L_l x (sum_r C[l,r] R_r)^T`` (or the mirror),
+i.e. an exact refactorisation with ``min(#left, #right)`` GEMM pairs.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class SiteTerms:
+    field: list            # [(op, coef)] on-site operators at p
+    lm: list               # [(op_b, [(left_open_idx, coef)])]  groups paired with O_b at p
+    mr: list               # [(op_a, [(right_open_idx, coef)])] groups paired with O_a at p
+    strad_left: bool       # True: factors are raw left opens, right side combined
+    strad: list            # [(raw_idx, [(other_idx, coef)])]
+
+
+@dataclass
+class Terms:
+    n: int
+    d: int
+    ops: np.ndarray                       # (n_ops, d, d) float64
+    left_opens: list = field(default_factory=list)    # per bond b: [(i, a)]
+    right_opens: list = field(default_factory=list)   # per bond b: [(j, b)]
+    sites: list = field(default_factory=list)         # per site: SiteTerms
+    n_two: int = 0
+    n_one: int = 0
+
+    def left_index(self, b):
+        return {k: t for t, k in enumerate(self.left_opens[b])}
+
+    def right_index(self, b):
+        return {k: t for t, k in enumerate(self.right_opens[b])}
+
+
+def _real_ops(ops, used):
+    arr = []
+    for k, o in enumerate(ops):
+        o = np.asarray(o)
+        if k in used and np.iscomplexobj(o) and np.abs(o.imag).max() > 0:
+            raise ValueError('operator %d has an imaginary part; the real-arithmetic engine '
+                             'supports real Hamiltonians (is_real=True)' % k)
+        arr.append(np.real(o).astype(np.float64))
+    return np.stack(arr)
+
+
+def compile_terms(para):
+    """Build the :class:`Terms` of a ``para`` dict (keys ``l, op, index1, coeff1,
+    index2, coeff2, eigs_tol``)."""
+    n = int(para['l'])
+    tol = float(para.get('eigs_tol', 0.0))
+    index1 = np.asarray(para['index1'], dtype=int).reshape(-1, 2)
+    index2 = np.asarray(para['index2'], dtype=int).reshape(-1, 4)
+    coeff1 = np.asarray(para['coeff1'], dtype=float).reshape(-1)
+    coeff2 = np.asarray(para['coeff2'], dtype=float).reshape(-1)
+    ops_in = para['op']
+    two, one = [], []
+    used = set()
+    for r in range(index2.shape[0]):
+        c = coeff2[r]
+        if not abs(c) > tol:
+            continue
+        i, j, a, b = (int(x) for x in index2[r])
+        if i == j:
+            raise ValueError('index2 row %d couples site %d to itself' % (r, i))
+        if i > j:
+            i, j, a, b = j, i, b, a
+        if not (0 <= i and j < n):
+            raise ValueError('index2 row %d has a site outside 0..%d' % (r, n - 1))
+        two.append((i, j, a, b, float(c)))
+        used.update((a, b))
+    for r in range(index1.shape[0]):
+        c = coeff1[r]
+        q, k = (int(x) for x in index1[r])
+        if abs(c) > tol and np.linalg.norm(np.asarray(ops_in[k]).reshape(-1)) > tol:
+            if not 0 <= q < n:
+                raise ValueError('index1 row %d has a site outside 0..%d' % (r, n - 1))
+            one.append((q, k, float(c)))
+            used.add(k)
+    ops = _real_ops(ops_in, used)
+    d = ops.shape[1]
+    if 'd' in para and int(para['d']) != d:
+        raise ValueError('para["d"]=%s but the operators are %dx%d' % (para['d'], d, d))
+    T = Terms(n=n, d=d, ops=ops, n_two=len(two), n_one=len(one))
+    for b in range(n + 1):
+        T.left_opens.append(sorted({(i, a) for (i, j, a, _, _) in two if i < b <= j}))
+        T.right_opens.append(sorted({(j, bb) for (i, j, _, bb, _) in two if i < b <= j}))
+    for p in range(n):
+        lidx = T.left_index(p)
+        ridx = T.right_index(p + 1)
+        fld = {}
+        for (q, k, c) in one:
+            if q == p:
+                fld[k] = fld.get(k, 0.0) + c
+        lm, mr = {}, {}
+        pairs = {}
+        for (i, j, a, b, c) in two:
+            if j == p:
+                lm.setdefault(b, []).append((lidx[(i, a)], c))
+            elif i == p:
+                mr.setdefault(a, []).append((ridx[(j, b)], c))
+            elif i < p < j:
+                key = (lidx[(i, a)], ridx[(j, b)])
+                pairs[key] = pairs.get(key, 0.0) + c
+        lset = sorted({l for (l, _) in pairs})
+        rset = sorted({r for (_, r) in pairs})
+        strad_left = len(lset) <= len(rset)
+        strad = []
+        if strad_left:
+            for l in lset:
+                strad.append((l, [(r, c) for (ll, r), c in sorted(pairs.items()) if ll == l]))
+        else:
+            for r in rset:
+                strad.append((r, [(l, c) for (l, rr), c in sorted(pairs.items()) if rr == r]))
+        T.sites.append(SiteTerms(field=sorted(fld.items()),
+                                 lm=sorted((b, v) for b, v in lm.items()),
+                                 mr=sorted((a, v) for a, v in mr.items()),
+                                 strad_left=strad_left, strad=strad))
+    return T
+
+
+def dense_heff(T, p, left_env, right_env, dl, dr):
+    """Dense ``Heff`` at site ``p`` from explicit block environments (host
+    numpy; used by the tests to check the term tables against the oracle).
+    ``left_env = (HL, [L_open...])``, ``right_env = (HR, [R_open...])``."""
+    d = T.d
+    dim = dl * d * dr
+    H = np.zeros((dim, dim))
+    el, ed, er = np.eye(dl), np.eye(d), np.eye(dr)
+
+    def k3(a, b, c):
+        return np.kron(np.kron(a, b), c)
+
+    st = T.sites[p]
+    HL, Lo = left_env
+    HR, Ro = right_env
+    if p > 0:
+        H += k3(HL, ed, er)
+    if p < T.n - 1:
+        H += k3(el, ed, HR)
+    for k, c in st.field:
+        H += c * k3(el, T.ops[k], er)
+    for b, grp in st.lm:
+        Lt = sum(c * Lo[l] for l, c in grp)
+        H += k3(Lt, T.ops[b], er)
+    for a, grp in st.mr:
+        Rt = sum(c * Ro[r] for r, c in grp)
+        H += k3(el, T.ops[a], Rt)
+    for raw, grp in st.strad:
+        if st.strad_left:
+            H += k3(Lo[raw], ed, sum(c * Ro[r] for r, c in grp))
+        else:
+            H += k3(sum(c * Lo[l] for l, c in grp), ed, Ro[raw])
+    return H

@@ -1,0 +1,101 @@
+"""End-to-end parity of the GPU DMRG engine (``dmrg_finite_size``) with the
+oracle, exact diagonalisation and the reference's saved runs.
+
+Golden numbers of the reference runs (``data_dmrg/chainN12_j(1,1)_h(0,0)_chi16open.pr``
+and ``..._chi24open.pr``, N=12 open chain, every coeff2 = 1): converged
+energies -8.676296538186877 (chi=16) and -8.676296659449305 (chi=24); chi=16
+bond energies eb_full[0:3] = (-0.44060803847908186, -0.44060803847908186,
+-0.20531997892240417), eb[0:2] = (-1.086536055880568, -0.5156163135305695),
+ent[5] = 0.567208838192397, lm[5][:4] = (0.92381354272716787,
+0.24934275621291499, 0.24934275619104823, 0.14810633922157662).
+"""
+import numpy as np
+import pytest
+
+from oracle import ed, model
+
+pytestmark = pytest.mark.gpu
+
+
+def _para(couplings, chi, sweeps, tau=1e-4, eigs_tol=1e-10, break_tol=1e-14, dt_ob=1):
+    from tnalg_amd import parameters as P
+    para = P.common_parameters_dmrg()
+    para.update(couplings)
+    para.update(chi=chi, sweep_time=sweeps, tau=tau, eigs_tol=eigs_tol, break_tol=break_tol,
+                dt_ob=dt_ob, spin='half', d=2, lattice='arbitrary')
+    para['nh'] = para['index2'].shape[0]
+    return para
+
+
+def test_heisenberg_n8_exact(gpu):
+    from tnalg_amd import dmrg_finite_size
+    c = model.chain_couplings(8, 1.0, 1.0, hz=0.2)
+    para = _para(c, 16, 6)
+    print('case: N=8 chi=16 heisenberg hz=0.2', flush=True)
+    ob, A, info, _ = dmrg_finite_size(para, seed=1)
+    e0, _ = ed.ground_state(8, c['op'], c['index1'], c['coeff1'], c['index2'], c['coeff2'])
+    e = float(ob['e_per_site'][0]) * 8
+    assert abs(e - e0) < 1e-10 * abs(e0), (e, e0)
+    assert abs(A.norm_mps() - 1) < 1e-12
+
+
+def test_fixture_chain_chi16(gpu):
+    from tnalg_amd import dmrg_finite_size
+    c = model.chain_couplings(12, 1.0, 1.0, convention='easystart')
+    para = _para(c, 16, 30, tau=1e-3, eigs_tol=1e-12)
+    print('case: fixture N=12 chi=16', flush=True)
+    ob, A, info, _ = dmrg_finite_size(para, seed=3)
+    e = float(ob['e_per_site'][0]) * 12
+    assert abs(e - (-8.676296538186877)) < 2e-8 * 8.7, e
+    assert np.allclose(ob['eb_full'][:3, 0], [-0.44060803847908186, -0.44060803847908186,
+                                              -0.20531997892240417], atol=2e-7)
+    assert np.allclose(ob['eb'][:2, 0], [-1.086536055880568, -0.5156163135305695], atol=2e-7)
+    assert np.allclose(A.lm[5][:4], [0.92381354272716787, 0.24934275621291499,
+                                     0.24934275619104823, 0.14810633922157662], atol=1e-6)
+    assert abs(A.ent[5, 0] - 0.567208838192397) < 1e-6
+
+
+def test_matches_oracle_c1(gpu):
+    """Config-1 Hamiltonian (Parameters convention) with a field: GPU vs the
+    reference restatement from the same seeded MPS, converged."""
+    from oracle import dmrg_ref
+    from tnalg_amd import dmrg_finite_size
+    c = model.chain_couplings(12, 1.0, 1.0, hz=0.15)
+    para = _para(c, 16, 12)
+    print('case: C1-like N=12 chi=16 hz=0.15', flush=True)
+    ob, A, info, _ = dmrg_finite_size(para, seed=5)
+    ref = dmrg_ref.make_para(c, 16, 12, eigs_tol=1e-10, break_tol=1e-14, dt_ob=1)
+    rob, rA, rinfo, _ = dmrg_ref.dmrg_finite_size(ref, seed=5)
+    e, er = float(ob['e_per_site'][0]), float(rob['e_per_site'][0])
+    assert abs(e - er) < 1e-9 * abs(er), (e, er)
+    assert np.allclose(ob['mz'], rob['mz'], atol=1e-6)
+    assert np.allclose(ob['eb_full'], rob['eb_full'], atol=1e-6)
+    assert np.allclose(ob['corr_z'], rob['corr_z'], atol=1e-6)
+    for b in range(11):
+        k = min(len(A.lm[b]), len(rA.lm[b]))
+        assert np.allclose(A.lm[b][:k], rA.lm[b][:k], atol=1e-5)
+
+
+def test_long_range_j1j2_exact(gpu):
+    """Straddling (long-range) terms: J1-J2 chain, chi large enough to be exact."""
+    from tnalg_amd import dmrg_finite_size
+    bonds = [(i, i + 1) for i in range(9)] + [(i, i + 2) for i in range(8)] + [(0, 9), (2, 7)]
+    jj = [1.0] * 9 + [0.4] * 8 + [0.3, -0.2]
+    c = model.couplings_from_bonds(10, bonds, jj, jj, hz=0.1)
+    para = _para(c, 32, 8)
+    print('case: J1-J2 N=10 chi=32 with long bonds', flush=True)
+    ob, A, info, _ = dmrg_finite_size(para, seed=2)
+    e0, _ = ed.ground_state(10, c['op'], c['index1'], c['coeff1'], c['index2'], c['coeff2'])
+    e = float(ob['e_per_site'][0]) * 10
+    assert abs(e - e0) < 1e-9 * abs(e0), (e, e0)
+
+
+@pytest.mark.parametrize('precision', ['fp32', 'bf16'])
+def test_mixed_precision_reaches_fp64(gpu, precision):
+    from tnalg_amd import dmrg_finite_size
+    c = model.chain_couplings(16, 1.0, 1.0, hz=0.1)
+    print('case: mixed', precision, flush=True)
+    ob64, _, _, _ = dmrg_finite_size(_para(c, 32, 8), seed=4, precision='fp64')
+    obm, _, _, _ = dmrg_finite_size(_para(c, 32, 8), seed=4, precision=precision)
+    e64, em = float(ob64['e_per_site'][0]), float(obm['e_per_site'][0])
+    assert abs(e64 - em) < 1e-9 * abs(e64), (e64, em)

@@ -1,0 +1,279 @@
+"""Kernel-level parity of libtnalg_hip.so against numpy (f64 references).
+
+Each test prints the case it runs before launching, so a device fault names it.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _vp(x):
+    return ctypes.c_void_p(int(x))
+
+
+def _stream():
+    import torch
+    return _vp(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev(a, dtype=None):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def _upload(arrs):
+    import torch
+    out = []
+    for a in arrs:
+        b = torch.from_numpy(np.frombuffer(np.ascontiguousarray(a).tobytes(), np.uint8).copy()).cuda()
+        out.append(b)
+    return out
+
+
+MODES = {
+    'f64': (0, 'float64', 'float64', 1e-12),
+    'f32': (1, 'float32', 'float32', 2e-5),
+    'bf16_f32': (2, 'bfloat16', 'float32', 2e-2),
+    'bf16_bf16': (3, 'bfloat16', 'bfloat16', 3e-2),
+}
+
+
+@pytest.mark.parametrize('mode', list(MODES))
+def test_gemm_grouped_matches_numpy(gpu, mode):
+    import torch
+    from tnalg_amd import _lib as L
+    from tnalg_amd import records as R
+    lib = L.load()
+    code, tin, tout, tol = MODES[mode]
+    tin_t, tout_t = getattr(torch, tin), getattr(torch, tout)
+    rng = np.random.RandomState(7)
+    shapes = [(1, 1, 1), (2, 8, 4), (64, 64, 64), (65, 130, 33), (128, 256, 128), (37, 3, 200),
+              (256, 128, 512)]
+    keep = []
+    g = R.GemmList()
+    expect = []
+    outs = []
+    for pi, (M, N, K) in enumerate(shapes):
+        nseg = 1 + pi % 3
+        ref = np.zeros((M, N))
+        segs = []
+        for s in range(nseg):
+            ta, tb = (pi + s) % 2, (pi // 2 + s) % 2
+            A = rng.randn(K, M) if ta else rng.randn(M, K)
+            B = rng.randn(N, K) if tb else rng.randn(K, N)
+            alpha = 0.5 + s
+            Ad, Bd = _dev(A).to(tin_t), _dev(B).to(tin_t)
+            keep += [Ad, Bd]
+            Aq = Ad.to(torch.float64).cpu().numpy()
+            Bq = Bd.to(torch.float64).cpu().numpy()
+            ref += alpha * ((Aq.T if ta else Aq) @ (Bq.T if tb else Bq))
+            segs.append((Ad.data_ptr(), A.shape[1], ta, Bd.data_ptr(), B.shape[1], tb, K, alpha))
+        C0 = rng.randn(M, N)
+        C0d = _dev(C0).to(tout_t)
+        keep.append(C0d)
+        ref += -0.75 * C0d.to(torch.float64).cpu().numpy()
+        C = torch.zeros((M, N), dtype=tout_t, device='cuda')
+        outs.append(C)
+        g.add(C.data_ptr(), M, N, N, segs, c0=C0d.data_ptr(), ldc0=N, beta=-0.75)
+        expect.append(ref)
+    p, s = g.arrays()
+    nt = R.gemm_tiles(p)
+    dp, ds = _upload([p, s])
+    print('gemm mode', mode, 'problems', len(p), 'tiles', nt, flush=True)
+    L.check(lib.tnx_gemm_grouped(code, _vp(dp.data_ptr()), len(p), _vp(ds.data_ptr()), nt, None,
+                                 _stream()), 'gemm')
+    torch.cuda.synchronize()
+    for C, ref, sh in zip(outs, expect, shapes):
+        got = C.to(torch.float64).cpu().numpy()
+        scale = np.abs(ref).max() + 1e-300
+        err = np.abs(got - ref).max() / scale
+        assert err < tol, (mode, sh, err)
+
+
+def test_gemm_active_mask_skips_problems(gpu):
+    import torch
+    from tnalg_amd import _lib as L
+    from tnalg_amd import records as R
+    lib = L.load()
+    A = _dev(np.ones((8, 8)))
+    C1 = torch.full((8, 8), 7.0, dtype=torch.float64, device='cuda')
+    C2 = torch.full((8, 8), 7.0, dtype=torch.float64, device='cuda')
+    g = R.GemmList()
+    g.add(C1.data_ptr(), 8, 8, 8, [(A.data_ptr(), 8, 0, A.data_ptr(), 8, 0, 8, 1.0)], active=0)
+    g.add(C2.data_ptr(), 8, 8, 8, [(A.data_ptr(), 8, 0, A.data_ptr(), 8, 0, 8, 1.0)], active=1)
+    p, s = g.arrays()
+    act = torch.tensor([1, 0], dtype=torch.int32, device='cuda')
+    dp, ds = _upload([p, s])
+    L.check(lib.tnx_gemm_grouped(0, _vp(dp.data_ptr()), 2, _vp(ds.data_ptr()), R.gemm_tiles(p),
+                                 _vp(act.data_ptr()), _stream()), 'gemm')
+    torch.cuda.synchronize()
+    assert torch.all(C1 == 8.0) and torch.all(C2 == 7.0)
+
+
+def test_mix_matches_numpy(gpu):
+    import torch
+    from tnalg_amd import _lib as L
+    from tnalg_amd import records as R
+    lib = L.load()
+    rng = np.random.RandomState(3)
+    ops = rng.randn(5, 2, 2)
+    opsd = _dev(ops.reshape(-1))
+    mx = R.MixList()
+    keep, expect, outs = [], [], []
+    for (Dl, Dr, acc) in [(1, 1, 0), (3, 5, 1), (40, 33, 0), (128, 64, 1)]:
+        srcs = [rng.randn(Dl, 2, Dr) for _ in range(3)]
+        sd = [_dev(s) for s in srcs]
+        keep += sd
+        dst0 = rng.randn(Dl, 2, Dr)
+        dst = _dev(dst0)
+        outs.append(dst)
+        terms = [(sd[0].data_ptr(), L.F64, 0.5, -1), (sd[1].data_ptr(), L.F64, -1.5, 2),
+                 (sd[2].data_ptr(), L.F64, 2.0, 4)]
+        ref = 0.5 * srcs[0] - 1.5 * np.einsum('st,atb->asb', ops[2], srcs[1]) + \
+            2.0 * np.einsum('st,atb->asb', ops[4], srcs[2])
+        if acc:
+            ref = ref + dst0
+        expect.append(ref)
+        mx.add(dst.data_ptr(), L.F64, Dl, 2, Dr, terms, accumulate=acc)
+    j, t = mx.arrays()
+    dj, dt = _upload([j, t])
+    print('mix jobs', len(j), flush=True)
+    L.check(lib.tnx_mix(_vp(dj.data_ptr()), len(j), _vp(dt.data_ptr()), R.mix_blocks(j),
+                        _vp(opsd.data_ptr()), None, _stream()), 'mix')
+    torch.cuda.synchronize()
+    for o, r in zip(outs, expect):
+        assert np.allclose(o.cpu().numpy(), r, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize('m,n,trans', [(4, 2, 0), (2, 16, 0), (256, 128, 0), (256, 128, 1),
+                                       (100, 37, 1), (64, 64, 0), (8, 8, 1)])
+def test_qr_f64(gpu, m, n, trans):
+    import torch
+    from tnalg_amd import _lib as L
+    lib = L.load()
+    rng = np.random.RandomState(m * 31 + n)
+    M = rng.randn(m, n)
+    M[:, -1] = M[:, 0]  # rank deficient column
+    src = M.T.copy() if trans else M.copy()
+    k = min(m, n)
+    srcd = _dev(src)
+    q = torch.zeros(m * k, dtype=torch.float64, device='cuda')
+    r = torch.zeros(k * n, dtype=torch.float64, device='cuda')
+    work = torch.zeros(m * n + m * k, dtype=torch.float64, device='cuda')
+    ld = m if trans else n
+    job = np.array([(srcd.data_ptr(), q.data_ptr(), r.data_ptr(), work.data_ptr(), m, n, ld, trans,
+                     trans, 0)], dtype=L.QR_JOB)
+    (dj,) = _upload([job])
+    print('qr', m, n, trans, flush=True)
+    L.check(lib.tnx_qr_f64(_vp(dj.data_ptr()), 1, _stream()), 'qr')
+    torch.cuda.synchronize()
+    Q = q.cpu().numpy().reshape(k, m).T if trans else q.cpu().numpy().reshape(m, k)
+    Rm = r.cpu().numpy().reshape(k, n)
+    assert np.allclose(Q.T @ Q, np.eye(k), atol=1e-13)
+    assert np.allclose(Q @ Rm, M, atol=1e-12 * np.abs(M).max())
+    assert np.allclose(np.tril(Rm, -1), 0)
+
+
+@pytest.mark.parametrize('m,n,trans', [(4, 2, 0), (256, 128, 0), (256, 128, 1), (33, 33, 0),
+                                       (96, 17, 1)])
+def test_svd_jacobi_f64(gpu, m, n, trans):
+    import torch
+    from tnalg_amd import _lib as L
+    lib = L.load()
+    rng = np.random.RandomState(m + 7 * n)
+    M = rng.randn(m, n) @ np.diag(np.logspace(0, -9, n))
+    src = M.T.copy() if trans else M.copy()
+    srcd = _dev(src)
+    u = torch.zeros(m * n, dtype=torch.float64, device='cuda')
+    s = torch.zeros(n, dtype=torch.float64, device='cuda')
+    svt = torch.zeros(n * n, dtype=torch.float64, device='cuda')
+    work = torch.zeros(m * n + n * n, dtype=torch.float64, device='cuda')
+    job = np.array([(srcd.data_ptr(), u.data_ptr(), s.data_ptr(), svt.data_ptr(), work.data_ptr(),
+                     m, n, m if trans else n, trans, trans, 0)], dtype=L.SVD_JOB)
+    (dj,) = _upload([job])
+    print('svd', m, n, trans, flush=True)
+    L.check(lib.tnx_svd_jacobi_f64(_vp(dj.data_ptr()), 1, 60, _stream()), 'svd')
+    torch.cuda.synchronize()
+    U = u.cpu().numpy().reshape(n, m).T if trans else u.cpu().numpy().reshape(m, n)
+    sv = s.cpu().numpy()
+    SVT = svt.cpu().numpy().reshape(n, n)
+    ref = np.linalg.svd(M, compute_uv=False)
+    assert np.allclose(sv, ref, rtol=1e-10, atol=1e-14 * ref[0])
+    assert np.all(np.diff(sv) <= 0)
+    assert np.allclose(U.T @ U, np.eye(n), atol=1e-10)
+    assert np.allclose(U @ SVT, M, atol=1e-12 * np.abs(M).max())
+
+
+def test_trace_dot(gpu):
+    import torch
+    from tnalg_amd import _lib as L
+    lib = L.load()
+    rng = np.random.RandomState(1)
+    X, Y = rng.randn(37, 37), rng.randn(37, 37)
+    Xd, Yd = _dev(X), _dev(Y)
+    jobs = np.array([(Xd.data_ptr(), Yd.data_ptr(), 37, 37, 37, 37), (Xd.data_ptr(), 0, 37, 37, 37, 0)],
+                    dtype=L.DOT_JOB)
+    (dj,) = _upload([jobs])
+    out = torch.zeros(2, dtype=torch.float64, device='cuda')
+    L.check(lib.tnx_trace_dot(_vp(dj.data_ptr()), 2, _vp(out.data_ptr()), _stream()), 'dot')
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    assert abs(o[0] - np.sum(X * Y)) < 1e-11 and abs(o[1] - np.trace(X)) < 1e-12
+
+
+def test_lanczos_dense_symmetric(gpu):
+    """Lowest eigenpairs of a batch of dense symmetric matrices (matvec = one
+    GEMM per solve) against numpy.linalg.eigh."""
+    import torch
+    from tnalg_amd import _lib as L
+    from tnalg_amd import records as R
+    lib = L.load()
+    rng = np.random.RandomState(11)
+    sizes = [1, 3, 16, 200, 2500, 5000]
+    ncv = 20
+    keep, solves, refs = [], [], []
+    g2 = R.GemmList()
+    for k, n in enumerate(sizes):
+        A = rng.randn(n, n)
+        diag = np.linspace(0.0, 10.0, n)
+        diag[0] = -2.0
+        H = 0.002 * (A + A.T) / 2 + np.diag(diag)
+        w_ref, v_ref = np.linalg.eigh(H)
+        refs.append((w_ref[0], v_ref[:, 0], H))
+        Hd = _dev(H)
+        V = torch.zeros((ncv + 1) * n, dtype=torch.float64, device='cuda')
+        w = torch.zeros(n, dtype=torch.float64, device='cuda')
+        x = torch.zeros(n, dtype=torch.float64, device='cuda')
+        init = _dev(rng.randn(n))
+        out = torch.zeros(n, dtype=torch.float64, device='cuda')
+        keep += [Hd, V, w, x, init, out]
+        g2.add(w.data_ptr(), n, 1, 1, [(Hd.data_ptr(), n, 0, x.data_ptr(), 1, 0, n, 1.0)], active=k)
+        solves.append((V.data_ptr(), w.data_ptr(), x.data_ptr(), init.data_ptr(), out.data_ptr(), n,
+                       n, min(ncv, n), 0, 1e-12 / 1e-4, 1e-4, 0.0, 0.0))
+    lz = np.array(solves, dtype=L.LANCZOS_SOLVE)
+    ch = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
+    lz['chunk_begin'] = np.concatenate([[0], np.cumsum(ch)[:-1]])
+    nch = int(ch.sum())
+    p, s = g2.arrays()
+    dp, ds, dlz = _upload([p, s, lz])
+    plan = L.MatvecPlan(0, 0, 0, 0, 0, 0, 0, 0, dp.data_ptr(), ds.data_ptr(), len(p),
+                        R.gemm_tiles(p), 0, 0, 0)
+    state = torch.zeros(int(lib.tnx_lanczos_state_bytes(len(sizes), nch, ncv)), dtype=torch.uint8,
+                        device='cuda')
+    res = np.zeros(len(sizes), dtype=L.LANCZOS_RESULT)
+    print('lanczos sizes', sizes, flush=True)
+    rc = lib.tnx_lanczos_run_ex(plan, _vp(dlz.data_ptr()), len(sizes), nch, ncv, 50, 1, 0, 0,
+                                _vp(state.data_ptr()), _vp(res.ctypes.data), _stream())
+    assert rc == 0, (rc, lib.tnx_last_error())
+    for k, n in enumerate(sizes):
+        e0, v0, H = refs[k]
+        out = keep[6 * k + 5].cpu().numpy()
+        assert abs(res[k]['theta'] - e0) < 1e-9 * max(1, abs(e0)), (n, res[k], e0)
+        assert abs(abs(out @ v0) - 1) < 1e-8, n
+        assert np.linalg.norm(H @ out - e0 * out) < 1e-6 * max(1, abs(e0))
