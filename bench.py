@@ -1,0 +1,241 @@
+"""Benchmark: DMRG solves/s + p50/p99 latency on one MI355X.
+
+Workload (default): XXZ chain, N=64, open BC, Jxy=1, Jz=1, h=0, chi=128,
+20 Krylov vectors, tol 1e-10 (tau=1e-4), 8 sweeps, seeded random initial
+MPS; a *step* is one batched solve of ``--batch`` such chains (all 8 sweeps,
+observables at the end).  ``value`` = solves/s of the batched mixed-precision
+engine; the plain-f64 path and the CPU baseline are reported beside it, with
+the relative energy difference of every mixed solve against the f64 solve of
+the same seed.
+
+Extra JSON objects: ``roofline`` (dominant kernel = grouped matvec GEMM,
+timed with HIP events on the engine's stream) and ``cpu_baseline`` (the
+oracle's restatement of the reference algorithm timed on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, 't-nalg_amd')):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+MFMA_PEAK = {'fp64': 78.6, 'fp32': 157.3, 'bf16': 2500.0}  # TFLOP/s, dense
+HBM_PEAK = 8000.0  # GB/s
+
+
+def c2_para(n=64, chi=128, sweeps=8, jz=1.0, hz=0.0, tol=1e-10):
+    from tnalg_amd import parameters as P
+    para = P.parameter_dmrg_chain()
+    para.update(P.common_parameters_dmrg())
+    para.update(l=n, chi=chi, jxy=1.0, jz=jz, hz=hz, sweep_time=sweeps, eigs_tol=tol, tau=1e-4,
+                dt_ob=sweeps, break_tol=0.0)
+    return P.make_consistent_parameter_dmrg(para)
+
+
+def run_batch(para, seeds, precision, ncv=20):
+    """One batched solve: every chain of ``seeds`` through all sweeps in lockstep."""
+    import torch
+    from tnalg_amd.dmrg import _schedule_from, observe_energy
+    from tnalg_amd.engine import Engine
+    eng = Engine(precision_schedule=_schedule_from(precision), ncv=ncv,
+                 lowp=precision not in (None, 'fp64'))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solves = [eng.add_solve(para, seed=s) for s in seeds]
+    for s in solves:
+        s.schedule_canonicalise()
+    eng.run_queues(solves)
+    for t in range(para['sweep_time']):
+        for s in solves:
+            s.schedule_sweep(t)
+        eng.run_queues(solves)
+    energies = []
+    for s in solves:
+        ob = observe_energy(eng, s, para)
+        energies.append(float(np.ravel(ob['e_per_site'])[0]) * s.n)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stats = dict(eng.counters)
+    for s in list(solves):
+        eng.remove_solve(s)
+    return dt, np.array(energies), stats, eng
+
+
+def matvec_roofline(para, batch, precision, reps=20):
+    """Average duration of the dominant kernel (the grouped matvec GEMMs of a
+    mid-chain local step, all solves of the batch) with HIP events on the
+    engine's stream; algorithmic flops from the term structure."""
+    import ctypes
+    import torch
+    from tnalg_amd import _lib as L
+    from tnalg_amd import records as R
+    from tnalg_amd.dmrg import _schedule_from
+    from tnalg_amd.engine import PREC, Engine
+    eng = Engine(precision_schedule=_schedule_from(precision), ncv=20,
+                 lowp=precision not in (None, 'fp64'))
+    solves = [eng.add_solve(para, seed=100 + k) for k in range(batch)]
+    for s in solves:
+        s.schedule_canonicalise()
+    eng.run_queues(solves)
+    prec = 'fp64' if precision == 'fp64' else 'bf16'
+    site = para['l'] // 2
+    from tnalg_amd.engine import Step
+    plans = []
+    for s in solves:
+        # walk the centre to the middle with f64 steps, then plan one local step there
+        for p in range(1, site):
+            s.queue.append(Step('solve', p, +1, 0))
+        eng.run_queues([s])
+        if eng.uses_lowp and prec != 'fp64':
+            eng._mix([s.cast_all_records(PREC[prec][0])])
+        plans.append(s.plan(Step('solve', site, +1, para['sweep_time'] - 1), prec))
+    ti, tc, mode1, mode2, _ = PREC[prec]
+    g1p, g1s, g1t = R.concat_gemm([ph['mv_g1'] for ph in plans])
+    g2p, g2s, g2t = R.concat_gemm([ph['mv_g2'] for ph in plans])
+    mxj, mxt, mxb = R.concat_mix([ph['mv_mix'] for ph in plans])
+    ptr = eng.up.upload([g1p, g1s, g2p, g2s, mxj, mxt])
+    flops = float(sum(ph['mv_flops'] for ph in plans))
+    st = eng.stream()
+    lib = eng.lib
+    vp = ctypes.c_void_p
+
+    def once():
+        L.check(lib.tnx_gemm_grouped(mode1, vp(ptr[0]), len(g1p), vp(ptr[1]), g1t, None, st), 'g1')
+        L.check(lib.tnx_mix(vp(ptr[4]), len(mxj), vp(ptr[5]), mxb, vp(eng.ops.data_ptr()), None,
+                            st), 'mix')
+        L.check(lib.tnx_gemm_grouped(mode2, vp(ptr[2]), len(g2p), vp(ptr[3]), g2t, None, st), 'g2')
+
+    def gemm_only():
+        L.check(lib.tnx_gemm_grouped(mode1, vp(ptr[0]), len(g1p), vp(ptr[1]), g1t, None, st), 'g1')
+        L.check(lib.tnx_gemm_grouped(mode2, vp(ptr[2]), len(g2p), vp(ptr[3]), g2t, None, st), 'g2')
+
+    for _ in range(3):
+        once()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        gemm_only()
+    e1.record()
+    e1.synchronize()
+    t_gemm = e0.elapsed_time(e1) / 1e3 / reps
+    e0.record()
+    for _ in range(reps):
+        once()
+    e1.record()
+    e1.synchronize()
+    t_mv = e0.elapsed_time(e1) / 1e3 / reps
+    # algorithmic bytes: env operands + psi in/out + intermediates, once each
+    isz = {'fp64': 8, 'bf16': 2}[prec]
+    for s in list(solves):
+        eng.remove_solve(s)
+    return {'flops': flops, 't_gemm': t_gemm, 't_matvec': t_mv, 'prec': prec,
+            'tiles': int(g1t + g2t), 'itemsize': isz}
+
+
+def cpu_baseline(seconds=15.0):
+    """The oracle's restatement of the reference algorithm (per-term effective
+    operators, opt_env matvec, ARPACK eigsh ncv=20) on the C2 chain, timed on
+    a bounded window of local steps and extrapolated to one full solve."""
+    from oracle import dmrg_ref, model
+    import threadpoolctl  # noqa: F401
+    c = model.chain_couplings(64, 1.0, 1.0)
+    p = dmrg_ref.make_para(c, 128, 8, tau=1e-4, eigs_tol=1e-10, break_tol=0.0, dt_ob=8)
+    steps = 6
+    t0 = time.perf_counter()
+    while True:
+        dmrg_ref.dmrg_finite_size(p, seed=0, max_steps=steps)
+        dt = time.perf_counter() - t0
+        if dt > seconds or steps >= 126:
+            break
+        steps = min(126, steps * 2)
+        t0 = time.perf_counter()
+    per_step = dt / steps
+    total_steps = 8 * (2 * 64 - 2)
+    return {'value': 1.0 / (per_step * total_steps), 'unit': 'solves/s', 'cores': os.cpu_count(),
+            'kind': 'port',
+            'sample': 'oracle/dmrg_ref (reference restatement, numpy/scipy f64, ARPACK ncv=20) on '
+                      'the N=64 chi=128 XXZ chain: first %d local steps of sweep 1 from a random '
+                      'MPS in %.1f s (%.3f s/step), extrapolated to 8 sweeps x 126 steps' %
+                      (steps, dt, per_step)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=1)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--batch', type=int, default=32)
+    ap.add_argument('--precision', default='mixed')
+    ap.add_argument('--n', type=int, default=64)
+    ap.add_argument('--chi', type=int, default=128)
+    ap.add_argument('--sweeps', type=int, default=8)
+    ap.add_argument('--no-plain', action='store_true')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU only (one-GPU build); --gpus %d is not supported'
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from tnalg_amd import _lib
+    _lib.load()
+    assert torch.cuda.is_available()
+    para = c2_para(args.n, args.chi, args.sweeps)
+    seeds = list(range(args.batch))
+    for w in range(args.warmup):
+        run_batch(para, seeds[:max(1, min(4, args.batch))], args.precision)
+        print('warmup %d done' % w, file=sys.stderr, flush=True)
+    times, lat = [], []
+    energies = None
+    counters = None
+    for k in range(args.steps):
+        torch.cuda.synchronize()
+        dt, energies, counters, _ = run_batch(para, seeds, args.precision)
+        times.append(dt)
+        lat += [dt] * len(seeds)
+        print('step %d: %.3f s for %d solves' % (k, dt, len(seeds)), file=sys.stderr, flush=True)
+    total = sum(times)
+    value = args.batch * args.steps / total
+    out = {'metric': 'DMRG solves/sec + p50/p99 latency at |dE/E|<=1e-9 vs FP64 plain, 1 GPU',
+           'value': value, 'unit': 'solves/s', 'n_gpus': 1, 'steps': args.steps,
+           'warmup': args.warmup, 'ms_per_step': 1e3 * total / args.steps,
+           'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+           'dtype': 'f64' if args.precision == 'fp64' else 'bf16/f32 matvec, f64 Lanczos+QR',
+           'data': 'synthetic (seeded random initial MPS, uniform XXZ couplings)',
+           'config': {'workload': 'XXZ chain N=%d open, Jxy=1 Jz=1 h=0, chi=%d, ncv=20, tol=1e-10, '
+                                  '%d sweeps' % (args.n, args.chi, args.sweeps),
+                      'batch': args.batch, 'precision': args.precision},
+           'latency_s': {'p50': float(np.percentile(lat, 50)), 'p99': float(np.percentile(lat, 99))},
+           'energy_per_site_mean': float(np.mean(energies)) / args.n}
+    out['engine_counters'] = {k: (float(v) if isinstance(v, float) else v)
+                              for k, v in counters.items()}
+    if not args.no_plain:
+        dt_p, e_p, _, _ = run_batch(para, seeds[:1], 'fp64')
+        out['plain_fp64'] = {'solves_per_s': 1.0 / dt_p, 'latency_s': dt_p,
+                             'energy': float(e_p[0])}
+        out['dE_over_E_vs_fp64_seed0'] = float(abs(energies[0] - e_p[0]) / abs(e_p[0]))
+        out['speedup_vs_plain_fp64'] = value / (1.0 / dt_p)
+    rf = matvec_roofline(para, args.batch, args.precision)
+    prec = rf['prec']
+    achieved = rf['flops'] / rf['t_gemm'] / 1e12
+    out['roofline'] = {'bound': 'mfma', 'achieved': achieved, 'peak': MFMA_PEAK[prec],
+                       'unit': 'TFLOP/s', 'frac': achieved / MFMA_PEAK[prec], 'traffic': None,
+                       'kernel': 'gemm_grouped_kernel (matvec GEMM1+GEMM2, %s, %d solves, mid-chain '
+                                 'site)' % (prec, args.batch),
+                       'avg_launch_s': rf['t_gemm'] / 2, 'matvec_s': rf['t_matvec'],
+                       'flops_per_matvec_batch': rf['flops']}
+    if not args.no_cpu:
+        out['cpu_baseline'] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

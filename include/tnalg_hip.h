@@ -70,7 +70,7 @@ typedef struct tnx_gemm_problem {
   int32_t tile_begin;  /* prefix sum of 64x64 output tiles over problems */
   int32_t active;      /* index into the active-flag array, or -1 */
   double beta;
-} tnx_gemm_problem; /* 64 bytes */
+} tnx_gemm_problem; /* 56 bytes */
 
 /* One output tensor of the elementwise physical-index mixer:
  *   dst[a,s,b] (=|+=) sum_terms coef * sum_t O[s,t] src[a,t,b]
@@ -196,7 +196,7 @@ typedef struct tnx_qr_job {
   uint64_t src, q, r, work;
   int32_t m, n, ld, trans;
   int32_t q_trans, reserved;
-} tnx_qr_job; /* 48 bytes */
+} tnx_qr_job; /* 56 bytes */
 int tnx_qr_f64(const tnx_qr_job* jobs, int njob, tnx_stream_t stream);
 
 /* Batched f64 one-sided Jacobi (Hestenes) SVD of M (m x n, m >= n given by
@@ -211,7 +211,7 @@ typedef struct tnx_svd_job {
   uint64_t src, u, s, svt, work;
   int32_t m, n, ld, trans;
   int32_t u_trans, reserved;
-} tnx_svd_job; /* 56 bytes */
+} tnx_svd_job; /* 64 bytes */
 int tnx_svd_jacobi_f64(const tnx_svd_job* jobs, int njob, int max_sweeps, tnx_stream_t stream);
 
 /* Batched traces / Frobenius inner products: out[i] = sum_ab X_i[a,b] Y_i[a,b]
