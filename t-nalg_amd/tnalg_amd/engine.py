@@ -42,6 +42,56 @@ def _p(t, off_elems=0, itemsize=8):
     return t.data_ptr() + off_elems * itemsize
 
 
+def max_dims(dims, d, ob):
+    """Largest dimension each bond takes: the initial (random MPS) dims and
+    the dims after the initial canonicalisation around ``ob`` (bonds only
+    shrink afterwards, via the reduced QR of the first sweep)."""
+    n = len(dims) - 1
+    cd = list(dims)
+    for s in range(0, ob):
+        cd[s + 1] = min(cd[s] * d, cd[s + 1])
+    for s in range(n - 1, ob, -1):
+        cd[s] = min(cd[s], d * cd[s + 1])
+    return [max(a, b) for a, b in zip(dims, cd)]
+
+
+def buffer_layout(T, dm, d, ncv, lowp):
+    """Sizes of every device buffer of a solve (elements for f64 buffers,
+    bytes for the ``uint8`` ones); also the HBM footprint used for admission."""
+    n = T.n
+    st = T.sites
+    psi = max(dm[s] * d * dm[s + 1] for s in range(n))
+    dsq = max(x * x for x in dm)
+    nL = [len(T.left_opens[b]) for b in range(n + 1)]
+    nR = [len(T.right_opens[b]) for b in range(n + 1)]
+    n_lm = max(len(s.lm) for s in st)
+    n_mr = max(len(s.mr) for s in st)
+    n_sd = max(len(s.strad) for s in st)
+    n_lc = max(len(s.lm) + len(s.mr) + len(s.strad) for s in st)
+    lay = {
+        'psi': psi, 'dsq': dsq,
+        'mps': [dm[s] * d * dm[s + 1] for s in range(n)],
+        'Lenv': [(1 + nL[b]) * dm[b] * dm[b] for b in range(n + 1)],
+        'Renv': [(1 + nR[b]) * dm[b] * dm[b] for b in range(n + 1)],
+        'lc64': max(1, max(n_lm, n_mr)) * dsq,
+        'lc_lp': max(1, n_lc) * dsq * 8,
+        'Tb': (1 + n_lm + n_sd) * psi * 8,
+        'Ub': (1 + n_mr) * psi * 8,
+        'Yb': (1 + max(n_lm, n_mr) + max(max(nL), max(nR))) * psi,
+        'Wb': (1 + max(max(nL), max(nR))) * psi,
+        'V': (ncv + 1) * psi,
+        'Rq': dsq + max(dm) + 64,
+        'qrw': 2 * psi + dsq,
+    }
+    f64 = 8 * (sum(lay['mps']) + sum(lay['Lenv']) + sum(lay['Renv']) + lay['lc64'] + lay['Yb']
+               + lay['Wb'] + lay['V'] + 3 * psi + lay['Rq'] + lay['qrw'])
+    u8 = lay['lc_lp'] + lay['Tb'] + lay['Ub'] + 8 * psi
+    if lowp:
+        u8 += 4 * (sum(lay['Lenv']) + sum(lay['Renv']))
+    lay['bytes'] = f64 + u8
+    return lay
+
+
 class Step:
     """One schedule entry of a solve: kind in {'move', 'solve', 'svdmove'},
     site, direction (+1: centre moves right, -1: left)."""
@@ -86,13 +136,7 @@ class Solve:
             host = [np.asarray(a, dtype=np.float64) for a in mps]
         dims = [host[0].shape[0]] + [a.shape[2] for a in host]
         self.dims = list(dims)
-        # dims after the initial canonicalisation (bonds only shrink later)
-        cd = list(dims)
-        for s in range(0, self.ob):
-            cd[s + 1] = min(cd[s] * d, cd[s + 1])
-        for s in range(n - 1, self.ob, -1):
-            cd[s] = min(cd[s], d * cd[s + 1])
-        self.dmax = [max(a, b) for a, b in zip(dims, cd)]
+        self.dmax = max_dims(dims, d, self.ob)
         dm = self.dmax
         f64 = torch.float64
         self.mps = [torch.from_numpy(np.ascontiguousarray(host[s]).reshape(-1)).to(self.dev)
@@ -106,43 +150,29 @@ class Solve:
         T = self.T
         self.nL = [len(T.left_opens[b]) for b in range(n + 1)]
         self.nR = [len(T.right_opens[b]) for b in range(n + 1)]
-        self.Lenv = [torch.zeros((1 + self.nL[b]) * dm[b] * dm[b], dtype=f64, device=self.dev)
-                     for b in range(n + 1)]
-        self.Renv = [torch.zeros((1 + self.nR[b]) * dm[b] * dm[b], dtype=f64, device=self.dev)
-                     for b in range(n + 1)]
+        lay = buffer_layout(T, dm, d, engine.ncv, engine.uses_lowp)
+        self.layout = lay
+        self.psi_max, self.dsq_max = lay['psi'], lay['dsq']
+        z = lambda nbytes: torch.zeros(nbytes, dtype=torch.uint8, device=self.dev)
+        zf = lambda nel: torch.zeros(nel, dtype=f64, device=self.dev)
+        self.Lenv = [zf(x) for x in lay['Lenv']]
+        self.Renv = [zf(x) for x in lay['Renv']]
         self.lp_bytes = 4 if engine.uses_lowp else 0
         if self.lp_bytes:
-            self.Lenv_lp = [torch.zeros((1 + self.nL[b]) * dm[b] * dm[b] * self.lp_bytes,
-                                        dtype=torch.uint8, device=self.dev) for b in range(n + 1)]
-            self.Renv_lp = [torch.zeros((1 + self.nR[b]) * dm[b] * dm[b] * self.lp_bytes,
-                                        dtype=torch.uint8, device=self.dev) for b in range(n + 1)]
-        # scratch sizes
-        psi = max(dm[s] * d * dm[s + 1] for s in range(n))
-        dsq = max(x * x for x in dm)
-        st = T.sites
-        n_lm = max(len(s.lm) for s in st)
-        n_mr = max(len(s.mr) for s in st)
-        n_sd = max(len(s.strad) for s in st)
-        n_lc = max(len(s.lm) + len(s.mr) + len(s.strad) for s in st)
-        n_t = 1 + n_lm + n_sd
-        n_y = 1 + max(n_lm, n_mr) + max(max(self.nL), max(self.nR))
-        n_w = 1 + max(max(self.nL), max(self.nR))
-        ncv = engine.ncv
-        self.psi_max, self.dsq_max = psi, dsq
-        tb = 4 if engine.uses_lowp else 8
-        self.lc64 = torch.zeros(max(1, max(n_lm, n_mr)) * dsq, dtype=f64, device=self.dev)
-        self.lc_lp = torch.zeros(max(1, n_lc) * dsq * 8, dtype=torch.uint8, device=self.dev)
-        self.Tb = torch.zeros(n_t * psi * 8, dtype=torch.uint8, device=self.dev)
-        self.Ub = torch.zeros((1 + n_mr) * psi * 8, dtype=torch.uint8, device=self.dev)
-        self.Yb = torch.zeros(n_y * psi, dtype=f64, device=self.dev)
-        self.Wb = torch.zeros(n_w * psi, dtype=f64, device=self.dev)
-        self.V = torch.zeros((ncv + 1) * psi, dtype=f64, device=self.dev)
-        self.w = torch.zeros(psi, dtype=f64, device=self.dev)
-        self.x = torch.zeros(psi * 8, dtype=torch.uint8, device=self.dev)
-        self.init = torch.zeros(psi, dtype=f64, device=self.dev)
-        self.Rq = torch.zeros(dsq + max(dm) + 64, dtype=f64, device=self.dev)
-        self.qrw = torch.zeros(2 * psi + dsq, dtype=f64, device=self.dev)
-        del tb
+            self.Lenv_lp = [z(x * self.lp_bytes) for x in lay['Lenv']]
+            self.Renv_lp = [z(x * self.lp_bytes) for x in lay['Renv']]
+        self.lc64 = zf(lay['lc64'])
+        self.lc_lp = z(lay['lc_lp'])
+        self.Tb = z(lay['Tb'])
+        self.Ub = z(lay['Ub'])
+        self.Yb = zf(lay['Yb'])
+        self.Wb = zf(lay['Wb'])
+        self.V = zf(lay['V'])
+        self.w = zf(lay['psi'])
+        self.x = z(lay['psi'] * 8)
+        self.init = zf(lay['psi'])
+        self.Rq = zf(lay['Rq'])
+        self.qrw = zf(lay['qrw'])
         self.ops_slot = engine.alloc_ops(T.ops)
         self.d2 = d * d
         # bookkeeping
@@ -569,6 +599,29 @@ class Engine:
         s = Solve(self, para, seed=seed, mps=mps, tag=tag)
         self.solves.append(s)
         return s
+
+    def footprint(self, para):
+        """HBM bytes a solve of ``para`` allocates (admission control)."""
+        T = compile_terms(para)
+        n, chi, d = T.n, int(para['chi']), T.d
+        dims = [1] + [chi] * (n - 1) + [1]
+        dm = max_dims(dims, d, int(para.get('ob_position', 0)))
+        return buffer_layout(T, dm, d, self.ncv, self.uses_lowp)['bytes']
+
+    def finalize(self, solves):
+        """Total energies <H> of finished solves (one batched measurement)."""
+        from .observe import measure
+        reqs = []
+        for s in solves:
+            idx2 = np.asarray(s.para['index2'])
+            reqs.append(([(i, 1) for i in range(s.n)] + [(i, 3) for i in range(s.n)],
+                         [tuple(r) for r in idx2]))
+        out = []
+        for s, (vone, vtwo, _) in zip(solves, measure(self, solves, reqs)):
+            e = float(np.dot(np.asarray(s.para['coeff2'], dtype=float).reshape(-1), vtwo))
+            e -= s.para.get('hx', 0.0) * vone[:s.n].sum() + s.para.get('hz', 0.0) * vone[s.n:].sum()
+            out.append(e)
+        return out
 
     def remove_solve(self, s):
         if s in self.solves:

@@ -1,0 +1,142 @@
+"""Admission, slot reuse and latency bookkeeping of the serving loop, driven
+by a stand-in engine on a virtual clock (no GPU, no numerics)."""
+import numpy as np
+
+from tnalg_amd.serving import SolveServer, poisson_xxz_stream
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+    def advance(self, dt):
+        self.t += dt
+
+
+class FakeSolve:
+    def __init__(self, para, fail_at=None):
+        self.para = para
+        self.queue = []
+        self.failed = None
+        self.fail_at = fail_at
+        self.n = para['l']
+
+    def schedule_canonicalise(self):
+        self.queue += ['c'] * 2
+
+    def schedule_sweep(self, t):
+        self.queue += ['s'] * (2 * self.n - 2)
+
+
+class FakeEngine:
+    """One step costs 1 ms of virtual time; footprint = chi^2 * N bytes."""
+
+    def __init__(self, clock):
+        self.clock = clock
+        self.live = set()
+        self.max_live_bytes = 0
+        self.bytes = 0
+
+    def footprint(self, para):
+        return para['chi'] ** 2 * para['l']
+
+    def add_solve(self, para, seed=0, tag=None):
+        if para.get('bad'):
+            raise ValueError('bad request')
+        s = FakeSolve(para, fail_at=para.get('fail_at'))
+        self.live.add(s)
+        self.bytes += self.footprint(para)
+        self.max_live_bytes = max(self.max_live_bytes, self.bytes)
+        return s
+
+    def remove_solve(self, s):
+        self.live.discard(s)
+        self.bytes -= self.footprint(s.para)
+
+    def step(self, solves):
+        self.clock.t += 1e-3
+        for s in solves:
+            if s.queue:
+                s.queue.pop(0)
+                if s.fail_at is not None and len(s.queue) == s.fail_at:
+                    s.failed = RuntimeError('injected device failure')
+
+    def finalize(self, solves):
+        return [-0.44 * s.n for s in solves]
+
+
+def _para(n, chi, sweeps=2, **kw):
+    p = {'l': n, 'chi': chi, 'sweep_time': sweeps}
+    p.update(kw)
+    return p
+
+
+def test_admission_respects_budget_and_reuses_slots():
+    clk = Clock()
+    eng = FakeEngine(clk)
+    budget = 3 * 64 * 64 * 20
+    srv = SolveServer(eng, hbm_budget=budget, max_active=8, clock=clk)
+    for k in range(10):
+        srv.submit(k, _para(20, 64), arrival=0.0)
+    summ = srv.run()
+    assert summ['completed'] == 10 and summ['failed'] == 0
+    assert summ['peak_active'] == 3                # budget holds exactly three
+    assert eng.max_live_bytes <= budget
+    assert not eng.live                             # every slot released
+    # FIFO: completion order follows arrival order for identical requests
+    done = [r.rid for r in sorted(srv.finished, key=lambda r: r.done)]
+    assert done == sorted(done)
+
+
+def test_poisson_arrivals_and_latency_percentiles():
+    clk = Clock()
+    eng = FakeEngine(clk)
+    srv = SolveServer(eng, hbm_budget=10 ** 9, max_active=4, clock=clk)
+    rng = np.random.RandomState(0)
+    t = 0.0
+    for k in range(40):
+        t += rng.exponential(0.05)
+        srv.submit(k, _para(int(rng.randint(8, 20)), 32), arrival=t)
+    summ = srv.run()
+    assert summ['completed'] == 40
+    lat = sorted(r.done - r.arrival for r in srv.finished)
+    assert abs(summ['latency_p50_s'] - float(np.percentile(lat, 50))) < 1e-12
+    assert summ['latency_p99_s'] >= summ['latency_p50_s'] > 0
+    for r in srv.finished:
+        assert r.admitted >= r.arrival and r.done > r.admitted
+
+
+def test_failures_are_isolated():
+    clk = Clock()
+    eng = FakeEngine(clk)
+    srv = SolveServer(eng, hbm_budget=10 ** 9, max_active=4, clock=clk)
+    srv.submit(0, _para(10, 16), arrival=0.0)
+    srv.submit(1, _para(10, 16, bad=True), arrival=0.0)        # rejected at admission
+    srv.submit(2, _para(10, 16, fail_at=5), arrival=0.0)       # fails mid-run
+    srv.submit(3, _para(10, 10 ** 6), arrival=0.0)             # larger than the budget
+    srv.submit(4, _para(12, 16), arrival=0.0)
+    summ = srv.run()
+    by = {r.rid: r for r in srv.finished}
+    assert summ['completed'] == 2 and summ['failed'] == 3
+    assert by[0].error is None and by[4].error is None
+    assert by[0].energy == -0.44 * 10
+    assert 'bad request' in by[1].error
+    assert 'injected' in by[2].error
+    assert 'budget' in by[3].error
+    assert not eng.live
+
+
+def test_stream_generator_ranges():
+    reqs = poisson_xxz_stream(64, rate=5.0, seed=1)
+    ns = [p['l'] for _, p, _ in reqs]
+    chis = {p['chi'] for _, p, _ in reqs}
+    assert min(ns) >= 16 and max(ns) <= 96
+    assert chis <= {32, 64, 128}
+    t = [a for a, _, _ in reqs]
+    assert all(b > a for a, b in zip(t, t[1:]))
+    for _, p, _ in reqs:
+        jz = np.ravel(p['coeff2'])[2]
+        assert 0.5 <= jz <= 1.5 and 0.0 <= p['hz'] <= 0.5

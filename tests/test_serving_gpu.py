@@ -1,0 +1,43 @@
+"""Serving on the GPU: continuous batching of uneven requests gives every
+request the same result as running it alone."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    from tnalg_amd.serving import poisson_xxz_stream
+    return poisson_xxz_stream(6, rate=1000.0, seed=3, n_range=(10, 22), chis=(8, 16),
+                              sweeps=3, eigs_tol=1e-10)
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_alone_equals_in_mix(gpu, precision):
+    from tnalg_amd.dmrg import _schedule_from
+    from tnalg_amd.engine import Engine
+    from tnalg_amd.serving import SolveServer
+    reqs = _requests()
+    sched = _schedule_from(precision)
+    lowp = precision != 'fp64'
+    alone = []
+    for k, (t, para, seed) in enumerate(reqs):
+        print('alone request', k, 'N', para['l'], 'chi', para['chi'], flush=True)
+        eng = Engine(precision_schedule=sched, lowp=lowp)
+        srv = SolveServer(eng, hbm_budget=8 << 30, max_active=1)
+        srv.submit(k, para, seed=seed, arrival=0.0)
+        srv.run()
+        alone.append(srv.finished[0].energy)
+    print('mixed stream of', len(reqs), flush=True)
+    eng = Engine(precision_schedule=sched, lowp=lowp)
+    srv = SolveServer(eng, hbm_budget=8 << 30, max_active=4)
+    import time
+    t0 = time.perf_counter()
+    for k, (t, para, seed) in enumerate(reqs):
+        srv.submit(k, para, seed=seed, arrival=t0 + 0.002 * k)
+    summ = srv.run()
+    assert summ['completed'] == len(reqs)
+    assert summ['peak_active'] > 1
+    mixed = {r.rid: r.energy for r in srv.finished}
+    for k in range(len(reqs)):
+        assert mixed[k] == alone[k], (k, mixed[k], alone[k])
