@@ -120,7 +120,7 @@ typedef struct tnx_lanczos_solve {
   uint64_t out;        /* f64 result vector (written at the end)            */
   int32_t n, ldv;      /* dimension, basis stride (elements)                */
   int32_t m;           /* Krylov dimension used (<= ncv, <= n)              */
-  int32_t chunk_begin; /* prefix sum of 2048-element chunks over solves     */
+  int32_t chunk_begin; /* prefix sum of 1024-element chunks over solves     */
   double tol_scale;    /* converged when |r| <= tol_scale*|1 - tau*theta|   */
   double tau;          /*   (ARPACK's test on I - tau*H, MPSClass.py:886)   */
   double floor_rel;    /*   or |r| <= floor_rel*|theta| (precision floor)   */
@@ -198,6 +198,23 @@ typedef struct tnx_qr_job {
   int32_t q_trans, reserved;
 } tnx_qr_job; /* 56 bytes */
 int tnx_qr_f64(const tnx_qr_job* jobs, int njob, tnx_stream_t stream);
+
+/* Batched f64 Cholesky factor and inverse of small SPD matrices, the core of
+ * the shifted Cholesky-QR (CholeskyQR3) gauge move on tall panels:
+ *   S = G + shift*I, shift = shift_scale * trace(G);  S = R^T R (R upper);
+ *   r <- R, rinv <- R^{-1}  (all n x n row-major, n <= 128).
+ * Three passes (G = M^T M with shift, then Q^T Q twice) with the Gram and
+ * Q = M R^{-1} products on the MFMA GEMM give M = Q R with Q orthonormal to
+ * working accuracy for any conditioning of M.  Replaces np.linalg.qr of
+ * left2right_decompose_tensor / right2left_decompose_tensor for tall panels
+ * (TensorBasicModule.py:416-486).  status (device int32 per job, may be 0)
+ * receives 1 if a pivot was not positive (replaced by a tiny value).       */
+typedef struct tnx_chol_job {
+  uint64_t g, r, rinv, status;
+  int32_t n, reserved;
+  double shift_scale;
+} tnx_chol_job; /* 48 bytes */
+int tnx_chol_inv_f64(const tnx_chol_job* jobs, int njob, tnx_stream_t stream);
 
 /* Batched f64 one-sided Jacobi (Hestenes) SVD of M (m x n, m >= n given by
  * the same addressing as tnx_qr_job): u (m x n, layout as q above) holds the

@@ -11,6 +11,10 @@
 //   f64 : v_mfma_f64_16x16x4_f64   (K step 4,  LDS chunk 16)
 //   f32 : v_mfma_f32_16x16x4_f32   (K step 4,  LDS chunk 16)
 //   bf16: v_mfma_f32_16x16x32_bf16 (K step 32, LDS chunk 32, f32 accumulate)
+// The K chunks of all segments form one stream; the global loads of chunk
+// c+1 are issued into registers before the MFMAs of chunk c (register
+// double buffering), and runs of 16 bytes are loaded as one vector when they
+// are in bounds and aligned.
 #include <type_traits>
 
 #include "tnx_common.h"
@@ -20,19 +24,19 @@ namespace {
 template <int MODE> struct Cfg;
 template <> struct Cfg<TNX_GEMM_F64> {
   typedef double TI; typedef double TC; typedef double ACC; typedef double CT;
-  static constexpr int KB = 16;
+  static constexpr int KB = 16, NE = 4;
 };
 template <> struct Cfg<TNX_GEMM_F32> {
   typedef float TI; typedef float TC; typedef float ACC; typedef float CT;
-  static constexpr int KB = 16;
+  static constexpr int KB = 16, NE = 4;
 };
 template <> struct Cfg<TNX_GEMM_BF16_F32> {
   typedef __bf16 TI; typedef float TC; typedef float ACC; typedef __bf16 CT;
-  static constexpr int KB = 32;
+  static constexpr int KB = 32, NE = 8;
 };
 template <> struct Cfg<TNX_GEMM_BF16_BF16> {
   typedef __bf16 TI; typedef __bf16 TC; typedef float ACC; typedef __bf16 CT;
-  static constexpr int KB = 32;
+  static constexpr int KB = 32, NE = 8;
 };
 
 template <typename T> __device__ __forceinline__ double to_d(T v) { return (double)v; }
@@ -42,103 +46,131 @@ template <> __device__ __forceinline__ __bf16 from_d<__bf16>(double v) { return 
 
 constexpr int TM = 64, TN = 64, NT = 256;
 
-// ---- f64 / f32 path: LDS tiles stored k-major: As[k][i], Bs[k][j] ----------
-template <int MODE>
-__device__ __forceinline__ void load_tiles_k4(typename Cfg<MODE>::CT (*As)[TM + 4],
-                                              typename Cfg<MODE>::CT (*Bs)[TN + 4],
-                                              const tnx_gemm_seg& S, int M, int N, int row0,
-                                              int col0, int k0) {
-  typedef typename Cfg<MODE>::TI TI;
-  typedef typename Cfg<MODE>::CT CT;
-  const int t = threadIdx.x;
-  const TI* A = (const TI*)S.a;
-  const TI* B = (const TI*)S.b;
-  const double alpha = S.alpha;
-  // A tile: 64 rows x 16 k
-  if (!S.transA) {
-    int i = t >> 2, kq = (t & 3) * 4;
-    int gi = row0 + i;
+// NE contiguous elements starting at p, of which the first `valid` are in
+// bounds; out-of-bounds ones read as zero.  Loaded as 16-byte vectors when
+// every element is valid and p is 16-byte aligned.
+template <typename TI, int NE>
+__device__ __forceinline__ void load_run(const TI* p, int valid, TI (&v)[NE]) {
+  constexpr int VB = 16 / sizeof(TI);  // elements per 16-byte vector
+  if (valid >= NE && (((uintptr_t)p) & 15) == 0) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      int gk = k0 + kq + e;
-      double v = (gi < M && gk < S.K) ? to_d(A[(int64_t)gi * S.lda + gk]) : 0.0;
-      As[kq + e][i] = (CT)(alpha * v);
+    for (int q = 0; q < NE / VB; ++q) {
+      const uint4 w = *(const uint4*)(p + q * VB);
+      TI e[VB];
+      __builtin_memcpy(e, &w, 16);
+#pragma unroll
+      for (int t = 0; t < VB; ++t) v[q * VB + t] = e[t];
     }
   } else {
-    int k = t >> 4, iq = (t & 15) * 4;
-    int gk = k0 + k;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      int gi = row0 + iq + e;
-      double v = (gi < M && gk < S.K) ? to_d(A[(int64_t)gk * S.lda + gi]) : 0.0;
-      As[k][iq + e] = (CT)(alpha * v);
-    }
-  }
-  // B tile: 16 k x 64 cols
-  if (!S.transB) {
-    int k = t >> 4, jq = (t & 15) * 4;
-    int gk = k0 + k;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      int gj = col0 + jq + e;
-      double v = (gj < N && gk < S.K) ? to_d(B[(int64_t)gk * S.ldb + gj]) : 0.0;
-      Bs[k][jq + e] = (CT)v;
-    }
-  } else {
-    int j = t >> 2, kq = (t & 3) * 4;
-    int gj = col0 + j;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      int gk = k0 + kq + e;
-      double v = (gj < N && gk < S.K) ? to_d(B[(int64_t)gj * S.ldb + gk]) : 0.0;
-      Bs[kq + e][j] = (CT)v;
-    }
+    for (int e = 0; e < NE; ++e) v[e] = (e < valid) ? p[e] : (TI)0.0f;
   }
 }
 
-// ---- bf16 path: LDS tiles stored row-major in k: As[i][k], Bs[j][k] ------
-__device__ __forceinline__ void load_tiles_bf16(__bf16 (*As)[32 + 8], __bf16 (*Bs)[32 + 8],
-                                                const tnx_gemm_seg& S, int M, int N, int row0,
-                                                int col0, int k0) {
+// Per-thread fragment of one K chunk: NE elements of the A tile and NE of the
+// B tile, with the addressing mode they were fetched in.
+template <int MODE> struct Frag {
+  typename Cfg<MODE>::TI a[Cfg<MODE>::NE];
+  typename Cfg<MODE>::TI b[Cfg<MODE>::NE];
+  double alpha;
+  int ta, tb;
+};
+
+// Thread t's share of a 64 x KB A tile / KB x 64 B tile.  NE = 4 (KB = 16) or
+// NE = 8 (KB = 32): 256 threads x NE = 64 x KB.
+//   transA = 0 (k contiguous): row t/(KB/NE), k = (t%(KB/NE))*NE .. +NE
+//   transA = 1 (i contiguous): k  t/(64/NE), i = (t%(64/NE))*NE .. +NE
+template <int MODE>
+__device__ __forceinline__ void fetch(Frag<MODE>& f, const tnx_gemm_seg& S, int M, int N, int row0,
+                                      int col0, int k0) {
+  typedef typename Cfg<MODE>::TI TI;
+  constexpr int KB = Cfg<MODE>::KB, NE = Cfg<MODE>::NE;
   const int t = threadIdx.x;
-  const __bf16* A = (const __bf16*)S.a;
-  const __bf16* B = (const __bf16*)S.b;
-  const float alpha = (float)S.alpha;
+  const TI* A = (const TI*)S.a;
+  const TI* B = (const TI*)S.b;
+  f.alpha = S.alpha;
+  f.ta = S.transA;
+  f.tb = S.transB;
   if (!S.transA) {
-    int i = t >> 2, kq = (t & 3) * 8;
-    int gi = row0 + i;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      int gk = k0 + kq + e;
-      float v = (gi < M && gk < S.K) ? (float)A[(int64_t)gi * S.lda + gk] : 0.0f;
-      As[i][kq + e] = (__bf16)(alpha * v);
-    }
+    const int i = t / (KB / NE), kq = (t % (KB / NE)) * NE;
+    const int gi = row0 + i, gk = k0 + kq;
+    const int valid = (gi < M) ? min(NE, S.K - gk) : 0;
+    load_run<TI, NE>(A + (int64_t)(gi < M ? gi : 0) * S.lda + gk, valid, f.a);
   } else {
-    int k = t >> 3, iq = (t & 7) * 8;
-    int gk = k0 + k;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      int gi = row0 + iq + e;
-      float v = (gi < M && gk < S.K) ? (float)A[(int64_t)gk * S.lda + gi] : 0.0f;
-      As[iq + e][k] = (__bf16)(alpha * v);
-    }
+    const int k = t / (TM / NE), iq = (t % (TM / NE)) * NE;
+    const int gi = row0 + iq, gk = k0 + k;
+    const int valid = (gk < S.K) ? min(NE, M - gi) : 0;
+    load_run<TI, NE>(A + (int64_t)(gk < S.K ? gk : 0) * S.lda + gi, valid, f.a);
   }
   if (!S.transB) {
-    int k = t >> 3, jq = (t & 7) * 8;
-    int gk = k0 + k;
+    const int k = t / (TN / NE), jq = (t % (TN / NE)) * NE;
+    const int gj = col0 + jq, gk = k0 + k;
+    const int valid = (gk < S.K) ? min(NE, N - gj) : 0;
+    load_run<TI, NE>(B + (int64_t)(gk < S.K ? gk : 0) * S.ldb + gj, valid, f.b);
+  } else {
+    const int j = t / (KB / NE), kq = (t % (KB / NE)) * NE;
+    const int gj = col0 + j, gk = k0 + kq;
+    const int valid = (gj < N) ? min(NE, S.K - gk) : 0;
+    load_run<TI, NE>(B + (int64_t)(gj < N ? gj : 0) * S.ldb + gk, valid, f.b);
+  }
+}
+
+// f64/f32: LDS tiles k-major, As[k][i], Bs[k][j] (conflict-free MFMA reads)
+template <int MODE>
+__device__ __forceinline__ void commit_k4(const Frag<MODE>& f, typename Cfg<MODE>::CT (*As)[TM + 4],
+                                          typename Cfg<MODE>::CT (*Bs)[TN + 4]) {
+  typedef typename Cfg<MODE>::CT CT;
+  constexpr int KB = Cfg<MODE>::KB, NE = Cfg<MODE>::NE;
+  const int t = threadIdx.x;
+  if (!f.ta) {
+    const int i = t / (KB / NE), kq = (t % (KB / NE)) * NE;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      int gj = col0 + jq + e;
-      Bs[jq + e][k] = (gj < N && gk < S.K) ? B[(int64_t)gk * S.ldb + gj] : (__bf16)0.0f;
+    for (int e = 0; e < NE; ++e) As[kq + e][i] = (CT)(f.alpha * to_d(f.a[e]));
+  } else {
+    const int k = t / (TM / NE), iq = (t % (TM / NE)) * NE;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) As[k][iq + e] = (CT)(f.alpha * to_d(f.a[e]));
+  }
+  if (!f.tb) {
+    const int k = t / (TN / NE), jq = (t % (TN / NE)) * NE;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) Bs[k][jq + e] = (CT)to_d(f.b[e]);
+  } else {
+    const int j = t / (KB / NE), kq = (t % (KB / NE)) * NE;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) Bs[kq + e][j] = (CT)to_d(f.b[e]);
+  }
+}
+
+// bf16: LDS tiles row-major in k, As[i][k], Bs[j][k] (8 consecutive k per lane)
+template <int MODE>
+__device__ __forceinline__ void commit_bf16(const Frag<MODE>& f, __bf16 (*As)[32 + 8],
+                                            __bf16 (*Bs)[32 + 8]) {
+  constexpr int KB = 32, NE = 8;
+  const int t = threadIdx.x;
+  const float alpha = (float)f.alpha;
+  if (!f.ta) {
+    const int i = t / (KB / NE), kq = (t % (KB / NE)) * NE;
+    if (alpha == 1.0f) {
+#pragma unroll
+      for (int e = 0; e < NE; ++e) As[i][kq + e] = f.a[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < NE; ++e) As[i][kq + e] = (__bf16)(alpha * (float)f.a[e]);
     }
   } else {
-    int j = t >> 2, kq = (t & 3) * 8;
-    int gj = col0 + j;
+    const int k = t / (TM / NE), iq = (t % (TM / NE)) * NE;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      int gk = k0 + kq + e;
-      Bs[j][kq + e] = (gj < N && gk < S.K) ? B[(int64_t)gj * S.ldb + gk] : (__bf16)0.0f;
-    }
+    for (int e = 0; e < NE; ++e) As[iq + e][k] = (__bf16)(alpha * (float)f.a[e]);
+  }
+  if (!f.tb) {
+    const int k = t / (TN / NE), jq = (t % (TN / NE)) * NE;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) Bs[jq + e][k] = f.b[e];
+  } else {
+    const int j = t / (KB / NE), kq = (t % (KB / NE)) * NE;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) Bs[j][kq + e] = f.b[e];
   }
 }
 
@@ -150,6 +182,7 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
   typedef Cfg<MODE> C;
   typedef typename C::ACC ACC;
   typedef typename C::TC TC;
+  constexpr int KB = C::KB;
   const int bid = tnx_xcd_remap(blockIdx.x, gridDim.x);
   const int pi = tnx_find_owner(probs, sizeof(tnx_gemm_problem),
                                 offsetof(tnx_gemm_problem, tile_begin), nprob, bid);
@@ -158,7 +191,7 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
   const int tiles_n = (P.N + TN - 1) / TN;
   const int t_local = bid - P.tile_begin;
   const int row0 = (t_local / tiles_n) * TM, col0 = (t_local % tiles_n) * TN;
-  if (row0 >= P.M) return;  // stray block past this problem's tiles (cannot happen with exact prefix sums)
+  if (row0 >= P.M) return;  // cannot happen with exact prefix sums
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
@@ -170,55 +203,67 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = V4{0, 0, 0, 0};
 
-  if constexpr (MODE == TNX_GEMM_F64 || MODE == TNX_GEMM_F32) {
-    typedef typename C::CT CT;
-    __shared__ CT As[16][TM + 4];
-    __shared__ CT Bs[16][TN + 4];
-    for (int s = P.seg_begin; s < P.seg_begin + P.seg_count; ++s) {
-      const tnx_gemm_seg S = segs[s];
-      for (int k0 = 0; k0 < S.K; k0 += 16) {
-        __syncthreads();
-        load_tiles_k4<MODE>(As, Bs, S, P.M, P.N, row0, col0, k0);
-        __syncthreads();
+  typedef typename std::conditional<(MODE == TNX_GEMM_F64 || MODE == TNX_GEMM_F32),
+                                    typename C::CT, __bf16>::type CT;
+  constexpr bool K4 = (MODE == TNX_GEMM_F64 || MODE == TNX_GEMM_F32);
+  constexpr int AR = K4 ? KB : TM, AC = K4 ? TM + 4 : 32 + 8;
+  constexpr int BR = K4 ? KB : TN, BC = K4 ? TN + 4 : 32 + 8;
+  __shared__ __attribute__((aligned(16))) CT As[AR][AC];
+  __shared__ __attribute__((aligned(16))) CT Bs[BR][BC];
+
+  // flattened stream of (segment, k0) chunks
+  int s = P.seg_begin, k0 = 0;
+  const int s_end = P.seg_begin + P.seg_count;
+  while (s < s_end && segs[s].K <= 0) ++s;
+  Frag<MODE> f;
+  tnx_gemm_seg S;
+  if (s < s_end) {
+    S = segs[s];
+    fetch<MODE>(f, S, P.M, P.N, row0, col0, 0);
+  }
+  while (s < s_end) {
+    __syncthreads();
+    if constexpr (K4) commit_k4<MODE>(f, As, Bs);
+    else commit_bf16<MODE>(f, As, Bs);
+    __syncthreads();
+    // advance and prefetch the next chunk
+    k0 += KB;
+    if (k0 >= S.K) {
+      k0 = 0;
+      ++s;
+      while (s < s_end && segs[s].K <= 0) ++s;
+      if (s < s_end) S = segs[s];
+    }
+    if (s < s_end) fetch<MODE>(f, S, P.M, P.N, row0, col0, k0);
+    if constexpr (K4) {
 #pragma unroll
-        for (int kk = 0; kk < 16; kk += 4) {
-          CT a0 = As[kk + (lane >> 4)][wr + (lane & 15)];
-          CT a1 = As[kk + (lane >> 4)][wr + 16 + (lane & 15)];
-          CT b0 = Bs[kk + (lane >> 4)][wc + (lane & 15)];
-          CT b1 = Bs[kk + (lane >> 4)][wc + 16 + (lane & 15)];
-          if constexpr (MODE == TNX_GEMM_F64) {
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-          } else {
-            acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
-          }
+      for (int kk = 0; kk < KB; kk += 4) {
+        CT a0 = As[kk + (lane >> 4)][wr + (lane & 15)];
+        CT a1 = As[kk + (lane >> 4)][wr + 16 + (lane & 15)];
+        CT b0 = Bs[kk + (lane >> 4)][wc + (lane & 15)];
+        CT b1 = Bs[kk + (lane >> 4)][wc + 16 + (lane & 15)];
+        if constexpr (MODE == TNX_GEMM_F64) {
+          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        } else {
+          acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+          acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+          acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+          acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
         }
       }
-    }
-  } else {
-    __shared__ __bf16 As[TM][32 + 8];
-    __shared__ __bf16 Bs[TN][32 + 8];
-    for (int s = P.seg_begin; s < P.seg_begin + P.seg_count; ++s) {
-      const tnx_gemm_seg S = segs[s];
-      for (int k0 = 0; k0 < S.K; k0 += 32) {
-        __syncthreads();
-        load_tiles_bf16(As, Bs, S, P.M, P.N, row0, col0, k0);
-        __syncthreads();
-        const int kq = (lane >> 4) * 8;
-        tnx_bf8 a0 = *(const tnx_bf8*)&As[wr + (lane & 15)][kq];
-        tnx_bf8 a1 = *(const tnx_bf8*)&As[wr + 16 + (lane & 15)][kq];
-        tnx_bf8 b0 = *(const tnx_bf8*)&Bs[wc + (lane & 15)][kq];
-        tnx_bf8 b1 = *(const tnx_bf8*)&Bs[wc + 16 + (lane & 15)][kq];
-        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
-      }
+    } else {
+      const int kq = (lane >> 4) * 8;
+      tnx_bf8 a0 = *(const tnx_bf8*)&As[wr + (lane & 15)][kq];
+      tnx_bf8 a1 = *(const tnx_bf8*)&As[wr + 16 + (lane & 15)][kq];
+      tnx_bf8 b0 = *(const tnx_bf8*)&Bs[wc + (lane & 15)][kq];
+      tnx_bf8 b1 = *(const tnx_bf8*)&Bs[wc + 16 + (lane & 15)][kq];
+      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
     }
   }
 
@@ -234,8 +279,8 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int rr = (MODE == TNX_GEMM_F64) ? ((lane >> 4) + 4 * r) : ((lane >> 4) * 4 + r);
-        int gi = row0 + wr + mi * 16 + rr;
-        int gj = col0 + wc + nj * 16 + (lane & 15);
+        const int gi = row0 + wr + mi * 16 + rr;
+        const int gj = col0 + wc + nj * 16 + (lane & 15);
         if (gi < P.M && gj < P.N) {
           double v = (double)acc[mi][nj][r];
           if (C0 != nullptr) v += P.beta * to_d(C0[(int64_t)gi * P.ldc0 + gj]);
@@ -249,8 +294,8 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
 extern "C" int tnx_gemm_grouped(int mode, const tnx_gemm_problem* prob, int nprob,
                                 const tnx_gemm_seg* seg, int ntiles, const int32_t* active,
                                 tnx_stream_t stream) {
+  if (nprob == 0 && ntiles == 0) return TNX_OK;
   if (nprob < 0 || ntiles < 0 || prob == nullptr || seg == nullptr) {
-    if (nprob == 0 && ntiles == 0) return TNX_OK;
     tnx_set_error("tnx_gemm_grouped: bad arguments (nprob=%d ntiles=%d)", nprob, ntiles);
     return TNX_EINVAL;
   }

@@ -3,7 +3,7 @@
 //
 // All solves of a batch advance in lockstep over Krylov index j; a solve
 // that has converged is masked out (its active flag is 0) so the matvec and
-// the kernels below skip it.  Vectors are cut into 2048-element chunks, one
+// the kernels below skip it.  Vectors are cut into 1024-element chunks, one
 // workgroup per chunk; per-solve reductions use the "last block reduces"
 // pattern (partials + fence + atomic ticket), so a Lanczos step is
 //   matvec (3 launches) + dots + update/dots + update/norm/tridiag + scale.
@@ -17,7 +17,7 @@
 namespace {
 
 constexpr int NCV_MAX = 32;
-constexpr int CHUNK = 2048;  // 256 lanes x 8 elements
+constexpr int CHUNK = 1024;  // 256 lanes x 4 elements
 constexpr int NT = 256;
 
 struct LzState {
@@ -77,6 +77,57 @@ __device__ __forceinline__ ChunkInfo chunk_of(const tnx_lanczos_solve* solves, i
 __device__ __forceinline__ void wave_to_lds(double s, int k, double (*red)[NCV_MAX + 1]) {
   s = tnx_wave_sum(s);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = s;
+}
+
+// Partial dots V[k] . x over this chunk for k = 0..j, wave sums into
+// red[wave][k].  Rows are taken four at a time so their loads are in flight
+// together (the loop is latency bound otherwise).
+__device__ __forceinline__ void chunk_dots(const double* __restrict__ V, int ldv, int j, int i0,
+                                           int i1, const double (&x)[CHUNK / NT],
+                                           double (*red)[NCV_MAX + 1]) {
+  for (int k0 = 0; k0 <= j; k0 += 4) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + u;
+      if (k <= j) {
+        const double* row = V + (int64_t)k * ldv;
+#pragma unroll
+        for (int q = 0; q < CHUNK / NT; ++q) {
+          const int i = i0 + q * NT + threadIdx.x;
+          if (i < i1) s[u] += row[i] * x[q];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k0 + u <= j) wave_to_lds(s[u], k0 + u, red);
+  }
+}
+
+// x -= sum_{k<=j} h[k] V[k] over this chunk (rows four at a time).
+__device__ __forceinline__ void chunk_axpy(const double* __restrict__ V, int ldv, int j, int i0,
+                                           int i1, const double* h, double (&x)[CHUNK / NT]) {
+  for (int k0 = 0; k0 <= j; k0 += 4) {
+    double v[4][CHUNK / NT];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + u;
+      const double* row = V + (int64_t)(k <= j ? k : j) * ldv;
+#pragma unroll
+      for (int q = 0; q < CHUNK / NT; ++q) {
+        const int i = i0 + q * NT + threadIdx.x;
+        v[u][q] = (k <= j && i < i1) ? row[i] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int kk = (k0 + u <= j) ? k0 + u : j;
+      const double hk = (k0 + u <= j) ? h[kk] : 0.0;
+#pragma unroll
+      for (int q = 0; q < CHUNK / NT; ++q) x[q] -= hk * v[u][q];
+    }
+  }
 }
 
 // Block-wide sum of one value per thread into red[0..3][0].
@@ -170,7 +221,7 @@ __device__ void tridiag_lowest(const double* a, const double* b, int k, double* 
   double u0[NCV_MAX], u1[NCV_MAX], u2[NCV_MAX];
   const double tiny = 2.220446049250313e-16 * span;
   for (int i = 0; i < k; ++i) z[i] = 1.0 / sqrt((double)k);
-  for (int it = 0; it < 3; ++it) {
+  for (int it = 0; it < 2; ++it) {
     for (int i = 0; i < k; ++i) rhs[i] = z[i];
     // current pivot-row candidate (cols i, i+1, i+2)
     double p0 = a[0] - th, p1 = b[0], p2 = 0.0, pr = rhs[0];
@@ -277,15 +328,7 @@ __global__ __launch_bounds__(NT) void k_dots(const tnx_lanczos_solve* __restrict
   }
   const double* V = (const double*)D.V;
   __shared__ double red[4][NCV_MAX + 1];
-  for (int k = 0; k <= j; ++k) {
-    double s = 0.0;
-#pragma unroll
-    for (int q = 0; q < CHUNK / NT; ++q) {
-      int i = ci.i0 + q * NT + threadIdx.x;
-      if (i < ci.i1) s += V[(int64_t)k * D.ldv + i] * w[q];
-    }
-    wave_to_lds(s, k, red);
-  }
+  chunk_dots(V, D.ldv, j, ci.i0, ci.i1, w, red);
   __syncthreads();
   for (int k = threadIdx.x; k <= j; k += NT)
     L.part[(int64_t)blockIdx.x * (NCV_MAX + 1) + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
@@ -313,14 +356,7 @@ __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __r
     int i = ci.i0 + q * NT + threadIdx.x;
     r[q] = (i < ci.i1) ? tnx_load((const void*)D.w, w_dtype, i) : 0.0;
   }
-  for (int k = 0; k <= j; ++k) {
-    const double hk = h[k];
-#pragma unroll
-    for (int q = 0; q < CHUNK / NT; ++q) {
-      int i = ci.i0 + q * NT + threadIdx.x;
-      if (i < ci.i1) r[q] -= hk * V[(int64_t)k * D.ldv + i];
-    }
-  }
+  chunk_axpy(V, D.ldv, j, ci.i0, ci.i1, h, r);
   double* Vn = (double*)D.V + (int64_t)(j + 1) * D.ldv;
 #pragma unroll
   for (int q = 0; q < CHUNK / NT; ++q) {
@@ -328,15 +364,7 @@ __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __r
     if (i < ci.i1) Vn[i] = r[q];
   }
   __shared__ double red[4][NCV_MAX + 1];
-  for (int k = 0; k <= j; ++k) {
-    double s = 0.0;
-#pragma unroll
-    for (int q = 0; q < CHUNK / NT; ++q) {
-      int i = ci.i0 + q * NT + threadIdx.x;
-      if (i < ci.i1) s += V[(int64_t)k * D.ldv + i] * r[q];
-    }
-    wave_to_lds(s, k, red);
-  }
+  chunk_dots(V, D.ldv, j, ci.i0, ci.i1, r, red);
   __syncthreads();
   for (int k = threadIdx.x; k <= j; k += NT)
     L.part[(int64_t)blockIdx.x * (NCV_MAX + 1) + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
@@ -366,14 +394,7 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
     int i = ci.i0 + q * NT + threadIdx.x;
     r[q] = (i < ci.i1) ? Vn[i] : 0.0;
   }
-  for (int k = 0; k <= j; ++k) {
-    const double hk = h[k];
-#pragma unroll
-    for (int q = 0; q < CHUNK / NT; ++q) {
-      int i = ci.i0 + q * NT + threadIdx.x;
-      if (i < ci.i1) r[q] -= hk * V[(int64_t)k * D.ldv + i];
-    }
-  }
+  chunk_axpy(V, D.ldv, j, ci.i0, ci.i1, h, r);
   double nn = 0.0;
 #pragma unroll
   for (int q = 0; q < CHUNK / NT; ++q) {

@@ -271,7 +271,106 @@ __global__ __launch_bounds__(NT) void svd_kernel(const tnx_svd_job* __restrict__
   }
 }
 
+// Cholesky factor + inverse of one n x n SPD matrix in LDS (n <= 128),
+// 256 lanes.  S = G + shift I = R^T R, right-looking; then in-place upper
+// triangular inversion column by column.
+constexpr int CHOL_NMAX = 128;
+
+__global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __restrict__ jobs) {
+  extern __shared__ double S[];  // n x n, row stride n
+  const tnx_chol_job J = jobs[blockIdx.x];
+  const int n = J.n;
+  if (n <= 0 || n > CHOL_NMAX) return;  // rejected on the host
+  const double* G = (const double*)J.g;
+  __shared__ double red[4];
+  __shared__ double shift;
+  __shared__ int bad;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double tr = 0.0;
+  for (int e = threadIdx.x; e < n * n; e += NT) {
+    const double v = G[e];
+    S[e] = v;
+    if (e / n == e % n) tr += v;
+  }
+  tr = tnx_wave_sum(tr);
+  if (lane == 0) red[wave] = tr;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    shift = J.shift_scale * ((red[0] + red[1]) + (red[2] + red[3]));
+    bad = 0;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += NT) S[i * n + i] += shift;
+  __syncthreads();
+  const double tiny = 1e-300;
+  // S = R^T R, R upper, stored in the upper triangle of S
+  for (int k = 0; k < n; ++k) {
+    if (threadIdx.x == 0) {
+      double d = S[k * n + k];
+      if (!(d > 0.0)) {
+        d = tiny;
+        bad = 1;
+      }
+      S[k * n + k] = sqrt(d);
+    }
+    __syncthreads();
+    const double rk = 1.0 / S[k * n + k];
+    for (int j = k + 1 + threadIdx.x; j < n; j += NT) S[k * n + j] *= rk;
+    __syncthreads();
+    const int m = n - k - 1;  // trailing upper triangle, (i, j) with k < i <= j
+    for (int e = threadIdx.x; e < m * m; e += NT) {
+      const int i = k + 1 + e / m, j = k + 1 + e % m;
+      if (j >= i) S[i * n + j] -= S[k * n + i] * S[k * n + j];
+    }
+    __syncthreads();
+  }
+  double* R = (double*)J.r;
+  for (int e = threadIdx.x; e < n * n; e += NT) {
+    const int i = e / n, j = e % n;
+    R[e] = (j >= i) ? S[e] : 0.0;
+  }
+  __syncthreads();
+  // in-place inverse of the upper triangle: column j from the inverted
+  // leading block, Sinv[i][j] = -(sum_{k=i}^{j-1} Sinv[i][k] R[k][j]) / R[j][j]
+  for (int j = 0; j < n; ++j) {
+    const double inv = 1.0 / S[j * n + j];
+    double t = 0.0;
+    const int i = threadIdx.x;
+    if (i < j) {
+      for (int k = i; k < j; ++k) t += S[i * n + k] * S[k * n + j];
+    }
+    __syncthreads();
+    if (i < j) S[i * n + j] = -t * inv;
+    if (i == j) S[j * n + j] = inv;
+    __syncthreads();
+  }
+  double* Ri = (double*)J.rinv;
+  for (int e = threadIdx.x; e < n * n; e += NT) {
+    const int i = e / n, j = e % n;
+    Ri[e] = (j >= i) ? S[e] : 0.0;
+  }
+  if (threadIdx.x == 0 && J.status != 0) *(int32_t*)J.status = bad;
+}
+
 }  // namespace
+
+extern "C" int tnx_chol_inv_f64(const tnx_chol_job* jobs, int njob, tnx_stream_t stream) {
+  if (njob == 0) return TNX_OK;
+  if (jobs == nullptr || njob < 0) {
+    tnx_set_error("tnx_chol_inv_f64: bad arguments");
+    return TNX_EINVAL;
+  }
+  static bool attr_set = false;
+  const size_t lds = sizeof(double) * CHOL_NMAX * CHOL_NMAX;
+  if (!attr_set) {
+    TNX_CHECK_HIP(hipFuncSetAttribute((const void*)chol_inv_kernel,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(chol_inv_kernel, dim3(njob), dim3(NT), lds, (hipStream_t)stream, jobs);
+  TNX_CHECK_LAUNCH();
+  return TNX_OK;
+}
 
 extern "C" int tnx_qr_f64(const tnx_qr_job* jobs, int njob, tnx_stream_t stream) {
   if (njob == 0) return TNX_OK;

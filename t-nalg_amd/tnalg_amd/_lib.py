@@ -41,10 +41,13 @@ SVD_JOB = np.dtype([('src', '<u8'), ('u', '<u8'), ('s', '<u8'), ('svt', '<u8'), 
                     ('u_trans', '<i4'), ('reserved', '<i4')])
 DOT_JOB = np.dtype([('x', '<u8'), ('y', '<u8'), ('rows', '<i4'), ('cols', '<i4'),
                     ('ldx', '<i4'), ('ldy', '<i4')])
+CHOL_JOB = np.dtype([('g', '<u8'), ('r', '<u8'), ('rinv', '<u8'), ('status', '<u8'), ('n', '<i4'),
+                     ('reserved', '<i4'), ('shift_scale', '<f8')])
+CHOL_NMAX = 128
 
 GEMM_TILE = 64
 MIX_BLOCK = 1024
-LZ_CHUNK = 2048
+LZ_CHUNK = 1024
 LZ_NCV_MAX = 32
 QR_KMAX = 4096
 
@@ -75,6 +78,7 @@ SIGNATURES = {
                                 _vp, _vp]),
     'tnx_qr_f64': (_i, [_vp, _i, _vp]),
     'tnx_svd_jacobi_f64': (_i, [_vp, _i, _i, _vp]),
+    'tnx_chol_inv_f64': (_i, [_vp, _i, _vp]),
     'tnx_trace_dot': (_i, [_vp, _i, _vp, _vp]),
 }
 
@@ -103,7 +107,7 @@ def load(path=None):
     n = lib.tnx_abi_sizes(sizes, 16)
     want = [GEMM_SEG.itemsize, GEMM_PROBLEM.itemsize, MIX_JOB.itemsize, MIX_TERM.itemsize,
             ctypes.sizeof(MatvecPlan), LANCZOS_SOLVE.itemsize, LANCZOS_RESULT.itemsize,
-            QR_JOB.itemsize, SVD_JOB.itemsize, DOT_JOB.itemsize]
+            QR_JOB.itemsize, SVD_JOB.itemsize, DOT_JOB.itemsize, CHOL_JOB.itemsize]
     got = list(sizes[:n])
     if got != want:
         raise RuntimeError('ABI size mismatch: library %s, binding %s' % (got, want))

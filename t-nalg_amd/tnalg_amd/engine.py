@@ -19,7 +19,8 @@ low-precision copies of the environments, while gauge moves, environment
 updates, the Krylov basis, reorthogonalisation and the tridiagonal solve stay
 f64.
 """
-import math
+import collections
+import itertools
 import time
 
 import numpy as np
@@ -38,8 +39,34 @@ PREC = {
 }
 
 
+_PLAN_UID = itertools.count(1)
+
+
 def _p(t, off_elems=0, itemsize=8):
     return t.data_ptr() + off_elems * itemsize
+
+
+class _Resident:
+    """A set of descriptor arrays uploaded once and kept in device memory
+    (steps that repeat every sweep reuse them without host work)."""
+    __slots__ = ('buf', 'ptrs', 'meta', 'host')
+
+    def __init__(self, device, arrays, meta):
+        offs, total = [], 0
+        for a in arrays:
+            offs.append(total)
+            total += (a.nbytes + 255) // 256 * 256
+        # pinned staging + stream-ordered copy: no host/device synchronisation
+        self.host = torch.empty(max(total, 256), dtype=torch.uint8, pin_memory=True)
+        hv = self.host.numpy()
+        for a, o in zip(arrays, offs):
+            if a.nbytes:
+                hv[o:o + a.nbytes] = np.frombuffer(np.ascontiguousarray(a).tobytes(), np.uint8)
+        self.buf = torch.empty(self.host.numel(), dtype=torch.uint8, device=device)
+        self.buf.copy_(self.host, non_blocking=True)
+        base = self.buf.data_ptr()
+        self.ptrs = [base + o for o in offs]
+        self.meta = meta
 
 
 def max_dims(dims, d, ob):
@@ -82,9 +109,10 @@ def buffer_layout(T, dm, d, ncv, lowp):
         'V': (ncv + 1) * psi,
         'Rq': dsq + max(dm) + 64,
         'qrw': 2 * psi + dsq,
+        'chol': 5 * dsq + 64,
     }
     f64 = 8 * (sum(lay['mps']) + sum(lay['Lenv']) + sum(lay['Renv']) + lay['lc64'] + lay['Yb']
-               + lay['Wb'] + lay['V'] + 3 * psi + lay['Rq'] + lay['qrw'])
+               + lay['Wb'] + lay['V'] + 3 * psi + lay['Rq'] + lay['qrw'] + lay['chol'])
     u8 = lay['lc_lp'] + lay['Tb'] + lay['Ub'] + 8 * psi
     if lowp:
         u8 += 4 * (sum(lay['Lenv']) + sum(lay['Renv']))
@@ -173,6 +201,7 @@ class Solve:
         self.init = zf(lay['psi'])
         self.Rq = zf(lay['Rq'])
         self.qrw = zf(lay['qrw'])
+        self.cholb = zf(lay['chol'])
         self.ops_slot = engine.alloc_ops(T.ops)
         self.d2 = d * d
         # bookkeeping
@@ -261,8 +290,61 @@ class Solve:
         elif st.kind == 'svdmove':
             self._plan_svd(ph, st.site, st.dir)
         ph['dims_after'] = list(self.dims)
+        ph['uid'] = next(_PLAN_UID)
         self.cache[key] = ph
         return ph
+
+    def _plan_cholqr(self, ph, q, m, n, right):
+        """Shifted CholeskyQR3 of the tall panel M (m x n) of site q: M is
+        A_q (row-major m x n) for a left move, A_q^T (A_q row-major n x m) for
+        a right move.  Stages (each one launch over all solves):
+        G=M^T M | chol+inv (shifted) | Q1=M Ri | G=Q1^T Q1 | chol+inv |
+        Q2=Q1 Ri, R21=R2 R1 | G=Q2^T Q2 | chol+inv | Q=Q2 Ri, R=R3 R21."""
+        A = self.mps[q].data_ptr()
+        dsq = self.dsq_max
+        Q1 = _p(self.qrw, 0)
+        Q2 = _p(self.qrw, self.psi_max)
+        G = _p(self.qrw, 2 * self.psi_max)
+        Ri = _p(self.cholb, 0)
+        R1, R2, R3, R21 = (_p(self.cholb, k * dsq) for k in (1, 2, 3, 4))
+        st = _p(self.cholb, 5 * dsq)
+        shift = 11.0 * (m * n + n * (n + 1)) * 1.1102230246251565e-16
+
+        def gemm(c, M, N, ldc, segs):
+            g = R.GemmList()
+            g.add(c, M, N, ldc, segs)
+            return g
+
+        def chol(r, s):
+            return np.array([(G, r, Ri, st, n, 0, s)], dtype=L.CHOL_JOB)
+
+        stages = []
+        if not right:
+            stages.append(('g', gemm(G, n, n, n, [(A, n, 1, A, n, 0, m, 1.0)])))
+        else:
+            stages.append(('g', gemm(G, n, n, n, [(A, m, 0, A, m, 1, m, 1.0)])))
+        stages.append(('c', chol(R1, shift)))
+        if not right:
+            stages.append(('g', gemm(Q1, m, n, n, [(A, n, 0, Ri, n, 0, n, 1.0)])))
+        else:
+            stages.append(('g', gemm(Q1, m, n, n, [(A, m, 1, Ri, n, 0, n, 1.0)])))
+        stages.append(('g', gemm(G, n, n, n, [(Q1, n, 1, Q1, n, 0, m, 1.0)])))
+        stages.append(('c', chol(R2, 0.0)))
+        g = gemm(Q2, m, n, n, [(Q1, n, 0, Ri, n, 0, n, 1.0)])
+        g.add(R21, n, n, n, [(R2, n, 0, R1, n, 0, n, 1.0)])
+        stages.append(('g', g))
+        stages.append(('g', gemm(G, n, n, n, [(Q2, n, 1, Q2, n, 0, m, 1.0)])))
+        stages.append(('c', chol(R3, 0.0)))
+        if not right:
+            g = gemm(A, m, n, n, [(Q2, n, 0, Ri, n, 0, n, 1.0)])
+        else:
+            g = gemm(A, n, m, m, [(Ri, n, 1, Q2, n, 1, n, 1.0)])
+        g.add(self.Rq.data_ptr(), n, n, n, [(R3, n, 0, R21, n, 0, n, 1.0)])
+        stages.append(('g', g))
+        ph['cq'] = [(kind, obj.arrays() if kind == 'g' else obj) for kind, obj in stages]
+
+    def _use_cholqr(self, m, n):
+        return m >= n and 8 <= n <= L.CHOL_NMAX
 
     def _plan_gauge(self, ph, q, dirn, copy_back):
         d, dims = self.d, self.dims
@@ -272,8 +354,12 @@ class Solve:
             k = min(m, nn)
             if k > L.QR_KMAX:
                 raise ValueError('bond dimension %d exceeds the QR kernel limit %d' % (k, L.QR_KMAX))
-            ph['qr'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(), self.Rq.data_ptr(),
-                                  self.qrw.data_ptr(), m, nn, nn, 0, 0, 0)], dtype=L.QR_JOB)
+            if self._use_cholqr(m, nn):
+                self._plan_cholqr(ph, q, m, nn, right=False)
+            else:
+                ph['qr'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(),
+                                      self.Rq.data_ptr(), self.qrw.data_ptr(), m, nn, nn, 0, 0, 0)],
+                                    dtype=L.QR_JOB)
             g = R.GemmList()
             g.add(self.init.data_ptr(), k, d * Dn, d * Dn,
                   [(self.Rq.data_ptr(), nn, 0, self.mps[q + 1].data_ptr(), d * Dn, 0, Dm, 1.0)])
@@ -290,8 +376,12 @@ class Solve:
             k = min(m, nn)
             if k > L.QR_KMAX:
                 raise ValueError('bond dimension %d exceeds the QR kernel limit %d' % (k, L.QR_KMAX))
-            ph['qr'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(), self.Rq.data_ptr(),
-                                  self.qrw.data_ptr(), m, nn, m, 1, 1, 0)], dtype=L.QR_JOB)
+            if self._use_cholqr(m, nn):
+                self._plan_cholqr(ph, q, m, nn, right=True)
+            else:
+                ph['qr'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(),
+                                      self.Rq.data_ptr(), self.qrw.data_ptr(), m, nn, m, 1, 1, 0)],
+                                    dtype=L.QR_JOB)
             g = R.GemmList()
             g.add(self.init.data_ptr(), Dp * d, k, k,
                   [(self.mps[q - 1].data_ptr(), Dm, 0, self.Rq.data_ptr(), Dm, 1, Dm, 1.0)])
@@ -565,8 +655,12 @@ class Engine:
         self.ops_d2 = None
         self.solves = []
         self.lz_state = None
+        self._resident = collections.OrderedDict()
+        self._resident_bytes = 0
+        self.resident_limit = 256 << 20
         self.counters = {'matvec_flops': 0.0, 'matvecs': 0, 'lanczos_calls': 0, 'steps': 0,
-                         'lz_time': 0.0}
+                         'lz_time': 0.0, 't_plan': 0.0, 't_pre': 0.0, 't_lz_total': 0.0,
+                         'lz_iters': 0}
 
     # ------------------------------------------------------------ op table
     def alloc_ops(self, ops):
@@ -593,6 +687,22 @@ class Engine:
 
     def stream(self):
         return ctypes_void(torch.cuda.current_stream().cuda_stream)
+
+    # ------------------------------------------------- resident descriptors
+    def _resident_get(self, key):
+        ent = self._resident.get(key)
+        if ent is not None:
+            self._resident.move_to_end(key)
+        return ent
+
+    def _resident_put(self, key, arrays, meta):
+        ent = _Resident(self.device, arrays, meta)
+        self._resident[key] = ent
+        self._resident_bytes += ent.buf.numel()
+        while self._resident_bytes > self.resident_limit and len(self._resident) > 1:
+            _, old = self._resident.popitem(last=False)
+            self._resident_bytes -= old.buf.numel()
+        return ent
 
     # ------------------------------------------------------------- solves
     def add_solve(self, para, seed=0, mps=None, tag=None):
@@ -662,22 +772,29 @@ class Engine:
         """group: list of (solve, plan) sharing one precision."""
         prec = group[0][1]['prec']
         ti, tc, mode1, mode2, _ = PREC[prec]
-        actives = list(range(len(group)))
-        lc_j, lc_t, lc_nb = R.concat_mix([ph['mv_lc'] for _, ph in group])
-        g1p, g1s, g1t = R.concat_gemm([ph['mv_g1'] for _, ph in group], actives)
-        mxj, mxt, mxb = R.concat_mix([ph['mv_mix'] for _, ph in group], actives)
-        g2p, g2s, g2t = R.concat_gemm([ph['mv_g2'] for _, ph in group], actives)
-        lz = np.concatenate([ph['lz'] for _, ph in group])
-        chunks = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
-        lz['chunk_begin'] = np.concatenate([[0], np.cumsum(chunks)[:-1]])
-        nchunks = int(chunks.sum())
-        ptrs = self.up.upload([lc_j, lc_t, g1p, g1s, mxj, mxt, g2p, g2s, lz])
-        if lc_nb:
-            L.check(self.lib.tnx_mix(ctypes_void(ptrs[0]), len(lc_j), ctypes_void(ptrs[1]), lc_nb,
-                                     ctypes_void(self.ops.data_ptr()), None, self.stream()),
-                    'tnx_mix')
-        plan = L.MatvecPlan(ptrs[2], ptrs[3], len(g1p), g1t, ptrs[4], ptrs[5], len(mxj), mxb,
-                            ptrs[6], ptrs[7], len(g2p), g2t, self.ops.data_ptr(), mode1, mode2)
+        rkey = ('lz',) + tuple(ph['uid'] for _, ph in group)
+        ent = self._resident_get(rkey)
+        if ent is None:
+            actives = list(range(len(group)))
+            lc_j, lc_t, lc_nb = R.concat_mix([ph['mv_lc'] for _, ph in group])
+            g1p, g1s, g1t = R.concat_gemm([ph['mv_g1'] for _, ph in group], actives)
+            mxj, mxt, mxb = R.concat_mix([ph['mv_mix'] for _, ph in group], actives)
+            g2p, g2s, g2t = R.concat_gemm([ph['mv_g2'] for _, ph in group], actives)
+            lz = np.concatenate([ph['lz'] for _, ph in group])
+            chunks = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
+            lz['chunk_begin'] = np.concatenate([[0], np.cumsum(chunks)[:-1]])
+            meta = {'lc': (len(lc_j), lc_nb), 'g1': (len(g1p), g1t), 'mix': (len(mxj), mxb),
+                    'g2': (len(g2p), g2t), 'nchunks': int(chunks.sum())}
+            ent = self._resident_put(rkey, [lc_j, lc_t, g1p, g1s, mxj, mxt, g2p, g2s, lz], meta)
+        ptrs, meta = ent.ptrs, ent.meta
+        nchunks = meta['nchunks']
+        if meta['lc'][1]:
+            L.check(self.lib.tnx_mix(ctypes_void(ptrs[0]), meta['lc'][0], ctypes_void(ptrs[1]),
+                                     meta['lc'][1], ctypes_void(self.ops.data_ptr()), None,
+                                     self.stream()), 'tnx_mix')
+        plan = L.MatvecPlan(ptrs[2], ptrs[3], meta['g1'][0], meta['g1'][1], ptrs[4], ptrs[5],
+                            meta['mix'][0], meta['mix'][1], ptrs[6], ptrs[7], meta['g2'][0],
+                            meta['g2'][1], self.ops.data_ptr(), mode1, mode2)
         need = int(self.lib.tnx_lanczos_state_bytes(len(group), nchunks, self.ncv))
         if self.lz_state is None or self.lz_state.numel() < need:
             self.lz_state = torch.empty(max(need, 1 << 20), dtype=torch.uint8, device=self.device)
@@ -705,6 +822,7 @@ class Engine:
         solves = [s for s in (solves or self.solves) if s.queue and s.failed is None]
         if not solves:
             return 0
+        t_a = time.perf_counter()
         plans = []
         for s in solves:
             st = s.queue.pop(0)
@@ -716,40 +834,69 @@ class Engine:
             ph = s.plan(st, prec)
             plans.append((s, st, ph))
 
+        t_b = time.perf_counter()
+
         def sel(key):
             return [ph[key] for _, _, ph in plans if key in ph]
 
-        # one upload for every descriptor of the step's pre-Lanczos phases
-        qr = R.concat_plain(sel('qr'), L.QR_JOB)
-        svd = R.concat_plain(sel('svd'), L.SVD_JOB)
-        gemms = [R.concat_gemm(sel(k)) for k in ('absorb', 'env_g1', 'env_gT')]
-        mixes = [R.concat_mix(sel(k)) for k in ('copy', 'env_lc', 'env_mix', 'cast')]
-        arrs = [qr, svd]
-        for p, sg, _ in gemms:
-            arrs += [p, sg]
-        for j, t, _ in mixes:
-            arrs += [j, t]
-        ptr = self.up.upload(arrs)
+        # every descriptor of the step's pre-Lanczos phases, resident on the
+        # device per combination of plans (steps repeat every sweep)
+        rkey = ('pre',) + tuple(ph['uid'] for _, _, ph in plans)
+        ent = self._resident_get(rkey)
+        if ent is None:
+            qr = R.concat_plain(sel('qr'), L.QR_JOB)
+            svd = R.concat_plain(sel('svd'), L.SVD_JOB)
+            gemms = [R.concat_gemm(sel(k)) for k in ('absorb', 'env_g1', 'env_gT')]
+            mixes = [R.concat_mix(sel(k)) for k in ('copy', 'env_lc', 'env_mix', 'cast')]
+            arrs = [qr, svd]
+            for p, sg, _ in gemms:
+                arrs += [p, sg]
+            for j, t, _ in mixes:
+                arrs += [j, t]
+            cq_meta = []
+            cqs = sel('cq')
+            if cqs:
+                for k in range(len(cqs[0])):
+                    if cqs[0][k][0] == 'g':
+                        p, sg, nt = R.concat_gemm([c[k][1] for c in cqs])
+                        cq_meta.append(('g', len(p), nt, len(arrs)))
+                        arrs += [p, sg]
+                    else:
+                        jobs = R.concat_plain([c[k][1] for c in cqs], L.CHOL_JOB)
+                        cq_meta.append(('c', len(jobs), 0, len(arrs)))
+                        arrs.append(jobs)
+            meta = {'nqr': len(qr), 'nsvd': len(svd),
+                    'gemm': [(len(p), nt) for p, _, nt in gemms],
+                    'mix': [(len(j), nb) for j, _, nb in mixes], 'cq': cq_meta}
+            ent = self._resident_put(rkey, arrs, meta)
+        ptr, meta = ent.ptrs, ent.meta
         st_ = self.stream()
         lib = self.lib
         opsp = ctypes_void(self.ops.data_ptr())
 
         def run_gemm(i):
-            p, sg, nt = gemms[i]
+            n_p, nt = meta['gemm'][i]
             if nt:
-                L.check(lib.tnx_gemm_grouped(L.GEMM_F64, ctypes_void(ptr[2 + 2 * i]), len(p),
+                L.check(lib.tnx_gemm_grouped(L.GEMM_F64, ctypes_void(ptr[2 + 2 * i]), n_p,
                                              ctypes_void(ptr[3 + 2 * i]), nt, None, st_), 'gemm')
 
         def run_mix(i):
-            j, t, nb = mixes[i]
+            n_j, nb = meta['mix'][i]
             if nb:
-                L.check(lib.tnx_mix(ctypes_void(ptr[8 + 2 * i]), len(j), ctypes_void(ptr[9 + 2 * i]),
+                L.check(lib.tnx_mix(ctypes_void(ptr[8 + 2 * i]), n_j, ctypes_void(ptr[9 + 2 * i]),
                                     nb, opsp, None, st_), 'mix')
 
-        if len(qr):
-            L.check(lib.tnx_qr_f64(ctypes_void(ptr[0]), len(qr), st_), 'tnx_qr_f64')
-        if len(svd):
-            L.check(lib.tnx_svd_jacobi_f64(ctypes_void(ptr[1]), len(svd), 60, st_), 'tnx_svd')
+        if meta['nqr']:
+            L.check(lib.tnx_qr_f64(ctypes_void(ptr[0]), meta['nqr'], st_), 'tnx_qr_f64')
+        if meta['nsvd']:
+            L.check(lib.tnx_svd_jacobi_f64(ctypes_void(ptr[1]), meta['nsvd'], 60, st_), 'tnx_svd')
+        for (kind, cnt, nt, ai) in meta['cq']:   # CholeskyQR3 stages
+            if kind == 'g':
+                if nt:
+                    L.check(lib.tnx_gemm_grouped(L.GEMM_F64, ctypes_void(ptr[ai]), cnt,
+                                                 ctypes_void(ptr[ai + 1]), nt, None, st_), 'cq gemm')
+            elif cnt:
+                L.check(lib.tnx_chol_inv_f64(ctypes_void(ptr[ai]), cnt, st_), 'tnx_chol_inv_f64')
         run_gemm(0)      # absorb R into the neighbour
         run_mix(0)       # copy back (moves)
         run_mix(1)       # linear combinations for the environment update
@@ -757,12 +904,17 @@ class Engine:
         run_mix(2)       # physical mixing
         run_gemm(2)      # Q^T . W
         run_mix(3)       # low-precision copies of the new block
+        t_c = time.perf_counter()
         groups = {}
         for s, st, ph in plans:
             if 'lz' in ph:
                 groups.setdefault(ph['prec'], []).append((s, ph))
         for prec in sorted(groups):
             self._lanczos(groups[prec])
+        t_d = time.perf_counter()
+        self.counters['t_plan'] += t_b - t_a
+        self.counters['t_pre'] += t_c - t_b
+        self.counters['t_lz_total'] += t_d - t_c
         for s, st, ph in plans:
             s.stats['steps'] += 1
             if st.kind == 'solve':

@@ -210,6 +210,34 @@ def test_svd_jacobi_f64(gpu, m, n, trans):
     assert np.allclose(U @ SVT, M, atol=1e-12 * np.abs(M).max())
 
 
+@pytest.mark.parametrize('n', [1, 8, 37, 128])
+def test_chol_inv_f64(gpu, n):
+    import torch
+    from tnalg_amd import _lib as L
+    lib = L.load()
+    rng = np.random.RandomState(n)
+    A = rng.randn(2 * n + 3, n)
+    G = A.T @ A
+    Gd = _dev(G)
+    r = torch.zeros(n * n, dtype=torch.float64, device='cuda')
+    ri = torch.zeros(n * n, dtype=torch.float64, device='cuda')
+    st = torch.zeros(2, dtype=torch.int32, device='cuda')
+    shift = 1e-12
+    jobs = np.array([(Gd.data_ptr(), r.data_ptr(), ri.data_ptr(), st.data_ptr(), n, 0, shift)],
+                    dtype=L.CHOL_JOB)
+    (dj,) = _upload([jobs])
+    print('chol_inv n', n, flush=True)
+    L.check(lib.tnx_chol_inv_f64(_vp(dj.data_ptr()), 1, _stream()), 'chol')
+    torch.cuda.synchronize()
+    Rm = r.cpu().numpy().reshape(n, n)
+    Ri = ri.cpu().numpy().reshape(n, n)
+    S = G + shift * np.trace(G) * np.eye(n)
+    assert np.allclose(np.tril(Rm, -1), 0) and np.all(np.diag(Rm) > 0)
+    assert np.allclose(Rm.T @ Rm, S, rtol=1e-12, atol=1e-12 * np.abs(S).max())
+    assert np.allclose(Rm @ Ri, np.eye(n), atol=1e-10)
+    assert int(st.cpu()[0]) == 0
+
+
 def test_trace_dot(gpu):
     import torch
     from tnalg_amd import _lib as L
