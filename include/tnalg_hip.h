@@ -211,9 +211,11 @@ int tnx_qr_f64(const tnx_qr_job* jobs, int njob, tnx_stream_t stream);
  * receives 1 if a pivot was not positive (replaced by a tiny value).       */
 typedef struct tnx_chol_job {
   uint64_t g, r, rinv, status;
-  int32_t n, reserved;
+  uint64_t shift_ref;  /* device f64*: if set, shift = shift_scale * *shift_ref
+                          (a blocked factorisation passes the full trace)    */
+  int32_t n, ld;       /* order (<= 128) and leading dimension of g, r, rinv */
   double shift_scale;
-} tnx_chol_job; /* 48 bytes */
+} tnx_chol_job; /* 56 bytes */
 int tnx_chol_inv_f64(const tnx_chol_job* jobs, int njob, tnx_stream_t stream);
 
 /* Batched f64 one-sided Jacobi (Hestenes) SVD of M (m x n, m >= n given by

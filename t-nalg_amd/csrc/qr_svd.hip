@@ -282,21 +282,25 @@ __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __rest
   const int n = J.n;
   if (n <= 0 || n > CHOL_NMAX) return;  // rejected on the host
   const double* G = (const double*)J.g;
+  const int ld = J.ld > 0 ? J.ld : n;
   __shared__ double red[4];
   __shared__ double shift;
   __shared__ int bad;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double tr = 0.0;
   for (int e = threadIdx.x; e < n * n; e += NT) {
-    const double v = G[e];
+    const int i = e / n, j = e % n;
+    const double v = G[(int64_t)i * ld + j];
     S[e] = v;
-    if (e / n == e % n) tr += v;
+    if (i == j) tr += v;
   }
   tr = tnx_wave_sum(tr);
   if (lane == 0) red[wave] = tr;
   __syncthreads();
   if (threadIdx.x == 0) {
-    shift = J.shift_scale * ((red[0] + red[1]) + (red[2] + red[3]));
+    const double t = J.shift_ref ? *(const double*)J.shift_ref
+                                 : ((red[0] + red[1]) + (red[2] + red[3]));
+    shift = J.shift_scale * t;
     bad = 0;
   }
   __syncthreads();
@@ -327,7 +331,7 @@ __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __rest
   double* R = (double*)J.r;
   for (int e = threadIdx.x; e < n * n; e += NT) {
     const int i = e / n, j = e % n;
-    R[e] = (j >= i) ? S[e] : 0.0;
+    R[(int64_t)i * ld + j] = (j >= i) ? S[e] : 0.0;
   }
   __syncthreads();
   // in-place inverse of the upper triangle: column j from the inverted
@@ -347,7 +351,7 @@ __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __rest
   double* Ri = (double*)J.rinv;
   for (int e = threadIdx.x; e < n * n; e += NT) {
     const int i = e / n, j = e % n;
-    Ri[e] = (j >= i) ? S[e] : 0.0;
+    Ri[(int64_t)i * ld + j] = (j >= i) ? S[e] : 0.0;
   }
   if (threadIdx.x == 0 && J.status != 0) *(int32_t*)J.status = bad;
 }

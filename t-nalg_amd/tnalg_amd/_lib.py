@@ -41,8 +41,8 @@ SVD_JOB = np.dtype([('src', '<u8'), ('u', '<u8'), ('s', '<u8'), ('svt', '<u8'), 
                     ('u_trans', '<i4'), ('reserved', '<i4')])
 DOT_JOB = np.dtype([('x', '<u8'), ('y', '<u8'), ('rows', '<i4'), ('cols', '<i4'),
                     ('ldx', '<i4'), ('ldy', '<i4')])
-CHOL_JOB = np.dtype([('g', '<u8'), ('r', '<u8'), ('rinv', '<u8'), ('status', '<u8'), ('n', '<i4'),
-                     ('reserved', '<i4'), ('shift_scale', '<f8')])
+CHOL_JOB = np.dtype([('g', '<u8'), ('r', '<u8'), ('rinv', '<u8'), ('status', '<u8'),
+                     ('shift_ref', '<u8'), ('n', '<i4'), ('ld', '<i4'), ('shift_scale', '<f8')])
 CHOL_NMAX = 128
 
 GEMM_TILE = 64

@@ -109,7 +109,7 @@ def buffer_layout(T, dm, d, ncv, lowp):
         'V': (ncv + 1) * psi,
         'Rq': dsq + max(dm) + 64,
         'qrw': 2 * psi + dsq,
-        'chol': 5 * dsq + 64,
+        'chol': 6 * dsq + max(dm) + 64,
     }
     f64 = 8 * (sum(lay['mps']) + sum(lay['Lenv']) + sum(lay['Renv']) + lay['lc64'] + lay['Yb']
                + lay['Wb'] + lay['V'] + 3 * psi + lay['Rq'] + lay['qrw'] + lay['chol'])
@@ -202,6 +202,11 @@ class Solve:
         self.Rq = zf(lay['Rq'])
         self.qrw = zf(lay['qrw'])
         self.cholb = zf(lay['chol'])
+        self.dmax_max = max(dm)
+        # identity (stride dmax) and ones for the blocked CholeskyQR
+        eye = torch.arange(self.dmax_max, device=self.dev) * (self.dmax_max + 1)
+        self.cholb[5 * lay['dsq'] + eye] = 1.0
+        self.cholb[6 * lay['dsq']:6 * lay['dsq'] + self.dmax_max] = 1.0
         self.ops_slot = engine.alloc_ops(T.ops)
         self.d2 = d * d
         # bookkeeping
@@ -307,7 +312,7 @@ class Solve:
         G = _p(self.qrw, 2 * self.psi_max)
         Ri = _p(self.cholb, 0)
         R1, R2, R3, R21 = (_p(self.cholb, k * dsq) for k in (1, 2, 3, 4))
-        st = _p(self.cholb, 5 * dsq)
+        st = _p(self.cholb, 6 * dsq + self.dmax_max + 1)
         shift = 11.0 * (m * n + n * (n + 1)) * 1.1102230246251565e-16
 
         def gemm(c, M, N, ldc, segs):
@@ -316,7 +321,7 @@ class Solve:
             return g
 
         def chol(r, s):
-            return np.array([(G, r, Ri, st, n, 0, s)], dtype=L.CHOL_JOB)
+            return np.array([(G, r, Ri, st, 0, n, n, s)], dtype=L.CHOL_JOB)
 
         stages = []
         if not right:
@@ -343,8 +348,75 @@ class Solve:
         stages.append(('g', g))
         ph['cq'] = [(kind, obj.arrays() if kind == 'g' else obj) for kind, obj in stages]
 
+    def _plan_cholqr_blocked(self, ph, q, m, n, right):
+        """Shifted CholeskyQR3 for n > 128: the panel X (m x n, row-major) is
+        overwritten in place by Q1, Q2, Q; each pass is the Gram GEMM, a
+        blocked right-looking Cholesky (128 x 128 diagonal blocks in LDS,
+        panel and trailing updates on the MFMA GEMM) and a blocked
+        triangular solve Q_j = (X_j - Q_<j R_<j,j) R_jj^-1."""
+        B = L.CHOL_NMAX
+        nb = (n + B - 1) // B
+        dsq = self.dsq_max
+        A = self.mps[q].data_ptr()
+        X = _p(self.qrw, 0) if right else A          # row-major m x n panel
+        Wt = _p(self.qrw, self.psi_max)              # m x B scratch
+        G = _p(self.qrw, 2 * self.psi_max)           # n x n
+        Ri = _p(self.cholb, 0)                       # diagonal-block inverses
+        Rs = [_p(self.cholb, k * dsq) for k in (1, 2, 3)]
+        R21 = _p(self.cholb, 4 * dsq)
+        I, LDI = _p(self.cholb, 5 * dsq), self.dmax_max   # identity, stride dmax
+        ones = _p(self.cholb, 6 * dsq)
+        tr = _p(self.cholb, 6 * dsq + self.dmax_max)
+        st = tr + 8
+        shift = 11.0 * (m * n + n * (n + 1)) * 1.1102230246251565e-16
+
+        def G1(c, M, N, ldc, segs, c0=0, ldc0=0, beta=0.0):
+            g = R.GemmList()
+            g.add(c, M, N, ldc, segs, c0=c0, ldc0=ldc0, beta=beta)
+            return ('g', g.arrays())
+
+        stages = []
+        if right:  # materialise X = A^T (A row-major n x m) with the identity
+            stages.append(G1(X, m, n, n, [(A, m, 1, I, LDI, 0, n, 1.0)]))
+        for ps in range(3):
+            Rp = Rs[ps]
+            zero = R.MixList()
+            zero.add(Rp, L.F64, 1, 1, n * n, [])      # R strictly-lower blocks stay 0
+            stages.append(('m', zero.arrays()))
+            stages.append(G1(G, n, n, n, [(X, n, 1, X, n, 0, m, 1.0)]))
+            if ps == 0:
+                stages.append(G1(tr, 1, 1, 1, [(G, n + 1, 1, ones, 1, 0, n, 1.0)]))
+            for k in range(nb):
+                b0, bk = k * B, min(B, n - k * B)
+                t = b0 + bk
+                stages.append(('c', np.array([(G + 8 * (b0 * n + b0), Rp + 8 * (b0 * n + b0),
+                                               Ri + 8 * (b0 * n + b0), st,
+                                               tr if ps == 0 else 0, bk, n,
+                                               shift if ps == 0 else 0.0)], dtype=L.CHOL_JOB)))
+                if t < n:
+                    stages.append(G1(Rp + 8 * (b0 * n + t), bk, n - t, n,
+                                     [(Ri + 8 * (b0 * n + b0), n, 1, G + 8 * (b0 * n + t), n, 0, bk,
+                                       1.0)]))
+                    stages.append(G1(G + 8 * (t * n + t), n - t, n - t, n,
+                                     [(Rp + 8 * (b0 * n + t), n, 1, Rp + 8 * (b0 * n + t), n, 0, bk,
+                                       -1.0)], c0=G + 8 * (t * n + t), ldc0=n, beta=1.0))
+            for k in range(nb):
+                b0, bk = k * B, min(B, n - k * B)
+                segs = [(X, n, 0, Rp + 8 * b0, n, 0, b0, -1.0)] if b0 > 0 else []
+                stages.append(G1(Wt, m, bk, bk, segs, c0=X + 8 * b0, ldc0=n, beta=1.0))
+                stages.append(G1(X + 8 * b0, m, bk, n, [(Wt, bk, 0, Ri + 8 * (b0 * n + b0), n, 0,
+                                                         bk, 1.0)]))
+        if right:  # A_q <- Q^T
+            stages.append(G1(A, n, m, m, [(I, LDI, 0, X, n, 1, n, 1.0)]))
+        stages.append(G1(R21, n, n, n, [(Rs[1], n, 0, Rs[0], n, 0, n, 1.0)]))
+        stages.append(G1(self.Rq.data_ptr(), n, n, n, [(Rs[2], n, 0, R21, n, 0, n, 1.0)]))
+        ph['cq'] = stages
+
     def _use_cholqr(self, m, n):
         return m >= n and 8 <= n <= L.CHOL_NMAX
+
+    def _use_cholqr_blocked(self, m, n):
+        return m >= n and n > L.CHOL_NMAX
 
     def _plan_gauge(self, ph, q, dirn, copy_back):
         d, dims = self.d, self.dims
@@ -356,6 +428,8 @@ class Solve:
                 raise ValueError('bond dimension %d exceeds the QR kernel limit %d' % (k, L.QR_KMAX))
             if self._use_cholqr(m, nn):
                 self._plan_cholqr(ph, q, m, nn, right=False)
+            elif self._use_cholqr_blocked(m, nn):
+                self._plan_cholqr_blocked(ph, q, m, nn, right=False)
             else:
                 ph['qr'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(),
                                       self.Rq.data_ptr(), self.qrw.data_ptr(), m, nn, nn, 0, 0, 0)],
@@ -378,6 +452,8 @@ class Solve:
                 raise ValueError('bond dimension %d exceeds the QR kernel limit %d' % (k, L.QR_KMAX))
             if self._use_cholqr(m, nn):
                 self._plan_cholqr(ph, q, m, nn, right=True)
+            elif self._use_cholqr_blocked(m, nn):
+                self._plan_cholqr_blocked(ph, q, m, nn, right=True)
             else:
                 ph['qr'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(),
                                       self.Rq.data_ptr(), self.qrw.data_ptr(), m, nn, m, 1, 1, 0)],
@@ -854,13 +930,19 @@ class Engine:
             for j, t, _ in mixes:
                 arrs += [j, t]
             cq_meta = []
-            cqs = sel('cq')
-            if cqs:
-                for k in range(len(cqs[0])):
-                    if cqs[0][k][0] == 'g':
+            groups_cq = collections.OrderedDict()   # same stage structure -> one launch/stage
+            for c in sel('cq'):
+                groups_cq.setdefault(tuple(kind for kind, _ in c), []).append(c)
+            for sig, cqs in groups_cq.items():
+                for k, kind in enumerate(sig):
+                    if kind == 'g':
                         p, sg, nt = R.concat_gemm([c[k][1] for c in cqs])
                         cq_meta.append(('g', len(p), nt, len(arrs)))
                         arrs += [p, sg]
+                    elif kind == 'm':
+                        j, t, nb = R.concat_mix([c[k][1] for c in cqs])
+                        cq_meta.append(('m', len(j), nb, len(arrs)))
+                        arrs += [j, t]
                     else:
                         jobs = R.concat_plain([c[k][1] for c in cqs], L.CHOL_JOB)
                         cq_meta.append(('c', len(jobs), 0, len(arrs)))
@@ -895,6 +977,10 @@ class Engine:
                 if nt:
                     L.check(lib.tnx_gemm_grouped(L.GEMM_F64, ctypes_void(ptr[ai]), cnt,
                                                  ctypes_void(ptr[ai + 1]), nt, None, st_), 'cq gemm')
+            elif kind == 'm':
+                if nt:
+                    L.check(lib.tnx_mix(ctypes_void(ptr[ai]), cnt, ctypes_void(ptr[ai + 1]), nt,
+                                        opsp, None, st_), 'cq mix')
             elif cnt:
                 L.check(lib.tnx_chol_inv_f64(ctypes_void(ptr[ai]), cnt, st_), 'tnx_chol_inv_f64')
         run_gemm(0)      # absorb R into the neighbour

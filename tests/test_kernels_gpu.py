@@ -223,7 +223,7 @@ def test_chol_inv_f64(gpu, n):
     ri = torch.zeros(n * n, dtype=torch.float64, device='cuda')
     st = torch.zeros(2, dtype=torch.int32, device='cuda')
     shift = 1e-12
-    jobs = np.array([(Gd.data_ptr(), r.data_ptr(), ri.data_ptr(), st.data_ptr(), n, 0, shift)],
+    jobs = np.array([(Gd.data_ptr(), r.data_ptr(), ri.data_ptr(), st.data_ptr(), 0, n, n, shift)],
                     dtype=L.CHOL_JOB)
     (dj,) = _upload([jobs])
     print('chol_inv n', n, flush=True)
