@@ -155,13 +155,28 @@ __device__ __forceinline__ double ld_part(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Reduce the partials of solve s for k in [0, nk) into dst (last block only).
+// Reduce the partials of solve s for k in [0, nk) into dst (last block only,
+// all NT lanes).  Wave w sums the entries k = w, w + 4, ...: its 64 lanes
+// stride over the chunks and four entries are read per pass, so the loads
+// (which come from other XCDs' writes, past this XCD's L2) are in flight
+// together instead of one dependent round trip per chunk.  The summation
+// order depends only on nch.
 __device__ __forceinline__ void reduce_partials(const double* part, int chunk_begin, int nch,
                                                 int nk, double* dst) {
-  for (int k = threadIdx.x; k < nk; k += NT) {
-    double s = 0.0;
-    for (int c = 0; c < nch; ++c) s += ld_part(part + (int64_t)(chunk_begin + c) * (NCV_MAX + 1) + k);
-    dst[k] = s;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int k0 = wave; k0 < nk; k0 += 16) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = lane; c < nch; c += 64) {
+      const double* p = part + (int64_t)(chunk_begin + c) * (NCV_MAX + 1);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (k0 + 4 * u < nk) s[u] += ld_part(p + k0 + 4 * u);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double v = tnx_wave_sum(s[u]);
+      if (lane == 0 && k0 + 4 * u < nk) dst[k0 + 4 * u] = v;
+    }
   }
 }
 
@@ -179,15 +194,20 @@ __device__ inline int sturm_count(const double* a, const double* b, int k, doubl
   return cnt;
 }
 
-// Lowest eigenpair of the k x k tridiagonal (a, b); called by wave 0 only;
-// lane 0 writes y[0..k) and *theta.
+// Lowest eigenpair of the k x k tridiagonal (a, b), all NT lanes of the
+// block (a, b in LDS); thread 0 writes y[0..k) and *theta.  `work` is LDS
+// scratch of 5 * NCV_MAX doubles for the inverse iteration.  The bracket
+// shrinks 257-fold per round: lane t evaluates the Sturm count at the
+// (t + 1)-th of 256 interior points and the first point with a count >= 1
+// bounds the lowest eigenvalue.
 __device__ void tridiag_lowest(const double* a, const double* b, int k, double* y,
-                               double* theta) {
-  const int lane = threadIdx.x & 63;
+                               double* theta, double* work) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (k == 1) {
-    if (lane == 0) { y[0] = 1.0; *theta = a[0]; }
+    if (threadIdx.x == 0) { y[0] = 1.0; *theta = a[0]; }
     return;
   }
+  __shared__ unsigned long long masks[NT / 64];
   double lo = a[0], hi = a[0];
   for (int i = 0; i < k; ++i) {
     double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < k - 1 ? fabs(b[i]) : 0.0);
@@ -197,28 +217,40 @@ __device__ void tridiag_lowest(const double* a, const double* b, int k, double* 
   const double span = fmax(hi - lo, 1e-300);
   lo -= 1e-12 * span;
   hi += 1e-12 * span;
+  constexpr double NSUB = NT + 1;
   for (int round = 0; round < 16; ++round) {
-    const double w = hi - lo;
+    const double w = hi - lo;  // identical in every lane
     if (w <= 4.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)) + 1e-300) break;
-    const double x = lo + w * (double)(lane + 1) / 65.0;
+    const double x = lo + w * (double)(threadIdx.x + 1) / NSUB;
     const int cnt = sturm_count(a, b, k, x);
     const unsigned long long m = __ballot(cnt >= 1);
-    if (m == 0ull) {
-      lo = lo + w * 64.0 / 65.0;
+    if (lane == 0) masks[wave] = m;
+    __syncthreads();
+    int f = -1;
+    for (int q = 0; q < NT / 64; ++q)
+      if (masks[q] != 0ull) {
+        f = q * 64 + __ffsll((long long)masks[q]) - 1;
+        break;
+      }
+    __syncthreads();  // masks are rewritten next round
+    if (f < 0) {
+      lo = lo + w * (double)NT / NSUB;
     } else {
-      const int f = __ffsll((long long)m) - 1;
-      const double xf = lo + w * (double)(f + 1) / 65.0;
-      const double xp = lo + w * (double)f / 65.0;
+      const double xf = lo + w * (double)(f + 1) / NSUB;
+      const double xp = lo + w * (double)f / NSUB;
       hi = xf;
       lo = xp;
     }
   }
-  if (lane != 0) return;
+  if (threadIdx.x != 0) return;
   const double th = 0.5 * (lo + hi);
   *theta = th;
   // inverse iteration with partial pivoting on (T - th I)
-  double z[NCV_MAX], rhs[NCV_MAX];
-  double u0[NCV_MAX], u1[NCV_MAX], u2[NCV_MAX];
+  double* z = work;
+  double* rhs = work + NCV_MAX;
+  double* u0 = work + 2 * NCV_MAX;
+  double* u1 = work + 3 * NCV_MAX;
+  double* u2 = work + 4 * NCV_MAX;
   const double tiny = 2.220446049250313e-16 * span;
   for (int i = 0; i < k; ++i) z[i] = 1.0 / sqrt((double)k);
   for (int it = 0; it < 2; ++it) {
@@ -276,10 +308,10 @@ __global__ __launch_bounds__(NT) void k_init_norm(const tnx_lanczos_solve* __res
   if (threadIdx.x == 0) L.part[(int64_t)blockIdx.x * (NCV_MAX + 1)] = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
   if (last_block(&L.st[ci.s].counter, ci.nch)) {
     __shared__ double tot;
+    reduce_partials(L.part, D.chunk_begin, ci.nch, 1, &tot);
+    __syncthreads();
     if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int c = 0; c < ci.nch; ++c) s += ld_part(L.part + (int64_t)(D.chunk_begin + c) * (NCV_MAX + 1));
-      tot = s;
+      const double s = tot;
       LzState& st = L.st[ci.s];
       st.scale = s > 0.0 ? 1.0 / sqrt(s) : -1.0;
       st.converged = 0;
@@ -410,11 +442,12 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
     L.part[(int64_t)blockIdx.x * (NCV_MAX + 1)] = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
   if (last_block(&L.st[ci.s].counter, ci.nch)) {
     LzState& st = L.st[ci.s];
-    __shared__ double a_s[NCV_MAX], b_s[NCV_MAX], y_s[NCV_MAX], th_s;
+    __shared__ double a_s[NCV_MAX], b_s[NCV_MAX], y_s[NCV_MAX], th_s, nn_s;
+    __shared__ double work[5 * NCV_MAX];
+    reduce_partials(L.part, D.chunk_begin, ci.nch, 1, &nn_s);
+    __syncthreads();
     if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int c = 0; c < ci.nch; ++c) s += ld_part(L.part + (int64_t)(D.chunk_begin + c) * (NCV_MAX + 1));
-      const double beta = sqrt(s);
+      const double beta = sqrt(nn_s);
       st.alpha[j] = st.h1[j] + st.h2[j];
       st.beta[j] = beta;
       st.iters += 1;
@@ -424,7 +457,7 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
       }
     }
     __syncthreads();
-    if (threadIdx.x < 64) tridiag_lowest(a_s, b_s, j + 1, y_s, &th_s);
+    tridiag_lowest(a_s, b_s, j + 1, y_s, &th_s, work);
     __syncthreads();
     if (threadIdx.x == 0) {
       const double beta = b_s[j];
@@ -488,9 +521,11 @@ __global__ __launch_bounds__(NT) void k_ritz(const tnx_lanczos_solve* __restrict
   if (threadIdx.x == 0)
     L.part[(int64_t)blockIdx.x * (NCV_MAX + 1)] = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
   if (last_block(&L.st[ci.s].counter, ci.nch)) {
+    __shared__ double tot;
+    reduce_partials(L.part, D.chunk_begin, ci.nch, 1, &tot);
+    __syncthreads();
     if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int c = 0; c < ci.nch; ++c) s += ld_part(L.part + (int64_t)(D.chunk_begin + c) * (NCV_MAX + 1));
+      const double s = tot;
       L.st[ci.s].scale = 1.0 / sqrt(s);
       L.st[ci.s].restarts += 1;
       L.st[ci.s].counter = 0;

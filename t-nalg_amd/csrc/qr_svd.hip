@@ -271,13 +271,22 @@ __global__ __launch_bounds__(NT) void svd_kernel(const tnx_svd_job* __restrict__
   }
 }
 
-// Cholesky factor + inverse of one n x n SPD matrix in LDS (n <= 128),
-// 256 lanes.  S = G + shift I = R^T R, right-looking; then in-place upper
-// triangular inversion column by column.
+// Cholesky factor + inverse of one n x n SPD matrix (n <= 128), 256 lanes.
+// Lane (tr, tc) = (tid / 16, tid % 16) holds the 8 x 8 register tile of rows
+// {tr + 16a} and columns {tc + 16b}; the cyclic split keeps the shrinking
+// trailing matrix spread over every lane.  Right-looking Cholesky of
+// S = G + shift I = R^T R: step k broadcasts row k through LDS (rows
+// double-buffered, so one barrier per step) and each lane applies the rank-1
+// update to its tile.  R is then parked in LDS and R^{-1} is formed from the
+// bottom row up: X[k,:] /= R[k,k], then X[i,:] -= R[i,k] X[k,:] for i < k.
 constexpr int CHOL_NMAX = 128;
+constexpr int CHOL_LD = CHOL_NMAX + 1;  // LDS row stride of R: column reads hit distinct banks
+constexpr size_t CHOL_LDS = sizeof(double) * (CHOL_NMAX * CHOL_LD + 2 * CHOL_NMAX);
 
 __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __restrict__ jobs) {
-  extern __shared__ double S[];  // n x n, row stride n
+  extern __shared__ double lds[];
+  double* Rl = lds;                         // R, row stride CHOL_LD
+  double* rowbuf = lds + CHOL_NMAX * CHOL_LD;  // two broadcast rows
   const tnx_chol_job J = jobs[blockIdx.x];
   const int n = J.n;
   if (n <= 0 || n > CHOL_NMAX) return;  // rejected on the host
@@ -285,73 +294,135 @@ __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __rest
   const int ld = J.ld > 0 ? J.ld : n;
   __shared__ double red[4];
   __shared__ double shift;
-  __shared__ int bad;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double tr = 0.0;
-  for (int e = threadIdx.x; e < n * n; e += NT) {
-    const int i = e / n, j = e % n;
-    const double v = G[(int64_t)i * ld + j];
-    S[e] = v;
-    if (i == j) tr += v;
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  double s[8][8];
+  double trace = 0.0;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int i = tr + 16 * a;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int j = tc + 16 * b;
+      s[a][b] = (i < n && j < n) ? G[(int64_t)i * ld + j] : 0.0;
+      if (i == j) trace += s[a][b];
+    }
   }
-  tr = tnx_wave_sum(tr);
-  if (lane == 0) red[wave] = tr;
+  trace = tnx_wave_sum(trace);
+  if (lane == 0) red[wave] = trace;
   __syncthreads();
   if (threadIdx.x == 0) {
     const double t = J.shift_ref ? *(const double*)J.shift_ref
                                  : ((red[0] + red[1]) + (red[2] + red[3]));
     shift = J.shift_scale * t;
-    bad = 0;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += NT) S[i * n + i] += shift;
-  __syncthreads();
+  {
+    const double sh = shift;
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int b = 0; b < 8; ++b)
+        if (tr + 16 * a == tc + 16 * b && tr + 16 * a < n) s[a][b] += sh;
+  }
   const double tiny = 1e-300;
-  // S = R^T R, R upper, stored in the upper triangle of S
+  int bad = 0;
+  // Cholesky: row k of the tiles becomes row k of R at step k.  The whole
+  // symmetric trailing block is updated (both triangles), so row k is
+  // current when its owners broadcast it.
   for (int k = 0; k < n; ++k) {
-    if (threadIdx.x == 0) {
-      double d = S[k * n + k];
-      if (!(d > 0.0)) {
-        d = tiny;
-        bad = 1;
-      }
-      S[k * n + k] = sqrt(d);
+    const int ka = k >> 4;
+    double* rb = rowbuf + (k & 1) * CHOL_NMAX;
+    if (tr == (k & 15)) {
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+        if (a == ka) {
+#pragma unroll
+          for (int b = 0; b < 8; ++b) rb[tc + 16 * b] = s[a][b];
+        }
     }
     __syncthreads();
-    const double rk = 1.0 / S[k * n + k];
-    for (int j = k + 1 + threadIdx.x; j < n; j += NT) S[k * n + j] *= rk;
-    __syncthreads();
-    const int m = n - k - 1;  // trailing upper triangle, (i, j) with k < i <= j
-    for (int e = threadIdx.x; e < m * m; e += NT) {
-      const int i = k + 1 + e / m, j = k + 1 + e % m;
-      if (j >= i) S[i * n + j] -= S[k * n + i] * S[k * n + j];
+    double d = rb[k];
+    if (!(d > 0.0)) {
+      d = tiny;
+      bad = 1;
     }
-    __syncthreads();
+    const double sq = sqrt(d), ri = 1.0 / sq;
+    double ci[8], cj[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) ci[a] = (tr + 16 * a > k) ? rb[tr + 16 * a] * ri : 0.0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) cj[b] = (tc + 16 * b > k) ? rb[tc + 16 * b] * ri : 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int b = 0; b < 8; ++b) s[a][b] = fma(-ci[a], cj[b], s[a][b]);
+    if (tr == (k & 15)) {
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+        if (a == ka) {
+#pragma unroll
+          for (int b = 0; b < 8; ++b) {
+            const int j = tc + 16 * b;
+            s[a][b] = (j > k) ? s[a][b] * ri : (j == k ? sq : s[a][b]);
+          }
+        }
+    }
   }
   double* R = (double*)J.r;
-  for (int e = threadIdx.x; e < n * n; e += NT) {
-    const int i = e / n, j = e % n;
-    R[(int64_t)i * ld + j] = (j >= i) ? S[e] : 0.0;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int i = tr + 16 * a;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int j = tc + 16 * b;
+      if (i < n && j < n) R[(int64_t)i * ld + j] = (j >= i) ? s[a][b] : 0.0;
+      Rl[i * CHOL_LD + j] = s[a][b];
+    }
   }
   __syncthreads();
-  // in-place inverse of the upper triangle: column j from the inverted
-  // leading block, Sinv[i][j] = -(sum_{k=i}^{j-1} Sinv[i][k] R[k][j]) / R[j][j]
-  for (int j = 0; j < n; ++j) {
-    const double inv = 1.0 / S[j * n + j];
-    double t = 0.0;
-    const int i = threadIdx.x;
-    if (i < j) {
-      for (int k = i; k < j; ++k) t += S[i * n + k] * S[k * n + j];
+  // X = R^{-1}: the tiles restart as the identity; row k is final once the
+  // rows below it have been subtracted.
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      s[a][b] = (tr + 16 * a == tc + 16 * b && tr + 16 * a < n) ? 1.0 : 0.0;
+  for (int k = n - 1; k >= 0; --k) {
+    const int ka = k >> 4;
+    double* xb = rowbuf + (k & 1) * CHOL_NMAX;
+    if (tr == (k & 15)) {
+      const double rkk = 1.0 / Rl[k * CHOL_LD + k];
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+        if (a == ka) {
+#pragma unroll
+          for (int b = 0; b < 8; ++b) {
+            s[a][b] *= rkk;
+            xb[tc + 16 * b] = s[a][b];
+          }
+        }
     }
     __syncthreads();
-    if (i < j) S[i * n + j] = -t * inv;
-    if (i == j) S[j * n + j] = inv;
-    __syncthreads();
+    double ra[8], xv[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) ra[a] = (tr + 16 * a < k) ? Rl[(tr + 16 * a) * CHOL_LD + k] : 0.0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) xv[b] = xb[tc + 16 * b];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int b = 0; b < 8; ++b) s[a][b] = fma(-ra[a], xv[b], s[a][b]);
   }
   double* Ri = (double*)J.rinv;
-  for (int e = threadIdx.x; e < n * n; e += NT) {
-    const int i = e / n, j = e % n;
-    Ri[(int64_t)i * ld + j] = (j >= i) ? S[e] : 0.0;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int i = tr + 16 * a;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int j = tc + 16 * b;
+      if (i < n && j < n) Ri[(int64_t)i * ld + j] = (j >= i) ? s[a][b] : 0.0;
+    }
   }
   if (threadIdx.x == 0 && J.status != 0) *(int32_t*)J.status = bad;
 }
@@ -365,13 +436,12 @@ extern "C" int tnx_chol_inv_f64(const tnx_chol_job* jobs, int njob, tnx_stream_t
     return TNX_EINVAL;
   }
   static bool attr_set = false;
-  const size_t lds = sizeof(double) * CHOL_NMAX * CHOL_NMAX;
   if (!attr_set) {
     TNX_CHECK_HIP(hipFuncSetAttribute((const void*)chol_inv_kernel,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)CHOL_LDS));
     attr_set = true;
   }
-  hipLaunchKernelGGL(chol_inv_kernel, dim3(njob), dim3(NT), lds, (hipStream_t)stream, jobs);
+  hipLaunchKernelGGL(chol_inv_kernel, dim3(njob), dim3(NT), CHOL_LDS, (hipStream_t)stream, jobs);
   TNX_CHECK_LAUNCH();
   return TNX_OK;
 }
