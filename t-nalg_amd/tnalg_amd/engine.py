@@ -20,6 +20,7 @@ updates, the Krylov basis, reorthogonalisation and the tridiagonal solve stay
 f64.
 """
 import collections
+import hashlib
 import itertools
 import time
 
@@ -40,6 +41,8 @@ PREC = {
 
 
 _PLAN_UID = itertools.count(1)
+ARENA_ALIGN = 256
+ARENA_HEADER = 256
 
 
 def _p(t, off_elems=0, itemsize=8):
@@ -111,12 +114,33 @@ def buffer_layout(T, dm, d, ncv, lowp):
         'qrw': 2 * psi + dsq,
         'chol': 6 * dsq + max(dm) + 64,
     }
-    f64 = 8 * (sum(lay['mps']) + sum(lay['Lenv']) + sum(lay['Renv']) + lay['lc64'] + lay['Yb']
-               + lay['Wb'] + lay['V'] + 3 * psi + lay['Rq'] + lay['qrw'] + lay['chol'])
-    u8 = lay['lc_lp'] + lay['Tb'] + lay['Ub'] + 8 * psi
+    # arena: (name, index, bytes, f64 view?) in address order.  The MPS and
+    # the CholeskyQR constants come first (staged from the host in one
+    # copy); everything after them starts zeroed.  The first ARENA_HEADER
+    # bytes stay unused, so no buffer sits at relative address 0 (0 marks a
+    # null pointer in the descriptor records).
+    spec = [('mps', s, 8 * x, True) for s, x in enumerate(lay['mps'])]
+    spec.append(('cholb', None, 8 * lay['chol'], True))
+    stage_end = len(spec)
+    spec += [('Lenv', b, 8 * x, True) for b, x in enumerate(lay['Lenv'])]
+    spec += [('Renv', b, 8 * x, True) for b, x in enumerate(lay['Renv'])]
     if lowp:
-        u8 += 4 * (sum(lay['Lenv']) + sum(lay['Renv']))
-    lay['bytes'] = f64 + u8
+        spec += [('Lenv_lp', b, 4 * x, False) for b, x in enumerate(lay['Lenv'])]
+        spec += [('Renv_lp', b, 4 * x, False) for b, x in enumerate(lay['Renv'])]
+    spec += [('lc64', None, 8 * lay['lc64'], True), ('lc_lp', None, lay['lc_lp'], False),
+             ('Tb', None, lay['Tb'], False), ('Ub', None, lay['Ub'], False),
+             ('Yb', None, 8 * lay['Yb'], True), ('Wb', None, 8 * lay['Wb'], True),
+             ('V', None, 8 * lay['V'], True), ('w', None, 8 * psi, True),
+             ('x', None, 8 * psi, False), ('init', None, 8 * psi, True),
+             ('Rq', None, 8 * lay['Rq'], True), ('qrw', None, 8 * lay['qrw'], True)]
+    offs, o = [], ARENA_HEADER
+    for k, (_, _, nb, _) in enumerate(spec):
+        if k == stage_end:
+            lay['stage_bytes'] = o
+        offs.append(o)
+        o += (nb + ARENA_ALIGN - 1) // ARENA_ALIGN * ARENA_ALIGN
+    lay['arena'] = [(name, idx, off, nb, f64) for (name, idx, nb, f64), off in zip(spec, offs)]
+    lay['bytes'] = o
     return lay
 
 
@@ -166,48 +190,53 @@ class Solve:
         self.dims = list(dims)
         self.dmax = max_dims(dims, d, self.ob)
         dm = self.dmax
-        f64 = torch.float64
-        self.mps = [torch.from_numpy(np.ascontiguousarray(host[s]).reshape(-1)).to(self.dev)
-                    for s in range(n)]
-        for s in range(n):  # room for the largest shape the site takes
-            need = dm[s] * d * dm[s + 1]
-            if self.mps[s].numel() < need:
-                t = torch.zeros(need, dtype=f64, device=self.dev)
-                t[:self.mps[s].numel()] = self.mps[s]
-                self.mps[s] = t
         T = self.T
         self.nL = [len(T.left_opens[b]) for b in range(n + 1)]
         self.nR = [len(T.right_opens[b]) for b in range(n + 1)]
         lay = buffer_layout(T, dm, d, engine.ncv, engine.uses_lowp)
         self.layout = lay
         self.psi_max, self.dsq_max = lay['psi'], lay['dsq']
-        z = lambda nbytes: torch.zeros(nbytes, dtype=torch.uint8, device=self.dev)
-        zf = lambda nel: torch.zeros(nel, dtype=f64, device=self.dev)
-        self.Lenv = [zf(x) for x in lay['Lenv']]
-        self.Renv = [zf(x) for x in lay['Renv']]
-        self.lp_bytes = 4 if engine.uses_lowp else 0
-        if self.lp_bytes:
-            self.Lenv_lp = [z(x * self.lp_bytes) for x in lay['Lenv']]
-            self.Renv_lp = [z(x * self.lp_bytes) for x in lay['Renv']]
-        self.lc64 = zf(lay['lc64'])
-        self.lc_lp = z(lay['lc_lp'])
-        self.Tb = z(lay['Tb'])
-        self.Ub = z(lay['Ub'])
-        self.Yb = zf(lay['Yb'])
-        self.Wb = zf(lay['Wb'])
-        self.V = zf(lay['V'])
-        self.w = zf(lay['psi'])
-        self.x = z(lay['psi'] * 8)
-        self.init = zf(lay['psi'])
-        self.Rq = zf(lay['Rq'])
-        self.qrw = zf(lay['qrw'])
-        self.cholb = zf(lay['chol'])
         self.dmax_max = max(dm)
-        # identity (stride dmax) and ones for the blocked CholeskyQR
-        eye = torch.arange(self.dmax_max, device=self.dev) * (self.dmax_max + 1)
-        self.cholb[5 * lay['dsq'] + eye] = 1.0
-        self.cholb[6 * lay['dsq']:6 * lay['dsq'] + self.dmax_max] = 1.0
+        self.lp_bytes = 4 if engine.uses_lowp else 0
+        # one arena per solve; every buffer is a view into it
+        self.arena = engine.arena_get(lay['bytes'])
+        self.base = self.arena.data_ptr()
+        views = {'mps': [None] * n, 'Lenv': [None] * (n + 1), 'Renv': [None] * (n + 1),
+                 'Lenv_lp': [None] * (n + 1), 'Renv_lp': [None] * (n + 1)}
+        for name, idx, off, nb, f64 in lay['arena']:
+            v = self.arena[off:off + nb]
+            if f64:
+                v = v.view(torch.float64)
+            if idx is None:
+                setattr(self, name, v)
+            else:
+                views[name][idx] = v
+        self.mps, self.Lenv, self.Renv = views['mps'], views['Lenv'], views['Renv']
+        if self.lp_bytes:
+            self.Lenv_lp, self.Renv_lp = views['Lenv_lp'], views['Renv_lp']
+        # staged region: the initial MPS (each site padded to the largest
+        # shape it takes) and the blocked-CholeskyQR identity (stride dmax)
+        # and ones; the rest of the arena is zero
+        stage = np.zeros(lay['stage_bytes'], dtype=np.uint8)
+        offs = {(name, idx): off for name, idx, off, _, _ in lay['arena']}
+        for s in range(n):
+            a = np.ascontiguousarray(host[s], dtype=np.float64).reshape(-1)
+            o = offs[('mps', s)]
+            stage[o:o + a.nbytes] = a.view(np.uint8)
+        cb = stage[offs[('cholb', None)]:].view(np.float64)
+        dsq, dmx = lay['dsq'], self.dmax_max
+        cb[5 * dsq + np.arange(dmx) * (dmx + 1)] = 1.0
+        cb[6 * dsq:6 * dsq + dmx] = 1.0
+        engine.arena_fill(self.arena, stage)
         self.ops_slot = engine.alloc_ops(T.ops)
+        self.reloc = (self.base, self.ops_slot)
+        # plans depend on the term tables (couplings, operator indices), the
+        # shapes and the solver tolerances, not on the operator values
+        self.sig = hashlib.sha1(repr((
+            n, d, self.ob, tuple(dims), tuple(dm), self.eigs_tol, self.tau, T.ops.shape[0],
+            T.left_opens, T.right_opens,
+            [(st.field, st.lm, st.mr, st.strad_left, st.strad) for st in T.sites]
+        )).encode()).hexdigest()
         self.d2 = d * d
         # bookkeeping
         self.center = -1
@@ -219,7 +248,6 @@ class Solve:
         self.stats = {'matvecs': 0, 'steps': 0, 'unconverged': 0, 'restarts': 0}
         self.lm = [np.zeros(0) for _ in range(n - 1)]
         self.failed = None
-        self.cache = {}
 
     # ----------------------------------------------------------- addresses
     def op_index(self, k):
@@ -242,12 +270,17 @@ class Solve:
         return self.Renv_lp[b].data_ptr() + m * self.dmax[b] ** 2 * ITEMSIZE[ti]
 
     def free(self):
+        if self.arena is None:
+            return
         self.engine.free_ops(self.ops_slot)
         for name in ('mps', 'Lenv', 'Renv', 'Lenv_lp', 'Renv_lp'):
             if hasattr(self, name):
                 setattr(self, name, None)
-        for name in ('lc64', 'lc_lp', 'Tb', 'Ub', 'Yb', 'Wb', 'V', 'w', 'x', 'init', 'Rq', 'qrw'):
+        for name in ('lc64', 'lc_lp', 'Tb', 'Ub', 'Yb', 'Wb', 'V', 'w', 'x', 'init', 'Rq', 'qrw',
+                     'cholb'):
             setattr(self, name, None)
+        self.engine.arena_release(self.arena)
+        self.arena = None
 
     # ------------------------------------------------------------ schedule
     def schedule_canonicalise(self):
@@ -277,12 +310,22 @@ class Solve:
 
     # ------------------------------------------------------------- planning
     def plan(self, st, prec):
-        """Descriptor records of one step (dict phase -> arrays)."""
-        key = (st.kind, st.site, st.dir, tuple(self.dims), prec)
-        hit = self.cache.get(key)
+        """Descriptor records of one step (dict phase -> arrays), shared by
+        every solve of the engine with the same signature: device addresses
+        are relative to the solve's arena and operator indices to its slot of
+        the operator table (``records.relocate`` moves them to a solve)."""
+        key = (self.sig, st.kind, st.site, st.dir, tuple(self.dims), prec)
+        cache = self.engine.plan_cache
+        hit = cache.get(key)
         if hit is not None:
             self.dims = list(hit['dims_after'])
             return hit
+        ph = self._build_plan(st, prec)
+        R.make_relative(ph, self.base, self.base + self.layout['bytes'], self.ops_slot)
+        cache[key] = ph
+        return ph
+
+    def _build_plan(self, st, prec):
         ph = {}
         if st.kind in ('move', 'solve'):
             q = st.site if st.kind == 'move' else st.site - st.dir
@@ -296,7 +339,6 @@ class Solve:
             self._plan_svd(ph, st.site, st.dir)
         ph['dims_after'] = list(self.dims)
         ph['uid'] = next(_PLAN_UID)
-        self.cache[key] = ph
         return ph
 
     def _plan_cholqr(self, ph, q, m, n, right):
@@ -733,7 +775,9 @@ class Engine:
         self.lz_state = None
         self._resident = collections.OrderedDict()
         self._resident_bytes = 0
-        self.resident_limit = 256 << 20
+        self.resident_limit = 1 << 30   # descriptor sets of every distinct step of a sweep
+        self.plan_cache = {}
+        self._arena_pool = {}   # bytes -> freed arenas (reused by the next solve of that size)
         self.counters = {'matvec_flops': 0.0, 'matvecs': 0, 'lanczos_calls': 0, 'steps': 0,
                          'lz_time': 0.0, 't_plan': 0.0, 't_pre': 0.0, 't_lz_total': 0.0,
                          'lz_iters': 0}
@@ -757,12 +801,44 @@ class Engine:
         raise RuntimeError('operator table full')
 
     def free_ops(self, slot):
+        """Return a slot; the free list stays sorted and coalesced, so the
+        next allocation takes the lowest free slot (a batch admitted after
+        another of the same shape gets the same slots back)."""
         cnt = self._ops_owner.pop(slot, 0)
-        if cnt:
-            self.ops_free.append((slot, cnt))
+        if not cnt:
+            return
+        free = sorted(self.ops_free + [(slot, cnt)])
+        merged = [free[0]]
+        for s, c in free[1:]:
+            ps, pc = merged[-1]
+            if ps + pc == s:
+                merged[-1] = (ps, pc + c)
+            else:
+                merged.append((s, c))
+        self.ops_free = merged
 
     def stream(self):
         return ctypes_void(torch.cuda.current_stream().cuda_stream)
+
+    # ------------------------------------------------------------- arenas
+    def arena_get(self, nbytes):
+        pool = self._arena_pool.get(nbytes)
+        if pool:
+            return pool.pop(0)   # first released, first reused: a repeated batch keeps its arenas
+        return torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+
+    def arena_fill(self, arena, stage):
+        """Arena prefix <- ``stage`` (host bytes, one pinned copy); the rest
+        of the arena <- 0.  Both are ordered on the engine's stream."""
+        n = stage.nbytes
+        pinned = torch.empty(n, dtype=torch.uint8, pin_memory=True)
+        pinned.numpy()[:] = stage
+        arena[:n].copy_(pinned, non_blocking=True)
+        arena[n:].zero_()
+        self._stage_keep = pinned   # alive until the next fill (same stream, ordered)
+
+    def arena_release(self, arena):
+        self._arena_pool.setdefault(arena.numel(), []).append(arena)
 
     # ------------------------------------------------- resident descriptors
     def _resident_get(self, key):
@@ -848,15 +924,16 @@ class Engine:
         """group: list of (solve, plan) sharing one precision."""
         prec = group[0][1]['prec']
         ti, tc, mode1, mode2, _ = PREC[prec]
-        rkey = ('lz',) + tuple(ph['uid'] for _, ph in group)
+        rkey = ('lz',) + tuple((ph['uid'],) + s.reloc for s, ph in group)
         ent = self._resident_get(rkey)
         if ent is None:
             actives = list(range(len(group)))
-            lc_j, lc_t, lc_nb = R.concat_mix([ph['mv_lc'] for _, ph in group])
-            g1p, g1s, g1t = R.concat_gemm([ph['mv_g1'] for _, ph in group], actives)
-            mxj, mxt, mxb = R.concat_mix([ph['mv_mix'] for _, ph in group], actives)
-            g2p, g2s, g2t = R.concat_gemm([ph['mv_g2'] for _, ph in group], actives)
-            lz = np.concatenate([ph['lz'] for _, ph in group])
+            rls = [s.reloc for s, _ in group]
+            lc_j, lc_t, lc_nb = R.concat_mix([ph['mv_lc'] for _, ph in group], relocs=rls)
+            g1p, g1s, g1t = R.concat_gemm([ph['mv_g1'] for _, ph in group], actives, rls)
+            mxj, mxt, mxb = R.concat_mix([ph['mv_mix'] for _, ph in group], actives, rls)
+            g2p, g2s, g2t = R.concat_gemm([ph['mv_g2'] for _, ph in group], actives, rls)
+            lz = R.concat_plain([ph['lz'] for _, ph in group], L.LANCZOS_SOLVE, relocs=rls)
             chunks = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
             lz['chunk_begin'] = np.concatenate([[0], np.cumsum(chunks)[:-1]])
             meta = {'lc': (len(lc_j), lc_nb), 'g1': (len(g1p), g1t), 'mix': (len(mxj), mxb),
@@ -913,17 +990,23 @@ class Engine:
         t_b = time.perf_counter()
 
         def sel(key):
-            return [ph[key] for _, _, ph in plans if key in ph]
+            """(relative parts, relocations) of the plans that have phase ``key``."""
+            got = [(ph[key], s.reloc) for s, _, ph in plans if key in ph]
+            return [g[0] for g in got], [g[1] for g in got]
 
         # every descriptor of the step's pre-Lanczos phases, resident on the
-        # device per combination of plans (steps repeat every sweep)
-        rkey = ('pre',) + tuple(ph['uid'] for _, _, ph in plans)
+        # device per combination of (plan, solve) (steps repeat every sweep)
+        rkey = ('pre',) + tuple((ph['uid'],) + s.reloc for s, _, ph in plans)
         ent = self._resident_get(rkey)
         if ent is None:
-            qr = R.concat_plain(sel('qr'), L.QR_JOB)
-            svd = R.concat_plain(sel('svd'), L.SVD_JOB)
-            gemms = [R.concat_gemm(sel(k)) for k in ('absorb', 'env_g1', 'env_gT')]
-            mixes = [R.concat_mix(sel(k)) for k in ('copy', 'env_lc', 'env_mix', 'cast')]
+            def cat(fn, key, *extra):
+                parts, rls = sel(key)
+                return fn(parts, *extra, relocs=rls)
+
+            qr = cat(R.concat_plain, 'qr', L.QR_JOB)
+            svd = cat(R.concat_plain, 'svd', L.SVD_JOB)
+            gemms = [cat(R.concat_gemm, k) for k in ('absorb', 'env_g1', 'env_gT')]
+            mixes = [cat(R.concat_mix, k) for k in ('copy', 'env_lc', 'env_mix', 'cast')]
             arrs = [qr, svd]
             for p, sg, _ in gemms:
                 arrs += [p, sg]
@@ -931,20 +1014,21 @@ class Engine:
                 arrs += [j, t]
             cq_meta = []
             groups_cq = collections.OrderedDict()   # same stage structure -> one launch/stage
-            for c in sel('cq'):
-                groups_cq.setdefault(tuple(kind for kind, _ in c), []).append(c)
+            for c, rl in zip(*sel('cq')):
+                groups_cq.setdefault(tuple(kind for kind, _ in c), []).append((c, rl))
             for sig, cqs in groups_cq.items():
+                rls = [rl for _, rl in cqs]
                 for k, kind in enumerate(sig):
                     if kind == 'g':
-                        p, sg, nt = R.concat_gemm([c[k][1] for c in cqs])
+                        p, sg, nt = R.concat_gemm([c[k][1] for c, _ in cqs], relocs=rls)
                         cq_meta.append(('g', len(p), nt, len(arrs)))
                         arrs += [p, sg]
                     elif kind == 'm':
-                        j, t, nb = R.concat_mix([c[k][1] for c in cqs])
+                        j, t, nb = R.concat_mix([c[k][1] for c, _ in cqs], relocs=rls)
                         cq_meta.append(('m', len(j), nb, len(arrs)))
                         arrs += [j, t]
                     else:
-                        jobs = R.concat_plain([c[k][1] for c in cqs], L.CHOL_JOB)
+                        jobs = R.concat_plain([c[k][1] for c, _ in cqs], L.CHOL_JOB, relocs=rls)
                         cq_meta.append(('c', len(jobs), 0, len(arrs)))
                         arrs.append(jobs)
             meta = {'nqr': len(qr), 'nsvd': len(svd),

@@ -88,15 +88,90 @@ def mix_blocks(j):
                                       + L.MIX_BLOCK - 1) // L.MIX_BLOCK)
 
 
-def concat_gemm(parts, actives=None):
+# ------------------------------------------------------------ relocation
+# A plan is built once per solve signature with the addresses of one solve;
+# ``make_relative`` rewrites it against that solve's arena and operator slot,
+# ``relocate`` moves a copy to any other solve.  Every 64-bit unsigned field
+# of a record is a device address (0 = null, left alone); the ``op`` field of
+# a mixer term is an operator-table index (-1 = identity, left alone).
+_PTR_FIELDS = {}
+_U64 = np.dtype('<u8')
+_MASK64 = (1 << 64) - 1
+
+
+def ptr_fields(dt):
+    f = _PTR_FIELDS.get(dt)
+    if f is None:
+        f = tuple(n for n in dt.names if dt.fields[n][0] == _U64)
+        _PTR_FIELDS[dt] = f
+    return f
+
+
+def _shift(arr, delta, slot_delta):
+    if len(arr) == 0:
+        return
+    d = np.uint64(delta & _MASK64)
+    for f in ptr_fields(arr.dtype):
+        v = arr[f]
+        nz = v != 0
+        v[nz] += d
+    if slot_delta and 'op' in arr.dtype.names:
+        o = arr['op']
+        o[o >= 0] += slot_delta
+
+
+def relocate(arr, reloc):
+    """Copy of a relative record array moved to ``reloc = (base, slot)``."""
+    out = arr.copy()
+    if reloc is not None:
+        _shift(out, reloc[0], reloc[1])
+    return out
+
+
+def _rel_one(arr, lo, hi, slot):
+    for f in ptr_fields(arr.dtype):
+        v = arr[f]
+        nz = v[v != 0]
+        if len(nz) and (int(nz.min()) < lo or int(nz.max()) >= hi):
+            raise ValueError('plan record field %r points outside the solve arena' % f)
+    _shift(arr, -lo, -slot)
+
+
+def make_relative(ph, lo, hi, slot):
+    """In place: every record array of plan ``ph`` made relative to the
+    arena ``[lo, hi)`` and operator slot ``slot``."""
+    def walk(obj):
+        if isinstance(obj, np.ndarray) and obj.dtype.names is not None:
+            _rel_one(obj, lo, hi, slot)
+        elif isinstance(obj, (tuple, list)):
+            for o in obj:
+                walk(o)
+    for k, v in ph.items():
+        if k not in ('dims_after', 'uid', 'prec', 'mv_flops', 'svd_n', 'lm_bond'):
+            walk(v)
+
+
+def _cat(arrs, dtype):
+    """Concatenation of same-dtype record arrays as raw bytes (np.concatenate
+    on structured dtypes re-derives the field promotion per call)."""
+    if not arrs:
+        return np.zeros(0, dtype)
+    if len(arrs) == 1:
+        return arrs[0]
+    return np.concatenate([a.view(np.uint8) for a in arrs]).view(dtype)
+
+
+def concat_gemm(parts, actives=None, relocs=None):
     """``parts``: list of ``(probs, segs)`` -> merged ``(probs, segs, ntiles)``.
-    ``actives[k]`` (optional) overrides the ``active`` field of part k."""
+    ``actives[k]`` (optional) overrides the ``active`` field of part k;
+    ``relocs[k]`` (optional) moves part k from relative to device addresses."""
     ps, ss = [], []
     tile_off = seg_off = 0
     for k, (p, s) in enumerate(parts):
         if len(p) == 0:
             continue
-        p = p.copy()
+        rl = relocs[k] if relocs is not None else None
+        p = relocate(p, rl)
         p['seg_begin'] += seg_off
         p['tile_begin'] += tile_off
         if actives is not None:
@@ -104,19 +179,22 @@ def concat_gemm(parts, actives=None):
         tile_off += gemm_tiles(parts[k][0])
         seg_off += len(s)
         ps.append(p)
-        ss.append(s)
+        ss.append(relocate(s, rl) if rl is not None else s)
     if not ps:
         return EMPTY_PROB.copy(), EMPTY_SEG.copy(), 0
-    return np.concatenate(ps), np.concatenate(ss), tile_off
+    segs = _cat(ss, GEMM_SEG)
+    return _cat(ps, GEMM_PROBLEM), (segs.copy() if len(ss) == 1 and relocs is None else segs), \
+        tile_off
 
 
-def concat_mix(parts, actives=None):
+def concat_mix(parts, actives=None, relocs=None):
     js, ts = [], []
     blk_off = term_off = 0
     for k, (j, t) in enumerate(parts):
         if len(j) == 0:
             continue
-        j = j.copy()
+        rl = relocs[k] if relocs is not None else None
+        j = relocate(j, rl)
         j['term_begin'] += term_off
         j['block_begin'] += blk_off
         if actives is not None:
@@ -124,14 +202,19 @@ def concat_mix(parts, actives=None):
         blk_off += mix_blocks(parts[k][0])
         term_off += len(t)
         js.append(j)
-        ts.append(t)
+        ts.append(relocate(t, rl) if rl is not None else t)
     if not js:
         return EMPTY_JOB.copy(), EMPTY_TERM.copy(), 0
-    return np.concatenate(js), np.concatenate(ts), blk_off
+    terms = _cat(ts, MIX_TERM)
+    return _cat(js, MIX_JOB), (terms.copy() if len(ts) == 1 and relocs is None else terms), \
+        blk_off
 
 
-def concat_plain(parts, dtype):
+def concat_plain(parts, dtype, relocs=None):
+    if relocs is not None:
+        parts = [relocate(p, rl) for p, rl in zip(parts, relocs) if len(p)]
+        return _cat(parts, dtype)
     parts = [p for p in parts if len(p)]
     if not parts:
         return np.zeros(0, dtype)
-    return np.concatenate(parts)
+    return _cat(parts, dtype).copy() if len(parts) == 1 else _cat(parts, dtype)
