@@ -4,9 +4,12 @@
 // All solves of a batch advance in lockstep over Krylov index j; a solve
 // that has converged is masked out (its active flag is 0) so the matvec and
 // the kernels below skip it.  Vectors are cut into 1024-element chunks, one
-// workgroup per chunk; per-solve reductions use the "last block reduces"
-// pattern (partials + fence + atomic ticket), so a Lanczos step is
-//   matvec (3 launches) + dots + update/dots + update/norm/tridiag + scale.
+// workgroup per chunk.  A per-solve reduction never synchronises blocks
+// inside a kernel: each kernel leaves one partial sum per chunk, and every
+// block of the next kernel on the stream (ordered after it) sums the
+// partials of its own solve in the same fixed order.  A Lanczos step is
+//   matvec (3 launches) + dots + update/dots + update/norm
+//   + normalise (block 0 of each solve: tridiagonal solve, convergence).
 // The Krylov basis and all reductions are f64 (two-pass classical
 // Gram-Schmidt against the full basis); the matvec may run in f32 or bf16.
 // The (j+1)x(j+1) tridiagonal eigenproblem is solved on the device by 65-way
@@ -27,14 +30,17 @@ struct LzState {
   double h2[NCV_MAX + 1];
   double y[NCV_MAX];
   double theta, resid, scale, spare;
-  int32_t converged, iters, jlast, counter;
-  int32_t restarts, pad[3];
+  int32_t converged, iters, jlast, restarts;
 };
 
+// partA / partB: per-chunk partial sums, [nchunks][NCV_MAX + 1].  Passes that
+// follow each other alternate between them (a kernel reads one while it
+// writes the other).
 struct LzLayout {
   LzState* st;
   int32_t* act;
-  double* part;
+  double* partA;
+  double* partB;
   int32_t* nact;
 };
 
@@ -47,7 +53,9 @@ __host__ __device__ inline LzLayout lz_layout(void* state, int nsolve, int nchun
   p += align256(sizeof(LzState) * (size_t)nsolve);
   L.act = (int32_t*)p;
   p += align256(sizeof(int32_t) * (size_t)nsolve);
-  L.part = (double*)p;
+  L.partA = (double*)p;
+  p += align256(sizeof(double) * (size_t)nchunks * (NCV_MAX + 1));
+  L.partB = (double*)p;
   p += align256(sizeof(double) * (size_t)nchunks * (NCV_MAX + 1));
   L.nact = (int32_t*)p;
   return L;
@@ -137,30 +145,13 @@ __device__ __forceinline__ void block_sums(const double* v, int nv, double (*red
   __syncthreads();
 }
 
-// Ticket: returns true in exactly one block per solve (the last to finish).
-__device__ __forceinline__ bool last_block(int32_t* counter, int nch) {
-  __shared__ int is_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int prev = atomicAdd(counter, 1);
-    is_last = (prev == nch - 1);
-  }
-  __syncthreads();
-  if (is_last) __threadfence();
-  return is_last != 0;
-}
-
-__device__ __forceinline__ double ld_part(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Reduce the partials of solve s for k in [0, nk) into dst (last block only,
-// all NT lanes).  Wave w sums the entries k = w, w + 4, ...: its 64 lanes
-// stride over the chunks and four entries are read per pass, so the loads
-// (which come from other XCDs' writes, past this XCD's L2) are in flight
-// together instead of one dependent round trip per chunk.  The summation
-// order depends only on nch.
+// Sum the partials of one solve (chunks [chunk_begin, chunk_begin + nch))
+// for k in [0, nk) into dst[k] (all NT lanes; dst is read after a barrier).
+// The partials were written by the previous kernel on the stream.  Wave w
+// sums the entries k = w, w + 4, ...: its 64 lanes stride over the chunks
+// and four entries are read per pass, so the loads are in flight together.
+// The summation order depends only on nch, so every block of a solve gets
+// the same bits.
 __device__ __forceinline__ void reduce_partials(const double* part, int chunk_begin, int nch,
                                                 int nk, double* dst) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -170,7 +161,7 @@ __device__ __forceinline__ void reduce_partials(const double* part, int chunk_be
       const double* p = part + (int64_t)(chunk_begin + c) * (NCV_MAX + 1);
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        if (k0 + 4 * u < nk) s[u] += ld_part(p + k0 + 4 * u);
+        if (k0 + 4 * u < nk) s[u] += p[k0 + 4 * u];
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -294,7 +285,23 @@ template <typename T> __device__ __forceinline__ void store_x(void* x, int dtype
 
 // ---------------------------------------------------------------- kernels
 
-// ||init||^2 -> scale; state reset; act = 1
+// Block sum of one value per thread (all NT lanes; result in every lane).
+__device__ __forceinline__ double block_total(double v, double (*red)[NCV_MAX + 1]) {
+  block_sums(&v, 1, red);
+  return (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+}
+
+// act[s], read once per block through LDS: block 0 of a solve may clear it
+// while the solve's other blocks run k_normalize, and every lane of a block
+// must take the same early return.
+__device__ __forceinline__ bool solve_active(const int32_t* act, int s) {
+  __shared__ int a;
+  if (threadIdx.x == 0) a = __hip_atomic_load(act + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  return a != 0;
+}
+
+// chunk partial of ||init||^2 -> partA[block][0]
 __global__ __launch_bounds__(NT) void k_init_norm(const tnx_lanczos_solve* __restrict__ S, int nsolve,
                                                   void* state, int nchunks) {
   LzLayout L = lz_layout(state, nsolve, nchunks);
@@ -304,37 +311,35 @@ __global__ __launch_bounds__(NT) void k_init_norm(const tnx_lanczos_solve* __res
   double acc = 0.0;
   for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) acc += v[i] * v[i];
   __shared__ double red[4][NCV_MAX + 1];
-  block_sums(&acc, 1, red);
-  if (threadIdx.x == 0) L.part[(int64_t)blockIdx.x * (NCV_MAX + 1)] = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
-  if (last_block(&L.st[ci.s].counter, ci.nch)) {
-    __shared__ double tot;
-    reduce_partials(L.part, D.chunk_begin, ci.nch, 1, &tot);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const double s = tot;
-      LzState& st = L.st[ci.s];
-      st.scale = s > 0.0 ? 1.0 / sqrt(s) : -1.0;
-      st.converged = 0;
-      st.iters = 0;
-      st.jlast = 0;
-      st.restarts = 0;
-      st.theta = 0.0;
-      st.resid = 0.0;
-      st.y[0] = 1.0;
-      for (int k = 1; k < NCV_MAX; ++k) st.y[k] = 0.0;
-      L.act[ci.s] = 1;
-      st.counter = 0;
-    }
-  }
+  const double tot = block_total(acc, red);
+  if (threadIdx.x == 0) L.partA[(int64_t)blockIdx.x * (NCV_MAX + 1)] = tot;
 }
 
-// V[0] = init * scale (or a flat vector when init == 0); x = cast(V[0])
+// scale = 1 / ||init|| (summed by every block); V[0] = init * scale (a flat
+// vector when init == 0); x = cast(V[0]).  Block 0 of the solve resets the
+// solve's state and marks it active.
 __global__ __launch_bounds__(NT) void k_init_scale(const tnx_lanczos_solve* __restrict__ S, int nsolve,
                                                    void* state, int nchunks, int x_dtype) {
   LzLayout L = lz_layout(state, nsolve, nchunks);
   ChunkInfo ci = chunk_of(S, nsolve);
   const tnx_lanczos_solve D = S[ci.s];
-  const double sc = L.st[ci.s].scale;
+  __shared__ double tot;
+  reduce_partials(L.partA, D.chunk_begin, ci.nch, 1, &tot);
+  __syncthreads();
+  const double sc = tot > 0.0 ? 1.0 / sqrt(tot) : -1.0;
+  if (ci.c == 0 && threadIdx.x == 0) {
+    LzState& st = L.st[ci.s];
+    st.scale = sc;
+    st.converged = 0;
+    st.iters = 0;
+    st.jlast = 0;
+    st.restarts = 0;
+    st.theta = 0.0;
+    st.resid = 0.0;
+    st.y[0] = 1.0;
+    for (int k = 1; k < NCV_MAX; ++k) st.y[k] = 0.0;
+    L.act[ci.s] = 1;
+  }
   const double* v = (const double*)D.init;
   double* V0 = (double*)D.V;
   const double flat = 1.0 / sqrt((double)D.n);
@@ -345,7 +350,7 @@ __global__ __launch_bounds__(NT) void k_init_scale(const tnx_lanczos_solve* __re
   }
 }
 
-// pass 1: h1[k] = V[k] . w, k <= j
+// pass 1: chunk partials of V[k] . w, k <= j -> partA
 __global__ __launch_bounds__(NT) void k_dots(const tnx_lanczos_solve* __restrict__ S, int nsolve,
                                              void* state, int nchunks, int j, int w_dtype) {
   LzLayout L = lz_layout(state, nsolve, nchunks);
@@ -363,15 +368,11 @@ __global__ __launch_bounds__(NT) void k_dots(const tnx_lanczos_solve* __restrict
   chunk_dots(V, D.ldv, j, ci.i0, ci.i1, w, red);
   __syncthreads();
   for (int k = threadIdx.x; k <= j; k += NT)
-    L.part[(int64_t)blockIdx.x * (NCV_MAX + 1) + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
-  if (last_block(&L.st[ci.s].counter, ci.nch)) {
-    reduce_partials(L.part, D.chunk_begin, ci.nch, j + 1, L.st[ci.s].h1);
-    __syncthreads();
-    if (threadIdx.x == 0) L.st[ci.s].counter = 0;
-  }
+    L.partA[(int64_t)blockIdx.x * (NCV_MAX + 1) + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
 }
 
-// r = w - sum h1[k] V[k] -> V[j+1]; pass 2: h2[k] = V[k] . r
+// h1 = sum of the pass-1 partials; r = w - sum h1[k] V[k] -> V[j+1];
+// pass 2: chunk partials of V[k] . r -> partB
 __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __restrict__ S, int nsolve,
                                                     void* state, int nchunks, int j, int w_dtype) {
   LzLayout L = lz_layout(state, nsolve, nchunks);
@@ -379,8 +380,10 @@ __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __r
   if (L.act[ci.s] == 0) return;
   const tnx_lanczos_solve D = S[ci.s];
   __shared__ double h[NCV_MAX + 1];
-  for (int k = threadIdx.x; k <= j; k += NT) h[k] = L.st[ci.s].h1[k];
+  reduce_partials(L.partA, D.chunk_begin, ci.nch, j + 1, h);
   __syncthreads();
+  if (ci.c == 0)
+    for (int k = threadIdx.x; k <= j; k += NT) L.st[ci.s].h1[k] = h[k];
   const double* V = (const double*)D.V;
   double r[CHUNK / NT];
 #pragma unroll
@@ -399,25 +402,22 @@ __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __r
   chunk_dots(V, D.ldv, j, ci.i0, ci.i1, r, red);
   __syncthreads();
   for (int k = threadIdx.x; k <= j; k += NT)
-    L.part[(int64_t)blockIdx.x * (NCV_MAX + 1) + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
-  if (last_block(&L.st[ci.s].counter, ci.nch)) {
-    reduce_partials(L.part, D.chunk_begin, ci.nch, j + 1, L.st[ci.s].h2);
-    __syncthreads();
-    if (threadIdx.x == 0) L.st[ci.s].counter = 0;
-  }
+    L.partB[(int64_t)blockIdx.x * (NCV_MAX + 1) + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
 }
 
-// r -= sum h2[k] V[k]; beta = ||r||; alpha_j = h1[j] + h2[j]; tridiagonal
-// eigenpair; convergence test (last block)
+// h2 = sum of the pass-2 partials; r -= sum h2[k] V[k] -> V[j+1]; chunk
+// partial of ||r||^2 -> partA[block][0]
 __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __restrict__ S, int nsolve,
-                                                    void* state, int nchunks, int j, int min_iters) {
+                                                    void* state, int nchunks, int j) {
   LzLayout L = lz_layout(state, nsolve, nchunks);
   ChunkInfo ci = chunk_of(S, nsolve);
   if (L.act[ci.s] == 0) return;
   const tnx_lanczos_solve D = S[ci.s];
   __shared__ double h[NCV_MAX + 1];
-  for (int k = threadIdx.x; k <= j; k += NT) h[k] = L.st[ci.s].h2[k];
+  reduce_partials(L.partB, D.chunk_begin, ci.nch, j + 1, h);
   __syncthreads();
+  if (ci.c == 0)
+    for (int k = threadIdx.x; k <= j; k += NT) L.st[ci.s].h2[k] = h[k];
   const double* V = (const double*)D.V;
   double* Vn = (double*)D.V + (int64_t)(j + 1) * D.ldv;
   double r[CHUNK / NT];
@@ -437,17 +437,41 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
     }
   }
   __shared__ double red[4][NCV_MAX + 1];
-  block_sums(&nn, 1, red);
-  if (threadIdx.x == 0)
-    L.part[(int64_t)blockIdx.x * (NCV_MAX + 1)] = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
-  if (last_block(&L.st[ci.s].counter, ci.nch)) {
+  const double tot = block_total(nn, red);
+  if (threadIdx.x == 0) L.partA[(int64_t)blockIdx.x * (NCV_MAX + 1)] = tot;
+}
+
+// beta_j = sqrt(sum of the ||r||^2 partials); V[j+1] /= beta_j and
+// x = cast(V[j+1]) when another Krylov vector follows.  Block 0 of each
+// solve also records alpha_j = h1[j] + h2[j] and beta_j, solves the
+// tridiagonal eigenproblem and tests convergence (clearing act[s]; the
+// solve's other blocks may still normalise V[j+1], which is then unused).
+__global__ __launch_bounds__(NT) void k_normalize(const tnx_lanczos_solve* __restrict__ S, int nsolve,
+                                                  void* state, int nchunks, int j, int x_dtype,
+                                                  int normalize, int min_iters) {
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  ChunkInfo ci = chunk_of(S, nsolve);
+  if (!solve_active(L.act, ci.s)) return;
+  const tnx_lanczos_solve D = S[ci.s];
+  __shared__ double nn_s;
+  reduce_partials(L.partA, D.chunk_begin, ci.nch, 1, &nn_s);
+  __syncthreads();
+  const double beta_j = sqrt(nn_s);
+  if (normalize) {
+    const double inv = 1.0 / beta_j;
+    double* Vn = (double*)D.V + (int64_t)(j + 1) * D.ldv;
+    for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
+      double v = Vn[i] * inv;
+      Vn[i] = v;
+      tnx_store((void*)D.x, x_dtype, i, v);
+    }
+  }
+  if (ci.c == 0) {
     LzState& st = L.st[ci.s];
-    __shared__ double a_s[NCV_MAX], b_s[NCV_MAX], y_s[NCV_MAX], th_s, nn_s;
+    __shared__ double a_s[NCV_MAX], b_s[NCV_MAX], y_s[NCV_MAX], th_s;
     __shared__ double work[5 * NCV_MAX];
-    reduce_partials(L.part, D.chunk_begin, ci.nch, 1, &nn_s);
-    __syncthreads();
     if (threadIdx.x == 0) {
-      const double beta = sqrt(nn_s);
+      const double beta = beta_j;
       st.alpha[j] = st.h1[j] + st.h2[j];
       st.beta[j] = beta;
       st.iters += 1;
@@ -474,31 +498,15 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
       const bool full = (j + 1 >= D.n);
       if (full || breakdown || ((j + 1 >= min_iters) && res <= thr)) {
         st.converged = 1;
-        L.act[ci.s] = 0;
+        __hip_atomic_store(L.act + ci.s, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         atomicSub(L.nact, 1);
       }
-      st.counter = 0;
     }
   }
 }
 
-// V[j+1] /= beta_j; x = cast(V[j+1])
-__global__ __launch_bounds__(NT) void k_normalize(const tnx_lanczos_solve* __restrict__ S, int nsolve,
-                                                  void* state, int nchunks, int j, int x_dtype) {
-  LzLayout L = lz_layout(state, nsolve, nchunks);
-  ChunkInfo ci = chunk_of(S, nsolve);
-  if (L.act[ci.s] == 0) return;
-  const tnx_lanczos_solve D = S[ci.s];
-  const double inv = 1.0 / L.st[ci.s].beta[j];
-  double* Vn = (double*)D.V + (int64_t)(j + 1) * D.ldv;
-  for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
-    double v = Vn[i] * inv;
-    Vn[i] = v;
-    tnx_store((void*)D.x, x_dtype, i, v);
-  }
-}
-
-// Ritz vector of the active solves into the spare row V[m]; its norm
+// Ritz vector of the active solves into the spare row V[m]; chunk partial of
+// its squared norm -> partA[block][0]
 __global__ __launch_bounds__(NT) void k_ritz(const tnx_lanczos_solve* __restrict__ S, int nsolve,
                                              void* state, int nchunks, int m_rows) {
   LzLayout L = lz_layout(state, nsolve, nchunks);
@@ -517,30 +525,25 @@ __global__ __launch_bounds__(NT) void k_ritz(const tnx_lanczos_solve* __restrict
     nn += s * s;
   }
   __shared__ double red[4][NCV_MAX + 1];
-  block_sums(&nn, 1, red);
-  if (threadIdx.x == 0)
-    L.part[(int64_t)blockIdx.x * (NCV_MAX + 1)] = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
-  if (last_block(&L.st[ci.s].counter, ci.nch)) {
-    __shared__ double tot;
-    reduce_partials(L.part, D.chunk_begin, ci.nch, 1, &tot);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const double s = tot;
-      L.st[ci.s].scale = 1.0 / sqrt(s);
-      L.st[ci.s].restarts += 1;
-      L.st[ci.s].counter = 0;
-    }
-  }
+  const double tot = block_total(nn, red);
+  if (threadIdx.x == 0) L.partA[(int64_t)blockIdx.x * (NCV_MAX + 1)] = tot;
 }
 
-// restart: V[0] = V[m] * scale; x = cast(V[0])
+// restart: V[0] = V[m] / ||V[m]|| (summed by every block); x = cast(V[0])
 __global__ __launch_bounds__(NT) void k_restart(const tnx_lanczos_solve* __restrict__ S, int nsolve,
                                                 void* state, int nchunks, int m_rows, int x_dtype) {
   LzLayout L = lz_layout(state, nsolve, nchunks);
   ChunkInfo ci = chunk_of(S, nsolve);
   if (L.act[ci.s] == 0) return;
   const tnx_lanczos_solve D = S[ci.s];
-  const double sc = L.st[ci.s].scale;
+  __shared__ double tot;
+  reduce_partials(L.partA, D.chunk_begin, ci.nch, 1, &tot);
+  __syncthreads();
+  const double sc = 1.0 / sqrt(tot);
+  if (ci.c == 0 && threadIdx.x == 0) {
+    L.st[ci.s].scale = sc;
+    L.st[ci.s].restarts += 1;
+  }
   const double* X = (const double*)D.V + (int64_t)m_rows * D.ldv;
   double* V0 = (double*)D.V;
   for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
@@ -610,7 +613,7 @@ int ensure_pinned(int nsolve) {
 extern "C" size_t tnx_lanczos_state_bytes(int nsolve, int nchunks, int ncv) {
   (void)ncv;
   return align256(sizeof(LzState) * (size_t)nsolve) + align256(sizeof(int32_t) * (size_t)nsolve) +
-         align256(sizeof(double) * (size_t)nchunks * (NCV_MAX + 1)) + 256;
+         2 * align256(sizeof(double) * (size_t)nchunks * (NCV_MAX + 1)) + 256;
 }
 
 extern "C" int tnx_lanczos_run_ex(const tnx_matvec_plan* plan, const tnx_lanczos_solve* solves,
@@ -651,10 +654,9 @@ extern "C" int tnx_lanczos_run_ex(const tnx_matvec_plan* plan, const tnx_lanczos
       if ((rc = tnx_heff_matvec(plan, L.act, stream))) break;
       hipLaunchKernelGGL(k_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
       hipLaunchKernelGGL(k_update_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
-      hipLaunchKernelGGL(k_update_norm, grid, block, 0, s, solves, nsolve, state, nchunks, j,
-                         min_iters);
-      if (j + 1 < ncv)
-        hipLaunchKernelGGL(k_normalize, grid, block, 0, s, solves, nsolve, state, nchunks, j, x_dtype);
+      hipLaunchKernelGGL(k_update_norm, grid, block, 0, s, solves, nsolve, state, nchunks, j);
+      hipLaunchKernelGGL(k_normalize, grid, block, 0, s, solves, nsolve, state, nchunks, j, x_dtype,
+                         (int)(j + 1 < ncv), min_iters);
       TNX_CHECK_LAUNCH();
       // lagged poll: copy the active count of this step, then wait for the
       // previous step's copy (the device keeps one step of work queued)
