@@ -283,6 +283,77 @@ constexpr int CHOL_NMAX = 128;
 constexpr int CHOL_LD = CHOL_NMAX + 1;  // LDS row stride of R: column reads hit distinct banks
 constexpr size_t CHOL_LDS = sizeof(double) * (CHOL_NMAX * CHOL_LD + 2 * CHOL_NMAX);
 
+// Cholesky steps k in [16 KA, 16 KA + 16) (k < n).  With the tile-row block
+// KA of column k a compile-time constant, every register index is static,
+// and the blocks a < KA / b < KA (rows and columns already factored, whose
+// update factors are zero) drop out of the rank-1 update at compile time.
+template <int KA>
+__device__ __forceinline__ void chol_steps(double (&s)[8][8], double* rowbuf, int n, int tr, int tc,
+                                           int& bad) {
+  const int kend = min(16 * KA + 16, n);
+  for (int k = 16 * KA; k < kend; ++k) {
+    double* rb = rowbuf + (k & 1) * CHOL_NMAX;
+    if (tr == (k & 15)) {
+#pragma unroll
+      for (int b = 0; b < 8; ++b) rb[tc + 16 * b] = s[KA][b];
+    }
+    __syncthreads();
+    double d = rb[k];
+    if (!(d > 0.0)) {
+      d = 1e-300;
+      bad = 1;
+    }
+    const double sq = sqrt(d), ri = 1.0 / sq;
+    double ci[8], cj[8];
+#pragma unroll
+    for (int a = KA; a < 8; ++a) ci[a] = (a > KA || tr + 16 * a > k) ? rb[tr + 16 * a] * ri : 0.0;
+#pragma unroll
+    for (int b = KA; b < 8; ++b) cj[b] = (b > KA || tc + 16 * b > k) ? rb[tc + 16 * b] * ri : 0.0;
+#pragma unroll
+    for (int a = KA; a < 8; ++a)
+#pragma unroll
+      for (int b = KA; b < 8; ++b) s[a][b] = fma(-ci[a], cj[b], s[a][b]);
+    if (tr == (k & 15)) {
+#pragma unroll
+      for (int b = KA; b < 8; ++b) {
+        const int j = tc + 16 * b;
+        s[KA][b] = (j > k) ? s[KA][b] * ri : (j == k ? sq : s[KA][b]);
+      }
+    }
+  }
+}
+
+// Inverse steps k in [16 KA, 16 KA + 16) from the top down (k < n): rows of
+// blocks a > KA lie below k (no update), columns of blocks b < KA lie left of
+// k, where row k of X = R^{-1} is zero.
+template <int KA>
+__device__ __forceinline__ void inv_steps(double (&s)[8][8], const double* Rl, double* rowbuf, int n,
+                                          int tr, int tc) {
+  const int khi = min(16 * KA + 15, n - 1);
+  for (int k = khi; k >= 16 * KA; --k) {
+    double* xb = rowbuf + (k & 1) * CHOL_NMAX;
+    if (tr == (k & 15)) {
+      const double rkk = 1.0 / Rl[k * CHOL_LD + k];
+#pragma unroll
+      for (int b = KA; b < 8; ++b) {
+        s[KA][b] *= rkk;
+        xb[tc + 16 * b] = s[KA][b];
+      }
+    }
+    __syncthreads();
+    double ra[8], xv[8];
+#pragma unroll
+    for (int a = 0; a <= KA; ++a)
+      ra[a] = (a < KA || tr + 16 * a < k) ? Rl[(tr + 16 * a) * CHOL_LD + k] : 0.0;
+#pragma unroll
+    for (int b = KA; b < 8; ++b) xv[b] = xb[tc + 16 * b];
+#pragma unroll
+    for (int a = 0; a <= KA; ++a)
+#pragma unroll
+      for (int b = KA; b < 8; ++b) s[a][b] = fma(-ra[a], xv[b], s[a][b]);
+  }
+}
+
 __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __restrict__ jobs) {
   extern __shared__ double lds[];
   double* Rl = lds;                         // R, row stride CHOL_LD
@@ -325,50 +396,18 @@ __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __rest
       for (int b = 0; b < 8; ++b)
         if (tr + 16 * a == tc + 16 * b && tr + 16 * a < n) s[a][b] += sh;
   }
-  const double tiny = 1e-300;
   int bad = 0;
   // Cholesky: row k of the tiles becomes row k of R at step k.  The whole
   // symmetric trailing block is updated (both triangles), so row k is
   // current when its owners broadcast it.
-  for (int k = 0; k < n; ++k) {
-    const int ka = k >> 4;
-    double* rb = rowbuf + (k & 1) * CHOL_NMAX;
-    if (tr == (k & 15)) {
-#pragma unroll
-      for (int a = 0; a < 8; ++a)
-        if (a == ka) {
-#pragma unroll
-          for (int b = 0; b < 8; ++b) rb[tc + 16 * b] = s[a][b];
-        }
-    }
-    __syncthreads();
-    double d = rb[k];
-    if (!(d > 0.0)) {
-      d = tiny;
-      bad = 1;
-    }
-    const double sq = sqrt(d), ri = 1.0 / sq;
-    double ci[8], cj[8];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) ci[a] = (tr + 16 * a > k) ? rb[tr + 16 * a] * ri : 0.0;
-#pragma unroll
-    for (int b = 0; b < 8; ++b) cj[b] = (tc + 16 * b > k) ? rb[tc + 16 * b] * ri : 0.0;
-#pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-      for (int b = 0; b < 8; ++b) s[a][b] = fma(-ci[a], cj[b], s[a][b]);
-    if (tr == (k & 15)) {
-#pragma unroll
-      for (int a = 0; a < 8; ++a)
-        if (a == ka) {
-#pragma unroll
-          for (int b = 0; b < 8; ++b) {
-            const int j = tc + 16 * b;
-            s[a][b] = (j > k) ? s[a][b] * ri : (j == k ? sq : s[a][b]);
-          }
-        }
-    }
-  }
+  chol_steps<0>(s, rowbuf, n, tr, tc, bad);
+  chol_steps<1>(s, rowbuf, n, tr, tc, bad);
+  chol_steps<2>(s, rowbuf, n, tr, tc, bad);
+  chol_steps<3>(s, rowbuf, n, tr, tc, bad);
+  chol_steps<4>(s, rowbuf, n, tr, tc, bad);
+  chol_steps<5>(s, rowbuf, n, tr, tc, bad);
+  chol_steps<6>(s, rowbuf, n, tr, tc, bad);
+  chol_steps<7>(s, rowbuf, n, tr, tc, bad);
   double* R = (double*)J.r;
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
@@ -388,32 +427,14 @@ __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __rest
 #pragma unroll
     for (int b = 0; b < 8; ++b)
       s[a][b] = (tr + 16 * a == tc + 16 * b && tr + 16 * a < n) ? 1.0 : 0.0;
-  for (int k = n - 1; k >= 0; --k) {
-    const int ka = k >> 4;
-    double* xb = rowbuf + (k & 1) * CHOL_NMAX;
-    if (tr == (k & 15)) {
-      const double rkk = 1.0 / Rl[k * CHOL_LD + k];
-#pragma unroll
-      for (int a = 0; a < 8; ++a)
-        if (a == ka) {
-#pragma unroll
-          for (int b = 0; b < 8; ++b) {
-            s[a][b] *= rkk;
-            xb[tc + 16 * b] = s[a][b];
-          }
-        }
-    }
-    __syncthreads();
-    double ra[8], xv[8];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) ra[a] = (tr + 16 * a < k) ? Rl[(tr + 16 * a) * CHOL_LD + k] : 0.0;
-#pragma unroll
-    for (int b = 0; b < 8; ++b) xv[b] = xb[tc + 16 * b];
-#pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-      for (int b = 0; b < 8; ++b) s[a][b] = fma(-ra[a], xv[b], s[a][b]);
-  }
+  inv_steps<7>(s, Rl, rowbuf, n, tr, tc);
+  inv_steps<6>(s, Rl, rowbuf, n, tr, tc);
+  inv_steps<5>(s, Rl, rowbuf, n, tr, tc);
+  inv_steps<4>(s, Rl, rowbuf, n, tr, tc);
+  inv_steps<3>(s, Rl, rowbuf, n, tr, tc);
+  inv_steps<2>(s, Rl, rowbuf, n, tr, tc);
+  inv_steps<1>(s, Rl, rowbuf, n, tr, tc);
+  inv_steps<0>(s, Rl, rowbuf, n, tr, tc);
   double* Ri = (double*)J.rinv;
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
