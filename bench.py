@@ -64,13 +64,13 @@ def run_batch(para, seeds, precision, ncv=20):
     """One batched solve: every chain of ``seeds`` from its seeded random MPS
     through all sweeps in lockstep, then its energy <H>."""
     import torch
-    from tnalg_amd.dmrg import observe_energy
+    from tnalg_amd.dmrg import observe_energies
     eng = engine_for(precision, ncv)
     for k in eng.counters:
         eng.counters[k] = 0.0 if isinstance(eng.counters[k], float) else 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    solves = [eng.add_solve(para, seed=s) for s in seeds]
+    solves = eng.add_solves([(para, s) for s in seeds])
     for s in solves:
         s.schedule_canonicalise()
     eng.run_queues(solves)
@@ -78,10 +78,8 @@ def run_batch(para, seeds, precision, ncv=20):
         for s in solves:
             s.schedule_sweep(t)
         eng.run_queues(solves)
-    energies = []
-    for s in solves:
-        ob = observe_energy(eng, s, para)
-        energies.append(float(np.ravel(ob['e_per_site'])[0]) * s.n)
+    energies = [float(np.ravel(ob['e_per_site'])[0]) * s.n
+                for s, ob in zip(solves, observe_energies(eng, solves, [para] * len(solves)))]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     stats = dict(eng.counters)

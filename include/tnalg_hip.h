@@ -78,12 +78,13 @@ typedef struct tnx_gemm_problem {
  * identity.                                                                  */
 typedef struct tnx_mix_job {
   uint64_t dst;
-  int32_t Dl, d, Dr;
+  int32_t Dl, d, Dr;   /* d <= 4 */
   int32_t dst_dtype;   /* tnx_dtype */
   int32_t term_begin, term_count;
   int32_t accumulate;  /* 1: add into dst */
   int32_t active;      /* index into the active-flag array, or -1 */
-  int32_t block_begin; /* prefix sum of 1024-element blocks over jobs */
+  int32_t block_begin; /* prefix sum over jobs of ceil(Dl * Dr / 512) workgroups
+                          (a workgroup covers 512 (a, b) pairs, every s) */
   int32_t reserved;
 } tnx_mix_job; /* 48 bytes */
 

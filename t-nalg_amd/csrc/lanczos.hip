@@ -8,12 +8,13 @@
 // inside a kernel: each kernel leaves one partial sum per chunk, and every
 // block of the next kernel on the stream (ordered after it) sums the
 // partials of its own solve in the same fixed order.  A Lanczos step is
-//   matvec (3 launches) + dots + update/dots + update/norm
-//   + normalise (block 0 of each solve: tridiagonal solve, convergence).
+//   matvec (3 launches) + dots + update/dots/norm
+//   + update/normalise (block 0 of each solve: tridiagonal solve,
+//     convergence).
 // The Krylov basis and all reductions are f64 (two-pass classical
 // Gram-Schmidt against the full basis); the matvec may run in f32 or bf16.
-// The (j+1)x(j+1) tridiagonal eigenproblem is solved on the device by 65-way
-// multisection of the Sturm count (one wavefront) followed by inverse
+// The (j+1)x(j+1) tridiagonal eigenproblem is solved on the device by 257-way
+// multisection of the Sturm count (one workgroup) followed by inverse
 // iteration with partial pivoting.
 #include "tnx_common.h"
 
@@ -292,7 +293,7 @@ __device__ __forceinline__ double block_total(double v, double (*red)[NCV_MAX + 
 }
 
 // act[s], read once per block through LDS: block 0 of a solve may clear it
-// while the solve's other blocks run k_normalize, and every lane of a block
+// while the solve's other blocks run k_update_norm, and every lane of a block
 // must take the same early return.
 __device__ __forceinline__ bool solve_active(const int32_t* act, int s) {
   __shared__ int a;
@@ -372,7 +373,7 @@ __global__ __launch_bounds__(NT) void k_dots(const tnx_lanczos_solve* __restrict
 }
 
 // h1 = sum of the pass-1 partials; r = w - sum h1[k] V[k] -> V[j+1];
-// pass 2: chunk partials of V[k] . r -> partB
+// pass 2: chunk partials of V[k] . r (k <= j) and of ||r||^2 (slot j+1) -> partB
 __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __restrict__ S, int nsolve,
                                                     void* state, int nchunks, int j, int w_dtype) {
   LzLayout L = lz_layout(state, nsolve, nchunks);
@@ -400,70 +401,61 @@ __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __r
   }
   __shared__ double red[4][NCV_MAX + 1];
   chunk_dots(V, D.ldv, j, ci.i0, ci.i1, r, red);
+  double nn = 0.0;  // ||r||^2 (r is zero past the chunk end)
+#pragma unroll
+  for (int q = 0; q < CHUNK / NT; ++q) nn += r[q] * r[q];
+  wave_to_lds(nn, j + 1, red);
   __syncthreads();
-  for (int k = threadIdx.x; k <= j; k += NT)
+  for (int k = threadIdx.x; k <= j + 1; k += NT)
     L.partB[(int64_t)blockIdx.x * (NCV_MAX + 1) + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
 }
 
-// h2 = sum of the pass-2 partials; r -= sum h2[k] V[k] -> V[j+1]; chunk
-// partial of ||r||^2 -> partA[block][0]
+// h2 = sum of the pass-2 partials (slots 0..j), ||r||^2 = slot j+1;
+// beta_j = sqrt(||r||^2 - ||h2||^2), the norm of r - V h2 (V^T r = h2 and
+// V is orthonormal; ||h2|| is at rounding level next to ||r|| except at a
+// Lanczos breakdown, which the test below catches); V[j+1] =
+// (r - V h2) / beta_j and x = cast(V[j+1]) when another Krylov vector
+// follows.  Block 0 of each solve also records alpha_j = h1[j] + h2[j] and
+// beta_j, solves the tridiagonal eigenproblem and tests convergence
+// (clearing act[s]; the solve's other blocks may still write V[j+1], which
+// is then unused).
 __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __restrict__ S, int nsolve,
-                                                    void* state, int nchunks, int j) {
-  LzLayout L = lz_layout(state, nsolve, nchunks);
-  ChunkInfo ci = chunk_of(S, nsolve);
-  if (L.act[ci.s] == 0) return;
-  const tnx_lanczos_solve D = S[ci.s];
-  __shared__ double h[NCV_MAX + 1];
-  reduce_partials(L.partB, D.chunk_begin, ci.nch, j + 1, h);
-  __syncthreads();
-  if (ci.c == 0)
-    for (int k = threadIdx.x; k <= j; k += NT) L.st[ci.s].h2[k] = h[k];
-  const double* V = (const double*)D.V;
-  double* Vn = (double*)D.V + (int64_t)(j + 1) * D.ldv;
-  double r[CHUNK / NT];
-#pragma unroll
-  for (int q = 0; q < CHUNK / NT; ++q) {
-    int i = ci.i0 + q * NT + threadIdx.x;
-    r[q] = (i < ci.i1) ? Vn[i] : 0.0;
-  }
-  chunk_axpy(V, D.ldv, j, ci.i0, ci.i1, h, r);
-  double nn = 0.0;
-#pragma unroll
-  for (int q = 0; q < CHUNK / NT; ++q) {
-    int i = ci.i0 + q * NT + threadIdx.x;
-    if (i < ci.i1) {
-      Vn[i] = r[q];
-      nn += r[q] * r[q];
-    }
-  }
-  __shared__ double red[4][NCV_MAX + 1];
-  const double tot = block_total(nn, red);
-  if (threadIdx.x == 0) L.partA[(int64_t)blockIdx.x * (NCV_MAX + 1)] = tot;
-}
-
-// beta_j = sqrt(sum of the ||r||^2 partials); V[j+1] /= beta_j and
-// x = cast(V[j+1]) when another Krylov vector follows.  Block 0 of each
-// solve also records alpha_j = h1[j] + h2[j] and beta_j, solves the
-// tridiagonal eigenproblem and tests convergence (clearing act[s]; the
-// solve's other blocks may still normalise V[j+1], which is then unused).
-__global__ __launch_bounds__(NT) void k_normalize(const tnx_lanczos_solve* __restrict__ S, int nsolve,
-                                                  void* state, int nchunks, int j, int x_dtype,
-                                                  int normalize, int min_iters) {
+                                                    void* state, int nchunks, int j, int x_dtype,
+                                                    int normalize, int min_iters) {
   LzLayout L = lz_layout(state, nsolve, nchunks);
   ChunkInfo ci = chunk_of(S, nsolve);
   if (!solve_active(L.act, ci.s)) return;
   const tnx_lanczos_solve D = S[ci.s];
-  __shared__ double nn_s;
-  reduce_partials(L.partA, D.chunk_begin, ci.nch, 1, &nn_s);
+  __shared__ double h[NCV_MAX + 1];
+  __shared__ double beta_s;
+  reduce_partials(L.partB, D.chunk_begin, ci.nch, j + 2, h);
   __syncthreads();
-  const double beta_j = sqrt(nn_s);
+  if (threadIdx.x == 0) {
+    double hh = 0.0;
+    for (int k = 0; k <= j; ++k) hh += h[k] * h[k];
+    beta_s = sqrt(fmax(h[j + 1] - hh, 0.0));
+  }
+  __syncthreads();
+  const double beta_j = beta_s;
   if (normalize) {
-    const double inv = 1.0 / beta_j;
+    const double* V = (const double*)D.V;
     double* Vn = (double*)D.V + (int64_t)(j + 1) * D.ldv;
-    for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
-      double v = Vn[i] * inv;
-      Vn[i] = v;
-      tnx_store((void*)D.x, x_dtype, i, v);
+    double r[CHUNK / NT];
+#pragma unroll
+    for (int q = 0; q < CHUNK / NT; ++q) {
+      int i = ci.i0 + q * NT + threadIdx.x;
+      r[q] = (i < ci.i1) ? Vn[i] : 0.0;
+    }
+    chunk_axpy(V, D.ldv, j, ci.i0, ci.i1, h, r);
+    const double inv = 1.0 / beta_j;
+#pragma unroll
+    for (int q = 0; q < CHUNK / NT; ++q) {
+      int i = ci.i0 + q * NT + threadIdx.x;
+      if (i < ci.i1) {
+        const double v = r[q] * inv;
+        Vn[i] = v;
+        tnx_store((void*)D.x, x_dtype, i, v);
+      }
     }
   }
   if (ci.c == 0) {
@@ -472,7 +464,8 @@ __global__ __launch_bounds__(NT) void k_normalize(const tnx_lanczos_solve* __res
     __shared__ double work[5 * NCV_MAX];
     if (threadIdx.x == 0) {
       const double beta = beta_j;
-      st.alpha[j] = st.h1[j] + st.h2[j];
+      for (int k = 0; k <= j; ++k) st.h2[k] = h[k];
+      st.alpha[j] = st.h1[j] + h[j];
       st.beta[j] = beta;
       st.iters += 1;
       for (int k = 0; k <= j; ++k) {
@@ -654,9 +647,8 @@ extern "C" int tnx_lanczos_run_ex(const tnx_matvec_plan* plan, const tnx_lanczos
       if ((rc = tnx_heff_matvec(plan, L.act, stream))) break;
       hipLaunchKernelGGL(k_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
       hipLaunchKernelGGL(k_update_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
-      hipLaunchKernelGGL(k_update_norm, grid, block, 0, s, solves, nsolve, state, nchunks, j);
-      hipLaunchKernelGGL(k_normalize, grid, block, 0, s, solves, nsolve, state, nchunks, j, x_dtype,
-                         (int)(j + 1 < ncv), min_iters);
+      hipLaunchKernelGGL(k_update_norm, grid, block, 0, s, solves, nsolve, state, nchunks, j,
+                         x_dtype, (int)(j + 1 < ncv), min_iters);
       TNX_CHECK_LAUNCH();
       // lagged poll: copy the active count of this step, then wait for the
       // previous step's copy (the device keeps one step of work queued)

@@ -20,8 +20,13 @@ void tnx_set_error(const char* fmt, ...) {
 
 namespace {
 
-constexpr int MIX_BLOCK = 1024;  // elements per workgroup (256 lanes x 4)
+constexpr int MIX_PAIRS = 512;  // (a, b) index pairs per workgroup (256 lanes x 2)
+constexpr int MIX_DMAX = 4;     // largest physical dimension (the operator table holds 4 x 4)
 
+// One lane per (a, b): the d source values src[a, :, b] of a term are
+// loaded once and mixed by the d x d operator in registers, so each output
+// element costs one load per term and the index split is one 32-bit
+// division per pair.
 __global__ __launch_bounds__(256) void mix_kernel(const tnx_mix_job* __restrict__ jobs, int njob,
                                                   const tnx_mix_term* __restrict__ terms,
                                                   const double* __restrict__ ops,
@@ -31,33 +36,47 @@ __global__ __launch_bounds__(256) void mix_kernel(const tnx_mix_job* __restrict_
                                 njob, bid);
   const tnx_mix_job J = jobs[ji];
   if (J.active >= 0 && active != nullptr && active[J.active] == 0) return;
-  const int64_t total = (int64_t)J.Dl * J.d * J.Dr;
-  const int64_t dr = J.Dr, ddr = (int64_t)J.d * J.Dr;
-  const int64_t base = (int64_t)(bid - J.block_begin) * MIX_BLOCK;
-  for (int q = 0; q < 4; ++q) {
-    const int64_t e = base + q * 256 + threadIdx.x;
-    if (e >= total) break;
-    const int64_t a = e / ddr;
-    const int64_t rem = e - a * ddr;
-    const int s = (int)(rem / dr);
-    const int64_t b = rem - (int64_t)s * dr;
-    double acc = 0.0;
+  const int d = J.d, Dr = J.Dr;
+  const int npairs = J.Dl * Dr;
+  const int base = (bid - J.block_begin) * MIX_PAIRS;
+  for (int q = 0; q < MIX_PAIRS / 256; ++q) {
+    const int p = base + q * 256 + threadIdx.x;
+    if (p >= npairs) break;
+    const int a = p / Dr;
+    const int64_t e0 = (int64_t)a * d * Dr + (p - a * Dr);  // element (a, 0, b)
+    double acc[MIX_DMAX] = {0.0, 0.0, 0.0, 0.0};
     for (int ti = J.term_begin; ti < J.term_begin + J.term_count; ++ti) {
       const tnx_mix_term T = terms[ti];
+      double x[MIX_DMAX];
+#pragma unroll
+      for (int t = 0; t < MIX_DMAX; ++t)
+        x[t] = (t < d) ? tnx_load((const void*)T.src, T.src_dtype, e0 + (int64_t)t * Dr) : 0.0;
       if (T.op < 0) {
-        acc += T.coef * tnx_load((const void*)T.src, T.src_dtype, e);
+#pragma unroll
+        for (int s = 0; s < MIX_DMAX; ++s) acc[s] += T.coef * x[s];
       } else {
-        const double* O = ops + (int64_t)T.op * J.d * J.d + (int64_t)s * J.d;
-        double v = 0.0;
-        for (int t = 0; t < J.d; ++t) {
-          const double o = O[t];
-          if (o != 0.0) v += o * tnx_load((const void*)T.src, T.src_dtype, a * ddr + t * dr + b);
+        const double* O = ops + (int64_t)T.op * d * d;
+#pragma unroll
+        for (int s = 0; s < MIX_DMAX; ++s) {
+          if (s < d) {
+            double v = 0.0;
+#pragma unroll
+            for (int t = 0; t < MIX_DMAX; ++t)
+              if (t < d) v += O[s * d + t] * x[t];
+            acc[s] += T.coef * v;
+          }
         }
-        acc += T.coef * v;
       }
     }
-    if (J.accumulate) acc += tnx_load((const void*)J.dst, J.dst_dtype, e);
-    tnx_store((void*)J.dst, J.dst_dtype, e, acc);
+#pragma unroll
+    for (int s = 0; s < MIX_DMAX; ++s) {
+      if (s < d) {
+        const int64_t e = e0 + (int64_t)s * Dr;
+        double v = acc[s];
+        if (J.accumulate) v += tnx_load((const void*)J.dst, J.dst_dtype, e);
+        tnx_store((void*)J.dst, J.dst_dtype, e, v);
+      }
+    }
   }
 }
 

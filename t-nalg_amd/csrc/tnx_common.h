@@ -68,15 +68,25 @@ __device__ __forceinline__ double tnx_wave_sum(double v) {
 
 // Largest index i in [0, n) whose 32-bit field at byte offset `off` of the
 // i-th record (records of `rec` bytes) is <= key.  The fields are prefix
-// sums (non-decreasing, first one 0), so i is the record owning block `key`.
+// sums (non-decreasing, first one 0, key >= 0), so i is the record owning
+// block `key`.  Called by every lane of a full wavefront with uniform
+// arguments: each round the 64 lanes probe 64 evenly spaced records of the
+// remaining range [lo, hi] (lane 0 probes lo, known to qualify) and the range
+// shrinks to the gap after the last qualifying probe, so up to 4096 records
+// cost two dependent loads instead of twelve.
 __device__ __forceinline__ int tnx_find_owner(const void* base, int rec, int off, int n,
                                               int key) {
   const char* b = (const char*)base + off;
+  const int lane = threadIdx.x & 63;
   int lo = 0, hi = n - 1;
   while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (*(const int32_t*)(b + (int64_t)mid * rec) <= key) lo = mid;
-    else hi = mid - 1;
+    const int step = (hi - lo + 64) >> 6;  // ceil((hi - lo + 1) / 64)
+    const int idx = lo + lane * step;
+    bool ok = false;
+    if (idx <= hi) ok = *(const int32_t*)(b + (int64_t)idx * rec) <= key;
+    const unsigned long long m = __ballot(ok);
+    lo += (63 - __clzll((long long)m)) * step;
+    hi = min(hi, lo + step - 1);
   }
   return lo;
 }

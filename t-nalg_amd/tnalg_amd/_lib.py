@@ -46,7 +46,8 @@ CHOL_JOB = np.dtype([('g', '<u8'), ('r', '<u8'), ('rinv', '<u8'), ('status', '<u
 CHOL_NMAX = 128
 
 GEMM_TILE = 64
-MIX_BLOCK = 1024
+MIX_PAIRS = 512    # (a, b) pairs per mixer workgroup
+MIX_DMAX = 4       # largest physical dimension of the mixer
 LZ_CHUNK = 1024
 LZ_NCV_MAX = 32
 QR_KMAX = 4096

@@ -204,18 +204,28 @@ def correlator_pairs(length, ob_len=None):
 
 def observe_energy(engine, solve, para):
     """``eb_full, mx, mz, e_per_site`` as in ``DMRG_anyH.py:66-70``."""
+    return observe_energies(engine, [solve], [para])[0]
+
+
+def observe_energies(engine, solves, paras):
+    """``observe_energy`` of several solves in one batched measurement."""
     from .observe import measure
-    idx2 = np.asarray(para['index2'])
-    n = solve.n
-    one = [(i, 1) for i in range(n)] + [(i, 3) for i in range(n)]
-    (vone, vtwo, norm), = measure(engine, [solve], [(one, [tuple(r) for r in idx2])])
-    ob = {}
-    ob['eb_full'] = (np.asarray(para['coeff2'], dtype=float).reshape(-1) * vtwo).reshape(-1, 1)
-    ob['mx'] = vone[:n].reshape(-1, 1)
-    ob['mz'] = vone[n:].reshape(-1, 1)
-    ob['e_per_site'] = (sum(ob['eb_full']) - para['hx'] * sum(ob['mx'])
-                        - para['hz'] * sum(ob['mz'])) / n
-    return ob
+    reqs = []
+    for s, para in zip(solves, paras):
+        idx2 = np.asarray(para['index2'])
+        one = [(i, 1) for i in range(s.n)] + [(i, 3) for i in range(s.n)]
+        reqs.append((one, [tuple(r) for r in idx2]))
+    out = []
+    for s, para, (vone, vtwo, _) in zip(solves, paras, measure(engine, solves, reqs)):
+        n = s.n
+        ob = {}
+        ob['eb_full'] = (np.asarray(para['coeff2'], dtype=float).reshape(-1) * vtwo).reshape(-1, 1)
+        ob['mx'] = vone[:n].reshape(-1, 1)
+        ob['mz'] = vone[n:].reshape(-1, 1)
+        ob['e_per_site'] = (sum(ob['eb_full']) - para['hx'] * sum(ob['mx'])
+                            - para['hz'] * sum(ob['mz'])) / n
+        out.append(ob)
+    return out
 
 
 def schedule_spectrum(solve):

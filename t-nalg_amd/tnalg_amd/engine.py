@@ -20,6 +20,7 @@ updates, the Krylov basis, reorthogonalisation and the tridiagonal solve stay
 f64.
 """
 import collections
+import concurrent.futures
 import hashlib
 import itertools
 import time
@@ -30,7 +31,7 @@ import torch
 from . import _lib as L
 from . import records as R
 from .device import ITEMSIZE, TORCH_DTYPE, Uploader, ctypes_void, require_gpu
-from .terms import compile_terms
+from .terms import compile_terms_cached
 
 PREC = {
     # name: (TI dtype, TC dtype, GEMM1 mode, GEMM2 mode, floor_rel)
@@ -140,8 +141,61 @@ def buffer_layout(T, dm, d, ncv, lowp):
         offs.append(o)
         o += (nb + ARENA_ALIGN - 1) // ARENA_ALIGN * ARENA_ALIGN
     lay['arena'] = [(name, idx, off, nb, f64) for (name, idx, nb, f64), off in zip(spec, offs)]
+    lay['offsets'] = {(name, idx): off for name, idx, off, _, _ in lay['arena']}
+    lay['dmax_max'] = max(dm)
     lay['bytes'] = o
     return lay
+
+
+def solve_geometry(para, ncv, lowp, dims=None):
+    """(term tables, initial bond dims, per-bond maximum dims, arena layout)
+    of a solve of ``para``.  ``dims``: bond dims of a given initial MPS;
+    default those of the random MPS, [1, chi, ..., chi, 1].  Layouts are kept
+    on the (shared, read-only) term tables."""
+    T = compile_terms_cached(para)
+    if dims is None:
+        chi = int(para['chi'])
+        dims = [1] + [chi] * (T.n - 1) + [1]
+    ob = int(para.get('ob_position', 0))
+    dm = max_dims(dims, T.d, ob)
+    key = (tuple(dims), ob, ncv, bool(lowp))
+    lay = T.layouts.get(key)
+    if lay is None:
+        lay = buffer_layout(T, dm, T.d, ncv, lowp)
+        T.layouts[key] = lay
+    return T, list(dims), dm, lay
+
+
+def make_stage(lay, host, pin=True):
+    """Host image (pinned for a GPU engine) of the arena's staged prefix:
+    every site of the initial MPS ``host`` at its offset (zero padding up to
+    the largest shape the site takes) and the blocked-CholeskyQR identity
+    (stride dmax) and ones."""
+    st = torch.zeros(lay['stage_bytes'], dtype=torch.uint8, pin_memory=pin)
+    v = st.numpy()
+    offs = lay['offsets']
+    for s, a in enumerate(host):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        o = offs[('mps', s)]
+        v[o:o + a.nbytes] = a.view(np.uint8)
+    cb = v[offs[('cholb', None)]:].view(np.float64)
+    dsq, dmx = lay['dsq'], lay['dmax_max']
+    cb[5 * dsq + np.arange(dmx) * (dmx + 1)] = 1.0
+    cb[6 * dsq:6 * dsq + dmx] = 1.0
+    return st
+
+
+def random_mps_host(n, d, chi, seed):
+    """Initial MPS in the reference draw order (TensorBasicModule.py:179-211):
+    site 0 (1, d, chi), site n-1 (chi, d, 1), then sites 1..n-2 (chi, d, chi),
+    standard normals from ``RandomState(seed)``."""
+    rng = seed if hasattr(seed, 'randn') else np.random.RandomState(seed)
+    host = [None] * n
+    host[0] = rng.randn(1, d, chi)
+    host[n - 1] = rng.randn(chi, d, 1)
+    for s in range(1, n - 1):
+        host[s] = rng.randn(chi, d, chi)
+    return host
 
 
 class Step:
@@ -159,12 +213,16 @@ class Step:
 class Solve:
     """Device state of one ground-state solve."""
 
-    def __init__(self, engine, para, seed=0, mps=None, tag=None):
+    def __init__(self, engine, para, seed=0, mps=None, tag=None, stage=None):
+        """``mps``: initial site tensors (default: the seeded random MPS);
+        ``stage``: its prepared pinned image (``make_stage``), when the
+        caller built it already."""
         self.engine = engine
         self.dev = engine.device
         self.tag = tag
         self.para = para
-        self.T = compile_terms(para)
+        in_dims = None if mps is None else [np.shape(mps[0])[0]] + [np.shape(a)[2] for a in mps]
+        self.T, dims, dm, lay = solve_geometry(para, engine.ncv, engine.uses_lowp, in_dims)
         self.n = self.T.n
         self.d = self.T.d
         self.chi = int(para['chi'])
@@ -176,24 +234,15 @@ class Solve:
             raise ValueError('the chain needs at least 2 sites')
         if not 0 <= self.ob < n:
             raise ValueError('ob_position must be in [0, l)')
-        # initial MPS: reference draw order (TensorBasicModule.py:179-211)
-        if mps is None:
-            rng = seed if hasattr(seed, 'randn') else np.random.RandomState(seed)
-            host = [None] * n
-            host[0] = rng.randn(1, d, chi)
-            host[n - 1] = rng.randn(chi, d, 1)
-            for s in range(1, n - 1):
-                host[s] = rng.randn(chi, d, chi)
-        else:
-            host = [np.asarray(a, dtype=np.float64) for a in mps]
-        dims = [host[0].shape[0]] + [a.shape[2] for a in host]
+        if stage is None:
+            host = (random_mps_host(n, d, chi, seed) if mps is None
+                    else [np.asarray(a, dtype=np.float64) for a in mps])
+            stage = make_stage(lay, host, pin=engine.device.type != 'cpu')
         self.dims = list(dims)
-        self.dmax = max_dims(dims, d, self.ob)
-        dm = self.dmax
+        self.dmax = dm
         T = self.T
         self.nL = [len(T.left_opens[b]) for b in range(n + 1)]
         self.nR = [len(T.right_opens[b]) for b in range(n + 1)]
-        lay = buffer_layout(T, dm, d, engine.ncv, engine.uses_lowp)
         self.layout = lay
         self.psi_max, self.dsq_max = lay['psi'], lay['dsq']
         self.dmax_max = max(dm)
@@ -214,19 +263,8 @@ class Solve:
         self.mps, self.Lenv, self.Renv = views['mps'], views['Lenv'], views['Renv']
         if self.lp_bytes:
             self.Lenv_lp, self.Renv_lp = views['Lenv_lp'], views['Renv_lp']
-        # staged region: the initial MPS (each site padded to the largest
-        # shape it takes) and the blocked-CholeskyQR identity (stride dmax)
-        # and ones; the rest of the arena is zero
-        stage = np.zeros(lay['stage_bytes'], dtype=np.uint8)
-        offs = {(name, idx): off for name, idx, off, _, _ in lay['arena']}
-        for s in range(n):
-            a = np.ascontiguousarray(host[s], dtype=np.float64).reshape(-1)
-            o = offs[('mps', s)]
-            stage[o:o + a.nbytes] = a.view(np.uint8)
-        cb = stage[offs[('cholb', None)]:].view(np.float64)
-        dsq, dmx = lay['dsq'], self.dmax_max
-        cb[5 * dsq + np.arange(dmx) * (dmx + 1)] = 1.0
-        cb[6 * dsq:6 * dsq + dmx] = 1.0
+        # staged prefix (initial MPS, CholeskyQR constants) from the pinned
+        # image; the rest of the arena starts at zero
         engine.arena_fill(self.arena, stage)
         self.ops_slot = engine.alloc_ops(T.ops)
         self.reloc = (self.base, self.ops_slot)
@@ -828,14 +866,12 @@ class Engine:
         return torch.empty(nbytes, dtype=torch.uint8, device=self.device)
 
     def arena_fill(self, arena, stage):
-        """Arena prefix <- ``stage`` (host bytes, one pinned copy); the rest
-        of the arena <- 0.  Both are ordered on the engine's stream."""
-        n = stage.nbytes
-        pinned = torch.empty(n, dtype=torch.uint8, pin_memory=True)
-        pinned.numpy()[:] = stage
-        arena[:n].copy_(pinned, non_blocking=True)
+        """Arena prefix <- ``stage`` (pinned uint8 tensor, copied without a
+        host sync; the pinned block is not reused before the copy ends); the
+        rest of the arena <- 0.  Both are ordered on the engine's stream."""
+        n = stage.numel()
+        arena[:n].copy_(stage, non_blocking=True)
         arena[n:].zero_()
-        self._stage_keep = pinned   # alive until the next fill (same stream, ordered)
 
     def arena_release(self, arena):
         self._arena_pool.setdefault(arena.numel(), []).append(arena)
@@ -862,13 +898,31 @@ class Engine:
         self.solves.append(s)
         return s
 
+    def add_solves(self, items, threads=16):
+        """Solves of ``items = [(para, seed), ...]``; the seeded initial MPS
+        draws (host RNG, which runs without the GIL) go to a thread pool."""
+        geoms = [solve_geometry(para, self.ncv, self.uses_lowp) for para, _ in items]
+
+        def draw(k):
+            (para, seed), (T, _, _, lay) = items[k], geoms[k]
+            return make_stage(lay, random_mps_host(T.n, T.d, int(para['chi']), seed),
+                              pin=self.device.type != 'cpu')
+        workers = max(1, min(threads, len(items)))
+        if workers > 1:
+            with concurrent.futures.ThreadPoolExecutor(workers) as ex:
+                stages = list(ex.map(draw, range(len(items))))
+        else:
+            stages = [draw(k) for k in range(len(items))]
+        out = []
+        for (para, seed), st in zip(items, stages):
+            s = Solve(self, para, seed=seed, stage=st)
+            self.solves.append(s)
+            out.append(s)
+        return out
+
     def footprint(self, para):
         """HBM bytes a solve of ``para`` allocates (admission control)."""
-        T = compile_terms(para)
-        n, chi, d = T.n, int(para['chi']), T.d
-        dims = [1] + [chi] * (n - 1) + [1]
-        dm = max_dims(dims, d, int(para.get('ob_position', 0)))
-        return buffer_layout(T, dm, d, self.ncv, self.uses_lowp)['bytes']
+        return solve_geometry(para, self.ncv, self.uses_lowp)[3]['bytes']
 
     def finalize(self, solves):
         """Total energies <H> of finished solves (one batched measurement)."""

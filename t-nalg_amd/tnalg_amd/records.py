@@ -58,6 +58,8 @@ class MixList:
         """``terms``: iterable of ``(src, src_dtype, coef, op)`` (op = -1: identity)."""
         if Dl * d * Dr <= 0:
             return
+        if d > L.MIX_DMAX:
+            raise ValueError('physical dimension %d exceeds the mixer limit %d' % (d, L.MIX_DMAX))
         start = len(self.terms)
         for (src, sdt, coef, op) in terms:
             self.terms.append((src, coef, op, sdt))
@@ -68,7 +70,7 @@ class MixList:
         j = np.array(self.jobs, dtype=MIX_JOB) if self.jobs else EMPTY_JOB.copy()
         t = np.array(self.terms, dtype=MIX_TERM) if self.terms else EMPTY_TERM.copy()
         if len(j):
-            blocks = (j['Dl'].astype(np.int64) * j['d'] * j['Dr'] + L.MIX_BLOCK - 1) // L.MIX_BLOCK
+            blocks = (j['Dl'].astype(np.int64) * j['Dr'] + L.MIX_PAIRS - 1) // L.MIX_PAIRS
             j['block_begin'] = np.concatenate([[0], np.cumsum(blocks)[:-1]])
         return j, t
 
@@ -84,8 +86,8 @@ def mix_blocks(j):
     if len(j) == 0:
         return 0
     last = j[-1]
-    return int(last['block_begin'] + (int(last['Dl']) * int(last['d']) * int(last['Dr'])
-                                      + L.MIX_BLOCK - 1) // L.MIX_BLOCK)
+    return int(last['block_begin'] + (int(last['Dl']) * int(last['Dr'])
+                                      + L.MIX_PAIRS - 1) // L.MIX_PAIRS)
 
 
 # ------------------------------------------------------------ relocation

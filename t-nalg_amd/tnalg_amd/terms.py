@@ -18,6 +18,8 @@ cut of ``MPSClass.py:725-734``).  The straddling double sum is regrouped
 over the smaller side: ``sum_l L_l x (sum_r C[l,r] R_r)^T`` (or the mirror),
 i.e. an exact refactorisation with ``min(#left, #right)`` GEMM pairs.
 """
+import collections
+import hashlib
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -42,6 +44,7 @@ class Terms:
     sites: list = field(default_factory=list)         # per site: SiteTerms
     n_two: int = 0
     n_one: int = 0
+    layouts: dict = field(default_factory=dict, repr=False)  # device layouts built from these tables
 
     def left_index(self, b):
         return {k: t for t, k in enumerate(self.left_opens[b])}
@@ -133,6 +136,36 @@ def compile_terms(para):
                                  lm=sorted((b, v) for b, v in lm.items()),
                                  mr=sorted((a, v) for a, v in mr.items()),
                                  strad_left=strad_left, strad=strad))
+    return T
+
+
+_COMPILED = collections.OrderedDict()
+
+
+def compile_terms_cached(para, maxsize=128):
+    """``compile_terms`` memoised on the content it reads (``l``, ``eigs_tol``,
+    ``index1/2``, ``coeff1/2``, ``op``, ``d``): solves of the same
+    Hamiltonian share one read-only ``Terms``."""
+    h = hashlib.sha1()
+    h.update(repr((int(para['l']), float(para.get('eigs_tol', 0.0)), para.get('d'))).encode())
+    for k, dt in (('index1', np.int64), ('index2', np.int64), ('coeff1', np.float64),
+                  ('coeff2', np.float64)):
+        a = np.ascontiguousarray(np.asarray(para[k], dtype=dt))
+        h.update(repr(a.shape).encode())
+        h.update(a.tobytes())
+    for o in para['op']:
+        a = np.ascontiguousarray(np.asarray(o))
+        h.update(repr((a.dtype.str, a.shape)).encode())
+        h.update(a.tobytes())
+    key = h.hexdigest()
+    T = _COMPILED.get(key)
+    if T is None:
+        T = compile_terms(para)
+        _COMPILED[key] = T
+        if len(_COMPILED) > maxsize:
+            _COMPILED.popitem(last=False)
+    else:
+        _COMPILED.move_to_end(key)
     return T
 
 

@@ -25,8 +25,8 @@ class CpuEngine:
         return torch.empty(nbytes, dtype=torch.uint8)
 
     def arena_fill(self, arena, stage):
-        arena[:stage.nbytes].copy_(torch.from_numpy(stage))
-        arena[stage.nbytes:].zero_()
+        arena[:stage.numel()].copy_(stage)
+        arena[stage.numel():].zero_()
 
     def arena_release(self, arena):
         pass
