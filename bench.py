@@ -96,9 +96,10 @@ def matvec_roofline(para, batch, precision, reps=20):
     import torch
     from tnalg_amd import _lib as L
     from tnalg_amd import records as R
-    from tnalg_amd.dmrg import _schedule_from
     from tnalg_amd.engine import PREC, Engine
-    eng = Engine(precision_schedule=_schedule_from(precision), ncv=20,
+    # the walk to the measured site runs in f64, so the low-precision GEMM
+    # kernels run only in the timed loops below
+    eng = Engine(precision_schedule=lambda sweep, nsweeps: 'fp64', ncv=20,
                  lowp=precision not in (None, 'fp64'))
     solves = [eng.add_solve(para, seed=100 + k) for k in range(batch)]
     for s in solves:
@@ -107,12 +108,14 @@ def matvec_roofline(para, batch, precision, reps=20):
     prec = 'fp64' if precision == 'fp64' else 'bf16'
     site = para['l'] // 2
     from tnalg_amd.engine import Step
-    plans = []
+    # walk every centre to the middle with f64 steps (one batch), then plan
+    # one local step there
     for s in solves:
-        # walk the centre to the middle with f64 steps, then plan one local step there
         for p in range(1, site):
             s.queue.append(Step('solve', p, +1, 0))
-        eng.run_queues([s])
+    eng.run_queues(solves)
+    plans = []
+    for s in solves:
         if eng.uses_lowp and prec != 'fp64':
             eng._mix([s.cast_all_records(PREC[prec][0])])
         plans.append(s.plan(Step('solve', site, +1, para['sweep_time'] - 1), prec))
@@ -127,38 +130,80 @@ def matvec_roofline(para, batch, precision, reps=20):
     lib = eng.lib
     vp = ctypes.c_void_p
 
-    def once():
+    def g1():
         L.check(lib.tnx_gemm_grouped(mode1, vp(ptr[0]), len(g1p), vp(ptr[1]), g1t, None, st), 'g1')
+
+    def g2():
+        L.check(lib.tnx_gemm_grouped(mode2, vp(ptr[2]), len(g2p), vp(ptr[3]), g2t, None, st), 'g2')
+
+    def mix():
         L.check(lib.tnx_mix(vp(ptr[4]), len(mxj), vp(ptr[5]), mxb, vp(eng.ops.data_ptr()), None,
                             st), 'mix')
-        L.check(lib.tnx_gemm_grouped(mode2, vp(ptr[2]), len(g2p), vp(ptr[3]), g2t, None, st), 'g2')
 
-    def gemm_only():
-        L.check(lib.tnx_gemm_grouped(mode1, vp(ptr[0]), len(g1p), vp(ptr[1]), g1t, None, st), 'g1')
-        L.check(lib.tnx_gemm_grouped(mode2, vp(ptr[2]), len(g2p), vp(ptr[3]), g2t, None, st), 'g2')
+    def timed(fns):
+        """Average seconds per repetition of ``fns`` (HIP events recorded on
+        the stream the kernels run on, torch's current stream)."""
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            for f in fns:
+                f()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / reps
 
     for _ in range(3):
-        once()
+        g1(), mix(), g2()
     torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        gemm_only()
-    e1.record()
-    e1.synchronize()
-    t_gemm = e0.elapsed_time(e1) / 1e3 / reps
-    e0.record()
-    for _ in range(reps):
-        once()
-    e1.record()
-    e1.synchronize()
-    t_mv = e0.elapsed_time(e1) / 1e3 / reps
-    # algorithmic bytes: env operands + psi in/out + intermediates, once each
-    isz = {'fp64': 8, 'bf16': 2}[prec]
+    t1, t2, t_mv = timed([g1]), timed([g2]), timed([g1, mix, g2])
     for s in list(solves):
         eng.remove_solve(s)
-    return {'flops': flops, 't_gemm': t_gemm, 't_matvec': t_mv, 'prec': prec,
-            'tiles': int(g1t + g2t), 'itemsize': isz}
+    return {'flops': flops, 'prec': prec, 't_matvec': t_mv,
+            'g1': {'t': t1, 'flops': gemm_flops(g1p, g1s), 'tiles': int(g1t), 'mode': mode1},
+            'g2': {'t': t2, 'flops': gemm_flops(g2p, g2s), 'tiles': int(g2t), 'mode': mode2}}
+
+
+def roofline_object(para, args):
+    """Roofline of the dominant matvec GEMM of a mid-chain local step (the
+    longer of GEMM1 and GEMM2; both are listed under ``kernels``)."""
+    rf = matvec_roofline(para, args.batch, args.precision)
+    prec = rf['prec']
+    peak = MFMA_PEAK[prec]
+    kern = []
+    for what, g in (('GEMM1 T = L x', rf['g1']), ('GEMM2 w = U R^T', rf['g2'])):
+        ach = g['flops'] / g['t'] / 1e12
+        kern.append({'kernel': 'gemm_grouped_kernel<%d>' % g['mode'], 'role': 'matvec ' + what,
+                     'achieved': ach, 'frac': ach / peak, 'avg_launch_s': g['t'],
+                     'flops_per_launch': g['flops'], 'tiles': g['tiles']})
+    dom = max(kern, key=lambda k: k['avg_launch_s'])
+    return {'bound': 'mfma', 'achieved': dom['achieved'], 'peak': peak, 'unit': 'TFLOP/s',
+            'frac': dom['achieved'] / peak, 'traffic': None,
+            'kernel': '%s (%s, %s MFMA, %d solves at site %d of the N=%d chi=%d chain)' % (
+                dom['kernel'], dom['role'], prec, args.batch, para['l'] // 2, para['l'],
+                para['chi']),
+            'avg_launch_s': dom['avg_launch_s'], 'flops_per_launch': dom['flops_per_launch'],
+            'matvec_s': rf['t_matvec'], 'kernels': kern}
+
+
+def gemm_flops(p, s):
+    """Algorithmic flops of one grouped-GEMM launch: 2 M N K summed over the
+    problems and their K segments."""
+    fl = 0.0
+    for q in p:
+        k = int(s['K'][q['seg_begin']:q['seg_begin'] + q['seg_count']].sum())
+        fl += 2.0 * float(q['M']) * float(q['N']) * k
+    return fl
+
+
+def copy_calibration(mib=256):
+    """One elementwise kernel of known HBM traffic (reads and writes ``mib``
+    MiB once), the scale for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    x = torch.ones(mib << 17, dtype=torch.float64, device='cuda')
+    y = torch.empty_like(x)
+    torch.mul(x, 2.0, out=y)
+    torch.cuda.synchronize()
+    return mib << 20
 
 
 def serving_run(n_req, rate, max_active=64, seed=0):
@@ -226,6 +271,9 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--serving-requests', type=int, default=0)
     ap.add_argument('--serving-rate', type=float, default=4.0)
+    ap.add_argument('--roofline-only', action='store_true',
+                    help='only the matvec-GEMM roofline measurement and a copy kernel of known '
+                         'size (for rocprofv3 kernel-trace / counter runs)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only (one-GPU build); --gpus %d is not supported'
@@ -236,6 +284,11 @@ def main():
     _lib.load()
     assert torch.cuda.is_available()
     para = c2_para(args.n, args.chi, args.sweeps)
+    if args.roofline_only:
+        rl = roofline_object(para, args)
+        rl['copy_calibration_bytes'] = copy_calibration()
+        print(json.dumps({'roofline': rl}))
+        return
     seeds = list(range(args.batch))
     cold = None
     for w in range(args.warmup):
@@ -286,15 +339,7 @@ def main():
         out['speedup_vs_plain_fp64'] = value / (1.0 / dt_p)
         print('fp64 batched %.2f solves/s, plain %.3f solves/s, max dE/E %.2e'
               % (len(seeds) / dt_b, 1.0 / dt_p, rel.max()), file=sys.stderr, flush=True)
-    rf = matvec_roofline(para, args.batch, args.precision)
-    prec = rf['prec']
-    achieved = rf['flops'] / rf['t_gemm'] / 1e12
-    out['roofline'] = {'bound': 'mfma', 'achieved': achieved, 'peak': MFMA_PEAK[prec],
-                       'unit': 'TFLOP/s', 'frac': achieved / MFMA_PEAK[prec], 'traffic': None,
-                       'kernel': 'gemm_grouped_kernel (matvec GEMM1+GEMM2, %s, %d solves, mid-chain '
-                                 'site)' % (prec, args.batch),
-                       'avg_launch_s': rf['t_gemm'] / 2, 'matvec_s': rf['t_matvec'],
-                       'flops_per_matvec_batch': rf['flops']}
+    out['roofline'] = roofline_object(para, args)
     if args.serving_requests > 0:
         out['serving'] = serving_run(args.serving_requests, args.serving_rate)
         print('serving: %s' % out['serving'], file=sys.stderr, flush=True)

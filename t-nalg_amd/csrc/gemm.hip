@@ -21,22 +21,26 @@
 
 namespace {
 
+// KB: K depth of one staged chunk (one global round trip and two barriers);
+// NE: elements of A and of B each lane fetches per chunk (256 x NE = 64 x KB).
+// The problems here are small (K of a few hundred), so the chunk is deep:
+// the K loop is bound by the round-trip latency, not by the MFMA rate.
 template <int MODE> struct Cfg;
 template <> struct Cfg<TNX_GEMM_F64> {
   typedef double TI; typedef double TC; typedef double ACC; typedef double CT;
-  static constexpr int KB = 16, NE = 4;
+  static constexpr int KB = 64, NE = 16;
 };
 template <> struct Cfg<TNX_GEMM_F32> {
   typedef float TI; typedef float TC; typedef float ACC; typedef float CT;
-  static constexpr int KB = 16, NE = 4;
+  static constexpr int KB = 64, NE = 16;
 };
 template <> struct Cfg<TNX_GEMM_BF16_F32> {
   typedef __bf16 TI; typedef float TC; typedef float ACC; typedef __bf16 CT;
-  static constexpr int KB = 32, NE = 8;
+  static constexpr int KB = 128, NE = 32;
 };
 template <> struct Cfg<TNX_GEMM_BF16_BF16> {
   typedef __bf16 TI; typedef __bf16 TC; typedef float ACC; typedef __bf16 CT;
-  static constexpr int KB = 32, NE = 8;
+  static constexpr int KB = 128, NE = 32;
 };
 
 template <typename T> __device__ __forceinline__ double to_d(T v) { return (double)v; }
@@ -76,8 +80,8 @@ template <int MODE> struct Frag {
   int ta, tb;
 };
 
-// Thread t's share of a 64 x KB A tile / KB x 64 B tile.  NE = 4 (KB = 16) or
-// NE = 8 (KB = 32): 256 threads x NE = 64 x KB.
+// Thread t's share of a 64 x KB A tile / KB x 64 B tile (256 threads x NE =
+// 64 x KB).
 //   transA = 0 (k contiguous): row t/(KB/NE), k = (t%(KB/NE))*NE .. +NE
 //   transA = 1 (i contiguous): k  t/(64/NE), i = (t%(64/NE))*NE .. +NE
 template <int MODE>
@@ -144,9 +148,9 @@ __device__ __forceinline__ void commit_k4(const Frag<MODE>& f, typename Cfg<MODE
 
 // bf16: LDS tiles row-major in k, As[i][k], Bs[j][k] (8 consecutive k per lane)
 template <int MODE>
-__device__ __forceinline__ void commit_bf16(const Frag<MODE>& f, __bf16 (*As)[32 + 8],
-                                            __bf16 (*Bs)[32 + 8]) {
-  constexpr int KB = 32, NE = 8;
+__device__ __forceinline__ void commit_bf16(const Frag<MODE>& f, __bf16 (*As)[Cfg<MODE>::KB + 8],
+                                            __bf16 (*Bs)[Cfg<MODE>::KB + 8]) {
+  constexpr int KB = Cfg<MODE>::KB, NE = Cfg<MODE>::NE;
   const int t = threadIdx.x;
   const float alpha = (float)f.alpha;
   if (!f.ta) {
@@ -206,8 +210,8 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
   typedef typename std::conditional<(MODE == TNX_GEMM_F64 || MODE == TNX_GEMM_F32),
                                     typename C::CT, __bf16>::type CT;
   constexpr bool K4 = (MODE == TNX_GEMM_F64 || MODE == TNX_GEMM_F32);
-  constexpr int AR = K4 ? KB : TM, AC = K4 ? TM + 4 : 32 + 8;
-  constexpr int BR = K4 ? KB : TN, BC = K4 ? TN + 4 : 32 + 8;
+  constexpr int AR = K4 ? KB : TM, AC = K4 ? TM + 4 : KB + 8;
+  constexpr int BR = K4 ? KB : TN, BC = K4 ? TN + 4 : KB + 8;
   __shared__ __attribute__((aligned(16))) CT As[AR][AC];
   __shared__ __attribute__((aligned(16))) CT Bs[BR][BC];
 
@@ -255,15 +259,18 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
         }
       }
     } else {
-      const int kq = (lane >> 4) * 8;
-      tnx_bf8 a0 = *(const tnx_bf8*)&As[wr + (lane & 15)][kq];
-      tnx_bf8 a1 = *(const tnx_bf8*)&As[wr + 16 + (lane & 15)][kq];
-      tnx_bf8 b0 = *(const tnx_bf8*)&Bs[wc + (lane & 15)][kq];
-      tnx_bf8 b1 = *(const tnx_bf8*)&Bs[wc + 16 + (lane & 15)][kq];
-      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
+#pragma unroll
+      for (int kk = 0; kk < KB; kk += 32) {
+        const int kq = kk + (lane >> 4) * 8;
+        tnx_bf8 a0 = *(const tnx_bf8*)&As[wr + (lane & 15)][kq];
+        tnx_bf8 a1 = *(const tnx_bf8*)&As[wr + 16 + (lane & 15)][kq];
+        tnx_bf8 b0 = *(const tnx_bf8*)&Bs[wc + (lane & 15)][kq];
+        tnx_bf8 b1 = *(const tnx_bf8*)&Bs[wc + 16 + (lane & 15)][kq];
+        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
+      }
     }
   }
 
