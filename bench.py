@@ -261,7 +261,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=1)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--batch', type=int, default=32)
+    ap.add_argument('--batch', type=int, default=64)
     ap.add_argument('--precision', default='mixed')
     ap.add_argument('--n', type=int, default=64)
     ap.add_argument('--chi', type=int, default=128)
