@@ -234,6 +234,32 @@ typedef struct tnx_svd_job {
 } tnx_svd_job; /* 64 bytes */
 int tnx_svd_jacobi_f64(const tnx_svd_job* jobs, int njob, int max_sweeps, tnx_stream_t stream);
 
+/* Batched f64 SVD of the n x n triangular factor R of an SVD gauge move
+ * (n <= 1024), by block one-sided Jacobi over many workgroups, for
+ * calculate_entanglement_spectrum (MPSClass.py:965-985) at large bond
+ * dimension: with A = Q R from CholeskyQR3, np.linalg.svd(A) of
+ * left2right_decompose_tensor / right2left_decompose_tensor
+ * (TensorBasicModule.py:438-444,474-480) is A = (Q u^T) (v), lm = s.
+ * Outputs (row-major n x n): row k of u is the k-th left singular vector of
+ * R, row k of v is s_k times the k-th right singular vector, so R = u^T v;
+ * s holds the singular values in descending order.  The columns of R^T are
+ * rotated in blocks of bw (a power of two, 4 <= bw <= 32, bw * n <= 4096);
+ * one launch per round of the round-robin over block pairs, sweeps until
+ * none rotates (flag: one device int32 shared by the call).  Returns
+ * TNX_ENOCONV after max_sweeps sweeps that still rotated.                   */
+typedef struct tnx_svdj_job {
+  uint64_t r;          /* R, row-major n x n, leading dimension ld_r (read)   */
+  uint64_t work;       /* scratch: 2 n^2 + n doubles                          */
+  uint64_t u, v, s;    /* outputs                                             */
+  int32_t n, ld_r;
+  int32_t bw;          /* columns per block                                   */
+  int32_t nblk;        /* ceil(n / bw), rounded up to even                    */
+  int32_t pair_begin;  /* prefix sum of nblk / 2 over the jobs                */
+  int32_t reserved;
+} tnx_svdj_job; /* 64 bytes */
+int tnx_svd_block_jacobi(const tnx_svdj_job* jobs, int njob, int npairs, int max_nblk, int max_n,
+                         int32_t* flag, int max_sweeps, tnx_stream_t stream);
+
 /* Batched traces / Frobenius inner products: out[i] = sum_ab X_i[a,b] Y_i[a,b]
  * (Y = NULL: trace of X).  Replaces the np.trace closings of observation_s1
  * and observation_s1_s2 (MPSClass.py:1105-1157).                            */

@@ -115,7 +115,7 @@ extern "C" int tnx_abi_sizes(int32_t* sizes, int n) {
                          (int32_t)sizeof(tnx_matvec_plan),   (int32_t)sizeof(tnx_lanczos_solve),
                          (int32_t)sizeof(tnx_lanczos_result), (int32_t)sizeof(tnx_qr_job),
                          (int32_t)sizeof(tnx_svd_job),       (int32_t)sizeof(tnx_dot_job),
-                         (int32_t)sizeof(tnx_chol_job)};
+                         (int32_t)sizeof(tnx_chol_job),      (int32_t)sizeof(tnx_svdj_job)};
   const int na = (int)(sizeof(all) / sizeof(all[0]));
   if (sizes == nullptr) return na;
   for (int i = 0; i < n && i < na; ++i) sizes[i] = all[i];

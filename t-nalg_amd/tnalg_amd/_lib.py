@@ -43,6 +43,22 @@ DOT_JOB = np.dtype([('x', '<u8'), ('y', '<u8'), ('rows', '<i4'), ('cols', '<i4')
                     ('ldx', '<i4'), ('ldy', '<i4')])
 CHOL_JOB = np.dtype([('g', '<u8'), ('r', '<u8'), ('rinv', '<u8'), ('status', '<u8'),
                      ('shift_ref', '<u8'), ('n', '<i4'), ('ld', '<i4'), ('shift_scale', '<f8')])
+SVDJ_JOB = np.dtype([('r', '<u8'), ('work', '<u8'), ('u', '<u8'), ('v', '<u8'), ('s', '<u8'),
+                     ('n', '<i4'), ('ld_r', '<i4'), ('bw', '<i4'), ('nblk', '<i4'),
+                     ('pair_begin', '<i4'), ('reserved', '<i4')])
+SVDJ_NMAX = 1024
+OK, EINVAL, ENOCONV = 0, -1, -3   # tnx_status values
+
+
+def svdj_blocking(n):
+    """(bw, nblk) of tnx_svd_block_jacobi for order n: bw the largest power of
+    two <= min(32, 4096 / n) (at least 4), nblk = ceil(n / bw) rounded up to
+    even."""
+    bw = 32
+    while bw > 4 and bw * n > 4096:
+        bw //= 2
+    nblk = -(-n // bw)
+    return bw, nblk + (nblk % 2)
 CHOL_NMAX = 128
 
 GEMM_TILE = 64
@@ -80,6 +96,7 @@ SIGNATURES = {
     'tnx_qr_f64': (_i, [_vp, _i, _vp]),
     'tnx_svd_jacobi_f64': (_i, [_vp, _i, _i, _vp]),
     'tnx_chol_inv_f64': (_i, [_vp, _i, _vp]),
+    'tnx_svd_block_jacobi': (_i, [_vp, _i, _i, _i, _i, _vp, _i, _vp]),
     'tnx_trace_dot': (_i, [_vp, _i, _vp, _vp]),
 }
 
@@ -108,7 +125,8 @@ def load(path=None):
     n = lib.tnx_abi_sizes(sizes, 16)
     want = [GEMM_SEG.itemsize, GEMM_PROBLEM.itemsize, MIX_JOB.itemsize, MIX_TERM.itemsize,
             ctypes.sizeof(MatvecPlan), LANCZOS_SOLVE.itemsize, LANCZOS_RESULT.itemsize,
-            QR_JOB.itemsize, SVD_JOB.itemsize, DOT_JOB.itemsize, CHOL_JOB.itemsize]
+            QR_JOB.itemsize, SVD_JOB.itemsize, DOT_JOB.itemsize, CHOL_JOB.itemsize,
+            SVDJ_JOB.itemsize]
     got = list(sizes[:n])
     if got != want:
         raise RuntimeError('ABI size mismatch: library %s, binding %s' % (got, want))

@@ -77,6 +77,16 @@ def _schedule_from(precision):
         return mixed_schedule(low='fp32', mid='fp32', n_fp64=1, n_mid=0)
     if precision in ('bf16', 'mixed'):
         return default_mixed_schedule
+    if isinstance(precision, str) and precision.startswith('mixed:'):
+        # 'mixed:<n_bf16>:<n_fp64>': bf16 for the first n_bf16 sweeps, f64 for
+        # the last n_fp64, f32 in between
+        n_low, n_fp64 = (int(v) for v in precision.split(':')[1:3])
+
+        def sched(sweep, nsweeps):
+            if sweep >= nsweeps - n_fp64:
+                return 'fp64'
+            return 'bf16' if sweep < n_low else 'fp32'
+        return sched
     if callable(precision):
         return precision
     raise ValueError('unknown precision %r' % (precision,))

@@ -114,6 +114,7 @@ def buffer_layout(T, dm, d, ncv, lowp):
         'Rq': dsq + max(dm) + 64,
         'qrw': 2 * psi + dsq,
         'chol': 6 * dsq + max(dm) + 64,
+        'lmv': max(1, n - 1) * max(dm),   # entanglement spectra, one row of max(dm) per bond
     }
     # arena: (name, index, bytes, f64 view?) in address order.  The MPS and
     # the CholeskyQR constants come first (staged from the host in one
@@ -133,7 +134,8 @@ def buffer_layout(T, dm, d, ncv, lowp):
              ('Yb', None, 8 * lay['Yb'], True), ('Wb', None, 8 * lay['Wb'], True),
              ('V', None, 8 * lay['V'], True), ('w', None, 8 * psi, True),
              ('x', None, 8 * psi, False), ('init', None, 8 * psi, True),
-             ('Rq', None, 8 * lay['Rq'], True), ('qrw', None, 8 * lay['qrw'], True)]
+             ('Rq', None, 8 * lay['Rq'], True), ('qrw', None, 8 * lay['qrw'], True),
+             ('lmv', None, 8 * lay['lmv'], True)]
     offs, o = [], ARENA_HEADER
     for k, (_, _, nb, _) in enumerate(spec):
         if k == stage_end:
@@ -284,8 +286,21 @@ class Solve:
         self.last_theta = float('nan')
         self.thetas = []
         self.stats = {'matvecs': 0, 'steps': 0, 'unconverged': 0, 'restarts': 0}
-        self.lm = [np.zeros(0) for _ in range(n - 1)]
+        self._lm = [np.zeros(0) for _ in range(n - 1)]
+        self._lm_pending = {}   # bond -> length of a spectrum written on the device
         self.failed = None
+
+    @property
+    def lm(self):
+        """Entanglement spectra per bond (empty until an SVD move visits the
+        bond).  The SVD moves write them to the device; they are copied to
+        the host in one transfer when read."""
+        if self._lm_pending:
+            host = self.lmv.cpu().numpy().reshape(self.n - 1, self.dmax_max)
+            for b, k in self._lm_pending.items():
+                self._lm[b] = host[b, :k].copy()
+            self._lm_pending = {}
+        return self._lm
 
     # ----------------------------------------------------------- addresses
     def op_index(self, k):
@@ -315,7 +330,7 @@ class Solve:
             if hasattr(self, name):
                 setattr(self, name, None)
         for name in ('lc64', 'lc_lp', 'Tb', 'Ub', 'Yb', 'Wb', 'V', 'w', 'x', 'init', 'Rq', 'qrw',
-                     'cholb'):
+                     'cholb', 'lmv'):
             setattr(self, name, None)
         self.engine.arena_release(self.arena)
         self.arena = None
@@ -745,13 +760,27 @@ class Solve:
         return 2.0 * d * Dl * Dr * (Dl * n1 + Dr * n2)
 
     def _plan_svd(self, ph, q, dirn):
+        """SVD centre move (``left2right/right2left_decompose_tensor`` with
+        way='svd', TensorBasicModule.py:438-444,474-480): the spectrum goes
+        to row ``bond`` of the device array ``lmv``.  Orders 8..1024 take
+        CholeskyQR3 of the panel then the multi-workgroup block Jacobi SVD
+        of its R (``_plan_svd_block``); smaller ones the single-workgroup
+        Jacobi SVD of the panel."""
         d, dims = self.d, self.dims
+        bond = q if dirn > 0 else q - 1
+        lm_ptr = _p(self.lmv, bond * self.dmax_max)
+        nn = dims[q + 1] if dirn > 0 else dims[q]
+        m = dims[q] * d if dirn > 0 else d * dims[q + 1]
+        if m < nn:
+            raise ValueError('SVD move needs the panel to be tall (site %d: %d x %d)' % (q, m, nn))
+        ph['lm_bond'] = bond
+        ph['svd_n'] = nn
+        if 8 <= nn <= L.SVDJ_NMAX:
+            self._plan_svd_block(ph, q, dirn, m, nn, lm_ptr)
+            return
         if dirn > 0:
             Dl, Dm, Dn = dims[q], dims[q + 1], dims[q + 2]
-            m, nn = Dl * d, Dm
-            if m < nn:
-                raise ValueError('SVD move needs Dl*d >= Dr (site %d)' % q)
-            ph['svd'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(), self.Rq.data_ptr(),
+            ph['svd'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(), lm_ptr,
                                    _p(self.Rq, nn), self.qrw.data_ptr(), m, nn, nn, 0, 0, 0)],
                                  dtype=L.SVD_JOB)
             g = R.GemmList()
@@ -760,13 +789,9 @@ class Solve:
             mx = R.MixList()
             mx.add(self.mps[q + 1].data_ptr(), L.F64, 1, 1, nn * d * Dn,
                    [(self.init.data_ptr(), L.F64, 1.0, -1)])
-            ph['lm_bond'] = q
         else:
             Dp, Dm, Dr = dims[q - 1], dims[q], dims[q + 1]
-            m, nn = d * Dr, Dm
-            if m < nn:
-                raise ValueError('SVD move needs d*Dr >= Dl (site %d)' % q)
-            ph['svd'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(), self.Rq.data_ptr(),
+            ph['svd'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(), lm_ptr,
                                    _p(self.Rq, nn), self.qrw.data_ptr(), m, nn, m, 1, 1, 0)],
                                  dtype=L.SVD_JOB)
             g = R.GemmList()
@@ -775,10 +800,43 @@ class Solve:
             mx = R.MixList()
             mx.add(self.mps[q - 1].data_ptr(), L.F64, 1, 1, Dp * d * nn,
                    [(self.init.data_ptr(), L.F64, 1.0, -1)])
-            ph['lm_bond'] = q - 1
         ph['absorb'] = g.arrays()
         ph['copy'] = mx.arrays()
-        ph['svd_n'] = nn
+
+    def _plan_svd_block(self, ph, q, dirn, m, nn, lm_ptr):
+        """A = Q R (CholeskyQR3 of the panel, Q in place of A_q), R = u^T v
+        (``tnx_svd_block_jacobi``), then A_q <- Q u^T and the neighbour
+        absorbs v.  Scratch in the CholeskyQR buffer below its constants:
+        B, W, column norms at 0 .. 2 dsq + n; u at 3 dsq; v at 4 dsq."""
+        d, dims = self.d, self.dims
+        if nn <= L.CHOL_NMAX:
+            self._plan_cholqr(ph, q, m, nn, right=dirn < 0)
+        else:
+            self._plan_cholqr_blocked(ph, q, m, nn, right=dirn < 0)
+        dsq = self.dsq_max
+        U, V = _p(self.cholb, 3 * dsq), _p(self.cholb, 4 * dsq)
+        bw, nblk = L.svdj_blocking(nn)
+        ph['svdj'] = np.array([(self.Rq.data_ptr(), _p(self.cholb, 0), U, V, lm_ptr, nn, nn, bw,
+                                nblk, 0, 0)], dtype=L.SVDJ_JOB)
+        A = self.mps[q].data_ptr()
+        T1, T2 = self.qrw.data_ptr(), self.init.data_ptr()
+        g, mx = R.GemmList(), R.MixList()
+        if dirn > 0:   # A_q (m x n) holds Q: A_q <- Q u^T, A_{q+1} <- v A_{q+1}
+            Dn = dims[q + 2]
+            nb = self.mps[q + 1].data_ptr()
+            g.add(T1, m, nn, nn, [(A, nn, 0, U, nn, 1, nn, 1.0)])
+            g.add(T2, nn, d * Dn, d * Dn, [(V, nn, 0, nb, d * Dn, 0, nn, 1.0)])
+            mx.add(A, L.F64, 1, 1, m * nn, [(T1, L.F64, 1.0, -1)])
+            mx.add(nb, L.F64, 1, 1, nn * d * Dn, [(T2, L.F64, 1.0, -1)])
+        else:          # A_q (n x m) holds Q^T: A_q <- u Q^T, A_{q-1} <- A_{q-1} v^T
+            Dp = dims[q - 1]
+            nb = self.mps[q - 1].data_ptr()
+            g.add(T1, nn, m, m, [(U, nn, 0, A, m, 0, nn, 1.0)])
+            g.add(T2, Dp * d, nn, nn, [(nb, nn, 0, V, nn, 1, nn, 1.0)])
+            mx.add(A, L.F64, 1, 1, nn * m, [(T1, L.F64, 1.0, -1)])
+            mx.add(nb, L.F64, 1, 1, Dp * d * nn, [(T2, L.F64, 1.0, -1)])
+        ph['svd_post_g'] = g.arrays()
+        ph['svd_post_m'] = mx.arrays()
 
     # ----------------------------------------------------------- host view
     def host_mps(self):
@@ -815,6 +873,7 @@ class Engine:
         self._resident_bytes = 0
         self.resident_limit = 1 << 30   # descriptor sets of every distinct step of a sweep
         self.plan_cache = {}
+        self.svd_flag = torch.zeros(64, dtype=torch.int32, device=self.device)  # rotation flag
         self._arena_pool = {}   # bytes -> freed arenas (reused by the next solve of that size)
         self.counters = {'matvec_flops': 0.0, 'matvecs': 0, 'lanczos_calls': 0, 'steps': 0,
                          'lz_time': 0.0, 't_plan': 0.0, 't_pre': 0.0, 't_lz_total': 0.0,
@@ -1085,9 +1144,24 @@ class Engine:
                         jobs = R.concat_plain([c[k][1] for c, _ in cqs], L.CHOL_JOB, relocs=rls)
                         cq_meta.append(('c', len(jobs), 0, len(arrs)))
                         arrs.append(jobs)
+            # block-Jacobi SVD moves: jobs (pair prefix sums over the batch),
+            # then the GEMMs and copies that apply u and v
+            svdj = cat(R.concat_plain, 'svdj', L.SVDJ_JOB)
+            svdj_meta = None
+            if len(svdj):
+                halves = svdj['nblk'] // 2
+                svdj['pair_begin'] = np.concatenate([[0], np.cumsum(halves)[:-1]])
+                svdj_meta = (len(svdj), int(halves.sum()), int(svdj['nblk'].max()),
+                             int(svdj['n'].max()), len(arrs))
+                arrs.append(svdj)
+                pg, ps, pnt = cat(R.concat_gemm, 'svd_post_g')
+                pj, pt, pnb = cat(R.concat_mix, 'svd_post_m')
+                svdj_meta += ((len(pg), pnt, len(arrs)), (len(pj), pnb, len(arrs) + 2))
+                arrs += [pg, ps, pj, pt]
             meta = {'nqr': len(qr), 'nsvd': len(svd),
                     'gemm': [(len(p), nt) for p, _, nt in gemms],
-                    'mix': [(len(j), nb) for j, _, nb in mixes], 'cq': cq_meta}
+                    'mix': [(len(j), nb) for j, _, nb in mixes], 'cq': cq_meta,
+                    'svdj': svdj_meta}
             ent = self._resident_put(rkey, arrs, meta)
         ptr, meta = ent.ptrs, ent.meta
         st_ = self.stream()
@@ -1121,6 +1195,20 @@ class Engine:
                                         opsp, None, st_), 'cq mix')
             elif cnt:
                 L.check(lib.tnx_chol_inv_f64(ctypes_void(ptr[ai]), cnt, st_), 'tnx_chol_inv_f64')
+        if meta['svdj'] is not None:   # SVD of the CholeskyQR factors, then apply u and v
+            nj, npairs, max_nblk, max_n, ai, (gp, gnt, gi), (mj, mnb, mi) = meta['svdj']
+            rc = lib.tnx_svd_block_jacobi(ctypes_void(ptr[ai]), nj, npairs, max_nblk, max_n,
+                                          ctypes_void(self.svd_flag.data_ptr()), 60, st_)
+            if rc == L.ENOCONV:
+                self.counters['svd_unconverged'] = self.counters.get('svd_unconverged', 0) + 1
+            else:
+                L.check(rc, 'tnx_svd_block_jacobi')
+            if gnt:
+                L.check(lib.tnx_gemm_grouped(L.GEMM_F64, ctypes_void(ptr[gi]), gp,
+                                             ctypes_void(ptr[gi + 1]), gnt, None, st_), 'svd gemm')
+            if mnb:
+                L.check(lib.tnx_mix(ctypes_void(ptr[mi]), mj, ctypes_void(ptr[mi + 1]), mnb, opsp,
+                                    None, st_), 'svd copy')
         run_gemm(0)      # absorb R into the neighbour
         run_mix(0)       # copy back (moves)
         run_mix(1)       # linear combinations for the environment update
@@ -1145,8 +1233,8 @@ class Engine:
                 s.center = st.site
             elif st.kind in ('move', 'svdmove'):
                 s.center = st.site + st.dir
-            if 'lm_bond' in ph:
-                s.lm[ph['lm_bond']] = s.Rq[:ph['svd_n']].cpu().numpy().copy()
+            if 'lm_bond' in ph:   # spectrum written to lmv on the device; read lazily
+                s._lm_pending[ph['lm_bond']] = ph['svd_n']
         self.counters['steps'] += 1
         return len(plans)
 

@@ -76,6 +76,29 @@ def test_matches_oracle_c1(gpu):
         assert np.allclose(A.lm[b][:k], rA.lm[b][:k], atol=1e-5)
 
 
+@pytest.mark.parametrize('n,chi', [(16, 64), (20, 256)])
+def test_spectrum_is_schmidt_values(gpu, n, chi):
+    """The spectra of the SVD moves (block-Jacobi SVD of the CholeskyQR3
+    factor; chi=256 takes 15 rounds per sweep) are the Schmidt coefficients
+    of the final MPS state at every bond."""
+    from tnalg_amd import dmrg_finite_size
+    c = model.chain_couplings(n, 1.0, 0.8, hz=0.1)
+    para = _para(c, chi, 2)
+    print('case: spectrum N=%d chi=%d' % (n, chi), flush=True)
+    ob, A, info, _ = dmrg_finite_size(para, seed=2)
+    psi = A.mps[0].reshape(-1, A.mps[0].shape[2])
+    for a in A.mps[1:]:
+        psi = (psi @ a.reshape(a.shape[0], -1)).reshape(-1, a.shape[2])
+    psi = psi.reshape(-1)
+    assert abs(np.linalg.norm(psi) - 1.0) < 1e-10
+    for b in range(n - 1):
+        sv = np.linalg.svd(psi.reshape(2 ** (b + 1), -1), compute_uv=False)
+        lm = np.asarray(A.lm[b])
+        assert len(lm) == min(chi, 2 ** (b + 1), 2 ** (n - b - 1)), b
+        assert np.all(np.diff(lm) <= 0), b
+        assert np.allclose(lm, sv[:len(lm)], atol=1e-10), (b, np.abs(lm - sv[:len(lm)]).max())
+
+
 def test_long_range_j1j2_exact(gpu):
     """Straddling (long-range) terms: J1-J2 chain, chi large enough to be exact."""
     from tnalg_amd import dmrg_finite_size

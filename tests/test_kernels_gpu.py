@@ -238,6 +238,56 @@ def test_chol_inv_f64(gpu, n):
     assert int(st.cpu()[0]) == 0
 
 
+def test_svd_block_jacobi(gpu):
+    """Batched block-Jacobi SVD of triangular factors of mixed orders (one
+    launch per round over all jobs): R = u^T v, u orthogonal, s the singular
+    values of R in descending order (numpy's SVD), a rank-deficient R kept
+    exact."""
+    import torch
+    from tnalg_amd import _lib as L
+    lib = L.load()
+    rng = np.random.RandomState(5)
+    Rs = []
+    for n in (8, 37, 128, 300, 512):
+        A = rng.randn(2 * n, n) @ np.diag(np.logspace(0, -10, n))
+        Rs.append(np.linalg.qr(A)[1])
+    Rd = np.triu(rng.randn(24, 24))
+    Rd[9:] = 0.0           # rank 9
+    Rs.append(Rd)
+    bufs, recs = [], []
+    for R in Rs:
+        n = R.shape[0]
+        bw, nblk = L.svdj_blocking(n)
+        r = _dev(R)
+        work = torch.zeros(2 * n * n + n, dtype=torch.float64, device='cuda')
+        u = torch.zeros(n * n, dtype=torch.float64, device='cuda')
+        v = torch.zeros(n * n, dtype=torch.float64, device='cuda')
+        s = torch.zeros(n, dtype=torch.float64, device='cuda')
+        bufs.append((r, work, u, v, s))
+        recs.append((r.data_ptr(), work.data_ptr(), u.data_ptr(), v.data_ptr(), s.data_ptr(),
+                     n, n, bw, nblk, 0, 0))
+    jobs = np.array(recs, dtype=L.SVDJ_JOB)
+    jobs['pair_begin'] = np.concatenate([[0], np.cumsum(jobs['nblk'] // 2)[:-1]])
+    (dj,) = _upload([jobs])
+    flag = torch.zeros(4, dtype=torch.int32, device='cuda')
+    print('svd_block_jacobi orders', [R.shape[0] for R in Rs], flush=True)
+    L.check(lib.tnx_svd_block_jacobi(_vp(dj.data_ptr()), len(jobs), int((jobs['nblk'] // 2).sum()),
+                                     int(jobs['nblk'].max()), int(jobs['n'].max()),
+                                     _vp(flag.data_ptr()), 60, _stream()), 'svd_block_jacobi')
+    torch.cuda.synchronize()
+    for R, (_, _, u, v, s) in zip(Rs, bufs):
+        n = R.shape[0]
+        U = u.cpu().numpy().reshape(n, n)
+        Vs = v.cpu().numpy().reshape(n, n)
+        sv = s.cpu().numpy()
+        ref = np.linalg.svd(R, compute_uv=False)
+        assert np.abs(U.T @ Vs - R).max() < 1e-12 * np.abs(R).max() * n, n
+        assert np.abs(U @ U.T - np.eye(n)).max() < 1e-12 * n, n
+        assert np.all(np.diff(sv) <= 0), n
+        assert np.abs(sv - ref).max() < 1e-13 * ref[0] * n, n
+        assert np.allclose(np.linalg.norm(Vs, axis=1), sv, rtol=1e-12, atol=1e-14), n
+
+
 def test_trace_dot(gpu):
     import torch
     from tnalg_amd import _lib as L
