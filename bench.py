@@ -229,6 +229,69 @@ def serving_run(n_req, rate, max_active=64, seed=0):
     return summ
 
 
+LARGE = {
+    # dense-coupling stress: J1-J2 (J2 = 0.5) 6x6 open square lattice along the snake
+    'c3': dict(width=6, height=6, j2=0.5, chi=512, sweeps=10),
+    # large single solve: Heisenberg 8x8 open square lattice along the snake
+    'c5': dict(width=8, height=8, j2=0.0, chi=1024, sweeps=12),
+}
+
+
+def large_para(name):
+    from tnalg_amd import parameters as P
+    w = LARGE[name]
+    para = P.generate_parameters_dmrg('snakeJ1J2')
+    para.update(square_width=w['width'], square_height=w['height'], j1=1.0, j2=w['j2'],
+                chi=w['chi'], sweep_time=w['sweeps'], dt_ob=w['sweeps'], eigs_tol=1e-10,
+                tau=1e-4, break_tol=0.0)
+    return P.make_consistent_parameter_dmrg(para)
+
+
+def large_solve(name, fp64_check=True):
+    """One large single solve (f32 MFMA matvec sweeps, the last sweep f64;
+    f64 Lanczos, gauge moves and SVD), seeded random MPS, timed end to end
+    (first solve of its engine), with the f64 solve of the same seed for
+    the energy difference."""
+    import torch
+    from tnalg_amd.dmrg import _schedule_from, dmrg_finite_size
+    from tnalg_amd.engine import Engine
+    para = large_para(name)
+    out = {}
+    for prec in (('fp32', 'fp64') if fp64_check else ('fp32',)):
+        eng = Engine(precision_schedule=_schedule_from(prec), lowp=prec != 'fp64')
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ob, A, info, _ = dmrg_finite_size(para, seed=0, precision=prec, engine=eng)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        c = eng.counters
+        out[prec] = {'seconds': dt, 'energy': float(np.ravel(ob['e_per_site'])[0]) * para['l'],
+                     'matvecs': int(c['matvecs']),
+                     'matvec_tflops': c['matvec_flops'] / max(c['lz_time'], 1e-9) / 1e12,
+                     'hbm_gb': eng.footprint(para) / 1e9}
+        del eng, ob, A
+        torch.cuda.empty_cache()
+        print('%s %s: %.1f s, E = %.12f' % (name, prec, dt, out[prec]['energy']), file=sys.stderr,
+              flush=True)
+    w = LARGE[name]
+    res = {'metric': 'DMRG solves/sec (one large solve), 1 GPU', 'value': 1.0 / out['fp32']['seconds'],
+           'unit': 'solves/s', 'n_gpus': 1, 'steps': 1, 'warmup': 0,
+           'ms_per_step': 1e3 * out['fp32']['seconds'], 'higher_is_better': True,
+           'scaling': 'strong', 'vs_baseline': None, 'dtype': 'f32 matvec, f64 Lanczos+QR+SVD',
+           'data': 'synthetic (seeded random initial MPS)',
+           'config': {'workload': '%s: %s %dx%d open square lattice along the snake, J1=1, J2=%g, '
+                                  'chi=%d, %d sweeps, ncv=20, tol=1e-10' % (
+                                      name, 'J1-J2' if w['j2'] else 'Heisenberg', w['width'],
+                                      w['height'], w['j2'], w['chi'], w['sweeps']),
+                      'batch': 1, 'precision': 'fp32 schedule (last sweep f64)'},
+           'solves': out}
+    if fp64_check:
+        e, e64 = out['fp32']['energy'], out['fp64']['energy']
+        res['dE_over_E_vs_fp64'] = {'value': abs(e - e64) / abs(e64), 'target': 1e-9,
+                                    'met': abs(e - e64) <= 1e-9 * abs(e64)}
+    return res
+
+
 def cpu_baseline(seconds=15.0):
     """The oracle's restatement of the reference algorithm (per-term effective
     operators, opt_env matvec, ARPACK eigsh ncv=20) on the C2 chain, timed on
@@ -271,6 +334,9 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--serving-requests', type=int, default=0)
     ap.add_argument('--serving-rate', type=float, default=4.0)
+    ap.add_argument('--workload', default='c2', choices=['c2', 'c3', 'c5'],
+                    help='c2: the batched N=64 chi=128 XXZ chain (default); c3 / c5: one large '
+                         'snake-mapped square-lattice solve (chi 512 / 1024)')
     ap.add_argument('--roofline-only', action='store_true',
                     help='only the matvec-GEMM roofline measurement and a copy kernel of known '
                          'size (for rocprofv3 kernel-trace / counter runs)')
@@ -283,6 +349,9 @@ def main():
     from tnalg_amd import _lib
     _lib.load()
     assert torch.cuda.is_available()
+    if args.workload != 'c2':
+        print(json.dumps(large_solve(args.workload, fp64_check=not args.no_plain)))
+        return
     para = c2_para(args.n, args.chi, args.sweeps)
     if args.roofline_only:
         rl = roofline_object(para, args)

@@ -50,24 +50,49 @@ def _p(t, off_elems=0, itemsize=8):
     return t.data_ptr() + off_elems * itemsize
 
 
+class _Stager:
+    """Pinned host staging for uploads into persistent device buffers: a
+    ring of pinned buffers that only grow, each reused once the copy out of
+    it has completed (its event)."""
+
+    def __init__(self, ring=4):
+        self.host = [None] * ring
+        self.events = [None] * ring
+        self.slot = 0
+
+    def upload(self, dst, arrays, offs):
+        """dst[offs[i]:] <- arrays[i] (stream-ordered, no host sync unless the
+        ring has wrapped onto a copy still in flight)."""
+        total = dst.numel()
+        k = self.slot
+        self.slot = (self.slot + 1) % len(self.host)
+        if self.events[k] is not None:
+            self.events[k].synchronize()
+        if self.host[k] is None or self.host[k].numel() < total:
+            cap = 1 << max(16, (total - 1).bit_length())
+            self.host[k] = torch.empty(cap, dtype=torch.uint8, pin_memory=True)
+        hv = self.host[k].numpy()
+        for a, o in zip(arrays, offs):
+            if a.nbytes:
+                hv[o:o + a.nbytes] = np.ascontiguousarray(a).reshape(-1).view(np.uint8)
+        dst.copy_(self.host[k][:total], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        self.events[k] = ev
+
+
 class _Resident:
     """A set of descriptor arrays uploaded once and kept in device memory
     (steps that repeat every sweep reuse them without host work)."""
-    __slots__ = ('buf', 'ptrs', 'meta', 'host')
+    __slots__ = ('buf', 'ptrs', 'meta')
 
-    def __init__(self, device, arrays, meta):
+    def __init__(self, device, arrays, meta, stager):
         offs, total = [], 0
         for a in arrays:
             offs.append(total)
             total += (a.nbytes + 255) // 256 * 256
-        # pinned staging + stream-ordered copy: no host/device synchronisation
-        self.host = torch.empty(max(total, 256), dtype=torch.uint8, pin_memory=True)
-        hv = self.host.numpy()
-        for a, o in zip(arrays, offs):
-            if a.nbytes:
-                hv[o:o + a.nbytes] = np.frombuffer(np.ascontiguousarray(a).tobytes(), np.uint8)
-        self.buf = torch.empty(self.host.numel(), dtype=torch.uint8, device=device)
-        self.buf.copy_(self.host, non_blocking=True)
+        self.buf = torch.empty(max(total, 256), dtype=torch.uint8, device=device)
+        stager.upload(self.buf, arrays, offs)
         base = self.buf.data_ptr()
         self.ptrs = [base + o for o in offs]
         self.meta = meta
@@ -863,6 +888,7 @@ class Engine:
         self.precision_schedule = precision_schedule or (lambda sweep, nsweeps: 'fp64')
         self.uses_lowp = bool(lowp)
         self.up = Uploader(self.device)
+        self._stager = _Stager()
         self.ops_capacity = ops_capacity
         self.ops = torch.zeros(ops_capacity * 16, dtype=torch.float64, device=self.device)
         self.ops_free = [(0, ops_capacity)]
@@ -943,7 +969,7 @@ class Engine:
         return ent
 
     def _resident_put(self, key, arrays, meta):
-        ent = _Resident(self.device, arrays, meta)
+        ent = _Resident(self.device, arrays, meta, self._stager)
         self._resident[key] = ent
         self._resident_bytes += ent.buf.numel()
         while self._resident_bytes > self.resident_limit and len(self._resident) > 1:

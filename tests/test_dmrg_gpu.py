@@ -113,6 +113,25 @@ def test_long_range_j1j2_exact(gpu):
     assert abs(e - e0) < 1e-9 * abs(e0), (e, e0)
 
 
+def test_snake_j1j2_4x4_exact(gpu):
+    """The 2D path of the dense-coupling configuration at a size ED reaches:
+    J1-J2 (J2 = 0.5) on a 4x4 open square lattice mapped along the snake
+    (couplings up to 7 sites apart), chi = 256 (exact), mixed precision,
+    against exact diagonalisation of the same coupling list."""
+    from tnalg_amd import dmrg_finite_size
+    from tnalg_amd import parameters as P
+    para = P.generate_parameters_dmrg('snakeJ1J2')
+    para.update(square_width=4, square_height=4, chi=256, sweep_time=8, dt_ob=8,
+                eigs_tol=1e-10, tau=1e-4, break_tol=0.0)
+    para = P.make_consistent_parameter_dmrg(para)
+    print('case: snake J1-J2 4x4 chi=256 mixed', flush=True)
+    ob, A, info, _ = dmrg_finite_size(para, seed=4, precision='mixed')
+    e0, _ = ed.ground_state(16, para['op'], para['index1'], para['coeff1'], para['index2'],
+                            para['coeff2'])
+    e = float(ob['e_per_site'][0]) * 16
+    assert abs(e - e0) < 1e-9 * abs(e0), (e, e0)
+
+
 @pytest.mark.parametrize('precision', ['fp32', 'bf16'])
 def test_mixed_precision_reaches_fp64(gpu, precision):
     from tnalg_amd import dmrg_finite_size
