@@ -222,7 +222,12 @@ def serving_run(n_req, rate, max_active=64, seed=0):
     t0 = time.perf_counter()
     for k, (t, para, s) in enumerate(reqs):
         srv.submit(k, para, seed=s, arrival=t0 + t)
-    summ = srv.run()
+
+    def progress(st):
+        print('serving: %.1f s, %d done, %d active, %d waiting' % (
+            time.perf_counter() - t0, st['completed'], st['active'], st['waiting']),
+            file=sys.stderr, flush=True)
+    summ = srv.run(progress=progress)
     summ['requests'] = n_req
     summ['arrival_rate_per_s'] = rate
     summ['max_active'] = max_active
@@ -334,9 +339,10 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--serving-requests', type=int, default=0)
     ap.add_argument('--serving-rate', type=float, default=4.0)
-    ap.add_argument('--workload', default='c2', choices=['c2', 'c3', 'c5'],
+    ap.add_argument('--workload', default='c2', choices=['c2', 'c3', 'c4', 'c5'],
                     help='c2: the batched N=64 chi=128 XXZ chain (default); c3 / c5: one large '
-                         'snake-mapped square-lattice solve (chi 512 / 1024)')
+                         'snake-mapped square-lattice solve (chi 512 / 1024); c4: the serving '
+                         'stream (--serving-requests requests at --serving-rate per second)')
     ap.add_argument('--roofline-only', action='store_true',
                     help='only the matvec-GEMM roofline measurement and a copy kernel of known '
                          'size (for rocprofv3 kernel-trace / counter runs)')
@@ -349,6 +355,22 @@ def main():
     from tnalg_amd import _lib
     _lib.load()
     assert torch.cuda.is_available()
+    if args.workload == 'c4':
+        n_req = args.serving_requests or 512
+        summ = serving_run(n_req, args.serving_rate)
+        print(json.dumps({
+            'metric': 'DMRG solves/sec + p50/p99 latency (serving stream), 1 GPU',
+            'value': summ['solves_per_s'], 'unit': 'solves/s', 'n_gpus': 1, 'steps': 1,
+            'warmup': 0, 'ms_per_step': 1e3 * summ['span_s'], 'higher_is_better': True,
+            'scaling': 'strong', 'vs_baseline': None, 'dtype': 'bf16/f32 matvec, f64 Lanczos+QR',
+            'data': 'synthetic (Poisson arrivals, seeded random initial MPS)',
+            'config': {'workload': 'c4: %d XXZ chain requests, N~U{16..96}, chi~U{32,64,128}, '
+                                   'Jz~U[0.5,1.5], hz~U[0,0.5], 8 sweeps, Poisson arrivals at '
+                                   '%g/s' % (n_req, args.serving_rate),
+                       'batch': 'continuous', 'precision': 'mixed'},
+            'latency_s': {'p50': summ['latency_p50_s'], 'p99': summ['latency_p99_s']},
+            'serving': summ}))
+        return
     if args.workload != 'c2':
         print(json.dumps(large_solve(args.workload, fp64_check=not args.no_plain)))
         return

@@ -294,13 +294,17 @@ class Solve:
         # image; the rest of the arena starts at zero
         engine.arena_fill(self.arena, stage)
         self.ops_slot = engine.alloc_ops(T.ops)
-        self.reloc = (self.base, self.ops_slot)
-        # plans depend on the term tables (couplings, operator indices), the
-        # shapes and the solver tolerances, not on the operator values
+        # relocation of a shared plan to this solve: arena base, operator
+        # slot, coupling-coefficient values
+        self.reloc = (self.base, self.ops_slot, T.coef_values)
+        if T.coef_digest is None:
+            T.coef_digest = hashlib.sha1(T.coef_values.tobytes()).hexdigest()
+        self.reloc_key = (self.base, self.ops_slot, T.coef_digest)
+        # plans depend on the structure of the term tables (which couplings,
+        # operator indices), the shapes and the solver tolerances, not on the
+        # coupling or operator values
         self.sig = hashlib.sha1(repr((
-            n, d, self.ob, tuple(dims), tuple(dm), self.eigs_tol, self.tau, T.ops.shape[0],
-            T.left_opens, T.right_opens,
-            [(st.field, st.lm, st.mr, st.strad_left, st.strad) for st in T.sites]
+            self.ob, tuple(dims), tuple(dm), self.eigs_tol, self.tau, T.structure()
         )).encode()).hexdigest()
         self.d2 = d * d
         # bookkeeping
@@ -1063,7 +1067,7 @@ class Engine:
         """group: list of (solve, plan) sharing one precision."""
         prec = group[0][1]['prec']
         ti, tc, mode1, mode2, _ = PREC[prec]
-        rkey = ('lz',) + tuple((ph['uid'],) + s.reloc for s, ph in group)
+        rkey = ('lz',) + tuple((ph['uid'],) + s.reloc_key for s, ph in group)
         ent = self._resident_get(rkey)
         if ent is None:
             actives = list(range(len(group)))
@@ -1135,7 +1139,7 @@ class Engine:
 
         # every descriptor of the step's pre-Lanczos phases, resident on the
         # device per combination of (plan, solve) (steps repeat every sweep)
-        rkey = ('pre',) + tuple((ph['uid'],) + s.reloc for s, _, ph in plans)
+        rkey = ('pre',) + tuple((ph['uid'],) + s.reloc_key for s, _, ph in plans)
         ent = self._resident_get(rkey)
         if ent is None:
             def cat(fn, key, *extra):

@@ -62,7 +62,7 @@ class MixList:
             raise ValueError('physical dimension %d exceeds the mixer limit %d' % (d, L.MIX_DMAX))
         start = len(self.terms)
         for (src, sdt, coef, op) in terms:
-            self.terms.append((src, coef, op, sdt))
+            self.terms.append((src, coef_field(coef), op, sdt))
         self.jobs.append((dst, Dl, d, Dr, dst_dtype, start, len(self.terms) - start, accumulate,
                           active, 0, 0))
 
@@ -95,10 +95,34 @@ def mix_blocks(j):
 # ``make_relative`` rewrites it against that solve's arena and operator slot,
 # ``relocate`` moves a copy to any other solve.  Every 64-bit unsigned field
 # of a record is a device address (0 = null, left alone); the ``op`` field of
-# a mixer term is an operator-table index (-1 = identity, left alone).
+# a mixer term is an operator-table index (-1 = identity, left alone); the
+# ``coef`` of a mixer term that comes from the coupling tables is written
+# as a quiet NaN whose payload is the coefficient's index + 1, and takes the
+# solve's value on relocation.
 _PTR_FIELDS = {}
 _U64 = np.dtype('<u8')
 _MASK64 = (1 << 64) - 1
+_QNAN = 0x7FF8000000000000
+_PAYLOAD = (1 << 51) - 1
+_LITERAL_COEFS = (1.0, -1.0, 0.0)
+
+
+def coef_field(c):
+    """The ``coef`` field of a mixer term: the index of a tagged coupling
+    coefficient (``terms.Coef``), NaN-boxed, or the literal value."""
+    k = getattr(c, 'k', None)
+    if k is None:
+        return float(c)
+    return float(np.array(_QNAN | (k + 1), dtype=np.uint64).view(np.float64))
+
+
+def _unbox(coefs, values):
+    """In place: NaN-boxed coefficient indices -> ``values[k]``."""
+    bits = coefs.view(np.uint64)
+    boxed = (bits & np.uint64(0xFFF8000000000000)) == np.uint64(_QNAN)
+    boxed &= (bits & np.uint64(_PAYLOAD)) != 0
+    if boxed.any():
+        coefs[boxed] = values[(bits[boxed] & np.uint64(_PAYLOAD)).astype(np.int64) - 1]
 
 
 def ptr_fields(dt):
@@ -123,10 +147,13 @@ def _shift(arr, delta, slot_delta):
 
 
 def relocate(arr, reloc):
-    """Copy of a relative record array moved to ``reloc = (base, slot)``."""
+    """Copy of a relative record array moved to ``reloc = (base, slot[,
+    coefficient values])``."""
     out = arr.copy()
     if reloc is not None:
         _shift(out, reloc[0], reloc[1])
+        if len(reloc) > 2 and len(out) and 'coef' in out.dtype.names:
+            _unbox(out['coef'], reloc[2])
     return out
 
 
@@ -136,6 +163,13 @@ def _rel_one(arr, lo, hi, slot):
         nz = v[v != 0]
         if len(nz) and (int(nz.min()) < lo or int(nz.max()) >= hi):
             raise ValueError('plan record field %r points outside the solve arena' % f)
+    if len(arr) and 'coef' in arr.dtype.names:
+        c = arr['coef']
+        lit = ~np.isnan(c)
+        if not np.isin(c[lit], _LITERAL_COEFS).all():
+            # a coupling value that lost its tag would be shared by solves
+            # with other couplings
+            raise ValueError('plan mixer term has an untagged coefficient')
     _shift(arr, -lo, -slot)
 
 

@@ -109,11 +109,16 @@ class SolveServer:
                 still.append(r)
         self.active = still
 
-    def run(self, until_empty=True, max_steps=None):
-        """Serve until every submitted request has finished."""
+    def run(self, until_empty=True, max_steps=None, progress=None, every=5.0):
+        """Serve until every submitted request has finished.  ``progress``
+        (optional) is called with the summary every ``every`` seconds."""
         if self.t0 is None:
             self.t0 = self.clock()
+        last = self.clock()
         while self.waiting or self.active:
+            if progress is not None and self.clock() - last >= every:
+                last = self.clock()
+                progress(dict(self.summary(), active=len(self.active), waiting=len(self.waiting)))
             now = self.clock()
             self._admit(now)
             if not self.active:

@@ -25,6 +25,17 @@ from dataclasses import dataclass, field
 import numpy as np
 
 
+class Coef(float):
+    """A coefficient of the term tables: its value, and its index ``k`` in
+    ``Terms.coef_values``.  Step plans refer to coefficients by index, so
+    solves whose couplings differ only in value share one plan."""
+
+    def __new__(cls, value, k):
+        obj = float.__new__(cls, value)
+        obj.k = k
+        return obj
+
+
 @dataclass
 class SiteTerms:
     field: list            # [(op, coef)] on-site operators at p
@@ -45,6 +56,23 @@ class Terms:
     n_two: int = 0
     n_one: int = 0
     layouts: dict = field(default_factory=dict, repr=False)  # device layouts built from these tables
+    coef_values: np.ndarray = field(default=None, repr=False)  # value of coefficient k
+    coef_digest: str = field(default=None, repr=False)         # hash of coef_values
+
+    def structure(self):
+        """Everything about the tables except the coefficient values (each
+        coefficient appears as its index)."""
+        def sub(v):
+            return ('c', v.k) if isinstance(v, Coef) else v
+        sites = []
+        for st in self.sites:
+            sites.append((tuple((op, sub(c)) for op, c in st.field),
+                          tuple((b, tuple((i, sub(c)) for i, c in g)) for b, g in st.lm),
+                          tuple((a, tuple((j, sub(c)) for j, c in g)) for a, g in st.mr),
+                          st.strad_left,
+                          tuple((r, tuple((o, sub(c)) for o, c in g)) for r, g in st.strad)))
+        return (self.n, self.d, self.ops.shape[0], tuple(map(tuple, self.left_opens)),
+                tuple(map(tuple, self.right_opens)), tuple(sites))
 
     def left_index(self, b):
         return {k: t for t, k in enumerate(self.left_opens[b])}
@@ -102,6 +130,11 @@ def compile_terms(para):
     if 'd' in para and int(para['d']) != d:
         raise ValueError('para["d"]=%s but the operators are %dx%d' % (para['d'], d, d))
     T = Terms(n=n, d=d, ops=ops, n_two=len(two), n_one=len(one))
+    coef_values = []
+
+    def tag(c):
+        coef_values.append(float(c))
+        return Coef(c, len(coef_values) - 1)
     for b in range(n + 1):
         T.left_opens.append(sorted({(i, a) for (i, j, a, _, _) in two if i < b <= j}))
         T.right_opens.append(sorted({(j, bb) for (i, j, _, bb, _) in two if i < b <= j}))
@@ -132,10 +165,15 @@ def compile_terms(para):
         else:
             for r in rset:
                 strad.append((r, [(l, c) for (l, rr), c in sorted(pairs.items()) if rr == r]))
-        T.sites.append(SiteTerms(field=sorted(fld.items()),
-                                 lm=sorted((b, v) for b, v in lm.items()),
-                                 mr=sorted((a, v) for a, v in mr.items()),
-                                 strad_left=strad_left, strad=strad))
+        # tag every coefficient with its index, in an order fixed by the
+        # structure alone (sorted keys, then list order)
+        field_t = [(k, tag(c)) for k, c in sorted(fld.items(), key=lambda kv: kv[0])]
+        lm_t = [(b, [(i, tag(c)) for i, c in v]) for b, v in sorted(lm.items(), key=lambda kv: kv[0])]
+        mr_t = [(a, [(j, tag(c)) for j, c in v]) for a, v in sorted(mr.items(), key=lambda kv: kv[0])]
+        strad_t = [(r, [(o, tag(c)) for o, c in g]) for r, g in strad]
+        T.sites.append(SiteTerms(field=field_t, lm=lm_t, mr=mr_t, strad_left=strad_left,
+                                 strad=strad_t))
+    T.coef_values = np.array(coef_values, dtype=np.float64)
     return T
 
 
