@@ -39,6 +39,7 @@ def c2_para(n=64, chi=128, sweeps=8, jz=1.0, hz=0.0, tol=1e-10):
 
 
 _ENGINES = {}
+_PLAN_AHEAD = [True]   # --no-plan-ahead: next-step host work in line (for comparison)
 
 
 def engine_for(precision, ncv=20):
@@ -50,7 +51,7 @@ def engine_for(precision, ncv=20):
     key = (precision, ncv)
     if key not in _ENGINES:
         _ENGINES[key] = Engine(precision_schedule=_schedule_from(precision), ncv=ncv,
-                               lowp=precision not in (None, 'fp64'))
+                               lowp=precision not in (None, 'fp64'), plan_ahead=_PLAN_AHEAD[0])
     return _ENGINES[key]
 
 
@@ -216,7 +217,7 @@ def serving_run(n_req, rate, max_active=64, seed=0):
     from tnalg_amd.engine import Engine
     from tnalg_amd.serving import SolveServer, poisson_xxz_stream
     reqs = poisson_xxz_stream(n_req, rate=rate, seed=seed)
-    eng = Engine(precision_schedule=_schedule_from('mixed'), lowp=True)
+    eng = Engine(precision_schedule=_schedule_from('mixed'), lowp=True, plan_ahead=_PLAN_AHEAD[0])
     free = torch.cuda.mem_get_info()[0]
     srv = SolveServer(eng, hbm_budget=int(0.6 * free), max_active=max_active)
     t0 = time.perf_counter()
@@ -228,6 +229,12 @@ def serving_run(n_req, rate, max_active=64, seed=0):
             time.perf_counter() - t0, st['completed'], st['active'], st['waiting']),
             file=sys.stderr, flush=True)
     summ = srv.run(progress=progress)
+    c = eng.counters
+    # where the wall time of the engine steps went (host planning and
+    # descriptor assembly vs the Lanczos calls, which wait on the GPU)
+    summ['step_time_s'] = {'plan': c['t_plan'], 'pre_phases': c['t_pre'],
+                           'lanczos_total': c['t_lz_total'], 'lanczos_calls': c['lz_time'],
+                           'wait_for_plan_ahead': c['t_settle'], 'engine_steps': c['steps']}
     summ['requests'] = n_req
     summ['arrival_rate_per_s'] = rate
     summ['max_active'] = max_active
@@ -336,6 +343,8 @@ def main():
     ap.add_argument('--sweeps', type=int, default=8)
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-plan-ahead', action='store_true',
+                    help='do the next step\'s host work in line instead of on a worker thread')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--serving-requests', type=int, default=0)
     ap.add_argument('--serving-rate', type=float, default=4.0)
@@ -347,6 +356,7 @@ def main():
                     help='only the matvec-GEMM roofline measurement and a copy kernel of known '
                          'size (for rocprofv3 kernel-trace / counter runs)')
     args = ap.parse_args()
+    _PLAN_AHEAD[0] = not args.no_plan_ahead
     if args.gpus != 1:
         print('bench.py measures one GPU only (one-GPU build); --gpus %d is not supported'
               % args.gpus, file=sys.stderr)

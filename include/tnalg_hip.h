@@ -39,8 +39,9 @@ enum tnx_status {
   TNX_ENOCONV = -3,  /* eigensolver hit its restart limit (result still written) */
 };
 
-/* element types of operands */
-enum tnx_dtype { TNX_F64 = 0, TNX_F32 = 1, TNX_BF16 = 2 };
+/* element types of operands; TNX_DTYPE_PER_SOLVE as the x_dtype / w_dtype
+ * argument of a Lanczos call takes each solve's own x_dtype / w_dtype field */
+enum tnx_dtype { TNX_F64 = 0, TNX_F32 = 1, TNX_BF16 = 2, TNX_DTYPE_PER_SOLVE = -1 };
 
 /* GEMM precision modes: (operand type -> accumulate -> output type)       */
 enum tnx_gemm_mode {
@@ -125,7 +126,8 @@ typedef struct tnx_lanczos_solve {
   double tol_scale;    /* converged when |r| <= tol_scale*|1 - tau*theta|   */
   double tau;          /*   (ARPACK's test on I - tau*H, MPSClass.py:886)   */
   double floor_rel;    /*   or |r| <= floor_rel*|theta| (precision floor)   */
-  double reserved;
+  int32_t x_dtype;     /* tnx_dtype of x and of w, read when the call passes */
+  int32_t w_dtype;     /*   TNX_DTYPE_PER_SOLVE                             */
 } tnx_lanczos_solve; /* 88 bytes */
 
 /* Per-solve results of tnx_lanczos_run (host-visible after the call).      */
@@ -185,6 +187,17 @@ int tnx_lanczos_run_ex(const tnx_matvec_plan* plan, const tnx_lanczos_solve* sol
                        int nchunks, int ncv, int max_restarts, int min_iters, int x_dtype,
                        int w_dtype, void* state, tnx_lanczos_result* results,
                        tnx_stream_t stream);
+/* As tnx_lanczos_run_ex for solves whose matvecs run at different
+ * precisions: each Krylov step applies the nplan matvec plans (host array)
+ * in turn, and the solves' own x_dtype / w_dtype fields give the dtypes.
+ * The `active` field of a plan's GEMM problems and mixer jobs is the
+ * solve's index in `solves`.  The same replacement of MPSClass.py:883-889
+ * as tnx_lanczos_run, for a batch that mixes the precisions of the sweep
+ * schedule (one Lanczos loop instead of one per precision).               */
+int tnx_lanczos_run_multi(const tnx_matvec_plan* plans, int nplan,
+                          const tnx_lanczos_solve* solves, int nsolve, int nchunks, int ncv,
+                          int max_restarts, int min_iters, void* state,
+                          tnx_lanczos_result* results, tnx_stream_t stream);
 
 /* Batched f64 Householder QR gauge move.  For each matrix i, the logical
  * matrix M (m x n) is src[r*ld+c] (trans=0) or src[c*ld+r] (trans=1); on

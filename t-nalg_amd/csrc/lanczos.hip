@@ -280,8 +280,10 @@ __device__ void tridiag_lowest(const double* a, const double* b, int k, double* 
   for (int i = 0; i < k; ++i) y[i] = z[i];
 }
 
-template <typename T> __device__ __forceinline__ void store_x(void* x, int dtype, int64_t i, double v) {
-  tnx_store(x, dtype, i, v);
+// dtype of a solve's x or w: the call's, or the solve's own field when the
+// call passes TNX_DTYPE_PER_SOLVE
+__device__ __forceinline__ int dtype_of(int call_dtype, int32_t own) {
+  return call_dtype >= 0 ? call_dtype : own;
 }
 
 // ---------------------------------------------------------------- kernels
@@ -344,10 +346,11 @@ __global__ __launch_bounds__(NT) void k_init_scale(const tnx_lanczos_solve* __re
   const double* v = (const double*)D.init;
   double* V0 = (double*)D.V;
   const double flat = 1.0 / sqrt((double)D.n);
+  const int xd = dtype_of(x_dtype, D.x_dtype);
   for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
     double val = sc > 0.0 ? v[i] * sc : flat;
     V0[i] = val;
-    tnx_store((void*)D.x, x_dtype, i, val);
+    tnx_store((void*)D.x, xd, i, val);
   }
 }
 
@@ -358,11 +361,12 @@ __global__ __launch_bounds__(NT) void k_dots(const tnx_lanczos_solve* __restrict
   ChunkInfo ci = chunk_of(S, nsolve);
   if (L.act[ci.s] == 0) return;
   const tnx_lanczos_solve D = S[ci.s];
+  const int wd = dtype_of(w_dtype, D.w_dtype);
   double w[CHUNK / NT];
 #pragma unroll
   for (int q = 0; q < CHUNK / NT; ++q) {
     int i = ci.i0 + q * NT + threadIdx.x;
-    w[q] = (i < ci.i1) ? tnx_load((const void*)D.w, w_dtype, i) : 0.0;
+    w[q] = (i < ci.i1) ? tnx_load((const void*)D.w, wd, i) : 0.0;
   }
   const double* V = (const double*)D.V;
   __shared__ double red[4][NCV_MAX + 1];
@@ -386,11 +390,12 @@ __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __r
   if (ci.c == 0)
     for (int k = threadIdx.x; k <= j; k += NT) L.st[ci.s].h1[k] = h[k];
   const double* V = (const double*)D.V;
+  const int wd = dtype_of(w_dtype, D.w_dtype);
   double r[CHUNK / NT];
 #pragma unroll
   for (int q = 0; q < CHUNK / NT; ++q) {
     int i = ci.i0 + q * NT + threadIdx.x;
-    r[q] = (i < ci.i1) ? tnx_load((const void*)D.w, w_dtype, i) : 0.0;
+    r[q] = (i < ci.i1) ? tnx_load((const void*)D.w, wd, i) : 0.0;
   }
   chunk_axpy(V, D.ldv, j, ci.i0, ci.i1, h, r);
   double* Vn = (double*)D.V + (int64_t)(j + 1) * D.ldv;
@@ -448,13 +453,14 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
     }
     chunk_axpy(V, D.ldv, j, ci.i0, ci.i1, h, r);
     const double inv = 1.0 / beta_j;
+    const int xd = dtype_of(x_dtype, D.x_dtype);
 #pragma unroll
     for (int q = 0; q < CHUNK / NT; ++q) {
       int i = ci.i0 + q * NT + threadIdx.x;
       if (i < ci.i1) {
         const double v = r[q] * inv;
         Vn[i] = v;
-        tnx_store((void*)D.x, x_dtype, i, v);
+        tnx_store((void*)D.x, xd, i, v);
       }
     }
   }
@@ -539,10 +545,11 @@ __global__ __launch_bounds__(NT) void k_restart(const tnx_lanczos_solve* __restr
   }
   const double* X = (const double*)D.V + (int64_t)m_rows * D.ldv;
   double* V0 = (double*)D.V;
+  const int xd = dtype_of(x_dtype, D.x_dtype);
   for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
     double v = X[i] * sc;
     V0[i] = v;
-    tnx_store((void*)D.x, x_dtype, i, v);
+    tnx_store((void*)D.x, xd, i, v);
   }
 }
 
@@ -609,14 +616,19 @@ extern "C" size_t tnx_lanczos_state_bytes(int nsolve, int nchunks, int ncv) {
          2 * align256(sizeof(double) * (size_t)nchunks * (NCV_MAX + 1)) + 256;
 }
 
-extern "C" int tnx_lanczos_run_ex(const tnx_matvec_plan* plan, const tnx_lanczos_solve* solves,
-                                  int nsolve, int nchunks, int ncv, int max_restarts,
-                                  int min_iters, int x_dtype, int w_dtype, void* state,
-                                  tnx_lanczos_result* results, tnx_stream_t stream) {
-  if (plan == nullptr || solves == nullptr || state == nullptr || nsolve <= 0 || nchunks <= 0 ||
-      ncv < 2 || ncv > NCV_MAX || max_restarts < 0) {
-    tnx_set_error("tnx_lanczos_run: bad arguments (nsolve=%d nchunks=%d ncv=%d)", nsolve, nchunks,
-                  ncv);
+namespace {
+
+// The Lanczos loop of every entry point: each Krylov step applies the nplan
+// matvec plans in turn (one per precision of the batch), then the shared
+// orthogonalisation kernels over all solves.
+int lanczos_run(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve* solves,
+                int nsolve, int nchunks, int ncv, int max_restarts, int min_iters, int x_dtype,
+                int w_dtype, void* state, tnx_lanczos_result* results, tnx_stream_t stream) {
+  if (plans == nullptr || nplan <= 0 || solves == nullptr || state == nullptr || nsolve <= 0 ||
+      nchunks <= 0 || ncv < 2 || ncv > NCV_MAX || max_restarts < 0 || x_dtype < -1 ||
+      x_dtype > TNX_BF16 || w_dtype < -1 || w_dtype > TNX_BF16) {
+    tnx_set_error("tnx_lanczos_run: bad arguments (nplan=%d nsolve=%d nchunks=%d ncv=%d)", nplan,
+                  nsolve, nchunks, ncv);
     return TNX_EINVAL;
   }
   if (int rc = ensure_pinned(nsolve)) return rc;
@@ -644,7 +656,8 @@ extern "C" int tnx_lanczos_run_ex(const tnx_matvec_plan* plan, const tnx_lanczos
       hipLaunchKernelGGL(k_restart, grid, block, 0, s, solves, nsolve, state, nchunks, ncv, x_dtype);
     }
     for (int j = 0; j < ncv && !done; ++j) {
-      if ((rc = tnx_heff_matvec(plan, L.act, stream))) break;
+      for (int p = 0; p < nplan && !rc; ++p) rc = tnx_heff_matvec(plans + p, L.act, stream);
+      if (rc) break;
       hipLaunchKernelGGL(k_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
       hipLaunchKernelGGL(k_update_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
       hipLaunchKernelGGL(k_update_norm, grid, block, 0, s, solves, nsolve, state, nchunks, j,
@@ -685,6 +698,24 @@ extern "C" int tnx_lanczos_run_ex(const tnx_matvec_plan* plan, const tnx_lanczos
     nconv += g_pin.res[i].converged;
   }
   return nconv == nsolve ? TNX_OK : TNX_ENOCONV;
+}
+
+}  // namespace
+
+extern "C" int tnx_lanczos_run_ex(const tnx_matvec_plan* plan, const tnx_lanczos_solve* solves,
+                                  int nsolve, int nchunks, int ncv, int max_restarts,
+                                  int min_iters, int x_dtype, int w_dtype, void* state,
+                                  tnx_lanczos_result* results, tnx_stream_t stream) {
+  return lanczos_run(plan, 1, solves, nsolve, nchunks, ncv, max_restarts, min_iters, x_dtype,
+                     w_dtype, state, results, stream);
+}
+
+extern "C" int tnx_lanczos_run_multi(const tnx_matvec_plan* plans, int nplan,
+                                     const tnx_lanczos_solve* solves, int nsolve, int nchunks,
+                                     int ncv, int max_restarts, int min_iters, void* state,
+                                     tnx_lanczos_result* results, tnx_stream_t stream) {
+  return lanczos_run(plans, nplan, solves, nsolve, nchunks, ncv, max_restarts, min_iters,
+                     TNX_DTYPE_PER_SOLVE, TNX_DTYPE_PER_SOLVE, state, results, stream);
 }
 
 extern "C" int tnx_lanczos_run(const tnx_matvec_plan* plan, const tnx_lanczos_solve* solves,

@@ -30,7 +30,7 @@ MIX_TERM = np.dtype([('src', '<u8'), ('coef', '<f8'), ('op', '<i4'), ('src_dtype
 LANCZOS_SOLVE = np.dtype([('V', '<u8'), ('w', '<u8'), ('x', '<u8'), ('init', '<u8'),
                           ('out', '<u8'), ('n', '<i4'), ('ldv', '<i4'), ('m', '<i4'),
                           ('chunk_begin', '<i4'), ('tol_scale', '<f8'), ('tau', '<f8'),
-                          ('floor_rel', '<f8'), ('reserved', '<f8')])
+                          ('floor_rel', '<f8'), ('x_dtype', '<i4'), ('w_dtype', '<i4')])
 LANCZOS_RESULT = np.dtype([('theta', '<f8'), ('resid', '<f8'), ('iters', '<i4'),
                            ('converged', '<i4')])
 QR_JOB = np.dtype([('src', '<u8'), ('q', '<u8'), ('r', '<u8'), ('work', '<u8'),
@@ -93,6 +93,8 @@ SIGNATURES = {
                              _vp]),
     'tnx_lanczos_run_ex': (_i, [ctypes.POINTER(MatvecPlan), _vp, _i, _i, _i, _i, _i, _i, _i, _vp,
                                 _vp, _vp]),
+    'tnx_lanczos_run_multi': (_i, [ctypes.POINTER(MatvecPlan), _i, _vp, _i, _i, _i, _i, _i, _vp,
+                                   _vp, _vp]),
     'tnx_qr_f64': (_i, [_vp, _i, _vp]),
     'tnx_svd_jacobi_f64': (_i, [_vp, _i, _i, _vp]),
     'tnx_chol_inv_f64': (_i, [_vp, _i, _vp]),

@@ -318,6 +318,10 @@ class Solve:
         self._lm = [np.zeros(0) for _ in range(n - 1)]
         self._lm_pending = {}   # bond -> length of a spectrum written on the device
         self.failed = None
+        # shared plan parts relocated to this solve, by (plan uid, phase);
+        # a solve revisits the same plans every sweep
+        self.moved = {}
+        self._next = None   # (step, precision, plan) planned ahead
 
     @property
     def lm(self):
@@ -363,6 +367,23 @@ class Solve:
             setattr(self, name, None)
         self.engine.arena_release(self.arena)
         self.arena = None
+        self.moved = {}
+        self._next = None
+
+    def part(self, ph, key):
+        """Phase ``key`` of the shared plan ``ph`` at this solve's addresses
+        and coefficients (relocated once per plan, then reused)."""
+        uid = ph['uid']
+        got = self.moved.get((uid, key))
+        if got is None:
+            pk = ph['packed']
+            buf = self.moved.get(uid)
+            if buf is None:
+                buf = pk.relocate(self.reloc)
+                self.moved[uid] = buf
+            got = pk.views(buf, key)
+            self.moved[(uid, key)] = got
+        return got
 
     # ------------------------------------------------------------ schedule
     def schedule_canonicalise(self):
@@ -395,7 +416,7 @@ class Solve:
         """Descriptor records of one step (dict phase -> arrays), shared by
         every solve of the engine with the same signature: device addresses
         are relative to the solve's arena and operator indices to its slot of
-        the operator table (``records.relocate`` moves them to a solve)."""
+        the operator table (``Solve.part`` moves a phase to a solve)."""
         key = (self.sig, st.kind, st.site, st.dir, tuple(self.dims), prec)
         cache = self.engine.plan_cache
         hit = cache.get(key)
@@ -403,7 +424,7 @@ class Solve:
             self.dims = list(hit['dims_after'])
             return hit
         ph = self._build_plan(st, prec)
-        R.make_relative(ph, self.base, self.base + self.layout['bytes'], self.ops_slot)
+        R.pack_relative(ph, self.base, self.base + self.layout['bytes'], self.ops_slot)
         cache[key] = ph
         return ph
 
@@ -776,7 +797,8 @@ class Solve:
         tol = self.eigs_tol if self.eigs_tol > 0 else 2.220446049250313e-16
         ph['lz'] = np.array([(self.V.data_ptr(), self.w.data_ptr(), Xp, self.init.data_ptr(),
                               self.mps[p].data_ptr(), n_el, psi, min(self.engine.ncv, n_el), 0,
-                              tol / self.tau, self.tau, floor_rel, 0.0)], dtype=L.LANCZOS_SOLVE)
+                              tol / self.tau, self.tau, floor_rel, ti, tc)],
+                            dtype=L.LANCZOS_SOLVE)
         ph['prec'] = prec
         ph['mv_flops'] = self._matvec_flops(p)
 
@@ -869,6 +891,7 @@ class Solve:
 
     # ----------------------------------------------------------- host view
     def host_mps(self):
+        self.engine.settle()
         out = []
         for s in range(self.n):
             a = self.mps[s][:self.dims[s] * self.d * self.dims[s + 1]].cpu().numpy()
@@ -876,11 +899,121 @@ class Solve:
         return out
 
 
+# ------------------------------------------------------ batch descriptors
+def _parts(pairs, key):
+    """Phase ``key`` of each ``(solve, plan)`` that has it, at the solve's
+    addresses."""
+    return [s.part(ph, key) for s, ph in pairs if key in ph]
+
+
+def assemble_pre(pairs):
+    """Descriptor arrays (one launch per phase over the batch) and launch
+    metadata of the pre-Lanczos phases of one step of every ``(solve, plan)``.
+    Array order: qr, svd, 3 x (gemm problems, segments), 4 x (mix jobs,
+    terms), the CholeskyQR stages, then the SVD-move records."""
+    qr = R.concat_plain(_parts(pairs, 'qr'), L.QR_JOB)
+    svd = R.concat_plain(_parts(pairs, 'svd'), L.SVD_JOB)
+    gemms = [R.concat_gemm(_parts(pairs, k)) for k in ('absorb', 'env_g1', 'env_gT')]
+    mixes = [R.concat_mix(_parts(pairs, k)) for k in ('copy', 'env_lc', 'env_mix', 'cast')]
+    arrs = [qr, svd]
+    for p, sg, _ in gemms:
+        arrs += [p, sg]
+    for j, t, _ in mixes:
+        arrs += [j, t]
+    cq_meta = []
+    groups_cq = collections.OrderedDict()   # same stage structure -> one launch per stage
+    for c in _parts(pairs, 'cq'):
+        groups_cq.setdefault(tuple(kind for kind, _ in c), []).append(c)
+    for sig, cqs in groups_cq.items():
+        for k, kind in enumerate(sig):
+            stage = [c[k][1] for c in cqs]
+            if kind == 'g':
+                p, sg, nt = R.concat_gemm(stage)
+                cq_meta.append(('g', len(p), nt, len(arrs)))
+                arrs += [p, sg]
+            elif kind == 'm':
+                j, t, nb = R.concat_mix(stage)
+                cq_meta.append(('m', len(j), nb, len(arrs)))
+                arrs += [j, t]
+            else:
+                jobs = R.concat_plain(stage, L.CHOL_JOB)
+                cq_meta.append(('c', len(jobs), 0, len(arrs)))
+                arrs.append(jobs)
+    # block-Jacobi SVD moves: jobs (pair prefix sums over the batch), then
+    # the GEMMs and copies that apply u and v
+    svdj = R.concat_plain(_parts(pairs, 'svdj'), L.SVDJ_JOB)
+    svdj_meta = None
+    if len(svdj):
+        halves = svdj['nblk'] // 2
+        svdj['pair_begin'] = np.concatenate([[0], np.cumsum(halves)[:-1]])
+        svdj_meta = (len(svdj), int(halves.sum()), int(svdj['nblk'].max()),
+                     int(svdj['n'].max()), len(arrs))
+        arrs.append(svdj)
+        pg, ps, pnt = R.concat_gemm(_parts(pairs, 'svd_post_g'))
+        pj, pt, pnb = R.concat_mix(_parts(pairs, 'svd_post_m'))
+        svdj_meta += ((len(pg), pnt, len(arrs)), (len(pj), pnb, len(arrs) + 2))
+        arrs += [pg, ps, pj, pt]
+    meta = {'nqr': len(qr), 'nsvd': len(svd),
+            'gemm': [(len(p), nt) for p, _, nt in gemms],
+            'mix': [(len(j), nb) for j, _, nb in mixes], 'cq': cq_meta, 'svdj': svdj_meta}
+    return arrs, meta
+
+
+def _lanczos_groups(pairs):
+    """The ``(solve, plan)`` pairs that run a Lanczos solve, one list per
+    matvec precision (in a fixed precision order)."""
+    by = {}
+    for s, ph in pairs:
+        if 'lz' in ph:
+            by.setdefault(ph['prec'], []).append((s, ph))
+    return [by[p] for p in sorted(by)]
+
+
+def _lanczos_key(groups):
+    return ('lz',) + tuple(tuple((ph['uid'],) + s.reloc_key for s, ph in g) for g in groups)
+
+
+def assemble_lanczos(groups):
+    """Descriptor arrays and metadata of one batched Lanczos call over
+    ``groups``, lists of ``(solve, plan)`` that share a precision: the
+    linear-combination mix, the solve records (in group order, with their
+    chunk prefix sums) and, per group, the three matvec phases whose
+    ``active`` field is the solve's index in the whole batch.  Array order:
+    mix jobs, mix terms, solve records, then 6 per group."""
+    flat = [sp for g in groups for sp in g]
+    lc_j, lc_t, lc_nb = R.concat_mix([s.part(ph, 'mv_lc') for s, ph in flat])
+    lz = R.concat_plain([s.part(ph, 'lz') for s, ph in flat], L.LANCZOS_SOLVE)
+    chunks = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
+    lz['chunk_begin'] = np.concatenate([[0], np.cumsum(chunks)[:-1]])
+    arrs = [lc_j, lc_t, lz]
+    gmeta = []
+    first = 0
+    for g in groups:
+        actives = list(range(first, first + len(g)))
+        first += len(g)
+        g1p, g1s, g1t = R.concat_gemm([s.part(ph, 'mv_g1') for s, ph in g], actives)
+        mxj, mxt, mxb = R.concat_mix([s.part(ph, 'mv_mix') for s, ph in g], actives)
+        g2p, g2s, g2t = R.concat_gemm([s.part(ph, 'mv_g2') for s, ph in g], actives)
+        _, _, mode1, mode2, _ = PREC[g[0][1]['prec']]
+        gmeta.append((mode1, mode2, (len(g1p), g1t), (len(mxj), mxb), (len(g2p), g2t)))
+        arrs += [g1p, g1s, mxj, mxt, g2p, g2s]
+    meta = {'lc': (len(lc_j), lc_nb), 'nchunks': int(chunks.sum()), 'groups': gmeta}
+    return arrs, meta
+
+
 class Engine:
-    """Owns the device, the stream, the operator table and the launch loop."""
+    """Owns the device, the stream, the operator table and the launch loop.
+
+    With ``plan_ahead`` (default), the host work of step k+1 (planning,
+    relocating the shared plans, assembling the descriptor sets) runs on a
+    worker thread while step k's Lanczos call, which holds no GIL, keeps the
+    GPU busy.  Every method that reads solve state first waits for that work
+    (``settle``).  The interpreter's thread switch interval is lowered to
+    0.2 ms the first time, so the main thread gets the GIL back promptly when
+    the Lanczos call returns."""
 
     def __init__(self, precision_schedule=None, ncv=20, max_restarts=20, min_iters=1,
-                 device='cuda:0', lowp=True, ops_capacity=4096):
+                 device='cuda:0', lowp=True, ops_capacity=4096, plan_ahead=True):
         require_gpu()
         self.lib = L.load()
         self.device = torch.device(device)
@@ -907,7 +1040,66 @@ class Engine:
         self._arena_pool = {}   # bytes -> freed arenas (reused by the next solve of that size)
         self.counters = {'matvec_flops': 0.0, 'matvecs': 0, 'lanczos_calls': 0, 'steps': 0,
                          'lz_time': 0.0, 't_plan': 0.0, 't_pre': 0.0, 't_lz_total': 0.0,
-                         'lz_iters': 0}
+                         'lz_iters': 0, 't_settle': 0.0, 'ahead_used': 0}
+        self.plan_ahead = bool(plan_ahead)
+        self._ahead_pool = None
+        self._ahead = None     # future of the worker's next-step host work
+        self._spec = None      # its speculative descriptor sets
+
+    # ------------------------------------------------------ planning ahead
+    def _prec_of(self, s, st):
+        """Matvec precision of step ``st`` of solve ``s`` (moves: f64)."""
+        if st.kind != 'solve':
+            return 'fp64'
+        return self.precision_schedule(max(st.sweep, 0), s.para.get('sweep_time', 1))
+
+    def settle(self):
+        """Wait for the worker's next-step host work (a no-op without it)."""
+        if self._ahead is not None:
+            t0 = time.perf_counter()
+            fut, self._ahead = self._ahead, None
+            self._spec = fut.result()   # re-raises a planning error here
+            self.counters['t_settle'] += time.perf_counter() - t0
+
+    def _start_ahead(self, cands):
+        if not cands:
+            return
+        if self._ahead_pool is None:
+            import sys
+            if sys.getswitchinterval() > 2e-4:
+                sys.setswitchinterval(2e-4)
+            self._ahead_pool = concurrent.futures.ThreadPoolExecutor(1)
+        self._ahead = self._ahead_pool.submit(self._plan_ahead, cands)
+
+    def _plan_ahead(self, cands):
+        """Worker: plan each candidate's next queued step (leaving its
+        ``dims`` as they are; the step applies ``dims_after``), relocate the
+        plans, and assemble the descriptor sets the next step would need if
+        it runs exactly these solves."""
+        pairs = []
+        for s in cands:
+            st = s.queue[0]
+            dims0 = s.dims
+            s.dims = list(dims0)
+            try:
+                prec = self._prec_of(s, st)
+                ph = s.plan(st, prec)
+            finally:
+                s.dims = dims0
+            for key in ph['packed'].trees:
+                s.part(ph, key)
+            s._next = (st, prec, ph)
+            pairs.append((s, ph))
+        spec = {}
+        rkey = ('pre',) + tuple((ph['uid'],) + s.reloc_key for s, ph in pairs)
+        if rkey not in self._resident:
+            spec['pre'] = (rkey, assemble_pre(pairs))
+        groups = _lanczos_groups(pairs)
+        if groups:
+            rkey = _lanczos_key(groups)
+            if rkey not in self._resident:
+                spec['lz'] = (rkey, assemble_lanczos(groups))
+        return spec
 
     # ------------------------------------------------------------ op table
     def alloc_ops(self, ops):
@@ -1029,6 +1221,7 @@ class Engine:
         return out
 
     def remove_solve(self, s):
+        self.settle()
         if s in self.solves:
             self.solves.remove(s)
         s.free()
@@ -1063,48 +1256,47 @@ class Engine:
         L.check(self.lib.tnx_svd_jacobi_f64(ctypes_void(dj), len(jobs), max_sweeps, self.stream()),
                 'tnx_svd_jacobi_f64')
 
-    def _lanczos(self, group):
-        """group: list of (solve, plan) sharing one precision."""
-        prec = group[0][1]['prec']
-        ti, tc, mode1, mode2, _ = PREC[prec]
-        rkey = ('lz',) + tuple((ph['uid'],) + s.reloc_key for s, ph in group)
+    def _lanczos(self, groups, spec=None, ahead=None):
+        """One batched Lanczos call over ``groups``: lists of (solve, plan),
+        one list per matvec precision (each Krylov step runs every group's
+        matvec, then the shared f64 kernels).  ``spec``: descriptor sets
+        assembled ahead; ``ahead``: solves whose next step the worker plans
+        while the call runs."""
+        flat = [sp for g in groups for sp in g]
+        rkey = _lanczos_key(groups)
         ent = self._resident_get(rkey)
         if ent is None:
-            actives = list(range(len(group)))
-            rls = [s.reloc for s, _ in group]
-            lc_j, lc_t, lc_nb = R.concat_mix([ph['mv_lc'] for _, ph in group], relocs=rls)
-            g1p, g1s, g1t = R.concat_gemm([ph['mv_g1'] for _, ph in group], actives, rls)
-            mxj, mxt, mxb = R.concat_mix([ph['mv_mix'] for _, ph in group], actives, rls)
-            g2p, g2s, g2t = R.concat_gemm([ph['mv_g2'] for _, ph in group], actives, rls)
-            lz = R.concat_plain([ph['lz'] for _, ph in group], L.LANCZOS_SOLVE, relocs=rls)
-            chunks = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
-            lz['chunk_begin'] = np.concatenate([[0], np.cumsum(chunks)[:-1]])
-            meta = {'lc': (len(lc_j), lc_nb), 'g1': (len(g1p), g1t), 'mix': (len(mxj), mxb),
-                    'g2': (len(g2p), g2t), 'nchunks': int(chunks.sum())}
-            ent = self._resident_put(rkey, [lc_j, lc_t, g1p, g1s, mxj, mxt, g2p, g2s, lz], meta)
+            got = spec.get('lz') if spec else None
+            ent = self._resident_put(rkey, *(got[1] if got and got[0] == rkey
+                                             else assemble_lanczos(groups)))
         ptrs, meta = ent.ptrs, ent.meta
         nchunks = meta['nchunks']
         if meta['lc'][1]:
             L.check(self.lib.tnx_mix(ctypes_void(ptrs[0]), meta['lc'][0], ctypes_void(ptrs[1]),
                                      meta['lc'][1], ctypes_void(self.ops.data_ptr()), None,
                                      self.stream()), 'tnx_mix')
-        plan = L.MatvecPlan(ptrs[2], ptrs[3], meta['g1'][0], meta['g1'][1], ptrs[4], ptrs[5],
-                            meta['mix'][0], meta['mix'][1], ptrs[6], ptrs[7], meta['g2'][0],
-                            meta['g2'][1], self.ops.data_ptr(), mode1, mode2)
-        need = int(self.lib.tnx_lanczos_state_bytes(len(group), nchunks, self.ncv))
+        plans = (L.MatvecPlan * len(groups))()
+        for k, (mode1, mode2, g1, mx, g2) in enumerate(meta['groups']):
+            b = 3 + 6 * k
+            plans[k] = L.MatvecPlan(ptrs[b], ptrs[b + 1], g1[0], g1[1], ptrs[b + 2], ptrs[b + 3],
+                                    mx[0], mx[1], ptrs[b + 4], ptrs[b + 5], g2[0], g2[1],
+                                    self.ops.data_ptr(), mode1, mode2)
+        need = int(self.lib.tnx_lanczos_state_bytes(len(flat), nchunks, self.ncv))
         if self.lz_state is None or self.lz_state.numel() < need:
             self.lz_state = torch.empty(max(need, 1 << 20), dtype=torch.uint8, device=self.device)
-        res = np.zeros(len(group), dtype=L.LANCZOS_RESULT)
+        res = np.zeros(len(flat), dtype=L.LANCZOS_RESULT)
+        if ahead:
+            self._start_ahead(ahead)
         t0 = time.perf_counter()
-        rc = self.lib.tnx_lanczos_run_ex(plan, ctypes_void(ptrs[8]), len(group), nchunks, self.ncv,
-                                         self.max_restarts, self.min_iters, ti, tc,
-                                         ctypes_void(self.lz_state.data_ptr()),
-                                         ctypes_void(res.ctypes.data), self.stream())
+        rc = self.lib.tnx_lanczos_run_multi(plans, len(groups), ctypes_void(ptrs[2]), len(flat),
+                                            nchunks, self.ncv, self.max_restarts, self.min_iters,
+                                            ctypes_void(self.lz_state.data_ptr()),
+                                            ctypes_void(res.ctypes.data), self.stream())
         self.counters['lz_time'] += time.perf_counter() - t0
         if rc not in (0, -3):
             L.check(rc, 'tnx_lanczos_run')
         self.counters['lanczos_calls'] += 1
-        for (s, ph), r in zip(group, res):
+        for (s, ph), r in zip(flat, res):
             s.last_theta = float(r['theta'])
             s.thetas.append(s.last_theta)
             s.stats['matvecs'] += int(r['iters'])
@@ -1115,6 +1307,8 @@ class Engine:
 
     def step(self, solves=None):
         """Advance every solve with a queued entry by one step."""
+        self.settle()
+        spec, self._spec = self._spec, None
         solves = [s for s in (solves or self.solves) if s.queue and s.failed is None]
         if not solves:
             return 0
@@ -1122,77 +1316,30 @@ class Engine:
         plans = []
         for s in solves:
             st = s.queue.pop(0)
-            nsw = s.para.get('sweep_time', 1)
-            prec = self.precision_schedule(max(st.sweep, 0), nsw) if st.kind == 'solve' else 'fp64'
+            nx, s._next = s._next, None
+            if nx is not None and nx[0] is st:   # planned ahead by the worker
+                prec, ph = nx[1], nx[2]
+                self.counters['ahead_used'] += 1
+            else:
+                prec, ph = self._prec_of(s, st), None
             if self.uses_lowp and prec != 'fp64' and s.lowp_dtype != PREC[prec][0]:
                 self._mix([s.cast_all_records(PREC[prec][0])])
                 s.lowp_dtype = PREC[prec][0]
-            ph = s.plan(st, prec)
+            if ph is None:
+                ph = s.plan(st, prec)
+            else:
+                s.dims = list(ph['dims_after'])
             plans.append((s, st, ph))
 
         t_b = time.perf_counter()
-
-        def sel(key):
-            """(relative parts, relocations) of the plans that have phase ``key``."""
-            got = [(ph[key], s.reloc) for s, _, ph in plans if key in ph]
-            return [g[0] for g in got], [g[1] for g in got]
-
         # every descriptor of the step's pre-Lanczos phases, resident on the
         # device per combination of (plan, solve) (steps repeat every sweep)
         rkey = ('pre',) + tuple((ph['uid'],) + s.reloc_key for s, _, ph in plans)
         ent = self._resident_get(rkey)
         if ent is None:
-            def cat(fn, key, *extra):
-                parts, rls = sel(key)
-                return fn(parts, *extra, relocs=rls)
-
-            qr = cat(R.concat_plain, 'qr', L.QR_JOB)
-            svd = cat(R.concat_plain, 'svd', L.SVD_JOB)
-            gemms = [cat(R.concat_gemm, k) for k in ('absorb', 'env_g1', 'env_gT')]
-            mixes = [cat(R.concat_mix, k) for k in ('copy', 'env_lc', 'env_mix', 'cast')]
-            arrs = [qr, svd]
-            for p, sg, _ in gemms:
-                arrs += [p, sg]
-            for j, t, _ in mixes:
-                arrs += [j, t]
-            cq_meta = []
-            groups_cq = collections.OrderedDict()   # same stage structure -> one launch/stage
-            for c, rl in zip(*sel('cq')):
-                groups_cq.setdefault(tuple(kind for kind, _ in c), []).append((c, rl))
-            for sig, cqs in groups_cq.items():
-                rls = [rl for _, rl in cqs]
-                for k, kind in enumerate(sig):
-                    if kind == 'g':
-                        p, sg, nt = R.concat_gemm([c[k][1] for c, _ in cqs], relocs=rls)
-                        cq_meta.append(('g', len(p), nt, len(arrs)))
-                        arrs += [p, sg]
-                    elif kind == 'm':
-                        j, t, nb = R.concat_mix([c[k][1] for c, _ in cqs], relocs=rls)
-                        cq_meta.append(('m', len(j), nb, len(arrs)))
-                        arrs += [j, t]
-                    else:
-                        jobs = R.concat_plain([c[k][1] for c, _ in cqs], L.CHOL_JOB, relocs=rls)
-                        cq_meta.append(('c', len(jobs), 0, len(arrs)))
-                        arrs.append(jobs)
-            # block-Jacobi SVD moves: jobs (pair prefix sums over the batch),
-            # then the GEMMs and copies that apply u and v
-            svdj = cat(R.concat_plain, 'svdj', L.SVDJ_JOB)
-            svdj_meta = None
-            if len(svdj):
-                halves = svdj['nblk'] // 2
-                svdj['pair_begin'] = np.concatenate([[0], np.cumsum(halves)[:-1]])
-                svdj_meta = (len(svdj), int(halves.sum()), int(svdj['nblk'].max()),
-                             int(svdj['n'].max()), len(arrs))
-                arrs.append(svdj)
-                pg, ps, pnt = cat(R.concat_gemm, 'svd_post_g')
-                pj, pt, pnb = cat(R.concat_mix, 'svd_post_m')
-                svdj_meta += ((len(pg), pnt, len(arrs)), (len(pj), pnb, len(arrs) + 2))
-                arrs += [pg, ps, pj, pt]
-            meta = {'nqr': len(qr), 'nsvd': len(svd),
-                    'gemm': [(len(p), nt) for p, _, nt in gemms],
-                    'mix': [(len(j), nb) for j, _, nb in mixes], 'cq': cq_meta,
-                    'svdj': svdj_meta}
-            ent = self._resident_put(rkey, arrs, meta)
+            got = spec.get('pre') if spec else None
+            ent = self._resident_put(rkey, *(got[1] if got and got[0] == rkey else
+                                             assemble_pre([(s, ph) for s, _, ph in plans])))
         ptr, meta = ent.ptrs, ent.meta
         st_ = self.stream()
         lib = self.lib
@@ -1247,12 +1394,11 @@ class Engine:
         run_gemm(2)      # Q^T . W
         run_mix(3)       # low-precision copies of the new block
         t_c = time.perf_counter()
-        groups = {}
-        for s, st, ph in plans:
-            if 'lz' in ph:
-                groups.setdefault(ph['prec'], []).append((s, ph))
-        for prec in sorted(groups):
-            self._lanczos(groups[prec])
+        groups = _lanczos_groups([(s, ph) for s, _, ph in plans])
+        if groups:
+            # the next step's host work overlaps this Lanczos call
+            ahead = [s for s, _, _ in plans if s.queue] if self.plan_ahead else None
+            self._lanczos(groups, spec, ahead)
         t_d = time.perf_counter()
         self.counters['t_plan'] += t_b - t_a
         self.counters['t_pre'] += t_c - t_b

@@ -87,6 +87,7 @@ def measure(engine, solves, requests):
     """``requests[k] = (one, two)`` for ``solves[k]``: ``one`` = [(site, op)],
     ``two`` = [(i, j, op_i, op_j)].  Returns, per solve, ``(one_vals, two_vals,
     norm)`` as numpy arrays (real parts)."""
+    engine.settle()   # solve state is final once the engine's host work is done
     plans = [_MeasurePlan(s, one, two) for s, (one, two) in zip(solves, requests)]
     lib = engine.lib
     stream = engine.stream()

@@ -125,6 +125,9 @@ def _unbox(coefs, values):
         coefs[boxed] = values[(bits[boxed] & np.uint64(_PAYLOAD)).astype(np.int64) - 1]
 
 
+_PTR_WORDS = {}
+
+
 def ptr_fields(dt):
     f = _PTR_FIELDS.get(dt)
     if f is None:
@@ -133,15 +136,27 @@ def ptr_fields(dt):
     return f
 
 
+def _ptr_block(arr):
+    """The device-address fields of ``arr`` as an ``(n, k)`` uint64 view.
+    Every record type of the ABI starts with its address fields, packed."""
+    dt = arr.dtype
+    k = _PTR_WORDS.get(dt)
+    if k is None:
+        offs = [dt.fields[n][1] for n in ptr_fields(dt)]
+        if offs != [8 * i for i in range(len(offs))] or dt.itemsize % 8:
+            raise TypeError('record type without a leading block of address fields: %r' % dt)
+        k = len(offs)
+        _PTR_WORDS[dt] = k
+    if k == 0 or len(arr) == 0:
+        return None
+    return arr.view(np.uint64).reshape(len(arr), dt.itemsize // 8)[:, :k]
+
+
 def _shift(arr, delta, slot_delta):
-    if len(arr) == 0:
-        return
-    d = np.uint64(delta & _MASK64)
-    for f in ptr_fields(arr.dtype):
-        v = arr[f]
-        nz = v != 0
-        v[nz] += d
-    if slot_delta and 'op' in arr.dtype.names:
+    w = _ptr_block(arr)
+    if w is not None:
+        np.add(w, np.uint64(delta & _MASK64), out=w, where=w != 0)
+    if slot_delta and len(arr) and 'op' in arr.dtype.names:
         o = arr['op']
         o[o >= 0] += slot_delta
 
@@ -157,25 +172,48 @@ def relocate(arr, reloc):
     return out
 
 
+def relocate_tree(obj, reloc):
+    """``relocate`` applied to every record array of a nested tuple/list
+    (other leaves, e.g. stage kinds, are kept)."""
+    if isinstance(obj, np.ndarray) and obj.dtype.names is not None:
+        return relocate(obj, reloc)
+    if isinstance(obj, tuple):
+        return tuple(relocate_tree(o, reloc) for o in obj)
+    if isinstance(obj, list):
+        return [relocate_tree(o, reloc) for o in obj]
+    return obj
+
+
 def _rel_one(arr, lo, hi, slot):
-    for f in ptr_fields(arr.dtype):
-        v = arr[f]
-        nz = v[v != 0]
-        if len(nz) and (int(nz.min()) < lo or int(nz.max()) >= hi):
-            raise ValueError('plan record field %r points outside the solve arena' % f)
+    w = _ptr_block(arr)
+    if w is not None:
+        nz = w != 0
+        if nz.any():
+            v = w[nz]
+            if int(v.min()) < lo or int(v.max()) >= hi:
+                raise ValueError('plan record of type %r points outside the solve arena'
+                                 % (arr.dtype.names,))
+            np.subtract(w, np.uint64(lo), out=w, where=nz)
     if len(arr) and 'coef' in arr.dtype.names:
         c = arr['coef']
-        lit = ~np.isnan(c)
-        if not np.isin(c[lit], _LITERAL_COEFS).all():
+        ok = np.isnan(c)
+        for lit in _LITERAL_COEFS:
+            ok |= c == lit
+        if not ok.all():
             # a coupling value that lost its tag would be shared by solves
             # with other couplings
             raise ValueError('plan mixer term has an untagged coefficient')
-    _shift(arr, -lo, -slot)
+    if slot and len(arr) and 'op' in arr.dtype.names:
+        o = arr['op']
+        o[o >= 0] -= slot
+
+
+PLAN_META = ('dims_after', 'uid', 'prec', 'mv_flops', 'svd_n', 'lm_bond', 'packed')
 
 
 def make_relative(ph, lo, hi, slot):
     """In place: every record array of plan ``ph`` made relative to the
-    arena ``[lo, hi)`` and operator slot ``slot``."""
+    arena ``[lo, hi)`` and operator slot ``slot`` (array by array)."""
     def walk(obj):
         if isinstance(obj, np.ndarray) and obj.dtype.names is not None:
             _rel_one(obj, lo, hi, slot)
@@ -183,8 +221,151 @@ def make_relative(ph, lo, hi, slot):
             for o in obj:
                 walk(o)
     for k, v in ph.items():
-        if k not in ('dims_after', 'uid', 'prec', 'mv_flops', 'svd_n', 'lm_bond'):
+        if k not in PLAN_META:
             walk(v)
+
+
+# ------------------------------------------------------------ packed plans
+# A plan holds a few dozen small record arrays.  Packed, they sit in one byte
+# buffer with the word indices of every non-null device address, every
+# NaN-boxed coefficient (with its index) and every operator index >= 0, so
+# that making the plan relative and moving it to a solve are a fixed number of
+# array operations whatever the number of records.
+class _Leaf:
+    __slots__ = ('i',)
+
+    def __init__(self, i):
+        self.i = i
+
+
+_FIELD_WORDS = {}
+
+
+def _field_words(dt):
+    """(stride in 8-byte words, address words, coef word or -1, op int32 or -1)."""
+    got = _FIELD_WORDS.get(dt)
+    if got is None:
+        _ptr_block(np.zeros(1, dt))      # checks the leading address block
+        cw = dt.fields['coef'][1] // 8 if 'coef' in dt.names else -1
+        ow = dt.fields['op'][1] // 4 if 'op' in dt.names else -1
+        got = (dt.itemsize // 8, _PTR_WORDS[dt], cw, ow)
+        _FIELD_WORDS[dt] = got
+    return got
+
+
+def _flatten(obj, arrs):
+    """``obj`` with each record array replaced by a leaf holding its index
+    in ``arrs`` (appended)."""
+    to = type(obj)
+    if to is np.ndarray and obj.dtype.names is not None:
+        arrs.append(obj)
+        return _Leaf(len(arrs) - 1)
+    if to is tuple:
+        return tuple([_flatten(o, arrs) for o in obj])
+    if to is list:
+        return [_flatten(o, arrs) for o in obj]
+    return obj
+
+
+def _inflate(tree, buf, specs):
+    """``tree`` with each leaf replaced by its records viewed in ``buf``
+    (``frombuffer``: writable, and cheaper than a structured ``view``)."""
+    tt = type(tree)
+    if tt is _Leaf:
+        off, dt, n = specs[tree.i]
+        return np.frombuffer(buf, dt, n, off)
+    if tt is tuple:
+        return tuple([_inflate(t, buf, specs) for t in tree])
+    if tt is list:
+        return [_inflate(t, buf, specs) for t in tree]
+    return tree
+
+
+class Packed:
+    """Relative plan records in one buffer.  ``ptr``: uint64 word indices of
+    the non-null addresses; ``cidx``/``ck``: float64 word indices of boxed
+    coefficients and their coefficient indices; ``op``: int32 indices of
+    operator-table references."""
+    __slots__ = ('buf', 'trees', 'specs', 'ptr', 'cidx', 'ck', 'op')
+
+    def views(self, buf, key):
+        """The records of phase ``key`` as views into ``buf``."""
+        return _inflate(self.trees[key], buf, self.specs)
+
+    def relocate(self, reloc):
+        """A copy of the buffer moved to ``reloc = (base, slot, coefficient
+        values)``."""
+        buf = self.buf.copy()
+        if len(self.ptr):
+            w = buf.view(np.uint64)
+            w[self.ptr] += np.uint64(reloc[0] & _MASK64)
+        if len(self.op) and reloc[1]:
+            buf.view(np.int32)[self.op] += np.int32(reloc[1])
+        if len(self.cidx):
+            buf.view(np.float64)[self.cidx] = np.asarray(reloc[2], dtype=np.float64)[self.ck]
+        return buf
+
+
+def pack_relative(ph, lo, hi, slot):
+    """Plan ``ph`` packed and made relative to the arena ``[lo, hi)`` and
+    operator slot ``slot``; its phases are replaced by views of the packed
+    buffer and the packing is stored under ``ph['packed']``."""
+    arrs = []
+    keys = [k for k in ph if k not in PLAN_META]
+    trees = {k: _flatten(ph[k], arrs) for k in keys}
+    pk = Packed()
+    pk.trees = trees
+    if arrs:
+        sizes = np.fromiter((a.nbytes for a in arrs), np.int64, len(arrs))
+        offs = np.cumsum(sizes) - sizes
+        buf = np.frombuffer(bytearray(b''.join([a.tobytes() for a in arrs])), np.uint8)
+        fw = np.array([_field_words(a.dtype) for a in arrs], dtype=np.int64).reshape(-1, 4)
+        n = np.fromiter((len(a) for a in arrs), np.int64, len(arrs))
+    else:
+        offs = sizes = n = np.zeros(0, np.int64)
+        buf = np.zeros(0, np.uint8)
+        fw = np.zeros((0, 4), np.int64)
+    pk.specs = [(int(o), a.dtype, len(a)) for o, a in zip(offs, arrs)]
+    # per record: array id and first word
+    aid = np.repeat(np.arange(len(arrs)), n)
+    first = np.cumsum(n) - n
+    rec = offs[aid] // 8 + (np.arange(len(aid)) - first[aid]) * fw[aid, 0]
+    kp = fw[aid, 1]
+    ptr = np.repeat(rec, kp) + (np.arange(int(kp.sum())) - np.repeat(np.cumsum(kp) - kp, kp))
+    w = buf.view(np.uint64)
+    v = w[ptr]
+    nz = v != 0
+    ptr, v = ptr[nz], v[nz]
+    if len(v) and (int(v.min()) < lo or int(v.max()) >= hi):
+        raise ValueError('plan record points outside the solve arena')
+    w[ptr] = v - np.uint64(lo)
+    pk.ptr = ptr
+    has_c = fw[aid, 2] >= 0
+    cidx = rec[has_c] + fw[aid[has_c], 2]
+    bits = w[cidx]
+    boxed = ((bits & np.uint64(0xFFF8000000000000)) == np.uint64(_QNAN)) & \
+        ((bits & np.uint64(_PAYLOAD)) != 0)
+    c = buf.view(np.float64)[cidx[~boxed]]
+    ok = np.zeros(len(c), bool)
+    for lit in _LITERAL_COEFS:
+        ok |= c == lit
+    if not ok.all():
+        # a coupling value that lost its tag would be shared by solves with
+        # other couplings
+        raise ValueError('plan mixer term has an untagged coefficient')
+    pk.cidx = cidx[boxed]
+    pk.ck = (bits[boxed] & np.uint64(_PAYLOAD)).astype(np.int64) - 1
+    has_o = fw[aid, 3] >= 0
+    oidx = 2 * rec[has_o] + fw[aid[has_o], 3]
+    iv = buf.view(np.int32)
+    o = iv[oidx]
+    pk.op = oidx[o >= 0]
+    iv[pk.op] -= np.int32(slot)
+    pk.buf = buf
+    for k in keys:
+        ph[k] = pk.views(buf, k)
+    ph['packed'] = pk
+    return pk
 
 
 def _cat(arrs, dtype):
@@ -197,60 +378,73 @@ def _cat(arrs, dtype):
     return np.concatenate([a.view(np.uint8) for a in arrs]).view(dtype)
 
 
+def _cat_copy(arrs, dtype):
+    """Concatenation that is always a new array (the caller edits it)."""
+    if not arrs:
+        return np.zeros(0, dtype)
+    if len(arrs) == 1:
+        return arrs[0].copy()
+    return _cat(arrs, dtype)
+
+
+def _rebase(arr, lens, counts, field):
+    """``arr[field] +=`` the exclusive prefix sum of ``counts`` (one entry per
+    part), each entry repeated over that part's ``lens`` records."""
+    counts = np.asarray(counts, dtype=np.int64)
+    arr[field] += np.repeat(np.cumsum(counts) - counts, lens).astype(arr.dtype[field])
+
+
+def _merge(parts, actives, relocs, rec_dt, sub_dt, sub_field, unit_field, units_of_last,
+           empty):
+    """Shared body of ``concat_gemm`` / ``concat_mix``: one byte-level
+    concatenation per record kind, then vectorised fix-ups of the prefix
+    fields (sub-record index ``sub_field`` and work-unit index ``unit_field``)
+    and of ``active``."""
+    if relocs is not None:
+        parts = [(relocate(a, r), relocate(b, r)) for (a, b), r in zip(parts, relocs)]
+    keep = [k for k, (a, _) in enumerate(parts) if len(a)]
+    if not keep:
+        return empty[0].copy(), empty[1].copy(), 0
+    rs = [parts[k][0] for k in keep]
+    ss = [parts[k][1] for k in keep]
+    lens = np.fromiter((len(a) for a in rs), np.int64, len(rs))
+    out = _cat_copy(rs, rec_dt)
+    last = np.cumsum(lens) - 1
+    units = units_of_last(out[last])     # work units of each part: its last record's end
+    _rebase(out, lens, [len(b) for b in ss], sub_field)
+    _rebase(out, lens, units, unit_field)
+    if actives is not None:
+        out['active'] = np.repeat(np.asarray([actives[k] for k in keep], dtype=np.int32), lens)
+    return out, _cat_copy(ss, sub_dt), int(units.sum())
+
+
+def _gemm_end(last):
+    return last['tile_begin'].astype(np.int64) + \
+        ((last['M'].astype(np.int64) + 63) // 64) * ((last['N'].astype(np.int64) + 63) // 64)
+
+
+def _mix_end(last):
+    return last['block_begin'].astype(np.int64) + \
+        (last['Dl'].astype(np.int64) * last['Dr'] + L.MIX_PAIRS - 1) // L.MIX_PAIRS
+
+
 def concat_gemm(parts, actives=None, relocs=None):
     """``parts``: list of ``(probs, segs)`` -> merged ``(probs, segs, ntiles)``.
     ``actives[k]`` (optional) overrides the ``active`` field of part k;
-    ``relocs[k]`` (optional) moves part k from relative to device addresses."""
-    ps, ss = [], []
-    tile_off = seg_off = 0
-    for k, (p, s) in enumerate(parts):
-        if len(p) == 0:
-            continue
-        rl = relocs[k] if relocs is not None else None
-        p = relocate(p, rl)
-        p['seg_begin'] += seg_off
-        p['tile_begin'] += tile_off
-        if actives is not None:
-            p['active'] = actives[k]
-        tile_off += gemm_tiles(parts[k][0])
-        seg_off += len(s)
-        ps.append(p)
-        ss.append(relocate(s, rl) if rl is not None else s)
-    if not ps:
-        return EMPTY_PROB.copy(), EMPTY_SEG.copy(), 0
-    segs = _cat(ss, GEMM_SEG)
-    return _cat(ps, GEMM_PROBLEM), (segs.copy() if len(ss) == 1 and relocs is None else segs), \
-        tile_off
+    ``relocs[k]`` (optional) moves part k from relative to device addresses
+    (parts given already relocated skip that)."""
+    return _merge(parts, actives, relocs, GEMM_PROBLEM, GEMM_SEG, 'seg_begin', 'tile_begin',
+                  _gemm_end, (EMPTY_PROB, EMPTY_SEG))
 
 
 def concat_mix(parts, actives=None, relocs=None):
-    js, ts = [], []
-    blk_off = term_off = 0
-    for k, (j, t) in enumerate(parts):
-        if len(j) == 0:
-            continue
-        rl = relocs[k] if relocs is not None else None
-        j = relocate(j, rl)
-        j['term_begin'] += term_off
-        j['block_begin'] += blk_off
-        if actives is not None:
-            j['active'] = actives[k]
-        blk_off += mix_blocks(parts[k][0])
-        term_off += len(t)
-        js.append(j)
-        ts.append(relocate(t, rl) if rl is not None else t)
-    if not js:
-        return EMPTY_JOB.copy(), EMPTY_TERM.copy(), 0
-    terms = _cat(ts, MIX_TERM)
-    return _cat(js, MIX_JOB), (terms.copy() if len(ts) == 1 and relocs is None else terms), \
-        blk_off
+    """``parts``: list of ``(jobs, terms)`` -> merged ``(jobs, terms, nblocks)``
+    (as ``concat_gemm``)."""
+    return _merge(parts, actives, relocs, MIX_JOB, MIX_TERM, 'term_begin', 'block_begin',
+                  _mix_end, (EMPTY_JOB, EMPTY_TERM))
 
 
 def concat_plain(parts, dtype, relocs=None):
     if relocs is not None:
-        parts = [relocate(p, rl) for p, rl in zip(parts, relocs) if len(p)]
-        return _cat(parts, dtype)
-    parts = [p for p in parts if len(p)]
-    if not parts:
-        return np.zeros(0, dtype)
-    return _cat(parts, dtype).copy() if len(parts) == 1 else _cat(parts, dtype)
+        parts = [relocate(p, rl) for p, rl in zip(parts, relocs)]
+    return _cat_copy([p for p in parts if len(p)], dtype)

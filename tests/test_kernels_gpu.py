@@ -333,7 +333,7 @@ def test_lanczos_dense_symmetric(gpu):
         keep += [Hd, V, w, x, init, out]
         g2.add(w.data_ptr(), n, 1, 1, [(Hd.data_ptr(), n, 0, x.data_ptr(), 1, 0, n, 1.0)], active=k)
         solves.append((V.data_ptr(), w.data_ptr(), x.data_ptr(), init.data_ptr(), out.data_ptr(), n,
-                       n, min(ncv, n), 0, 1e-12 / 1e-4, 1e-4, 0.0, 0.0))
+                       n, min(ncv, n), 0, 1e-12 / 1e-4, 1e-4, 0.0, 0, 0))
     lz = np.array(solves, dtype=L.LANCZOS_SOLVE)
     ch = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
     lz['chunk_begin'] = np.concatenate([[0], np.cumsum(ch)[:-1]])
@@ -355,3 +355,97 @@ def test_lanczos_dense_symmetric(gpu):
         assert abs(res[k]['theta'] - e0) < 1e-9 * max(1, abs(e0)), (n, res[k], e0)
         assert abs(abs(out @ v0) - 1) < 1e-8, n
         assert np.linalg.norm(H @ out - e0 * out) < 1e-6 * max(1, abs(e0))
+
+
+def test_lanczos_multi_precision(gpu):
+    """One tnx_lanczos_run_multi call over solves whose matvecs run at two
+    precisions (an f64 and an f32 matvec plan, per-solve x/w dtypes): every
+    solve's result is bit-identical to a separate single-precision call on
+    its own group, and matches numpy.linalg.eigh (1e-9 for f64, the f32
+    matvec floor for f32)."""
+    import torch
+    from tnalg_amd import _lib as L
+    from tnalg_amd import records as R
+    lib = L.load()
+    rng = np.random.RandomState(5)
+    sizes = [16, 300, 2000, 4100, 64]
+    dts = [L.F64, L.F32, L.F64, L.F32, L.F32]
+    ncv = 20
+    tdt = {L.F64: torch.float64, L.F32: torch.float32}
+    bufs, refs = [], []
+    for k, (n, dt) in enumerate(zip(sizes, dts)):
+        A = rng.randn(n, n)
+        diag = np.linspace(0.0, 10.0, n)
+        diag[0] = -2.0
+        H = 0.002 * (A + A.T) / 2 + np.diag(diag)
+        w_ref, v_ref = np.linalg.eigh(H)
+        refs.append((w_ref[0], v_ref[:, 0]))
+        bufs.append(dict(H=_dev(H, tdt[dt]), V=torch.zeros((ncv + 1) * n, dtype=torch.float64,
+                                                              device='cuda'),
+                         w=torch.zeros(n, dtype=tdt[dt], device='cuda'),
+                         x=torch.zeros(n, dtype=tdt[dt], device='cuda'), init=_dev(rng.randn(n)),
+                         out=torch.zeros(n, dtype=torch.float64, device='cuda')))
+
+    def run(idx, multi):
+        """idx: solve indices; multi: one call with a plan per dtype, else
+        the call's own dtypes (all idx share one dtype)."""
+        plans, keep = [], []
+        for dt, mode in ((L.F64, L.GEMM_F64), (L.F32, L.GEMM_F32)):
+            g = R.GemmList()
+            for pos, k in enumerate(idx):
+                if dts[k] == dt:
+                    b, n = bufs[k], sizes[k]
+                    g.add(b['w'].data_ptr(), n, 1, 1,
+                          [(b['H'].data_ptr(), n, 0, b['x'].data_ptr(), 1, 0, n, 1.0)],
+                          active=pos)
+            p, s = g.arrays()
+            if len(p):
+                dp, ds = _upload([p, s])
+                keep += [dp, ds]
+                plans.append(L.MatvecPlan(0, 0, 0, 0, 0, 0, 0, 0, dp.data_ptr(), ds.data_ptr(),
+                                          len(p), R.gemm_tiles(p), 0, 0, mode))
+        recs = []
+        for k in idx:
+            b, n = bufs[k], sizes[k]
+            floor = 2e-6 if dts[k] == L.F32 else 0.0
+            recs.append((b['V'].data_ptr(), b['w'].data_ptr(), b['x'].data_ptr(),
+                         b['init'].data_ptr(), b['out'].data_ptr(), n, n, min(ncv, n), 0,
+                         1e-12 / 1e-4, 1e-4, floor, dts[k], dts[k]))
+        lz = np.array(recs, dtype=L.LANCZOS_SOLVE)
+        ch = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
+        lz['chunk_begin'] = np.concatenate([[0], np.cumsum(ch)[:-1]])
+        nch = int(ch.sum())
+        (dlz,) = _upload([lz])
+        state = torch.zeros(int(lib.tnx_lanczos_state_bytes(len(idx), nch, ncv)),
+                            dtype=torch.uint8, device='cuda')
+        res = np.zeros(len(idx), dtype=L.LANCZOS_RESULT)
+        if multi:
+            arr = (L.MatvecPlan * len(plans))(*plans)
+            rc = lib.tnx_lanczos_run_multi(arr, len(plans), _vp(dlz.data_ptr()), len(idx), nch,
+                                           ncv, 50, 1, _vp(state.data_ptr()),
+                                           _vp(res.ctypes.data), _stream())
+        else:
+            assert len(plans) == 1
+            dt = dts[idx[0]]
+            rc = lib.tnx_lanczos_run_ex(plans[0], _vp(dlz.data_ptr()), len(idx), nch, ncv, 50, 1,
+                                        dt, dt, _vp(state.data_ptr()), _vp(res.ctypes.data),
+                                        _stream())
+        assert rc == 0, (rc, lib.tnx_last_error())
+        torch.cuda.synchronize()
+        return res, [bufs[k]['out'].cpu().numpy() for k in idx]
+
+    print('lanczos multi sizes', sizes, flush=True)
+    allidx = list(range(len(sizes)))
+    res_m, out_m = run(allidx, True)
+    for dt in (L.F64, L.F32):
+        idx = [k for k in allidx if dts[k] == dt]
+        res_s, out_s = run(idx, False)
+        for pos, k in enumerate(idx):
+            assert res_s[pos]['theta'] == res_m[k]['theta'], (k, res_s[pos], res_m[k])
+            assert res_s[pos]['iters'] == res_m[k]['iters'], k
+            assert np.array_equal(out_s[pos], out_m[k]), k
+    for k in allidx:
+        e0, v0 = refs[k]
+        tol = 1e-9 if dts[k] == L.F64 else 2e-5
+        assert abs(res_m[k]['theta'] - e0) < tol * max(1, abs(e0)), (k, res_m[k], e0)
+        assert abs(abs(out_m[k] @ v0) - 1) < (1e-8 if dts[k] == L.F64 else 1e-3), k

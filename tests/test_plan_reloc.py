@@ -57,8 +57,7 @@ def _records(ph):
         elif isinstance(obj, (tuple, list)):
             for k, o in enumerate(obj):
                 walk(path + (k,), o)
-    for key in sorted(k for k in ph if k not in ('dims_after', 'uid', 'prec', 'mv_flops',
-                                                  'svd_n', 'lm_bond')):
+    for key in sorted(k for k in ph if k not in R.PLAN_META):
         walk((key,), ph[key])
     return out
 
@@ -96,6 +95,11 @@ def test_relocated_plan_equals_own_plan(lattice, kw, kw_b, prec):
             moved = R.relocate(rel, b.reloc)
             want = R.relocate(absr, (0, 0, b.T.coef_values))   # b's coefficient values
             assert moved.tobytes() == want.tobytes(), (st, path)
+            # the packed relocation that steps use gives the same records
+            got = b.part(ph_rel, path[0])
+            for k in path[1:]:
+                got = got[k]
+            assert got.tobytes() == want.tobytes(), (st, path)
             checked += 1
     assert checked > 0
 
@@ -120,6 +124,39 @@ def test_relocation_keeps_null_and_identity():
     m.add(4096, L.F64, 1, 2, 1, [(8192, L.F64, 2.0, 3)])
     with pytest.raises(ValueError):   # an untagged coupling value cannot be shared
         R.make_relative({'m': m.arrays()}, 4000, 9000, 0)
+    with pytest.raises(ValueError):
+        R.pack_relative({'m': m.arrays()}, 4000, 9000, 0)
+
+
+def test_packed_plan_relocation():
+    """Packing: nulls, identity operators and literal coefficients stay, every
+    other address, operator index and tagged coefficient moves; empty phases
+    and nested stage lists keep their shape."""
+    g = R.GemmList()
+    g.add(4096, 2, 2, 2, [(8192, 2, 0, 8200, 2, 0, 2, 1.0)])          # c0 = 0 (null)
+    m = R.MixList()
+    m.add(4104, L.F64, 1, 2, 1, [(8192, L.F64, 1.0, -1), (8192, L.F64, Coef(2.0, 1), 3),
+                                 (8800, L.F64, Coef(7.0, 0), 2)])
+    e = R.MixList()
+    ph = {'g': g.arrays(), 'm': m.arrays(), 'e': e.arrays(),
+          'cq': [('g', g.arrays()), ('c', np.zeros(0, L.CHOL_JOB))], 'uid': 5, 'prec': 'bf16'}
+    pk = R.pack_relative(ph, 4000, 9000, 2)
+    assert ph['packed'] is pk and ph['uid'] == 5
+    p, s = ph['g']
+    assert p['c'].tolist() == [96] and p['c0'].tolist() == [0] and s['b'].tolist() == [4200]
+    j, t = ph['m']
+    assert j['dst'].tolist() == [104] and t['op'].tolist() == [-1, 1, 0]
+    assert len(ph['e'][0]) == 0 and ph['cq'][0][0] == 'g' and len(ph['cq'][1][1]) == 0
+    buf = pk.relocate((100000, 10, np.array([5.0, -0.25])))
+    j2, t2 = pk.views(buf, 'm')
+    assert j2['dst'].tolist() == [100104] and t2['src'].tolist() == [104192, 104192, 104800]
+    assert t2['op'].tolist() == [-1, 11, 10]
+    assert t2['coef'].tolist() == [1.0, -0.25, 5.0]
+    kind, (p2, s2) = pk.views(buf, 'cq')[0]
+    assert p2['c0'].tolist() == [0] and s2['a'].tolist() == [104192]
+    assert t['src'].tolist() == [4192, 4192, 4800]    # the relative plan is unchanged
+    with pytest.raises(ValueError):   # an address outside the arena is a planning bug
+        R.pack_relative({'s': g.arrays()}, 9000, 9100, 0)
 
 
 def test_concat_relocates_each_part():
@@ -132,3 +169,161 @@ def test_concat_relocates_each_part():
     assert p['c'].tolist() == [(1 << 20) + 96, (1 << 30) + 96]
     assert s['a'].tolist() == [(1 << 20) + 4192, (1 << 30) + 4192]
     assert part[0]['c'][0] == 96      # the shared relative plan is left unchanged
+
+
+def _naive_concat(parts, rec_dt, sub_dt, sub_field, unit_field, units, actives=None):
+    """Part-by-part concatenation (the definition the batched form must match)."""
+    recs, subs = [], []
+    sub_off = unit_off = 0
+    for k, (a, b) in enumerate(parts):
+        if len(a) == 0:
+            continue
+        a = a.copy()
+        a[sub_field] += sub_off
+        a[unit_field] += unit_off
+        if actives is not None:
+            a['active'] = actives[k]
+        unit_off += units(parts[k][0])
+        sub_off += len(b)
+        recs.append(a)
+        subs.append(b)
+    if not recs:
+        return np.zeros(0, rec_dt), np.zeros(0, sub_dt), 0
+    return np.concatenate(recs), np.concatenate(subs), unit_off
+
+
+def test_batched_concat_matches_part_by_part():
+    rng = np.random.RandomState(3)
+    gparts, mparts = [], []
+    for k in range(9):
+        g = R.GemmList()
+        m = R.MixList()
+        for _ in range(rng.randint(0, 4)):   # some parts empty
+            M, N = int(rng.randint(1, 200)), int(rng.randint(1, 200))
+            g.add(4096 * k, M, N, M, [(8192, M, 0, 8200, 2, 0, int(rng.randint(1, 9)), 1.0)] *
+                  int(rng.randint(1, 4)))
+            Dl, Dr = int(rng.randint(1, 90)), int(rng.randint(1, 90))
+            m.add(4096 * k, L.F64, Dl, 2, Dr, [(8192, L.F64, 1.0, -1)] * int(rng.randint(1, 3)))
+        gparts.append(g.arrays())
+        mparts.append(m.arrays())
+    acts = list(range(100, 109))
+    for actives in (None, acts):
+        p, s, nt = R.concat_gemm(gparts, actives)
+        q, t, nq = _naive_concat(gparts, L.GEMM_PROBLEM, L.GEMM_SEG, 'seg_begin', 'tile_begin',
+                                 R.gemm_tiles, actives)
+        assert nt == nq and p.tobytes() == q.tobytes() and s.tobytes() == t.tobytes()
+        j, u, nb = R.concat_mix(mparts, actives)
+        q, t, nq = _naive_concat(mparts, L.MIX_JOB, L.MIX_TERM, 'term_begin', 'block_begin',
+                                 R.mix_blocks, actives)
+        assert nb == nq and j.tobytes() == q.tobytes() and u.tobytes() == t.tobytes()
+    assert R.concat_gemm([gparts[0]][:0])[2] == 0
+
+
+def test_assembled_step_matches_relocate_then_concat():
+    """A batch of solves of different structures at different steps: the
+    assembled descriptors equal relocating every part and concatenating part
+    by part, and the cached relocated parts are left unchanged."""
+    eng = CpuEngine(lowp=True)
+    solves = [Solve(eng, _para('chain', l=10, chi=16), seed=1),
+              Solve(eng, _para('chain', l=12, chi=300), seed=2),
+              Solve(eng, _para('chain', l=10, chi=16, jz=0.6), seed=3),
+              Solve(eng, _para('snakeJ1J2', square_width=3, square_height=4, chi=12), seed=4)]
+    for s in solves:
+        s.schedule_canonicalise()
+        s.schedule_sweep(0)
+    from tnalg_amd.engine import assemble_pre, assemble_lanczos
+    steps = 0
+    while all(s.queue for s in solves):
+        pairs = [(s, s.plan(s.queue.pop(0), 'bf16')) for s in solves]
+        arrs, meta = assemble_pre(pairs)
+        # the same phases, relocated part by part
+        for i, key in enumerate(('absorb', 'env_g1', 'env_gT')):
+            parts = [R.relocate_tree(ph[key], s.reloc) for s, ph in pairs if key in ph]
+            p, sg, nt = _naive_concat(parts, L.GEMM_PROBLEM, L.GEMM_SEG, 'seg_begin',
+                                      'tile_begin', R.gemm_tiles)
+            assert arrs[2 + 2 * i].tobytes() == p.tobytes() and meta['gemm'][i][1] == nt
+            assert arrs[3 + 2 * i].tobytes() == sg.tobytes()
+        for i, key in enumerate(('copy', 'env_lc', 'env_mix', 'cast')):
+            parts = [R.relocate_tree(ph[key], s.reloc) for s, ph in pairs if key in ph]
+            j, t, nb = _naive_concat(parts, L.MIX_JOB, L.MIX_TERM, 'term_begin', 'block_begin',
+                                     R.mix_blocks)
+            assert arrs[8 + 2 * i].tobytes() == j.tobytes() and meta['mix'][i][1] == nb
+            assert arrs[9 + 2 * i].tobytes() == t.tobytes()
+        group = [(s, ph) for s, ph in pairs if 'lz' in ph]
+        if group:
+            # two groups (as for two matvec precisions): the second group's
+            # actives continue the first's, the solve records follow group order
+            groups = [group[:1], group[1:]] if len(group) > 1 else [group]
+            la, lm = assemble_lanczos(groups)
+            first = 0
+            for k, g in enumerate(groups):
+                parts = [R.relocate_tree(ph['mv_g1'], s.reloc) for s, ph in g]
+                p, sg, nt = _naive_concat(parts, L.GEMM_PROBLEM, L.GEMM_SEG, 'seg_begin',
+                                          'tile_begin', R.gemm_tiles,
+                                          list(range(first, first + len(g))))
+                assert la[3 + 6 * k].tobytes() == p.tobytes() and lm['groups'][k][2][1] == nt
+                first += len(g)
+            lz = np.concatenate([R.relocate(ph['lz'], s.reloc) for s, ph in group])
+            assert (la[2]['V'] == lz['V']).all() and la[2]['chunk_begin'][0] == 0
+            assert (la[2]['x_dtype'] == L.BF16).all() and (la[2]['w_dtype'] == L.F32).all()
+        # a cached relocated part is returned as is the next time
+        s, ph = pairs[0]
+        key = next(k for k in ('env_g1', 'absorb', 'copy') if k in ph)
+        before = R.relocate_tree(ph[key], s.reloc)
+        got = s.part(ph, key)
+        assert got is s.part(ph, key)
+        assert all(a.tobytes() == b.tobytes() for a, b in zip(got, before))
+        steps += 1
+    assert steps > 5
+
+
+def test_plan_ahead_matches_inline_planning():
+    """The worker's next-step host work: each solve's planned-ahead step is
+    the plan inline planning returns, the solve's dims are left as they were,
+    and the speculative descriptor sets equal the ones assembled inline."""
+    from tnalg_amd.engine import Engine, assemble_lanczos, assemble_pre, _lanczos_groups
+
+    class AheadEngine(CpuEngine):
+        _plan_ahead = Engine._plan_ahead
+        _prec_of = Engine._prec_of
+
+        def __init__(self):
+            super().__init__(lowp=True)
+            self._resident = {}
+            self.precision_schedule = lambda sweep, nsweeps: ('bf16', 'fp32', 'fp64')[sweep]
+
+    eng = AheadEngine()
+    solves = [Solve(eng, _para('chain', l=10, chi=16, sweep_time=3), seed=1),
+              Solve(eng, _para('chain', l=8, chi=8, sweep_time=3), seed=2),
+              Solve(eng, _para('snakeJ1J2', square_width=3, square_height=4, chi=12,
+                               sweep_time=3), seed=3)]
+    for s in solves:
+        s.schedule_canonicalise()
+        for t in range(3):
+            s.schedule_sweep(t)
+    steps = 0
+    while all(s.queue for s in solves):
+        dims0 = [list(s.dims) for s in solves]
+        spec = eng._plan_ahead(solves)
+        assert [s.dims for s in solves] == dims0
+        pairs = []
+        for s in solves:
+            st, prec, ph = s._next
+            assert st is s.queue[0] and prec == eng._prec_of(s, st)
+            assert s.plan(st, prec) is ph          # inline planning: the same shared plan
+            assert s.dims == list(ph['dims_after'])
+            pairs.append((s, ph))
+            s.queue.pop(0)
+            s._next = None
+        arrs, meta = assemble_pre(pairs)
+        assert spec['pre'][1][1] == meta
+        assert all(a.tobytes() == b.tobytes() for a, b in zip(spec['pre'][1][0], arrs))
+        groups = _lanczos_groups(pairs)
+        if groups:
+            la, lm = assemble_lanczos(groups)
+            assert spec['lz'][1][1] == lm
+            assert all(a.tobytes() == b.tobytes() for a, b in zip(spec['lz'][1][0], la))
+        else:
+            assert 'lz' not in spec
+        steps += 1
+    assert steps > 20

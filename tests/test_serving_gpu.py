@@ -14,6 +14,9 @@ def _requests():
 
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
 def test_alone_equals_in_mix(gpu, precision):
+    """Each request alone (host work done in line) and in a mixed stream
+    (next-step host work planned ahead on a worker thread, one Lanczos call
+    over every precision of the step) gives bit-identical energies."""
     from tnalg_amd.dmrg import _schedule_from
     from tnalg_amd.engine import Engine
     from tnalg_amd.serving import SolveServer
@@ -23,7 +26,7 @@ def test_alone_equals_in_mix(gpu, precision):
     alone = []
     for k, (t, para, seed) in enumerate(reqs):
         print('alone request', k, 'N', para['l'], 'chi', para['chi'], flush=True)
-        eng = Engine(precision_schedule=sched, lowp=lowp)
+        eng = Engine(precision_schedule=sched, lowp=lowp, plan_ahead=False)
         srv = SolveServer(eng, hbm_budget=8 << 30, max_active=1)
         srv.submit(k, para, seed=seed, arrival=0.0)
         srv.run()
