@@ -40,6 +40,7 @@ def c2_para(n=64, chi=128, sweeps=8, jz=1.0, hz=0.0, tol=1e-10):
 
 _ENGINES = {}
 _PLAN_AHEAD = [True]   # --no-plan-ahead: next-step host work in line (for comparison)
+_CYCLE_STEPS = [True]  # serving: one Krylov cycle per Lanczos call (--no-cycle-steps: all)
 
 
 def engine_for(precision, ncv=20):
@@ -217,7 +218,8 @@ def serving_run(n_req, rate, max_active=64, seed=0):
     from tnalg_amd.engine import Engine
     from tnalg_amd.serving import SolveServer, poisson_xxz_stream
     reqs = poisson_xxz_stream(n_req, rate=rate, seed=seed)
-    eng = Engine(precision_schedule=_schedule_from('mixed'), lowp=True, plan_ahead=_PLAN_AHEAD[0])
+    eng = Engine(precision_schedule=_schedule_from('mixed'), lowp=True, plan_ahead=_PLAN_AHEAD[0],
+                 cycle_steps=_CYCLE_STEPS[0])
     free = torch.cuda.mem_get_info()[0]
     srv = SolveServer(eng, hbm_budget=int(0.6 * free), max_active=max_active)
     t0 = time.perf_counter()
@@ -235,6 +237,9 @@ def serving_run(n_req, rate, max_active=64, seed=0):
     summ['step_time_s'] = {'plan': c['t_plan'], 'pre_phases': c['t_pre'],
                            'lanczos_total': c['t_lz_total'], 'lanczos_calls': c['lz_time'],
                            'wait_for_plan_ahead': c['t_settle'], 'engine_steps': c['steps']}
+    summ['lanczos'] = {'calls': c['lanczos_calls'], 'matvecs': c['matvecs'],
+                       'resumed_cycles': c['resumes'], 'cycle_steps': eng.cycle_steps,
+                       'plan_ahead': eng.plan_ahead}
     summ['requests'] = n_req
     summ['arrival_rate_per_s'] = rate
     summ['max_active'] = max_active
@@ -345,6 +350,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-plan-ahead', action='store_true',
                     help='do the next step\'s host work in line instead of on a worker thread')
+    ap.add_argument('--no-cycle-steps', action='store_true',
+                    help='serving: restart unconverged local solves inside the Lanczos call')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--serving-requests', type=int, default=0)
     ap.add_argument('--serving-rate', type=float, default=4.0)
@@ -357,6 +364,7 @@ def main():
                          'size (for rocprofv3 kernel-trace / counter runs)')
     args = ap.parse_args()
     _PLAN_AHEAD[0] = not args.no_plan_ahead
+    _CYCLE_STEPS[0] = not args.no_cycle_steps
     if args.gpus != 1:
         print('bench.py measures one GPU only (one-GPU build); --gpus %d is not supported'
               % args.gpus, file=sys.stderr)

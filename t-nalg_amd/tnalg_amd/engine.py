@@ -226,8 +226,10 @@ def random_mps_host(n, d, chi, seed):
 
 
 class Step:
-    """One schedule entry of a solve: kind in {'move', 'solve', 'svdmove'},
-    site, direction (+1: centre moves right, -1: left)."""
+    """One schedule entry of a solve: kind in {'move', 'solve', 'svdmove',
+    'resume'}, site, direction (+1: centre moves right, -1: left).  'resume'
+    continues the local eigensolve of a 'solve' step from its Ritz vector
+    (the MPS site tensor) when one Krylov cycle did not converge."""
     __slots__ = ('kind', 'site', 'dir', 'sweep')
 
     def __init__(self, kind, site, direction, sweep=-1):
@@ -322,6 +324,8 @@ class Solve:
         # a solve revisits the same plans every sweep
         self.moved = {}
         self._next = None   # (step, precision, plan) planned ahead
+        self.cycles = 0       # unconverged Krylov cycles of the current local solve
+        self.resume = False   # the last cycle did not converge: resume next step
 
     @property
     def lm(self):
@@ -423,23 +427,69 @@ class Solve:
         if hit is not None:
             self.dims = list(hit['dims_after'])
             return hit
-        ph = self._build_plan(st, prec)
-        R.pack_relative(ph, self.base, self.base + self.layout['bytes'], self.ops_slot)
+        lo, hi = self.base, self.base + self.layout['bytes']
+        if st.kind != 'solve':
+            ph = self._build_plan(st, prec)
+            R.pack_relative(ph, lo, hi, self.ops_slot)
+            cache[key] = ph
+            return ph
+        # a solve step = the gauge move and environment update, which do not
+        # depend on the matvec precision (built once and shared by the plans
+        # of every precision), + the precision's cast and local records
+        ckey = (self.sig, 'solve-env', st.site, st.dir, tuple(self.dims))
+        com = cache.get(ckey)
+        if com is None:
+            com = {}
+            self._plan_common(com, st)
+            com['dims_after'] = list(self.dims)
+            R.pack_relative(com, lo, hi, self.ops_slot)
+            cache[ckey] = com
+        else:
+            self.dims = list(com['dims_after'])
+        loc = {}
+        self._plan_precision(loc, st, prec)
+        R.pack_relative(loc, lo, hi, self.ops_slot)
+        meta = {k: loc[k] for k in ('prec', 'mv_flops')}
+        meta['dims_after'] = list(self.dims)
+        meta['uid'] = next(_PLAN_UID)
+        ph = R.merge_packed([com, loc], meta)
         cache[key] = ph
         return ph
+
+    def _plan_common(self, ph, st):
+        """Records of a 'move' or 'solve' step that do not depend on the
+        matvec precision: gauge move of the previous centre, environment
+        update through it."""
+        q = st.site if st.kind == 'move' else st.site - st.dir
+        self._plan_gauge(ph, q, st.dir, copy_back=(st.kind == 'move'))
+        self._plan_env(ph, q, st.dir)
+
+    def _plan_precision(self, ph, st, prec):
+        """Records of a 'move' or 'solve' step that depend on the precision:
+        low-precision copy of the new environment block, local eigensolve."""
+        q = st.site if st.kind == 'move' else st.site - st.dir
+        if self.engine.uses_lowp and prec != 'fp64':
+            self._plan_cast_block(ph, q + 1 if st.dir > 0 else q, st.dir > 0, PREC[prec][0])
+        if st.kind == 'solve':
+            self._plan_local(ph, st.site, prec)
 
     def _build_plan(self, st, prec):
         ph = {}
         if st.kind in ('move', 'solve'):
-            q = st.site if st.kind == 'move' else st.site - st.dir
-            self._plan_gauge(ph, q, st.dir, copy_back=(st.kind == 'move'))
-            self._plan_env(ph, q, st.dir)
-            if self.engine.uses_lowp and prec != 'fp64':
-                self._plan_cast_block(ph, q + 1 if st.dir > 0 else q, st.dir > 0, PREC[prec][0])
-            if st.kind == 'solve':
-                self._plan_local(ph, st.site, prec)
+            self._plan_common(ph, st)
+            self._plan_precision(ph, st, prec)
         elif st.kind == 'svdmove':
             self._plan_svd(ph, st.site, st.dir)
+        elif st.kind == 'resume':
+            # start vector <- the Ritz vector the last cycle wrote to the site
+            # (the same vector, and so the same normalised V[0], as a restart
+            # inside one Lanczos call)
+            p = st.site
+            mx = R.MixList()
+            mx.add(self.init.data_ptr(), L.F64, 1, 1, self.dims[p] * self.d * self.dims[p + 1],
+                   [(self.mps[p].data_ptr(), L.F64, 1.0, -1)])
+            ph['copy'] = mx.arrays()
+            self._plan_local(ph, p, prec)
         ph['dims_after'] = list(self.dims)
         ph['uid'] = next(_PLAN_UID)
         return ph
@@ -1013,7 +1063,8 @@ class Engine:
     the Lanczos call returns."""
 
     def __init__(self, precision_schedule=None, ncv=20, max_restarts=20, min_iters=1,
-                 device='cuda:0', lowp=True, ops_capacity=4096, plan_ahead=True):
+                 device='cuda:0', lowp=True, ops_capacity=4096, plan_ahead=True,
+                 cycle_steps=False):
         require_gpu()
         self.lib = L.load()
         self.device = torch.device(device)
@@ -1040,8 +1091,13 @@ class Engine:
         self._arena_pool = {}   # bytes -> freed arenas (reused by the next solve of that size)
         self.counters = {'matvec_flops': 0.0, 'matvecs': 0, 'lanczos_calls': 0, 'steps': 0,
                          'lz_time': 0.0, 't_plan': 0.0, 't_pre': 0.0, 't_lz_total': 0.0,
-                         'lz_iters': 0, 't_settle': 0.0, 'ahead_used': 0}
+                         'lz_iters': 0, 't_settle': 0.0, 'ahead_used': 0, 'resumes': 0}
         self.plan_ahead = bool(plan_ahead)
+        # cycle_steps: each Lanczos call runs one Krylov cycle; a solve that
+        # has not converged continues from its Ritz vector in the next step
+        # ('resume', the same iterates as a restart inside the call) while
+        # the others move on, so no call waits for the batch's slowest solve
+        self.cycle_steps = bool(cycle_steps)
         self._ahead_pool = None
         self._ahead = None     # future of the worker's next-step host work
         self._spec = None      # its speculative descriptor sets
@@ -1049,7 +1105,7 @@ class Engine:
     # ------------------------------------------------------ planning ahead
     def _prec_of(self, s, st):
         """Matvec precision of step ``st`` of solve ``s`` (moves: f64)."""
-        if st.kind != 'solve':
+        if st.kind not in ('solve', 'resume'):
             return 'fp64'
         return self.precision_schedule(max(st.sweep, 0), s.para.get('sweep_time', 1))
 
@@ -1288,8 +1344,9 @@ class Engine:
         if ahead:
             self._start_ahead(ahead)
         t0 = time.perf_counter()
+        restarts = 0 if self.cycle_steps else self.max_restarts
         rc = self.lib.tnx_lanczos_run_multi(plans, len(groups), ctypes_void(ptrs[2]), len(flat),
-                                            nchunks, self.ncv, self.max_restarts, self.min_iters,
+                                            nchunks, self.ncv, restarts, self.min_iters,
                                             ctypes_void(self.lz_state.data_ptr()),
                                             ctypes_void(res.ctypes.data), self.stream())
         self.counters['lz_time'] += time.perf_counter() - t0
@@ -1297,13 +1354,20 @@ class Engine:
             L.check(rc, 'tnx_lanczos_run')
         self.counters['lanczos_calls'] += 1
         for (s, ph), r in zip(flat, res):
+            it = int(r['iters'])
+            s.stats['matvecs'] += it
+            self.counters['matvecs'] += it
+            self.counters['matvec_flops'] += ph['mv_flops'] * it
+            if not r['converged'] and self.cycle_steps and s.cycles < self.max_restarts:
+                s.cycles += 1
+                s.resume = True   # the step re-queues the eigensolve
+                self.counters['resumes'] += 1
+                continue
+            s.cycles = 0
             s.last_theta = float(r['theta'])
             s.thetas.append(s.last_theta)
-            s.stats['matvecs'] += int(r['iters'])
             if not r['converged']:
                 s.stats['unconverged'] += 1
-            self.counters['matvecs'] += int(r['iters'])
-            self.counters['matvec_flops'] += ph['mv_flops'] * int(r['iters'])
 
     def step(self, solves=None):
         """Advance every solve with a queued entry by one step."""
@@ -1336,6 +1400,7 @@ class Engine:
         # device per combination of (plan, solve) (steps repeat every sweep)
         rkey = ('pre',) + tuple((ph['uid'],) + s.reloc_key for s, _, ph in plans)
         ent = self._resident_get(rkey)
+        host_miss = ent is None   # a new batch composition: host work worth overlapping
         if ent is None:
             got = spec.get('pre') if spec else None
             ent = self._resident_put(rkey, *(got[1] if got and got[0] == rkey else
@@ -1396,8 +1461,13 @@ class Engine:
         t_c = time.perf_counter()
         groups = _lanczos_groups([(s, ph) for s, _, ph in plans])
         if groups:
-            # the next step's host work overlaps this Lanczos call
-            ahead = [s for s, _, _ in plans if s.queue] if self.plan_ahead else None
+            # the next step's host work overlaps this Lanczos call, when this
+            # step needed host work (a batch whose descriptor sets are all
+            # resident repeats a known sequence and has none to overlap)
+            host_miss = host_miss or _lanczos_key(groups) not in self._resident
+            ahead = None
+            if self.plan_ahead and host_miss:
+                ahead = [s for s, _, _ in plans if s.queue]
             self._lanczos(groups, spec, ahead)
         t_d = time.perf_counter()
         self.counters['t_plan'] += t_b - t_a
@@ -1405,7 +1475,10 @@ class Engine:
         self.counters['t_lz_total'] += t_d - t_c
         for s, st, ph in plans:
             s.stats['steps'] += 1
-            if st.kind == 'solve':
+            if s.resume:
+                s.resume = False
+                s.queue.insert(0, Step('resume', st.site, st.dir, st.sweep))
+            if st.kind in ('solve', 'resume'):
                 s.center = st.site
             elif st.kind in ('move', 'svdmove'):
                 s.center = st.site + st.dir

@@ -306,6 +306,48 @@ class Packed:
         return buf
 
 
+def _shift_tree(tree, d):
+    tt = type(tree)
+    if tt is _Leaf:
+        return _Leaf(tree.i + d)
+    if tt is tuple:
+        return tuple([_shift_tree(t, d) for t in tree])
+    if tt is list:
+        return [_shift_tree(t, d) for t in tree]
+    return tree
+
+
+def merge_packed(subs, meta):
+    """One plan from packed relative sub-plans with disjoint phases (e.g. the
+    precision-independent gauge/environment records shared by every
+    precision, and one precision's local records): buffers concatenated,
+    indices and leaves shifted.  ``meta``: the plan's non-record entries."""
+    pk = Packed()
+    pks = [s['packed'] for s in subs]
+    offs, off = [], 0
+    for p in pks:
+        offs.append(off)
+        off += len(p.buf)
+    pk.buf = np.concatenate([p.buf for p in pks]) if pks else np.zeros(0, np.uint8)
+    pk.ptr = np.concatenate([p.ptr + o // 8 for p, o in zip(pks, offs)])
+    pk.cidx = np.concatenate([p.cidx + o // 8 for p, o in zip(pks, offs)])
+    pk.ck = np.concatenate([p.ck for p in pks])
+    pk.op = np.concatenate([p.op + o // 4 for p, o in zip(pks, offs)])
+    pk.specs, pk.trees = [], {}
+    for p, o in zip(pks, offs):
+        d = len(pk.specs)
+        pk.specs += [(so + o, dt, n) for so, dt, n in p.specs]
+        for k, t in p.trees.items():
+            if k in pk.trees:
+                raise ValueError('sub-plans share the phase %r' % k)
+            pk.trees[k] = _shift_tree(t, d)
+    ph = dict(meta)
+    for k in pk.trees:
+        ph[k] = pk.views(pk.buf, k)
+    ph['packed'] = pk
+    return ph
+
+
 def pack_relative(ph, lo, hi, slot):
     """Plan ``ph`` packed and made relative to the arena ``[lo, hi)`` and
     operator slot ``slot``; its phases are replaced by views of the packed

@@ -5,6 +5,8 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
+NCV = 8   # short Krylov cycles, so that local solves restart
+
 
 def _requests():
     from tnalg_amd.serving import poisson_xxz_stream
@@ -14,9 +16,11 @@ def _requests():
 
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
 def test_alone_equals_in_mix(gpu, precision):
-    """Each request alone (host work done in line) and in a mixed stream
-    (next-step host work planned ahead on a worker thread, one Lanczos call
-    over every precision of the step) gives bit-identical energies."""
+    """Each request alone (host work done in line, restarts inside the
+    Lanczos call) and in a mixed stream (next-step host work planned ahead on
+    a worker thread, one Lanczos call over every precision of the step; with
+    cycle_steps, an unconverged local solve resumes from its Ritz vector in
+    the next step) gives bit-identical energies."""
     from tnalg_amd.dmrg import _schedule_from
     from tnalg_amd.engine import Engine
     from tnalg_amd.serving import SolveServer
@@ -26,21 +30,23 @@ def test_alone_equals_in_mix(gpu, precision):
     alone = []
     for k, (t, para, seed) in enumerate(reqs):
         print('alone request', k, 'N', para['l'], 'chi', para['chi'], flush=True)
-        eng = Engine(precision_schedule=sched, lowp=lowp, plan_ahead=False)
+        eng = Engine(precision_schedule=sched, lowp=lowp, ncv=NCV, plan_ahead=False)
         srv = SolveServer(eng, hbm_budget=8 << 30, max_active=1)
         srv.submit(k, para, seed=seed, arrival=0.0)
         srv.run()
         alone.append(srv.finished[0].energy)
-    print('mixed stream of', len(reqs), flush=True)
-    eng = Engine(precision_schedule=sched, lowp=lowp)
-    srv = SolveServer(eng, hbm_budget=8 << 30, max_active=4)
     import time
-    t0 = time.perf_counter()
-    for k, (t, para, seed) in enumerate(reqs):
-        srv.submit(k, para, seed=seed, arrival=t0 + 0.002 * k)
-    summ = srv.run()
-    assert summ['completed'] == len(reqs)
-    assert summ['peak_active'] > 1
-    mixed = {r.rid: r.energy for r in srv.finished}
-    for k in range(len(reqs)):
-        assert mixed[k] == alone[k], (k, mixed[k], alone[k])
+    for cycle in (False, True):
+        print('mixed stream of', len(reqs), 'cycle_steps', cycle, flush=True)
+        eng = Engine(precision_schedule=sched, lowp=lowp, ncv=NCV, cycle_steps=cycle)
+        srv = SolveServer(eng, hbm_budget=8 << 30, max_active=4)
+        t0 = time.perf_counter()
+        for k, (t, para, seed) in enumerate(reqs):
+            srv.submit(k, para, seed=seed, arrival=t0 + 0.002 * k)
+        summ = srv.run()
+        assert summ['completed'] == len(reqs)
+        assert summ['peak_active'] > 1
+        assert (eng.counters['resumes'] > 0) == cycle
+        mixed = {r.rid: r.energy for r in srv.finished}
+        for k in range(len(reqs)):
+            assert mixed[k] == alone[k], (cycle, k, mixed[k], alone[k])
