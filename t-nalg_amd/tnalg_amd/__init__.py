@@ -5,15 +5,17 @@ Public API (mirrors the reference's names):
     dmrg_finite_size(para=None, ...) -> (ob, A, info, para)
     parameters.generate_parameters_dmrg / make_consistent_parameter_dmrg / ...
     MpsOpenBoundaryClass  (the returned ``A``)
+    save_pr / load_pr     (``.pr`` result files, library/BasicFunctions.py:29-95)
     Engine / Solve        (batched device engine)
     serving.SolveServer   (continuous-batching request loop)
 The compute runs in ``libtnalg_hip.so`` (HIP, gfx950); there is no CPU path.
 """
 from . import parameters  # noqa: F401
+from .basic_functions import load_pr, save_pr  # noqa: F401
 from .parameters import generate_parameters_dmrg, make_consistent_parameter_dmrg  # noqa: F401
 
 __all__ = ['parameters', 'generate_parameters_dmrg', 'make_consistent_parameter_dmrg',
-           'dmrg_finite_size', 'Engine', 'MpsOpenBoundaryClass']
+           'dmrg_finite_size', 'Engine', 'MpsOpenBoundaryClass', 'save_pr', 'load_pr']
 
 
 def __getattr__(name):

@@ -92,17 +92,26 @@ def _schedule_from(precision):
     raise ValueError('unknown precision %r' % (precision,))
 
 
+_REBUILD_ENGINE = []   # engine that re-creates device solves of saved/cleaned MPSs
+
+
 class MpsOpenBoundaryClass:
     """Result MPS of the drop-in (the reference's ``MpsOpenBoundaryClass``,
     ``library/MPSClass.py:28-108``): host copies of the tensors plus a handle
-    to the device solve, which the observation methods use."""
+    to the device solve, which the observation methods use.
+
+    It pickles (``save_pr``) as host data only. After loading, or after
+    ``clean_to_save``, the first method that needs the device re-creates the
+    device solve from the host tensors and the stored ``para``."""
 
     def __init__(self, solve, engine, spin='half'):
         self._solve = solve
         self._engine = engine
+        self._para = solve.para
         self.spin = spin
         self.decomp_way = 'qr'
         self.refresh_from_device()
+        self.ent = np.zeros((self.length - 1, 1))
 
     def refresh_from_device(self):
         s = self._solve
@@ -116,14 +125,47 @@ class MpsOpenBoundaryClass:
         self.orthogonality[s.center + 1:] = 1
         self.operators = [np.asarray(o) for o in s.para['op']]
         self.lm = [np.asarray(x) for x in s.lm]
-        self.ent = np.zeros((s.n - 1, 1))
+
+    def _device(self):
+        """The device solve holding this MPS (re-created from the host
+        tensors when it was released or never existed in this process)."""
+        if self._solve is None:
+            from .engine import Engine
+            if self._para is None:
+                raise RuntimeError('this MPS has no parameters to rebuild its device state from')
+            if not _REBUILD_ENGINE:
+                _REBUILD_ENGINE.append(Engine(lowp=False, plan_ahead=False))
+            eng = _REBUILD_ENGINE[0]
+            s = eng.add_solve(self._para, mps=self.mps)
+            s.center = self.center
+            s._sched_center = self.center
+            for b, x in enumerate(self.lm):   # spectra already known stay known
+                s._lm[b] = np.asarray(x, dtype=np.float64).copy()
+            self._solve, self._engine = s, eng
+        return self._solve
+
+    def calculate_entanglement_spectrum(self, if_fast=True):
+        """``MPSClass.py:965-985``: SVD centre moves that fill every empty
+        ``lm[b]`` (``if_fast``; otherwise every bond), back to the centre.
+        Changes the gauge of ``mps``, not the state."""
+        s = self._device()
+        if not if_fast:
+            for b in range(s.n - 1):
+                s._lm[b] = np.zeros(0)
+        schedule_spectrum(s)
+        self._engine.run_queues([s])
+        self.refresh_from_device()
+
+    def calculate_entanglement_entropy(self):
+        """``MPSClass.py:987-992``: ``ent[b] = -1`` where no spectrum is known."""
+        for i in range(self.length - 1):
+            self.ent[i] = -1 if self.lm[i].size == 0 else entanglement_entropy(self.lm[i])
 
     # -- observation methods (MPSClass.py:1105-1212) --------------------------
     def _measure(self, one, two):
         from .observe import measure
-        if self._solve is None:
-            raise RuntimeError('the device state of this MPS was released (clean_to_save)')
-        return measure(self._engine, [self._solve], [(one, two)])[0]
+        s = self._device()
+        return measure(self._engine, [s], [(one, two)])[0]
 
     def observation_s1(self, inputs):
         sn, position = inputs
@@ -181,10 +223,12 @@ class MpsOpenBoundaryClass:
         return v.reshape(-1)
 
     def clean_to_save(self):
-        """Release the device state (``MPSClass.py:1396-1405`` drops caches)."""
+        """Release the device state (``MPSClass.py:1396-1405`` drops caches);
+        a later device method re-creates it from the host tensors."""
         if self._solve is not None and self._engine is not None:
             self._engine.remove_solve(self._solve)
         self._solve = None
+        self._engine = None
 
     def __getstate__(self):
         st = dict(self.__dict__)
@@ -295,8 +339,7 @@ def dmrg_finite_size(para=None, seed=None, precision='fp64', ncv=20, engine=None
     schedule_spectrum(s)
     eng.run_queues([s])
     A = MpsOpenBoundaryClass(s, eng, spin=para.get('spin', 'half'))
-    for i in range(s.n - 1):
-        A.ent[i] = -1 if A.lm[i].size == 0 else entanglement_entropy(A.lm[i])
+    A.calculate_entanglement_entropy()
     ob['corr_x'] = A.observe_correlators_from_middle(1, 1)
     ob['corr_z'] = A.observe_correlators_from_middle(3, 3)
     info['t_cost'] = time.time() - t_start

@@ -141,3 +141,33 @@ def test_mixed_precision_reaches_fp64(gpu, precision):
     obm, _, _, _ = dmrg_finite_size(_para(c, 32, 8), seed=4, precision=precision)
     e64, em = float(ob64['e_per_site'][0]), float(obm['e_per_site'][0])
     assert abs(e64 - em) < 1e-9 * abs(e64), (e64, em)
+
+
+def test_saved_result_reloads(gpu, tmp_path):
+    """``save_pr`` / ``load_pr`` of ``(ob, A, info, para)`` (the reference's
+    result cache, ``library/BasicFunctions.py:29-95``): the loaded ``A``
+    rebuilds its device state from the host tensors and gives the same
+    observables; ``calculate_entanglement_spectrum(if_fast=False)`` on it
+    recomputes every spectrum with SVD moves and gives the same spectra and
+    entropies; an ``A`` released by ``clean_to_save`` still observes."""
+    from tnalg_amd import dmrg_finite_size, load_pr, save_pr
+    c = model.chain_couplings(10, 1.0, 0.9, hz=0.15)
+    para = _para(c, 16, 3)
+    print('case: save/load N=10 chi=16', flush=True)
+    ob, A, info, para = dmrg_finite_size(para, seed=4)
+    mz = A.observe_magnetization(3)
+    eb = A.observe_bond_energy(para['index2'], para['coeff2'])
+    save_pr(str(tmp_path), 'run.pr', (ob, A, info, para), ('ob', 'A', 'info', 'para'))
+    B, ob2 = load_pr(str(tmp_path / 'run.pr'), ('A', 'ob'))
+    assert B._solve is None and np.array_equal(ob2['eb_full'], ob['eb_full'])
+    assert np.allclose(B.observe_magnetization(3), mz, atol=1e-12)
+    assert np.allclose(B.observe_bond_energy(para['index2'], para['coeff2']), eb, atol=1e-12)
+    B.calculate_entanglement_spectrum(if_fast=False)
+    B.calculate_entanglement_entropy()
+    for b in range(9):
+        assert np.allclose(B.lm[b], A.lm[b], atol=1e-10), b
+    assert np.allclose(B.ent, A.ent, atol=1e-9)
+    assert abs(B.norm_mps() - 1) < 1e-12
+    A.clean_to_save()
+    assert A._solve is None
+    assert np.allclose(A.observe_magnetization(3), mz, atol=1e-12)
