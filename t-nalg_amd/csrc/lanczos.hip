@@ -9,7 +9,7 @@
 // block of the next kernel on the stream (ordered after it) sums the
 // partials of its own solve in the same fixed order.  A Lanczos step is
 //   matvec (3 launches) + dots + update/dots/norm
-//   + update/normalise (block 0 of each solve: tridiagonal solve,
+//   + update/normalise (plus one block per solve: tridiagonal solve,
 //     convergence).
 // The Krylov basis and all reductions are f64 (two-pass classical
 // Gram-Schmidt against the full basis); the matvec may run in f32 or bf16.
@@ -418,22 +418,33 @@ __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __r
 // h2 = sum of the pass-2 partials (slots 0..j), ||r||^2 = slot j+1;
 // beta_j = sqrt(||r||^2 - ||h2||^2), the norm of r - V h2 (V^T r = h2 and
 // V is orthonormal; ||h2|| is at rounding level next to ||r|| except at a
-// Lanczos breakdown, which the test below catches); V[j+1] =
-// (r - V h2) / beta_j and x = cast(V[j+1]) when another Krylov vector
-// follows.  Block 0 of each solve also records alpha_j = h1[j] + h2[j] and
-// beta_j, solves the tridiagonal eigenproblem and tests convergence
-// (clearing act[s]; the solve's other blocks may still write V[j+1], which
-// is then unused).
+// Lanczos breakdown, which the test below catches).
+// Blocks [0, nchunks): V[j+1] = (r - V h2) / beta_j and x = cast(V[j+1])
+// over their chunk, when another Krylov vector follows.
+// Blocks [nchunks, nchunks + nsolve), one per solve: alpha_j = h1[j] + h2[j]
+// and beta_j (the same reduction, so the same bits), the tridiagonal
+// eigenproblem and the convergence test (clearing act[s]; the solve's chunk
+// blocks may still write V[j+1], which is then unused).  The eigensolve runs
+// beside the chunks' pass over V instead of after it.
 __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __restrict__ S, int nsolve,
                                                     void* state, int nchunks, int j, int x_dtype,
                                                     int normalize, int min_iters) {
   LzLayout L = lz_layout(state, nsolve, nchunks);
-  ChunkInfo ci = chunk_of(S, nsolve);
-  if (!solve_active(L.act, ci.s)) return;
-  const tnx_lanczos_solve D = S[ci.s];
+  const bool solver = (int)blockIdx.x >= nchunks;
+  ChunkInfo ci;
+  int s;
+  if (solver) {
+    s = (int)blockIdx.x - nchunks;
+  } else {
+    if (!normalize) return;
+    ci = chunk_of(S, nsolve);
+    s = ci.s;
+  }
+  if (!solve_active(L.act, s)) return;
+  const tnx_lanczos_solve D = S[s];
   __shared__ double h[NCV_MAX + 1];
   __shared__ double beta_s;
-  reduce_partials(L.partB, D.chunk_begin, ci.nch, j + 2, h);
+  reduce_partials(L.partB, D.chunk_begin, (D.n + CHUNK - 1) / CHUNK, j + 2, h);
   __syncthreads();
   if (threadIdx.x == 0) {
     double hh = 0.0;
@@ -442,7 +453,7 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
   }
   __syncthreads();
   const double beta_j = beta_s;
-  if (normalize) {
+  if (!solver) {
     const double* V = (const double*)D.V;
     double* Vn = (double*)D.V + (int64_t)(j + 1) * D.ldv;
     double r[CHUNK / NT];
@@ -463,43 +474,42 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
         tnx_store((void*)D.x, xd, i, v);
       }
     }
+    return;
   }
-  if (ci.c == 0) {
-    LzState& st = L.st[ci.s];
-    __shared__ double a_s[NCV_MAX], b_s[NCV_MAX], y_s[NCV_MAX], th_s;
-    __shared__ double work[5 * NCV_MAX];
-    if (threadIdx.x == 0) {
-      const double beta = beta_j;
-      for (int k = 0; k <= j; ++k) st.h2[k] = h[k];
-      st.alpha[j] = st.h1[j] + h[j];
-      st.beta[j] = beta;
-      st.iters += 1;
-      for (int k = 0; k <= j; ++k) {
-        a_s[k] = st.alpha[k];
-        b_s[k] = st.beta[k];
-      }
+  LzState& st = L.st[s];
+  __shared__ double a_s[NCV_MAX], b_s[NCV_MAX], y_s[NCV_MAX], th_s;
+  __shared__ double work[5 * NCV_MAX];
+  if (threadIdx.x == 0) {
+    const double beta = beta_j;
+    for (int k = 0; k <= j; ++k) st.h2[k] = h[k];
+    st.alpha[j] = st.h1[j] + h[j];
+    st.beta[j] = beta;
+    st.iters += 1;
+    for (int k = 0; k <= j; ++k) {
+      a_s[k] = st.alpha[k];
+      b_s[k] = st.beta[k];
     }
-    __syncthreads();
-    tridiag_lowest(a_s, b_s, j + 1, y_s, &th_s, work);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const double beta = b_s[j];
-      const double th = th_s;
-      const double res = fabs(beta * y_s[j]);
-      for (int k = 0; k <= j; ++k) st.y[k] = y_s[k];
-      st.theta = th;
-      st.resid = res;
-      st.jlast = j;
-      const double thr = fmax(D.tol_scale * fabs(1.0 - D.tau * th), D.floor_rel * fabs(th));
-      double anorm = 0.0;
-      for (int k = 0; k <= j; ++k) anorm = fmax(anorm, fabs(a_s[k]) + (k > 0 ? fabs(b_s[k - 1]) : 0.0));
-      const bool breakdown = beta <= 1e-14 * fmax(anorm, 1e-300);
-      const bool full = (j + 1 >= D.n);
-      if (full || breakdown || ((j + 1 >= min_iters) && res <= thr)) {
-        st.converged = 1;
-        __hip_atomic_store(L.act + ci.s, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        atomicSub(L.nact, 1);
-      }
+  }
+  __syncthreads();
+  tridiag_lowest(a_s, b_s, j + 1, y_s, &th_s, work);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double beta = b_s[j];
+    const double th = th_s;
+    const double res = fabs(beta * y_s[j]);
+    for (int k = 0; k <= j; ++k) st.y[k] = y_s[k];
+    st.theta = th;
+    st.resid = res;
+    st.jlast = j;
+    const double thr = fmax(D.tol_scale * fabs(1.0 - D.tau * th), D.floor_rel * fabs(th));
+    double anorm = 0.0;
+    for (int k = 0; k <= j; ++k) anorm = fmax(anorm, fabs(a_s[k]) + (k > 0 ? fabs(b_s[k - 1]) : 0.0));
+    const bool breakdown = beta <= 1e-14 * fmax(anorm, 1e-300);
+    const bool full = (j + 1 >= D.n);
+    if (full || breakdown || ((j + 1 >= min_iters) && res <= thr)) {
+      st.converged = 1;
+      __hip_atomic_store(L.act + s, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      atomicSub(L.nact, 1);
     }
   }
 }
@@ -660,7 +670,8 @@ int lanczos_run(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve
       if (rc) break;
       hipLaunchKernelGGL(k_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
       hipLaunchKernelGGL(k_update_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
-      hipLaunchKernelGGL(k_update_norm, grid, block, 0, s, solves, nsolve, state, nchunks, j,
+      hipLaunchKernelGGL(k_update_norm, dim3(nchunks + nsolve), block, 0, s, solves, nsolve, state,
+                         nchunks, j,
                          x_dtype, (int)(j + 1 < ncv), min_iters);
       TNX_CHECK_LAUNCH();
       // lagged poll: copy the active count of this step, then wait for the
