@@ -8,9 +8,10 @@ engine; the plain-f64 path and the CPU baseline are reported beside it, with
 the relative energy difference of every mixed solve against the f64 solve of
 the same seed.
 
-Extra JSON objects: ``roofline`` (dominant kernel = grouped matvec GEMM,
-timed with HIP events on the engine's stream) and ``cpu_baseline`` (the
-oracle's restatement of the reference algorithm timed on a bounded sample).
+Extra JSON objects: ``roofline`` (dominant kernel = the f64 grouped GEMM,
+measured on its matvec launches with HIP events on the engine's stream; the
+bf16 matvec GEMMs are listed beside it) and ``cpu_baseline`` (the oracle's
+restatement of the reference algorithm timed on a bounded sample).
 """
 import argparse
 import json
@@ -90,57 +91,34 @@ def run_batch(para, seeds, precision, ncv=20):
     return dt, np.array(energies), stats, eng
 
 
-def matvec_roofline(para, batch, precision, reps=20):
-    """Average duration of the dominant kernel (the grouped matvec GEMMs of a
-    mid-chain local step, all solves of the batch) with HIP events on the
-    engine's stream; algorithmic flops from the term structure."""
+def matvec_roofline(para, batch, precs=('fp64', 'bf16'), reps=20):
+    """Average launch durations of the matvec kernels (the grouped GEMMs and
+    the mixer of a mid-chain local step, all solves of the batch) at each
+    precision in ``precs``, with HIP events on the engine's stream;
+    algorithmic flops from the launches' own records."""
     import ctypes
     import torch
     from tnalg_amd import _lib as L
     from tnalg_amd import records as R
-    from tnalg_amd.engine import PREC, Engine
+    from tnalg_amd.engine import PREC, Engine, Step
     # the walk to the measured site runs in f64, so the low-precision GEMM
     # kernels run only in the timed loops below
     eng = Engine(precision_schedule=lambda sweep, nsweeps: 'fp64', ncv=20,
-                 lowp=precision not in (None, 'fp64'))
+                 lowp=any(p != 'fp64' for p in precs))
     solves = [eng.add_solve(para, seed=100 + k) for k in range(batch)]
     for s in solves:
         s.schedule_canonicalise()
     eng.run_queues(solves)
-    prec = 'fp64' if precision == 'fp64' else 'bf16'
     site = para['l'] // 2
-    from tnalg_amd.engine import Step
     # walk every centre to the middle with f64 steps (one batch), then plan
-    # one local step there
+    # one local step there per precision
     for s in solves:
         for p in range(1, site):
             s.queue.append(Step('solve', p, +1, 0))
     eng.run_queues(solves)
-    plans = []
-    for s in solves:
-        if eng.uses_lowp and prec != 'fp64':
-            eng._mix([s.cast_all_records(PREC[prec][0])])
-        plans.append(s.plan(Step('solve', site, +1, para['sweep_time'] - 1), prec))
-    ti, tc, mode1, mode2, _ = PREC[prec]
-    rls = [s.reloc for s in solves]
-    g1p, g1s, g1t = R.concat_gemm([ph['mv_g1'] for ph in plans], relocs=rls)
-    g2p, g2s, g2t = R.concat_gemm([ph['mv_g2'] for ph in plans], relocs=rls)
-    mxj, mxt, mxb = R.concat_mix([ph['mv_mix'] for ph in plans], relocs=rls)
-    ptr = eng.up.upload([g1p, g1s, g2p, g2s, mxj, mxt])
-    flops = float(sum(ph['mv_flops'] for ph in plans))
-    st = eng.stream()
+    st_ = eng.stream()
     lib = eng.lib
     vp = ctypes.c_void_p
-
-    def g1():
-        L.check(lib.tnx_gemm_grouped(mode1, vp(ptr[0]), len(g1p), vp(ptr[1]), g1t, None, st), 'g1')
-
-    def g2():
-        L.check(lib.tnx_gemm_grouped(mode2, vp(ptr[2]), len(g2p), vp(ptr[3]), g2t, None, st), 'g2')
-
-    def mix():
-        L.check(lib.tnx_mix(vp(ptr[4]), len(mxj), vp(ptr[5]), mxb, vp(eng.ops.data_ptr()), None,
-                            st), 'mix')
 
     def timed(fns):
         """Average seconds per repetition of ``fns`` (HIP events recorded on
@@ -154,37 +132,78 @@ def matvec_roofline(para, batch, precision, reps=20):
         e1.synchronize()
         return e0.elapsed_time(e1) / 1e3 / reps
 
-    for _ in range(3):
-        g1(), mix(), g2()
-    torch.cuda.synchronize()
-    t1, t2, t_mv = timed([g1]), timed([g2]), timed([g1, mix, g2])
+    out = {}
+    for prec in precs:
+        plans = []
+        for s in solves:
+            dims0 = list(s.dims)
+            if eng.uses_lowp and prec != 'fp64':
+                eng._mix([s.cast_all_records(PREC[prec][0])])
+            plans.append(s.plan(Step('solve', site, +1, para['sweep_time'] - 1), prec))
+            s.dims = dims0
+        ti, tc, mode1, mode2, _ = PREC[prec]
+        parts = [(s, ph) for s, ph in zip(solves, plans)]
+        g1p, g1s, g1t = R.concat_gemm([s.part(ph, 'mv_g1') for s, ph in parts])
+        g2p, g2s, g2t = R.concat_gemm([s.part(ph, 'mv_g2') for s, ph in parts])
+        mxj, mxt, mxb = R.concat_mix([s.part(ph, 'mv_mix') for s, ph in parts])
+        ptr = eng.up.upload([g1p, g1s, g2p, g2s, mxj, mxt])
+        flops = float(sum(ph['mv_flops'] for ph in plans))
+
+        def g1(ptr=ptr, n=len(g1p), t=g1t, m=mode1):
+            L.check(lib.tnx_gemm_grouped(m, vp(ptr[0]), n, vp(ptr[1]), t, None, st_), 'g1')
+
+        def g2(ptr=ptr, n=len(g2p), t=g2t, m=mode2):
+            L.check(lib.tnx_gemm_grouped(m, vp(ptr[2]), n, vp(ptr[3]), t, None, st_), 'g2')
+
+        def mix(ptr=ptr, n=len(mxj), b=mxb):
+            L.check(lib.tnx_mix(vp(ptr[4]), n, vp(ptr[5]), b, vp(eng.ops.data_ptr()), None,
+                                st_), 'mix')
+
+        for _ in range(3):
+            g1(), mix(), g2()
+        torch.cuda.synchronize()
+        t1, t2, t_mv = timed([g1]), timed([g2]), timed([g1, mix, g2])
+        out[prec] = {'flops': flops, 't_matvec': t_mv,
+                     'g1': {'t': t1, 'flops': gemm_flops(g1p, g1s), 'tiles': int(g1t),
+                            'mode': mode1},
+                     'g2': {'t': t2, 'flops': gemm_flops(g2p, g2s), 'tiles': int(g2t),
+                            'mode': mode2}}
     for s in list(solves):
         eng.remove_solve(s)
-    return {'flops': flops, 'prec': prec, 't_matvec': t_mv,
-            'g1': {'t': t1, 'flops': gemm_flops(g1p, g1s), 'tiles': int(g1t), 'mode': mode1},
-            'g2': {'t': t2, 'flops': gemm_flops(g2p, g2s), 'tiles': int(g2t), 'mode': mode2}}
+    return out
+
+
+GEMM_MODE_NAME = {0: 'f64', 1: 'f32', 2: 'bf16 -> f32', 3: 'bf16 -> bf16'}
 
 
 def roofline_object(para, args):
-    """Roofline of the dominant matvec GEMM of a mid-chain local step (the
-    longer of GEMM1 and GEMM2; both are listed under ``kernels``)."""
-    rf = matvec_roofline(para, args.batch, args.precision)
-    prec = rf['prec']
-    peak = MFMA_PEAK[prec]
+    """Roofline of the dominant kernel.  In the default mixed-precision run
+    that is ``gemm_grouped_kernel<0>``, the f64 MFMA GEMM (the f64 sweeps'
+    matvecs, every gauge move and environment update: about a quarter of the
+    run's GPU time, the largest share of any kernel); it is measured on its
+    matvec launches at a mid-chain step (the longer of GEMM1 and GEMM2).  The
+    low-precision matvec GEMMs of the same step are listed under
+    ``kernels``."""
+    precs = ('fp64',) if args.precision == 'fp64' else ('fp64', 'bf16')
+    rf = matvec_roofline(para, args.batch, precs)
     kern = []
-    for what, g in (('GEMM1 T = L x', rf['g1']), ('GEMM2 w = U R^T', rf['g2'])):
-        ach = g['flops'] / g['t'] / 1e12
-        kern.append({'kernel': 'gemm_grouped_kernel<%d>' % g['mode'], 'role': 'matvec ' + what,
-                     'achieved': ach, 'frac': ach / peak, 'avg_launch_s': g['t'],
-                     'flops_per_launch': g['flops'], 'tiles': g['tiles']})
-    dom = max(kern, key=lambda k: k['avg_launch_s'])
-    return {'bound': 'mfma', 'achieved': dom['achieved'], 'peak': peak, 'unit': 'TFLOP/s',
-            'frac': dom['achieved'] / peak, 'traffic': None,
-            'kernel': '%s (%s, %s MFMA, %d solves at site %d of the N=%d chi=%d chain)' % (
-                dom['kernel'], dom['role'], prec, args.batch, para['l'] // 2, para['l'],
+    for prec in precs:
+        peak = MFMA_PEAK[prec]
+        for what, g in (('GEMM1 T = L x', rf[prec]['g1']), ('GEMM2 w = U R^T', rf[prec]['g2'])):
+            ach = g['flops'] / g['t'] / 1e12
+            kern.append({'kernel': 'gemm_grouped_kernel<%d>' % g['mode'],
+                         'role': 'matvec %s (%s MFMA)' % (what, GEMM_MODE_NAME[g['mode']]),
+                         'prec': prec, 'achieved': ach, 'peak': peak, 'frac': ach / peak,
+                         'avg_launch_s': g['t'], 'flops_per_launch': g['flops'],
+                         'tiles': g['tiles']})
+    dom = max((k for k in kern if k['prec'] == 'fp64'), key=lambda k: k['avg_launch_s'])
+    return {'bound': 'mfma', 'achieved': dom['achieved'], 'peak': dom['peak'], 'unit': 'TFLOP/s',
+            'frac': dom['frac'], 'traffic': None,
+            'kernel': '%s (%s, %d solves at site %d of the N=%d chi=%d chain)' % (
+                dom['kernel'], dom['role'], args.batch, para['l'] // 2, para['l'],
                 para['chi']),
             'avg_launch_s': dom['avg_launch_s'], 'flops_per_launch': dom['flops_per_launch'],
-            'matvec_s': rf['t_matvec'], 'kernels': kern}
+            'matvec_s': {p: rf[p]['t_matvec'] for p in precs}, 'kernels': kern}
 
 
 def gemm_flops(p, s):

@@ -20,13 +20,15 @@ void tnx_set_error(const char* fmt, ...) {
 
 namespace {
 
-constexpr int MIX_PAIRS = 512;  // (a, b) index pairs per workgroup (256 lanes x 2)
-constexpr int MIX_DMAX = 4;     // largest physical dimension (the operator table holds 4 x 4)
+constexpr int MIX_PPT = 2;                // (a, b) pairs per lane
+constexpr int MIX_PAIRS = 256 * MIX_PPT;  // (a, b) index pairs per workgroup
+constexpr int MIX_DMAX = 4;               // largest physical dimension (the operator table holds 4 x 4)
 
-// One lane per (a, b): the d source values src[a, :, b] of a term are
-// loaded once and mixed by the d x d operator in registers, so each output
-// element costs one load per term and the index split is one 32-bit
-// division per pair.
+// One lane per (a, b) pair, MIX_PPT pairs per lane: the d source values
+// src[a, :, b] of a term are loaded once and mixed by the d x d operator in
+// registers, so each output element costs one load per term.  Terms are the
+// outer loop: a term's record and operator are read once per lane, and the
+// loads of the lane's pairs are issued together.
 __global__ __launch_bounds__(256) void mix_kernel(const tnx_mix_job* __restrict__ jobs, int njob,
                                                   const tnx_mix_term* __restrict__ terms,
                                                   const double* __restrict__ ops,
@@ -39,40 +41,64 @@ __global__ __launch_bounds__(256) void mix_kernel(const tnx_mix_job* __restrict_
   const int d = J.d, Dr = J.Dr;
   const int npairs = J.Dl * Dr;
   const int base = (bid - J.block_begin) * MIX_PAIRS;
-  for (int q = 0; q < MIX_PAIRS / 256; ++q) {
+  int64_t e0[MIX_PPT];  // element (a, 0, b) of each pair
+  bool ok[MIX_PPT];
+#pragma unroll
+  for (int q = 0; q < MIX_PPT; ++q) {
     const int p = base + q * 256 + threadIdx.x;
-    if (p >= npairs) break;
-    const int a = p / Dr;
-    const int64_t e0 = (int64_t)a * d * Dr + (p - a * Dr);  // element (a, 0, b)
-    double acc[MIX_DMAX] = {0.0, 0.0, 0.0, 0.0};
-    for (int ti = J.term_begin; ti < J.term_begin + J.term_count; ++ti) {
-      const tnx_mix_term T = terms[ti];
-      double x[MIX_DMAX];
+    ok[q] = p < npairs;
+    const int a = ok[q] ? p / Dr : 0;
+    e0[q] = ok[q] ? (int64_t)a * d * Dr + (p - a * Dr) : 0;
+  }
+  double acc[MIX_PPT][MIX_DMAX];
+#pragma unroll
+  for (int q = 0; q < MIX_PPT; ++q)
+#pragma unroll
+    for (int s = 0; s < MIX_DMAX; ++s) acc[q][s] = 0.0;
+  for (int ti = J.term_begin; ti < J.term_begin + J.term_count; ++ti) {
+    const tnx_mix_term T = terms[ti];
+    double x[MIX_PPT][MIX_DMAX];
+#pragma unroll
+    for (int q = 0; q < MIX_PPT; ++q)
 #pragma unroll
       for (int t = 0; t < MIX_DMAX; ++t)
-        x[t] = (t < d) ? tnx_load((const void*)T.src, T.src_dtype, e0 + (int64_t)t * Dr) : 0.0;
-      if (T.op < 0) {
+        x[q][t] = (ok[q] && t < d) ? tnx_load((const void*)T.src, T.src_dtype,
+                                              e0[q] + (int64_t)t * Dr) : 0.0;
+    if (T.op < 0) {
 #pragma unroll
-        for (int s = 0; s < MIX_DMAX; ++s) acc[s] += T.coef * x[s];
-      } else {
-        const double* O = ops + (int64_t)T.op * d * d;
+      for (int q = 0; q < MIX_PPT; ++q)
+#pragma unroll
+        for (int s = 0; s < MIX_DMAX; ++s) acc[q][s] += T.coef * x[q][s];
+    } else {
+      double O[MIX_DMAX * MIX_DMAX];
+      const double* Op = ops + (int64_t)T.op * d * d;
+#pragma unroll
+      for (int s = 0; s < MIX_DMAX; ++s)
+#pragma unroll
+        for (int t = 0; t < MIX_DMAX; ++t)
+          O[s * MIX_DMAX + t] = (s < d && t < d) ? Op[s * d + t] : 0.0;
+#pragma unroll
+      for (int q = 0; q < MIX_PPT; ++q)
 #pragma unroll
         for (int s = 0; s < MIX_DMAX; ++s) {
           if (s < d) {
             double v = 0.0;
 #pragma unroll
             for (int t = 0; t < MIX_DMAX; ++t)
-              if (t < d) v += O[s * d + t] * x[t];
-            acc[s] += T.coef * v;
+              if (t < d) v += O[s * MIX_DMAX + t] * x[q][t];
+            acc[q][s] += T.coef * v;
           }
         }
-      }
     }
+  }
+#pragma unroll
+  for (int q = 0; q < MIX_PPT; ++q) {
+    if (!ok[q]) continue;
 #pragma unroll
     for (int s = 0; s < MIX_DMAX; ++s) {
       if (s < d) {
-        const int64_t e = e0 + (int64_t)s * Dr;
-        double v = acc[s];
+        const int64_t e = e0[q] + (int64_t)s * Dr;
+        double v = acc[q][s];
         if (J.accumulate) v += tnx_load((const void*)J.dst, J.dst_dtype, e);
         tnx_store((void*)J.dst, J.dst_dtype, e, v);
       }
