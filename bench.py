@@ -149,25 +149,35 @@ def matvec_roofline(para, batch, precs=('fp64', 'bf16'), reps=20):
         ptr = eng.up.upload([g1p, g1s, g2p, g2s, mxj, mxt])
         flops = float(sum(ph['mv_flops'] for ph in plans))
 
-        def g1(ptr=ptr, n=len(g1p), t=g1t, m=mode1):
-            L.check(lib.tnx_gemm_grouped(m, vp(ptr[0]), n, vp(ptr[1]), t, None, st_), 'g1')
-
-        def g2(ptr=ptr, n=len(g2p), t=g2t, m=mode2):
-            L.check(lib.tnx_gemm_grouped(m, vp(ptr[2]), n, vp(ptr[3]), t, None, st_), 'g2')
+        def gemm(i, n, t, m):
+            L.check(lib.tnx_gemm_grouped(m, vp(ptr[i]), n, vp(ptr[i + 1]), t, None, st_), 'gemm')
 
         def mix(ptr=ptr, n=len(mxj), b=mxb):
             L.check(lib.tnx_mix(vp(ptr[4]), n, vp(ptr[5]), b, vp(eng.ops.data_ptr()), None,
                                 st_), 'mix')
 
+        res = {'flops': flops}
+        best = {}
+        for key, i, n, t, mode, fl in (('g1', 0, len(g1p), g1t, mode1, gemm_flops(g1p, g1s)),
+                                       ('g2', 2, len(g2p), g2t, mode2, gemm_flops(g2p, g2s))):
+            # every kernel variant of the mode (the engine's tuner keeps the
+            # fastest); the same launches, the same bits
+            tv = []
+            for v in range(int(lib.tnx_gemm_variants(mode))):
+                f = (lambda i=i, n=n, t=t, m=mode | (v << 8): gemm(i, n, t, m))
+                for _ in range(3):
+                    f()
+                torch.cuda.synchronize()
+                tv.append(timed([f]))
+            v = int(np.argmin(tv))
+            best[key] = (lambda i=i, n=n, t=t, m=mode | (v << 8): gemm(i, n, t, m))
+            res[key] = {'t': tv[v], 'flops': fl, 'tiles': int(t), 'mode': mode, 'variant': v,
+                        't_variants': tv}
         for _ in range(3):
-            g1(), mix(), g2()
+            best['g1'](), mix(), best['g2']()
         torch.cuda.synchronize()
-        t1, t2, t_mv = timed([g1]), timed([g2]), timed([g1, mix, g2])
-        out[prec] = {'flops': flops, 't_matvec': t_mv,
-                     'g1': {'t': t1, 'flops': gemm_flops(g1p, g1s), 'tiles': int(g1t),
-                            'mode': mode1},
-                     'g2': {'t': t2, 'flops': gemm_flops(g2p, g2s), 'tiles': int(g2t),
-                            'mode': mode2}}
+        res['t_matvec'] = timed([best['g1'], mix, best['g2']])
+        out[prec] = res
     for s in list(solves):
         eng.remove_solve(s)
     return out
@@ -184,7 +194,7 @@ def roofline_object(para, args):
     matvec launches at a mid-chain step (the longer of GEMM1 and GEMM2).  The
     low-precision matvec GEMMs of the same step are listed under
     ``kernels``."""
-    precs = ('fp64',) if args.precision == 'fp64' else ('fp64', 'bf16')
+    precs = ('fp64',) if args.precision == 'fp64' else ('fp64', 'fp32', 'bf16')
     rf = matvec_roofline(para, args.batch, precs)
     kern = []
     for prec in precs:
@@ -195,13 +205,14 @@ def roofline_object(para, args):
                          'role': 'matvec %s (%s MFMA)' % (what, GEMM_MODE_NAME[g['mode']]),
                          'prec': prec, 'achieved': ach, 'peak': peak, 'frac': ach / peak,
                          'avg_launch_s': g['t'], 'flops_per_launch': g['flops'],
-                         'tiles': g['tiles']})
+                         'tiles': g['tiles'], 'variant': g['variant'],
+                         'avg_launch_s_per_variant': g['t_variants']})
     dom = max((k for k in kern if k['prec'] == 'fp64'), key=lambda k: k['avg_launch_s'])
     return {'bound': 'mfma', 'achieved': dom['achieved'], 'peak': dom['peak'], 'unit': 'TFLOP/s',
             'frac': dom['frac'], 'traffic': None,
-            'kernel': '%s (%s, %d solves at site %d of the N=%d chi=%d chain)' % (
-                dom['kernel'], dom['role'], args.batch, para['l'] // 2, para['l'],
-                para['chi']),
+            'kernel': '%s (%s, kernel variant %d, %d solves at site %d of the N=%d chi=%d '
+                      'chain)' % (dom['kernel'], dom['role'], dom['variant'], args.batch,
+                                  para['l'] // 2, para['l'], para['chi']),
             'avg_launch_s': dom['avg_launch_s'], 'flops_per_launch': dom['flops_per_launch'],
             'matvec_s': {p: rf[p]['t_matvec'] for p in precs}, 'kernels': kern}
 

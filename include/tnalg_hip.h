@@ -149,10 +149,19 @@ int tnx_abi_sizes(int32_t* sizes, int n);
  * contractions of library/TensorBasicModule.py:489-613 (absorb_matrix2tensor,
  * absorb_matrices2tensor[_full_fast]) and :632-721 (bound_vec_operator_left2right
  * / right2left) as used by MPSClass.py:837-858 and MPSClass.py:555-611.
- * ntiles = total 64x64 tiles; active = device int flags or NULL.            */
+ * ntiles = total 64x64 tiles; active = device int flags or NULL.
+ * mode = tnx_gemm_mode | (variant << 8): a mode may have several kernel
+ * variants (LDS staging depth and occupancy), all giving the same bits;
+ * variant 0 is the default.  The matvec plan's mode / mode2 fields take the
+ * same encoding.                                                           */
 int tnx_gemm_grouped(int mode, const tnx_gemm_problem* prob, int nprob,
                      const tnx_gemm_seg* seg, int ntiles, const int32_t* active,
                      tnx_stream_t stream);
+
+/* Number of kernel variants of a GEMM mode (TNX_EINVAL for an unknown mode).
+ * No reference counterpart: the engine's tuner times the variants of a mode
+ * on the first launches of each shape class and keeps the fastest.         */
+int tnx_gemm_variants(int mode);
 
 /* Elementwise physical-index mixer / linear combination / dtype cast.
  * Replaces absorb_matrix2tensor(tensor, op.T, 1) (TensorBasicModule.py:489-526

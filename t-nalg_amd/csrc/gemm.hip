@@ -8,39 +8,63 @@
 // sum_k L_k x_k or sum_r U_r R_r^T of one local step is one accumulation in
 // registers.  Workgroup: 256 lanes = 4 wavefronts of 64, each wavefront owns
 // a 32x32 quarter of the tile as 2x2 MFMA 16x16 blocks.
-//   f64 : v_mfma_f64_16x16x4_f64   (K step 4,  LDS chunk 16)
-//   f32 : v_mfma_f32_16x16x4_f32   (K step 4,  LDS chunk 16)
-//   bf16: v_mfma_f32_16x16x32_bf16 (K step 32, LDS chunk 32, f32 accumulate)
-// The K chunks of all segments form one stream; the global loads of chunk
-// c+1 are issued into registers before the MFMAs of chunk c (register
-// double buffering), and runs of 16 bytes are loaded as one vector when they
-// are in bounds and aligned.
+//   f64 : v_mfma_f64_16x16x4_f64   (K step 4,  LDS chunk 32, two LDS stages)
+//   f32 : v_mfma_f32_16x16x4_f32   (K step 4,  LDS chunk 64, one stage)
+//   bf16: v_mfma_f32_16x16x32_bf16 (K step 32, LDS chunk 128, one stage, f32
+//         accumulate)
+// The K chunks of all segments form one stream; the global loads of the next
+// chunk are in flight in registers during the MFMAs of the current one, and
+// runs of 16 bytes are loaded as one vector when they are in bounds and
+// aligned.
 #include <type_traits>
 
 #include "tnx_common.h"
 
 namespace {
 
-// KB: K depth of one staged chunk (one global round trip and two barriers);
-// NE: elements of A and of B each lane fetches per chunk (256 x NE = 64 x KB).
-// The problems here are small (K of a few hundred), so the chunk is deep:
-// the K loop is bound by the round-trip latency, not by the MFMA rate.
-template <int MODE> struct Cfg;
-template <> struct Cfg<TNX_GEMM_F64> {
+// A configuration (kernel template argument CFG) fixes the GEMM mode, the K
+// depth KB of one staged chunk, NE (elements of A and of B each lane fetches
+// per chunk; 256 x NE = 64 x KB) and DB (LDS double buffering).
+// - f64 is MFMA bound at these shapes. Two variants, both with two LDS stages
+//   (one barrier per chunk, the commit of chunk c+1 overlapping the MFMAs of
+//   chunk c); they differ in depth and occupancy, and neither wins
+//   everywhere.  Mid-chain matvec, 64 solves:
+//     variant 0, 32-deep chunks, 69.6 KB, 2 workgroups per CU:
+//       GEMM 1 (2048 tiles, K 128) 69 us, GEMM 2 (512 tiles, K 512) 64 us;
+//     variant 1, 16-deep chunks, 34.8 KB, 4 workgroups per CU:
+//       GEMM 1 59 us, GEMM 2 75 us.
+//   (One stage of 64: 75 / 69 us.)  The caller picks the variant per launch
+//   (the mode's second byte); the engine times both on first use.
+// - f32 / bf16 are latency bound at these shapes: one stage of a deep chunk
+//   (34.8 KB, four workgroups per CU), two barriers per chunk; a second stage
+//   measured slower (f32 GEMM 2: 38 -> 42 us; bf16 GEMM 1: 16 -> 22 us).
+// All variants of a mode issue the same MFMAs in the same k order, so they
+// give the same bits.
+template <int CFG> struct Cfg;
+template <> struct Cfg<0> {
   typedef double TI; typedef double TC; typedef double ACC; typedef double CT;
-  static constexpr int KB = 64, NE = 16;
+  static constexpr int MODE = TNX_GEMM_F64, KB = 32, NE = 8;
+  static constexpr bool DB = true;
 };
-template <> struct Cfg<TNX_GEMM_F32> {
+template <> struct Cfg<1> {
   typedef float TI; typedef float TC; typedef float ACC; typedef float CT;
-  static constexpr int KB = 64, NE = 16;
+  static constexpr int MODE = TNX_GEMM_F32, KB = 64, NE = 16;
+  static constexpr bool DB = false;
 };
-template <> struct Cfg<TNX_GEMM_BF16_F32> {
+template <> struct Cfg<2> {
   typedef __bf16 TI; typedef float TC; typedef float ACC; typedef __bf16 CT;
-  static constexpr int KB = 128, NE = 32;
+  static constexpr int MODE = TNX_GEMM_BF16_F32, KB = 128, NE = 32;
+  static constexpr bool DB = false;
 };
-template <> struct Cfg<TNX_GEMM_BF16_BF16> {
+template <> struct Cfg<3> {
   typedef __bf16 TI; typedef __bf16 TC; typedef float ACC; typedef __bf16 CT;
-  static constexpr int KB = 128, NE = 32;
+  static constexpr int MODE = TNX_GEMM_BF16_BF16, KB = 128, NE = 32;
+  static constexpr bool DB = false;
+};
+template <> struct Cfg<4> {   // f64, variant 1
+  typedef double TI; typedef double TC; typedef double ACC; typedef double CT;
+  static constexpr int MODE = TNX_GEMM_F64, KB = 16, NE = 4;
+  static constexpr bool DB = true;
 };
 
 template <typename T> __device__ __forceinline__ double to_d(T v) { return (double)v; }
@@ -73,9 +97,9 @@ __device__ __forceinline__ void load_run(const TI* p, int valid, TI (&v)[NE]) {
 
 // Per-thread fragment of one K chunk: NE elements of the A tile and NE of the
 // B tile, with the addressing mode they were fetched in.
-template <int MODE> struct Frag {
-  typename Cfg<MODE>::TI a[Cfg<MODE>::NE];
-  typename Cfg<MODE>::TI b[Cfg<MODE>::NE];
+template <int CFG> struct Frag {
+  typename Cfg<CFG>::TI a[Cfg<CFG>::NE];
+  typename Cfg<CFG>::TI b[Cfg<CFG>::NE];
   double alpha;
   int ta, tb;
 };
@@ -84,11 +108,11 @@ template <int MODE> struct Frag {
 // 64 x KB).
 //   transA = 0 (k contiguous): row t/(KB/NE), k = (t%(KB/NE))*NE .. +NE
 //   transA = 1 (i contiguous): k  t/(64/NE), i = (t%(64/NE))*NE .. +NE
-template <int MODE>
-__device__ __forceinline__ void fetch(Frag<MODE>& f, const tnx_gemm_seg& S, int M, int N, int row0,
+template <int CFG>
+__device__ __forceinline__ void fetch(Frag<CFG>& f, const tnx_gemm_seg& S, int M, int N, int row0,
                                       int col0, int k0) {
-  typedef typename Cfg<MODE>::TI TI;
-  constexpr int KB = Cfg<MODE>::KB, NE = Cfg<MODE>::NE;
+  typedef typename Cfg<CFG>::TI TI;
+  constexpr int KB = Cfg<CFG>::KB, NE = Cfg<CFG>::NE;
   const int t = threadIdx.x;
   const TI* A = (const TI*)S.a;
   const TI* B = (const TI*)S.b;
@@ -120,11 +144,11 @@ __device__ __forceinline__ void fetch(Frag<MODE>& f, const tnx_gemm_seg& S, int 
 }
 
 // f64/f32: LDS tiles k-major, As[k][i], Bs[k][j] (conflict-free MFMA reads)
-template <int MODE>
-__device__ __forceinline__ void commit_k4(const Frag<MODE>& f, typename Cfg<MODE>::CT (*As)[TM + 4],
-                                          typename Cfg<MODE>::CT (*Bs)[TN + 4]) {
-  typedef typename Cfg<MODE>::CT CT;
-  constexpr int KB = Cfg<MODE>::KB, NE = Cfg<MODE>::NE;
+template <int CFG>
+__device__ __forceinline__ void commit_k4(const Frag<CFG>& f, typename Cfg<CFG>::CT (*As)[TM + 4],
+                                          typename Cfg<CFG>::CT (*Bs)[TN + 4]) {
+  typedef typename Cfg<CFG>::CT CT;
+  constexpr int KB = Cfg<CFG>::KB, NE = Cfg<CFG>::NE;
   const int t = threadIdx.x;
   if (!f.ta) {
     const int i = t / (KB / NE), kq = (t % (KB / NE)) * NE;
@@ -147,10 +171,10 @@ __device__ __forceinline__ void commit_k4(const Frag<MODE>& f, typename Cfg<MODE
 }
 
 // bf16: LDS tiles row-major in k, As[i][k], Bs[j][k] (8 consecutive k per lane)
-template <int MODE>
-__device__ __forceinline__ void commit_bf16(const Frag<MODE>& f, __bf16 (*As)[Cfg<MODE>::KB + 8],
-                                            __bf16 (*Bs)[Cfg<MODE>::KB + 8]) {
-  constexpr int KB = Cfg<MODE>::KB, NE = Cfg<MODE>::NE;
+template <int CFG>
+__device__ __forceinline__ void commit_bf16(const Frag<CFG>& f, __bf16 (*As)[Cfg<CFG>::KB + 8],
+                                            __bf16 (*Bs)[Cfg<CFG>::KB + 8]) {
+  constexpr int KB = Cfg<CFG>::KB, NE = Cfg<CFG>::NE;
   const int t = threadIdx.x;
   const float alpha = (float)f.alpha;
   if (!f.ta) {
@@ -178,12 +202,13 @@ __device__ __forceinline__ void commit_bf16(const Frag<MODE>& f, __bf16 (*As)[Cf
   }
 }
 
-template <int MODE>
+template <int CFG>
 __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem* __restrict__ probs,
                                                           int nprob,
                                                           const tnx_gemm_seg* __restrict__ segs,
                                                           const int32_t* __restrict__ active) {
-  typedef Cfg<MODE> C;
+  typedef Cfg<CFG> C;
+  constexpr int MODE = C::MODE;
   typedef typename C::ACC ACC;
   typedef typename C::TC TC;
   constexpr int KB = C::KB;
@@ -212,25 +237,23 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
   constexpr bool K4 = (MODE == TNX_GEMM_F64 || MODE == TNX_GEMM_F32);
   constexpr int AR = K4 ? KB : TM, AC = K4 ? TM + 4 : KB + 8;
   constexpr int BR = K4 ? KB : TN, BC = K4 ? TN + 4 : KB + 8;
-  __shared__ __attribute__((aligned(16))) CT As[AR][AC];
-  __shared__ __attribute__((aligned(16))) CT Bs[BR][BC];
+  constexpr bool DB = C::DB;
+  __shared__ __attribute__((aligned(16))) CT As[DB ? 2 : 1][AR][AC];
+  __shared__ __attribute__((aligned(16))) CT Bs[DB ? 2 : 1][BR][BC];
 
-  // flattened stream of (segment, k0) chunks
+  // Flattened stream of (segment, k0) chunks; (s, k0) is the next chunk to
+  // fetch.  Pipeline: chunk c+1 is in registers while chunk c is in LDS;
+  // each step runs chunk c's MFMAs, commits chunk c+1 (double buffered: to
+  // the other stage, which the barrier of the previous step guarantees every
+  // wave has finished reading; single stage: after a barrier), one barrier,
+  // then fetches chunk c+2, whose loads fly during the next MFMAs.
   int s = P.seg_begin, k0 = 0;
   const int s_end = P.seg_begin + P.seg_count;
   while (s < s_end && segs[s].K <= 0) ++s;
-  Frag<MODE> f;
+  Frag<CFG> f;
   tnx_gemm_seg S;
-  if (s < s_end) {
-    S = segs[s];
-    fetch<MODE>(f, S, P.M, P.N, row0, col0, 0);
-  }
-  while (s < s_end) {
-    __syncthreads();
-    if constexpr (K4) commit_k4<MODE>(f, As, Bs);
-    else commit_bf16<MODE>(f, As, Bs);
-    __syncthreads();
-    // advance and prefetch the next chunk
+  if (s < s_end) S = segs[s];
+  auto advance = [&]() {
     k0 += KB;
     if (k0 >= S.K) {
       k0 = 0;
@@ -238,14 +261,31 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
       while (s < s_end && segs[s].K <= 0) ++s;
       if (s < s_end) S = segs[s];
     }
-    if (s < s_end) fetch<MODE>(f, S, P.M, P.N, row0, col0, k0);
+  };
+  bool staged = false;  // chunk c is in LDS
+  if (s < s_end) {
+    fetch<CFG>(f, S, P.M, P.N, row0, col0, k0);
+    advance();
+    if constexpr (K4) commit_k4<CFG>(f, As[0], Bs[0]);
+    else commit_bf16<CFG>(f, As[0], Bs[0]);
+    staged = true;
+  }
+  __syncthreads();
+  bool held = false;  // chunk c+1 is in registers
+  if (s < s_end) {
+    fetch<CFG>(f, S, P.M, P.N, row0, col0, k0);
+    advance();
+    held = true;
+  }
+  int stage = 0;
+  while (staged) {
     if constexpr (K4) {
 #pragma unroll
       for (int kk = 0; kk < KB; kk += 4) {
-        CT a0 = As[kk + (lane >> 4)][wr + (lane & 15)];
-        CT a1 = As[kk + (lane >> 4)][wr + 16 + (lane & 15)];
-        CT b0 = Bs[kk + (lane >> 4)][wc + (lane & 15)];
-        CT b1 = Bs[kk + (lane >> 4)][wc + 16 + (lane & 15)];
+        CT a0 = As[stage][kk + (lane >> 4)][wr + (lane & 15)];
+        CT a1 = As[stage][kk + (lane >> 4)][wr + 16 + (lane & 15)];
+        CT b0 = Bs[stage][kk + (lane >> 4)][wc + (lane & 15)];
+        CT b1 = Bs[stage][kk + (lane >> 4)][wc + 16 + (lane & 15)];
         if constexpr (MODE == TNX_GEMM_F64) {
           acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
           acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
@@ -262,16 +302,31 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
 #pragma unroll
       for (int kk = 0; kk < KB; kk += 32) {
         const int kq = kk + (lane >> 4) * 8;
-        tnx_bf8 a0 = *(const tnx_bf8*)&As[wr + (lane & 15)][kq];
-        tnx_bf8 a1 = *(const tnx_bf8*)&As[wr + 16 + (lane & 15)][kq];
-        tnx_bf8 b0 = *(const tnx_bf8*)&Bs[wc + (lane & 15)][kq];
-        tnx_bf8 b1 = *(const tnx_bf8*)&Bs[wc + 16 + (lane & 15)][kq];
+        tnx_bf8 a0 = *(const tnx_bf8*)&As[stage][wr + (lane & 15)][kq];
+        tnx_bf8 a1 = *(const tnx_bf8*)&As[stage][wr + 16 + (lane & 15)][kq];
+        tnx_bf8 b0 = *(const tnx_bf8*)&Bs[stage][wc + (lane & 15)][kq];
+        tnx_bf8 b1 = *(const tnx_bf8*)&Bs[stage][wc + 16 + (lane & 15)][kq];
         acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
         acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
         acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
         acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
       }
     }
+    staged = held;
+    if constexpr (!DB) __syncthreads();  // every wave has read the single stage
+    const int next = DB ? stage ^ 1 : 0;
+    if (held) {
+      if constexpr (K4) commit_k4<CFG>(f, As[next], Bs[next]);
+      else commit_bf16<CFG>(f, As[next], Bs[next]);
+    }
+    __syncthreads();
+    held = false;
+    if (staged && s < s_end) {
+      fetch<CFG>(f, S, P.M, P.N, row0, col0, k0);
+      advance();
+      held = true;
+    }
+    stage = next;
   }
 
   // epilogue: column lane%16 of each 16x16 block; the row of accumulator
@@ -298,6 +353,18 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
 
 }  // namespace
 
+extern "C" int tnx_gemm_variants(int mode) {
+  switch (mode) {
+    case TNX_GEMM_F64: return 2;
+    case TNX_GEMM_F32:
+    case TNX_GEMM_BF16_F32:
+    case TNX_GEMM_BF16_BF16: return 1;
+    default:
+      tnx_set_error("tnx_gemm_variants: unknown mode %d", mode);
+      return TNX_EINVAL;
+  }
+}
+
 extern "C" int tnx_gemm_grouped(int mode, const tnx_gemm_problem* prob, int nprob,
                                 const tnx_gemm_seg* seg, int ntiles, const int32_t* active,
                                 tnx_stream_t stream) {
@@ -306,25 +373,31 @@ extern "C" int tnx_gemm_grouped(int mode, const tnx_gemm_problem* prob, int npro
     tnx_set_error("tnx_gemm_grouped: bad arguments (nprob=%d ntiles=%d)", nprob, ntiles);
     return TNX_EINVAL;
   }
+  const int base = mode & 0xFF, var = (mode >> 8) & 0xFF;
+  const int nvar = tnx_gemm_variants(base);
+  if (nvar < 0 || (mode >> 16) != 0 || var >= nvar) {
+    tnx_set_error("tnx_gemm_grouped: unknown mode %d (variant %d)", base, var);
+    return TNX_EINVAL;
+  }
   if (ntiles == 0) return TNX_OK;
   hipStream_t s = (hipStream_t)stream;
   dim3 grid(ntiles), block(NT);
-  switch (mode) {
+  switch (base) {
     case TNX_GEMM_F64:
-      hipLaunchKernelGGL(gemm_grouped_kernel<TNX_GEMM_F64>, grid, block, 0, s, prob, nprob, seg, active);
+      if (var == 0)
+        hipLaunchKernelGGL(gemm_grouped_kernel<0>, grid, block, 0, s, prob, nprob, seg, active);
+      else
+        hipLaunchKernelGGL(gemm_grouped_kernel<4>, grid, block, 0, s, prob, nprob, seg, active);
       break;
     case TNX_GEMM_F32:
-      hipLaunchKernelGGL(gemm_grouped_kernel<TNX_GEMM_F32>, grid, block, 0, s, prob, nprob, seg, active);
+      hipLaunchKernelGGL(gemm_grouped_kernel<1>, grid, block, 0, s, prob, nprob, seg, active);
       break;
     case TNX_GEMM_BF16_F32:
-      hipLaunchKernelGGL(gemm_grouped_kernel<TNX_GEMM_BF16_F32>, grid, block, 0, s, prob, nprob, seg, active);
-      break;
-    case TNX_GEMM_BF16_BF16:
-      hipLaunchKernelGGL(gemm_grouped_kernel<TNX_GEMM_BF16_BF16>, grid, block, 0, s, prob, nprob, seg, active);
+      hipLaunchKernelGGL(gemm_grouped_kernel<2>, grid, block, 0, s, prob, nprob, seg, active);
       break;
     default:
-      tnx_set_error("tnx_gemm_grouped: unknown mode %d", mode);
-      return TNX_EINVAL;
+      hipLaunchKernelGGL(gemm_grouped_kernel<3>, grid, block, 0, s, prob, nprob, seg, active);
+      break;
   }
   TNX_CHECK_LAUNCH();
   return TNX_OK;

@@ -86,6 +86,7 @@ SIGNATURES = {
     'tnx_last_error': (ctypes.c_char_p, []),
     'tnx_abi_sizes': (_i, [_vp, _i]),
     'tnx_gemm_grouped': (_i, [_i, _vp, _i, _vp, _i, _vp, _vp]),
+    'tnx_gemm_variants': (_i, [_i]),
     'tnx_mix': (_i, [_vp, _i, _vp, _i, _vp, _vp, _vp]),
     'tnx_heff_matvec': (_i, [ctypes.POINTER(MatvecPlan), _vp, _vp]),
     'tnx_lanczos_state_bytes': (_sz, [_i, _i, _i]),

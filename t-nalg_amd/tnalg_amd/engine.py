@@ -81,6 +81,61 @@ class _Stager:
         self.events[k] = ev
 
 
+def gemm_kmax(p, s):
+    """Largest total K (sum over its segments) of the problems of a grouped
+    GEMM launch."""
+    if len(p) == 0:
+        return 0
+    k = np.concatenate([[0], np.cumsum(s['K'].astype(np.int64))])
+    return int((k[p['seg_begin'] + p['seg_count']] - k[p['seg_begin']]).max())
+
+
+class GemmTuner:
+    """Kernel variant of each grouped-GEMM launch, by shape class (mode, bit
+    length of the tile count, bit length of the largest K).  A class is
+    tuned on its first matvec launches: every variant of the mode is timed
+    with HIP events on the launch's stream and the fastest is kept.  The
+    variants give the same bits, so tuning never changes results."""
+
+    def __init__(self, lib, reps=3):
+        self.lib = lib
+        self.reps = reps
+        self.nvar = {m: int(lib.tnx_gemm_variants(m)) for m in (L.GEMM_F64, L.GEMM_F32,
+                                                                  L.GEMM_BF16_F32,
+                                                                  L.GEMM_BF16_BF16)}
+        self.table = {}
+
+    @staticmethod
+    def shape_class(mode, ntiles, kmax):
+        return (mode, int(ntiles).bit_length(), int(kmax).bit_length())
+
+    def mode_for(self, mode, ntiles, kmax):
+        """The encoded mode (variant in the second byte) of a launch."""
+        return mode | (self.table.get(self.shape_class(mode, ntiles, kmax), 0) << 8)
+
+    def tune(self, mode, ntiles, kmax, launch):
+        """Time every variant of ``mode`` with ``launch(encoded_mode)`` (a
+        launch that may run repeatedly: it must not read its own output)."""
+        key = self.shape_class(mode, ntiles, kmax)
+        if key in self.table or self.nvar.get(mode, 1) < 2 or ntiles == 0:
+            return
+        best, best_t = 0, None
+        for v in range(self.nvar[mode]):
+            m = mode | (v << 8)
+            launch(m)   # warm-up
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(self.reps):
+                launch(m)
+            e1.record()
+            e1.synchronize()
+            t = e0.elapsed_time(e1)
+            if best_t is None or t < best_t:
+                best, best_t = v, t
+        self.table[key] = best
+
+
 class _Resident:
     """A set of descriptor arrays uploaded once and kept in device memory
     (steps that repeat every sweep reuse them without host work)."""
@@ -979,15 +1034,15 @@ def assemble_pre(pairs):
             stage = [c[k][1] for c in cqs]
             if kind == 'g':
                 p, sg, nt = R.concat_gemm(stage)
-                cq_meta.append(('g', len(p), nt, len(arrs)))
+                cq_meta.append(('g', len(p), nt, len(arrs), gemm_kmax(p, sg)))
                 arrs += [p, sg]
             elif kind == 'm':
                 j, t, nb = R.concat_mix(stage)
-                cq_meta.append(('m', len(j), nb, len(arrs)))
+                cq_meta.append(('m', len(j), nb, len(arrs), 0))
                 arrs += [j, t]
             else:
                 jobs = R.concat_plain(stage, L.CHOL_JOB)
-                cq_meta.append(('c', len(jobs), 0, len(arrs)))
+                cq_meta.append(('c', len(jobs), 0, len(arrs), 0))
                 arrs.append(jobs)
     # block-Jacobi SVD moves: jobs (pair prefix sums over the batch), then
     # the GEMMs and copies that apply u and v
@@ -1001,10 +1056,10 @@ def assemble_pre(pairs):
         arrs.append(svdj)
         pg, ps, pnt = R.concat_gemm(_parts(pairs, 'svd_post_g'))
         pj, pt, pnb = R.concat_mix(_parts(pairs, 'svd_post_m'))
-        svdj_meta += ((len(pg), pnt, len(arrs)), (len(pj), pnb, len(arrs) + 2))
+        svdj_meta += ((len(pg), pnt, len(arrs), gemm_kmax(pg, ps)), (len(pj), pnb, len(arrs) + 2))
         arrs += [pg, ps, pj, pt]
     meta = {'nqr': len(qr), 'nsvd': len(svd),
-            'gemm': [(len(p), nt) for p, _, nt in gemms],
+            'gemm': [(len(p), nt, gemm_kmax(p, sg)) for p, sg, nt in gemms],
             'mix': [(len(j), nb) for j, _, nb in mixes], 'cq': cq_meta, 'svdj': svdj_meta}
     return arrs, meta
 
@@ -1045,7 +1100,8 @@ def assemble_lanczos(groups):
         mxj, mxt, mxb = R.concat_mix([s.part(ph, 'mv_mix') for s, ph in g], actives)
         g2p, g2s, g2t = R.concat_gemm([s.part(ph, 'mv_g2') for s, ph in g], actives)
         _, _, mode1, mode2, _ = PREC[g[0][1]['prec']]
-        gmeta.append((mode1, mode2, (len(g1p), g1t), (len(mxj), mxb), (len(g2p), g2t)))
+        gmeta.append((mode1, mode2, (len(g1p), g1t, gemm_kmax(g1p, g1s)), (len(mxj), mxb),
+                      (len(g2p), g2t, gemm_kmax(g2p, g2s))))
         arrs += [g1p, g1s, mxj, mxt, g2p, g2s]
     meta = {'lc': (len(lc_j), lc_nb), 'nchunks': int(chunks.sum()), 'groups': gmeta}
     return arrs, meta
@@ -1064,7 +1120,7 @@ class Engine:
 
     def __init__(self, precision_schedule=None, ncv=20, max_restarts=20, min_iters=1,
                  device='cuda:0', lowp=True, ops_capacity=4096, plan_ahead=True,
-                 cycle_steps=False):
+                 cycle_steps=False, autotune=True):
         require_gpu()
         self.lib = L.load()
         self.device = torch.device(device)
@@ -1093,6 +1149,8 @@ class Engine:
                          'lz_time': 0.0, 't_plan': 0.0, 't_pre': 0.0, 't_lz_total': 0.0,
                          'lz_iters': 0, 't_settle': 0.0, 'ahead_used': 0, 'resumes': 0}
         self.plan_ahead = bool(plan_ahead)
+        # kernel variant per grouped-GEMM shape class, timed on first use
+        self.tuner = GemmTuner(self.lib) if autotune else None
         # cycle_steps: each Lanczos call runs one Krylov cycle; a solve that
         # has not converged continues from its Ritz vector in the next step
         # ('resume', the same iterates as a restart inside the call) while
@@ -1312,6 +1370,27 @@ class Engine:
         L.check(self.lib.tnx_svd_jacobi_f64(ctypes_void(dj), len(jobs), max_sweeps, self.stream()),
                 'tnx_svd_jacobi_f64')
 
+    def _tuned_gemm(self, mode, prob, seg, g):
+        """Encoded mode of a matvec GEMM launch ``g = (nprob, ntiles, kmax)``
+        at device descriptors ``prob``/``seg``; a new shape class is tuned
+        first (a matvec GEMM does not read its own output, so the timing
+        launches leave no trace)."""
+        if self.tuner is None:
+            return mode
+        n, nt, kmax = g
+        if nt:
+            st = self.stream()
+
+            def launch(m):
+                L.check(self.lib.tnx_gemm_grouped(m, ctypes_void(prob), n, ctypes_void(seg), nt,
+                                                  None, st), 'tnx_gemm_grouped (tuning)')
+            self.tuner.tune(mode, nt, kmax, launch)
+        return self.tuner.mode_for(mode, nt, kmax)
+
+    def _gemm_mode(self, mode, ntiles, kmax):
+        """Encoded mode of a launch whose shape class may have been tuned."""
+        return mode if self.tuner is None else self.tuner.mode_for(mode, ntiles, kmax)
+
     def _lanczos(self, groups, spec=None, ahead=None):
         """One batched Lanczos call over ``groups``: lists of (solve, plan),
         one list per matvec precision (each Krylov step runs every group's
@@ -1334,9 +1413,11 @@ class Engine:
         plans = (L.MatvecPlan * len(groups))()
         for k, (mode1, mode2, g1, mx, g2) in enumerate(meta['groups']):
             b = 3 + 6 * k
+            m1, m2 = self._tuned_gemm(mode1, ptrs[b], ptrs[b + 1], g1), \
+                self._tuned_gemm(mode2, ptrs[b + 4], ptrs[b + 5], g2)
             plans[k] = L.MatvecPlan(ptrs[b], ptrs[b + 1], g1[0], g1[1], ptrs[b + 2], ptrs[b + 3],
                                     mx[0], mx[1], ptrs[b + 4], ptrs[b + 5], g2[0], g2[1],
-                                    self.ops.data_ptr(), mode1, mode2)
+                                    self.ops.data_ptr(), m1, m2)
         need = int(self.lib.tnx_lanczos_state_bytes(len(flat), nchunks, self.ncv))
         if self.lz_state is None or self.lz_state.numel() < need:
             self.lz_state = torch.empty(max(need, 1 << 20), dtype=torch.uint8, device=self.device)
@@ -1411,9 +1492,10 @@ class Engine:
         opsp = ctypes_void(self.ops.data_ptr())
 
         def run_gemm(i):
-            n_p, nt = meta['gemm'][i]
+            n_p, nt, kmax = meta['gemm'][i]
             if nt:
-                L.check(lib.tnx_gemm_grouped(L.GEMM_F64, ctypes_void(ptr[2 + 2 * i]), n_p,
+                L.check(lib.tnx_gemm_grouped(self._gemm_mode(L.GEMM_F64, nt, kmax),
+                                             ctypes_void(ptr[2 + 2 * i]), n_p,
                                              ctypes_void(ptr[3 + 2 * i]), nt, None, st_), 'gemm')
 
         def run_mix(i):
@@ -1426,10 +1508,11 @@ class Engine:
             L.check(lib.tnx_qr_f64(ctypes_void(ptr[0]), meta['nqr'], st_), 'tnx_qr_f64')
         if meta['nsvd']:
             L.check(lib.tnx_svd_jacobi_f64(ctypes_void(ptr[1]), meta['nsvd'], 60, st_), 'tnx_svd')
-        for (kind, cnt, nt, ai) in meta['cq']:   # CholeskyQR3 stages
+        for (kind, cnt, nt, ai, kmax) in meta['cq']:   # CholeskyQR3 stages
             if kind == 'g':
                 if nt:
-                    L.check(lib.tnx_gemm_grouped(L.GEMM_F64, ctypes_void(ptr[ai]), cnt,
+                    L.check(lib.tnx_gemm_grouped(self._gemm_mode(L.GEMM_F64, nt, kmax),
+                                                 ctypes_void(ptr[ai]), cnt,
                                                  ctypes_void(ptr[ai + 1]), nt, None, st_), 'cq gemm')
             elif kind == 'm':
                 if nt:
@@ -1438,7 +1521,7 @@ class Engine:
             elif cnt:
                 L.check(lib.tnx_chol_inv_f64(ctypes_void(ptr[ai]), cnt, st_), 'tnx_chol_inv_f64')
         if meta['svdj'] is not None:   # SVD of the CholeskyQR factors, then apply u and v
-            nj, npairs, max_nblk, max_n, ai, (gp, gnt, gi), (mj, mnb, mi) = meta['svdj']
+            nj, npairs, max_nblk, max_n, ai, (gp, gnt, gi, gk), (mj, mnb, mi) = meta['svdj']
             rc = lib.tnx_svd_block_jacobi(ctypes_void(ptr[ai]), nj, npairs, max_nblk, max_n,
                                           ctypes_void(self.svd_flag.data_ptr()), 60, st_)
             if rc == L.ENOCONV:
@@ -1446,7 +1529,8 @@ class Engine:
             else:
                 L.check(rc, 'tnx_svd_block_jacobi')
             if gnt:
-                L.check(lib.tnx_gemm_grouped(L.GEMM_F64, ctypes_void(ptr[gi]), gp,
+                L.check(lib.tnx_gemm_grouped(self._gemm_mode(L.GEMM_F64, gnt, gk),
+                                             ctypes_void(ptr[gi]), gp,
                                              ctypes_void(ptr[gi + 1]), gnt, None, st_), 'svd gemm')
             if mnb:
                 L.check(lib.tnx_mix(ctypes_void(ptr[mi]), mj, ctypes_void(ptr[mi + 1]), mnb, opsp,

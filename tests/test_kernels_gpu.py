@@ -96,6 +96,47 @@ def test_gemm_grouped_matches_numpy(gpu, mode):
         assert err < tol, (mode, sh, err)
 
 
+def test_gemm_f64_variants_agree(gpu):
+    """Every kernel variant of the f64 GEMM gives the same bits (they issue
+    the same MFMAs in the same k order); an unknown variant is refused."""
+    import torch
+    from tnalg_amd import _lib as L
+    from tnalg_amd import records as R
+    lib = L.load()
+    nvar = lib.tnx_gemm_variants(L.GEMM_F64)
+    assert nvar >= 2 and lib.tnx_gemm_variants(L.GEMM_BF16_F32) == 1
+    rng = np.random.RandomState(12)
+    g = R.GemmList()
+    keep, outs = [], []
+    for (M, N, K, ta, tb) in [(64, 64, 128, 0, 0), (130, 70, 512, 1, 0), (33, 200, 37, 0, 1),
+                              (256, 128, 300, 1, 1)]:
+        A = _dev(rng.randn(K, M) if ta else rng.randn(M, K))
+        B = _dev(rng.randn(N, K) if tb else rng.randn(K, N))
+        C0 = _dev(rng.randn(M, N))
+        keep += [A, B, C0]
+        C = torch.zeros((M, N), dtype=torch.float64, device='cuda')
+        outs.append(C)
+        g.add(C.data_ptr(), M, N, N, [(A.data_ptr(), A.shape[1], ta, B.data_ptr(), B.shape[1], tb,
+                                       K, 0.75)], c0=C0.data_ptr(), ldc0=N, beta=-0.5)
+    p, s = g.arrays()
+    dp, ds = _upload([p, s])
+    got = []
+    for v in range(nvar):
+        print('gemm f64 variant', v, flush=True)
+        for C in outs:
+            C.zero_()
+        L.check(lib.tnx_gemm_grouped(L.GEMM_F64 | (v << 8), _vp(dp.data_ptr()), len(p),
+                                     _vp(ds.data_ptr()), R.gemm_tiles(p), None, _stream()), 'gemm')
+        torch.cuda.synchronize()
+        got.append([C.cpu().numpy().copy() for C in outs])
+    for v in range(1, nvar):
+        for a, b in zip(got[0], got[v]):
+            assert np.array_equal(a, b), v
+    rc = lib.tnx_gemm_grouped(L.GEMM_F64 | (nvar << 8), _vp(dp.data_ptr()), len(p),
+                              _vp(ds.data_ptr()), R.gemm_tiles(p), None, _stream())
+    assert rc == L.EINVAL
+
+
 def test_gemm_active_mask_skips_problems(gpu):
     import torch
     from tnalg_amd import _lib as L

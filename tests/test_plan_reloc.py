@@ -327,3 +327,32 @@ def test_plan_ahead_matches_inline_planning():
             assert 'lz' not in spec
         steps += 1
     assert steps > 20
+
+
+def test_gemm_kmax_and_tuner_encoding():
+    """Largest total K of a launch's problems; the tuner's shape classes and
+    the variant carried in the mode's second byte (untuned classes: 0)."""
+    from tnalg_amd.engine import GemmTuner, gemm_kmax
+    g = R.GemmList()
+    g.add(4096, 8, 8, 8, [(8192, 8, 0, 8200, 8, 0, 100, 1.0), (8192, 8, 0, 8200, 8, 0, 28, 1.0)])
+    g.add(4096, 8, 8, 8, [(8192, 8, 0, 8200, 8, 0, 300, 1.0)])
+    g.add(4096, 8, 8, 8, [(8192, 8, 0, 8200, 8, 0, 0, 1.0)])   # no segment with K > 0
+    p, s = g.arrays()
+    assert gemm_kmax(p, s) == 300 and gemm_kmax(p[:1], s) == 128
+    assert gemm_kmax(p[:0], s) == 0
+
+    class FakeLib:
+        @staticmethod
+        def tnx_gemm_variants(m):
+            return 2 if m == L.GEMM_F64 else 1
+
+    t = GemmTuner(FakeLib())
+    assert t.nvar[L.GEMM_F64] == 2 and t.nvar[L.GEMM_BF16_F32] == 1
+    assert t.mode_for(L.GEMM_F64, 2048, 128) == L.GEMM_F64
+    t.table[t.shape_class(L.GEMM_F64, 2048, 128)] = 1
+    assert t.mode_for(L.GEMM_F64, 2048, 128) == L.GEMM_F64 | (1 << 8)
+    assert t.mode_for(L.GEMM_F64, 3000, 200) == L.GEMM_F64 | (1 << 8)   # same class
+    assert t.mode_for(L.GEMM_F64, 1500, 100) == L.GEMM_F64              # another class
+    assert t.mode_for(L.GEMM_F64, 512, 512) == L.GEMM_F64
+    t.tune(L.GEMM_F32, 512, 512, None)      # a mode with one variant is not timed
+    assert t.shape_class(L.GEMM_F32, 512, 512) not in t.table
