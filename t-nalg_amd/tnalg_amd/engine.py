@@ -1074,6 +1074,18 @@ def _lanczos_groups(pairs):
     return [by[p] for p in sorted(by)]
 
 
+def _split_groups(groups, nstreams, split_min):
+    """The Lanczos calls of a step: all ``groups`` in one call, or (two
+    streams and at least ``split_min`` solves) every precision group halved
+    into two calls."""
+    n = sum(len(g) for g in groups)
+    if nstreams < 2 or n < split_min:
+        return [groups]
+    a = [g[:(len(g) + 1) // 2] for g in groups]
+    b = [g[(len(g) + 1) // 2:] for g in groups]
+    return [[g for g in a if g], [g for g in b if g]]
+
+
 def _lanczos_key(groups):
     return ('lz',) + tuple(tuple((ph['uid'],) + s.reloc_key for s, ph in g) for g in groups)
 
@@ -1120,7 +1132,7 @@ class Engine:
 
     def __init__(self, precision_schedule=None, ncv=20, max_restarts=20, min_iters=1,
                  device='cuda:0', lowp=True, ops_capacity=4096, plan_ahead=True,
-                 cycle_steps=False, autotune=True):
+                 cycle_steps=False, autotune=True, lanczos_streams=2):
         require_gpu()
         self.lib = L.load()
         self.device = torch.device(device)
@@ -1138,7 +1150,13 @@ class Engine:
         self.ops_free = [(0, ops_capacity)]
         self.ops_d2 = None
         self.solves = []
-        self.lz_state = None
+        self.lz_states = [None, None]   # Lanczos state buffers of the (at most two) calls
+        self.lanczos_streams = int(lanczos_streams)
+        if self.lanczos_streams not in (1, 2):
+            raise ValueError('lanczos_streams must be 1 or 2')
+        self.split_min = 16   # smallest batch split over two streams
+        self._lz_stream = None
+        self._lz_pool = None
         self._resident = collections.OrderedDict()
         self._resident_bytes = 0
         self.resident_limit = 1 << 30   # descriptor sets of every distinct step of a sweep
@@ -1210,9 +1228,11 @@ class Engine:
             spec['pre'] = (rkey, assemble_pre(pairs))
         groups = _lanczos_groups(pairs)
         if groups:
-            rkey = _lanczos_key(groups)
-            if rkey not in self._resident:
-                spec['lz'] = (rkey, assemble_lanczos(groups))
+            spec['lz'] = []
+            for part in _split_groups(groups, self.lanczos_streams, self.split_min):
+                rkey = _lanczos_key(part)
+                if rkey not in self._resident:
+                    spec['lz'].append((rkey, assemble_lanczos(part)))
         return spec
 
     # ------------------------------------------------------------ op table
@@ -1392,63 +1412,102 @@ class Engine:
         return mode if self.tuner is None else self.tuner.mode_for(mode, ntiles, kmax)
 
     def _lanczos(self, groups, spec=None, ahead=None):
-        """One batched Lanczos call over ``groups``: lists of (solve, plan),
+        """One batched Lanczos solve over ``groups``: lists of (solve, plan),
         one list per matvec precision (each Krylov step runs every group's
         matvec, then the shared f64 kernels).  ``spec``: descriptor sets
         assembled ahead; ``ahead``: solves whose next step the worker plans
-        while the call runs."""
+        while the calls run.
+
+        With ``lanczos_streams = 2`` a large batch is split in two halves
+        (every precision group halved) that run as two Lanczos calls on two
+        streams, each driven by its own host thread: one half's matvec
+        kernels (latency bound) overlap the other half's passes over its
+        Krylov basis (bandwidth bound).  A solve's arithmetic does not depend
+        on the other solves of its call, so the results are the same bits."""
+        parts = _split_groups(groups, self.lanczos_streams, self.split_min)
+        got = spec.get('lz') if spec else None
+        preps = [self._lanczos_prepare(k, part, got) for k, part in enumerate(parts)]
+        if ahead:
+            self._start_ahead(ahead)
+        restarts = 0 if self.cycle_steps else self.max_restarts
+        t0 = time.perf_counter()
+        if len(preps) == 1:
+            rcs = [self._lanczos_call(preps[0], self.stream(), restarts)]
+        else:
+            # the second stream starts after everything queued so far
+            if self._lz_stream is None:
+                self._lz_stream = torch.cuda.Stream(device=self.device)
+            ev = torch.cuda.Event()
+            ev.record()
+            self._lz_stream.wait_event(ev)
+            if self._lz_pool is None:
+                self._lz_pool = concurrent.futures.ThreadPoolExecutor(1)
+            fut = self._lz_pool.submit(self._lanczos_call, preps[1],
+                                       ctypes_void(self._lz_stream.cuda_stream), restarts)
+            rc0 = self._lanczos_call(preps[0], self.stream(), restarts)
+            rcs = [rc0, fut.result()]   # that call ends with its stream synchronised
+        self.counters['lz_time'] += time.perf_counter() - t0
+        for rc in rcs:
+            if rc not in (0, -3):
+                L.check(rc, 'tnx_lanczos_run')
+        self.counters['lanczos_calls'] += 1
+        for prep in preps:
+            for (s, ph), r in zip(prep['flat'], prep['res']):
+                it = int(r['iters'])
+                s.stats['matvecs'] += it
+                self.counters['matvecs'] += it
+                self.counters['matvec_flops'] += ph['mv_flops'] * it
+                if not r['converged'] and self.cycle_steps and s.cycles < self.max_restarts:
+                    s.cycles += 1
+                    s.resume = True   # the step re-queues the eigensolve
+                    self.counters['resumes'] += 1
+                    continue
+                s.cycles = 0
+                s.last_theta = float(r['theta'])
+                s.thetas.append(s.last_theta)
+                if not r['converged']:
+                    s.stats['unconverged'] += 1
+
+    def _lanczos_prepare(self, k, groups, got):
+        """Resident descriptors, the linear-combination mix (on the engine's
+        stream), matvec plans with tuned GEMM variants, the state buffer and
+        the result array of Lanczos call ``k``."""
         flat = [sp for g in groups for sp in g]
         rkey = _lanczos_key(groups)
         ent = self._resident_get(rkey)
         if ent is None:
-            got = spec.get('lz') if spec else None
-            ent = self._resident_put(rkey, *(got[1] if got and got[0] == rkey
+            ahead = dict(got).get(rkey) if got else None
+            ent = self._resident_put(rkey, *(ahead if ahead is not None
                                              else assemble_lanczos(groups)))
         ptrs, meta = ent.ptrs, ent.meta
-        nchunks = meta['nchunks']
         if meta['lc'][1]:
             L.check(self.lib.tnx_mix(ctypes_void(ptrs[0]), meta['lc'][0], ctypes_void(ptrs[1]),
                                      meta['lc'][1], ctypes_void(self.ops.data_ptr()), None,
                                      self.stream()), 'tnx_mix')
         plans = (L.MatvecPlan * len(groups))()
-        for k, (mode1, mode2, g1, mx, g2) in enumerate(meta['groups']):
-            b = 3 + 6 * k
+        for q, (mode1, mode2, g1, mx, g2) in enumerate(meta['groups']):
+            b = 3 + 6 * q
             m1, m2 = self._tuned_gemm(mode1, ptrs[b], ptrs[b + 1], g1), \
                 self._tuned_gemm(mode2, ptrs[b + 4], ptrs[b + 5], g2)
-            plans[k] = L.MatvecPlan(ptrs[b], ptrs[b + 1], g1[0], g1[1], ptrs[b + 2], ptrs[b + 3],
+            plans[q] = L.MatvecPlan(ptrs[b], ptrs[b + 1], g1[0], g1[1], ptrs[b + 2], ptrs[b + 3],
                                     mx[0], mx[1], ptrs[b + 4], ptrs[b + 5], g2[0], g2[1],
                                     self.ops.data_ptr(), m1, m2)
-        need = int(self.lib.tnx_lanczos_state_bytes(len(flat), nchunks, self.ncv))
-        if self.lz_state is None or self.lz_state.numel() < need:
-            self.lz_state = torch.empty(max(need, 1 << 20), dtype=torch.uint8, device=self.device)
-        res = np.zeros(len(flat), dtype=L.LANCZOS_RESULT)
-        if ahead:
-            self._start_ahead(ahead)
-        t0 = time.perf_counter()
-        restarts = 0 if self.cycle_steps else self.max_restarts
-        rc = self.lib.tnx_lanczos_run_multi(plans, len(groups), ctypes_void(ptrs[2]), len(flat),
-                                            nchunks, self.ncv, restarts, self.min_iters,
-                                            ctypes_void(self.lz_state.data_ptr()),
-                                            ctypes_void(res.ctypes.data), self.stream())
-        self.counters['lz_time'] += time.perf_counter() - t0
-        if rc not in (0, -3):
-            L.check(rc, 'tnx_lanczos_run')
-        self.counters['lanczos_calls'] += 1
-        for (s, ph), r in zip(flat, res):
-            it = int(r['iters'])
-            s.stats['matvecs'] += it
-            self.counters['matvecs'] += it
-            self.counters['matvec_flops'] += ph['mv_flops'] * it
-            if not r['converged'] and self.cycle_steps and s.cycles < self.max_restarts:
-                s.cycles += 1
-                s.resume = True   # the step re-queues the eigensolve
-                self.counters['resumes'] += 1
-                continue
-            s.cycles = 0
-            s.last_theta = float(r['theta'])
-            s.thetas.append(s.last_theta)
-            if not r['converged']:
-                s.stats['unconverged'] += 1
+        need = int(self.lib.tnx_lanczos_state_bytes(len(flat), meta['nchunks'], self.ncv))
+        if self.lz_states[k] is None or self.lz_states[k].numel() < need:
+            self.lz_states[k] = torch.empty(max(need, 1 << 20), dtype=torch.uint8,
+                                            device=self.device)
+        return {'flat': flat, 'plans': plans, 'ngroups': len(groups), 'ptrs': ptrs,
+                'nchunks': meta['nchunks'], 'state': self.lz_states[k],
+                'res': np.zeros(len(flat), dtype=L.LANCZOS_RESULT)}
+
+    def _lanczos_call(self, prep, stream, restarts):
+        """The C Lanczos loop of one prepared call on ``stream`` (holds no
+        GIL; returns when the stream's work is done)."""
+        return self.lib.tnx_lanczos_run_multi(prep['plans'], prep['ngroups'],
+                                              ctypes_void(prep['ptrs'][2]), len(prep['flat']),
+                                              prep['nchunks'], self.ncv, restarts, self.min_iters,
+                                              ctypes_void(prep['state'].data_ptr()),
+                                              ctypes_void(prep['res'].ctypes.data), stream)
 
     def step(self, solves=None):
         """Advance every solve with a queued entry by one step."""
@@ -1548,7 +1607,9 @@ class Engine:
             # the next step's host work overlaps this Lanczos call, when this
             # step needed host work (a batch whose descriptor sets are all
             # resident repeats a known sequence and has none to overlap)
-            host_miss = host_miss or _lanczos_key(groups) not in self._resident
+            host_miss = host_miss or any(
+                _lanczos_key(part) not in self._resident
+                for part in _split_groups(groups, self.lanczos_streams, self.split_min))
             ahead = None
             if self.plan_ahead and host_miss:
                 ahead = [s for s, _, _ in plans if s.queue]

@@ -281,7 +281,8 @@ def test_plan_ahead_matches_inline_planning():
     """The worker's next-step host work: each solve's planned-ahead step is
     the plan inline planning returns, the solve's dims are left as they were,
     and the speculative descriptor sets equal the ones assembled inline."""
-    from tnalg_amd.engine import Engine, assemble_lanczos, assemble_pre, _lanczos_groups
+    from tnalg_amd.engine import (Engine, assemble_lanczos, assemble_pre, _lanczos_groups,
+                                  _lanczos_key, _split_groups)
 
     class AheadEngine(CpuEngine):
         _plan_ahead = Engine._plan_ahead
@@ -291,6 +292,7 @@ def test_plan_ahead_matches_inline_planning():
             super().__init__(lowp=True)
             self._resident = {}
             self.precision_schedule = lambda sweep, nsweeps: ('bf16', 'fp32', 'fp64')[sweep]
+            self.lanczos_streams, self.split_min = 2, 2   # the Lanczos batch in two calls
 
     eng = AheadEngine()
     solves = [Solve(eng, _para('chain', l=10, chi=16, sweep_time=3), seed=1),
@@ -321,10 +323,17 @@ def test_plan_ahead_matches_inline_planning():
         groups = _lanczos_groups(pairs)
         if groups:
             la, lm = assemble_lanczos(groups)
-            assert spec['lz'][1][1] == lm
-            assert all(a.tobytes() == b.tobytes() for a, b in zip(spec['lz'][1][0], la))
+            parts = _split_groups(groups, 2, 2)
+            nsolve = sum(1 for _, ph in pairs if 'lz' in ph)
+            assert len(parts) == (2 if nsolve >= 2 else 1)
+            assert sum(len(g) for p in parts for g in p) == nsolve
+            assert [k for k, _ in spec['lz']] == [_lanczos_key(p) for p in parts]
+            for part, (_, (arrs_a, meta_a)) in zip(parts, spec['lz']):
+                la, lm = assemble_lanczos(part)
+                assert meta_a == lm
+                assert all(a.tobytes() == b.tobytes() for a, b in zip(arrs_a, la))
         else:
-            assert 'lz' not in spec
+            assert not spec.get('lz')
         steps += 1
     assert steps > 20
 

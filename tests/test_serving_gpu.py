@@ -50,3 +50,28 @@ def test_alone_equals_in_mix(gpu, precision):
         mixed = {r.rid: r.energy for r in srv.finished}
         for k in range(len(reqs)):
             assert mixed[k] == alone[k], (cycle, k, mixed[k], alone[k])
+
+
+def test_two_lanczos_streams_same_bits(gpu):
+    """A batch large enough to be split (every precision group halved into
+    two Lanczos calls on two streams, each driven by its own host thread)
+    gives bit-identical energies to one call on one stream."""
+    from tnalg_amd.dmrg import _schedule_from
+    from tnalg_amd.engine import Engine
+    from tnalg_amd.serving import poisson_xxz_stream
+    reqs = poisson_xxz_stream(20, rate=1000.0, seed=7, n_range=(10, 18), chis=(8, 16),
+                              sweeps=3, eigs_tol=1e-10)
+    out = {}
+    for streams in (1, 2):
+        print('batch of', len(reqs), 'lanczos_streams', streams, flush=True)
+        eng = Engine(precision_schedule=_schedule_from('mixed'), lowp=True, ncv=NCV,
+                     lanczos_streams=streams)
+        solves = eng.add_solves([(para, seed) for _, para, seed in reqs])
+        for s, (_, para, _) in zip(solves, reqs):
+            s.schedule_canonicalise()
+            for t in range(int(para['sweep_time'])):
+                s.schedule_sweep(t)
+        eng.run_queues(solves)
+        out[streams] = eng.finalize(solves)
+    assert len(out[2]) == len(reqs)
+    assert out[1] == out[2]
