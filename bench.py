@@ -42,7 +42,9 @@ def c2_para(n=64, chi=128, sweeps=8, jz=1.0, hz=0.0, tol=1e-10):
 _ENGINES = {}
 _PLAN_AHEAD = [True]   # --no-plan-ahead: next-step host work in line (for comparison)
 _CYCLE_STEPS = [True]  # serving: one Krylov cycle per Lanczos call (--no-cycle-steps: all)
-_LZ_STREAMS = [2]      # Lanczos calls per step over two streams (--lanczos-streams 1: one)
+_LZ_STREAMS = [None]   # Lanczos calls of a step over this many streams (None: 2 for the
+                       # fixed batch, 1 for serving, where one Krylov cycle per call and
+                       # small mixed groups measured faster on one stream)
 
 
 def engine_for(precision, ncv=20):
@@ -55,7 +57,7 @@ def engine_for(precision, ncv=20):
     if key not in _ENGINES:
         _ENGINES[key] = Engine(precision_schedule=_schedule_from(precision), ncv=ncv,
                                lowp=precision not in (None, 'fp64'), plan_ahead=_PLAN_AHEAD[0],
-                               lanczos_streams=_LZ_STREAMS[0])
+                               lanczos_streams=_LZ_STREAMS[0] or 2)
     return _ENGINES[key]
 
 
@@ -251,7 +253,7 @@ def serving_run(n_req, rate, max_active=64, seed=0):
     from tnalg_amd.serving import SolveServer, poisson_xxz_stream
     reqs = poisson_xxz_stream(n_req, rate=rate, seed=seed)
     eng = Engine(precision_schedule=_schedule_from('mixed'), lowp=True, plan_ahead=_PLAN_AHEAD[0],
-                 cycle_steps=_CYCLE_STEPS[0], lanczos_streams=_LZ_STREAMS[0])
+                 cycle_steps=_CYCLE_STEPS[0], lanczos_streams=_LZ_STREAMS[0] or 1)
     free = torch.cuda.mem_get_info()[0]
     srv = SolveServer(eng, hbm_budget=int(0.6 * free), max_active=max_active)
     t0 = time.perf_counter()
@@ -271,7 +273,7 @@ def serving_run(n_req, rate, max_active=64, seed=0):
                            'wait_for_plan_ahead': c['t_settle'], 'engine_steps': c['steps']}
     summ['lanczos'] = {'calls': c['lanczos_calls'], 'matvecs': c['matvecs'],
                        'resumed_cycles': c['resumes'], 'cycle_steps': eng.cycle_steps,
-                       'plan_ahead': eng.plan_ahead}
+                       'plan_ahead': eng.plan_ahead, 'streams': eng.lanczos_streams}
     summ['requests'] = n_req
     summ['arrival_rate_per_s'] = rate
     summ['max_active'] = max_active
@@ -382,8 +384,9 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-plan-ahead', action='store_true',
                     help='do the next step\'s host work in line instead of on a worker thread')
-    ap.add_argument('--lanczos-streams', type=int, default=2, choices=[1, 2],
-                    help='Lanczos calls of a step split over this many streams')
+    ap.add_argument('--lanczos-streams', type=int, default=None, choices=[1, 2],
+                    help='Lanczos calls of a step split over this many streams '
+                         '(default: 2 for the batch, 1 for serving)')
     ap.add_argument('--no-cycle-steps', action='store_true',
                     help='serving: restart unconverged local solves inside the Lanczos call')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
