@@ -193,7 +193,7 @@ GEMM_MODE_NAME = {0: 'f64', 1: 'f32', 2: 'bf16 -> f32', 3: 'bf16 -> bf16'}
 def roofline_object(para, args):
     """Roofline of the dominant kernel.  In the default mixed-precision run
     that is ``gemm_grouped_kernel<0>``, the f64 MFMA GEMM (the f64 sweeps'
-    matvecs, every gauge move and environment update: about a quarter of the
+    matvecs, every gauge move and environment update: about a fifth of the
     run's GPU time, the largest share of any kernel); it is measured on its
     matvec launches at a mid-chain step (the longer of GEMM1 and GEMM2).  The
     low-precision matvec GEMMs of the same step are listed under
