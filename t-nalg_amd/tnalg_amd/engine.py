@@ -168,11 +168,20 @@ def max_dims(dims, d, ob):
 
 def buffer_layout(T, dm, d, ncv, lowp):
     """Sizes of every device buffer of a solve (elements for f64 buffers,
-    bytes for the ``uint8`` ones); also the HBM footprint used for admission."""
+    bytes for the ``uint8`` ones); also the HBM footprint used for admission.
+    With two-site tables the Krylov and matvec buffers hold a merged pair
+    (Dl d^2 Dr), the pair tensor theta has its own buffer, and the split's
+    square matrices (order up to max(Dl d, d Dr)) fit the CholeskyQR and
+    scratch buffers."""
     n = T.n
-    st = T.sites
+    st = T.sites + T.pairs
     psi = max(dm[s] * d * dm[s + 1] for s in range(n))
     dsq = max(x * x for x in dm)
+    dmx = max(dm)
+    if T.pairs:
+        psi = max(psi, max(dm[p] * d * d * dm[p + 2] for p in range(n - 1)))
+        dmx = max(dmx, max(max(dm[p], dm[p + 2]) * d for p in range(n - 1)))
+        dsq = max(dsq, dmx * dmx)
     nL = [len(T.left_opens[b]) for b in range(n + 1)]
     nR = [len(T.right_opens[b]) for b in range(n + 1)]
     n_lm = max(len(s.lm) for s in st)
@@ -191,10 +200,10 @@ def buffer_layout(T, dm, d, ncv, lowp):
         'Yb': (1 + max(n_lm, n_mr) + max(max(nL), max(nR))) * psi,
         'Wb': (1 + max(max(nL), max(nR))) * psi,
         'V': (ncv + 1) * psi,
-        'Rq': dsq + max(dm) + 64,
+        'Rq': dsq + dmx + 64,
         'qrw': 2 * psi + dsq,
-        'chol': 6 * dsq + max(dm) + 64,
-        'lmv': max(1, n - 1) * max(dm),   # entanglement spectra, one row of max(dm) per bond
+        'chol': 6 * dsq + dmx + 64,
+        'lmv': max(1, n - 1) * dmx,   # entanglement spectra, one row of dmx per bond
     }
     # arena: (name, index, bytes, f64 view?) in address order.  The MPS and
     # the CholeskyQR constants come first (staged from the host in one
@@ -216,6 +225,8 @@ def buffer_layout(T, dm, d, ncv, lowp):
              ('x', None, 8 * psi, False), ('init', None, 8 * psi, True),
              ('Rq', None, 8 * lay['Rq'], True), ('qrw', None, 8 * lay['qrw'], True),
              ('lmv', None, 8 * lay['lmv'], True)]
+    if T.pairs:
+        spec.append(('th', None, 8 * psi, True))
     offs, o = [], ARENA_HEADER
     for k, (_, _, nb, _) in enumerate(spec):
         if k == stage_end:
@@ -224,23 +235,54 @@ def buffer_layout(T, dm, d, ncv, lowp):
         o += (nb + ARENA_ALIGN - 1) // ARENA_ALIGN * ARENA_ALIGN
     lay['arena'] = [(name, idx, off, nb, f64) for (name, idx, nb, f64), off in zip(spec, offs)]
     lay['offsets'] = {(name, idx): off for name, idx, off, _, _ in lay['arena']}
-    lay['dmax_max'] = max(dm)
+    lay['dmax_max'] = dmx
     lay['bytes'] = o
     return lay
+
+
+def dmrg_sites(para):
+    """1 (one-site sweeps, the reference's scheme) or 2 (two-site sweeps with
+    SVD truncation to chi), from ``para['dmrg_sites']`` (default 1)."""
+    k = int(para.get('dmrg_sites', 1))
+    if k not in (1, 2):
+        raise ValueError('dmrg_sites must be 1 or 2 (got %r)' % para.get('dmrg_sites'))
+    return k
+
+
+def initial_chi(para):
+    """Bond dimension of the random initial MPS: ``chi``, or for two-site
+    sweeps ``para['chi_init']`` when given (the truncation then grows the
+    bonds up to ``chi``)."""
+    chi = int(para['chi'])
+    if dmrg_sites(para) == 2:
+        chi0 = int(para.get('chi_init', chi))
+        if not 1 <= chi0 <= chi:
+            raise ValueError('chi_init must be in [1, chi]')
+        return chi0
+    return chi
 
 
 def solve_geometry(para, ncv, lowp, dims=None):
     """(term tables, initial bond dims, per-bond maximum dims, arena layout)
     of a solve of ``para``.  ``dims``: bond dims of a given initial MPS;
-    default those of the random MPS, [1, chi, ..., chi, 1].  Layouts are kept
-    on the (shared, read-only) term tables."""
-    T = compile_terms_cached(para)
+    default those of the random MPS, [1, chi, ..., chi, 1] (chi_init for
+    two-site sweeps).  Buffers are sized for bonds up to chi.  Layouts are
+    kept on the (shared, read-only) term tables."""
+    two = dmrg_sites(para) == 2
+    T = compile_terms_cached(para, pairs=two)
+    chi = int(para['chi'])
     if dims is None:
-        chi = int(para['chi'])
-        dims = [1] + [chi] * (T.n - 1) + [1]
+        chi0 = initial_chi(para)
+        dims = [1] + [chi0] * (T.n - 1) + [1]
     ob = int(para.get('ob_position', 0))
-    dm = max_dims(dims, T.d, ob)
-    key = (tuple(dims), ob, ncv, bool(lowp))
+    if two:
+        if T.d * T.d > L.MIX_DMAX:
+            raise ValueError('two-site sweeps need d^2 <= %d (d = %d)' % (L.MIX_DMAX, T.d))
+        full = [1] + [chi] * (T.n - 1) + [1]
+        dm = max_dims([max(a, b) for a, b in zip(dims, full)], T.d, ob)
+    else:
+        dm = max_dims(dims, T.d, ob)
+    key = (tuple(dims), ob, ncv, bool(lowp), chi if two else 0)
     lay = T.layouts.get(key)
     if lay is None:
         lay = buffer_layout(T, dm, T.d, ncv, lowp)
@@ -319,7 +361,7 @@ class Solve:
         if not 0 <= self.ob < n:
             raise ValueError('ob_position must be in [0, l)')
         if stage is None:
-            host = (random_mps_host(n, d, chi, seed) if mps is None
+            host = (random_mps_host(n, d, initial_chi(para), seed) if mps is None
                     else [np.asarray(a, dtype=np.float64) for a in mps])
             stage = make_stage(lay, host, pin=engine.device.type != 'cpu')
         self.dims = list(dims)
@@ -329,7 +371,8 @@ class Solve:
         self.nR = [len(T.right_opens[b]) for b in range(n + 1)]
         self.layout = lay
         self.psi_max, self.dsq_max = lay['psi'], lay['dsq']
-        self.dmax_max = max(dm)
+        self.dmax_max = lay['dmax_max']
+        self.sites = dmrg_sites(para)
         self.lp_bytes = 4 if engine.uses_lowp else 0
         # one arena per solve; every buffer is a view into it
         self.arena = engine.arena_get(lay['bytes'])
@@ -350,7 +393,7 @@ class Solve:
         # staged prefix (initial MPS, CholeskyQR constants) from the pinned
         # image; the rest of the arena starts at zero
         engine.arena_fill(self.arena, stage)
-        self.ops_slot = engine.alloc_ops(T.ops)
+        self.ops_slot = engine.alloc_ops(T.ops, T.ops2 if T.pairs else None)
         # relocation of a shared plan to this solve: arena base, operator
         # slot, coupling-coefficient values
         self.reloc = (self.base, self.ops_slot, T.coef_values)
@@ -398,6 +441,12 @@ class Solve:
     def op_index(self, k):
         return self.ops_slot + k
 
+    def op_index2(self, k):
+        """Index of merged two-site operator k (d^2 x d^2).  The operator table
+        keeps them in a second region at a fixed offset from the solve's slot,
+        so a plan relocates them with the same slot shift as the site ones."""
+        return self.ops_slot + self.engine.ops_off2 + k
+
     def L(self, b, m):
         return _p(self.Lenv[b], m * self.dmax[b] ** 2)
 
@@ -422,7 +471,7 @@ class Solve:
             if hasattr(self, name):
                 setattr(self, name, None)
         for name in ('lc64', 'lc_lp', 'Tb', 'Ub', 'Yb', 'Wb', 'V', 'w', 'x', 'init', 'Rq', 'qrw',
-                     'cholb', 'lmv'):
+                     'cholb', 'lmv', 'th'):
             setattr(self, name, None)
         self.engine.arena_release(self.arena)
         self.arena = None
@@ -456,6 +505,8 @@ class Solve:
 
     def schedule_sweep(self, sweep):
         """One sweep of ``DMRG_anyH.py:47-64``: ob+1..N-1, N-2..0, 1..ob-1."""
+        if self.sites == 2:
+            return self._schedule_sweep2(sweep)
         n, ob = self.n, self.ob
         sites = [(s, +1) for s in range(ob + 1, n)] + [(s, -1) for s in range(n - 2, -1, -1)] + \
             [(s, +1) for s in range(1, ob)]
@@ -470,6 +521,39 @@ class Solve:
             c = s
         self._sched_center = c
 
+    def two_site_pairs(self):
+        """Pairs (p, p+1) of one two-site sweep from the centre ob, the same
+        route as the one-site sweep: right to the end, left to the start,
+        right back to ob; the turning pairs are solved once."""
+        n, ob = self.n, self.ob
+        right = list(range(ob, n - 1))
+        left = list(range(n - 3 if right else n - 2, -1, -1))
+        back = list(range(1 if left else ob, ob))
+        return right + left + back
+
+    def _schedule_sweep2(self, sweep):
+        """One two-site sweep.  Step ('solve2', p, dir) first splits the
+        previous pair p - dir (dir = +1: centre moves right onto p; -1: left
+        onto p+1; 0: nothing to split), updates the environment through the
+        new isometry, forms theta = A_p A_{p+1} and solves the pair.  The
+        closing ('split2', p, dir) splits the last pair so the sweep ends with
+        the centre at ob and a whole MPS."""
+        if self._sched_center != self.ob:
+            raise ValueError('a two-site sweep starts with the centre at ob_position')
+        if self.n < 2:
+            raise ValueError('two-site sweeps need at least 2 sites')
+        seq = self.two_site_pairs()
+        dirn = 0
+        for t, p in enumerate(seq):
+            self.queue.append(Step('solve2', p, dirn, sweep))
+            nxt = seq[t + 1] if t + 1 < len(seq) else None
+            if nxt is None:
+                dirn = +1 if self.ob == p + 1 else -1
+            else:
+                dirn = nxt - p
+        self.queue.append(Step('split2', seq[-1], dirn, sweep))
+        self._sched_center = self.ob
+
     # ------------------------------------------------------------- planning
     def plan(self, st, prec):
         """Descriptor records of one step (dict phase -> arrays), shared by
@@ -483,15 +567,16 @@ class Solve:
             self.dims = list(hit['dims_after'])
             return hit
         lo, hi = self.base, self.base + self.layout['bytes']
-        if st.kind != 'solve':
+        if st.kind not in ('solve', 'solve2'):
             ph = self._build_plan(st, prec)
             R.pack_relative(ph, lo, hi, self.ops_slot)
             cache[key] = ph
             return ph
-        # a solve step = the gauge move and environment update, which do not
-        # depend on the matvec precision (built once and shared by the plans
-        # of every precision), + the precision's cast and local records
-        ckey = (self.sig, 'solve-env', st.site, st.dir, tuple(self.dims))
+        # a solve step = the gauge move (two-site: split of the previous
+        # pair) and environment update, which do not depend on the matvec
+        # precision (built once and shared by the plans of every precision),
+        # + the precision's cast and local records
+        ckey = (self.sig, st.kind + '-env', st.site, st.dir, tuple(self.dims))
         com = cache.get(ckey)
         if com is None:
             com = {}
@@ -511,26 +596,46 @@ class Solve:
         cache[key] = ph
         return ph
 
+    @staticmethod
+    def _env_site(st):
+        """Site whose new isometry the environment update of step ``st`` goes
+        through (the new block is at bond q+1 for dir > 0, q for dir < 0)."""
+        if st.kind == 'move':
+            return st.site
+        if st.kind == 'solve2':   # the split pair is site - dir
+            return st.site - 1 if st.dir > 0 else st.site + 2
+        if st.kind == 'split2':
+            return st.site if st.dir > 0 else st.site + 1
+        return st.site - st.dir
+
     def _plan_common(self, ph, st):
-        """Records of a 'move' or 'solve' step that do not depend on the
-        matvec precision: gauge move of the previous centre, environment
-        update through it."""
-        q = st.site if st.kind == 'move' else st.site - st.dir
+        """Records of a 'move', 'solve', 'solve2' or 'split2' step that do not
+        depend on the matvec precision: gauge move of the previous centre
+        (two-site: split of the previous pair), environment update through
+        it, and for 'solve2' the pair tensor theta."""
+        q = self._env_site(st)
+        if st.kind in ('solve2', 'split2'):
+            if st.dir:
+                self._plan_split(ph, st.site - st.dir if st.kind == 'solve2' else st.site, st.dir)
+                self._plan_env(ph, q, st.dir)
+            if st.kind == 'solve2':
+                self._plan_theta(ph, st.site)
+            return
         self._plan_gauge(ph, q, st.dir, copy_back=(st.kind == 'move'))
         self._plan_env(ph, q, st.dir)
 
     def _plan_precision(self, ph, st, prec):
-        """Records of a 'move' or 'solve' step that depend on the precision:
-        low-precision copy of the new environment block, local eigensolve."""
-        q = st.site if st.kind == 'move' else st.site - st.dir
-        if self.engine.uses_lowp and prec != 'fp64':
+        """Records of a step that depend on the precision: low-precision copy
+        of the new environment block, local eigensolve."""
+        q = self._env_site(st)
+        if st.dir and self.engine.uses_lowp and prec != 'fp64':
             self._plan_cast_block(ph, q + 1 if st.dir > 0 else q, st.dir > 0, PREC[prec][0])
-        if st.kind == 'solve':
-            self._plan_local(ph, st.site, prec)
+        if st.kind in ('solve', 'solve2'):
+            self._plan_local(ph, st.site, prec, pair=st.kind == 'solve2')
 
     def _build_plan(self, st, prec):
         ph = {}
-        if st.kind in ('move', 'solve'):
+        if st.kind in ('move', 'solve', 'solve2', 'split2'):
             self._plan_common(ph, st)
             self._plan_precision(ph, st, prec)
         elif st.kind == 'svdmove':
@@ -545,6 +650,14 @@ class Solve:
                    [(self.mps[p].data_ptr(), L.F64, 1.0, -1)])
             ph['copy'] = mx.arrays()
             self._plan_local(ph, p, prec)
+        elif st.kind == 'resume2':   # the same for a pair: theta holds the Ritz vector
+            p = st.site
+            mx = R.MixList()
+            mx.add(self.init.data_ptr(), L.F64, 1, 1,
+                   self.dims[p] * self.d * self.d * self.dims[p + 2],
+                   [(self.th.data_ptr(), L.F64, 1.0, -1)])
+            ph['copy'] = mx.arrays()
+            self._plan_local(ph, p, prec, pair=True)
         ph['dims_after'] = list(self.dims)
         ph['uid'] = next(_PLAN_UID)
         return ph
@@ -719,6 +832,100 @@ class Solve:
                 ph['copy'] = mx.arrays()
             self.dims[q] = k
 
+    def _plan_theta(self, ph, p):
+        """theta = A_p A_{p+1} (row-major Dl d x d Dr) into the Lanczos start
+        vector of the pair (p, p+1)."""
+        d, dims = self.d, self.dims
+        Dl, D, Dr = dims[p], dims[p + 1], dims[p + 2]
+        g = R.GemmList()
+        g.add(self.init.data_ptr(), Dl * d, d * Dr, d * Dr,
+              [(self.mps[p].data_ptr(), D, 0, self.mps[p + 1].data_ptr(), d * Dr, 0, D, 1.0)])
+        ph['absorb'] = g.arrays()
+
+    def _plan_split(self, ph, p, dirn):
+        """Split the pair tensor theta (``th``, M = row-major m x n, m = Dl d,
+        n = d Dr) into A_p (m x k) A_{p+1} (k x n) with k = min(chi, m, n):
+        dir > 0 leaves A_p left-orthonormal and the centre on p+1, dir < 0
+        leaves A_{p+1} right-orthonormal and the centre on p (the truncated
+        two-site update of ``MPSClass.py:894-922``, on a finite chain).
+
+        Orders N = max(m, n) >= 8: the block Jacobi SVD of X = [M | 0] or
+        [M ; 0] (dir > 0) or the same padding of M^T (dir < 0), N x N, gives
+        X = u^T v with the singular values sorted, so
+        - dir > 0: A_p = (u[:k, :m])^T, A_{p+1} = v[:k, :n];
+        - dir < 0: A_p = (v[:k, :m])^T, A_{p+1} = u[:k, :n].
+        A zero padding row never rotates (its products with every row are
+        0), so the kept rows of u stay inside the first m (n) coordinates
+        even when M is rank-deficient.  The copies into X and the transposed
+        factors are GEMMs with the identity.  Smaller pairs are never
+        truncated (k = min(m, n)): an identity factor when M already has the
+        kept shape, else Householder QR of M or M^T."""
+        d, dims, chi = self.d, self.dims, self.chi
+        Dl, Dr = dims[p], dims[p + 2]
+        m, n = Dl * d, d * Dr
+        k = min(chi, m, n)
+        N = max(m, n)
+        th = self.th.data_ptr()
+        A, B = self.mps[p].data_ptr(), self.mps[p + 1].data_ptr()
+        dsq = self.dsq_max
+        I, LDI = _p(self.cholb, 5 * dsq), self.dmax_max
+        if N > L.SVDJ_NMAX:
+            raise ValueError('two-site split of order %d exceeds the SVD limit %d (chi too large)'
+                             % (N, L.SVDJ_NMAX))
+        self.dims[p + 1] = k
+        if N < 8:
+            if k < min(m, n):
+                raise ValueError('two-site truncation of a %d x %d pair to %d needs chi >= 4' % (m, n, k))
+            stages = []
+            g, mx = R.GemmList(), R.MixList()
+            if dirn > 0 and m <= n:      # A_p = 1, A_{p+1} = M
+                g.add(A, m, m, m, [(I, LDI, 0, I, LDI, 0, m, 1.0)])
+                mx.add(B, L.F64, 1, 1, m * n, [(th, L.F64, 1.0, -1)])
+            elif dirn < 0 and n <= m:    # A_{p+1} = 1, A_p = M
+                g.add(B, n, n, n, [(I, LDI, 0, I, LDI, 0, n, 1.0)])
+                mx.add(A, L.F64, 1, 1, m * n, [(th, L.F64, 1.0, -1)])
+            elif dirn > 0:               # M = Q R: A_p = Q (m x n), A_{p+1} = R
+                ph['qr'] = np.array([(th, A, B, self.qrw.data_ptr(), m, n, n, 0, 0, 0)],
+                                    dtype=L.QR_JOB)
+            else:                        # M^T = Q R: A_{p+1} = Q^T (m x n), A_p = R^T
+                ph['qr'] = np.array([(th, B, self.Rq.data_ptr(), self.qrw.data_ptr(), n, m, n, 1,
+                                      1, 0)], dtype=L.QR_JOB)
+                g.add(A, m, m, m, [(I, LDI, 0, self.Rq.data_ptr(), m, 1, m, 1.0)])
+            if len(g.arrays()[0]):
+                stages.append(('g', g.arrays()))
+            if len(mx.arrays()[0]):
+                stages.append(('m', mx.arrays()))
+            if stages:
+                ph['cq'] = stages
+            return
+        X = th
+        if not (dirn > 0 and m == n):
+            X = _p(self.qrw, 2 * self.psi_max)
+            stages = []
+            if m != n:
+                zero = R.MixList()
+                zero.add(X, L.F64, 1, 1, N * N, [])
+                stages.append(('m', zero.arrays()))
+            g = R.GemmList()
+            if dirn > 0:   # X[:m, :n] = M
+                g.add(X, m, n, N, [(I, LDI, 0, th, n, 0, m, 1.0)])
+            else:          # X[:n, :m] = M^T
+                g.add(X, n, m, N, [(I, LDI, 0, th, n, 1, n, 1.0)])
+            stages.append(('g', g.arrays()))
+            ph['cq'] = stages
+        U, V = _p(self.cholb, 3 * dsq), _p(self.cholb, 4 * dsq)
+        bw, nblk = L.svdj_blocking(N)
+        ph['svdj'] = np.array([(X, _p(self.cholb, 0), U, V, self.Rq.data_ptr(), N, N, bw, nblk,
+                                0, 0)], dtype=L.SVDJ_JOB)
+        g = R.GemmList()
+        if dirn > 0:
+            g.add(A, m, k, k, [(I, LDI, 0, U, N, 1, m, 1.0)])
+            g.add(B, k, n, n, [(V, N, 0, I, LDI, 0, n, 1.0)])
+        else:
+            g.add(A, m, k, k, [(I, LDI, 0, V, N, 1, m, 1.0)])
+            g.add(B, k, n, n, [(U, N, 0, I, LDI, 0, n, 1.0)])
+        ph['svd_post_g'] = g.arrays()
+
     def _plan_env(self, ph, q, dirn):
         """Block environment through the (new) isometry at site q."""
         T, d, dims = self.T, self.d, self.dims
@@ -832,11 +1039,17 @@ class Solve:
                 mx.add(self.R_lp(b, m, ti), ti, D, 1, D, [(self.Rr(b, m), L.F64, 1.0, -1)])
         return mx.arrays()
 
-    def _plan_local(self, ph, p, prec):
+    def _plan_local(self, ph, p, prec, pair=False):
+        """Matvec records and the Lanczos entry of the local eigensolve at
+        site p, or (``pair``) at the merged sites p, p+1: physical dimension
+        d^2, right block at bond p+2, merged operators, result into theta."""
         ti, tc, mode1, mode2, floor_rel = PREC[prec]
-        T, d, dims = self.T, self.d, self.dims
-        site = T.sites[p]
-        Dl, Dr = dims[p], dims[p + 1]
+        T, dims = self.T, self.dims
+        if pair:
+            site, d, rb, opi, out = T.pairs[p], self.d * self.d, p + 2, self.op_index2, self.th
+        else:
+            site, d, rb, opi, out = T.sites[p], self.d, p + 1, self.op_index, self.mps[p]
+        Dl, Dr = dims[p], dims[rb]
         n_el = Dl * d * Dr
         psi, dsq = self.psi_max, self.dsq_max
         isz = ITEMSIZE[ti]
@@ -850,11 +1063,11 @@ class Solve:
         for g, (_, grp) in enumerate(site.lm):
             lc.add(LCp(g), ti, Dl, 1, Dl, [(self.L(p, 1 + l), L.F64, c, -1) for l, c in grp])
         for g, (_, grp) in enumerate(site.mr):
-            lc.add(LCp(nlm + g), ti, Dr, 1, Dr, [(self.Rr(p + 1, 1 + r), L.F64, c, -1) for r, c in grp])
+            lc.add(LCp(nlm + g), ti, Dr, 1, Dr, [(self.Rr(rb, 1 + r), L.F64, c, -1) for r, c in grp])
         for u, (raw, grp) in enumerate(site.strad):
             if site.strad_left:
                 lc.add(LCp(nlm + nmr + u), ti, Dr, 1, Dr,
-                       [(self.Rr(p + 1, 1 + r), L.F64, c, -1) for r, c in grp])
+                       [(self.Rr(rb, 1 + r), L.F64, c, -1) for r, c in grp])
             else:
                 lc.add(LCp(nlm + nmr + u), ti, Dl, 1, Dl,
                        [(self.L(p, 1 + l), L.F64, c, -1) for l, c in grp])
@@ -880,19 +1093,19 @@ class Solve:
         if tH is not None:
             terms.append((Tp(tH), ti, 1.0, -1))
         for g, (opb, _) in enumerate(site.lm):
-            terms.append((Tp(tG[g]), ti, 1.0, self.op_index(opb)))
+            terms.append((Tp(tG[g]), ti, 1.0, opi(opb)))
         for (kop, c) in site.field:
-            terms.append((Xp, ti, c, self.op_index(kop)))
+            terms.append((Xp, ti, c, opi(kop)))
         mx.add(base, tc, Dl, d, Dr, terms)
         for g, (opa, _) in enumerate(site.mr):
-            mx.add(Up(g), ti, Dl, d, Dr, [(Xp, ti, 1.0, self.op_index(opa))])
+            mx.add(Up(g), ti, Dl, d, Dr, [(Xp, ti, 1.0, opi(opa))])
         segs = []
-        if p < self.n - 1:
-            segs.append((Xp, Dr, 0, self.R_lp(p + 1, 0, ti), Dr, 1, Dr, 1.0))
+        if rb < self.n:
+            segs.append((Xp, Dr, 0, self.R_lp(rb, 0, ti), Dr, 1, Dr, 1.0))
         for g in range(nmr):
             segs.append((Up(g), Dr, 0, LCp(nlm + g), Dr, 1, Dr, 1.0))
         for u, (raw, _) in enumerate(site.strad):
-            B = LCp(nlm + nmr + u) if site.strad_left else self.R_lp(p + 1, 1 + raw, ti)
+            B = LCp(nlm + nmr + u) if site.strad_left else self.R_lp(rb, 1 + raw, ti)
             segs.append((Tp(tS[u]), Dr, 0, B, Dr, 1, Dr, 1.0))
         g2.add(self.w.data_ptr(), Dl * d, Dr, Dr, segs, c0=base, ldc0=Dr, beta=1.0)
         ph['mv_lc'] = lc.arrays()
@@ -901,18 +1114,20 @@ class Solve:
         ph['mv_g2'] = g2.arrays()
         tol = self.eigs_tol if self.eigs_tol > 0 else 2.220446049250313e-16
         ph['lz'] = np.array([(self.V.data_ptr(), self.w.data_ptr(), Xp, self.init.data_ptr(),
-                              self.mps[p].data_ptr(), n_el, psi, min(self.engine.ncv, n_el), 0,
+                              out.data_ptr(), n_el, psi, min(self.engine.ncv, n_el), 0,
                               tol / self.tau, self.tau, floor_rel, ti, tc)],
                             dtype=L.LANCZOS_SOLVE)
         ph['prec'] = prec
-        ph['mv_flops'] = self._matvec_flops(p)
+        ph['mv_flops'] = self._matvec_flops(p, pair)
 
-    def _matvec_flops(self, p):
-        """Algorithmic flops of one matvec at site p (GEMM terms executed)."""
-        s = self.T.sites[p]
-        Dl, Dr, d = self.dims[p], self.dims[p + 1], self.d
+    def _matvec_flops(self, p, pair=False):
+        """Algorithmic flops of one matvec at site p, or at the merged pair
+        p, p+1 (GEMM terms executed)."""
+        rb = p + 2 if pair else p + 1
+        s = self.T.pairs[p] if pair else self.T.sites[p]
+        Dl, Dr, d = self.dims[p], self.dims[rb], self.d * self.d if pair else self.d
         n1 = (p > 0) + len(s.lm) + len(s.strad)
-        n2 = (p < self.n - 1) + len(s.mr) + len(s.strad)
+        n2 = (rb < self.n) + len(s.mr) + len(s.strad)
         return 2.0 * d * Dl * Dr * (Dl * n1 + Dr * n2)
 
     def _plan_svd(self, ph, q, dirn):
@@ -1146,9 +1361,13 @@ class Engine:
         self.up = Uploader(self.device)
         self._stager = _Stager()
         self.ops_capacity = ops_capacity
-        self.ops = torch.zeros(ops_capacity * 16, dtype=torch.float64, device=self.device)
+        # operator table: slot s holds site operator k at element (s + k) d^2;
+        # merged two-site operators (d^4 elements, d <= 2) sit in a second
+        # region, operator j of slot s at element (s + ops_off2 + j) d^4
+        self.ops = torch.zeros(ops_capacity * 20, dtype=torch.float64, device=self.device)
         self.ops_free = [(0, ops_capacity)]
         self.ops_d2 = None
+        self.ops_off2 = None
         self.solves = []
         self.lz_states = [None, None]   # Lanczos state buffers of the (at most two) calls
         self.lanczos_streams = int(lanczos_streams)
@@ -1236,18 +1455,30 @@ class Engine:
         return spec
 
     # ------------------------------------------------------------ op table
-    def alloc_ops(self, ops):
+    def alloc_ops(self, ops, ops2=None):
+        """A slot of the operator table for a solve's site operators ``ops``
+        (n, d, d) and, for two-site sweeps, its merged operators ``ops2``
+        (n2, d^2, d^2); the slot spans max(n, n2) entries, so the merged
+        regions of different slots do not overlap."""
         d2 = ops.shape[1] * ops.shape[2]
         if self.ops_d2 is None:
             self.ops_d2 = d2
+            self.ops_off2 = -(-self.ops_capacity * d2 // (d2 * d2))
         if d2 != self.ops_d2:
             raise ValueError('all solves of an engine must share the physical dimension')
-        cnt = ops.shape[0]
+        cnt = ops.shape[0] if ops2 is None else max(ops.shape[0], ops2.shape[0])
+        if ops2 is not None and (self.ops_off2 + self.ops_capacity) * d2 * d2 > self.ops.numel():
+            raise ValueError('merged two-site operators need d <= 2')
         for t, (s, c) in enumerate(self.ops_free):
             if c >= cnt:
                 self.ops_free[t] = (s + cnt, c - cnt)
                 flat = torch.from_numpy(np.ascontiguousarray(ops).reshape(-1)).to(self.device)
-                self.ops[s * d2:(s + cnt) * d2].copy_(flat)
+                self.ops[s * d2:(s + ops.shape[0]) * d2].copy_(flat)
+                if ops2 is not None:
+                    d4 = d2 * d2
+                    o = (s + self.ops_off2) * d4
+                    flat = torch.from_numpy(np.ascontiguousarray(ops2).reshape(-1)).to(self.device)
+                    self.ops[o:o + ops2.shape[0] * d4].copy_(flat)
                 self._ops_owner = getattr(self, '_ops_owner', {})
                 self._ops_owner[s] = cnt
                 return s
@@ -1320,7 +1551,7 @@ class Engine:
 
         def draw(k):
             (para, seed), (T, _, _, lay) = items[k], geoms[k]
-            return make_stage(lay, random_mps_host(T.n, T.d, int(para['chi']), seed),
+            return make_stage(lay, random_mps_host(T.n, T.d, initial_chi(para), seed),
                               pin=self.device.type != 'cpu')
         workers = max(1, min(threads, len(items)))
         if workers > 1:
@@ -1622,11 +1853,14 @@ class Engine:
             s.stats['steps'] += 1
             if s.resume:
                 s.resume = False
-                s.queue.insert(0, Step('resume', st.site, st.dir, st.sweep))
-            if st.kind in ('solve', 'resume'):
+                kind = 'resume2' if st.kind in ('solve2', 'resume2') else 'resume'
+                s.queue.insert(0, Step(kind, st.site, st.dir, st.sweep))
+            if st.kind in ('solve', 'resume', 'solve2', 'resume2'):
                 s.center = st.site
             elif st.kind in ('move', 'svdmove'):
                 s.center = st.site + st.dir
+            elif st.kind == 'split2':
+                s.center = st.site + (st.dir > 0)
             if 'lm_bond' in ph:   # spectrum written to lmv on the device; read lazily
                 s._lm_pending[ph['lm_bond']] = ph['svd_n']
         self.counters['steps'] += 1

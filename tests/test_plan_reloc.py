@@ -32,9 +32,12 @@ class CpuEngine:
     def arena_release(self, arena):
         pass
 
-    def alloc_ops(self, ops):
+    ops_off2 = 1024
+
+    def alloc_ops(self, ops, ops2=None):
         s = self._next_slot
-        self._next_slot += ops.shape[0] + 3   # distinct slots per solve
+        n = ops.shape[0] if ops2 is None else max(ops.shape[0], ops2.shape[0])
+        self._next_slot += n + 3   # distinct slots per solve
         return s
 
     def free_ops(self, slot):
@@ -69,6 +72,12 @@ def _records(ph):
     # same structure, other couplings and field: the plan is still shared
     ('chain', dict(l=10, chi=16, jz=1.0, hz=0.2), dict(jz=0.7, hz=0.45), 'bf16'),
     ('snakeJ1J2', dict(square_width=3, square_height=4, chi=12), dict(j2=0.3), 'fp64'),
+    # two-site sweeps: truncating SVD splits, small QR splits, growing bonds
+    ('chain', dict(l=10, chi=16, dmrg_sites=2), {}, 'bf16'),
+    ('chain', dict(l=7, chi=8, chi_init=2, dmrg_sites=2, ob_position=3), {}, 'fp64'),
+    ('chain', dict(l=10, chi=16, dmrg_sites=2, jz=1.0, hz=0.2), dict(jz=0.7, hz=0.45), 'fp32'),
+    ('snakeJ1J2', dict(square_width=3, square_height=4, chi=12, dmrg_sites=2), dict(j2=0.3),
+     'fp64'),
 ])
 def test_relocated_plan_equals_own_plan(lattice, kw, kw_b, prec):
     para = _para(lattice, **kw)
