@@ -271,7 +271,7 @@ int tnx_svd_jacobi_f64(const tnx_svd_job* jobs, int njob, int max_sweeps, tnx_st
  * TNX_ENOCONV after max_sweeps sweeps that still rotated.                   */
 typedef struct tnx_svdj_job {
   uint64_t r;          /* R, row-major n x n, leading dimension ld_r (read)   */
-  uint64_t work;       /* scratch: 2 n^2 + n doubles                          */
+  uint64_t work;       /* scratch: 2 n^2 + n + 1 doubles                      */
   uint64_t u, v, s;    /* outputs                                             */
   int32_t n, ld_r;
   int32_t bw;          /* columns per block                                   */

@@ -15,7 +15,11 @@
 // round of the round-robin over block pairs: a workgroup loads its pair into
 // LDS, runs one round-robin sweep of plane rotations over the 2 bw columns
 // (bw disjoint column pairs at a time, 256 / bw lanes each) and writes the
-// columns back.  Rounds repeat until a sweep rotates nothing.
+// columns back.  Rounds repeat until a sweep rotates nothing.  A column
+// whose norm is at most n eps ||R||_F is numerical zero (rounding residue of
+// a rank-deficient R) and takes part in no rotation: rotating pairs of such
+// columns never settles, and leaving them alone keeps R = u^T v exact and u
+// orthogonal.
 #include "tnx_common.h"
 
 namespace {
@@ -46,6 +50,24 @@ __global__ __launch_bounds__(NT) void k_svdj_init(const tnx_svdj_job* __restrict
       B[(int64_t)j * n + i] = R[(int64_t)j * J.ld_r + i];
       W[(int64_t)j * n + i] = (i == j) ? 1.0 : 0.0;
     }
+}
+
+// ||R||_F^2 of each job into work[2 n^2 + n] (one workgroup per job, a fixed
+// summation order, so the zero test below does not depend on the batch).
+__global__ __launch_bounds__(NT) void k_svdj_scale(const tnx_svdj_job* __restrict__ jobs) {
+  const tnx_svdj_job J = jobs[blockIdx.x];
+  const int64_t nn = (int64_t)J.n * J.n;
+  double* B = (double*)J.work;
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < nn; i += NT) {
+    const double b = B[i];
+    s += b * b;
+  }
+  s = tnx_wave_sum(s);
+  __shared__ double red[NT / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) B[2 * nn + J.n] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 __global__ void k_svdj_clear(int32_t* flag) {
@@ -81,6 +103,7 @@ __global__ __launch_bounds__(NT) void k_svdj_round(const tnx_svdj_job* __restric
   const int T = NT / bw;  // lanes per column pair: 8 .. 64, aligned groups of one wavefront
   const int k = threadIdx.x / T, l = threadIdx.x % T;
   const double eps = 2.220446049250313e-16 * (double)(n > 8 ? n : 8);
+  const double zero2 = eps * eps * B[2 * (int64_t)n * n + n];  // column norm^2 of numerical zero
   for (int ir = 0; ir < w2 - 1; ++ir) {
     int u, v;
     pair_of(w2, ir, k, u, v);
@@ -98,7 +121,7 @@ __global__ __launch_bounds__(NT) void k_svdj_round(const tnx_svdj_job* __restric
       c += __shfl_xor(c, o, 64);
       g += __shfl_xor(g, o, 64);
     }
-    if (g != 0.0 && fabs(g) > eps * sqrt(a * c)) {
+    if (g != 0.0 && fabs(g) > eps * sqrt(a * c) && a > zero2 && c > zero2) {
       const double zeta = (c - a) / (2.0 * g);
       const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
       const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
@@ -197,6 +220,7 @@ extern "C" int tnx_svd_block_jacobi(const tnx_svdj_job* jobs, int njob, int npai
   if (g_flag_pin == nullptr)
     TNX_CHECK_HIP(hipHostMalloc((void**)&g_flag_pin, 64, hipHostMallocDefault));
   hipLaunchKernelGGL(k_svdj_init, dim3(max_n < 1024 ? max_n : 1024, njob), dim3(NT), 0, s, jobs);
+  hipLaunchKernelGGL(k_svdj_scale, dim3(njob), dim3(NT), 0, s, jobs);
   TNX_CHECK_LAUNCH();
   bool converged = false;
   for (int sweep = 0; sweep < max_sweeps && !converged; ++sweep) {

@@ -324,7 +324,7 @@ def random_mps_host(n, d, chi, seed):
 
 class Step:
     """One schedule entry of a solve: kind in {'move', 'solve', 'svdmove',
-    'resume'}, site, direction (+1: centre moves right, -1: left).  'resume'
+    'resume', 'solve2', 'split2', 'resume2'}, site, direction (+1: centre moves right, -1: left).  'resume'
     continues the local eigensolve of a 'solve' step from its Ritz vector
     (the MPS site tensor) when one Krylov cycle did not converge."""
     __slots__ = ('kind', 'site', 'dir', 'sweep')
@@ -1178,7 +1178,7 @@ class Solve:
         """A = Q R (CholeskyQR3 of the panel, Q in place of A_q), R = u^T v
         (``tnx_svd_block_jacobi``), then A_q <- Q u^T and the neighbour
         absorbs v.  Scratch in the CholeskyQR buffer below its constants:
-        B, W, column norms at 0 .. 2 dsq + n; u at 3 dsq; v at 4 dsq."""
+        B, W, column norms and the scale at 0 .. 2 dsq + n + 1; u at 3 dsq; v at 4 dsq."""
         d, dims = self.d, self.dims
         if nn <= L.CHOL_NMAX:
             self._plan_cholqr(ph, q, m, nn, right=dirn < 0)
@@ -1400,7 +1400,7 @@ class Engine:
     # ------------------------------------------------------ planning ahead
     def _prec_of(self, s, st):
         """Matvec precision of step ``st`` of solve ``s`` (moves: f64)."""
-        if st.kind not in ('solve', 'resume'):
+        if st.kind not in ('solve', 'resume', 'solve2', 'resume2'):
             return 'fp64'
         return self.precision_schedule(max(st.sweep, 0), s.para.get('sweep_time', 1))
 

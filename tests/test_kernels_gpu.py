@@ -295,12 +295,16 @@ def test_svd_block_jacobi(gpu):
     Rd = np.triu(rng.randn(24, 24))
     Rd[9:] = 0.0           # rank 9
     Rs.append(Rd)
+    # singular values down to 1e-22 of the largest: the tail is rounding
+    # residue, which must not keep the sweeps going (a non-zero return fails)
+    A = rng.randn(256, 256) @ np.diag(np.logspace(0, -22, 256)) @ np.linalg.qr(rng.randn(256, 256))[0]
+    Rs.append(np.linalg.qr(A)[1])
     bufs, recs = [], []
     for R in Rs:
         n = R.shape[0]
         bw, nblk = L.svdj_blocking(n)
         r = _dev(R)
-        work = torch.zeros(2 * n * n + n, dtype=torch.float64, device='cuda')
+        work = torch.zeros(2 * n * n + n + 1, dtype=torch.float64, device='cuda')
         u = torch.zeros(n * n, dtype=torch.float64, device='cuda')
         v = torch.zeros(n * n, dtype=torch.float64, device='cuda')
         s = torch.zeros(n, dtype=torch.float64, device='cuda')
