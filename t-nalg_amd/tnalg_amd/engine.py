@@ -1539,10 +1539,21 @@ class Engine:
         return ent
 
     # ------------------------------------------------------------- solves
-    def add_solve(self, para, seed=0, mps=None, tag=None):
-        s = Solve(self, para, seed=seed, mps=mps, tag=tag)
+    def add_solve(self, para, seed=0, mps=None, tag=None, stage=None):
+        """A solve of ``para``; ``stage``: its initial image from
+        :meth:`prepare` (default: drawn and staged here)."""
+        s = Solve(self, para, seed=seed, mps=mps, tag=tag, stage=stage)
         self.solves.append(s)
         return s
+
+    def prepare(self, para, seed=0):
+        """The host work of ``add_solve(para, seed)`` that needs no device
+        state: term tables and layout, the seeded initial MPS drawn (numpy's
+        generator runs without the GIL) and staged in pinned memory.  Safe on
+        a worker thread; the result goes to ``add_solve(..., stage=...)``."""
+        T, _, _, lay = solve_geometry(para, self.ncv, self.uses_lowp)
+        return make_stage(lay, random_mps_host(T.n, T.d, initial_chi(para), seed),
+                          pin=self.device.type != 'cpu')
 
     def add_solves(self, items, threads=16):
         """Solves of ``items = [(para, seed), ...]``; the seeded initial MPS

@@ -8,15 +8,27 @@ all admitted solves together: every engine step moves each active solve by
 one entry of its own schedule (solves sit at different sites, sweeps, chain
 lengths, bond dimensions and precisions), and each phase of the step is one
 kernel launch over all of them.  A solve that finishes its sweeps is
-measured, retired and its memory freed; the next request takes its place in
-the following step.  Per request the server records arrival, admission and
+retired and its memory freed; the next request takes its place in the
+following step.  Per request the server records arrival, admission and
 completion times; ``summary()`` gives solves/s and p50/p99 latency.
 
+Host work stays off the step loop where it can:
+- a request that has arrived is prepared on a worker thread (term tables,
+  the seeded initial MPS drawn and staged in pinned memory) when the engine
+  offers ``prepare``;
+- the energy of a finished one-site solve is the Ritz value of its last
+  local eigensolve (``energy='ritz'``).  Its centre tensor is that Ritz vector
+  and the MPS is in mixed canonical form, so <psi|H|psi> = x^T Heff x = theta
+  to rounding.  ``energy='measure'`` (and every two-site solve, whose last
+  split truncates after the last eigensolve) measures <H> on the device
+  instead.
+
 The engine is duck-typed (``add_solve``, ``remove_solve``, ``step``,
-``footprint``, ``finalize``) so the admission logic can be exercised without a
-GPU.
+``footprint``, ``finalize``; optionally ``prepare``) so the admission logic
+can be exercised without a GPU.
 """
 import collections
+import concurrent.futures
 import time
 
 import numpy as np
@@ -24,7 +36,7 @@ import numpy as np
 
 class Request:
     __slots__ = ('rid', 'para', 'seed', 'arrival', 'admitted', 'done', 'energy', 'solve', 'error',
-                 'footprint', 'sweeps_left')
+                 'footprint', 'sweeps_left', 'prep')
 
     def __init__(self, rid, para, seed, arrival):
         self.rid, self.para, self.seed, self.arrival = rid, para, seed, arrival
@@ -33,11 +45,19 @@ class Request:
         self.solve = None
         self.error = None
         self.footprint = 0
+        self.prep = None   # future of the engine's host preparation
 
 
 class SolveServer:
-    def __init__(self, engine, hbm_budget, max_active=256, clock=time.perf_counter):
+    def __init__(self, engine, hbm_budget, max_active=256, clock=time.perf_counter,
+                 energy='ritz', prepare_ahead=8):
+        if energy not in ('ritz', 'measure'):
+            raise ValueError("energy must be 'ritz' or 'measure'")
+        self.energy = energy
         self.engine = engine
+        # requests prepared ahead of admission (0: prepare at admission)
+        self.prepare_ahead = int(prepare_ahead) if hasattr(engine, 'prepare') else 0
+        self._prep_pool = None
         self.budget = int(hbm_budget)
         self.max_active = int(max_active)
         self.clock = clock
@@ -69,7 +89,11 @@ class SolveServer:
                 break  # FIFO: wait for memory to free up
             self.waiting.popleft()
             try:
-                r.solve = self.engine.add_solve(r.para, seed=r.seed, tag=r.rid)
+                if r.prep is not None:
+                    stage, r.prep = r.prep.result(), None
+                    r.solve = self.engine.add_solve(r.para, seed=r.seed, tag=r.rid, stage=stage)
+                else:
+                    r.solve = self.engine.add_solve(r.para, seed=r.seed, tag=r.rid)
                 r.solve.schedule_canonicalise()
                 for t in range(int(r.para['sweep_time'])):
                     r.solve.schedule_sweep(t)
@@ -86,14 +110,36 @@ class SolveServer:
         self.peak_bytes = max(self.peak_bytes, self.in_use)
         return admitted
 
+    def _prepare(self, now):
+        """Start the host preparation of the next arrived requests."""
+        if not self.prepare_ahead:
+            return
+        k = 0
+        for r in self.waiting:
+            if k >= self.prepare_ahead or r.arrival > now:
+                break
+            k += 1
+            if r.prep is None:
+                if self._prep_pool is None:
+                    self._prep_pool = concurrent.futures.ThreadPoolExecutor(2)
+                r.prep = self._prep_pool.submit(self.engine.prepare, r.para, r.seed)
+
     def _retire(self, now):
         still = []
         done = [r for r in self.active if not r.solve.queue or r.solve.failed is not None]
         if done:
             ok = [r for r in done if r.solve.failed is None]
-            if ok:
-                energies = self.engine.finalize([r.solve for r in ok])
-                for r, e in zip(ok, energies):
+            meas = []
+            for r in ok:
+                th = getattr(r.solve, 'last_theta', None)
+                if (self.energy == 'ritz' and getattr(r.solve, 'sites', 1) == 1 and th is not None
+                        and th == th):
+                    r.energy = float(th)
+                else:
+                    meas.append(r)
+            if meas:
+                energies = self.engine.finalize([r.solve for r in meas])
+                for r, e in zip(meas, energies):
                     r.energy = e
             t = self.clock()
             for r in done:
@@ -121,6 +167,7 @@ class SolveServer:
                 progress(dict(self.summary(), active=len(self.active), waiting=len(self.waiting)))
             now = self.clock()
             self._admit(now)
+            self._prepare(now)
             if not self.active:
                 if self.waiting:
                     # idle until the next arrival (a virtual clock is advanced)

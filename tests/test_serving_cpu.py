@@ -140,3 +140,70 @@ def test_stream_generator_ranges():
     for _, p, _ in reqs:
         jz = np.ravel(p['coeff2'])[2]
         assert 0.5 <= jz <= 1.5 and 0.0 <= p['hz'] <= 0.5
+
+
+class RitzSolve(FakeSolve):
+    def __init__(self, para, fail_at=None):
+        super().__init__(para, fail_at)
+        self.last_theta = -0.5 * self.n
+        self.sites = para.get('dmrg_sites', 1)
+
+
+class PreparingEngine(FakeEngine):
+    """Offers ``prepare``; ``add_solve`` records which stage it was given."""
+
+    def __init__(self, clock):
+        super().__init__(clock)
+        self.finalized = []
+        self.stages = {}
+
+    def prepare(self, para, seed=0):
+        return ('stage', para['l'], seed)
+
+    def add_solve(self, para, seed=0, tag=None, stage=None):
+        s = super().add_solve(para, seed=seed, tag=tag)
+        s.__class__ = RitzSolve
+        s.last_theta = -0.5 * s.n
+        s.sites = para.get('dmrg_sites', 1)
+        self.stages[tag] = stage
+        return s
+
+    def finalize(self, solves):
+        self.finalized += list(solves)
+        return super().finalize(solves)
+
+
+def test_ritz_energies_skip_the_measurement():
+    """energy='ritz': one-site solves report their last Ritz value without a
+    device measurement; two-site solves (truncated after the last solve) and
+    energy='measure' measure."""
+    for mode in ('ritz', 'measure'):
+        clk = Clock()
+        eng = PreparingEngine(clk)
+        srv = SolveServer(eng, hbm_budget=1 << 40, max_active=4, clock=clk, energy=mode)
+        srv.submit(0, _para(10, 8), seed=1, arrival=0.0)
+        srv.submit(1, _para(12, 8, dmrg_sites=2), seed=2, arrival=0.0)
+        srv.run()
+        e = {r.rid: r.energy for r in srv.finished}
+        if mode == 'ritz':
+            assert e[0] == -5.0 and e[1] == -0.44 * 12
+            assert [s.n for s in eng.finalized] == [12]
+        else:
+            assert e[0] == -0.44 * 10 and e[1] == -0.44 * 12
+            assert sorted(s.n for s in eng.finalized) == [10, 12]
+
+
+def test_prepared_stages_reach_admission():
+    """Arrived requests are prepared ahead (engine.prepare on a worker) and
+    admitted with their own stage; a request that arrives while the server
+    idles is not prepared early, it is admitted at once and staged inline."""
+    clk = Clock()
+    eng = PreparingEngine(clk)
+    srv = SolveServer(eng, hbm_budget=1 << 40, max_active=1, clock=clk)
+    for k in range(3):
+        srv.submit(k, _para(8 + k, 8), seed=10 + k, arrival=0.0)
+    srv.submit(3, _para(9, 8), seed=13, arrival=5.0)
+    srv.run()
+    assert len(srv.finished) == 4 and all(r.error is None for r in srv.finished)
+    assert eng.stages[1] == ('stage', 9, 11) and eng.stages[2] == ('stage', 10, 12)
+    assert eng.stages[3] is None

@@ -75,3 +75,28 @@ def test_two_lanczos_streams_same_bits(gpu):
         out[streams] = eng.finalize(solves)
     assert len(out[2]) == len(reqs)
     assert out[1] == out[2]
+
+
+def test_ritz_energy_equals_measured(gpu):
+    """The served energy of a one-site solve (the Ritz value of its last
+    local eigensolve) equals the measured <H> of its final MPS; the same
+    stream served with prepared stages and without gives the same bits."""
+    from tnalg_amd.dmrg import _schedule_from
+    from tnalg_amd.engine import Engine
+    from tnalg_amd.serving import SolveServer
+    reqs = _requests()
+    got = {}
+    for mode, ahead in (('ritz', 8), ('ritz', 0), ('measure', 8)):
+        print('stream energy', mode, 'prepare_ahead', ahead, flush=True)
+        eng = Engine(precision_schedule=_schedule_from('mixed'), lowp=True, ncv=NCV)
+        srv = SolveServer(eng, hbm_budget=8 << 30, max_active=4, energy=mode,
+                          prepare_ahead=ahead)
+        for k, (t, para, seed) in enumerate(reqs):
+            srv.submit(k, para, seed=seed, arrival=0.0)
+        summ = srv.run()
+        assert summ['completed'] == len(reqs)
+        got[(mode, ahead)] = {r.rid: r.energy for r in srv.finished}
+    for k in range(len(reqs)):
+        e_r, e_m = got[('ritz', 8)][k], got[('measure', 8)][k]
+        assert got[('ritz', 0)][k] == e_r
+        assert abs(e_r - e_m) < 1e-10 * abs(e_m), (k, e_r, e_m)
