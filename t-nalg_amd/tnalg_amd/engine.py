@@ -504,8 +504,10 @@ class Solve:
         self._sched_center = self.ob
 
     def schedule_sweep(self, sweep):
-        """One sweep of ``DMRG_anyH.py:47-64``: ob+1..N-1, N-2..0, 1..ob-1."""
-        if self.sites == 2:
+        """One sweep of ``DMRG_anyH.py:47-64``: ob+1..N-1, N-2..0, 1..ob-1.
+        With two-site sweeps, sweeps 0 .. two_site_sweeps-1 (default all)
+        sweep pairs and the later ones are one-site."""
+        if self.sites == 2 and sweep < int(self.para.get('two_site_sweeps', 1 << 30)):
             return self._schedule_sweep2(sweep)
         n, ob = self.n, self.ob
         sites = [(s, +1) for s in range(ob + 1, n)] + [(s, -1) for s in range(n - 2, -1, -1)] + \

@@ -80,9 +80,9 @@ def test_two_site_grows_bonds_from_chi_init(gpu):
 
 
 def test_two_site_truncation_keeps_canonical_form(gpu):
-    """N=24 Heisenberg chain at chi=12: every bond is truncated to chi, the
-    MPS stays canonical, and the energy lies above the exact one and within
-    the one-site result's accuracy range."""
+    """N=24 Heisenberg chain at chi=12: the middle bonds are truncated to
+    chi, the MPS stays canonical and normalised, and the energy agrees with
+    the one-site result at the same chi within the truncation error."""
     from tnalg_amd import dmrg_finite_size
     c = model.chain_couplings(24, 1.0, 1.0)
     para = _para(c, 12, 6)
@@ -138,3 +138,18 @@ def test_two_site_mixed_precision_matches_fp64(gpu):
     obm, _, _, _ = dmrg_finite_size(dict(para), seed=4, precision='mixed')
     e64, em = float(ob64['e_per_site'][0]), float(obm['e_per_site'][0])
     assert abs(em - e64) < 1e-9 * abs(e64), (em, e64)
+
+
+def test_two_site_growth_then_one_site(gpu):
+    """Three two-site sweeps grow the bonds from chi_init = 2 to their
+    maxima (each pass about doubles them), the one-site sweeps after them
+    finish at the exact energy (N=12 XXZ chain, chi=64)."""
+    from tnalg_amd import dmrg_finite_size
+    c = model.chain_couplings(12, 1.0, 0.9, hz=0.05)
+    para = _para(c, 64, 6, chi_init=2, two_site_sweeps=3)
+    print('case: two-site x3 then one-site, N=12 chi_init=2 -> 64', flush=True)
+    ob, A, info, _ = dmrg_finite_size(para, seed=6)
+    e0 = _exact(c, 12)
+    e = float(ob['e_per_site'][0]) * 12
+    assert abs(e - e0) < 1e-10 * abs(e0), (e, e0)
+    assert A.virtual_dim == [min(2 ** b, 2 ** (12 - b), 64) for b in range(13)]
