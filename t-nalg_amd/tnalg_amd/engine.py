@@ -581,14 +581,14 @@ class Solve:
         ckey = (self.sig, st.kind + '-env', st.site, st.dir, tuple(self.dims))
         com = cache.get(ckey)
         if com is None:
-            com = {}
+            com = R.Plan()
             self._plan_common(com, st)
             com['dims_after'] = list(self.dims)
             R.pack_relative(com, lo, hi, self.ops_slot)
             cache[ckey] = com
         else:
             self.dims = list(com['dims_after'])
-        loc = {}
+        loc = R.Plan()
         self._plan_precision(loc, st, prec)
         R.pack_relative(loc, lo, hi, self.ops_slot)
         meta = {k: loc[k] for k in ('prec', 'mv_flops')}
@@ -636,7 +636,7 @@ class Solve:
             self._plan_local(ph, st.site, prec, pair=st.kind == 'solve2')
 
     def _build_plan(self, st, prec):
-        ph = {}
+        ph = R.Plan()
         if st.kind in ('move', 'solve', 'solve2', 'split2'):
             self._plan_common(ph, st)
             self._plan_precision(ph, st, prec)

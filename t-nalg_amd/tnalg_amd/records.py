@@ -281,6 +281,27 @@ def _inflate(tree, buf, specs):
     return tree
 
 
+_LAZY = object()
+
+
+class Plan(dict):
+    """A step plan.  Once packed, its record phases are views of the packed
+    relative buffer made on first read (the steps read phases through
+    ``Solve.part``, at the solve's addresses, so most are never read here)."""
+    __slots__ = ()
+
+    def __getitem__(self, k):
+        v = dict.__getitem__(self, k)
+        if v is _LAZY:
+            pk = dict.__getitem__(self, 'packed')
+            v = pk.views(pk.buf, k)
+            dict.__setitem__(self, k, v)
+        return v
+
+    def get(self, k, default=None):
+        return self[k] if k in self else default
+
+
 class Packed:
     """Relative plan records in one buffer.  ``ptr``: uint64 word indices of
     the non-null addresses; ``cidx``/``ck``: float64 word indices of boxed
@@ -341,9 +362,9 @@ def merge_packed(subs, meta):
             if k in pk.trees:
                 raise ValueError('sub-plans share the phase %r' % k)
             pk.trees[k] = _shift_tree(t, d)
-    ph = dict(meta)
+    ph = Plan(meta)
     for k in pk.trees:
-        ph[k] = pk.views(pk.buf, k)
+        dict.__setitem__(ph, k, _LAZY)
     ph['packed'] = pk
     return ph
 
@@ -360,7 +381,7 @@ def pack_relative(ph, lo, hi, slot):
     if arrs:
         sizes = np.fromiter((a.nbytes for a in arrs), np.int64, len(arrs))
         offs = np.cumsum(sizes) - sizes
-        buf = np.frombuffer(bytearray(b''.join([a.tobytes() for a in arrs])), np.uint8)
+        buf = np.concatenate([np.ascontiguousarray(a).view(np.uint8) for a in arrs])
         fw = np.array([_field_words(a.dtype) for a in arrs], dtype=np.int64).reshape(-1, 4)
         n = np.fromiter((len(a) for a in arrs), np.int64, len(arrs))
     else:
@@ -404,9 +425,12 @@ def pack_relative(ph, lo, hi, slot):
     pk.op = oidx[o >= 0]
     iv[pk.op] -= np.int32(slot)
     pk.buf = buf
-    for k in keys:
-        ph[k] = pk.views(buf, k)
     ph['packed'] = pk
+    for k in keys:
+        if type(ph) is Plan:
+            dict.__setitem__(ph, k, _LAZY)
+        else:
+            ph[k] = pk.views(buf, k)
     return pk
 
 
