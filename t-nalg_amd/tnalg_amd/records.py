@@ -26,6 +26,7 @@ class GemmList:
     def __init__(self):
         self.probs = []
         self.segs = []
+        self.tiles = 0   # output tiles of the problems so far (tile_begin of the next)
 
     def add(self, c, M, N, ldc, segs, c0=0, ldc0=0, beta=0.0, active=-1):
         """``segs``: iterable of ``(a, lda, transA, b, ldb, transB, K, alpha)``."""
@@ -35,15 +36,12 @@ class GemmList:
         start = len(self.segs)
         for (a, lda, ta, b, ldb, tb, K, alpha) in segs:
             self.segs.append((a, b, lda, ldb, ta, tb, K, 0, alpha))
-        self.probs.append((c, c0, M, N, ldc, ldc0, start, len(segs), 0, active, beta))
+        self.probs.append((c, c0, M, N, ldc, ldc0, start, len(segs), self.tiles, active, beta))
+        self.tiles += (-(-M // L.GEMM_TILE)) * (-(-N // L.GEMM_TILE))
 
     def arrays(self):
         p = np.array(self.probs, dtype=GEMM_PROBLEM) if self.probs else EMPTY_PROB.copy()
         s = np.array(self.segs, dtype=GEMM_SEG) if self.segs else EMPTY_SEG.copy()
-        if len(p):
-            tiles = ((p['M'] + L.GEMM_TILE - 1) // L.GEMM_TILE) * \
-                ((p['N'] + L.GEMM_TILE - 1) // L.GEMM_TILE)
-            p['tile_begin'] = np.concatenate([[0], np.cumsum(tiles)[:-1]])
         return p, s
 
 
@@ -53,6 +51,7 @@ class MixList:
     def __init__(self):
         self.jobs = []
         self.terms = []
+        self.blocks = 0   # workgroups of the jobs so far (block_begin of the next)
 
     def add(self, dst, dst_dtype, Dl, d, Dr, terms, accumulate=0, active=-1):
         """``terms``: iterable of ``(src, src_dtype, coef, op)`` (op = -1: identity)."""
@@ -64,14 +63,12 @@ class MixList:
         for (src, sdt, coef, op) in terms:
             self.terms.append((src, coef_field(coef), op, sdt))
         self.jobs.append((dst, Dl, d, Dr, dst_dtype, start, len(self.terms) - start, accumulate,
-                          active, 0, 0))
+                          active, self.blocks, 0))
+        self.blocks += -(-(Dl * Dr) // L.MIX_PAIRS)
 
     def arrays(self):
         j = np.array(self.jobs, dtype=MIX_JOB) if self.jobs else EMPTY_JOB.copy()
         t = np.array(self.terms, dtype=MIX_TERM) if self.terms else EMPTY_TERM.copy()
-        if len(j):
-            blocks = (j['Dl'].astype(np.int64) * j['Dr'] + L.MIX_PAIRS - 1) // L.MIX_PAIRS
-            j['block_begin'] = np.concatenate([[0], np.cumsum(blocks)[:-1]])
         return j, t
 
 
