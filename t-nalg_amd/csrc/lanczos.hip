@@ -479,25 +479,29 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
   LzState& st = L.st[s];
   __shared__ double a_s[NCV_MAX], b_s[NCV_MAX], y_s[NCV_MAX], th_s;
   __shared__ double work[5 * NCV_MAX];
+  // alpha[0..j-1], beta[0..j-1] were written by earlier launches: every lane
+  // loads its share (one round trip instead of a serial loop of j loads)
+  for (int k = threadIdx.x; k < j; k += NT) {
+    a_s[k] = st.alpha[k];
+    b_s[k] = st.beta[k];
+  }
+  for (int k = threadIdx.x; k <= j; k += NT) st.h2[k] = h[k];
   if (threadIdx.x == 0) {
-    const double beta = beta_j;
-    for (int k = 0; k <= j; ++k) st.h2[k] = h[k];
-    st.alpha[j] = st.h1[j] + h[j];
-    st.beta[j] = beta;
+    const double alpha = st.h1[j] + h[j];
+    st.alpha[j] = alpha;
+    st.beta[j] = beta_j;
     st.iters += 1;
-    for (int k = 0; k <= j; ++k) {
-      a_s[k] = st.alpha[k];
-      b_s[k] = st.beta[k];
-    }
+    a_s[j] = alpha;
+    b_s[j] = beta_j;
   }
   __syncthreads();
   tridiag_lowest(a_s, b_s, j + 1, y_s, &th_s, work);
   __syncthreads();
+  for (int k = threadIdx.x; k <= j; k += NT) st.y[k] = y_s[k];
   if (threadIdx.x == 0) {
     const double beta = b_s[j];
     const double th = th_s;
     const double res = fabs(beta * y_s[j]);
-    for (int k = 0; k <= j; ++k) st.y[k] = y_s[k];
     st.theta = th;
     st.resid = res;
     st.jlast = j;
