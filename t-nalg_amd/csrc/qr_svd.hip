@@ -275,70 +275,125 @@ __global__ __launch_bounds__(NT) void svd_kernel(const tnx_svd_job* __restrict__
 // Lane (tr, tc) = (tid / 16, tid % 16) holds the 8 x 8 register tile of rows
 // {tr + 16a} and columns {tc + 16b}; the cyclic split keeps the shrinking
 // trailing matrix spread over every lane.  Right-looking Cholesky of
-// S = G + shift I = R^T R: step k broadcasts row k through LDS (rows
-// double-buffered, so one barrier per step) and each lane applies the rank-1
-// update to its tile.  R is then parked in LDS and R^{-1} is formed from the
-// bottom row up: X[k,:] /= R[k,k], then X[i,:] -= R[i,k] X[k,:] for i < k.
+// S = G + shift I = R^T R: steps k and k+1 broadcast rows k and k+1 through
+// LDS (row pairs double-buffered, so one barrier per two steps) and each lane
+// applies the two rank-1 updates to its tile.  R is then parked in LDS and
+// R^{-1} is formed from the bottom row up, rows k and k-1 per barrier:
+// X[k,:] /= R[k,k], X[i,:] -= R[i,k] X[k,:] for i < k.  Every lane forms the
+// second row of a pair from the broadcast rows with the same operations, in
+// the same order, as the step that would have followed a barrier, so the
+// results are those of one step per barrier.
 constexpr int CHOL_NMAX = 128;
 constexpr int CHOL_LD = CHOL_NMAX + 1;  // LDS row stride of R: column reads hit distinct banks
-constexpr size_t CHOL_LDS = sizeof(double) * (CHOL_NMAX * CHOL_LD + 2 * CHOL_NMAX);
+constexpr size_t CHOL_LDS = sizeof(double) * (CHOL_NMAX * CHOL_LD + 4 * CHOL_NMAX);
 
-// Cholesky steps k in [16 KA, 16 KA + 16) (k < n).  With the tile-row block
-// KA of column k a compile-time constant, every register index is static,
-// and the blocks a < KA / b < KA (rows and columns already factored, whose
-// update factors are zero) drop out of the rank-1 update at compile time.
+// Cholesky steps k in [16 KA, 16 KA + 16) (k < n), in pairs (k, k+1) that
+// stay inside the block (16 KA is even).  With the tile-row block KA of
+// column k a compile-time constant, every register index is static, and the
+// blocks a < KA / b < KA (rows and columns already factored, whose update
+// factors are zero) drop out of the rank-1 updates at compile time.  `it`
+// counts barriers (buffer parity).
 template <int KA>
 __device__ __forceinline__ void chol_steps(double (&s)[8][8], double* rowbuf, int n, int tr, int tc,
-                                           int& bad) {
+                                           int& bad, int& it) {
   const int kend = min(16 * KA + 16, n);
-  for (int k = 16 * KA; k < kend; ++k) {
-    double* rb = rowbuf + (k & 1) * CHOL_NMAX;
+  for (int k = 16 * KA; k < kend; k += 2) {
+    double* r0 = rowbuf + (it & 1) * 2 * CHOL_NMAX;
+    double* r1 = r0 + CHOL_NMAX;
+    ++it;
+    const bool two = k + 1 < kend;
     if (tr == (k & 15)) {
 #pragma unroll
-      for (int b = 0; b < 8; ++b) rb[tc + 16 * b] = s[KA][b];
+      for (int b = 0; b < 8; ++b) r0[tc + 16 * b] = s[KA][b];
+    }
+    if (two && tr == ((k + 1) & 15)) {
+#pragma unroll
+      for (int b = 0; b < 8; ++b) r1[tc + 16 * b] = s[KA][b];
     }
     __syncthreads();
-    double d = rb[k];
-    if (!(d > 0.0)) {
-      d = 1e-300;
+    double d0 = r0[k];
+    if (!(d0 > 0.0)) {
+      d0 = 1e-300;
       bad = 1;
     }
-    const double sq = sqrt(d), ri = 1.0 / sq;
-    double ci[8], cj[8];
+    const double sq0 = sqrt(d0), ri0 = 1.0 / sq0;
+    double ci0[8], cj0[8];
 #pragma unroll
-    for (int a = KA; a < 8; ++a) ci[a] = (a > KA || tr + 16 * a > k) ? rb[tr + 16 * a] * ri : 0.0;
+    for (int a = KA; a < 8; ++a) ci0[a] = (a > KA || tr + 16 * a > k) ? r0[tr + 16 * a] * ri0 : 0.0;
 #pragma unroll
-    for (int b = KA; b < 8; ++b) cj[b] = (b > KA || tc + 16 * b > k) ? rb[tc + 16 * b] * ri : 0.0;
+    for (int b = KA; b < 8; ++b) cj0[b] = (b > KA || tc + 16 * b > k) ? r0[tc + 16 * b] * ri0 : 0.0;
 #pragma unroll
     for (int a = KA; a < 8; ++a)
 #pragma unroll
-      for (int b = KA; b < 8; ++b) s[a][b] = fma(-ci[a], cj[b], s[a][b]);
+      for (int b = KA; b < 8; ++b) s[a][b] = fma(-ci0[a], cj0[b], s[a][b]);
+    double sq1 = 0.0, ri1 = 0.0;
+    if (two) {
+      // row k+1 after step k: fma(-c0[k+1], c0[x], S[k+1][x]), as its owner
+      // would have broadcast it
+      const double c0k1 = r0[k + 1] * ri0;
+      double d1 = fma(-c0k1, c0k1, r1[k + 1]);
+      if (!(d1 > 0.0)) {
+        d1 = 1e-300;
+        bad = 1;
+      }
+      sq1 = sqrt(d1);
+      ri1 = 1.0 / sq1;
+      double ci1[8], cj1[8];
+#pragma unroll
+      for (int a = KA; a < 8; ++a) {
+        const int i = tr + 16 * a;
+        ci1[a] = (a > KA || i > k + 1) ? fma(-c0k1, r0[i] * ri0, r1[i]) * ri1 : 0.0;
+      }
+#pragma unroll
+      for (int b = KA; b < 8; ++b) {
+        const int j = tc + 16 * b;
+        cj1[b] = (b > KA || j > k + 1) ? fma(-c0k1, r0[j] * ri0, r1[j]) * ri1 : 0.0;
+      }
+#pragma unroll
+      for (int a = KA; a < 8; ++a)
+#pragma unroll
+        for (int b = KA; b < 8; ++b) s[a][b] = fma(-ci1[a], cj1[b], s[a][b]);
+    }
     if (tr == (k & 15)) {
 #pragma unroll
       for (int b = KA; b < 8; ++b) {
         const int j = tc + 16 * b;
-        s[KA][b] = (j > k) ? s[KA][b] * ri : (j == k ? sq : s[KA][b]);
+        s[KA][b] = (j > k) ? s[KA][b] * ri0 : (j == k ? sq0 : s[KA][b]);
+      }
+    }
+    if (two && tr == ((k + 1) & 15)) {
+#pragma unroll
+      for (int b = KA; b < 8; ++b) {
+        const int j = tc + 16 * b;
+        s[KA][b] = (j > k + 1) ? s[KA][b] * ri1 : (j == k + 1 ? sq1 : s[KA][b]);
       }
     }
   }
 }
 
-// Inverse steps k in [16 KA, 16 KA + 16) from the top down (k < n): rows of
-// blocks a > KA lie below k (no update), columns of blocks b < KA lie left of
-// k, where row k of X = R^{-1} is zero.
+// Inverse steps k in [16 KA, 16 KA + 16) from the top down (k < n), in pairs
+// (k, k-1) inside the block: rows of blocks a > KA lie below k (no update),
+// columns of blocks b < KA lie left of k, where row k of X = R^{-1} is zero.
 template <int KA>
 __device__ __forceinline__ void inv_steps(double (&s)[8][8], const double* Rl, double* rowbuf, int n,
-                                          int tr, int tc) {
+                                          int tr, int tc, int& it) {
   const int khi = min(16 * KA + 15, n - 1);
-  for (int k = khi; k >= 16 * KA; --k) {
-    double* xb = rowbuf + (k & 1) * CHOL_NMAX;
+  for (int k = khi; k >= 16 * KA; k -= 2) {
+    double* x0 = rowbuf + (it & 1) * 2 * CHOL_NMAX;
+    double* x1 = x0 + CHOL_NMAX;
+    ++it;
+    const bool two = k - 1 >= 16 * KA;
     if (tr == (k & 15)) {
       const double rkk = 1.0 / Rl[k * CHOL_LD + k];
 #pragma unroll
       for (int b = KA; b < 8; ++b) {
         s[KA][b] *= rkk;
-        xb[tc + 16 * b] = s[KA][b];
+        x0[tc + 16 * b] = s[KA][b];
       }
+    }
+    if (two && tr == ((k - 1) & 15)) {   // row k-1 before step k
+#pragma unroll
+      for (int b = KA; b < 8; ++b) x1[tc + 16 * b] = s[KA][b];
     }
     __syncthreads();
     double ra[8], xv[8];
@@ -346,18 +401,38 @@ __device__ __forceinline__ void inv_steps(double (&s)[8][8], const double* Rl, d
     for (int a = 0; a <= KA; ++a)
       ra[a] = (a < KA || tr + 16 * a < k) ? Rl[(tr + 16 * a) * CHOL_LD + k] : 0.0;
 #pragma unroll
-    for (int b = KA; b < 8; ++b) xv[b] = xb[tc + 16 * b];
+    for (int b = KA; b < 8; ++b) xv[b] = x0[tc + 16 * b];
 #pragma unroll
     for (int a = 0; a <= KA; ++a)
 #pragma unroll
       for (int b = KA; b < 8; ++b) s[a][b] = fma(-ra[a], xv[b], s[a][b]);
+    if (two) {
+      const int k1 = k - 1;
+      const double rk1 = 1.0 / Rl[k1 * CHOL_LD + k1];
+      const double rk1k = Rl[k1 * CHOL_LD + k];
+      if (tr == (k1 & 15)) {
+#pragma unroll
+        for (int b = KA; b < 8; ++b) s[KA][b] *= rk1;
+      }
+      // X[k-1,:] = (row k-1 after step k) / R[k-1,k-1], as its owner holds it
+      double xw[8];
+#pragma unroll
+      for (int b = KA; b < 8; ++b) xw[b] = fma(-rk1k, xv[b], x1[tc + 16 * b]) * rk1;
+#pragma unroll
+      for (int a = 0; a <= KA; ++a)
+        ra[a] = (a < KA || tr + 16 * a < k1) ? Rl[(tr + 16 * a) * CHOL_LD + k1] : 0.0;
+#pragma unroll
+      for (int a = 0; a <= KA; ++a)
+#pragma unroll
+        for (int b = KA; b < 8; ++b) s[a][b] = fma(-ra[a], xw[b], s[a][b]);
+    }
   }
 }
 
 __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __restrict__ jobs) {
   extern __shared__ double lds[];
   double* Rl = lds;                         // R, row stride CHOL_LD
-  double* rowbuf = lds + CHOL_NMAX * CHOL_LD;  // two broadcast rows
+  double* rowbuf = lds + CHOL_NMAX * CHOL_LD;  // two broadcast row pairs
   const tnx_chol_job J = jobs[blockIdx.x];
   const int n = J.n;
   if (n <= 0 || n > CHOL_NMAX) return;  // rejected on the host
@@ -396,18 +471,18 @@ __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __rest
       for (int b = 0; b < 8; ++b)
         if (tr + 16 * a == tc + 16 * b && tr + 16 * a < n) s[a][b] += sh;
   }
-  int bad = 0;
+  int bad = 0, it = 0;
   // Cholesky: row k of the tiles becomes row k of R at step k.  The whole
   // symmetric trailing block is updated (both triangles), so row k is
   // current when its owners broadcast it.
-  chol_steps<0>(s, rowbuf, n, tr, tc, bad);
-  chol_steps<1>(s, rowbuf, n, tr, tc, bad);
-  chol_steps<2>(s, rowbuf, n, tr, tc, bad);
-  chol_steps<3>(s, rowbuf, n, tr, tc, bad);
-  chol_steps<4>(s, rowbuf, n, tr, tc, bad);
-  chol_steps<5>(s, rowbuf, n, tr, tc, bad);
-  chol_steps<6>(s, rowbuf, n, tr, tc, bad);
-  chol_steps<7>(s, rowbuf, n, tr, tc, bad);
+  chol_steps<0>(s, rowbuf, n, tr, tc, bad, it);
+  chol_steps<1>(s, rowbuf, n, tr, tc, bad, it);
+  chol_steps<2>(s, rowbuf, n, tr, tc, bad, it);
+  chol_steps<3>(s, rowbuf, n, tr, tc, bad, it);
+  chol_steps<4>(s, rowbuf, n, tr, tc, bad, it);
+  chol_steps<5>(s, rowbuf, n, tr, tc, bad, it);
+  chol_steps<6>(s, rowbuf, n, tr, tc, bad, it);
+  chol_steps<7>(s, rowbuf, n, tr, tc, bad, it);
   double* R = (double*)J.r;
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
@@ -427,14 +502,14 @@ __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __rest
 #pragma unroll
     for (int b = 0; b < 8; ++b)
       s[a][b] = (tr + 16 * a == tc + 16 * b && tr + 16 * a < n) ? 1.0 : 0.0;
-  inv_steps<7>(s, Rl, rowbuf, n, tr, tc);
-  inv_steps<6>(s, Rl, rowbuf, n, tr, tc);
-  inv_steps<5>(s, Rl, rowbuf, n, tr, tc);
-  inv_steps<4>(s, Rl, rowbuf, n, tr, tc);
-  inv_steps<3>(s, Rl, rowbuf, n, tr, tc);
-  inv_steps<2>(s, Rl, rowbuf, n, tr, tc);
-  inv_steps<1>(s, Rl, rowbuf, n, tr, tc);
-  inv_steps<0>(s, Rl, rowbuf, n, tr, tc);
+  inv_steps<7>(s, Rl, rowbuf, n, tr, tc, it);
+  inv_steps<6>(s, Rl, rowbuf, n, tr, tc, it);
+  inv_steps<5>(s, Rl, rowbuf, n, tr, tc, it);
+  inv_steps<4>(s, Rl, rowbuf, n, tr, tc, it);
+  inv_steps<3>(s, Rl, rowbuf, n, tr, tc, it);
+  inv_steps<2>(s, Rl, rowbuf, n, tr, tc, it);
+  inv_steps<1>(s, Rl, rowbuf, n, tr, tc, it);
+  inv_steps<0>(s, Rl, rowbuf, n, tr, tc, it);
   double* Ri = (double*)J.rinv;
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
