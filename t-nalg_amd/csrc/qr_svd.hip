@@ -282,20 +282,24 @@ __global__ __launch_bounds__(NT) void svd_kernel(const tnx_svd_job* __restrict__
 // X[k,:] /= R[k,k], X[i,:] -= R[i,k] X[k,:] for i < k.  Every lane forms the
 // second row of a pair from the broadcast rows with the same operations, in
 // the same order, as the step that would have followed a barrier, so the
-// results are those of one step per barrier.
+// results are those of one step per barrier.  The factorisation keeps
+// 1 / R[k,k] (the 1 / sqrt it divides by) in LDS, so the inverse divides
+// nothing.
 constexpr int CHOL_NMAX = 128;
 constexpr int CHOL_LD = CHOL_NMAX + 1;  // LDS row stride of R: column reads hit distinct banks
-constexpr size_t CHOL_LDS = sizeof(double) * (CHOL_NMAX * CHOL_LD + 4 * CHOL_NMAX);
+constexpr size_t CHOL_LDS = sizeof(double) * (CHOL_NMAX * CHOL_LD + 5 * CHOL_NMAX);
 
 // Cholesky steps k in [16 KA, 16 KA + 16) (k < n), in pairs (k, k+1) that
 // stay inside the block (16 KA is even).  With the tile-row block KA of
 // column k a compile-time constant, every register index is static, and the
 // blocks a < KA / b < KA (rows and columns already factored, whose update
-// factors are zero) drop out of the rank-1 updates at compile time.  `it`
-// counts barriers (buffer parity).
+// factors are zero) drop out of the rank-1 updates at compile time, as do
+// the blocks a > b: they lie strictly below the diagonal (row tr + 16a >
+// column tc + 16b), an entry no broadcast or output reads.  `it` counts
+// barriers (buffer parity).
 template <int KA>
-__device__ __forceinline__ void chol_steps(double (&s)[8][8], double* rowbuf, int n, int tr, int tc,
-                                           int& bad, int& it) {
+__device__ __forceinline__ void chol_steps(double (&s)[8][8], double* rowbuf, double* rdiag, int n,
+                                           int tr, int tc, int& bad, int& it) {
   const int kend = min(16 * KA + 16, n);
   for (int k = 16 * KA; k < kend; k += 2) {
     double* r0 = rowbuf + (it & 1) * 2 * CHOL_NMAX;
@@ -317,6 +321,7 @@ __device__ __forceinline__ void chol_steps(double (&s)[8][8], double* rowbuf, in
       bad = 1;
     }
     const double sq0 = sqrt(d0), ri0 = 1.0 / sq0;
+    if (threadIdx.x == 0) rdiag[k] = ri0;
     double ci0[8], cj0[8];
 #pragma unroll
     for (int a = KA; a < 8; ++a) ci0[a] = (a > KA || tr + 16 * a > k) ? r0[tr + 16 * a] * ri0 : 0.0;
@@ -325,7 +330,7 @@ __device__ __forceinline__ void chol_steps(double (&s)[8][8], double* rowbuf, in
 #pragma unroll
     for (int a = KA; a < 8; ++a)
 #pragma unroll
-      for (int b = KA; b < 8; ++b) s[a][b] = fma(-ci0[a], cj0[b], s[a][b]);
+      for (int b = a; b < 8; ++b) s[a][b] = fma(-ci0[a], cj0[b], s[a][b]);
     double sq1 = 0.0, ri1 = 0.0;
     if (two) {
       // row k+1 after step k: fma(-c0[k+1], c0[x], S[k+1][x]), as its owner
@@ -338,6 +343,7 @@ __device__ __forceinline__ void chol_steps(double (&s)[8][8], double* rowbuf, in
       }
       sq1 = sqrt(d1);
       ri1 = 1.0 / sq1;
+      if (threadIdx.x == 0) rdiag[k + 1] = ri1;
       double ci1[8], cj1[8];
 #pragma unroll
       for (int a = KA; a < 8; ++a) {
@@ -352,7 +358,7 @@ __device__ __forceinline__ void chol_steps(double (&s)[8][8], double* rowbuf, in
 #pragma unroll
       for (int a = KA; a < 8; ++a)
 #pragma unroll
-        for (int b = KA; b < 8; ++b) s[a][b] = fma(-ci1[a], cj1[b], s[a][b]);
+        for (int b = a; b < 8; ++b) s[a][b] = fma(-ci1[a], cj1[b], s[a][b]);
     }
     if (tr == (k & 15)) {
 #pragma unroll
@@ -375,8 +381,8 @@ __device__ __forceinline__ void chol_steps(double (&s)[8][8], double* rowbuf, in
 // (k, k-1) inside the block: rows of blocks a > KA lie below k (no update),
 // columns of blocks b < KA lie left of k, where row k of X = R^{-1} is zero.
 template <int KA>
-__device__ __forceinline__ void inv_steps(double (&s)[8][8], const double* Rl, double* rowbuf, int n,
-                                          int tr, int tc, int& it) {
+__device__ __forceinline__ void inv_steps(double (&s)[8][8], const double* Rl, const double* rdiag,
+                                          double* rowbuf, int n, int tr, int tc, int& it) {
   const int khi = min(16 * KA + 15, n - 1);
   for (int k = khi; k >= 16 * KA; k -= 2) {
     double* x0 = rowbuf + (it & 1) * 2 * CHOL_NMAX;
@@ -384,7 +390,7 @@ __device__ __forceinline__ void inv_steps(double (&s)[8][8], const double* Rl, d
     ++it;
     const bool two = k - 1 >= 16 * KA;
     if (tr == (k & 15)) {
-      const double rkk = 1.0 / Rl[k * CHOL_LD + k];
+      const double rkk = rdiag[k];
 #pragma unroll
       for (int b = KA; b < 8; ++b) {
         s[KA][b] *= rkk;
@@ -408,7 +414,7 @@ __device__ __forceinline__ void inv_steps(double (&s)[8][8], const double* Rl, d
       for (int b = KA; b < 8; ++b) s[a][b] = fma(-ra[a], xv[b], s[a][b]);
     if (two) {
       const int k1 = k - 1;
-      const double rk1 = 1.0 / Rl[k1 * CHOL_LD + k1];
+      const double rk1 = rdiag[k1];
       const double rk1k = Rl[k1 * CHOL_LD + k];
       if (tr == (k1 & 15)) {
 #pragma unroll
@@ -433,6 +439,7 @@ __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __rest
   extern __shared__ double lds[];
   double* Rl = lds;                         // R, row stride CHOL_LD
   double* rowbuf = lds + CHOL_NMAX * CHOL_LD;  // two broadcast row pairs
+  double* rdiag = rowbuf + 4 * CHOL_NMAX;      // 1 / R[k,k]
   const tnx_chol_job J = jobs[blockIdx.x];
   const int n = J.n;
   if (n <= 0 || n > CHOL_NMAX) return;  // rejected on the host
@@ -472,17 +479,17 @@ __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __rest
         if (tr + 16 * a == tc + 16 * b && tr + 16 * a < n) s[a][b] += sh;
   }
   int bad = 0, it = 0;
-  // Cholesky: row k of the tiles becomes row k of R at step k.  The whole
-  // symmetric trailing block is updated (both triangles), so row k is
-  // current when its owners broadcast it.
-  chol_steps<0>(s, rowbuf, n, tr, tc, bad, it);
-  chol_steps<1>(s, rowbuf, n, tr, tc, bad, it);
-  chol_steps<2>(s, rowbuf, n, tr, tc, bad, it);
-  chol_steps<3>(s, rowbuf, n, tr, tc, bad, it);
-  chol_steps<4>(s, rowbuf, n, tr, tc, bad, it);
-  chol_steps<5>(s, rowbuf, n, tr, tc, bad, it);
-  chol_steps<6>(s, rowbuf, n, tr, tc, bad, it);
-  chol_steps<7>(s, rowbuf, n, tr, tc, bad, it);
+  // Cholesky: row k of the tiles becomes row k of R at step k.  The trailing
+  // block is updated on and above the diagonal blocks, so the part of row k
+  // right of column k is current when its owners broadcast it.
+  chol_steps<0>(s, rowbuf, rdiag, n, tr, tc, bad, it);
+  chol_steps<1>(s, rowbuf, rdiag, n, tr, tc, bad, it);
+  chol_steps<2>(s, rowbuf, rdiag, n, tr, tc, bad, it);
+  chol_steps<3>(s, rowbuf, rdiag, n, tr, tc, bad, it);
+  chol_steps<4>(s, rowbuf, rdiag, n, tr, tc, bad, it);
+  chol_steps<5>(s, rowbuf, rdiag, n, tr, tc, bad, it);
+  chol_steps<6>(s, rowbuf, rdiag, n, tr, tc, bad, it);
+  chol_steps<7>(s, rowbuf, rdiag, n, tr, tc, bad, it);
   double* R = (double*)J.r;
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
@@ -502,14 +509,14 @@ __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __rest
 #pragma unroll
     for (int b = 0; b < 8; ++b)
       s[a][b] = (tr + 16 * a == tc + 16 * b && tr + 16 * a < n) ? 1.0 : 0.0;
-  inv_steps<7>(s, Rl, rowbuf, n, tr, tc, it);
-  inv_steps<6>(s, Rl, rowbuf, n, tr, tc, it);
-  inv_steps<5>(s, Rl, rowbuf, n, tr, tc, it);
-  inv_steps<4>(s, Rl, rowbuf, n, tr, tc, it);
-  inv_steps<3>(s, Rl, rowbuf, n, tr, tc, it);
-  inv_steps<2>(s, Rl, rowbuf, n, tr, tc, it);
-  inv_steps<1>(s, Rl, rowbuf, n, tr, tc, it);
-  inv_steps<0>(s, Rl, rowbuf, n, tr, tc, it);
+  inv_steps<7>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
+  inv_steps<6>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
+  inv_steps<5>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
+  inv_steps<4>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
+  inv_steps<3>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
+  inv_steps<2>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
+  inv_steps<1>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
+  inv_steps<0>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
   double* Ri = (double*)J.rinv;
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
