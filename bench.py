@@ -399,6 +399,8 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--serving-requests', type=int, default=0)
     ap.add_argument('--serving-rate', type=float, default=4.0)
+    ap.add_argument('--serving-max-active', type=int, default=64,
+                    help='most requests the server advances together')
     ap.add_argument('--workload', default='c2', choices=['c2', 'c3', 'c4', 'c5'],
                     help='c2: the batched N=64 chi=128 XXZ chain (default); c3 / c5: one large '
                          'snake-mapped square-lattice solve (chi 512 / 1024); c4: the serving '
@@ -420,7 +422,7 @@ def main():
     assert torch.cuda.is_available()
     if args.workload == 'c4':
         n_req = args.serving_requests or 512
-        summ = serving_run(n_req, args.serving_rate)
+        summ = serving_run(n_req, args.serving_rate, max_active=args.serving_max_active)
         print(json.dumps({
             'metric': 'DMRG solves/sec + p50/p99 latency (serving stream), 1 GPU',
             'value': summ['solves_per_s'], 'unit': 'solves/s', 'n_gpus': 1, 'steps': 1,
