@@ -258,6 +258,11 @@ def initial_chi(para):
         chi0 = int(para.get('chi_init', chi))
         if not 1 <= chi0 <= chi:
             raise ValueError('chi_init must be in [1, chi]')
+        # a bond of 3 with chi < 6 would need a truncating split of order 6,
+        # below the block SVD's range (bonds 1 and 2 are never truncated,
+        # bonds >= 4 give orders >= 8)
+        if chi0 == 3 and chi < 6:
+            raise ValueError('two-site sweeps: chi_init = 3 needs chi >= 6')
         return chi0
     return chi
 
@@ -278,8 +283,13 @@ def solve_geometry(para, ncv, lowp, dims=None):
     if two:
         if T.d * T.d > L.MIX_DMAX:
             raise ValueError('two-site sweeps need d^2 <= %d (d = %d)' % (L.MIX_DMAX, T.d))
+        if chi < 4:
+            raise ValueError('two-site sweeps need chi >= 4')
         full = [1] + [chi] * (T.n - 1) + [1]
         dm = max_dims([max(a, b) for a, b in zip(dims, full)], T.d, ob)
+        if T.d * max(dm) > L.SVDJ_NMAX:
+            raise ValueError('two-site split of order %d exceeds the SVD limit %d (chi <= %d)'
+                             % (T.d * max(dm), L.SVDJ_NMAX, L.SVDJ_NMAX // T.d))
     else:
         dm = max_dims(dims, T.d, ob)
     key = (tuple(dims), ob, ncv, bool(lowp), chi if two else 0)

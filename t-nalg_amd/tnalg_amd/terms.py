@@ -20,6 +20,7 @@ i.e. an exact refactorisation with ``min(#left, #right)`` GEMM pairs.
 """
 import collections
 import hashlib
+import threading
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -251,6 +252,7 @@ def compile_terms(para, pairs=False):
 
 
 _COMPILED = collections.OrderedDict()
+_COMPILED_LOCK = threading.Lock()   # callers include request-preparation threads
 
 
 def compile_terms_cached(para, maxsize=128, pairs=False):
@@ -270,14 +272,17 @@ def compile_terms_cached(para, maxsize=128, pairs=False):
         h.update(repr((a.dtype.str, a.shape)).encode())
         h.update(a.tobytes())
     key = h.hexdigest()
-    T = _COMPILED.get(key)
-    if T is None:
-        T = compile_terms(para, pairs=pairs)
-        _COMPILED[key] = T
-        if len(_COMPILED) > maxsize:
-            _COMPILED.popitem(last=False)
-    else:
+    with _COMPILED_LOCK:
+        T = _COMPILED.get(key)
+        if T is not None:
+            _COMPILED.move_to_end(key)
+            return T
+    T = compile_terms(para, pairs=pairs)   # outside the lock; a race compiles twice
+    with _COMPILED_LOCK:
+        T = _COMPILED.setdefault(key, T)
         _COMPILED.move_to_end(key)
+        while len(_COMPILED) > maxsize:
+            _COMPILED.popitem(last=False)
     return T
 
 

@@ -130,3 +130,31 @@ def test_two_site_then_one_site_sweeps():
     s.schedule_sweep(1)
     kinds1 = {st.kind for st in s.queue}
     assert kinds0 == {'solve2', 'split2'} and kinds1 <= {'solve', 'move'} and 'solve' in kinds1
+
+
+@pytest.mark.parametrize('kw', [dict(chi=2), dict(chi=600), dict(chi=5, chi_init=3),
+                                dict(chi=8, chi_init=9), dict(dmrg_sites=3)])
+def test_two_site_limits_raise_at_admission(kw):
+    """Limits of the two-site split are checked when the solve is created
+    (a server rejects such a request alone), not when a step is planned."""
+    from tnalg_amd.engine import solve_geometry
+    para = P.make_consistent_parameter_dmrg(dict(P.generate_parameters_dmrg('chain'), l=10,
+                                                 **dict(dict(dmrg_sites=2), **kw)))
+    with pytest.raises(ValueError):
+        solve_geometry(para, 8, False)
+
+
+def test_compiled_terms_cache_is_thread_safe():
+    """Request preparation compiles terms on worker threads: concurrent
+    lookups, inserts and evictions of a small cache neither raise nor lose
+    entries' structure."""
+    import concurrent.futures
+    from tnalg_amd.terms import compile_terms_cached
+    paras = [P.make_consistent_parameter_dmrg(dict(P.generate_parameters_dmrg('chain'), l=6 + k % 5,
+                                                   jz=0.5 + 0.01 * k)) for k in range(40)]
+
+    def work(k):
+        T = compile_terms_cached(paras[k % 40], maxsize=4)
+        return T.n == paras[k % 40]['l']
+    with concurrent.futures.ThreadPoolExecutor(8) as ex:
+        assert all(ex.map(work, range(400)))
