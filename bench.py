@@ -30,7 +30,8 @@ MFMA_PEAK = {'fp64': 78.6, 'fp32': 157.3, 'bf16': 2500.0}  # TFLOP/s, dense
 HBM_PEAK = 8000.0  # GB/s
 
 
-def c2_para(n=64, chi=128, sweeps=8, jz=1.0, hz=0.0, tol=1e-10, sites=1, two_site_sweeps=None):
+def c2_para(n=64, chi=128, sweeps=8, jz=1.0, hz=0.0, tol=1e-10, sites=1, two_site_sweeps=None,
+            chi_init=None):
     from tnalg_amd import parameters as P
     para = P.parameter_dmrg_chain()
     para.update(P.common_parameters_dmrg())
@@ -38,6 +39,8 @@ def c2_para(n=64, chi=128, sweeps=8, jz=1.0, hz=0.0, tol=1e-10, sites=1, two_sit
                 dt_ob=sweeps, break_tol=0.0, dmrg_sites=sites)
     if two_site_sweeps is not None:
         para['two_site_sweeps'] = two_site_sweeps
+    if chi_init is not None:
+        para['chi_init'] = chi_init
     return P.make_consistent_parameter_dmrg(para)
 
 
@@ -386,6 +389,8 @@ def main():
     ap.add_argument('--dmrg-sites', type=int, default=1, choices=[1, 2],
                     help='2: two-site sweeps with SVD truncation to chi (the roofline stays '
                          'on the one-site matvec)')
+    ap.add_argument('--chi-init', type=int, default=None,
+                    help='with --dmrg-sites 2: bond dimension of the random initial MPS')
     ap.add_argument('--two-site-sweeps', type=int, default=None,
                     help='with --dmrg-sites 2: only the first K sweeps are two-site')
     ap.add_argument('--no-cpu', action='store_true')
@@ -440,7 +445,7 @@ def main():
         print(json.dumps(large_solve(args.workload, fp64_check=not args.no_plain)))
         return
     para = c2_para(args.n, args.chi, args.sweeps, sites=args.dmrg_sites,
-                   two_site_sweeps=args.two_site_sweeps)
+                   two_site_sweeps=args.two_site_sweeps, chi_init=args.chi_init)
     if args.roofline_only:
         rl = roofline_object(para, args)
         rl['copy_calibration_bytes'] = copy_calibration()
@@ -474,7 +479,8 @@ def main():
            'config': {'workload': 'XXZ chain N=%d open, Jxy=1 Jz=1 h=0, chi=%d, ncv=20, tol=1e-10, '
                                   '%d sweeps' % (args.n, args.chi, args.sweeps),
                       'batch': args.batch, 'precision': args.precision,
-                      'dmrg_sites': args.dmrg_sites, 'two_site_sweeps': args.two_site_sweeps},
+                      'dmrg_sites': args.dmrg_sites, 'two_site_sweeps': args.two_site_sweeps,
+                      'chi_init': args.chi_init},
            'latency_s': {'p50': float(np.percentile(lat, 50)), 'p99': float(np.percentile(lat, 99))},
            'energy_per_site_mean': float(np.mean(energies)) / args.n,
            'cold_start_s': cold}
