@@ -100,3 +100,28 @@ def test_ritz_energy_equals_measured(gpu):
         e_r, e_m = got[('ritz', 8)][k], got[('measure', 8)][k]
         assert got[('ritz', 0)][k] == e_r
         assert abs(e_r - e_m) < 1e-10 * abs(e_m), (k, e_r, e_m)
+
+
+def test_two_site_requests_in_the_stream(gpu):
+    """Two-site requests (prepared on a worker, measured at retirement) served
+    together with one-site ones give the same bits as each served alone."""
+    from tnalg_amd.dmrg import _schedule_from
+    from tnalg_amd.engine import Engine
+    from tnalg_amd.serving import SolveServer
+    reqs = _requests()[:4]
+    reqs = [(t, dict(para, dmrg_sites=2, chi_init=4) if k % 2 else para, seed)
+            for k, (t, para, seed) in enumerate(reqs)]
+
+    def serve(items, max_active):
+        eng = Engine(precision_schedule=_schedule_from('mixed'), lowp=True, ncv=NCV)
+        srv = SolveServer(eng, hbm_budget=8 << 30, max_active=max_active)
+        for k, (t, para, seed) in items:
+            srv.submit(k, para, seed=seed, arrival=0.0)
+        summ = srv.run()
+        assert summ['failed'] == 0
+        return {r.rid: r.energy for r in srv.finished}
+    print('two-site requests: mixed stream', flush=True)
+    mixed = serve(list(enumerate(reqs)), 4)
+    for k, item in enumerate(reqs):
+        print('two-site requests: alone', k, 'sites', item[1].get('dmrg_sites', 1), flush=True)
+        assert serve([(k, item)], 1)[k] == mixed[k], k
