@@ -378,14 +378,14 @@ def pack_relative(ph, lo, hi, slot):
     if arrs:
         sizes = np.fromiter((a.nbytes for a in arrs), np.int64, len(arrs))
         offs = np.cumsum(sizes) - sizes
-        buf = np.concatenate([np.ascontiguousarray(a).view(np.uint8) for a in arrs])
+        buf = np.frombuffer(bytearray(b''.join([a.tobytes() for a in arrs])), np.uint8)
         fw = np.array([_field_words(a.dtype) for a in arrs], dtype=np.int64).reshape(-1, 4)
         n = np.fromiter((len(a) for a in arrs), np.int64, len(arrs))
     else:
         offs = sizes = n = np.zeros(0, np.int64)
         buf = np.zeros(0, np.uint8)
         fw = np.zeros((0, 4), np.int64)
-    pk.specs = [(int(o), a.dtype, len(a)) for o, a in zip(offs, arrs)]
+    pk.specs = list(zip(offs.tolist(), [a.dtype for a in arrs], n.tolist()))
     # per record: array id and first word
     aid = np.repeat(np.arange(len(arrs)), n)
     first = np.cumsum(n) - n
