@@ -266,15 +266,21 @@ def _flatten(obj, arrs):
 
 def _inflate(tree, buf, specs):
     """``tree`` with each leaf replaced by its records viewed in ``buf``
-    (``frombuffer``: writable, and cheaper than a structured ``view``)."""
+    (``frombuffer``: writable, and cheaper than a structured ``view``).
+    Leaves inside a tuple or list are viewed in place, without a call each."""
     tt = type(tree)
     if tt is _Leaf:
         off, dt, n = specs[tree.i]
         return np.frombuffer(buf, dt, n, off)
-    if tt is tuple:
-        return tuple([_inflate(t, buf, specs) for t in tree])
-    if tt is list:
-        return [_inflate(t, buf, specs) for t in tree]
+    if tt is tuple or tt is list:
+        out = []
+        for t in tree:
+            if type(t) is _Leaf:
+                off, dt, n = specs[t.i]
+                out.append(np.frombuffer(buf, dt, n, off))
+            else:
+                out.append(_inflate(t, buf, specs))
+        return tuple(out) if tt is tuple else out
     return tree
 
 
