@@ -252,15 +252,21 @@ def _field_words(dt):
 
 def _flatten(obj, arrs):
     """``obj`` with each record array replaced by a leaf holding its index
-    in ``arrs`` (appended)."""
+    in ``arrs`` (appended).  Record arrays inside a tuple or list are taken
+    in place, without a call each."""
     to = type(obj)
     if to is np.ndarray and obj.dtype.names is not None:
         arrs.append(obj)
         return _Leaf(len(arrs) - 1)
-    if to is tuple:
-        return tuple([_flatten(o, arrs) for o in obj])
-    if to is list:
-        return [_flatten(o, arrs) for o in obj]
+    if to is tuple or to is list:
+        out = []
+        for o in obj:
+            if type(o) is np.ndarray and o.dtype.names is not None:
+                arrs.append(o)
+                out.append(_Leaf(len(arrs) - 1))
+            else:
+                out.append(_flatten(o, arrs))
+        return tuple(out) if to is tuple else out
     return obj
 
 
@@ -334,10 +340,9 @@ def _shift_tree(tree, d):
     tt = type(tree)
     if tt is _Leaf:
         return _Leaf(tree.i + d)
-    if tt is tuple:
-        return tuple([_shift_tree(t, d) for t in tree])
-    if tt is list:
-        return [_shift_tree(t, d) for t in tree]
+    if tt is tuple or tt is list:
+        out = [_Leaf(t.i + d) if type(t) is _Leaf else _shift_tree(t, d) for t in tree]
+        return tuple(out) if tt is tuple else out
     return tree
 
 
