@@ -184,6 +184,9 @@ class SolveServer:
             self._retire(self.clock())
             if max_steps is not None and self.steps >= max_steps:
                 break
+        if not self.waiting and self._prep_pool is not None:
+            self._prep_pool.shutdown(wait=True)   # every request admitted or failed
+            self._prep_pool = None
         return self.summary()
 
     def summary(self):
