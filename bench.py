@@ -47,6 +47,7 @@ def c2_para(n=64, chi=128, sweeps=8, jz=1.0, hz=0.0, tol=1e-10, sites=1, two_sit
 _ENGINES = {}
 _PLAN_AHEAD = [True]   # --no-plan-ahead: next-step host work in line (for comparison)
 _CYCLE_STEPS = [True]  # serving: one Krylov cycle per Lanczos call (--no-cycle-steps: all)
+_PIPELINES = [1]       # --pipelines k: the batch as k sub-batches on k engines, streams, threads
 _LZ_STREAMS = [None]   # Lanczos calls of a step over this many streams (None: 2 for the
                        # fixed batch, 1 for serving, where one Krylov cycle per call and
                        # small mixed groups measured faster on one stream)
@@ -72,16 +73,12 @@ def _release_engine(precision, ncv=20):
     torch.cuda.empty_cache()
 
 
-def run_batch(para, seeds, precision, ncv=20):
-    """One batched solve: every chain of ``seeds`` from its seeded random MPS
-    through all sweeps in lockstep, then its energy <H>."""
+def _solve_batch(eng, para, seeds):
+    """Every chain of ``seeds`` from its seeded random MPS through all sweeps
+    in lockstep on ``eng`` (on the calling thread's current stream), then its
+    energy <H>; the solves are released."""
     import torch
     from tnalg_amd.dmrg import observe_energies
-    eng = engine_for(precision, ncv)
-    for k in eng.counters:
-        eng.counters[k] = 0.0 if isinstance(eng.counters[k], float) else 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
     solves = eng.add_solves([(para, s) for s in seeds])
     for s in solves:
         s.schedule_canonicalise()
@@ -92,12 +89,78 @@ def run_batch(para, seeds, precision, ncv=20):
         eng.run_queues(solves)
     energies = [float(np.ravel(ob['e_per_site'])[0]) * s.n
                 for s, ob in zip(solves, observe_energies(eng, solves, [para] * len(solves)))]
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    stats = dict(eng.counters)
+    torch.cuda.current_stream().synchronize()
     for s in list(solves):
         eng.remove_solve(s)
-    return dt, np.array(energies), stats, eng
+    return energies
+
+
+def run_batch(para, seeds, precision, ncv=20):
+    """One batched solve: every chain of ``seeds`` from its seeded random MPS
+    through all sweeps in lockstep, then its energy <H>.  With ``--pipelines
+    k`` the batch is cut into k sub-batches, each advanced by its own engine on
+    its own stream and host thread, so one sub-batch's latency-bound gauge
+    moves overlap another's Lanczos calls."""
+    import torch
+    if _PIPELINES[0] > 1 and len(seeds) >= 2 * _PIPELINES[0]:
+        return _run_pipelined(para, seeds, precision, ncv)
+    eng = engine_for(precision, ncv)
+    for k in eng.counters:
+        eng.counters[k] = 0.0 if isinstance(eng.counters[k], float) else 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    energies = _solve_batch(eng, para, seeds)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return dt, np.array(energies), dict(eng.counters), eng
+
+
+def _run_pipelined(para, seeds, precision, ncv):
+    import threading
+    import torch
+    from tnalg_amd.dmrg import _schedule_from
+    from tnalg_amd.engine import Engine
+    k = _PIPELINES[0]
+    key = ('pipelines', k, precision, ncv)
+    if key not in _ENGINES:
+        streams = [torch.cuda.Stream() for _ in range(k)]
+        engs = []
+        for st in streams:
+            with torch.cuda.stream(st):
+                engs.append(Engine(precision_schedule=_schedule_from(precision), ncv=ncv,
+                                   lowp=precision not in (None, 'fp64'),
+                                   plan_ahead=_PLAN_AHEAD[0], lanczos_streams=1))
+        _ENGINES[key] = (engs, streams)
+    engs, streams = _ENGINES[key]
+    for eng in engs:
+        for c in eng.counters:
+            eng.counters[c] = 0.0 if isinstance(eng.counters[c], float) else 0
+    parts = [list(p) for p in np.array_split(np.asarray(seeds), k)]
+    out, err = [None] * k, [None] * k
+
+    def work(i):
+        try:
+            with torch.cuda.stream(streams[i]):
+                out[i] = _solve_batch(engs[i], para, parts[i])
+        except BaseException as e:   # re-raised on the calling thread
+            err[i] = e
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(k)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    for e in err:
+        if e is not None:
+            raise e
+    counters = {}
+    for eng in engs:
+        for c, v in eng.counters.items():
+            counters[c] = counters.get(c, 0) + v
+    return dt, np.array(sum(out, [])), counters, engs[0]
 
 
 def matvec_roofline(para, batch, precs=('fp64', 'bf16'), reps=20):
@@ -386,6 +449,9 @@ def main():
     ap.add_argument('--chi', type=int, default=128)
     ap.add_argument('--sweeps', type=int, default=8)
     ap.add_argument('--no-plain', action='store_true')
+    ap.add_argument('--pipelines', type=int, default=1,
+                    help='advance the batch as this many sub-batches, each on its own engine, '
+                         'stream and host thread')
     ap.add_argument('--dmrg-sites', type=int, default=1, choices=[1, 2],
                     help='2: two-site sweeps with SVD truncation to chi (the roofline stays '
                          'on the one-site matvec)')
@@ -417,6 +483,7 @@ def main():
     _PLAN_AHEAD[0] = not args.no_plan_ahead
     _CYCLE_STEPS[0] = not args.no_cycle_steps
     _LZ_STREAMS[0] = args.lanczos_streams
+    _PIPELINES[0] = max(1, args.pipelines)
     if args.gpus != 1:
         print('bench.py measures one GPU only (one-GPU build); --gpus %d is not supported'
               % args.gpus, file=sys.stderr)
@@ -480,7 +547,7 @@ def main():
                                   '%d sweeps' % (args.n, args.chi, args.sweeps),
                       'batch': args.batch, 'precision': args.precision,
                       'dmrg_sites': args.dmrg_sites, 'two_site_sweeps': args.two_site_sweeps,
-                      'chi_init': args.chi_init},
+                      'chi_init': args.chi_init, 'pipelines': _PIPELINES[0]},
            'latency_s': {'p50': float(np.percentile(lat, 50)), 'p99': float(np.percentile(lat, 99))},
            'energy_per_site_mean': float(np.mean(energies)) / args.n,
            'cold_start_s': cold}
