@@ -75,10 +75,14 @@ def _release_engine(precision, ncv=20):
 
 def _solve_batch(eng, para, seeds):
     """Every chain of ``seeds`` from its seeded random MPS through all sweeps
-    in lockstep on ``eng`` (on the calling thread's current stream), then its
-    energy <H>; the solves are released."""
+    in lockstep on ``eng`` (on the calling thread's current stream), then
+    every output ``dmrg_finite_size`` gives (``DMRG_anyH.py:66-70,91-95``:
+    eb_full, mx, mz, e_per_site, eb, the entanglement spectrum of every bond
+    by SVD centre moves, the entropies, corr_x and corr_z); the solves are
+    released.  Returns one dict per solve (the ob keys plus ``E``, ``lm``,
+    ``ent``)."""
     import torch
-    from tnalg_amd.dmrg import observe_energies
+    from tnalg_amd.dmrg import finish_outputs
     solves = eng.add_solves([(para, s) for s in seeds])
     for s in solves:
         s.schedule_canonicalise()
@@ -87,17 +91,21 @@ def _solve_batch(eng, para, seeds):
         for s in solves:
             s.schedule_sweep(t)
         eng.run_queues(solves)
-    energies = [float(np.ravel(ob['e_per_site'])[0]) * s.n
-                for s, ob in zip(solves, observe_energies(eng, solves, [para] * len(solves)))]
+    obs, spectra = finish_outputs(eng, solves, [para] * len(solves))
     torch.cuda.current_stream().synchronize()
+    for s, ob, (lm, ent) in zip(solves, obs, spectra):
+        ob['E'] = float(np.ravel(ob['e_per_site'])[0]) * s.n
+        ob['lm'], ob['ent'] = lm, ent
+        ob['warnings'] = list(s.warnings)
     for s in list(solves):
         eng.remove_solve(s)
-    return energies
+    return obs
 
 
 def run_batch(para, seeds, precision, ncv=20):
     """One batched solve: every chain of ``seeds`` from its seeded random MPS
-    through all sweeps in lockstep, then its energy <H>.  With ``--pipelines
+    through all sweeps in lockstep, then all its outputs (``_solve_batch``);
+    returns ``(seconds, outputs, engine counters, engine)``.  With ``--pipelines
     k`` the batch is cut into k sub-batches, each advanced by its own engine on
     its own stream and host thread, so one sub-batch's latency-bound gauge
     moves overlap another's Lanczos calls."""
@@ -109,10 +117,10 @@ def run_batch(para, seeds, precision, ncv=20):
         eng.counters[k] = 0.0 if isinstance(eng.counters[k], float) else 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    energies = _solve_batch(eng, para, seeds)
+    obs = _solve_batch(eng, para, seeds)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return dt, np.array(energies), dict(eng.counters), eng
+    return dt, obs, dict(eng.counters), eng
 
 
 def _run_pipelined(para, seeds, precision, ncv):
@@ -160,7 +168,7 @@ def _run_pipelined(para, seeds, precision, ncv):
     for eng in engs:
         for c, v in eng.counters.items():
             counters[c] = counters.get(c, 0) + v
-    return dt, np.array(sum(out, [])), counters, engs[0]
+    return dt, sum(out, []), counters, engs[0]
 
 
 def matvec_roofline(para, batch, precs=('fp64', 'bf16'), reps=20):
@@ -287,6 +295,20 @@ def roofline_object(para, args):
                                   para['l'] // 2, para['l'], para['chi']),
             'avg_launch_s': dom['avg_launch_s'], 'flops_per_launch': dom['flops_per_launch'],
             'matvec_s': {p: rf[p]['t_matvec'] for p in precs}, 'kernels': kern}
+
+
+def output_differences(obs, ref):
+    """Largest absolute difference, over the solves, of every output against
+    the f64 solve of the same seed (row N2 of the precision statement):
+    energy (relative), mx, mz, eb_full, eb, corr_x, corr_z, the spectra and
+    entropies."""
+    out = {'E_rel': max(abs(a['E'] - b['E']) / abs(b['E']) for a, b in zip(obs, ref))}
+    for key in ('mx', 'mz', 'eb_full', 'eb', 'corr_x', 'corr_z', 'ent'):
+        out[key] = max(float(np.abs(np.ravel(a[key]) - np.ravel(b[key])).max())
+                       for a, b in zip(obs, ref))
+    out['lm'] = max(float(np.abs(x - y[:len(x)]).max()) if len(x) == len(y) else float('inf')
+                    for a, b in zip(obs, ref) for x, y in zip(a['lm'], b['lm']))
+    return out
 
 
 def gemm_flops(p, s):
@@ -527,16 +549,17 @@ def main():
         cold = dt if cold is None else cold
         print('warmup %d: %.3f s' % (w, dt), file=sys.stderr, flush=True)
     times, lat = [], []
-    energies = None
+    obs = None
     counters = None
     for k in range(args.steps):
         torch.cuda.synchronize()
-        dt, energies, counters, _ = run_batch(para, seeds, args.precision)
+        dt, obs, counters, _ = run_batch(para, seeds, args.precision)
         times.append(dt)
         lat += [dt] * len(seeds)
         print('step %d: %.3f s for %d solves' % (k, dt, len(seeds)), file=sys.stderr, flush=True)
     total = sum(times)
     value = args.batch * args.steps / total
+    energies = np.array([o['E'] for o in obs])
     out = {'metric': 'DMRG solves/sec + p50/p99 latency at |dE/E|<=1e-9 vs FP64 plain, 1 GPU',
            'value': value, 'unit': 'solves/s', 'n_gpus': 1, 'steps': args.steps,
            'warmup': args.warmup, 'ms_per_step': 1e3 * total / args.steps,
@@ -550,23 +573,31 @@ def main():
                       'chi_init': args.chi_init, 'pipelines': _PIPELINES[0]},
            'latency_s': {'p50': float(np.percentile(lat, 50)), 'p99': float(np.percentile(lat, 99))},
            'energy_per_site_mean': float(np.mean(energies)) / args.n,
-           'cold_start_s': cold}
+           'cold_start_s': cold,
+           'timed_outputs': 'per solve: eb_full, mx, mz, e_per_site, eb, entanglement spectrum '
+                            'of every bond (SVD centre moves), ent, corr_x, corr_z',
+           'solves_with_warnings': sum(1 for o in obs if o['warnings'])}
     out['engine_counters'] = {k: (float(v) if isinstance(v, float) else v)
                               for k, v in counters.items()}
     if not args.no_plain:
-        # the f64 reference of every seed, batched, and the plain f64 path
-        # (one solve at a time, all-f64)
-        dt_b, e_b, _, _ = run_batch(para, seeds, 'fp64')
+        # the f64 reference of every seed, batched on an engine warmed by a
+        # batch of the same shape (other seeds), like the timed engine; and the
+        # plain f64 path (one solve at a time, all-f64, warmed likewise)
+        run_batch(para, [1000 + s for s in seeds], 'fp64')
+        dt_b, ob64, _, _ = run_batch(para, seeds, 'fp64')
+        e_b = np.array([o['E'] for o in ob64])
         rel = np.abs(energies - e_b) / np.abs(e_b)
         out['fp64_batched'] = {'solves_per_s': len(seeds) / dt_b, 'latency_s': dt_b,
-                               'note': 'first batch of its engine (cold plans)'}
+                               'note': 'same batch, all-f64 schedule, warmed engine'}
+        out['speedup_vs_fp64_batched'] = value * dt_b / len(seeds)
         out['dE_over_E_vs_fp64'] = {'max': float(rel.max()), 'median': float(np.median(rel)),
                                     'target': 1e-9, 'met': bool(rel.max() <= 1e-9)}
+        out['max_abs_diff_vs_fp64'] = output_differences(obs, ob64)
         _release_engine('fp64')
         run_batch(para, [1000], 'fp64')     # warm the one-solve engine like the timed one
-        dt_p, e_p, _, _ = run_batch(para, seeds[:1], 'fp64')
+        dt_p, ob_p, _, _ = run_batch(para, seeds[:1], 'fp64')
         out['plain_fp64'] = {'solves_per_s': 1.0 / dt_p, 'latency_s': dt_p,
-                             'energy': float(e_p[0])}
+                             'energy': float(ob_p[0]['E'])}
         out['speedup_vs_plain_fp64'] = value / (1.0 / dt_p)
         print('fp64 batched %.2f solves/s, plain %.3f solves/s, max dE/E %.2e'
               % (len(seeds) / dt_b, 1.0 / dt_p, rel.max()), file=sys.stderr, flush=True)

@@ -121,7 +121,9 @@ typedef struct tnx_lanczos_solve {
   uint64_t init;       /* f64 start vector (read at start)                  */
   uint64_t out;        /* f64 result vector (written at the end)            */
   int32_t n, ldv;      /* dimension, basis stride (elements)                */
-  int32_t m;           /* Krylov dimension used (<= ncv, <= n)              */
+  int32_t m;           /* reserved: not read by the kernels (the call's ncv
+                          bounds the basis, and a solve whose basis spans all
+                          n dimensions stops as converged)                   */
   int32_t chunk_begin; /* prefix sum of 1024-element chunks over solves     */
   double tol_scale;    /* converged when |r| <= tol_scale*|1 - tau*theta|   */
   double tau;          /*   (ARPACK's test on I - tau*H, MPSClass.py:886)   */

@@ -50,3 +50,36 @@ def xx_chain_energy(n, j=1.0):
     energies ``j cos(k pi/(n+1))``, k = 1..n."""
     eps = j * np.cos(np.arange(1, n + 1) * np.pi / (n + 1))
     return float(np.sum(eps[eps < 0]))
+
+
+# ---------------------------------------------------------------------------
+# Brute-force expectation values on a dense state vector (site 0 is the most
+# significant factor, as above).  These fix, by direct summation over the
+# 2^N amplitudes, what the reference's transfer-chain observables compute
+# (library/MPSClass.py:1105-1212): <O_i>, <O_i O'_j> and their combinations.
+
+def apply_site_op(op, i, psi, n, d=2):
+    """``(1 x .. x op_i x .. x 1) psi`` by reshaping, for any (complex) op."""
+    v = np.asarray(psi).reshape(d ** i, d, d ** (n - i - 1))
+    return np.einsum('st,atb->asb', np.asarray(op), v).reshape(-1)
+
+
+def expect_one(psi, op, i, n):
+    """``<psi|op_i|psi> / <psi|psi>`` (real part)."""
+    psi = np.asarray(psi)
+    nrm = np.vdot(psi, psi)
+    return float(np.real(np.vdot(psi, apply_site_op(op, i, psi, n)) / nrm))
+
+
+def expect_two(psi, op_a, i, op_b, j, n):
+    """``<psi|op_a(i) op_b(j)|psi> / <psi|psi>`` (real part), i != j."""
+    psi = np.asarray(psi)
+    nrm = np.vdot(psi, psi)
+    w = apply_site_op(op_b, j, psi, n)
+    w = apply_site_op(op_a, i, w, n)
+    return float(np.real(np.vdot(psi, w) / nrm))
+
+
+def spin_dot(psi, i, j, n, ops):
+    """``<S_i . S_j>`` with the complex ``sy`` (``ops`` = the operator list)."""
+    return sum(expect_two(psi, ops[k], i, ops[k], j, n) for k in (1, 2, 3))

@@ -428,6 +428,13 @@ class Solve:
         self._lm = [np.zeros(0) for _ in range(n - 1)]
         self._lm_pending = {}   # bond -> length of a spectrum written on the device
         self.failed = None
+        # non-fatal failures of this solve's steps (an eigensolve or an SVD
+        # that hit its iteration limit; the step's result is still written)
+        self.warnings = []
+        self._lz_site = -1    # site of the step being run (for the warnings)
+        # precision of every later sweep, replacing the schedule's (the driver
+        # switches a solve that converged at low precision to f64)
+        self.prec_override = None
         # shared plan parts relocated to this solve, by (plan uid, phase);
         # a solve revisits the same plans every sweep
         self.moved = {}
@@ -1414,7 +1421,14 @@ class Engine:
         """Matvec precision of step ``st`` of solve ``s`` (moves: f64)."""
         if st.kind not in ('solve', 'resume', 'solve2', 'resume2'):
             return 'fp64'
-        return self.precision_schedule(max(st.sweep, 0), s.para.get('sweep_time', 1))
+        return self.sweep_precision(s, st.sweep)
+
+    def sweep_precision(self, s, sweep):
+        """Matvec precision of sweep ``sweep`` of solve ``s``: the schedule's,
+        unless the solve has been switched to f64 (``s.prec_override``)."""
+        if s.prec_override is not None:
+            return s.prec_override
+        return self.precision_schedule(max(sweep, 0), s.para.get('sweep_time', 1))
 
     def settle(self):
         """Wait for the worker's next-step host work (a no-op without it)."""
@@ -1701,9 +1715,9 @@ class Engine:
             rc0 = self._lanczos_call(preps[0], self.stream(), restarts)
             rcs = [rc0, fut.result()]   # that call ends with its stream synchronised
         self.counters['lz_time'] += time.perf_counter() - t0
-        for rc in rcs:
-            if rc not in (0, -3):
-                L.check(rc, 'tnx_lanczos_run')
+        for rc, msg in rcs:
+            if rc not in (L.OK, L.ENOCONV):
+                raise L.HipError('tnx_lanczos_run_multi failed (rc=%d): %s' % (rc, msg))
         self.counters['lanczos_calls'] += 1
         for prep in preps:
             for (s, ph), r in zip(prep['flat'], prep['res']):
@@ -1721,6 +1735,9 @@ class Engine:
                 s.thetas.append(s.last_theta)
                 if not r['converged']:
                     s.stats['unconverged'] += 1
+                    s.warnings.append('local eigensolve at site %d did not converge '
+                                      '(residual %.3g after %d matvecs)'
+                                      % (s._lz_site, float(r['resid']), it))
 
     def _lanczos_prepare(self, k, groups, got):
         """Resident descriptors, the linear-combination mix (on the engine's
@@ -1756,12 +1773,16 @@ class Engine:
 
     def _lanczos_call(self, prep, stream, restarts):
         """The C Lanczos loop of one prepared call on ``stream`` (holds no
-        GIL; returns when the stream's work is done)."""
-        return self.lib.tnx_lanczos_run_multi(prep['plans'], prep['ngroups'],
-                                              ctypes_void(prep['ptrs'][2]), len(prep['flat']),
-                                              prep['nchunks'], self.ncv, restarts, self.min_iters,
-                                              ctypes_void(prep['state'].data_ptr()),
-                                              ctypes_void(prep['res'].ctypes.data), stream)
+        GIL; returns when the stream's work is done).  Returns ``(rc, msg)``:
+        the library keeps its last error per thread, so the message is read
+        on the thread that made the call."""
+        rc = self.lib.tnx_lanczos_run_multi(prep['plans'], prep['ngroups'],
+                                            ctypes_void(prep['ptrs'][2]), len(prep['flat']),
+                                            prep['nchunks'], self.ncv, restarts, self.min_iters,
+                                            ctypes_void(prep['state'].data_ptr()),
+                                            ctypes_void(prep['res'].ctypes.data), stream)
+        msg = '' if rc in (L.OK, L.ENOCONV) else self.lib.tnx_last_error().decode(errors='replace')
+        return rc, msg
 
     def step(self, solves=None):
         """Advance every solve with a queued entry by one step."""
@@ -1774,6 +1795,7 @@ class Engine:
         plans = []
         for s in solves:
             st = s.queue.pop(0)
+            s._lz_site = st.site
             nx, s._next = s._next, None
             if nx is not None and nx[0] is st:   # planned ahead by the worker
                 prec, ph = nx[1], nx[2]
@@ -1838,7 +1860,14 @@ class Engine:
             rc = lib.tnx_svd_block_jacobi(ctypes_void(ptr[ai]), nj, npairs, max_nblk, max_n,
                                           ctypes_void(self.svd_flag.data_ptr()), 60, st_)
             if rc == L.ENOCONV:
+                # the rotation flag is shared by the launch: every solve with
+                # an SVD job in it is flagged
                 self.counters['svd_unconverged'] = self.counters.get('svd_unconverged', 0) + 1
+                for s, st, ph in plans:
+                    if 'svdj' in ph:
+                        s.stats['svd_unconverged'] = s.stats.get('svd_unconverged', 0) + 1
+                        s.warnings.append('block Jacobi SVD of step %r reached 60 sweeps '
+                                          '(%d SVDs in the launch)' % (st, nj))
             else:
                 L.check(rc, 'tnx_svd_block_jacobi')
             if gnt:

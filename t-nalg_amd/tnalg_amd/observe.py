@@ -83,10 +83,71 @@ class _MeasurePlan:
         return _p(self.E, b * self.dsq)
 
 
+def real_rows(ops, a, b, tol=1e-12):
+    """Rows ``[(a', b', c)]`` of real operators whose sum ``sum c <a'(i) b'(j)>``
+    is ``Re <op_a(i) op_b(j)>`` for a real state.
+
+    The device table holds the real part ``R`` of every operator.  With
+    ``op = R + iI`` and a real state, ``<I_i R_j>`` is real, so only the
+    product of two imaginary parts reaches the real part:
+    ``Re <op_a op_b> = <R_a R_b> - <I_a I_b>``.  ``I_a`` and ``I_b`` are
+    written as real combinations of the operator list (``Sy = i (S+ - S-)/2``
+    is ``(op4 - op5) i/2``); an imaginary part outside that span raises.
+    The reference takes the same real part (``observe_bond_energy_from_jxyz``,
+    ``MPSClass.py:1184-1194``)."""
+    oa, ob = np.asarray(ops[a]), np.asarray(ops[b])
+    ia, ib = np.imag(oa), np.imag(ob)
+    if not (np.abs(ia).max() > 0 and np.abs(ib).max() > 0):
+        return [(a, b, 1.0)]
+    basis = [k for k, o in enumerate(ops) if np.abs(np.real(np.asarray(o))).max() > 0]
+    M = np.stack([np.real(np.asarray(ops[k])).reshape(-1) for k in basis], axis=1)
+
+    def decompose(im, k):
+        c, *_ = np.linalg.lstsq(M, im.reshape(-1), rcond=None)
+        if np.abs(M @ c - im.reshape(-1)).max() > tol * max(1.0, np.abs(im).max()):
+            raise ValueError('operator %d: its imaginary part is not a real combination of the '
+                             'operator list, so <op op> cannot be measured in real arithmetic'
+                             % k)
+        return [(basis[t], float(x)) for t, x in enumerate(c) if abs(x) > tol]
+    rows = [] if not (np.abs(np.real(oa)).max() > 0 and np.abs(np.real(ob)).max() > 0) \
+        else [(a, b, 1.0)]
+    for ka, ca in decompose(ia, a):
+        for kb, cb in decompose(ib, b):
+            rows.append((ka, kb, -ca * cb))
+    return rows
+
+
 def measure(engine, solves, requests):
     """``requests[k] = (one, two)`` for ``solves[k]``: ``one`` = [(site, op)],
     ``two`` = [(i, j, op_i, op_j)].  Returns, per solve, ``(one_vals, two_vals,
-    norm)`` as numpy arrays (real parts)."""
+    norm)`` as numpy arrays: the real parts of the expectation values
+    (``one``: ``Re <op>``, which for a real state is that of the operator's
+    real part; ``two``: see :func:`real_rows`)."""
+    expanded, maps = [], []
+    for s, (one, two) in zip(solves, requests):
+        ops = s.para['op']
+        rows, where = [], []
+        cache = {}
+        for (i, j, a, b) in two:
+            key = (int(a), int(b))
+            if key not in cache:
+                cache[key] = real_rows(ops, *key)
+            lst = []
+            for (ka, kb, c) in cache[key]:
+                lst.append((len(rows), c))
+                rows.append((i, j, ka, kb))
+            where.append(lst)
+        expanded.append((one, rows))
+        maps.append(where)
+    out = []
+    for (vone, vtwo, norm), where in zip(_measure_real(engine, solves, expanded), maps):
+        two = np.array([sum(c * vtwo[t] for t, c in lst) for lst in where], dtype=np.float64)
+        out.append((vone, two, norm))
+    return out
+
+
+def _measure_real(engine, solves, requests):
+    """:func:`measure` for operators whose real parts are meant."""
     engine.settle()   # solve state is final once the engine's host work is done
     plans = [_MeasurePlan(s, one, two) for s, (one, two) in zip(solves, requests)]
     lib = engine.lib

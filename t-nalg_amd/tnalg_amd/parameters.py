@@ -12,6 +12,8 @@ Conventions: ``op = [id, sx, sy, sz, su, sd, -hx*sx - hz*sz]``; an
 ``index2`` row ``(i, j, a, b)`` with ``coeff2`` c is ``c op[a]_i op[b]_j``;
 Heisenberg bonds expand to rows ``(4,5), (5,4), (3,3)`` with coefficients
 ``(jxy/2, jxy/2, jz)`` (``is_pauli``: ``(2jxy, 2jxy, 4jz)``, ``coeff1 = 2``).
+``para['coeff_convention'] = 'easystart'`` switches chain and square lattices
+to the ``(jxy, jxy, jz)`` rows of ``EasyStart_DMRG.py:142-150``.
 """
 import os
 from math import comb
@@ -253,10 +255,25 @@ def generate_parameters_dmrg(lattice='chain'):
     return para
 
 
-def _heisenberg_coeffs(jxy, jz, is_pauli=False):
+COEFF_CONVENTIONS = ('parameters', 'easystart')
+
+
+def _heisenberg_coeffs(jxy, jz, is_pauli=False, convention='parameters'):
+    """coeff2 rows ``(S+S-, S-S+, SzSz)`` per bond.
+
+    ``convention='parameters'`` (``library/Parameters.py:208-211``):
+    ``(jxy/2, jxy/2, jz)``, i.e. ``H = sum jxy (SxSx + SySy) + jz SzSz``;
+    ``is_pauli`` lattices use ``(2jxy, 2jxy, 4jz)`` (``Parameters.py:267-280``).
+    ``convention='easystart'`` (``EasyStart_DMRG.py:146-150``): ``(jxy, jxy, jz)``,
+    the convention of the reference's saved runs in ``data_dmrg``."""
     jxy = np.atleast_1d(np.asarray(jxy, dtype=float))
     jz = np.atleast_1d(np.asarray(jz, dtype=float))
-    if is_pauli:
+    if convention not in COEFF_CONVENTIONS:
+        raise ValueError('coeff_convention must be one of %s (got %r)'
+                         % (COEFF_CONVENTIONS, convention))
+    if convention == 'easystart':
+        trip = np.stack([jxy, jxy, jz], axis=1)
+    elif is_pauli:
         trip = np.stack([2 * jxy, 2 * jxy, 4 * jz], axis=1)
     else:
         trip = np.stack([jxy / 2, jxy / 2, jz], axis=1)
@@ -277,7 +294,8 @@ def make_consistent_parameter_dmrg(para):
         para['index1'], para['coeff1'] = _field_rows(para['l'])
         para['index2'] = interactions_position2full_index_heisenberg_two_body(para['positions_h2'])
         nb = para['positions_h2'].shape[0]
-        para['coeff2'] = _heisenberg_coeffs(np.full(nb, para['jxy']), np.full(nb, para['jz']))
+        para['coeff2'] = _heisenberg_coeffs(np.full(nb, para['jxy']), np.full(nb, para['jz']),
+                                            convention=para.get('coeff_convention', 'parameters'))
         para['data_exp'] = 'chainN%d_j(%g,%g)_h(%g,%g)_chi%d' % (
             para['l'], para['jxy'], para['jz'], para['hx'], para['hz'], para['chi']) + \
             para['bound_cond']
@@ -288,7 +306,8 @@ def make_consistent_parameter_dmrg(para):
         para['index1'], para['coeff1'] = _field_rows(para['l'])
         para['index2'] = interactions_position2full_index_heisenberg_two_body(para['positions_h2'])
         nb = para['positions_h2'].shape[0]
-        para['coeff2'] = _heisenberg_coeffs(np.full(nb, para['jxy']), np.full(nb, para['jz']))
+        para['coeff2'] = _heisenberg_coeffs(np.full(nb, para['jxy']), np.full(nb, para['jz']),
+                                            convention=para.get('coeff_convention', 'parameters'))
         para['data_exp'] = 'square(%d,%d)N%d_j(%g,%g)_h(%g,%g)_chi%d' % (
             para['square_width'], para['square_height'], para['l'], para['jxy'], para['jz'],
             para['hx'], para['hz'], para['chi']) + para['bound_cond']

@@ -36,7 +36,7 @@ import numpy as np
 
 class Request:
     __slots__ = ('rid', 'para', 'seed', 'arrival', 'admitted', 'done', 'energy', 'solve', 'error',
-                 'footprint', 'sweeps_left', 'prep')
+                 'footprint', 'sweeps_left', 'prep', 'warnings')
 
     def __init__(self, rid, para, seed, arrival):
         self.rid, self.para, self.seed, self.arrival = rid, para, seed, arrival
@@ -46,6 +46,7 @@ class Request:
         self.error = None
         self.footprint = 0
         self.prep = None   # future of the engine's host preparation
+        self.warnings = []  # non-fatal failures of the solve's steps (result still given)
 
 
 class SolveServer:
@@ -145,6 +146,7 @@ class SolveServer:
             for r in done:
                 if r.solve.failed is not None:
                     r.error = str(r.solve.failed)
+                r.warnings = list(getattr(r.solve, 'warnings', ()))
                 r.done = t
                 self.engine.remove_solve(r.solve)
                 r.solve = None
@@ -195,6 +197,7 @@ class SolveServer:
         span = (max(r.done for r in ok) - min(r.arrival for r in ok)) if ok else 0.0
         return {
             'completed': len(ok), 'failed': len(self.finished) - len(ok),
+            'with_warnings': sum(1 for r in ok if r.warnings),
             'solves_per_s': len(ok) / span if span > 0 else 0.0,
             'latency_p50_s': float(np.percentile(lat, 50)) if len(lat) else None,
             'latency_p99_s': float(np.percentile(lat, 99)) if len(lat) else None,

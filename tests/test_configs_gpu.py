@@ -1,0 +1,180 @@
+"""The benchmark configurations at their full sizes on the GPU.
+
+* C2: XXZ chain N=64, chi=128, 8 sweeps, 8 seeded solves in one batch: the
+  mixed-precision schedule (bf16 / f32 MFMA matvecs, last two sweeps f64)
+  against the plain f64 schedule per seed within 1e-9, and the f64 energy of
+  seed 0 against the f64 CPU oracle's run of the same seeded MPS
+  (tests/golden/c2_block_fp64.json, written by make_c2_golden.py).
+* C3: J1-J2 (J2 = 0.5) on a 6x6 open square lattice along the snake,
+  chi=512: mixed and f64 schedules each swept to convergence, within 1e-9.
+* C4: 16 requests drawn from the serving distribution (N 16..96, chi in
+  {32, 64, 128}, Jz in [0.5, 1.5], hz in [0, 0.5]): each request alone and
+  in the continuously batched stream give the same bits; each is within 1e-9
+  of its plain f64 solve.
+* C5: Heisenberg on an 8x8 open square lattice along the snake, chi=1024:
+  one solve of the f32 schedule (last sweep f64); one more f64 sweep moves
+  the energy by less than 1e-9; the MPS is in mixed canonical form; the
+  spectrum at the centre bond equals the singular values of the centre
+  tensor.
+"""
+import json
+import os
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), 'golden')
+
+
+def _chain_para(n, chi, sweeps, jz=1.0, hz=0.0):
+    from tnalg_amd import parameters as P
+    para = P.generate_parameters_dmrg('chain')
+    para.update(l=n, chi=chi, jxy=1.0, jz=jz, hz=hz, sweep_time=sweeps, eigs_tol=1e-10,
+                tau=1e-4, dt_ob=sweeps, break_tol=0.0)
+    return P.make_consistent_parameter_dmrg(para)
+
+
+def _batch(para, seeds, precision):
+    """Energies <H> of the seeded solves of ``para`` run in one batch."""
+    from tnalg_amd.dmrg import _schedule_from
+    from tnalg_amd.engine import Engine
+    eng = Engine(precision_schedule=_schedule_from(precision), lowp=precision != 'fp64')
+    solves = eng.add_solves([(para, s) for s in seeds])
+    for s in solves:
+        s.schedule_canonicalise()
+    eng.run_queues(solves)
+    for t in range(para['sweep_time']):
+        for s in solves:
+            s.schedule_sweep(t)
+        eng.run_queues(solves)
+    e = eng.finalize(solves)
+    for s in solves:
+        assert s.warnings == [], s.warnings
+    return np.array(e)
+
+
+def test_c2_mixed_vs_fp64_and_oracle(gpu):
+    para = _chain_para(64, 128, 8)
+    seeds = list(range(8))
+    print('case: C2 N=64 chi=128, 8 solves, mixed', flush=True)
+    em = _batch(para, seeds, 'mixed')
+    print('case: C2 N=64 chi=128, 8 solves, fp64', flush=True)
+    e64 = _batch(para, seeds, 'fp64')
+    rel = np.abs(em - e64) / np.abs(e64)
+    print('C2 max |dE/E| mixed vs fp64: %.2e' % rel.max(), flush=True)
+    assert rel.max() <= 1e-9, rel
+    # the f64 CPU oracle (ARPACK, block environments) from seed 0's MPS
+    with open(os.path.join(GOLD, 'c2_block_fp64.json')) as f:
+        g = json.load(f)['seed0']
+    assert abs(e64[0] - g['E_total']) <= 1e-9 * abs(g['E_total']), (e64[0], g['E_total'])
+
+
+def test_c3_snake_j1j2_converged(gpu):
+    from tnalg_amd import dmrg_finite_size
+    from tnalg_amd import parameters as P
+    para = P.generate_parameters_dmrg('snakeJ1J2')
+    para.update(square_width=6, square_height=6, j1=1.0, j2=0.5, chi=512, sweep_time=24,
+                dt_ob=1, eigs_tol=1e-10, tau=1e-4, break_tol=1e-12)
+    para = P.make_consistent_parameter_dmrg(para)
+    assert para['index2'].shape[0] == 330
+    out = {}
+    for prec in ('fp64', 'fp32'):
+        print('case: C3 6x6 J1-J2 chi=512 %s to convergence' % prec, flush=True)
+        t0 = time.perf_counter()
+        ob, A, info, _ = dmrg_finite_size(dict(para), seed=0, precision=prec)
+        out[prec] = float(ob['e_per_site'][0]) * 36
+        print('  E = %.12f after %d sweeps (%.1f s)' % (out[prec], info['sweeps'],
+                                                        time.perf_counter() - t0), flush=True)
+        assert info['sweeps'] < 24, 'not converged in 24 sweeps'
+        assert info['warnings'] == []
+    rel = abs(out['fp32'] - out['fp64']) / abs(out['fp64'])
+    assert rel <= 1e-9, (out, rel)
+
+
+def test_c4_requests_alone_equal_in_mix_and_fp64(gpu):
+    from tnalg_amd.dmrg import _schedule_from
+    from tnalg_amd.engine import Engine
+    from tnalg_amd.serving import SolveServer, poisson_xxz_stream
+    reqs = poisson_xxz_stream(16, rate=50.0, seed=11)
+    ns = [p['l'] for _, p, _ in reqs]
+    chis = {p['chi'] for _, p, _ in reqs}
+    assert min(ns) >= 16 and max(ns) <= 96 and chis <= {32, 64, 128} and len(chis) > 1
+    assert all(p['hz'] > 0 for _, p, _ in reqs)
+    sched = _schedule_from('mixed')
+    print('case: C4 stream of 16 requests (mixed, continuous batching)', flush=True)
+    eng = Engine(precision_schedule=sched, lowp=True, cycle_steps=True, lanczos_streams=1)
+    srv = SolveServer(eng, hbm_budget=64 << 30, max_active=64)
+    t0 = time.perf_counter()
+    for k, (t, para, seed) in enumerate(reqs):
+        srv.submit(k, para, seed=seed, arrival=t0 + t)
+    summ = srv.run()
+    assert summ['completed'] == 16 and summ['failed'] == 0
+    assert summ['peak_active'] > 1
+    mixed = {r.rid: r.energy for r in srv.finished}
+    for r in srv.finished:
+        assert r.warnings == [], (r.rid, r.warnings)
+    eng1 = Engine(precision_schedule=sched, lowp=True, plan_ahead=False, lanczos_streams=1)
+    for k, (t, para, seed) in enumerate(reqs):
+        print('case: C4 request %d alone: N=%d chi=%d' % (k, para['l'], para['chi']), flush=True)
+        one = SolveServer(eng1, hbm_budget=64 << 30, max_active=1)
+        one.submit(k, para, seed=seed, arrival=0.0)
+        one.run()
+        assert one.finished[0].energy == mixed[k], (k, one.finished[0].energy, mixed[k])
+    print('case: C4 requests in plain f64', flush=True)
+    eng64 = Engine(precision_schedule=_schedule_from('fp64'), lowp=False)
+    srv64 = SolveServer(eng64, hbm_budget=64 << 30, max_active=64)
+    for k, (t, para, seed) in enumerate(reqs):
+        srv64.submit(k, para, seed=seed, arrival=0.0)
+    srv64.run()
+    for r in srv64.finished:
+        rel = abs(mixed[r.rid] - r.energy) / abs(r.energy)
+        assert rel <= 1e-9, (r.rid, mixed[r.rid], r.energy, rel)
+
+
+def test_c5_large_single_solve(gpu):
+    from tnalg_amd import dmrg_finite_size
+    from tnalg_amd import parameters as P
+    from tnalg_amd.dmrg import observe_energy
+    para = P.generate_parameters_dmrg('snakeJ1J2')
+    para.update(square_width=8, square_height=8, j1=1.0, j2=0.0, chi=1024, sweep_time=12,
+                dt_ob=12, eigs_tol=1e-10, tau=1e-4, break_tol=0.0)
+    para = P.make_consistent_parameter_dmrg(para)
+    print('case: C5 8x8 Heisenberg chi=1024 fp32 schedule, 12 sweeps', flush=True)
+    t0 = time.perf_counter()
+    ob, A, info, _ = dmrg_finite_size(para, seed=0, precision='fp32')
+    e = float(ob['e_per_site'][0]) * 64
+    print('  E = %.12f (%.1f s)' % (e, time.perf_counter() - t0), flush=True)
+    assert info['warnings'] == []
+    assert max(A.virtual_dim) == 1024
+    # the mixed canonical form about the centre
+    c = A.center
+    for p in (0, 1, 30, c - 1, c + 1, 62, 63):
+        if not 0 <= p < 64 or p == c:
+            continue
+        a = A.mps[p]
+        dl, d, dr = a.shape
+        if p < c:
+            m = a.reshape(dl * d, dr)
+            assert np.abs(m.T @ m - np.eye(dr)).max() < 1e-10, p
+        else:
+            m = a.reshape(dl, d * dr)
+            assert np.abs(m @ m.T - np.eye(dl)).max() < 1e-10, p
+    # the spectrum of the bond right of the centre: singular values of the
+    # centre tensor as a (Dl d) x Dr matrix
+    a = A.mps[c]
+    sv = np.linalg.svd(a.reshape(a.shape[0] * a.shape[1], -1), compute_uv=False)
+    lm = np.asarray(A.lm[c])
+    assert len(lm) == len(sv)
+    assert np.abs(lm - sv).max() < 1e-10
+    # one more f64 sweep from the result leaves the energy within 1e-9
+    s, eng = A._device(), A._engine
+    s.schedule_sweep(para['sweep_time'])
+    eng.precision_schedule = lambda sweep, nsweeps: 'fp64'
+    eng.run_queues([s])
+    e2 = float(observe_energy(eng, s, para)['e_per_site'][0]) * 64
+    print('  one more f64 sweep: E = %.12f, |dE/E| = %.2e' % (e2, abs(e2 - e) / abs(e)),
+          flush=True)
+    assert abs(e2 - e) <= 1e-9 * abs(e)

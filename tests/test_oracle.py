@@ -163,3 +163,138 @@ def test_dense_heff_brute_force():
     P = np.kron(np.kron(PL, np.eye(2)), PR)
     H = ed.hamiltonian(6, c['op'], c['index1'], c['coeff1'], c['index2'], c['coeff2']).toarray()
     assert np.allclose(P.T @ H @ P, h, atol=1e-12)
+
+
+# ---- the oracle's observables against brute force on the dense state
+# (MPSClass.py:1105-1212).  A random MPS (not an eigenstate, with a field so
+# that <Sx>, <Sz> and <Sy Sy> are all non-trivial) is brought to mixed
+# canonical form at several centres; every observation routine must equal the
+# direct sum over the 2^N amplitudes.
+def _random_canonical(n, chi, center, seed):
+    from oracle.mps_ref import RefMps
+    c = model.chain_couplings(n, 1.0, 1.0, hx=0.3, hz=0.2)
+    A = RefMps(n, 2, chi, c['op'], np.random.RandomState(seed))
+    A.correct_orthogonal_center(center)
+    A.mps[center] /= np.linalg.norm(A.mps[center])
+    return A, c
+
+
+@pytest.mark.parametrize('center', [0, 3, 7])
+def test_oracle_observation_s1_brute_force(center):
+    A, c = _random_canonical(8, 6, center, seed=11 + center)
+    psi = A.full_state()
+    for sn in (1, 3, 6):
+        got = A.observe_magnetization(sn).reshape(-1)
+        want = [ed.expect_one(psi, c['op'][sn], i, 8) for i in range(8)]
+        assert np.allclose(got, want, atol=1e-12), (sn, got, want)
+    # the values are not trivially zero
+    assert np.abs(A.observe_magnetization(1)).max() > 1e-3
+
+
+@pytest.mark.parametrize('center', [0, 4, 7])
+def test_oracle_observation_s1_s2_brute_force(center):
+    A, c = _random_canonical(8, 6, center, seed=5 + center)
+    psi = A.full_state()
+    for (a, b) in ((1, 1), (3, 3), (4, 5), (5, 4), (1, 3), (3, 1)):
+        for (i, j) in ((0, 1), (2, 5), (6, 7), (0, 7), (5, 2)):
+            got = A.observation_s1_s2((a, b), (i, j)).real
+            # observation_s1_s2 swaps the operators with the positions
+            # (MPSClass.py:1126-1129), so the value is <op_a(i) op_b(j)>
+            want = ed.expect_two(psi, c['op'][a], i, c['op'][b], j, 8)
+            assert abs(got - want) < 1e-12, (a, b, i, j, got, want)
+
+
+def test_oracle_correlators_from_middle_brute_force():
+    A, c = _random_canonical(9, 8, 4, seed=3)
+    psi = A.full_state()
+    # site pairs of MPSClass.py:1200-1211 for N=9: the middle is
+    # int(round(4.5)) = 4 (round half to even), then pos1 and pos2 step
+    # outwards in turn until one leaves the chain
+    pairs = [(4, 5), (3, 5), (3, 6), (2, 6), (2, 7), (1, 7), (1, 8), (0, 8)]
+    for op in (1, 3):
+        got = A.observe_correlators_from_middle(op, op)
+        want = [ed.expect_two(psi, c['op'][op], i, c['op'][op], j, 9) for i, j in pairs]
+        assert len(got) == len(pairs)
+        assert np.allclose(got, want, atol=1e-12)
+
+
+def test_oracle_bond_energy_from_jxyz_brute_force():
+    """jx<SxSx> + jy<SySy> + jz<SzSz> with the complex Sy (MPSClass.py:1184-1194)
+    for every pair: the C1 all-pairs <S_i.S_j> output."""
+    A, c = _random_canonical(8, 6, 2, seed=21)
+    psi = A.full_state()
+    from tnalg_amd.parameters import positions_fully_connected
+    pos = positions_fully_connected(8)
+    assert pos.shape == (28, 2)
+    got = A.observe_bond_energy_from_jxyz(pos, 1.0, 1.0, 1.0).reshape(-1)
+    want = [ed.spin_dot(psi, int(i), int(j), 8, c['op']) for i, j in pos]
+    assert np.allclose(got, want, atol=1e-12)
+    # <SySy> alone is non-zero (a dropped imaginary unit would flip its sign)
+    gy = A.observe_bond_energy_from_jxyz(pos, 0.0, 1.0, 0.0).reshape(-1)
+    wy = [ed.expect_two(psi, c['op'][2], int(i), c['op'][2], int(j), 8) for i, j in pos]
+    assert np.allclose(gy, wy, atol=1e-12) and np.abs(wy).max() > 1e-3
+    # jx, jy, jz weights enter separately
+    g3 = A.observe_bond_energy_from_jxyz(pos, 0.5, -1.0, 2.0).reshape(-1)
+    w3 = [0.5 * ed.expect_two(psi, c['op'][1], int(i), c['op'][1], int(j), 8)
+          - ed.expect_two(psi, c['op'][2], int(i), c['op'][2], int(j), 8)
+          + 2.0 * ed.expect_two(psi, c['op'][3], int(i), c['op'][3], int(j), 8) for i, j in pos]
+    assert np.allclose(g3, w3, atol=1e-12)
+
+
+def test_brute_force_expectations_closed_forms():
+    """The brute-force helpers themselves: a product state and a singlet."""
+    ops = model.operator_list()
+    up = np.array([1.0, 0.0])
+    dn = np.array([0.0, 1.0])
+    psi = np.kron(np.kron(up, dn), up)   # |up, down, up>
+    assert ed.expect_one(psi, ops[3], 0, 3) == 0.5
+    assert ed.expect_one(psi, ops[3], 1, 3) == -0.5
+    assert ed.expect_two(psi, ops[3], 0, ops[3], 1, 3) == -0.25
+    assert ed.expect_one(psi, ops[1], 2, 3) == 0.0
+    singlet = (np.kron(up, dn) - np.kron(dn, up)) / np.sqrt(2)
+    for k in (1, 2, 3):
+        assert abs(ed.expect_two(singlet, ops[k], 0, ops[k], 1, 2) + 0.25) < 1e-15
+    assert abs(ed.spin_dot(singlet, 0, 1, 2, ops) + 0.75) < 1e-15
+    # Sy Sy on |up,up>: (i/2)(i/2)<dn,dn|...> = 0, and on a triplet-0 state +1/4
+    t0 = (np.kron(up, dn) + np.kron(dn, up)) / np.sqrt(2)
+    assert abs(ed.expect_two(t0, ops[2], 0, ops[2], 1, 2) - 0.25) < 1e-15
+
+
+def test_oracle_exact_chi_observables_match_ed():
+    """chi >= 2^(N/2): the oracle's converged outputs are the ED ground
+    state's expectation values (mz, mx, eb_full, corr_x, corr_z, all pairs)."""
+    n = 8
+    c = model.chain_couplings(n, 1.0, 0.8, hx=0.15, hz=0.25)
+    p = dmrg_ref.make_para(c, 16, 8, eigs_tol=1e-12, break_tol=1e-14, dt_ob=1)
+    ob, A, info, _ = dmrg_ref.dmrg_finite_size(p, seed=4)
+    e0, psi = ed.ground_state(n, c['op'], c['index1'], c['coeff1'], c['index2'], c['coeff2'])
+    ops = c['op']
+    assert np.allclose(ob['mz'].reshape(-1), [ed.expect_one(psi, ops[3], i, n) for i in range(n)],
+                       atol=1e-8)
+    assert np.allclose(ob['mx'].reshape(-1), [ed.expect_one(psi, ops[1], i, n) for i in range(n)],
+                       atol=1e-8)
+    want_eb = [float(np.ravel(p['coeff2'])[r]) * ed.expect_two(
+        psi, ops[int(a)], int(i), ops[int(b)], int(j), n) for r, (i, j, a, b) in
+        enumerate(p['index2'])]
+    assert np.allclose(ob['eb_full'].reshape(-1), want_eb, atol=1e-8)
+    from tnalg_amd.dmrg import correlator_pairs
+    pairs = correlator_pairs(n)
+    for key, op in (('corr_x', 1), ('corr_z', 3)):
+        want = [ed.expect_two(psi, ops[op], i, ops[op], j, n) for i, j in pairs]
+        assert np.allclose(ob[key], want, atol=1e-8)
+
+
+def test_oracle_reproduces_saved_run_chi24():
+    """The reference's chi=24 saved run (data_dmrg/chainN12_j(1,1)_h(0,0)_chi24open.pr):
+    the same Hamiltonian at chi=24 reaches its energy to 1e-9 and its ent[5]
+    to 1e-8, 2.4e-9 above the exact ground state (a truncated state)."""
+    g = _gold('reference_runs.json')['chi24']
+    c = model.chain_couplings(12, 1.0, 1.0, convention='easystart')
+    p = dmrg_ref.make_para(c, 24, 20, tau=1e-3, eigs_tol=1e-10, break_tol=1e-13, dt_ob=2)
+    ob, A, info, _ = dmrg_ref.dmrg_finite_size(p, seed=1)
+    e = float(ob['e_per_site'][0]) * 12
+    assert abs(e - g['E_total']) < 1e-9
+    assert abs(A.ent[5, 0] - g['ent_5']) < 1e-8
+    e0, _ = ed.ground_state(12, c['op'], c['index1'], c['coeff1'], c['index2'], c['coeff2'])
+    assert 0 < e - e0 < 1e-8
+    assert A.virtual_dim == [1, 2, 4, 8, 16, 24, 24, 24, 16, 8, 4, 2, 1]

@@ -1,0 +1,152 @@
+"""Every output of the drop-in for the CPU-runnable configuration (C1: spin-1/2
+Heisenberg chain, N=12, open, Jx=Jy=Jz=1, h=0, chi=16, f64, one-site, 4
+sweeps, seeded random MPS), checked three ways:
+
+* against brute force on the GPU's own final state: the MPS is contracted to
+  its 2^12 amplitudes and every expectation value (<Sx_i>, <Sz_i>, the bond
+  rows of eb_full, corr_x / corr_z at the reference's pairs, all-pairs
+  <S_i.S_j> with the complex Sy, <Sy_i Sy_j>) and every Schmidt spectrum is
+  summed directly (oracle/ed.py), so the device observables are pinned
+  whatever state the sweeps reached;
+* against the CPU restatement of the reference run from the same seeded MPS
+  (oracle/dmrg_ref, ARPACK): same energy and outputs;
+* at chi=64 (exact for N=12) against the exact ground state.
+
+The reference routines are ``algorithms/DMRG_anyH.py:18-101`` and
+``library/MPSClass.py:1105-1212,965-992``.
+"""
+import numpy as np
+import pytest
+
+from oracle import dmrg_ref, ed, model
+from oracle import tensor_ops as tb
+
+pytestmark = pytest.mark.gpu
+
+N = 12
+
+
+def _c1_para(chi=16, sweeps=4, hx=0.0, hz=0.0):
+    from tnalg_amd import parameters as P
+    para = P.generate_parameters_dmrg('chain')
+    para.update(l=N, jxy=1.0, jz=1.0, hx=hx, hz=hz, chi=chi, sweep_time=sweeps, dt_ob=1,
+                eigs_tol=1e-10, tau=1e-4, break_tol=0.0)
+    return P.make_consistent_parameter_dmrg(para)
+
+
+def _state(A):
+    psi = A.full_coefficients()
+    return psi / np.linalg.norm(psi)
+
+
+def _check_all_outputs(ob, A, para, psi, tol):
+    """Every output of dmrg_finite_size and the observation methods against
+    brute force on ``psi``."""
+    from tnalg_amd.dmrg import all_pair_spin_correlations, correlator_pairs
+    ops = para['op']
+    mz = [ed.expect_one(psi, ops[3], i, N) for i in range(N)]
+    mx = [ed.expect_one(psi, ops[1], i, N) for i in range(N)]
+    assert np.allclose(ob['mz'].reshape(-1), mz, atol=tol)
+    assert np.allclose(ob['mx'].reshape(-1), mx, atol=tol)
+    eb_full = [float(np.ravel(para['coeff2'])[r]) * ed.expect_two(
+        psi, ops[int(a)], int(i), ops[int(b)], int(j), N)
+        for r, (i, j, a, b) in enumerate(para['index2'])]
+    assert np.allclose(ob['eb_full'].reshape(-1), eb_full, atol=tol)
+    assert ob['eb_full'].shape == (para['index2'].shape[0], 1)
+    e = (sum(eb_full) - para['hx'] * sum(mx) - para['hz'] * sum(mz)) / N
+    assert abs(float(np.ravel(ob['e_per_site'])[0]) - e) < tol
+    assert np.allclose(ob['eb'].reshape(-1), np.reshape(eb_full, (-1, 3)).sum(axis=1), atol=tol)
+    pairs = correlator_pairs(N)
+    assert pairs[:3] == [(6, 7), (5, 7), (5, 8)]
+    for key, op in (('corr_x', 1), ('corr_z', 3)):
+        want = [ed.expect_two(psi, ops[op], i, ops[op], j, N) for i, j in pairs]
+        assert np.allclose(ob[key], want, atol=tol), key
+    # all-pairs <S_i . S_j> (observe_bond_energy_from_jxyz over every pair)
+    allp = all_pair_spin_correlations(A)
+    for i in range(N):
+        for j in range(i + 1, N):
+            assert abs(allp[i, j] - ed.spin_dot(psi, i, j, N, ops)) < tol, (i, j)
+            assert allp[j, i] == allp[i, j]
+    # <Sy_i Sy_j> through observation_s1_s2 (Sy is imaginary: measured as the
+    # real rows -(S+ - S-)(S+ - S-)/4)
+    for (i, j) in ((0, 1), (3, 9), (11, 2)):
+        want = ed.expect_two(psi, ops[2], i, ops[2], j, N)
+        assert abs(A.observation_s1_s2(([2, 2], [i, j])) - want) < tol, (i, j)
+        assert abs(want) > 1e-3
+    # Schmidt spectra and entropies at every bond
+    for b in range(N - 1):
+        sv = np.linalg.svd(psi.reshape(2 ** (b + 1), -1), compute_uv=False)
+        lm = np.asarray(A.lm[b])
+        assert len(lm) == min(para['chi'], 2 ** (b + 1), 2 ** (N - b - 1))
+        assert np.allclose(lm, sv[:len(lm)], atol=1e-10), b
+        assert abs(A.ent[b, 0] - tb.entanglement_entropy(sv[:len(lm)])) < 1e-9
+    # gauge: mixed canonical about the centre
+    c = A.center
+    for p, a in enumerate(A.mps):
+        dl, d, dr = a.shape
+        if p < c:
+            m = a.reshape(dl * d, dr)
+            assert np.allclose(m.T @ m, np.eye(dr), atol=1e-10), p
+        elif p > c:
+            m = a.reshape(dl, d * dr)
+            assert np.allclose(m @ m.T, np.eye(dl), atol=1e-10), p
+
+
+def test_c1_outputs_brute_force_and_oracle(gpu):
+    from tnalg_amd import dmrg_finite_size
+    para = _c1_para()
+    print('case: C1 N=12 chi=16 h=0 4 sweeps seed 0', flush=True)
+    ob, A, info, _ = dmrg_finite_size(para, seed=0)
+    psi = _state(A)
+    _check_all_outputs(ob, A, para, psi, tol=1e-10)
+    # the oracle's run of the same seeded MPS (ARPACK on I - tau Heff)
+    ref = dmrg_ref.make_para(model.chain_couplings(N, 1.0, 1.0), 16, 4, tau=1e-4,
+                             eigs_tol=1e-10, break_tol=0.0, dt_ob=1)
+    rob, rA, rinfo, _ = dmrg_ref.dmrg_finite_size(ref, seed=0)
+    e, er = float(ob['e_per_site'][0]) * N, float(rob['e_per_site'][0]) * N
+    assert abs(e - er) < 1e-9 * abs(er), (e, er)
+    # each sweep's energy (dt_ob = 1) along the way agrees as well once the
+    # first sweep is done
+    assert np.allclose(info['energies'][1:], rinfo['energies'][1:], rtol=1e-9, atol=0)
+    for key in ('mz', 'mx', 'eb_full', 'eb', 'corr_x', 'corr_z'):
+        assert np.allclose(np.ravel(ob[key]), np.ravel(rob[key]), atol=1e-7), key
+    for b in range(N - 1):
+        assert np.allclose(A.lm[b], rA.lm[b], atol=1e-6), b
+    assert np.allclose(A.ent, rA.ent, atol=1e-6)
+    # chi=16 is a truncation: within 1e-7 of the exact energy, above it
+    e0, _ = ed.ground_state(N, para['op'], para['index1'], para['coeff1'], para['index2'],
+                            para['coeff2'])
+    assert abs(e0 - (-5.142090632840530)) < 1e-11
+    assert 0 < e - e0 < 1e-7
+    assert info['warnings'] == []
+
+
+def test_c1_exact_chi_matches_ed(gpu):
+    """chi = 64 holds the exact ground state of N=12: every output equals the
+    exact diagonalisation's."""
+    from tnalg_amd import dmrg_finite_size
+    para = _c1_para(chi=64, sweeps=4, hx=0.1, hz=0.2)
+    print('case: C1 N=12 chi=64 hx=0.1 hz=0.2', flush=True)
+    ob, A, info, _ = dmrg_finite_size(para, seed=1)
+    e0, psi0 = ed.ground_state(N, para['op'], para['index1'], para['coeff1'], para['index2'],
+                               para['coeff2'])
+    e = float(ob['e_per_site'][0]) * N
+    assert abs(e - e0) < 1e-10 * abs(e0)
+    psi = _state(A)
+    assert abs(abs(psi @ psi0) - 1) < 1e-9
+    _check_all_outputs(ob, A, para, psi0 / np.linalg.norm(psi0), tol=1e-8)
+
+
+def test_initial_mps_on_device_is_the_reference_draw(gpu):
+    """The device copy of the seeded initial MPS equals the reference draw
+    order (TensorBasicModule.py:179-211) of the same RandomState, bit for bit."""
+    from tnalg_amd.engine import Engine
+    para = _c1_para()
+    eng = Engine(lowp=False, plan_ahead=False)
+    s = eng.add_solve(para, seed=7)
+    got = s.host_mps()
+    want, dims = tb.random_open_mps(N, 2, 16, np.random.RandomState(7))
+    for a, b in zip(got, want):
+        assert a.shape == b.shape and np.array_equal(a, b)
+    assert list(s.dims) == dims
+    eng.remove_solve(s)

@@ -296,6 +296,7 @@ def test_plan_ahead_matches_inline_planning():
     class AheadEngine(CpuEngine):
         _plan_ahead = Engine._plan_ahead
         _prec_of = Engine._prec_of
+        sweep_precision = Engine.sweep_precision
 
         def __init__(self):
             super().__init__(lowp=True)

@@ -17,11 +17,13 @@ class Clock:
 
 
 class FakeSolve:
-    def __init__(self, para, fail_at=None):
+    def __init__(self, para, fail_at=None, warn_at=None):
         self.para = para
         self.queue = []
         self.failed = None
+        self.warnings = []
         self.fail_at = fail_at
+        self.warn_at = warn_at
         self.n = para['l']
 
     def schedule_canonicalise(self):
@@ -46,7 +48,7 @@ class FakeEngine:
     def add_solve(self, para, seed=0, tag=None):
         if para.get('bad'):
             raise ValueError('bad request')
-        s = FakeSolve(para, fail_at=para.get('fail_at'))
+        s = FakeSolve(para, fail_at=para.get('fail_at'), warn_at=para.get('warn_at'))
         self.live.add(s)
         self.bytes += self.footprint(para)
         self.max_live_bytes = max(self.max_live_bytes, self.bytes)
@@ -63,6 +65,8 @@ class FakeEngine:
                 s.queue.pop(0)
                 if s.fail_at is not None and len(s.queue) == s.fail_at:
                     s.failed = RuntimeError('injected device failure')
+                if s.warn_at is not None and len(s.queue) == s.warn_at:
+                    s.warnings.append('block Jacobi SVD reached 60 sweeps')
 
     def finalize(self, solves):
         return [-0.44 * s.n for s in solves]
@@ -207,3 +211,20 @@ def test_prepared_stages_reach_admission():
     assert len(srv.finished) == 4 and all(r.error is None for r in srv.finished)
     assert eng.stages[1] == ('stage', 9, 11) and eng.stages[2] == ('stage', 10, 12)
     assert eng.stages[3] is None
+
+
+def test_nonfatal_failures_reach_the_request():
+    """A solve whose step hit an iteration limit (its result is still
+    written) finishes with an energy and carries the warning; the others
+    carry none."""
+    clk = Clock()
+    eng = FakeEngine(clk)
+    srv = SolveServer(eng, hbm_budget=10 ** 9, max_active=4, clock=clk)
+    srv.submit(0, _para(10, 16), arrival=0.0)
+    srv.submit(1, _para(10, 16, warn_at=7), arrival=0.0)
+    summ = srv.run()
+    by = {r.rid: r for r in srv.finished}
+    assert summ['completed'] == 2 and summ['failed'] == 0 and summ['with_warnings'] == 1
+    assert by[0].warnings == [] and by[0].energy is not None
+    assert by[1].warnings == ['block Jacobi SVD reached 60 sweeps']
+    assert by[1].error is None and by[1].energy == -0.44 * 10

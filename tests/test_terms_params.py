@@ -148,3 +148,55 @@ def test_library_abi_struct_sizes():
     sizes = (ctypes.c_int32 * 16)()
     n = lib.tnx_abi_sizes(sizes, 16)
     assert list(sizes[:n]) == [48, 56, 48, 24, 88, 88, 24, 56, 64, 32, 56, 64]
+
+
+def test_easystart_coefficient_convention():
+    """coeff_convention='easystart' gives the (jxy, jxy, jz) rows of
+    EasyStart_DMRG.py:142-150 for chains and square lattices; the default
+    stays Parameters.py's (jxy/2, jxy/2, jz)."""
+    para = P.generate_parameters_dmrg('chain')
+    para.update(l=12, jxy=1.0, jz=1.0, coeff_convention='easystart')
+    para = P.make_consistent_parameter_dmrg(para)
+    c = model.chain_couplings(12, 1.0, 1.0, convention='easystart')
+    assert np.array_equal(para['index2'], c['index2'])
+    assert np.array_equal(para['coeff2'], c['coeff2'])
+    assert np.all(para['coeff2'] == 1.0) and para['coeff2'].shape == (33, 1)
+    # the reference's saved runs are this Hamiltonian: its exact ground energy
+    from oracle import ed
+    e0, _ = ed.ground_state(12, para['op'], para['index1'], para['coeff1'], para['index2'],
+                            para['coeff2'])
+    assert abs(e0 - (-8.676296661854156)) < 1e-10
+    sq = P.generate_parameters_dmrg('square')
+    sq.update(jxy=0.6, jz=1.3, coeff_convention='easystart')
+    sq = P.make_consistent_parameter_dmrg(sq)
+    assert np.allclose(sq['coeff2'].reshape(-1, 3), [0.6, 0.6, 1.3])
+    sq['coeff_convention'] = 'parameters'
+    sq = P.make_consistent_parameter_dmrg(sq)
+    assert np.allclose(sq['coeff2'].reshape(-1, 3), [0.3, 0.3, 1.3])
+    bad = P.generate_parameters_dmrg('chain')
+    bad['coeff_convention'] = 'halved'
+    with pytest.raises(ValueError):
+        P.make_consistent_parameter_dmrg(bad)
+
+
+def test_initial_mps_is_the_reference_draw():
+    """The engine's seeded initial MPS is the reference's draw order
+    (TensorBasicModule.py:179-211): first tensor, last tensor, then the bulk,
+    standard normals from one RandomState."""
+    from oracle import tensor_ops as tb
+    from tnalg_amd.engine import random_mps_host
+    for n, chi, seed in ((12, 16, 0), (7, 5, 3), (2, 4, 9)):
+        got = random_mps_host(n, 2, chi, seed)
+        want, dims = tb.random_open_mps(n, 2, chi, np.random.RandomState(seed))
+        assert len(got) == n
+        for a, b in zip(got, want):
+            assert a.shape == b.shape and np.array_equal(a, b)
+        assert [a.shape[0] for a in got] + [1] == dims
+    # a RandomState passed in continues its stream (dmrg_finite_size(seed=None)
+    # draws from numpy.random's global state)
+    r1, r2 = np.random.RandomState(5), np.random.RandomState(5)
+    r1.randn(3)
+    r2.randn(3)
+    a = random_mps_host(6, 2, 4, r1)
+    b, _ = tb.random_open_mps(6, 2, 4, r2)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
