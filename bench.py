@@ -73,14 +73,15 @@ def _release_engine(precision, ncv=20):
     torch.cuda.empty_cache()
 
 
-def _solve_batch(eng, para, seeds):
+def _solve_batch(eng, para, seeds, keep_mps=False):
     """Every chain of ``seeds`` from its seeded random MPS through all sweeps
     in lockstep on ``eng`` (on the calling thread's current stream), then
     every output ``dmrg_finite_size`` gives (``DMRG_anyH.py:66-70,91-95``:
     eb_full, mx, mz, e_per_site, eb, the entanglement spectrum of every bond
     by SVD centre moves, the entropies, corr_x and corr_z); the solves are
     released.  Returns one dict per solve (the ob keys plus ``E``, ``lm``,
-    ``ent``)."""
+    ``ent``; with ``keep_mps`` also ``mps``, the final host tensors, whose
+    copy time ``t_keep`` the caller leaves out of its timing)."""
     import torch
     from tnalg_amd.dmrg import finish_outputs
     solves = eng.add_solves([(para, s) for s in seeds])
@@ -97,12 +98,16 @@ def _solve_batch(eng, para, seeds):
         ob['E'] = float(np.ravel(ob['e_per_site'])[0]) * s.n
         ob['lm'], ob['ent'] = lm, ent
         ob['warnings'] = list(s.warnings)
+        if keep_mps:
+            t0 = time.perf_counter()
+            ob['mps'] = s.host_mps()
+            ob['t_keep'] = time.perf_counter() - t0
     for s in list(solves):
         eng.remove_solve(s)
     return obs
 
 
-def run_batch(para, seeds, precision, ncv=20):
+def run_batch(para, seeds, precision, ncv=20, keep_mps=False):
     """One batched solve: every chain of ``seeds`` from its seeded random MPS
     through all sweeps in lockstep, then all its outputs (``_solve_batch``);
     returns ``(seconds, outputs, engine counters, engine)``.  With ``--pipelines
@@ -117,9 +122,9 @@ def run_batch(para, seeds, precision, ncv=20):
         eng.counters[k] = 0.0 if isinstance(eng.counters[k], float) else 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    obs = _solve_batch(eng, para, seeds)
+    obs = _solve_batch(eng, para, seeds, keep_mps)
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    dt = time.perf_counter() - t0 - sum(o.get('t_keep', 0.0) for o in obs)
     return dt, obs, dict(eng.counters), eng
 
 
@@ -433,31 +438,143 @@ def large_solve(name, fp64_check=True):
     return res
 
 
-def cpu_baseline(seconds=15.0):
-    """The oracle's restatement of the reference algorithm (per-term effective
-    operators, opt_env matvec, ARPACK eigsh ncv=20) on the C2 chain, timed on
-    a bounded window of local steps and extrapolated to one full solve."""
-    from oracle import dmrg_ref, model
+def host_description():
+    """CPU model, logical cores, and the numpy / scipy / BLAS builds the CPU
+    baseline runs on."""
+    import platform
+    import scipy
     from threadpoolctl import threadpool_info
-    c = model.chain_couplings(64, 1.0, 1.0)
-    p = dmrg_ref.make_para(c, 128, 8, tau=1e-4, eigs_tol=1e-10, break_tol=0.0, dt_ob=8)
-    # the first local steps of sweep 1 (from the random MPS); a window of
-    # `steps` is about 15 s on an 8-core Xeon.  Later steps cost more (the
-    # reference re-propagates two-body operators over O(p) sites), so the
-    # extrapolation overstates the CPU's solves/s.
-    steps = max(4, int(seconds / 2.0))
-    t0 = time.perf_counter()
-    dmrg_ref.dmrg_finite_size(p, seed=0, max_steps=steps)
-    dt = time.perf_counter() - t0
-    per_step = dt / steps
-    total_steps = 8 * (2 * 64 - 2)
+    model = platform.processor() or 'unknown'
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    blas = ['%s %s (%s, %d threads)' % (i.get('internal_api'), i.get('version'),
+                                        i.get('threading_layer', '?'), i.get('num_threads', 0))
+            for i in threadpool_info() if i.get('user_api') == 'blas']
+    return {'cpu_model': model, 'logical_cpus': os.cpu_count(), 'numpy': np.__version__,
+            'scipy': scipy.__version__, 'blas': blas}
+
+
+# sites whose local step is timed (chain of 64; the mirror image of a right
+# move at site p is a left move at site 63 - p, so a sweep's cost is twice the
+# sum over the sites of a right move)
+_CPU_SITES = (3, 17, 31, 45, 59)
+
+
+def _window_times(make, para, sites, warm=1, timed=1):
+    """Seconds per local step at each of ``sites``: a fresh solver from
+    ``make()`` (with its operator caches empty) is moved to ``p - warm``, runs
+    ``warm`` untimed right-moving steps that fill its caches as a sweep
+    would, then ``timed`` timed steps from site ``p``.  Returns
+    ``(per-site seconds, ARPACK matvecs of the timed steps, seconds spent)``."""
+    out, mv, t_all = [], [], time.perf_counter()
+    for p in sites:
+        step, counter = make(p - warm)
+        for q in range(p - warm, p):
+            step(q)
+        m0 = counter()
+        t0 = time.perf_counter()
+        for q in range(p, p + timed):
+            step(q)
+        out.append((time.perf_counter() - t0) / timed)
+        mv.append((counter() - m0) / timed)
+    return out, mv, time.perf_counter() - t_all
+
+
+def _sweep_seconds(sites, per_step, n):
+    """One sweep's time from per-site samples (linear interpolation over the
+    sites of a right move, then twice the sum: the left moves mirror it)."""
+    xs = np.arange(1, n)
+    return 2.0 * float(np.interp(xs, sites, per_step).sum())
+
+
+def cpu_baseline(seconds=15.0, converged=None, n=64, chi=128, sweeps=8):
+    """The oracle's restatement of the reference algorithm (per-term effective
+    operators re-propagated as ``MPSClass.py:585-611`` does, ``opt_env``
+    matvec, ARPACK ``eigsh`` ncv=20, QR gauge moves) on the C2 chain, timed
+    on local steps sampled across the chain and extrapolated to one full
+    solve; the block-environment restatement (``oracle/block_dmrg``, O(1)
+    transfers per step) is timed the same way as a second number.
+
+    Two states are sampled: the seeded random MPS (the first sweep's local
+    problems) and, when given, ``converged`` (an MPS of the same chain after
+    all sweeps; the later sweeps' problems, which ARPACK finishes in fewer
+    matvecs).  The sweep-1 samples price sweep 1, the converged samples
+    sweeps 2..8.  ``seconds`` only scales the number of sampled sites."""
+    from oracle import dmrg_ref, model
+    from oracle.block_dmrg import BlockDmrg
+    from oracle.mps_ref import RefMps
+    c = model.chain_couplings(n, 1.0, 1.0)
+    p = dmrg_ref.make_para(c, chi, sweeps, tau=1e-4, eigs_tol=1e-10, break_tol=0.0, dt_ob=sweeps)
+    nsites = max(3, min(len(_CPU_SITES), int(seconds / 5.0)))
+    sites = [int(round(x)) for x in np.linspace(_CPU_SITES[0], _CPU_SITES[-1], nsites)]
+    args = (p['index1'], p['index2'], p['coeff1'], p['coeff2'], p['tau'], True, p['eigs_tol'])
+
+    def make_ref(state):
+        def make(c0):
+            A = RefMps(n, 2, chi, p['op'], np.random.RandomState(0), way='qr', is_save_op=True,
+                       mps=state)
+            A.correct_orthogonal_center(c0)
+            return (lambda q: A.update_tensor_eigs(q, *args)), (lambda: A.n_matvec)
+        return make
+
+    def make_block(state):
+        def make(c0):
+            eng = BlockDmrg(p, np.random.RandomState(0))
+            if state is not None:
+                eng.A = RefMps(n, 2, chi, p['op'], None, way='qr', is_save_op=False, mps=state)
+            eng.A.correct_orthogonal_center(c0)
+            eng.initial_environments()
+            return eng.solve_site, (lambda: eng.n_matvec)
+        return make
+
+    from oracle import tensor_ops as tb
+    random_state, _ = tb.random_open_mps(n, 2, chi, np.random.RandomState(0))
+    res = {}
+    spent = 0.0
+    for kind, make in (('reference', make_ref), ('block', make_block)):
+        rows = {}
+        for label, state in (('sweep1', random_state), ('converged', converged)):
+            if state is None:
+                continue
+            use = sites if kind == 'reference' else [sites[len(sites) // 2]]
+            t, mv, dt = _window_times(make(state), p, use)
+            spent += dt
+            if kind == 'block':   # O(1) transfers per step: one sampled site
+                use, t, mv = [1, n - 1], [t[0], t[0]], [mv[0], mv[0]]
+            rows[label] = {'sites': use, 's_per_step': t, 'matvecs_per_step': mv,
+                           'sweep_s': _sweep_seconds(use, t, n)}
+        later = rows.get('converged', rows['sweep1'])['sweep_s']
+        solve_s = rows['sweep1']['sweep_s'] + (sweeps - 1) * later
+        res[kind] = {'solve_s_extrapolated': solve_s, 'solves_per_s': 1.0 / solve_s,
+                     'samples': rows}
+    host = host_description()
+    from threadpoolctl import threadpool_info
     threads = max([int(i.get('num_threads', 1)) for i in threadpool_info()] or [1])
-    return {'value': 1.0 / (per_step * total_steps), 'unit': 'solves/s', 'cores': threads,
-            'kind': 'port',
-            'sample': 'oracle/dmrg_ref (reference restatement, numpy/scipy f64, ARPACK ncv=20) on '
-                      'the N=64 chi=128 XXZ chain: first %d local steps of sweep 1 from a random '
-                      'MPS in %.1f s (%.3f s/step), extrapolated to 8 sweeps x 126 steps' %
-                      (steps, dt, per_step)}
+    ref = res['reference']
+    return {'value': ref['solves_per_s'], 'unit': 'solves/s', 'cores': threads, 'kind': 'port',
+            'sample': 'oracle/mps_ref + dmrg_ref (the reference restatement: per-term effective '
+                      'operators with its re-propagation, opt_env matvec, ARPACK eigsh ncv=20, '
+                      'QR moves; numpy/scipy f64) on the N=64 chi=128 XXZ chain: one timed '
+                      'local step (after one untimed step that fills the operator caches) at '
+                      'sites %s, from the seeded random MPS (prices sweep 1) and from %s '
+                      '(prices sweeps 2-%d); per-step times interpolated over the chain, '
+                      'doubled for the left moves: one solve = %.0f s extrapolated, '
+                      '%.1f s of sampling'
+                      % (sites, 'a converged MPS of the same chain (the f64 GPU solve of seed 0)'
+                         if converged is not None else 'the same random MPS', sweeps,
+                         ref['solve_s_extrapolated'], spent),
+            'block_env_variant': {'solves_per_s': res['block']['solves_per_s'],
+                                  'solve_s_extrapolated': res['block']['solve_s_extrapolated'],
+                                  'note': 'oracle/block_dmrg: the same local problems on block '
+                                          'environments (O(1) transfers per step)',
+                                  'samples': res['block']['samples']},
+            'reference_samples': ref['samples'], 'host': host}
 
 
 def main():
@@ -579,6 +696,7 @@ def main():
            'solves_with_warnings': sum(1 for o in obs if o['warnings'])}
     out['engine_counters'] = {k: (float(v) if isinstance(v, float) else v)
                               for k, v in counters.items()}
+    converged = None   # a converged host MPS of seed 0 (the CPU baseline's later sweeps)
     if not args.no_plain:
         # the f64 reference of every seed, batched on an engine warmed by a
         # batch of the same shape (other seeds), like the timed engine; and the
@@ -595,7 +713,8 @@ def main():
         out['max_abs_diff_vs_fp64'] = output_differences(obs, ob64)
         _release_engine('fp64')
         run_batch(para, [1000], 'fp64')     # warm the one-solve engine like the timed one
-        dt_p, ob_p, _, _ = run_batch(para, seeds[:1], 'fp64')
+        dt_p, ob_p, _, _ = run_batch(para, seeds[:1], 'fp64', keep_mps=True)
+        converged = ob_p[0]['mps']
         out['plain_fp64'] = {'solves_per_s': 1.0 / dt_p, 'latency_s': dt_p,
                              'energy': float(ob_p[0]['E'])}
         out['speedup_vs_plain_fp64'] = value / (1.0 / dt_p)
@@ -606,7 +725,8 @@ def main():
         out['serving'] = serving_run(args.serving_requests, args.serving_rate)
         print('serving: %s' % out['serving'], file=sys.stderr, flush=True)
     if not args.no_cpu:
-        out['cpu_baseline'] = cpu_baseline(args.cpu_seconds)
+        out['cpu_baseline'] = cpu_baseline(args.cpu_seconds, converged, args.n, args.chi,
+                                           args.sweeps)
     print(json.dumps(out))
 
 

@@ -632,28 +632,102 @@ extern "C" size_t tnx_lanczos_state_bytes(int nsolve, int nchunks, int ncv) {
 
 namespace {
 
-// The Lanczos loop of every entry point: each Krylov step applies the nplan
-// matvec plans in turn (one per precision of the batch), then the shared
-// orthogonalisation kernels over all solves.
+// Zeroes n 8-byte words (the solves' state at the start of a call; a kernel
+// rather than a memset node, so the launch sequence is plain kernels).
+__global__ void k_zero_words(uint64_t* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+int check_args(const char* who, const tnx_matvec_plan* plans, int nplan,
+               const tnx_lanczos_solve* solves, int nsolve, int nchunks, int ncv,
+               int max_restarts, int x_dtype, int w_dtype) {
+  if (plans == nullptr || nplan <= 0 || solves == nullptr || nsolve <= 0 || nchunks <= 0 ||
+      ncv < 2 || ncv > NCV_MAX || max_restarts < 0 || x_dtype < -1 || x_dtype > TNX_BF16 ||
+      w_dtype < -1 || w_dtype > TNX_BF16) {
+    tnx_set_error("%s: bad arguments (nplan=%d nsolve=%d nchunks=%d ncv=%d)", who, nplan, nsolve,
+                  nchunks, ncv);
+    return TNX_EINVAL;
+  }
+  return TNX_OK;
+}
+
+// The launches of a call before its first Krylov step: state zeroed, every
+// solve active, start vectors normalised into V[0] and x.
+int issue_start(const tnx_lanczos_solve* solves, int nsolve, void* state, int nchunks,
+                int x_dtype, hipStream_t s) {
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  const size_t words = sizeof(LzState) * (size_t)nsolve / sizeof(uint64_t);
+  static_assert(sizeof(LzState) % sizeof(uint64_t) == 0, "LzState is whole words");
+  hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((words + 255) / 256 < 1024 ? (words + 255) / 256 : 1024)),
+                     dim3(256), 0, s, (uint64_t*)L.st, words);
+  hipLaunchKernelGGL(k_set_nact, dim3(1), dim3(64), 0, s, state, nsolve, nchunks);
+  hipLaunchKernelGGL(k_init_norm, dim3(nchunks), dim3(NT), 0, s, solves, nsolve, state, nchunks);
+  hipLaunchKernelGGL(k_init_scale, dim3(nchunks), dim3(NT), 0, s, solves, nsolve, state, nchunks,
+                     x_dtype);
+  TNX_CHECK_LAUNCH();
+  return TNX_OK;
+}
+
+// Krylov step j of every active solve: each plan's matvec, then the dots,
+// the update with its dots and norm, and the normalisation with the
+// tridiagonal solve and the convergence test.
+int issue_step(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve* solves,
+               int nsolve, void* state, int nchunks, int j, int ncv, int min_iters, int x_dtype,
+               int w_dtype, tnx_stream_t stream) {
+  hipStream_t s = (hipStream_t)stream;
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  for (int p = 0; p < nplan; ++p)
+    if (int rc = tnx_heff_matvec(plans + p, L.act, stream)) return rc;
+  const dim3 grid(nchunks), block(NT);
+  hipLaunchKernelGGL(k_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
+  hipLaunchKernelGGL(k_update_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
+  hipLaunchKernelGGL(k_update_norm, dim3(nchunks + nsolve), block, 0, s, solves, nsolve, state,
+                     nchunks, j, x_dtype, (int)(j + 1 < ncv), min_iters);
+  TNX_CHECK_LAUNCH();
+  return TNX_OK;
+}
+
+// Restart of the still-active solves from their Ritz vector (kept in the
+// spare basis row ncv).
+int issue_restart(const tnx_lanczos_solve* solves, int nsolve, void* state, int nchunks, int ncv,
+                  int x_dtype, hipStream_t s) {
+  const dim3 grid(nchunks), block(NT);
+  hipLaunchKernelGGL(k_ritz, grid, block, 0, s, solves, nsolve, state, nchunks, ncv);
+  hipLaunchKernelGGL(k_restart, grid, block, 0, s, solves, nsolve, state, nchunks, ncv, x_dtype);
+  TNX_CHECK_LAUNCH();
+  return TNX_OK;
+}
+
+// Eigenvectors into every solve's `out`, results into `res` (device-visible).
+int issue_finish(const tnx_lanczos_solve* solves, int nsolve, void* state, int nchunks,
+                 tnx_lanczos_result* res, hipStream_t s) {
+  hipLaunchKernelGGL(k_final, dim3(nchunks), dim3(NT), 0, s, solves, nsolve, state, nchunks);
+  hipLaunchKernelGGL(k_results, dim3((nsolve + 63) / 64), dim3(64), 0, s, state, nsolve, nchunks,
+                     res);
+  TNX_CHECK_LAUNCH();
+  return TNX_OK;
+}
+
+// The Lanczos loop of the entry points below: each Krylov step applies the
+// nplan matvec plans in turn (one per precision of the batch), then the
+// shared orthogonalisation kernels over all solves.  The host polls the
+// active count one step behind the device, so one step stays queued.
 int lanczos_run(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve* solves,
                 int nsolve, int nchunks, int ncv, int max_restarts, int min_iters, int x_dtype,
                 int w_dtype, void* state, tnx_lanczos_result* results, tnx_stream_t stream) {
-  if (plans == nullptr || nplan <= 0 || solves == nullptr || state == nullptr || nsolve <= 0 ||
-      nchunks <= 0 || ncv < 2 || ncv > NCV_MAX || max_restarts < 0 || x_dtype < -1 ||
-      x_dtype > TNX_BF16 || w_dtype < -1 || w_dtype > TNX_BF16) {
-    tnx_set_error("tnx_lanczos_run: bad arguments (nplan=%d nsolve=%d nchunks=%d ncv=%d)", nplan,
-                  nsolve, nchunks, ncv);
+  if (int rc = check_args("tnx_lanczos_run", plans, nplan, solves, nsolve, nchunks, ncv,
+                          max_restarts, x_dtype, w_dtype))
+    return rc;
+  if (state == nullptr) {
+    tnx_set_error("tnx_lanczos_run: null state");
     return TNX_EINVAL;
   }
   if (int rc = ensure_pinned(nsolve)) return rc;
   hipStream_t s = (hipStream_t)stream;
   LzLayout L = lz_layout(state, nsolve, nchunks);
-  const dim3 grid(nchunks), block(NT);
-  TNX_CHECK_HIP(hipMemsetAsync(L.st, 0, sizeof(LzState) * (size_t)nsolve, s));
-  hipLaunchKernelGGL(k_set_nact, dim3(1), dim3(64), 0, s, state, nsolve, nchunks);
-  hipLaunchKernelGGL(k_init_norm, grid, block, 0, s, solves, nsolve, state, nchunks);
-  hipLaunchKernelGGL(k_init_scale, grid, block, 0, s, solves, nsolve, state, nchunks, x_dtype);
-  TNX_CHECK_LAUNCH();
+  if (int rc = issue_start(solves, nsolve, state, nchunks, x_dtype, s)) return rc;
 
   hipEvent_t ev[2];
   TNX_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
@@ -664,20 +738,12 @@ int lanczos_run(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve
   bool done = false;
   int rc = TNX_OK;
   int issued = 0;  // events recorded so far
-  for (int r = 0; r <= max_restarts && !done; ++r) {
-    if (r > 0) {
-      hipLaunchKernelGGL(k_ritz, grid, block, 0, s, solves, nsolve, state, nchunks, ncv);
-      hipLaunchKernelGGL(k_restart, grid, block, 0, s, solves, nsolve, state, nchunks, ncv, x_dtype);
-    }
+  for (int r = 0; r <= max_restarts && !done && !rc; ++r) {
+    if (r > 0 && (rc = issue_restart(solves, nsolve, state, nchunks, ncv, x_dtype, s))) break;
     for (int j = 0; j < ncv && !done; ++j) {
-      for (int p = 0; p < nplan && !rc; ++p) rc = tnx_heff_matvec(plans + p, L.act, stream);
-      if (rc) break;
-      hipLaunchKernelGGL(k_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
-      hipLaunchKernelGGL(k_update_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
-      hipLaunchKernelGGL(k_update_norm, dim3(nchunks + nsolve), block, 0, s, solves, nsolve, state,
-                         nchunks, j,
-                         x_dtype, (int)(j + 1 < ncv), min_iters);
-      TNX_CHECK_LAUNCH();
+      if ((rc = issue_step(plans, nplan, solves, nsolve, state, nchunks, j, ncv, min_iters,
+                           x_dtype, w_dtype, stream)))
+        break;
       // lagged poll: copy the active count of this step, then wait for the
       // previous step's copy (the device keeps one step of work queued)
       const int slot = issued & 1;
@@ -702,10 +768,7 @@ int lanczos_run(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_final, grid, block, 0, s, solves, nsolve, state, nchunks);
-  hipLaunchKernelGGL(k_results, dim3((nsolve + 63) / 64), dim3(64), 0, s, state, nsolve, nchunks,
-                     g_pin.res);
-  TNX_CHECK_LAUNCH();
+  if ((rc = issue_finish(solves, nsolve, state, nchunks, g_pin.res, s))) return rc;
   TNX_CHECK_HIP(hipStreamSynchronize(s));
   int nconv = 0;
   for (int i = 0; i < nsolve; ++i) {
@@ -740,3 +803,4 @@ extern "C" int tnx_lanczos_run(const tnx_matvec_plan* plan, const tnx_lanczos_so
   return tnx_lanczos_run_ex(plan, solves, nsolve, nchunks, ncv, max_restarts, 1, x_dtype, w_dtype,
                             state, results, stream);
 }
+

@@ -1720,6 +1720,10 @@ class Engine:
                 raise L.HipError('tnx_lanczos_run_multi failed (rc=%d): %s' % (rc, msg))
         self.counters['lanczos_calls'] += 1
         for prep in preps:
+            # Krylov steps the call ran in lockstep (its slowest solve)
+            self.counters['lz_iters'] += int(prep['res']['iters'].max())
+            if int(prep['res']['iters'].max()) > self.ncv:
+                self.counters['lz_calls_restarted'] = self.counters.get('lz_calls_restarted', 0) + 1
             for (s, ph), r in zip(prep['flat'], prep['res']):
                 it = int(r['iters'])
                 s.stats['matvecs'] += it
