@@ -670,10 +670,15 @@ def main():
     counters = None
     for k in range(args.steps):
         torch.cuda.synchronize()
+        # host clocks at the ends of the timed step (to place it in a
+        # profiler's kernel trace)
+        c0 = (time.monotonic_ns(), time.clock_gettime_ns(time.CLOCK_BOOTTIME))
         dt, obs, counters, _ = run_batch(para, seeds, args.precision)
+        c1 = (time.monotonic_ns(), time.clock_gettime_ns(time.CLOCK_BOOTTIME))
         times.append(dt)
         lat += [dt] * len(seeds)
-        print('step %d: %.3f s for %d solves' % (k, dt, len(seeds)), file=sys.stderr, flush=True)
+        print('step %d: %.3f s for %d solves (monotonic ns %d %d, boottime ns %d %d)'
+              % (k, dt, len(seeds), c0[0], c1[0], c0[1], c1[1]), file=sys.stderr, flush=True)
     total = sum(times)
     value = args.batch * args.steps / total
     energies = np.array([o['E'] for o in obs])

@@ -16,6 +16,12 @@
 // The (j+1)x(j+1) tridiagonal eigenproblem is solved on the device by 257-way
 // multisection of the Sturm count (one workgroup) followed by inverse
 // iteration with partial pivoting.
+// A solve that has not converged after ncv steps is restarted implicitly
+// (Sorensen's implicitly restarted Lanczos with exact shifts, as ARPACK's
+// dsaupd does for eigsh): the ncv - k largest Ritz values are applied as
+// shifts of implicit QR steps on the tridiagonal matrix, which compresses the
+// factorisation to k = min(LZ_KEEP, ncv - 1) vectors spanning the k lowest
+// Ritz vectors; the next cycle continues the Lanczos recurrence from step k.
 #include "tnx_common.h"
 
 namespace {
@@ -23,6 +29,7 @@ namespace {
 constexpr int NCV_MAX = 32;
 constexpr int CHUNK = 1024;  // 256 lanes x 4 elements
 constexpr int NT = 256;
+constexpr int LZ_KEEP = 4;   // vectors kept by an implicit restart (at most)
 
 struct LzState {
   double alpha[NCV_MAX];
@@ -30,9 +37,14 @@ struct LzState {
   double h1[NCV_MAX + 1];
   double h2[NCV_MAX + 1];
   double y[NCV_MAX];
+  // restart: row t of the new basis row u is C[t][u] (t <= ncv, u <= k)
+  double C[NCV_MAX + 1][LZ_KEEP + 1];
   double theta, resid, scale, spare;
   int32_t converged, iters, jlast, restarts;
 };
+
+// Vectors an implicit restart keeps for a call with ncv Krylov vectors.
+__host__ __device__ inline int lz_keep(int ncv) { return ncv - 1 < LZ_KEEP ? ncv - 1 : LZ_KEEP; }
 
 // partA / partB: per-chunk partial sums, [nchunks][NCV_MAX + 1].  Passes that
 // follow each other alternate between them (a kernel reads one while it
@@ -518,52 +530,183 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
   }
 }
 
-// Ritz vector of the active solves into the spare row V[m]; chunk partial of
-// its squared norm -> partA[block][0]
-__global__ __launch_bounds__(NT) void k_ritz(const tnx_lanczos_solve* __restrict__ S, int nsolve,
-                                             void* state, int nchunks, int m_rows) {
-  LzLayout L = lz_layout(state, nsolve, nchunks);
-  ChunkInfo ci = chunk_of(S, nsolve);
-  if (L.act[ci.s] == 0) return;
-  const tnx_lanczos_solve D = S[ci.s];
-  const LzState& st = L.st[ci.s];
-  const int kk = st.jlast + 1;
-  const double* V = (const double*)D.V;
-  double* X = (double*)D.V + (int64_t)m_rows * D.ldv;
-  double nn = 0.0;
-  for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
-    double s = 0.0;
-    for (int k = 0; k < kk; ++k) s += st.y[k] * V[(int64_t)k * D.ldv + i];
-    X[i] = s;
-    nn += s * s;
+// Eigenvalue `idx` (0-based, ascending) of the m x m tridiagonal (a, b) by
+// bisection of the Sturm count, to a relative width of a few ulps.
+__device__ double tridiag_eig_bisect(const double* a, const double* b, int m, int idx) {
+  double lo = a[0], hi = a[0];
+  for (int i = 0; i < m; ++i) {
+    const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < m - 1 ? fabs(b[i]) : 0.0);
+    lo = fmin(lo, a[i] - r);
+    hi = fmax(hi, a[i] + r);
   }
-  __shared__ double red[4][NCV_MAX + 1];
-  const double tot = block_total(nn, red);
-  if (threadIdx.x == 0) L.partA[(int64_t)blockIdx.x * (NCV_MAX + 1)] = tot;
+  for (int it = 0; it < 80; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (hi - lo <= 4.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)) + 1e-300) break;
+    if (sturm_count(a, b, m, mid) >= idx + 1) hi = mid;
+    else lo = mid;
+  }
+  return 0.5 * (lo + hi);
 }
 
-// restart: V[0] = V[m] / ||V[m]|| (summed by every block); x = cast(V[0])
-__global__ __launch_bounds__(NT) void k_restart(const tnx_lanczos_solve* __restrict__ S, int nsolve,
-                                                void* state, int nchunks, int m_rows, int x_dtype) {
+// Implicit restart, solve part (one workgroup per solve, active solves
+// only), after a full cycle of m = ncv steps (V[m] is the normalised next
+// Lanczos vector, beta[m-1] its coefficient):
+//  1. the m - k largest Ritz values (bisection, one lane each) are the shifts;
+//  2. lane 0 applies them as implicit QR steps to T = tridiag(alpha, beta)
+//     (a Givens bulge chase per shift, within the leading block when a
+//     subdiagonal entry has become negligible) and records the rotations;
+//  3. lane t < m accumulates row t of Q (the product of the rotations);
+//  4. the compressed factorisation: alpha[0..k), beta[0..k-1) from the
+//     rotated T, and the new residual direction
+//        r = bhat * V Q[:, k] + beta[m-1] Q[m-1, k-1] V[m],  beta[k-1] = |r|,
+//     give the coefficients C of the new basis rows 0..k (k_irl_apply).
+// A residual that vanishes (an invariant subspace) ends the solve as
+// converged with its current Ritz pair.
+__global__ __launch_bounds__(NT) void k_irl_solve(const tnx_lanczos_solve* __restrict__ S,
+                                                  int nsolve, void* state, int nchunks, int m,
+                                                  int k) {
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  const int s = blockIdx.x;
+  if (!solve_active(L.act, s)) return;
+  LzState& st = L.st[s];
+  __shared__ double a[NCV_MAX], b[NCV_MAX], sh[NCV_MAX];
+  __shared__ double rc[NCV_MAX][NCV_MAX], rs[NCV_MAX][NCV_MAX];
+  __shared__ int nrot[NCV_MAX];
+  __shared__ double qmk, rho_s, bhat_s;
+  const int t = threadIdx.x;
+  if (t < m) {
+    a[t] = st.alpha[t];
+    b[t] = st.beta[t];   // b[m-1]: coefficient of V[m]
+  }
+  __syncthreads();
+  const int np = m - k;
+  if (t < np) sh[t] = tridiag_eig_bisect(a, b, m, k + t);
+  __syncthreads();
+  if (t == 0) {
+    for (int p = 0; p < np; ++p) {
+      int iend = m - 1;   // chase rows/cols 0..iend
+      for (int u = 0; u < m - 1; ++u)
+        if (fabs(b[u]) <= 2.220446049250313e-16 * (fabs(a[u]) + fabs(a[u + 1]))) {
+          b[u] = 0.0;
+          iend = u;
+          break;
+        }
+      nrot[p] = iend;
+      double x = a[0] - sh[p], z = b[0], bulge = 0.0;
+      for (int u = 0; u < iend; ++u) {
+        double c = 1.0, sn = 0.0;
+        if (z != 0.0) {
+          const double r = hypot(x, z);
+          c = x / r;
+          sn = z / r;
+        }
+        rc[p][u] = c;
+        rs[p][u] = sn;
+        if (u > 0) b[u - 1] = c * b[u - 1] + sn * bulge;
+        const double au = a[u], au1 = a[u + 1], bu = b[u];
+        a[u] = c * c * au + 2.0 * c * sn * bu + sn * sn * au1;
+        a[u + 1] = sn * sn * au - 2.0 * c * sn * bu + c * c * au1;
+        b[u] = c * sn * (au1 - au) + (c * c - sn * sn) * bu;
+        if (u + 1 < iend) {
+          bulge = sn * b[u + 1];
+          b[u + 1] = c * b[u + 1];
+          x = b[u];
+          z = bulge;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  double q[NCV_MAX];   // row t of Q (static indices: kept in registers)
+#pragma unroll
+  for (int c = 0; c < NCV_MAX; ++c) q[c] = (c == t) ? 1.0 : 0.0;
+  if (t < m) {
+    for (int p = 0; p < np; ++p) {
+      const int n = nrot[p];
+#pragma unroll
+      for (int u = 0; u < NCV_MAX - 1; ++u) {
+        if (u < n) {
+          const double c = rc[p][u], sn = rs[p][u];
+          const double q0 = q[u], q1 = q[u + 1];
+          q[u] = c * q0 + sn * q1;
+          q[u + 1] = -sn * q0 + c * q1;
+        }
+      }
+    }
+    if (t == m - 1) {
+#pragma unroll
+      for (int c = 0; c < NCV_MAX; ++c)
+        if (c == k - 1) qmk = q[c];
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    const double bhat = b[k - 1];
+    const double sigma = st.beta[m - 1] * qmk;
+    const double rho = hypot(bhat, sigma);
+    bhat_s = bhat;
+    rho_s = rho;
+    for (int u = 0; u < k; ++u) st.alpha[u] = a[u];
+    for (int u = 0; u + 1 < k; ++u) st.beta[u] = b[u];
+    st.beta[k - 1] = rho;
+    st.restarts += 1;
+    for (int u = 0; u <= k; ++u) st.C[m][u] = 0.0;
+    if (rho > 0.0) st.C[m][k] = sigma / rho;
+    double anorm = 0.0;
+    for (int u = 0; u < m; ++u) anorm = fmax(anorm, fabs(st.alpha[u]) + fabs(st.beta[u]));
+    if (!(rho > 1e-14 * fmax(anorm, 1e-300))) {
+      // invariant subspace: the current Ritz pair is exact
+      st.converged = 1;
+      __hip_atomic_store(L.act + s, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      atomicSub(L.nact, 1);
+    }
+  }
+  __syncthreads();
+  if (t < m) {
+    const double f = rho_s > 0.0 ? bhat_s / rho_s : 0.0;
+#pragma unroll
+    for (int c = 0; c < NCV_MAX; ++c) {
+      if (c < k) st.C[t][c] = q[c];
+      else if (c == k) st.C[t][c] = f * q[c];
+    }
+  }
+}
+
+// Implicit restart, basis part (chunk workgroups of the active solves): the
+// new rows V'[u] = sum_t C[t][u] V[t] (u <= k, t <= m) in place (every
+// element is read in full before it is written), x = cast(V'[k]).
+__global__ __launch_bounds__(NT) void k_irl_apply(const tnx_lanczos_solve* __restrict__ S,
+                                                  int nsolve, void* state, int nchunks, int m,
+                                                  int k, int x_dtype) {
   LzLayout L = lz_layout(state, nsolve, nchunks);
   ChunkInfo ci = chunk_of(S, nsolve);
-  if (L.act[ci.s] == 0) return;
+  if (!solve_active(L.act, ci.s)) return;
   const tnx_lanczos_solve D = S[ci.s];
-  __shared__ double tot;
-  reduce_partials(L.partA, D.chunk_begin, ci.nch, 1, &tot);
+  const LzState& st = L.st[ci.s];
+  __shared__ double C[NCV_MAX + 1][LZ_KEEP + 1];
+  for (int e = threadIdx.x; e < (m + 1) * (LZ_KEEP + 1); e += NT)
+    C[e / (LZ_KEEP + 1)][e % (LZ_KEEP + 1)] = st.C[e / (LZ_KEEP + 1)][e % (LZ_KEEP + 1)];
   __syncthreads();
-  const double sc = 1.0 / sqrt(tot);
-  if (ci.c == 0 && threadIdx.x == 0) {
-    L.st[ci.s].scale = sc;
-    L.st[ci.s].restarts += 1;
-  }
-  const double* X = (const double*)D.V + (int64_t)m_rows * D.ldv;
-  double* V0 = (double*)D.V;
+  double* V = (double*)D.V;
   const int xd = dtype_of(x_dtype, D.x_dtype);
   for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
-    double v = X[i] * sc;
-    V0[i] = v;
-    tnx_store((void*)D.x, xd, i, v);
+    double acc[LZ_KEEP + 1];
+#pragma unroll
+    for (int u = 0; u <= LZ_KEEP; ++u) acc[u] = 0.0;
+    for (int t = 0; t <= m; ++t) {
+      const double v = V[(int64_t)t * D.ldv + i];
+#pragma unroll
+      for (int u = 0; u <= LZ_KEEP; ++u)
+        if (u <= k) acc[u] += C[t][u] * v;
+    }
+#pragma unroll
+    for (int u = 0; u <= LZ_KEEP; ++u)
+      if (u <= k) V[(int64_t)u * D.ldv + i] = acc[u];
+    double xk = 0.0;
+#pragma unroll
+    for (int u = 0; u <= LZ_KEEP; ++u)
+      if (u == k) xk = acc[u];
+    tnx_store((void*)D.x, xd, i, xk);
   }
 }
 
@@ -683,19 +826,24 @@ int issue_step(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve*
   const dim3 grid(nchunks), block(NT);
   hipLaunchKernelGGL(k_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
   hipLaunchKernelGGL(k_update_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
+  // the chunk blocks normalise V[j+1] at every step, the last of a cycle
+  // included (an implicit restart combines V[ncv] into the new basis)
+  (void)ncv;
   hipLaunchKernelGGL(k_update_norm, dim3(nchunks + nsolve), block, 0, s, solves, nsolve, state,
-                     nchunks, j, x_dtype, (int)(j + 1 < ncv), min_iters);
+                     nchunks, j, x_dtype, 1, min_iters);
   TNX_CHECK_LAUNCH();
   return TNX_OK;
 }
 
-// Restart of the still-active solves from their Ritz vector (kept in the
-// spare basis row ncv).
+// Implicit restart of the still-active solves after a full cycle: the next
+// cycle continues from Krylov step lz_keep(ncv).
 int issue_restart(const tnx_lanczos_solve* solves, int nsolve, void* state, int nchunks, int ncv,
                   int x_dtype, hipStream_t s) {
-  const dim3 grid(nchunks), block(NT);
-  hipLaunchKernelGGL(k_ritz, grid, block, 0, s, solves, nsolve, state, nchunks, ncv);
-  hipLaunchKernelGGL(k_restart, grid, block, 0, s, solves, nsolve, state, nchunks, ncv, x_dtype);
+  const int k = lz_keep(ncv);
+  hipLaunchKernelGGL(k_irl_solve, dim3(nsolve), dim3(NT), 0, s, solves, nsolve, state, nchunks,
+                     ncv, k);
+  hipLaunchKernelGGL(k_irl_apply, dim3(nchunks), dim3(NT), 0, s, solves, nsolve, state, nchunks,
+                     ncv, k, x_dtype);
   TNX_CHECK_LAUNCH();
   return TNX_OK;
 }
@@ -740,7 +888,7 @@ int lanczos_run(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve
   int issued = 0;  // events recorded so far
   for (int r = 0; r <= max_restarts && !done && !rc; ++r) {
     if (r > 0 && (rc = issue_restart(solves, nsolve, state, nchunks, ncv, x_dtype, s))) break;
-    for (int j = 0; j < ncv && !done; ++j) {
+    for (int j = (r > 0 ? lz_keep(ncv) : 0); j < ncv && !done; ++j) {
       if ((rc = issue_step(plans, nplan, solves, nsolve, state, nchunks, j, ncv, min_iters,
                            x_dtype, w_dtype, stream)))
         break;

@@ -282,11 +282,17 @@ def observe_energies(engine, solves, paras):
 
 def schedule_spectrum(solve):
     """``calculate_entanglement_spectrum(if_fast=True)`` (MPSClass.py:965-985):
-    SVD centre moves from the centre to ``p0``, to ``p1 + 1`` and back to the
+    centre moves from the centre to ``p0``, to ``p1 + 1`` and back to the
     centre, where ``[p0, p1]`` spans the bonds without a spectrum.  When every
     bond has one, ``p0 = l - 1`` and ``p1 = 0`` as in the reference, so the
     spectra are recomputed (one-site sweeps move the centre by QR and leave
-    earlier spectra in place, in the reference and here)."""
+    earlier spectra in place, in the reference and here).
+
+    The reference makes every move by SVD, so the bonds it crosses twice get
+    their spectrum twice, the same Schmidt values of the same state.  Here the
+    middle leg (``p0`` to ``p1 + 1``, every bond once) moves by SVD and the
+    other two by QR: the spectra are those of the reference's middle leg and
+    the state is the same, up to the gauge of the tensors off the centre."""
     from .engine import Step
     c = solve.center
     p0, p1 = solve.n - 1, 0
@@ -294,12 +300,13 @@ def schedule_spectrum(solve):
         if solve.lm[b].size == 0:
             p0, p1 = min(p0, b), max(p1, b)
     cur = c
-    for tgt in (p0, p1 + 1, c):
+    for leg, tgt in enumerate((p0, p1 + 1, c)):
+        kind = 'svdmove' if leg == 1 else 'move'
         while cur < tgt:
-            solve.queue.append(Step('svdmove', cur, +1))
+            solve.queue.append(Step(kind, cur, +1))
             cur += 1
         while cur > tgt:
-            solve.queue.append(Step('svdmove', cur, -1))
+            solve.queue.append(Step(kind, cur, -1))
             cur -= 1
 
 

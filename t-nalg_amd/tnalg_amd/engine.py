@@ -1366,7 +1366,7 @@ class Engine:
 
     def __init__(self, precision_schedule=None, ncv=20, max_restarts=20, min_iters=1,
                  device='cuda:0', lowp=True, ops_capacity=4096, plan_ahead=True,
-                 cycle_steps=False, autotune=True, lanczos_streams=2):
+                 cycle_steps=False, autotune=True, lanczos_streams=2, cycle_restarts=2):
         require_gpu()
         self.lib = L.load()
         self.device = torch.device(device)
@@ -1407,11 +1407,13 @@ class Engine:
         self.plan_ahead = bool(plan_ahead)
         # kernel variant per grouped-GEMM shape class, timed on first use
         self.tuner = GemmTuner(self.lib) if autotune else None
-        # cycle_steps: each Lanczos call runs one Krylov cycle; a solve that
-        # has not converged continues from its Ritz vector in the next step
-        # ('resume', the same iterates as a restart inside the call) while
-        # the others move on, so no call waits for the batch's slowest solve
+        # cycle_steps: each Lanczos call runs at most 1 + cycle_restarts
+        # Krylov cycles (implicit restarts inside the call); a solve that has
+        # not converged then continues from its Ritz vector in the next step
+        # ('resume') while the others move on, so no call waits long for the
+        # batch's slowest solve
         self.cycle_steps = bool(cycle_steps)
+        self.cycle_restarts = int(cycle_restarts)
         self._ahead_pool = None
         self._ahead = None     # future of the worker's next-step host work
         self._spec = None      # its speculative descriptor sets
@@ -1697,7 +1699,7 @@ class Engine:
         preps = [self._lanczos_prepare(k, part, got) for k, part in enumerate(parts)]
         if ahead:
             self._start_ahead(ahead)
-        restarts = 0 if self.cycle_steps else self.max_restarts
+        restarts = self.cycle_restarts if self.cycle_steps else self.max_restarts
         t0 = time.perf_counter()
         if len(preps) == 1:
             rcs = [self._lanczos_call(preps[0], self.stream(), restarts)]
@@ -1719,29 +1721,35 @@ class Engine:
             if rc not in (L.OK, L.ENOCONV):
                 raise L.HipError('tnx_lanczos_run_multi failed (rc=%d): %s' % (rc, msg))
         self.counters['lanczos_calls'] += 1
+        cnt = self.counters
         for prep in preps:
+            res = prep['res']
+            iters = res['iters'].tolist()
+            conv = res['converged'].tolist()
+            theta = res['theta'].tolist()
             # Krylov steps the call ran in lockstep (its slowest solve)
-            self.counters['lz_iters'] += int(prep['res']['iters'].max())
-            if int(prep['res']['iters'].max()) > self.ncv:
-                self.counters['lz_calls_restarted'] = self.counters.get('lz_calls_restarted', 0) + 1
-            for (s, ph), r in zip(prep['flat'], prep['res']):
-                it = int(r['iters'])
+            top = max(iters)
+            cnt['lz_iters'] += top
+            if top > self.ncv:
+                cnt['lz_calls_restarted'] = cnt.get('lz_calls_restarted', 0) + 1
+            cnt['matvecs'] += sum(iters)
+            for k, (s, ph) in enumerate(prep['flat']):
+                it = iters[k]
                 s.stats['matvecs'] += it
-                self.counters['matvecs'] += it
-                self.counters['matvec_flops'] += ph['mv_flops'] * it
-                if not r['converged'] and self.cycle_steps and s.cycles < self.max_restarts:
+                cnt['matvec_flops'] += ph['mv_flops'] * it
+                if not conv[k] and self.cycle_steps and s.cycles < self.max_restarts:
                     s.cycles += 1
                     s.resume = True   # the step re-queues the eigensolve
-                    self.counters['resumes'] += 1
+                    cnt['resumes'] += 1
                     continue
                 s.cycles = 0
-                s.last_theta = float(r['theta'])
-                s.thetas.append(s.last_theta)
-                if not r['converged']:
+                s.last_theta = theta[k]
+                s.thetas.append(theta[k])
+                if not conv[k]:
                     s.stats['unconverged'] += 1
                     s.warnings.append('local eigensolve at site %d did not converge '
                                       '(residual %.3g after %d matvecs)'
-                                      % (s._lz_site, float(r['resid']), it))
+                                      % (s._lz_site, float(res['resid'][k]), it))
 
     def _lanczos_prepare(self, k, groups, got):
         """Resident descriptors, the linear-combination mix (on the engine's

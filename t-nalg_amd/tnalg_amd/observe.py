@@ -18,6 +18,7 @@ All sites' transfers of all solves are batched into one grouped-GEMM /
 mixer / GEMM / dot launch per site.  Values are divided by the norm
 ``<psi|psi> = Tr E_N``.
 """
+import collections
 import ctypes
 
 import numpy as np
@@ -33,6 +34,14 @@ def _p(t, off=0):
 
 
 class _MeasurePlan:
+    """Buffers and bookkeeping of one solve's measurement.
+
+    A string ``(i, a)`` lives from site ``i`` (where it starts) to the last
+    site that closes it; strings whose lives do not overlap share a slot of
+    the string buffers, so the buffers hold the strings alive at one site
+    (three for the bond rows of a chain) rather than all of them.  The left
+    identity environment needs the current and the next bond only."""
+
     def __init__(self, solve, one, two):
         self.s = solve
         n = solve.n
@@ -51,11 +60,21 @@ class _MeasurePlan:
         for (i, j, a, b) in rows:
             self.chain_end[(i, a)] = max(self.chain_end.get((i, a), -1), j)
         self.chains = sorted(self.chain_end)
-        self.cidx = {c: t for t, c in enumerate(self.chains)}
+        # slot of each string: a slot is free again for strings that start at
+        # or after the site that last reads it
+        self.cidx, busy, nslot = {}, [], 0
+        for c in self.chains:
+            busy.sort()
+            if busy and busy[0][0] <= c[0]:
+                _, slot = busy.pop(0)
+            else:
+                slot, nslot = nslot, nslot + 1
+            self.cidx[c] = slot
+            busy.append((self.chain_end[c], slot))
         dm = solve.dmax
         self.dsq = max(x * x for x in dm)
         self.psi = solve.psi_max
-        nc = max(1, len(self.chains))
+        nc = max(1, nslot)
         dev = solve.dev
         f64 = torch.float64
         # close counts per site
@@ -65,14 +84,17 @@ class _MeasurePlan:
             self.close_at[i].append(('E', None, k, t))
         for t, (i, j, a, b) in enumerate(rows):
             self.close_at[j].append(('X', self.cidx[(i, a)], b, len(self.one) + t))
-        nclose = max(1, max(len(c) for c in self.close_at))
-        self.E = torch.zeros((n + 1) * self.dsq, dtype=f64, device=dev)
+        starts = [0] * n
+        for (i, a) in self.chains:
+            starts[i] += 1
+        nm = max(1, max(len(c) + st for c, st in zip(self.close_at, starts)))
+        self.E = torch.zeros(2 * self.dsq, dtype=f64, device=dev)
         self.F = torch.zeros((n + 1) * self.dsq, dtype=f64, device=dev)
         self.X = [torch.zeros(nc * self.dsq, dtype=f64, device=dev) for _ in range(2)]
         self.Z = torch.zeros((nc + 1) * self.psi, dtype=f64, device=dev)
-        self.M = torch.zeros((nclose + nc) * self.psi, dtype=f64, device=dev)
-        self.Y = torch.zeros(nclose * self.dsq, dtype=f64, device=dev)
-        self.out = torch.zeros(len(self.values) + 1, dtype=f64, device=dev)
+        self.M = torch.zeros(nm * self.psi, dtype=f64, device=dev)
+        self.Y = torch.zeros(nm * self.dsq, dtype=f64, device=dev)
+        self.out = torch.zeros(1, dtype=f64, device=dev)
         self.E[0] = 1.0
         self.F[n * self.dsq] = 1.0
 
@@ -80,7 +102,7 @@ class _MeasurePlan:
         return _p(self.F, b * self.dsq)
 
     def E_at(self, b):
-        return _p(self.E, b * self.dsq)
+        return _p(self.E, (b % 2) * self.dsq)
 
 
 def real_rows(ops, a, b, tol=1e-12):
@@ -146,113 +168,186 @@ def measure(engine, solves, requests):
     return out
 
 
+def _solve_key(s):
+    """Everything the records of a measurement take from a solve: the device
+    addresses (arena base and layout) and the operator slot."""
+    return (s.base, s.n, s.d, tuple(s.dims), tuple(s.dmax), s.psi_max, s.ops_slot)
+
+
+class _MeasureProgram:
+    """Every launch of one measurement of a batch of solves, built once: the
+    plans' device buffers, all descriptor records in one device buffer and
+    the launch list.  The records depend only on the solves' addresses, bond
+    dimensions and operator slots (``_solve_key``), so a batch of the same
+    shape on the same arenas (the next batch of a solver service, or the next
+    observation of the same solve) replays it with no host work beyond the
+    launches."""
+
+    def __init__(self, engine, solves, requests):
+        self.plans = plans = [_MeasurePlan(s, one, two) for s, (one, two) in zip(solves, requests)]
+        self.nbytes = sum(t.numel() * 8 for P in plans
+                          for t in [P.E, P.F, P.Z, P.M, P.Y, P.out] + P.X)
+        arrays, launches = [], []
+        nmax = max(s.n for s in solves)
+
+        def add_gemm(g):
+            pr, sg = g.arrays()
+            nt = R.gemm_tiles(pr)
+            if nt:
+                launches.append(('g', len(arrays), len(pr), nt))
+                arrays.extend([pr, sg])
+
+        def add_mix(mx):
+            j, t = mx.arrays()
+            nb = R.mix_blocks(j)
+            if nb:
+                launches.append(('m', len(arrays), len(j), nb))
+                arrays.extend([j, t])
+
+        # right identity environments F_b, b = N-1 .. 0
+        for b in range(nmax - 1, -1, -1):
+            g1, g2 = R.GemmList(), R.GemmList()
+            for P in plans:
+                s = P.s
+                if b >= s.n:
+                    continue
+                d, Dl, Dr = s.d, s.dims[b], s.dims[b + 1]
+                A = s.mps[b].data_ptr()
+                g1.add(_p(P.Z), Dl * d, Dr, Dr, [(A, Dr, 0, P.F_at(b + 1), Dr, 1, Dr, 1.0)])
+                g2.add(P.F_at(b), Dl, Dl, Dl, [(A, d * Dr, 0, _p(P.Z), d * Dr, 1, d * Dr, 1.0)])
+            add_gemm(g1)
+            add_gemm(g2)
+
+        # left pass with strings; every closing is a Frobenius product into
+        # dot slot k of the program
+        self.dot_map = []   # (slot, plan, output index)
+        ndot = 0
+        for site in range(nmax):
+            g1, mx, g2 = R.GemmList(), R.MixList(), R.GemmList()
+            dots = []
+            for P in plans:
+                s = P.s
+                if site >= s.n:
+                    continue
+                d, Dl, Dr = s.d, s.dims[site], s.dims[site + 1]
+                A = s.mps[site].data_ptr()
+                cur, nxt = P.X[site % 2], P.X[(site + 1) % 2]
+                psi = P.psi
+                Zp = lambda t: _p(P.Z, t * psi)
+                Mp = lambda t: _p(P.M, t * psi)
+                # Z_E and Z_X for chains alive at this site (started before it)
+                g1.add(Zp(0), Dl, d * Dr, d * Dr, [(P.E_at(site), Dl, 0, A, d * Dr, 0, Dl, 1.0)])
+                g2.add(P.E_at(site + 1), Dr, Dr, Dr, [(A, Dr, 1, Zp(0), Dr, 0, Dl * d, 1.0)])
+                alive = [c for c in P.chains if c[0] < site <= P.chain_end[c]]
+                for (i, a) in alive:
+                    t = P.cidx[(i, a)]
+                    g1.add(Zp(1 + t), Dl, d * Dr, d * Dr,
+                           [(_p(cur, t * P.dsq), Dl, 0, A, d * Dr, 0, Dl, 1.0)])
+                    if P.chain_end[(i, a)] > site:
+                        g2.add(_p(nxt, t * P.dsq), Dr, Dr, Dr,
+                               [(A, Dr, 1, Zp(1 + t), Dr, 0, Dl * d, 1.0)])
+                nm = 0
+                for (kind, ref, op, out) in P.close_at[site]:
+                    src = Zp(0) if kind == 'E' else Zp(1 + ref)
+                    mx.add(Mp(nm), L.F64, Dl, d, Dr, [(src, L.F64, 1.0, s.op_index(op))])
+                    g2.add(_p(P.Y, nm * P.dsq), Dr, Dr, Dr, [(A, Dr, 1, Mp(nm), Dr, 0, Dl * d, 1.0)])
+                    dots.append((_p(P.Y, nm * P.dsq), P.F_at(site + 1), Dr, Dr, Dr, Dr))
+                    self.dot_map.append((ndot + len(dots) - 1, P, out))
+                    nm += 1
+                # strings that start here: X_(site, a) = A^T O_a (E A)
+                for (i, a) in P.chains:
+                    if i != site:
+                        continue
+                    t = P.cidx[(i, a)]
+                    mx.add(Mp(nm), L.F64, Dl, d, Dr, [(Zp(0), L.F64, 1.0, s.op_index(a))])
+                    g2.add(_p(nxt, t * P.dsq), Dr, Dr, Dr,
+                           [(A, Dr, 1, Mp(nm), Dr, 0, Dl * d, 1.0)])
+                    nm += 1
+            add_gemm(g1)
+            add_mix(mx)
+            add_gemm(g2)
+            if dots:
+                launches.append(('d', len(arrays), len(dots), ndot))
+                arrays.append(np.array(dots, dtype=L.DOT_JOB))
+                ndot += len(dots)
+        # the norms <psi|psi> = Tr E_N, into the slots after the values
+        self.norm_slot = ndot
+        launches.append(('d', len(arrays), len(plans), ndot))
+        arrays.append(np.array([(P.E_at(P.s.n), 0, 1, 1, 1, 1) for P in plans], dtype=L.DOT_JOB))
+        ndot += len(plans)
+        self.dot_out = torch.zeros(max(ndot, 1), dtype=torch.float64, device=solves[0].dev)
+        # all records in one device buffer (256-byte aligned arrays)
+        offs, total = [], 0
+        for a in arrays:
+            offs.append(total)
+            total += (a.nbytes + 255) // 256 * 256
+        host = np.zeros(max(total, 256), dtype=np.uint8)
+        for a, o in zip(arrays, offs):
+            host[o:o + a.nbytes] = np.frombuffer(np.ascontiguousarray(a).tobytes(), np.uint8)
+        self.records = torch.from_numpy(host).to(solves[0].dev)
+        base = self.records.data_ptr()
+        self.ptrs = [base + o for o in offs]
+        self.launches = launches
+        self.nbytes += self.records.numel() + self.dot_out.numel() * 8
+        for P in plans:
+            P.s = None   # the records hold what the replay needs
+
+    def run(self, engine):
+        """Launch the measurement on the engine's stream; returns per plan
+        ``(one_vals, two_vals, norm)``."""
+        lib, stream, ptr = engine.lib, engine.stream(), self.ptrs
+        opsp = ctypes_void(engine.ops.data_ptr())
+        out = self.dot_out.data_ptr()
+        for kind, i, n, x in self.launches:
+            if kind == 'g':
+                L.check(lib.tnx_gemm_grouped(L.GEMM_F64, ctypes_void(ptr[i]), n,
+                                             ctypes_void(ptr[i + 1]), x, None, stream), 'gemm')
+            elif kind == 'm':
+                L.check(lib.tnx_mix(ctypes_void(ptr[i]), n, ctypes_void(ptr[i + 1]), x, opsp, None,
+                                    stream), 'mix')
+            else:
+                L.check(lib.tnx_trace_dot(ctypes_void(ptr[i]), n, ctypes_void(out + 8 * x),
+                                          stream), 'trace_dot')
+        vals = self.dot_out.cpu().numpy()
+        for P in self.plans:
+            P.values[:] = 0.0
+        for (k, P, o) in self.dot_map:
+            P.values[o] = vals[k]
+        results = []
+        for t, P in enumerate(self.plans):
+            norm = float(vals[self.norm_slot + t])
+            v = P.values / norm
+            results.append((v[:len(P.one)], v[len(P.one):], norm))
+        return results
+
+
+MEASURE_CACHE = 8            # programs kept per engine (least recently used dropped)
+MEASURE_CACHE_BYTES = 8 << 30
+
+
 def _measure_real(engine, solves, requests):
     """:func:`measure` for operators whose real parts are meant."""
     engine.settle()   # solve state is final once the engine's host work is done
-    plans = [_MeasurePlan(s, one, two) for s, (one, two) in zip(solves, requests)]
-    lib = engine.lib
-    stream = engine.stream()
-    nmax = max(s.n for s in solves)
-    up = engine.up
-
-    def launch_gemm(g):
-        p, sg = g.arrays()
-        nt = R.gemm_tiles(p)
-        if nt == 0:
-            return
-        dp, ds = up.upload([p, sg])
-        L.check(lib.tnx_gemm_grouped(L.GEMM_F64, ctypes_void(dp), len(p), ctypes_void(ds), nt, None,
-                                     stream), 'gemm')
-
-    def launch_mix(mx):
-        j, t = mx.arrays()
-        nb = R.mix_blocks(j)
-        if nb == 0:
-            return
-        dj, dt = up.upload([j, t])
-        L.check(lib.tnx_mix(ctypes_void(dj), len(j), ctypes_void(dt), nb,
-                            ctypes_void(engine.ops.data_ptr()), None, stream), 'mix')
-
-    # right identity environments F_b, b = N-1 .. 0
-    for b in range(nmax - 1, -1, -1):
-        g1, g2 = R.GemmList(), R.GemmList()
-        for P in plans:
-            s = P.s
-            if b >= s.n:
-                continue
-            d, Dl, Dr = s.d, s.dims[b], s.dims[b + 1]
-            A = s.mps[b].data_ptr()
-            g1.add(_p(P.Z), Dl * d, Dr, Dr, [(A, Dr, 0, P.F_at(b + 1), Dr, 1, Dr, 1.0)])
-            g2.add(P.F_at(b), Dl, Dl, Dl, [(A, d * Dr, 0, _p(P.Z), d * Dr, 1, d * Dr, 1.0)])
-        launch_gemm(g1)
-        launch_gemm(g2)
-
-    # left pass with strings
-    dot_out = torch.zeros(1024, dtype=torch.float64, device=solves[0].dev)
-    dot_used = 0
-    dot_map = []
-    for site in range(nmax):
-        g1, mx, g2 = R.GemmList(), R.MixList(), R.GemmList()
-        dots = []
-        for P in plans:
-            s = P.s
-            if site >= s.n:
-                continue
-            d, Dl, Dr = s.d, s.dims[site], s.dims[site + 1]
-            A = s.mps[site].data_ptr()
-            cur, nxt = P.X[site % 2], P.X[(site + 1) % 2]
-            psi = P.psi
-            Zp = lambda t: _p(P.Z, t * psi)
-            Mp = lambda t: _p(P.M, t * psi)
-            # Z_E and Z_X for chains alive at this site (started before it)
-            g1.add(Zp(0), Dl, d * Dr, d * Dr, [(P.E_at(site), Dl, 0, A, d * Dr, 0, Dl, 1.0)])
-            g2.add(P.E_at(site + 1), Dr, Dr, Dr, [(A, Dr, 1, Zp(0), Dr, 0, Dl * d, 1.0)])
-            alive = [c for c in P.chains if c[0] < site <= P.chain_end[c]]
-            for (i, a) in alive:
-                t = P.cidx[(i, a)]
-                g1.add(Zp(1 + t), Dl, d * Dr, d * Dr,
-                       [(_p(cur, t * P.dsq), Dl, 0, A, d * Dr, 0, Dl, 1.0)])
-                if P.chain_end[(i, a)] > site:
-                    g2.add(_p(nxt, t * P.dsq), Dr, Dr, Dr, [(A, Dr, 1, Zp(1 + t), Dr, 0, Dl * d, 1.0)])
-            nm = 0
-            for (kind, ref, op, out) in P.close_at[site]:
-                src = Zp(0) if kind == 'E' else Zp(1 + ref)
-                mx.add(Mp(nm), L.F64, Dl, d, Dr, [(src, L.F64, 1.0, s.op_index(op))])
-                g2.add(_p(P.Y, nm * P.dsq), Dr, Dr, Dr, [(A, Dr, 1, Mp(nm), Dr, 0, Dl * d, 1.0)])
-                dots.append((_p(P.Y, nm * P.dsq), P.F_at(site + 1), Dr, Dr, Dr, Dr, P, out))
-                nm += 1
-            # strings that start here: X_(site, a) = A^T O_a (E A)
-            for (i, a) in P.chains:
-                if i != site:
-                    continue
-                t = P.cidx[(i, a)]
-                mx.add(Mp(nm), L.F64, Dl, d, Dr, [(Zp(0), L.F64, 1.0, s.op_index(a))])
-                g2.add(_p(nxt, t * P.dsq), Dr, Dr, Dr, [(A, Dr, 1, Mp(nm), Dr, 0, Dl * d, 1.0)])
-                nm += 1
-        launch_gemm(g1)
-        launch_mix(mx)
-        launch_gemm(g2)
-        if dots:
-            jobs = np.array([(x, y, r, c, lx, ly) for (x, y, r, c, lx, ly, _, _) in dots],
-                            dtype=L.DOT_JOB)
-            if dot_used + len(dots) > dot_out.numel():
-                grow = torch.zeros(2 * (dot_used + len(dots)), dtype=torch.float64,
-                                   device=solves[0].dev)
-                grow[:dot_used] = dot_out[:dot_used]
-                dot_out = grow
-            (dj,) = up.upload([jobs])
-            L.check(lib.tnx_trace_dot(ctypes_void(dj), len(jobs),
-                                      ctypes_void(_p(dot_out, dot_used)), stream), 'trace_dot')
-            for k, (_, _, _, _, _, _, P, out) in enumerate(dots):
-                dot_map.append((dot_used + k, P, out))
-            dot_used += len(dots)
-    vals = dot_out[:dot_used].cpu().numpy()
-    for (k, P, out) in dot_map:
-        P.values[out] = vals[k]
-    # norms = E_N (1 x 1)
-    results = []
-    for P in plans:
-        norm = float(P.E[P.s.n * P.dsq].item())
-        vals = P.values / norm
-        results.append((vals[:len(P.one)], vals[len(P.one):], norm))
-    return results
+    # a program serves any batch of the same (solve, request) pairs, in any
+    # order: its results are permuted back to the caller's order
+    keys = [(_solve_key(s), tuple(map(tuple, one)), tuple(map(tuple, two)))
+            for s, (one, two) in zip(solves, requests)]
+    order = sorted(range(len(keys)), key=lambda k: keys[k])
+    key = tuple(keys[k] for k in order)
+    cache = getattr(engine, '_measure_cache', None)
+    if cache is None:
+        cache = engine._measure_cache = collections.OrderedDict()
+    prog = cache.get(key)
+    if prog is None:
+        prog = _MeasureProgram(engine, [solves[k] for k in order], [requests[k] for k in order])
+        cache[key] = prog
+        while len(cache) > MEASURE_CACHE or (
+                len(cache) > 1 and sum(p.nbytes for p in cache.values()) > MEASURE_CACHE_BYTES):
+            cache.popitem(last=False)
+    else:
+        cache.move_to_end(key)
+    res = prog.run(engine)
+    out = [None] * len(solves)
+    for t, k in enumerate(order):
+        out[k] = res[t]
+    return out

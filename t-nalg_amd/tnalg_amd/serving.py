@@ -51,10 +51,16 @@ class Request:
 
 class SolveServer:
     def __init__(self, engine, hbm_budget, max_active=256, clock=time.perf_counter,
-                 energy='ritz', prepare_ahead=8):
+                 energy='ritz', prepare_ahead=8, lowp_min_chi=64):
         if energy not in ('ritz', 'measure'):
             raise ValueError("energy must be 'ritz' or 'measure'")
         self.energy = energy
+        # requests with a bond dimension below this run every sweep in f64:
+        # their matvecs are latency bound, so bf16/f32 buys no time, and at
+        # small chi the low-precision sweeps leave long chains measurably
+        # behind the f64 trajectory after a fixed sweep count (chi = 32,
+        # N ~ 80: up to 1.1e-8 relative after 8 sweeps; chi >= 64: <= 5e-10)
+        self.lowp_min_chi = int(lowp_min_chi)
         self.engine = engine
         # requests prepared ahead of admission (0: prepare at admission)
         self.prepare_ahead = int(prepare_ahead) if hasattr(engine, 'prepare') else 0
@@ -95,6 +101,9 @@ class SolveServer:
                     r.solve = self.engine.add_solve(r.para, seed=r.seed, tag=r.rid, stage=stage)
                 else:
                     r.solve = self.engine.add_solve(r.para, seed=r.seed, tag=r.rid)
+                if int(r.para.get('chi', 0)) < self.lowp_min_chi and \
+                        hasattr(r.solve, 'prec_override'):
+                    r.solve.prec_override = 'fp64'
                 r.solve.schedule_canonicalise()
                 for t in range(int(r.para['sweep_time'])):
                     r.solve.schedule_sweep(t)

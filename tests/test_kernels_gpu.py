@@ -494,3 +494,67 @@ def test_lanczos_multi_precision(gpu):
         tol = 1e-9 if dts[k] == L.F64 else 2e-5
         assert abs(res_m[k]['theta'] - e0) < tol * max(1, abs(e0)), (k, res_m[k], e0)
         assert abs(abs(out_m[k] @ v0) - 1) < (1e-8 if dts[k] == L.F64 else 1e-3), k
+
+
+def test_lanczos_implicit_restart_clustered(gpu):
+    """Lowest eigenpairs whose next eigenvalue lies 1e-4 .. 1e-6 above: a
+    restart from the Ritz vector alone stalls on these (thousands of matvecs
+    on a CPU model of it); the implicit restart keeps the lowest Ritz
+    vectors and converges in a few hundred.  Checked against
+    numpy.linalg.eigh: Ritz value, eigenvector and residual."""
+    import torch
+    from tnalg_amd import _lib as L
+    from tnalg_amd import records as R
+    lib = L.load()
+    rng = np.random.RandomState(21)
+    cases = [(600, 1e-4), (1000, 1e-5), (1500, 1e-6), (300, 0.5)]
+    ncv = 20
+    keep, solves, refs = [], [], []
+    g2 = R.GemmList()
+    for k, (n, gap) in enumerate(cases):
+        d = np.sort(rng.rand(n)) * 10.0
+        d[1] = d[0] + gap
+        Q, _ = np.linalg.qr(rng.randn(n, n))
+        H = (Q * d) @ Q.T
+        H = (H + H.T) / 2
+        refs.append((d[0], Q[:, :2], H))
+        Hd = _dev(H)
+        V = torch.zeros((ncv + 1) * n, dtype=torch.float64, device='cuda')
+        w = torch.zeros(n, dtype=torch.float64, device='cuda')
+        x = torch.zeros(n, dtype=torch.float64, device='cuda')
+        init = _dev(rng.randn(n))
+        out = torch.zeros(n, dtype=torch.float64, device='cuda')
+        keep += [Hd, V, w, x, init, out]
+        g2.add(w.data_ptr(), n, 1, 1, [(Hd.data_ptr(), n, 0, x.data_ptr(), 1, 0, n, 1.0)], active=k)
+        solves.append((V.data_ptr(), w.data_ptr(), x.data_ptr(), init.data_ptr(), out.data_ptr(), n,
+                       n, 0, 0, 1e-10 / 1e-4, 1e-4, 0.0, 0, 0))
+    lz = np.array(solves, dtype=L.LANCZOS_SOLVE)
+    ch = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
+    lz['chunk_begin'] = np.concatenate([[0], np.cumsum(ch)[:-1]])
+    nch = int(ch.sum())
+    p, s = g2.arrays()
+    dp, ds, dlz = _upload([p, s, lz])
+    plan = L.MatvecPlan(0, 0, 0, 0, 0, 0, 0, 0, dp.data_ptr(), ds.data_ptr(), len(p),
+                        R.gemm_tiles(p), 0, 0, 0)
+    state = torch.zeros(int(lib.tnx_lanczos_state_bytes(len(cases), nch, ncv)), dtype=torch.uint8,
+                        device='cuda')
+    res = np.zeros(len(cases), dtype=L.LANCZOS_RESULT)
+    print('lanczos clustered', cases, flush=True)
+    rc = lib.tnx_lanczos_run_ex(plan, _vp(dlz.data_ptr()), len(cases), nch, ncv, 100, 1, 0, 0,
+                                _vp(state.data_ptr()), _vp(res.ctypes.data), _stream())
+    assert rc == 0, (rc, lib.tnx_last_error(), res)
+    for k, (n, gap) in enumerate(cases):
+        e0, v01, H = refs[k]
+        v = keep[6 * k + 5].cpu().numpy()
+        assert res[k]['converged'] == 1
+        assert abs(res[k]['theta'] - e0) < 1e-9 * max(1.0, abs(e0)), (n, gap, res[k], e0)
+        assert abs(np.linalg.norm(v) - 1) < 1e-12
+        assert np.linalg.norm(H @ v - res[k]['theta'] * v) < 2e-6, (n, gap)
+        # the eigenvector: within residual/gap of the lowest one, i.e. in
+        # the span of the two lowest for the clustered cases
+        c = v @ v01
+        if gap >= 0.1:
+            assert abs(abs(c[0]) - 1) < 1e-9, (n, gap, c)
+        else:
+            assert 1 - (c @ c) < 1e-6, (n, gap, c)
+        assert res[k]['iters'] < 1200, (n, gap, res[k])

@@ -26,11 +26,11 @@ pytestmark = pytest.mark.gpu
 N = 12
 
 
-def _c1_para(chi=16, sweeps=4, hx=0.0, hz=0.0):
+def _c1_para(chi=16, sweeps=4, hx=0.0, hz=0.0, eigs_tol=1e-10):
     from tnalg_amd import parameters as P
     para = P.generate_parameters_dmrg('chain')
     para.update(l=N, jxy=1.0, jz=1.0, hx=hx, hz=hz, chi=chi, sweep_time=sweeps, dt_ob=1,
-                eigs_tol=1e-10, tau=1e-4, break_tol=0.0)
+                eigs_tol=eigs_tol, tau=1e-4, break_tol=0.0)
     return P.make_consistent_parameter_dmrg(para)
 
 
@@ -123,9 +123,11 @@ def test_c1_outputs_brute_force_and_oracle(gpu):
 
 def test_c1_exact_chi_matches_ed(gpu):
     """chi = 64 holds the exact ground state of N=12: every output equals the
-    exact diagonalisation's."""
+    exact diagonalisation's.  The local eigensolves stop at a residual of
+    eigs_tol / tau = 1e-9 (the state is then within about that of the exact
+    one, and first-order observables with it)."""
     from tnalg_amd import dmrg_finite_size
-    para = _c1_para(chi=64, sweeps=4, hx=0.1, hz=0.2)
+    para = _c1_para(chi=64, sweeps=6, hx=0.1, hz=0.2, eigs_tol=1e-13)
     print('case: C1 N=12 chi=64 hx=0.1 hz=0.2', flush=True)
     ob, A, info, _ = dmrg_finite_size(para, seed=1)
     e0, psi0 = ed.ground_state(N, para['op'], para['index1'], para['coeff1'], para['index2'],

@@ -16,30 +16,35 @@ def _requests():
 
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
 def test_alone_equals_in_mix(gpu, precision):
-    """Each request alone (host work done in line, restarts inside the
-    Lanczos call) and in a mixed stream (next-step host work planned ahead on
-    a worker thread, one Lanczos call over every precision of the step; with
-    cycle_steps, an unconverged local solve resumes from its Ritz vector in
-    the next step) gives bit-identical energies."""
+    """Each request alone (host work done in line, one request per step) and
+    in a mixed stream (next-step host work planned ahead on a worker thread,
+    one Lanczos call over every precision of the step) gives bit-identical
+    energies, for both restart modes: implicit restarts inside the Lanczos
+    call, and cycle steps (a call runs at most 1 + cycle_restarts Krylov
+    cycles; an unconverged local solve resumes from its Ritz vector in the
+    next step)."""
+    import time
     from tnalg_amd.dmrg import _schedule_from
     from tnalg_amd.engine import Engine
     from tnalg_amd.serving import SolveServer
     reqs = _requests()
     sched = _schedule_from(precision)
     lowp = precision != 'fp64'
-    alone = []
-    for k, (t, para, seed) in enumerate(reqs):
-        print('alone request', k, 'N', para['l'], 'chi', para['chi'], flush=True)
-        eng = Engine(precision_schedule=sched, lowp=lowp, ncv=NCV, plan_ahead=False)
-        srv = SolveServer(eng, hbm_budget=8 << 30, max_active=1)
-        srv.submit(k, para, seed=seed, arrival=0.0)
-        srv.run()
-        alone.append(srv.finished[0].energy)
-    import time
     for cycle in (False, True):
+        alone = []
+        for k, (t, para, seed) in enumerate(reqs):
+            print('alone request', k, 'N', para['l'], 'chi', para['chi'], 'cycle_steps', cycle,
+                  flush=True)
+            eng = Engine(precision_schedule=sched, lowp=lowp, ncv=NCV, plan_ahead=False,
+                         cycle_steps=cycle, cycle_restarts=0)
+            srv = SolveServer(eng, hbm_budget=8 << 30, max_active=1, lowp_min_chi=0)
+            srv.submit(k, para, seed=seed, arrival=0.0)
+            srv.run()
+            alone.append(srv.finished[0].energy)
         print('mixed stream of', len(reqs), 'cycle_steps', cycle, flush=True)
-        eng = Engine(precision_schedule=sched, lowp=lowp, ncv=NCV, cycle_steps=cycle)
-        srv = SolveServer(eng, hbm_budget=8 << 30, max_active=4)
+        eng = Engine(precision_schedule=sched, lowp=lowp, ncv=NCV, cycle_steps=cycle,
+                     cycle_restarts=0)
+        srv = SolveServer(eng, hbm_budget=8 << 30, max_active=4, lowp_min_chi=0)
         t0 = time.perf_counter()
         for k, (t, para, seed) in enumerate(reqs):
             srv.submit(k, para, seed=seed, arrival=t0 + 0.002 * k)
@@ -90,7 +95,7 @@ def test_ritz_energy_equals_measured(gpu):
         print('stream energy', mode, 'prepare_ahead', ahead, flush=True)
         eng = Engine(precision_schedule=_schedule_from('mixed'), lowp=True, ncv=NCV)
         srv = SolveServer(eng, hbm_budget=8 << 30, max_active=4, energy=mode,
-                          prepare_ahead=ahead)
+                          prepare_ahead=ahead, lowp_min_chi=0)
         for k, (t, para, seed) in enumerate(reqs):
             srv.submit(k, para, seed=seed, arrival=0.0)
         summ = srv.run()
@@ -114,7 +119,7 @@ def test_two_site_requests_in_the_stream(gpu):
 
     def serve(items, max_active):
         eng = Engine(precision_schedule=_schedule_from('mixed'), lowp=True, ncv=NCV)
-        srv = SolveServer(eng, hbm_budget=8 << 30, max_active=max_active)
+        srv = SolveServer(eng, hbm_budget=8 << 30, max_active=max_active, lowp_min_chi=0)
         for k, (t, para, seed) in items:
             srv.submit(k, para, seed=seed, arrival=0.0)
         summ = srv.run()
