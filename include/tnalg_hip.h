@@ -210,6 +210,25 @@ int tnx_lanczos_run_multi(const tnx_matvec_plan* plans, int nplan,
                           int max_restarts, int min_iters, void* state,
                           tnx_lanczos_result* results, tnx_stream_t stream);
 
+/* No counterpart in the reference: tnx_lanczos_run_multi for callers that
+ * cannot allocate device or host memory (code under HIP graph capture of
+ * its neighbours, tight serving loops, harnesses that forbid allocation).
+ * The state lives in a library-owned device array of
+ * TNX_LANCZOS_STATIC_BYTES per GPU instead of a caller workspace, and no
+ * host staging buffer is used; results are copied into the caller's host
+ * array.  A call whose state (tnx_lanczos_state_bytes(nsolve, nchunks, ncv)
+ * plus 24 bytes per solve, each rounded up to 256) exceeds that size fails
+ * with TNX_EINVAL.  Every Krylov step of a cycle is queued without polling
+ * (converged solves are masked on the device) and the host reads the active
+ * count once per cycle; the results are bit-identical to
+ * tnx_lanczos_run_multi.  All calls on one GPU share its array: calls that
+ * overlap in time on one GPU race (serialise them).                        */
+#define TNX_LANCZOS_STATIC_BYTES (4u << 20)
+int tnx_lanczos_run_multi_static(const tnx_matvec_plan* plans, int nplan,
+                                 const tnx_lanczos_solve* solves, int nsolve, int nchunks,
+                                 int ncv, int max_restarts, int min_iters,
+                                 tnx_lanczos_result* results, tnx_stream_t stream);
+
 /* Batched f64 Householder QR gauge move.  For each matrix i, the logical
  * matrix M (m x n) is src[r*ld+c] (trans=0) or src[c*ld+r] (trans=1); on
  * return q holds the k=min(m,n) orthonormal columns (row-major m x k when

@@ -22,6 +22,8 @@
 // shifts of implicit QR steps on the tridiagonal matrix, which compresses the
 // factorisation to k = min(LZ_KEEP, ncv - 1) vectors spanning the k lowest
 // Ritz vectors; the next cycle continues the Lanczos recurrence from step k.
+#include <mutex>
+
 #include "tnx_common.h"
 
 namespace {
@@ -952,3 +954,98 @@ extern "C" int tnx_lanczos_run(const tnx_matvec_plan* plan, const tnx_lanczos_so
                             state, results, stream);
 }
 
+// ------------------------------------------------- no-allocation entry point
+//
+// tnx_lanczos_run_multi_static: the same Lanczos loop with its state in a
+// library-owned device array instead of a caller workspace, and no host
+// staging memory: it allocates nothing on any call.  Each GPU the library
+// runs on holds one copy of `scratch` until the process ends; every call on
+// that GPU shares it, so calls that overlap on one GPU race.  The state is
+// re-initialised by every call (nothing carries over from the last call).
+namespace {
+
+constexpr size_t LZ_STATIC_BYTES = TNX_LANCZOS_STATIC_BYTES;
+static __device__ __align__(256) unsigned char scratch[LZ_STATIC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+void* g_scratch_ptr[MAX_DEVICES];   // device address of `scratch` per GPU
+std::mutex g_scratch_mu;
+
+// Device address of the current GPU's `scratch` (looked up once per GPU,
+// with that GPU current).
+int scratch_of_current_device(void** out) {
+  int dev = -1;
+  TNX_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) {
+    tnx_set_error("tnx_lanczos_run_multi_static: device %d out of range", dev);
+    return TNX_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    TNX_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return TNX_OK;
+}
+
+}  // namespace
+
+extern "C" int tnx_lanczos_run_multi_static(const tnx_matvec_plan* plans, int nplan,
+                                            const tnx_lanczos_solve* solves, int nsolve,
+                                            int nchunks, int ncv, int max_restarts, int min_iters,
+                                            tnx_lanczos_result* results, tnx_stream_t stream) {
+  const char* who = "tnx_lanczos_run_multi_static";
+  if (int rc = check_args(who, plans, nplan, solves, nsolve, nchunks, ncv, max_restarts,
+                          TNX_DTYPE_PER_SOLVE, TNX_DTYPE_PER_SOLVE))
+    return rc;
+  const size_t state_bytes = tnx_lanczos_state_bytes(nsolve, nchunks, ncv);
+  const size_t need = state_bytes + align256(sizeof(tnx_lanczos_result) * (size_t)nsolve);
+  if (need > LZ_STATIC_BYTES) {
+    tnx_set_error("%s: %d solves over %d chunks need %zu bytes of state, more than the %zu of "
+                  "the library's scratch (use tnx_lanczos_run_multi with a workspace)",
+                  who, nsolve, nchunks, need, LZ_STATIC_BYTES);
+    return TNX_EINVAL;
+  }
+  void* base = nullptr;
+  if (int rc = scratch_of_current_device(&base)) return rc;
+  void* state = base;
+  tnx_lanczos_result* dres = (tnx_lanczos_result*)((char*)base + state_bytes);
+  hipStream_t s = (hipStream_t)stream;
+  LzLayout L = lz_layout(state, nsolve, nchunks);
+  if (int rc = issue_start(solves, nsolve, state, nchunks, TNX_DTYPE_PER_SOLVE, s)) return rc;
+  // Every Krylov step of a cycle is queued without polling: a solve that has
+  // converged is masked on the device, so its arithmetic, and the result,
+  // are those of tnx_lanczos_run_multi.  The active count is read at the end
+  // of each cycle (the restart decision).
+  bool done = false;
+  for (int r = 0; r <= max_restarts && !done; ++r) {
+    if (r > 0)
+      if (int rc = issue_restart(solves, nsolve, state, nchunks, ncv, TNX_DTYPE_PER_SOLVE, s))
+        return rc;
+    for (int j = (r > 0 ? lz_keep(ncv) : 0); j < ncv; ++j)
+      if (int rc = issue_step(plans, nplan, solves, nsolve, state, nchunks, j, ncv, min_iters,
+                              TNX_DTYPE_PER_SOLVE, TNX_DTYPE_PER_SOLVE, stream))
+        return rc;
+    int32_t nact = 0;
+    TNX_CHECK_HIP(hipMemcpyAsync(&nact, L.nact, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    TNX_CHECK_HIP(hipStreamSynchronize(s));
+    done = nact <= 0;
+  }
+  if (int rc = issue_finish(solves, nsolve, state, nchunks, dres, s)) return rc;
+  tnx_lanczos_result* host = results;
+  int nconv = 0;
+  for (int i0 = 0; i0 < nsolve; i0 += 64) {   // results through a bounded stack buffer
+    tnx_lanczos_result buf[64];
+    const int k = nsolve - i0 < 64 ? nsolve - i0 : 64;
+    TNX_CHECK_HIP(hipMemcpyAsync(buf, dres + i0, sizeof(tnx_lanczos_result) * (size_t)k,
+                                 hipMemcpyDeviceToHost, s));
+    TNX_CHECK_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < k; ++i) {
+      if (host) host[i0 + i] = buf[i];
+      nconv += buf[i].converged;
+    }
+  }
+  return nconv == nsolve ? TNX_OK : TNX_ENOCONV;
+}

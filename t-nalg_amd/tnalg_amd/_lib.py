@@ -96,6 +96,8 @@ SIGNATURES = {
                                 _vp, _vp]),
     'tnx_lanczos_run_multi': (_i, [ctypes.POINTER(MatvecPlan), _i, _vp, _i, _i, _i, _i, _i, _vp,
                                    _vp, _vp]),
+    'tnx_lanczos_run_multi_static': (_i, [ctypes.POINTER(MatvecPlan), _i, _vp, _i, _i, _i, _i,
+                                          _i, _vp, _vp]),
     'tnx_qr_f64': (_i, [_vp, _i, _vp]),
     'tnx_svd_jacobi_f64': (_i, [_vp, _i, _i, _vp]),
     'tnx_chol_inv_f64': (_i, [_vp, _i, _vp]),
