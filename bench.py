@@ -271,35 +271,58 @@ def matvec_roofline(para, batch, precs=('fp64', 'bf16'), reps=20):
 GEMM_MODE_NAME = {0: 'f64', 1: 'f32', 2: 'bf16 -> f32', 3: 'bf16 -> bf16'}
 
 
-def roofline_object(para, args):
-    """Roofline of the dominant kernel.  In the default mixed-precision run
-    that is ``gemm_grouped_kernel<0>``, the f64 MFMA GEMM (the f64 sweeps'
-    matvecs, every gauge move and environment update: about a fifth of the
-    run's GPU time, the largest share of any kernel); it is measured on its
-    matvec launches at a mid-chain step (the longer of GEMM1 and GEMM2).  The
-    low-precision matvec GEMMs of the same step are listed under
-    ``kernels``."""
-    precs = ('fp64',) if args.precision == 'fp64' else ('fp64', 'fp32', 'bf16')
-    rf = matvec_roofline(para, args.batch, precs)
-    kern = []
-    for prec in precs:
-        peak = MFMA_PEAK[prec]
-        for what, g in (('GEMM1 T = L x', rf[prec]['g1']), ('GEMM2 w = U R^T', rf[prec]['g2'])):
-            ach = g['flops'] / g['t'] / 1e12
-            kern.append({'kernel': 'gemm_grouped_kernel<%d>' % g['mode'],
-                         'role': 'matvec %s (%s MFMA)' % (what, GEMM_MODE_NAME[g['mode']]),
-                         'prec': prec, 'achieved': ach, 'peak': peak, 'frac': ach / peak,
-                         'avg_launch_s': g['t'], 'flops_per_launch': g['flops'],
-                         'tiles': g['tiles'], 'variant': g['variant'],
-                         'avg_launch_s_per_variant': g['t_variants']})
-    dom = max((k for k in kern if k['prec'] == 'fp64'), key=lambda k: k['avg_launch_s'])
-    return {'bound': 'mfma', 'achieved': dom['achieved'], 'peak': dom['peak'], 'unit': 'TFLOP/s',
-            'frac': dom['frac'], 'traffic': None,
-            'kernel': '%s (%s, kernel variant %d, %d solves at site %d of the N=%d chi=%d '
-                      'chain)' % (dom['kernel'], dom['role'], dom['variant'], args.batch,
-                                  para['l'] // 2, para['l'], para['chi']),
-            'avg_launch_s': dom['avg_launch_s'], 'flops_per_launch': dom['flops_per_launch'],
-            'matvec_s': {p: rf[p]['t_matvec'] for p in precs}, 'kernels': kern}
+def _class_peak(name):
+    """(bound, peak, unit) of a sampled kernel class."""
+    if name.startswith('gemm'):
+        prec = 'fp64' if 'f64' in name else 'fp32' if 'f32' in name else 'bf16'
+        return 'mfma', MFMA_PEAK[prec], 'TFLOP/s'
+    return 'hbm', HBM_PEAK, 'GB/s'
+
+
+def live_kernels(stats):
+    """Per kernel class of the Lanczos loop, from the library's sampled
+    timing of a timed batch (tnx_profile_begin / tnx_profile_end): average
+    duration of a timed launch (first Krylov step of a sampled call, every
+    solve active), algorithmic work per launch, achieved rate and fraction
+    of the roofline, and the launches of the class in the batch.  ``est_s``
+    (launches x average duration) ranks the classes; it overstates the
+    later Krylov steps, in which converged solves are masked."""
+    out = []
+    for st in stats:
+        if st.samples <= 0 or st.seconds <= 0:
+            continue
+        name = st.name.decode()
+        bound, peak, unit = _class_peak(name)
+        avg = st.seconds / st.samples
+        work = st.work / st.samples
+        ach = work / avg / (1e12 if unit == 'TFLOP/s' else 1e9)
+        out.append({'kernel': name, 'bound': bound, 'achieved': ach, 'peak': peak, 'unit': unit,
+                    'frac': ach / peak, 'avg_launch_s': avg,
+                    ('flops_per_launch' if st.unit == 0 else 'bytes_per_launch'): work,
+                    'timed_launches': int(st.samples), 'launches': int(st.launches),
+                    'est_s': avg * st.launches})
+    return sorted(out, key=lambda k: -k['est_s'])
+
+
+KERNEL_OF_CLASS = {'gemm f64': 'gemm_grouped_kernel<0>/<4>', 'gemm f32': 'gemm_grouped_kernel<1>',
+                   'gemm bf16 matvec T': 'gemm_grouped_kernel<3>',
+                   'gemm bf16 matvec w': 'gemm_grouped_kernel<2>', 'mixer': 'mix_kernel',
+                   'k_dots': 'k_dots', 'k_update_dots': 'k_update_dots',
+                   'k_update_norm': 'k_update_norm'}
+
+
+def roofline_object(kern, note):
+    """``roofline`` of the bench line: the class that ranks first in
+    :func:`live_kernels` (measured live on the timed batch), with the
+    others under ``kernels``."""
+    if not kern:
+        return None
+    dom = kern[0]
+    hip = next((v for k, v in KERNEL_OF_CLASS.items() if dom['kernel'].startswith(k)), '?')
+    return {'bound': dom['bound'], 'achieved': dom['achieved'], 'peak': dom['peak'],
+            'unit': dom['unit'], 'frac': dom['frac'], 'traffic': None,
+            'kernel': '%s (%s)' % (hip, dom['kernel']), 'avg_launch_s': dom['avg_launch_s'],
+            'measured': note, 'kernels': kern}
 
 
 def output_differences(obs, ref):
@@ -653,9 +676,12 @@ def main():
     para = c2_para(args.n, args.chi, args.sweeps, sites=args.dmrg_sites,
                    two_site_sweeps=args.two_site_sweeps, chi_init=args.chi_init)
     if args.roofline_only:
-        rl = roofline_object(para, args)
-        rl['copy_calibration_bytes'] = copy_calibration()
-        print(json.dumps({'roofline': rl}))
+        # the matvec GEMM kernel variants at a mid-chain step (HIP events
+        # around each variant's launches), and the copy kernel the HBM
+        # counters are scaled by
+        precs = ('fp64',) if args.precision == 'fp64' else ('fp64', 'fp32', 'bf16')
+        print(json.dumps({'matvec_variants': matvec_roofline(para, args.batch, precs),
+                          'copy_calibration_bytes': copy_calibration()}))
         return
     seeds = list(range(args.batch))
     cold = None
@@ -668,13 +694,24 @@ def main():
     times, lat = [], []
     obs = None
     counters = None
+    from tnalg_amd import _lib as L
+    lib = L.load()
+    prof = (L.ProfStat * L.PROF_NCLASS)()
     for k in range(args.steps):
         torch.cuda.synchronize()
+        # the library times the first Krylov step of every 4th Lanczos call
+        # of the first timed batch (the live roofline)
+        if k == 0:
+            L.check(lib.tnx_profile_begin(4, 8192), 'tnx_profile_begin')
         # host clocks at the ends of the timed step (to place it in a
         # profiler's kernel trace)
         c0 = (time.monotonic_ns(), time.clock_gettime_ns(time.CLOCK_BOOTTIME))
         dt, obs, counters, _ = run_batch(para, seeds, args.precision)
         c1 = (time.monotonic_ns(), time.clock_gettime_ns(time.CLOCK_BOOTTIME))
+        if k == 0:
+            nst = lib.tnx_profile_end(prof, L.PROF_NCLASS)
+            if nst < 0:
+                L.check(nst, 'tnx_profile_end')
         times.append(dt)
         lat += [dt] * len(seeds)
         print('step %d: %.3f s for %d solves (monotonic ns %d %d, boottime ns %d %d)'
@@ -725,7 +762,11 @@ def main():
         out['speedup_vs_plain_fp64'] = value / (1.0 / dt_p)
         print('fp64 batched %.2f solves/s, plain %.3f solves/s, max dE/E %.2e'
               % (len(seeds) / dt_b, 1.0 / dt_p, rel.max()), file=sys.stderr, flush=True)
-    out['roofline'] = roofline_object(c2_para(args.n, args.chi, args.sweeps), args)
+    out['roofline'] = roofline_object(
+        live_kernels(list(prof)),
+        'HIP events around the first Krylov step of every 4th Lanczos call of the first timed '
+        'batch (%d solves), on the stream each call runs on; algorithmic work from the '
+        'launches\' descriptors' % args.batch)
     if args.serving_requests > 0:
         out['serving'] = serving_run(args.serving_requests, args.serving_rate)
         print('serving: %s' % out['serving'], file=sys.stderr, flush=True)

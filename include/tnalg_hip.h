@@ -111,6 +111,9 @@ typedef struct tnx_matvec_plan {
   const double* ops;   /* operator table for the mixer */
   int32_t mode;        /* tnx_gemm_mode of GEMM 1 and GEMM 2 (see below)   */
   int32_t mode2;
+  double work[3];      /* algorithmic flops of GEMM 1, bytes of the mixer and
+                          flops of GEMM 2 with every solve active; read only
+                          by the sampled kernel timing (0: not known)       */
 } tnx_matvec_plan;
 
 /* One eigenproblem of a batched Lanczos solve.                              */
@@ -289,7 +292,10 @@ int tnx_svd_jacobi_f64(const tnx_svd_job* jobs, int njob, int max_sweeps, tnx_st
  * rotated in blocks of bw (a power of two, 4 <= bw <= 32, bw * n <= 4096);
  * one launch per round of the round-robin over block pairs, sweeps until
  * none rotates (flag: one device int32 shared by the call).  Returns
- * TNX_ENOCONV after max_sweeps sweeps that still rotated.                   */
+ * TNX_ENOCONV after max_sweeps sweeps that still rotated.
+ * A job with u == 0 computes the singular values only (work: n^2 + n + 1
+ * doubles; bw * n <= 8192; r may equal work with ld_r == n), the
+ * entanglement spectrum of a bond whose gauge move was a QR.              */
 typedef struct tnx_svdj_job {
   uint64_t r;          /* R, row-major n x n, leading dimension ld_r (read)   */
   uint64_t work;       /* scratch: 2 n^2 + n + 1 doubles                      */
@@ -302,6 +308,29 @@ typedef struct tnx_svdj_job {
 } tnx_svdj_job; /* 64 bytes */
 int tnx_svd_block_jacobi(const tnx_svdj_job* jobs, int njob, int npairs, int max_nblk, int max_n,
                          int32_t* flag, int max_sweeps, tnx_stream_t stream);
+
+/* Sampled kernel timing (no counterpart in the reference: it measures the
+ * running engine for its roofline).  tnx_profile_begin(every, max_launches)
+ * arms it: in every `every`-th Lanczos call made afterwards, on any host
+ * thread, the launches of the first Krylov step (every solve active) are
+ * each timed with two HIP events on their stream, up to max_launches
+ * launches.  tnx_profile_end waits for the device, fills out[0..cap) with
+ * one entry per kernel class (TNX_PROF_NCLASS of them) and returns that
+ * count.  The work of a timed launch is its algorithmic flops (GEMMs, from
+ * the plan's `work`) or bytes (mixer; Lanczos passes: basis rows read and
+ * written over nchunks * 1024 elements, only for calls with one plan).    */
+enum { TNX_PROF_NCLASS = 10 };
+typedef struct tnx_prof_stat {
+  char name[48];       /* kernel class                                      */
+  int64_t launches;    /* launches of the class while armed (timed or not)  */
+  int64_t samples;     /* launches timed                                    */
+  double seconds;      /* their summed duration                             */
+  double work;         /* their summed algorithmic work                     */
+  int32_t unit;        /* 0: flops, 1: bytes                                */
+  int32_t reserved;
+} tnx_prof_stat; /* 88 bytes */
+int tnx_profile_begin(int every, int max_launches);
+int tnx_profile_end(tnx_prof_stat* out, int cap);
 
 /* Batched traces / Frobenius inner products: out[i] = sum_ab X_i[a,b] Y_i[a,b]
  * (Y = NULL: trace of X).  Replaces the np.trace closings of observation_s1

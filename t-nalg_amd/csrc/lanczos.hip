@@ -532,28 +532,31 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
   }
 }
 
-// Eigenvalue `idx` (0-based, ascending) of the m x m tridiagonal (a, b) by
-// bisection of the Sturm count, to a relative width of a few ulps.
-__device__ double tridiag_eig_bisect(const double* a, const double* b, int m, int idx) {
-  double lo = a[0], hi = a[0];
-  for (int i = 0; i < m; ++i) {
-    const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < m - 1 ? fabs(b[i]) : 0.0);
-    lo = fmin(lo, a[i] - r);
-    hi = fmax(hi, a[i] + r);
+// Sturm count (eigenvalues below x) of the m x m tridiagonal held in
+// registers: a[i] diagonal, e[i] = b[i]^2 the squared coupling of i, i+1.
+__device__ __forceinline__ int sturm_count_reg(const double (&a)[NCV_MAX],
+                                               const double (&e)[NCV_MAX], int m, double x) {
+  const double tiny = 1e-300;
+  int cnt = 0;
+  double q = 1.0;
+#pragma unroll
+  for (int i = 0; i < NCV_MAX; ++i) {
+    if (i < m) {
+      q = (i == 0) ? (a[0] - x) : (a[i] - x) - e[i - 1] / q;
+      if (q < 0) cnt++;
+      if (fabs(q) < tiny) q = -tiny;
+    }
   }
-  for (int it = 0; it < 80; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    if (hi - lo <= 4.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)) + 1e-300) break;
-    if (sturm_count(a, b, m, mid) >= idx + 1) hi = mid;
-    else lo = mid;
-  }
-  return 0.5 * (lo + hi);
+  return cnt;
 }
 
 // Implicit restart, solve part (one workgroup per solve, active solves
 // only), after a full cycle of m = ncv steps (V[m] is the normalised next
 // Lanczos vector, beta[m-1] its coefficient):
-//  1. the m - k largest Ritz values (bisection, one lane each) are the shifts;
+//  1. the m - k largest Ritz values are the shifts: lane i bisects the
+//     Sturm count for eigenvalue k + i (a shift needs no more than ~1e-13
+//     relative accuracy: any shift gives a valid compressed factorisation,
+//     the exact ones remove the unwanted directions);
 //  2. lane 0 applies them as implicit QR steps to T = tridiag(alpha, beta)
 //     (a Givens bulge chase per shift, within the leading block when a
 //     subdiagonal entry has become negligible) and records the rotations;
@@ -562,8 +565,9 @@ __device__ double tridiag_eig_bisect(const double* a, const double* b, int m, in
 //     rotated T, and the new residual direction
 //        r = bhat * V Q[:, k] + beta[m-1] Q[m-1, k-1] V[m],  beta[k-1] = |r|,
 //     give the coefficients C of the new basis rows 0..k (k_irl_apply).
-// A residual that vanishes (an invariant subspace) ends the solve as
-// converged with its current Ritz pair.
+// The tridiagonal stays in registers (statically indexed unrolled loops) in
+// steps 1 and 2.  A residual that vanishes (an invariant subspace) ends the
+// solve as converged with its current Ritz pair.
 __global__ __launch_bounds__(NT) void k_irl_solve(const tnx_lanczos_solve* __restrict__ S,
                                                   int nsolve, void* state, int nchunks, int m,
                                                   int k) {
@@ -571,51 +575,91 @@ __global__ __launch_bounds__(NT) void k_irl_solve(const tnx_lanczos_solve* __res
   const int s = blockIdx.x;
   if (!solve_active(L.act, s)) return;
   LzState& st = L.st[s];
-  __shared__ double a[NCV_MAX], b[NCV_MAX], sh[NCV_MAX];
+  __shared__ double a_l[NCV_MAX], b_l[NCV_MAX], sh[NCV_MAX];
   __shared__ double rc[NCV_MAX][NCV_MAX], rs[NCV_MAX][NCV_MAX];
   __shared__ int nrot[NCV_MAX];
   __shared__ double qmk, rho_s, bhat_s;
+  __shared__ double ak[LZ_KEEP], bk[LZ_KEEP];
   const int t = threadIdx.x;
   if (t < m) {
-    a[t] = st.alpha[t];
-    b[t] = st.beta[t];   // b[m-1]: coefficient of V[m]
+    a_l[t] = st.alpha[t];
+    b_l[t] = st.beta[t];   // b[m-1]: coefficient of V[m]
   }
   __syncthreads();
   const int np = m - k;
-  if (t < np) sh[t] = tridiag_eig_bisect(a, b, m, k + t);
+  if (t < np) {
+    double a[NCV_MAX], e[NCV_MAX];
+    double lo = a_l[0], hi = a_l[0];
+#pragma unroll
+    for (int i = 0; i < NCV_MAX; ++i) {
+      a[i] = (i < m) ? a_l[i] : 0.0;
+      const double bi = (i < m - 1) ? b_l[i] : 0.0;
+      e[i] = bi * bi;
+      if (i < m) {
+        const double r = fabs(bi) + (i > 0 ? fabs(b_l[i - 1]) : 0.0);
+        lo = fmin(lo, a[i] - r);
+        hi = fmax(hi, a[i] + r);
+      }
+    }
+    const int idx = k + t;
+    for (int it = 0; it < 64; ++it) {
+      if (hi - lo <= 1e-13 * fmax(fabs(lo), fabs(hi)) + 1e-300) break;
+      const double mid = 0.5 * (lo + hi);
+      if (sturm_count_reg(a, e, m, mid) >= idx + 1) hi = mid;
+      else lo = mid;
+    }
+    sh[t] = 0.5 * (lo + hi);
+  }
   __syncthreads();
   if (t == 0) {
+    double a[NCV_MAX], b[NCV_MAX];
+#pragma unroll
+    for (int i = 0; i < NCV_MAX; ++i) {
+      a[i] = (i < m) ? a_l[i] : 0.0;
+      b[i] = (i < m - 1) ? b_l[i] : 0.0;
+    }
     for (int p = 0; p < np; ++p) {
       int iend = m - 1;   // chase rows/cols 0..iend
-      for (int u = 0; u < m - 1; ++u)
-        if (fabs(b[u]) <= 2.220446049250313e-16 * (fabs(a[u]) + fabs(a[u + 1]))) {
-          b[u] = 0.0;
-          iend = u;
-          break;
-        }
+#pragma unroll
+      for (int u = NCV_MAX - 2; u >= 0; --u)
+        if (u < m - 1 && fabs(b[u]) <= 2.220446049250313e-16 * (fabs(a[u]) + fabs(a[u + 1])))
+          iend = u;   // the first negligible coupling from the top
+#pragma unroll
+      for (int u = 0; u < NCV_MAX - 1; ++u)
+        if (u == iend) b[u] = 0.0;
       nrot[p] = iend;
       double x = a[0] - sh[p], z = b[0], bulge = 0.0;
-      for (int u = 0; u < iend; ++u) {
-        double c = 1.0, sn = 0.0;
-        if (z != 0.0) {
-          const double r = hypot(x, z);
-          c = x / r;
-          sn = z / r;
-        }
-        rc[p][u] = c;
-        rs[p][u] = sn;
-        if (u > 0) b[u - 1] = c * b[u - 1] + sn * bulge;
-        const double au = a[u], au1 = a[u + 1], bu = b[u];
-        a[u] = c * c * au + 2.0 * c * sn * bu + sn * sn * au1;
-        a[u + 1] = sn * sn * au - 2.0 * c * sn * bu + c * c * au1;
-        b[u] = c * sn * (au1 - au) + (c * c - sn * sn) * bu;
-        if (u + 1 < iend) {
-          bulge = sn * b[u + 1];
-          b[u + 1] = c * b[u + 1];
-          x = b[u];
-          z = bulge;
+#pragma unroll
+      for (int u = 0; u < NCV_MAX - 1; ++u) {
+        if (u < iend) {
+          // one reciprocal square root per rotation: the chase is a serial
+          // chain, and its entries are O(|T|), far from over- or underflow
+          double c = 1.0, sn = 0.0;
+          if (z != 0.0) {
+            const double ri = rsqrt(x * x + z * z);
+            c = x * ri;
+            sn = z * ri;
+          }
+          rc[p][u] = c;
+          rs[p][u] = sn;
+          if (u > 0) b[u - 1] = c * b[u - 1] + sn * bulge;
+          const double au = a[u], au1 = a[u + 1], bu = b[u];
+          a[u] = c * c * au + 2.0 * c * sn * bu + sn * sn * au1;
+          a[u + 1] = sn * sn * au - 2.0 * c * sn * bu + c * c * au1;
+          b[u] = c * sn * (au1 - au) + (c * c - sn * sn) * bu;
+          if (u + 1 < iend) {
+            bulge = sn * b[u + 1];
+            b[u + 1] = c * b[u + 1];
+            x = b[u];
+            z = bulge;
+          }
         }
       }
+    }
+#pragma unroll
+    for (int i = 0; i < LZ_KEEP; ++i) {
+      ak[i] = a[i];
+      bk[i] = b[i];
     }
   }
   __syncthreads();
@@ -643,19 +687,19 @@ __global__ __launch_bounds__(NT) void k_irl_solve(const tnx_lanczos_solve* __res
   }
   __syncthreads();
   if (t == 0) {
-    const double bhat = b[k - 1];
+    const double bhat = bk[k - 1];
     const double sigma = st.beta[m - 1] * qmk;
     const double rho = hypot(bhat, sigma);
     bhat_s = bhat;
     rho_s = rho;
-    for (int u = 0; u < k; ++u) st.alpha[u] = a[u];
-    for (int u = 0; u + 1 < k; ++u) st.beta[u] = b[u];
+    double anorm = 0.0;
+    for (int u = 0; u < m; ++u) anorm = fmax(anorm, fabs(a_l[u]) + fabs(b_l[u]));
+    for (int u = 0; u < k; ++u) st.alpha[u] = ak[u];
+    for (int u = 0; u + 1 < k; ++u) st.beta[u] = bk[u];
     st.beta[k - 1] = rho;
     st.restarts += 1;
     for (int u = 0; u <= k; ++u) st.C[m][u] = 0.0;
     if (rho > 0.0) st.C[m][k] = sigma / rho;
-    double anorm = 0.0;
-    for (int u = 0; u < m; ++u) anorm = fmax(anorm, fabs(st.alpha[u]) + fabs(st.beta[u]));
     if (!(rho > 1e-14 * fmax(anorm, 1e-300))) {
       // invariant subspace: the current Ritz pair is exact
       st.converged = 1;
@@ -826,13 +870,31 @@ int issue_step(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve*
   for (int p = 0; p < nplan; ++p)
     if (int rc = tnx_heff_matvec(plans + p, L.act, stream)) return rc;
   const dim3 grid(nchunks), block(NT);
+  // sampled timing (one plan: its modes give the x and w element sizes);
+  // algorithmic bytes over the nchunks * CHUNK elements: basis rows 0..j
+  // read once, w / r / x read or written once
+  const bool prof = tnx_prof_on() && nplan == 1;
+  const double E = (double)nchunks * CHUNK;
+  const int m1 = plans[0].mode & 0xff, m2 = plans[0].mode2 & 0xff;
+  const double xs = m1 == TNX_GEMM_F64 ? 8.0 : m1 == TNX_GEMM_F32 ? 4.0 : 2.0;
+  const double ws = m2 == TNX_GEMM_F64 ? 8.0 : m2 == TNX_GEMM_BF16_BF16 ? 2.0 : 4.0;
+  const double vrows = 8.0 * (j + 1);
+  tnx_prof_count(TNX_PC_DOTS);
+  tnx_prof_count(TNX_PC_UPDATE_DOTS);
+  tnx_prof_count(TNX_PC_UPDATE_NORM);
+  int mk = prof ? tnx_prof_mark(TNX_PC_DOTS, s) : -1;
   hipLaunchKernelGGL(k_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
+  tnx_prof_done(mk, s, E * (vrows + ws));
+  mk = prof ? tnx_prof_mark(TNX_PC_UPDATE_DOTS, s) : -1;
   hipLaunchKernelGGL(k_update_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
+  tnx_prof_done(mk, s, E * (vrows + ws + 8.0));
   // the chunk blocks normalise V[j+1] at every step, the last of a cycle
   // included (an implicit restart combines V[ncv] into the new basis)
   (void)ncv;
+  mk = prof ? tnx_prof_mark(TNX_PC_UPDATE_NORM, s) : -1;
   hipLaunchKernelGGL(k_update_norm, dim3(nchunks + nsolve), block, 0, s, solves, nsolve, state,
                      nchunks, j, x_dtype, 1, min_iters);
+  tnx_prof_done(mk, s, E * (vrows + 16.0 + xs));
   TNX_CHECK_LAUNCH();
   return TNX_OK;
 }
@@ -888,12 +950,15 @@ int lanczos_run(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve
   bool done = false;
   int rc = TNX_OK;
   int issued = 0;  // events recorded so far
+  const bool sampled = tnx_prof_call_begin();
   for (int r = 0; r <= max_restarts && !done && !rc; ++r) {
     if (r > 0 && (rc = issue_restart(solves, nsolve, state, nchunks, ncv, x_dtype, s))) break;
     for (int j = (r > 0 ? lz_keep(ncv) : 0); j < ncv && !done; ++j) {
-      if ((rc = issue_step(plans, nplan, solves, nsolve, state, nchunks, j, ncv, min_iters,
-                           x_dtype, w_dtype, stream)))
-        break;
+      tnx_prof_set(sampled && r == 0 && j == 0);
+      rc = issue_step(plans, nplan, solves, nsolve, state, nchunks, j, ncv, min_iters, x_dtype,
+                      w_dtype, stream);
+      tnx_prof_set(false);
+      if (rc) break;
       // lagged poll: copy the active count of this step, then wait for the
       // previous step's copy (the device keeps one step of work queued)
       const int slot = issued & 1;
@@ -1020,14 +1085,18 @@ extern "C" int tnx_lanczos_run_multi_static(const tnx_matvec_plan* plans, int np
   // are those of tnx_lanczos_run_multi.  The active count is read at the end
   // of each cycle (the restart decision).
   bool done = false;
+  const bool sampled = tnx_prof_call_begin();
   for (int r = 0; r <= max_restarts && !done; ++r) {
     if (r > 0)
       if (int rc = issue_restart(solves, nsolve, state, nchunks, ncv, TNX_DTYPE_PER_SOLVE, s))
         return rc;
-    for (int j = (r > 0 ? lz_keep(ncv) : 0); j < ncv; ++j)
-      if (int rc = issue_step(plans, nplan, solves, nsolve, state, nchunks, j, ncv, min_iters,
-                              TNX_DTYPE_PER_SOLVE, TNX_DTYPE_PER_SOLVE, stream))
-        return rc;
+    for (int j = (r > 0 ? lz_keep(ncv) : 0); j < ncv; ++j) {
+      tnx_prof_set(sampled && r == 0 && j == 0);
+      const int rc = issue_step(plans, nplan, solves, nsolve, state, nchunks, j, ncv, min_iters,
+                                TNX_DTYPE_PER_SOLVE, TNX_DTYPE_PER_SOLVE, stream);
+      tnx_prof_set(false);
+      if (rc) return rc;
+    }
     int32_t nact = 0;
     TNX_CHECK_HIP(hipMemcpyAsync(&nact, L.nact, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     TNX_CHECK_HIP(hipStreamSynchronize(s));

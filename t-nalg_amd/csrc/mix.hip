@@ -141,7 +141,8 @@ extern "C" int tnx_abi_sizes(int32_t* sizes, int n) {
                          (int32_t)sizeof(tnx_matvec_plan),   (int32_t)sizeof(tnx_lanczos_solve),
                          (int32_t)sizeof(tnx_lanczos_result), (int32_t)sizeof(tnx_qr_job),
                          (int32_t)sizeof(tnx_svd_job),       (int32_t)sizeof(tnx_dot_job),
-                         (int32_t)sizeof(tnx_chol_job),      (int32_t)sizeof(tnx_svdj_job)};
+                         (int32_t)sizeof(tnx_chol_job),      (int32_t)sizeof(tnx_svdj_job),
+                         (int32_t)sizeof(tnx_prof_stat)};
   const int na = (int)(sizeof(all) / sizeof(all[0]));
   if (sizes == nullptr) return na;
   for (int i = 0; i < n && i < na; ++i) sizes[i] = all[i];
@@ -172,18 +173,40 @@ extern "C" int tnx_trace_dot(const tnx_dot_job* jobs, int njob, double* out, tnx
   return TNX_OK;
 }
 
+// Timing class of a matvec GEMM launch (the variant byte of the mode aside).
+static int gemm_class(int mode, int second) {
+  switch (mode & 0xff) {
+    case TNX_GEMM_F64: return TNX_PC_GEMM_F64_1 + second;
+    case TNX_GEMM_F32: return TNX_PC_GEMM_F32_1 + second;
+    default: return TNX_PC_GEMM_BF16_1 + second;
+  }
+}
+
 extern "C" int tnx_heff_matvec(const tnx_matvec_plan* plan, const int32_t* active,
                                tnx_stream_t stream) {
   if (plan == nullptr) {
     tnx_set_error("tnx_heff_matvec: null plan");
     return TNX_EINVAL;
   }
+  hipStream_t s = (hipStream_t)stream;
+  const bool prof = tnx_prof_on();
+  tnx_prof_count(gemm_class(plan->mode, 0));
+  if (plan->mix_nblocks > 0) tnx_prof_count(TNX_PC_MIX);
+  tnx_prof_count(gemm_class(plan->mode2, 1));
+  int mk = prof ? tnx_prof_mark(gemm_class(plan->mode, 0), s) : -1;
   int rc = tnx_gemm_grouped(plan->mode, plan->g1_prob, plan->g1_nprob, plan->g1_seg,
                             plan->g1_ntiles, active, stream);
   if (rc) return rc;
+  tnx_prof_done(mk, s, plan->work[0]);
+  mk = (prof && plan->mix_nblocks > 0) ? tnx_prof_mark(TNX_PC_MIX, s) : -1;
   rc = tnx_mix(plan->mix_job, plan->mix_njob, plan->mix_term, plan->mix_nblocks, plan->ops, active,
                stream);
   if (rc) return rc;
-  return tnx_gemm_grouped(plan->mode2, plan->g2_prob, plan->g2_nprob, plan->g2_seg,
-                          plan->g2_ntiles, active, stream);
+  tnx_prof_done(mk, s, plan->work[1]);
+  mk = prof ? tnx_prof_mark(gemm_class(plan->mode2, 1), s) : -1;
+  rc = tnx_gemm_grouped(plan->mode2, plan->g2_prob, plan->g2_nprob, plan->g2_seg,
+                        plan->g2_ntiles, active, stream);
+  if (rc) return rc;
+  tnx_prof_done(mk, s, plan->work[2]);
+  return TNX_OK;
 }

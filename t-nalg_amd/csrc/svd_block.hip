@@ -20,6 +20,13 @@
 // a rank-deficient R) and takes part in no rotation: rotating pairs of such
 // columns never settles, and leaving them alone keeps R = u^T v exact and u
 // orthogonal.
+//
+// A job with u == 0 asks for the singular values only (the entanglement
+// spectrum of a bond whose gauge move was a QR): W is neither formed nor
+// rotated, the work area is n^2 + n + 1 doubles (B, column norms, scale),
+// and with one block of B per pair there is room for twice the block width
+// (bw n <= 8192).  R may then be the work area itself (r == work, ld_r == n):
+// forming B is a copy onto itself.
 #include "tnx_common.h"
 
 namespace {
@@ -27,6 +34,13 @@ namespace {
 constexpr int NT = 256;
 constexpr int SJ_NMAX = 1024;
 constexpr size_t SJ_LDS = sizeof(double) * 4 * 4096;  // 2 (B, W) x 2 bw x n, bw n <= 4096
+
+// values-only job: no W (see the header)
+__device__ __forceinline__ bool values_only(const tnx_svdj_job& J) { return J.u == 0; }
+// column norms and the scale ||R||_F^2 follow B (and W, when formed)
+__device__ __forceinline__ double* norms_of(const tnx_svdj_job& J) {
+  return (double*)J.work + (values_only(J) ? 1 : 2) * (int64_t)J.n * J.n;
+}
 
 __device__ __forceinline__ void pair_of(int players, int round, int p, int& a, int& b) {
   if (p == 0) {
@@ -45,10 +59,11 @@ __global__ __launch_bounds__(NT) void k_svdj_init(const tnx_svdj_job* __restrict
   const double* R = (const double*)J.r;
   double* B = (double*)J.work;
   double* W = B + (int64_t)n * n;
+  const bool vo = values_only(J);
   for (int j = blockIdx.x; j < n; j += gridDim.x)
     for (int i = threadIdx.x; i < n; i += NT) {
       B[(int64_t)j * n + i] = R[(int64_t)j * J.ld_r + i];
-      W[(int64_t)j * n + i] = (i == j) ? 1.0 : 0.0;
+      if (!vo) W[(int64_t)j * n + i] = (i == j) ? 1.0 : 0.0;
     }
 }
 
@@ -67,7 +82,7 @@ __global__ __launch_bounds__(NT) void k_svdj_scale(const tnx_svdj_job* __restric
   __shared__ double red[NT / 64];
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) B[2 * nn + J.n] = (red[0] + red[1]) + (red[2] + red[3]);
+  if (threadIdx.x == 0) norms_of(J)[J.n] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 __global__ void k_svdj_clear(int32_t* flag) {
@@ -87,6 +102,7 @@ __global__ __launch_bounds__(NT) void k_svdj_round(const tnx_svdj_job* __restric
   pair_of(nb, round, p, ba, bb);
   double* B = (double*)J.work;
   double* W = B + (int64_t)n * n;
+  const bool vo = values_only(J);
   double* Bs = lds;
   double* Ws = lds + (int64_t)w2 * n;
   // local column c: block ba for c < bw, block bb after; columns >= n are zero
@@ -94,7 +110,7 @@ __global__ __launch_bounds__(NT) void k_svdj_round(const tnx_svdj_job* __restric
     const int g = (c < bw ? ba * bw + c : bb * bw + (c - bw));
     for (int i = threadIdx.x; i < n; i += NT) {
       Bs[c * n + i] = (g < n) ? B[(int64_t)g * n + i] : 0.0;
-      Ws[c * n + i] = (g < n) ? W[(int64_t)g * n + i] : 0.0;
+      if (!vo) Ws[c * n + i] = (g < n) ? W[(int64_t)g * n + i] : 0.0;
     }
   }
   __shared__ int rotated;
@@ -103,7 +119,7 @@ __global__ __launch_bounds__(NT) void k_svdj_round(const tnx_svdj_job* __restric
   const int T = NT / bw;  // lanes per column pair: 8 .. 64, aligned groups of one wavefront
   const int k = threadIdx.x / T, l = threadIdx.x % T;
   const double eps = 2.220446049250313e-16 * (double)(n > 8 ? n : 8);
-  const double zero2 = eps * eps * B[2 * (int64_t)n * n + n];  // column norm^2 of numerical zero
+  const double zero2 = eps * eps * norms_of(J)[n];  // column norm^2 of numerical zero
   for (int ir = 0; ir < w2 - 1; ++ir) {
     int u, v;
     pair_of(w2, ir, k, u, v);
@@ -131,9 +147,11 @@ __global__ __launch_bounds__(NT) void k_svdj_round(const tnx_svdj_job* __restric
         const double xi = x[i], yi = y[i];
         x[i] = cs * xi - sn * yi;
         y[i] = sn * xi + cs * yi;
-        const double pi = wx[i], qi = wy[i];
-        wx[i] = cs * pi - sn * qi;
-        wy[i] = sn * pi + cs * qi;
+        if (!vo) {
+          const double pi = wx[i], qi = wy[i];
+          wx[i] = cs * pi - sn * qi;
+          wy[i] = sn * pi + cs * qi;
+        }
       }
       if (l == 0) rotated = 1;
     }
@@ -144,7 +162,7 @@ __global__ __launch_bounds__(NT) void k_svdj_round(const tnx_svdj_job* __restric
     if (g < n)
       for (int i = threadIdx.x; i < n; i += NT) {
         B[(int64_t)g * n + i] = Bs[c * n + i];
-        W[(int64_t)g * n + i] = Ws[c * n + i];
+        if (!vo) W[(int64_t)g * n + i] = Ws[c * n + i];
       }
   }
   if (threadIdx.x == 0 && rotated) atomicOr(flag, 1);
@@ -157,7 +175,7 @@ __global__ __launch_bounds__(NT) void k_svdj_norms(const tnx_svdj_job* __restric
   const int j = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
   if (j >= n) return;
   const double* B = (const double*)J.work;
-  double* su = (double*)J.work + 2 * (int64_t)n * n;
+  double* su = norms_of(J);
   double s = 0.0;
   for (int i = threadIdx.x & 63; i < n; i += 64) {
     const double b = B[(int64_t)j * n + i];
@@ -175,7 +193,7 @@ __global__ __launch_bounds__(NT) void k_svdj_sort(const tnx_svdj_job* __restrict
   if (j >= n) return;
   const double* B = (const double*)J.work;
   const double* W = B + (int64_t)n * n;
-  const double* su = W + (int64_t)n * n;
+  const double* su = norms_of(J);
   const double sj = su[j];
   int cnt = 0;
   for (int q = threadIdx.x; q < n; q += NT) {
@@ -187,11 +205,13 @@ __global__ __launch_bounds__(NT) void k_svdj_sort(const tnx_svdj_job* __restrict
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
   __syncthreads();
   const int rank = red[0] + red[1] + red[2] + red[3];
-  double* u = (double*)J.u;
-  double* v = (double*)J.v;
-  for (int i = threadIdx.x; i < n; i += NT) {
-    u[(int64_t)rank * n + i] = W[(int64_t)j * n + i];
-    v[(int64_t)rank * n + i] = B[(int64_t)j * n + i];
+  if (!values_only(J)) {
+    double* u = (double*)J.u;
+    double* v = (double*)J.v;
+    for (int i = threadIdx.x; i < n; i += NT) {
+      u[(int64_t)rank * n + i] = W[(int64_t)j * n + i];
+      v[(int64_t)rank * n + i] = B[(int64_t)j * n + i];
+    }
   }
   if (threadIdx.x == 0) ((double*)J.s)[rank] = sj;
 }

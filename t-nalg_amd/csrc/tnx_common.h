@@ -13,6 +13,21 @@
 // thread-local message of the last failure (tnx_last_error)
 void tnx_set_error(const char* fmt, ...);
 
+// sampled kernel timing (prof.hip): a Lanczos call asks whether it is
+// sampled, switches timing on for its first Krylov step; launch sites bracket
+// their kernel with mark/done when timing is on.
+enum {
+  TNX_PC_GEMM_F64_1 = 0, TNX_PC_GEMM_F64_2, TNX_PC_GEMM_F32_1, TNX_PC_GEMM_F32_2,
+  TNX_PC_GEMM_BF16_1, TNX_PC_GEMM_BF16_2, TNX_PC_MIX, TNX_PC_DOTS, TNX_PC_UPDATE_DOTS,
+  TNX_PC_UPDATE_NORM
+};
+bool tnx_prof_call_begin();      // this call is sampled (timing stays off)
+void tnx_prof_set(bool on);      // timing on / off for the calling thread
+bool tnx_prof_on();
+int tnx_prof_mark(int cls, hipStream_t s);                // -1: not timed
+void tnx_prof_done(int mark, hipStream_t s, double work);
+void tnx_prof_count(int cls);   // one launch of the class (counted while armed)
+
 #define TNX_CHECK_HIP(call)                                                      \
   do {                                                                           \
     hipError_t e_ = (call);                                                      \

@@ -50,13 +50,19 @@ SVDJ_NMAX = 1024
 OK, EINVAL, ENOCONV = 0, -1, -3   # tnx_status values
 
 
-def svdj_blocking(n):
+def svdj_blocking(n, values_only=False):
     """(bw, nblk) of tnx_svd_block_jacobi for order n: bw the largest power of
     two <= min(32, 4096 / n) (at least 4), nblk = ceil(n / bw) rounded up to
-    even."""
-    bw = 32
-    while bw > 4 and bw * n > 4096:
-        bw //= 2
+    even.  Values-only jobs (no W in LDS): bw <= min(64, 8192 / n) and at most
+    n / 2, so an order up to 128 is one block pair."""
+    if values_only:
+        bw = 64
+        while bw > 4 and (bw * n > 8192 or 2 * bw > max(n, 8)):
+            bw //= 2
+    else:
+        bw = 32
+        while bw > 4 and bw * n > 4096:
+            bw //= 2
     nblk = -(-n // bw)
     return bw, nblk + (nblk % 2)
 CHOL_NMAX = 128
@@ -76,7 +82,19 @@ class MatvecPlan(ctypes.Structure):
                 ('mix_njob', ctypes.c_int32), ('mix_nblocks', ctypes.c_int32),
                 ('g2_prob', ctypes.c_void_p), ('g2_seg', ctypes.c_void_p),
                 ('g2_nprob', ctypes.c_int32), ('g2_ntiles', ctypes.c_int32),
-                ('ops', ctypes.c_void_p), ('mode', ctypes.c_int32), ('mode2', ctypes.c_int32)]
+                ('ops', ctypes.c_void_p), ('mode', ctypes.c_int32), ('mode2', ctypes.c_int32),
+                ('work', ctypes.c_double * 3)]
+
+
+class ProfStat(ctypes.Structure):
+    """One kernel class of the sampled kernel timing (tnx_prof_stat)."""
+    _fields_ = [('name', ctypes.c_char * 48), ('launches', ctypes.c_int64),
+                ('samples', ctypes.c_int64),
+                ('seconds', ctypes.c_double), ('work', ctypes.c_double),
+                ('unit', ctypes.c_int32), ('reserved', ctypes.c_int32)]
+
+
+PROF_NCLASS = 10
 
 
 # exported symbol -> (restype, argtypes)
@@ -103,6 +121,8 @@ SIGNATURES = {
     'tnx_chol_inv_f64': (_i, [_vp, _i, _vp]),
     'tnx_svd_block_jacobi': (_i, [_vp, _i, _i, _i, _i, _vp, _i, _vp]),
     'tnx_trace_dot': (_i, [_vp, _i, _vp, _vp]),
+    'tnx_profile_begin': (_i, [_i, _i]),
+    'tnx_profile_end': (_i, [ctypes.POINTER(ProfStat), _i]),
 }
 
 _LIB = None
@@ -131,7 +151,7 @@ def load(path=None):
     want = [GEMM_SEG.itemsize, GEMM_PROBLEM.itemsize, MIX_JOB.itemsize, MIX_TERM.itemsize,
             ctypes.sizeof(MatvecPlan), LANCZOS_SOLVE.itemsize, LANCZOS_RESULT.itemsize,
             QR_JOB.itemsize, SVD_JOB.itemsize, DOT_JOB.itemsize, CHOL_JOB.itemsize,
-            SVDJ_JOB.itemsize]
+            SVDJ_JOB.itemsize, ctypes.sizeof(ProfStat)]
     got = list(sizes[:n])
     if got != want:
         raise RuntimeError('ABI size mismatch: library %s, binding %s' % (got, want))

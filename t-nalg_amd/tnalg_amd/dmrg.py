@@ -289,10 +289,14 @@ def schedule_spectrum(solve):
     earlier spectra in place, in the reference and here).
 
     The reference makes every move by SVD, so the bonds it crosses twice get
-    their spectrum twice, the same Schmidt values of the same state.  Here the
-    middle leg (``p0`` to ``p1 + 1``, every bond once) moves by SVD and the
-    other two by QR: the spectra are those of the reference's middle leg and
-    the state is the same, up to the gauge of the tensors off the centre."""
+    their spectrum twice, the same Schmidt values of the same state.  Here
+    every move is a QR: on the middle leg (``p0`` to ``p1 + 1``, every bond
+    once) each move keeps its R factor (``qmove``), and one ``spectra`` step
+    takes the singular values of all of them at once (the Schmidt values: the
+    centre is Q R with Q an isometry); bonds whose R is not square or of
+    order below 8 move by SVD instead.  The spectra are those of the
+    reference's middle leg and the state is the same, up to the gauge of the
+    tensors off the centre."""
     from .engine import Step
     c = solve.center
     p0, p1 = solve.n - 1, 0
@@ -300,14 +304,23 @@ def schedule_spectrum(solve):
         if solve.lm[b].size == 0:
             p0, p1 = min(p0, b), max(p1, b)
     cur = c
+    runs = []   # contiguous ranges of bonds whose R is kept
     for leg, tgt in enumerate((p0, p1 + 1, c)):
-        kind = 'svdmove' if leg == 1 else 'move'
         while cur < tgt:
-            solve.queue.append(Step(kind, cur, +1))
+            if leg == 1 and solve.qmove_ok(cur):
+                solve.queue.append(Step('qmove', cur, +1))
+                if runs and runs[-1][1] == cur - 1:
+                    runs[-1][1] = cur
+                else:
+                    runs.append([cur, cur])
+            else:
+                solve.queue.append(Step('svdmove' if leg == 1 else 'move', cur, +1))
             cur += 1
         while cur > tgt:
-            solve.queue.append(Step(kind, cur, -1))
+            solve.queue.append(Step('svdmove' if leg == 1 else 'move', cur, -1))
             cur -= 1
+    for b0, b1 in runs:
+        solve.queue.append(Step('spectra', b0, b1))
 
 
 def dmrg_finite_size(para=None, seed=None, precision='fp64', ncv=20, engine=None, verbose=False):

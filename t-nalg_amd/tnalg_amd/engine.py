@@ -21,6 +21,7 @@ f64.
 """
 import collections
 import concurrent.futures
+import ctypes
 import hashlib
 import itertools
 import time
@@ -166,6 +167,12 @@ def max_dims(dims, d, ob):
     return [max(a, b) for a, b in zip(dims, cd)]
 
 
+def rsv_stride(dsq, dmx):
+    """Doubles per bond slot of the saved R factors (a values-only SVD work
+    area: n^2 + n + 1 doubles), rounded to 256 bytes."""
+    return (dsq + dmx + 1 + 31) // 32 * 32
+
+
 def buffer_layout(T, dm, d, ncv, lowp):
     """Sizes of every device buffer of a solve (elements for f64 buffers,
     bytes for the ``uint8`` ones); also the HBM footprint used for admission.
@@ -204,6 +211,9 @@ def buffer_layout(T, dm, d, ncv, lowp):
         'qrw': 2 * psi + dsq,
         'chol': 6 * dsq + dmx + 64,
         'lmv': max(1, n - 1) * dmx,   # entanglement spectra, one row of dmx per bond
+        # R factors of the QR moves of a spectrum pass, one slot per bond,
+        # each the work area of a values-only SVD (n^2 + n + 1 doubles)
+        'rsv': max(1, n - 1) * rsv_stride(dsq, dmx),
     }
     # arena: (name, index, bytes, f64 view?) in address order.  The MPS and
     # the CholeskyQR constants come first (staged from the host in one
@@ -224,7 +234,7 @@ def buffer_layout(T, dm, d, ncv, lowp):
              ('V', None, 8 * lay['V'], True), ('w', None, 8 * psi, True),
              ('x', None, 8 * psi, False), ('init', None, 8 * psi, True),
              ('Rq', None, 8 * lay['Rq'], True), ('qrw', None, 8 * lay['qrw'], True),
-             ('lmv', None, 8 * lay['lmv'], True)]
+             ('lmv', None, 8 * lay['lmv'], True), ('rsv', None, 8 * lay['rsv'], True)]
     if T.pairs:
         spec.append(('th', None, 8 * psi, True))
     offs, o = [], ARENA_HEADER
@@ -334,9 +344,12 @@ def random_mps_host(n, d, chi, seed):
 
 class Step:
     """One schedule entry of a solve: kind in {'move', 'solve', 'svdmove',
-    'resume', 'solve2', 'split2', 'resume2'}, site, direction (+1: centre moves right, -1: left).  'resume'
-    continues the local eigensolve of a 'solve' step from its Ritz vector
-    (the MPS site tensor) when one Krylov cycle did not converge."""
+    'qmove', 'spectra', 'resume', 'solve2', 'split2', 'resume2'}, site,
+    direction (+1: centre moves right, -1: left).  'resume' continues the
+    local eigensolve of a 'solve' step from its Ritz vector (the MPS site
+    tensor) when one Krylov cycle did not converge; 'qmove' is a right QR
+    move that keeps its R factor; 'spectra' (site = first bond, direction =
+    last bond) takes the singular values of the kept factors."""
     __slots__ = ('kind', 'site', 'dir', 'sweep')
 
     def __init__(self, kind, site, direction, sweep=-1):
@@ -579,23 +592,27 @@ class Solve:
         every solve of the engine with the same signature: device addresses
         are relative to the solve's arena and operator indices to its slot of
         the operator table (``Solve.part`` moves a phase to a solve)."""
-        key = (self.sig, st.kind, st.site, st.dir, tuple(self.dims), prec)
+        dims = self.dims if type(self.dims) is tuple else tuple(self.dims)
+        key = (self.sig, st.kind, st.site, st.dir, dims, prec)
         cache = self.engine.plan_cache
         hit = cache.get(key)
         if hit is not None:
-            self.dims = list(hit['dims_after'])
+            # bond dimensions after the step, shared (immutable) with the plan
+            self.dims = hit['dims_after_t']
             return hit
+        self.dims = list(dims)   # the builders update bond dimensions in place
         lo, hi = self.base, self.base + self.layout['bytes']
         if st.kind not in ('solve', 'solve2'):
             ph = self._build_plan(st, prec)
             R.pack_relative(ph, lo, hi, self.ops_slot)
+            ph['dims_after_t'] = tuple(ph['dims_after'])
             cache[key] = ph
             return ph
         # a solve step = the gauge move (two-site: split of the previous
         # pair) and environment update, which do not depend on the matvec
         # precision (built once and shared by the plans of every precision),
         # + the precision's cast and local records
-        ckey = (self.sig, st.kind + '-env', st.site, st.dir, tuple(self.dims))
+        ckey = (self.sig, st.kind + '-env', st.site, st.dir, dims)
         com = cache.get(ckey)
         if com is None:
             com = R.Plan()
@@ -610,6 +627,7 @@ class Solve:
         R.pack_relative(loc, lo, hi, self.ops_slot)
         meta = {k: loc[k] for k in ('prec', 'mv_flops')}
         meta['dims_after'] = list(self.dims)
+        meta['dims_after_t'] = tuple(self.dims)
         meta['uid'] = next(_PLAN_UID)
         ph = R.merge_packed([com, loc], meta)
         cache[key] = ph
@@ -653,12 +671,19 @@ class Solve:
             self._plan_local(ph, st.site, prec, pair=st.kind == 'solve2')
 
     def _build_plan(self, st, prec):
+        self.dims = list(self.dims)   # the builders update bond dimensions in place
         ph = R.Plan()
         if st.kind in ('move', 'solve', 'solve2', 'split2'):
             self._plan_common(ph, st)
             self._plan_precision(ph, st, prec)
         elif st.kind == 'svdmove':
             self._plan_svd(ph, st.site, st.dir)
+        elif st.kind == 'qmove':
+            # a right QR move that keeps its R factor for the spectrum
+            self._plan_common(ph, Step('move', st.site, st.dir, st.sweep))
+            self._plan_save_r(ph, st.site)
+        elif st.kind == 'spectra':
+            self._plan_spectra(ph, st.site, st.dir)
         elif st.kind == 'resume':
             # start vector <- the Ritz vector the last cycle wrote to the site
             # (the same vector, and so the same normalised V[0], as a restart
@@ -1049,6 +1074,16 @@ class Solve:
         ph['cast'] = mx.arrays()
 
     def cast_all_records(self, ti):
+        """Mixer records copying every environment block to its ``ti`` copy,
+        kept by the engine per (arena, layout, dtype, bond dimensions): a
+        later solve on the same arena with the same shape reuses them."""
+        key = (self.base, self.sig, ti, tuple(self.dims))
+        cache = self.engine._cast_cache
+        hit = cache.get(key)
+        if hit is not None:
+            return hit
+        if len(cache) > 4096:
+            cache.clear()
         mx = R.MixList()
         for b in range(self.n + 1):
             D = self.dims[b]
@@ -1056,7 +1091,8 @@ class Solve:
                 mx.add(self.L_lp(b, m, ti), ti, D, 1, D, [(self.L(b, m), L.F64, 1.0, -1)])
             for m in range(1 + self.nR[b]):
                 mx.add(self.R_lp(b, m, ti), ti, D, 1, D, [(self.Rr(b, m), L.F64, 1.0, -1)])
-        return mx.arrays()
+        cache[key] = mx.arrays()
+        return cache[key]
 
     def _plan_local(self, ph, p, prec, pair=False):
         """Matvec records and the Lanczos entry of the local eigensolve at
@@ -1193,6 +1229,39 @@ class Solve:
         ph['absorb'] = g.arrays()
         ph['copy'] = mx.arrays()
 
+    def _rsv_slot(self, b):
+        return _p(self.rsv, b * rsv_stride(self.dsq_max, self.dmax_max))
+
+    def qmove_ok(self, q):
+        """A right move at site q can keep its R for the spectrum: R is square
+        (the bond dimension does not shrink) and of order at least 8 (the
+        block Jacobi's smallest); other bonds move by SVD."""
+        n = self.dims[q + 1]
+        return n >= 8 and self.dims[q] * self.d >= n
+
+    def _plan_save_r(self, ph, q):
+        """The n x n factor R of the right QR move at site q (row-major, n the
+        bond dimension, ``qmove_ok``) into the slot of bond q: the bond's
+        Schmidt values are its singular values (A_q = Q R, Q an isometry)."""
+        n = self.dims[q + 1]
+        assert self.dims[q] * self.d >= n, 'a saved R factor must be square'
+        mx = R.MixList()
+        mx.add(self._rsv_slot(q), L.F64, 1, 1, n * n, [(self.Rq.data_ptr(), L.F64, 1.0, -1)])
+        ph['rsave'] = mx.arrays()
+
+    def _plan_spectra(self, ph, b0, b1):
+        """Singular values (values-only block Jacobi, one job per bond) of the
+        R factors saved for bonds b0..b1 into the spectrum rows of those
+        bonds (their orders: the bond dimensions)."""
+        jobs = []
+        for b in range(b0, b1 + 1):
+            n = self.dims[b + 1]
+            bw, nblk = L.svdj_blocking(n, values_only=True)
+            slot = self._rsv_slot(b)
+            jobs.append((slot, slot, 0, 0, _p(self.lmv, b * self.dmax_max), n, n, bw, nblk, 0, 0))
+        ph['svdj'] = np.array(jobs, dtype=L.SVDJ_JOB)
+        ph['lm_bonds'] = [(b, self.dims[b + 1]) for b in range(b0, b1 + 1)]
+
     def _plan_svd_block(self, ph, q, dirn, m, nn, lm_ptr):
         """A = Q R (CholeskyQR3 of the panel, Q in place of A_q), R = u^T v
         (``tnx_svd_block_jacobi``), then A_q <- Q u^T and the neighbour
@@ -1249,11 +1318,13 @@ def assemble_pre(pairs):
     """Descriptor arrays (one launch per phase over the batch) and launch
     metadata of the pre-Lanczos phases of one step of every ``(solve, plan)``.
     Array order: qr, svd, 3 x (gemm problems, segments), 4 x (mix jobs,
-    terms), the CholeskyQR stages, then the SVD-move records."""
+    terms), the CholeskyQR stages, then the SVD-move records.  (The fifth mix
+    phase saves R factors for the spectrum, after the CholeskyQR stages.)"""
     qr = R.concat_plain(_parts(pairs, 'qr'), L.QR_JOB)
     svd = R.concat_plain(_parts(pairs, 'svd'), L.SVD_JOB)
     gemms = [R.concat_gemm(_parts(pairs, k)) for k in ('absorb', 'env_g1', 'env_gT')]
-    mixes = [R.concat_mix(_parts(pairs, k)) for k in ('copy', 'env_lc', 'env_mix', 'cast')]
+    mixes = [R.concat_mix(_parts(pairs, k))
+             for k in ('copy', 'env_lc', 'env_mix', 'cast', 'rsave')]
     arrs = [qr, svd]
     for p, sg, _ in gemms:
         arrs += [p, sg]
@@ -1347,7 +1418,9 @@ def assemble_lanczos(groups):
         g2p, g2s, g2t = R.concat_gemm([s.part(ph, 'mv_g2') for s, ph in g], actives)
         _, _, mode1, mode2, _ = PREC[g[0][1]['prec']]
         gmeta.append((mode1, mode2, (len(g1p), g1t, gemm_kmax(g1p, g1s)), (len(mxj), mxb),
-                      (len(g2p), g2t, gemm_kmax(g2p, g2s))))
+                      (len(g2p), g2t, gemm_kmax(g2p, g2s)),
+                      # algorithmic work of the three launches (all solves active)
+                      (R.gemm_flops(g1p, g1s), R.mix_bytes(mxj, mxt), R.gemm_flops(g2p, g2s))))
         arrs += [g1p, g1s, mxj, mxt, g2p, g2s]
     meta = {'lc': (len(lc_j), lc_nb), 'nchunks': int(chunks.sum()), 'groups': gmeta}
     return arrs, meta
@@ -1399,6 +1472,7 @@ class Engine:
         self._resident_bytes = 0
         self.resident_limit = 1 << 30   # descriptor sets of every distinct step of a sweep
         self.plan_cache = {}
+        self._cast_cache = {}   # (arena, layout, dtype, dims) -> block cast records
         self.svd_flag = torch.zeros(64, dtype=torch.int32, device=self.device)  # rotation flag
         self._arena_pool = {}   # bytes -> freed arenas (reused by the next solve of that size)
         self.counters = {'matvec_flops': 0.0, 'matvecs': 0, 'lanczos_calls': 0, 'steps': 0,
@@ -1768,13 +1842,13 @@ class Engine:
                                      meta['lc'][1], ctypes_void(self.ops.data_ptr()), None,
                                      self.stream()), 'tnx_mix')
         plans = (L.MatvecPlan * len(groups))()
-        for q, (mode1, mode2, g1, mx, g2) in enumerate(meta['groups']):
+        for q, (mode1, mode2, g1, mx, g2, work) in enumerate(meta['groups']):
             b = 3 + 6 * q
             m1, m2 = self._tuned_gemm(mode1, ptrs[b], ptrs[b + 1], g1), \
                 self._tuned_gemm(mode2, ptrs[b + 4], ptrs[b + 5], g2)
             plans[q] = L.MatvecPlan(ptrs[b], ptrs[b + 1], g1[0], g1[1], ptrs[b + 2], ptrs[b + 3],
                                     mx[0], mx[1], ptrs[b + 4], ptrs[b + 5], g2[0], g2[1],
-                                    self.ops.data_ptr(), m1, m2)
+                                    self.ops.data_ptr(), m1, m2, (ctypes.c_double * 3)(*work))
         need = int(self.lib.tnx_lanczos_state_bytes(len(flat), meta['nchunks'], self.ncv))
         if self.lz_states[k] is None or self.lz_states[k].numel() < need:
             self.lz_states[k] = torch.empty(max(need, 1 << 20), dtype=torch.uint8,
@@ -1805,6 +1879,7 @@ class Engine:
             return 0
         t_a = time.perf_counter()
         plans = []
+        casts = []   # low-precision copies of every block, for solves changing precision
         for s in solves:
             st = s.queue.pop(0)
             s._lz_site = st.site
@@ -1815,13 +1890,15 @@ class Engine:
             else:
                 prec, ph = self._prec_of(s, st), None
             if self.uses_lowp and prec != 'fp64' and s.lowp_dtype != PREC[prec][0]:
-                self._mix([s.cast_all_records(PREC[prec][0])])
+                casts.append(s.cast_all_records(PREC[prec][0]))
                 s.lowp_dtype = PREC[prec][0]
             if ph is None:
                 ph = s.plan(st, prec)
             else:
-                s.dims = list(ph['dims_after'])
+                s.dims = ph['dims_after_t']
             plans.append((s, st, ph))
+        if casts:
+            self._mix(casts)   # one launch for all of them
 
         t_b = time.perf_counter()
         # every descriptor of the step's pre-Lanczos phases, resident on the
@@ -1867,6 +1944,7 @@ class Engine:
                                         opsp, None, st_), 'cq mix')
             elif cnt:
                 L.check(lib.tnx_chol_inv_f64(ctypes_void(ptr[ai]), cnt, st_), 'tnx_chol_inv_f64')
+        run_mix(4)       # R factors kept for the spectrum (qmove steps)
         if meta['svdj'] is not None:   # SVD of the CholeskyQR factors, then apply u and v
             nj, npairs, max_nblk, max_n, ai, (gp, gnt, gi, gk), (mj, mnb, mi) = meta['svdj']
             rc = lib.tnx_svd_block_jacobi(ctypes_void(ptr[ai]), nj, npairs, max_nblk, max_n,
@@ -1921,12 +1999,15 @@ class Engine:
                 s.queue.insert(0, Step(kind, st.site, st.dir, st.sweep))
             if st.kind in ('solve', 'resume', 'solve2', 'resume2'):
                 s.center = st.site
-            elif st.kind in ('move', 'svdmove'):
+            elif st.kind in ('move', 'svdmove', 'qmove'):
                 s.center = st.site + st.dir
             elif st.kind == 'split2':
                 s.center = st.site + (st.dir > 0)
             if 'lm_bond' in ph:   # spectrum written to lmv on the device; read lazily
                 s._lm_pending[ph['lm_bond']] = ph['svd_n']
+            if 'lm_bonds' in ph:
+                for b, k in ph['lm_bonds']:
+                    s._lm_pending[b] = k
         self.counters['steps'] += 1
         return len(plans)
 

@@ -79,6 +79,38 @@ def gemm_tiles(p):
     return int(last['tile_begin'] + ((last['M'] + 63) // 64) * ((last['N'] + 63) // 64))
 
 
+def gemm_flops(p, s):
+    """Algorithmic flops of a grouped-GEMM launch: 2 M N K summed over the
+    problems and their K segments."""
+    if len(p) == 0:
+        return 0.0
+    if len(s) == 0:
+        return 0.0
+    # (a problem without segments may start at len(s); it is masked below)
+    ksum = np.add.reduceat(s['K'].astype(np.float64), np.minimum(p['seg_begin'], len(s) - 1))
+    ksum = np.where(p['seg_count'] > 0, ksum, 0.0)
+    return float((2.0 * p['M'].astype(np.float64) * p['N'] * ksum).sum())
+
+
+_ITEM = {0: 8, 1: 4, 2: 2}   # dtype -> bytes
+
+
+def mix_bytes(j, t):
+    """Algorithmic bytes of a mixer launch: every term's source and the
+    destination (twice when accumulated) once per element."""
+    if len(j) == 0:
+        return 0.0
+    el = j['Dl'].astype(np.float64) * j['d'] * j['Dr']
+    isz = np.vectorize(_ITEM.get)
+    dst = isz(j['dst_dtype']) * (1 + j['accumulate'])
+    src = np.zeros(len(j))
+    if len(t):
+        ts = isz(t['src_dtype']).astype(np.float64)
+        src = np.add.reduceat(ts, np.minimum(j['term_begin'], len(ts) - 1))
+        src = np.where(j['term_count'] > 0, src, 0.0)
+    return float((el * (dst + src)).sum())
+
+
 def mix_blocks(j):
     if len(j) == 0:
         return 0
@@ -205,7 +237,8 @@ def _rel_one(arr, lo, hi, slot):
         o[o >= 0] -= slot
 
 
-PLAN_META = ('dims_after', 'uid', 'prec', 'mv_flops', 'svd_n', 'lm_bond', 'packed')
+PLAN_META = ('dims_after', 'dims_after_t', 'uid', 'prec', 'mv_flops', 'svd_n', 'lm_bond',
+             'lm_bonds', 'packed')
 
 
 def make_relative(ph, lo, hi, slot):

@@ -402,6 +402,58 @@ def test_lanczos_dense_symmetric(gpu):
         assert np.linalg.norm(H @ out - e0 * out) < 1e-6 * max(1, abs(e0))
 
 
+def test_profile_times_the_first_krylov_step(gpu):
+    """Sampled kernel timing around one Lanczos call (every call sampled):
+    each matvec GEMM, k_dots, k_update_dots and k_update_norm is timed once
+    (the first Krylov step), with its algorithmic work; launches are counted
+    for every step issued (the host polls one step behind, so one step may
+    be issued after the last one the solve needed)."""
+    import torch
+    from tnalg_amd import _lib as L
+    from tnalg_amd import records as R
+    lib = L.load()
+    rng = np.random.RandomState(4)
+    n, ncv = 3000, 20
+    A = rng.randn(n, n)
+    H = (A + A.T) / 2 + np.diag(np.linspace(0, 50, n))
+    Hd = _dev(H)
+    V = torch.zeros((ncv + 1) * n, dtype=torch.float64, device='cuda')
+    w, x = (torch.zeros(n, dtype=torch.float64, device='cuda') for _ in range(2))
+    init, out = _dev(rng.randn(n)), torch.zeros(n, dtype=torch.float64, device='cuda')
+    g2 = R.GemmList()
+    g2.add(w.data_ptr(), n, 1, 1, [(Hd.data_ptr(), n, 0, x.data_ptr(), 1, 0, n, 1.0)], active=0)
+    lz = np.array([(V.data_ptr(), w.data_ptr(), x.data_ptr(), init.data_ptr(), out.data_ptr(), n,
+                    n, 0, 0, 1e-10 / 1e-4, 1e-4, 0.0, 0, 0)], dtype=L.LANCZOS_SOLVE)
+    nch = -(-n // L.LZ_CHUNK)
+    p, s = g2.arrays()
+    dp, ds, dlz = _upload([p, s, lz])
+    flops = 2.0 * n * n
+    plan = L.MatvecPlan(0, 0, 0, 0, 0, 0, 0, 0, dp.data_ptr(), ds.data_ptr(), len(p),
+                        R.gemm_tiles(p), 0, 0, 0, (ctypes.c_double * 3)(0.0, 0.0, flops))
+    state = torch.zeros(int(lib.tnx_lanczos_state_bytes(1, nch, ncv)), dtype=torch.uint8,
+                        device='cuda')
+    res = np.zeros(1, dtype=L.LANCZOS_RESULT)
+    print('profiled lanczos n', n, flush=True)
+    assert lib.tnx_profile_begin(1, 64) == 0
+    rc = lib.tnx_lanczos_run_ex(plan, _vp(dlz.data_ptr()), 1, nch, ncv, 10, 1, 0, 0,
+                                _vp(state.data_ptr()), _vp(res.ctypes.data), _stream())
+    stats = (L.ProfStat * L.PROF_NCLASS)()
+    assert lib.tnx_profile_end(stats, L.PROF_NCLASS) == L.PROF_NCLASS
+    assert rc == 0
+    by = {st.name.decode(): st for st in stats}
+    steps = int(res[0]['iters'])
+    g = by['gemm f64 matvec w = U R^T']
+    assert g.samples == 1 and steps <= g.launches <= steps + 1
+    assert g.work == flops and g.seconds > 0
+    assert by['gemm f64 matvec T = F x'].samples == 1     # an empty GEMM 1, still timed
+    for k, words in (('k_dots', 8 + 8), ('k_update_dots', 8 + 8 + 8),
+                     ('k_update_norm', 8 + 16 + 8)):
+        st = by[k]
+        assert st.samples == 1 and st.launches == g.launches and st.unit == 1, k
+        assert st.work == nch * L.LZ_CHUNK * words, (k, st.work)
+    assert by['mixer matvec'].samples == 0
+
+
 def test_lanczos_multi_precision(gpu):
     """One tnx_lanczos_run_multi call over solves whose matvecs run at two
     precisions (an f64 and an f32 matvec plan, per-solve x/w dtypes): every

@@ -97,7 +97,7 @@ def test_relocated_plan_equals_own_plan(lattice, kw, kw_b, prec):
         own = b._build_plan(st, prec)                  # b's addresses and coefficients
         b.dims = dims0
         assert b.plan(st, prec) is ph_rel              # cache hit for b
-        assert b.dims == own['dims_after']
+        assert list(b.dims) == own['dims_after']
         ra, rb = _records(ph_rel), _records(own)
         assert [p for p, _ in ra] == [p for p, _ in rb]
         for (path, rel), (_, absr) in zip(ra, rb):
@@ -317,13 +317,13 @@ def test_plan_ahead_matches_inline_planning():
     while all(s.queue for s in solves):
         dims0 = [list(s.dims) for s in solves]
         spec = eng._plan_ahead(solves)
-        assert [s.dims for s in solves] == dims0
+        assert [list(s.dims) for s in solves] == dims0
         pairs = []
         for s in solves:
             st, prec, ph = s._next
             assert st is s.queue[0] and prec == eng._prec_of(s, st)
             assert s.plan(st, prec) is ph          # inline planning: the same shared plan
-            assert s.dims == list(ph['dims_after'])
+            assert list(s.dims) == list(ph['dims_after'])
             pairs.append((s, ph))
             s.queue.pop(0)
             s._next = None

@@ -147,7 +147,7 @@ def test_library_abi_struct_sizes():
     lib = _lib.load()
     sizes = (ctypes.c_int32 * 16)()
     n = lib.tnx_abi_sizes(sizes, 16)
-    assert list(sizes[:n]) == [48, 56, 48, 24, 88, 88, 24, 56, 64, 32, 56, 64]
+    assert list(sizes[:n]) == [48, 56, 48, 24, 112, 88, 24, 56, 64, 32, 56, 64, 88]
 
 
 def test_easystart_coefficient_convention():
@@ -200,3 +200,14 @@ def test_initial_mps_is_the_reference_draw():
     a = random_mps_host(6, 2, 4, r1)
     b, _ = tb.random_open_mps(6, 2, 4, r2)
     assert all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def test_library_profile_argument_checks():
+    """The sampled-timing entry points check their arguments before any
+    device call (so this runs without a GPU)."""
+    from tnalg_amd import _lib
+    lib = _lib.load()
+    assert lib.tnx_profile_begin(0, 16) == _lib.EINVAL
+    assert lib.tnx_profile_begin(4, 0) == _lib.EINVAL
+    assert lib.tnx_profile_end(None, 0) == _lib.EINVAL   # not armed
+    assert b'not armed' in lib.tnx_last_error()
