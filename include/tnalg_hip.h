@@ -84,8 +84,9 @@ typedef struct tnx_mix_job {
   int32_t term_begin, term_count;
   int32_t accumulate;  /* 1: add into dst */
   int32_t active;      /* index into the active-flag array, or -1 */
-  int32_t block_begin; /* prefix sum over jobs of ceil(Dl * Dr / 512) workgroups
-                          (a workgroup covers 512 (a, b) pairs, every s) */
+  int32_t block_begin; /* prefix sum over jobs of ceil(Dl * Dr / P) workgroups,
+                          P = 1024 for d <= 2, 512 for d > 2 (a workgroup
+                          covers P (a, b) pairs, every s) */
   int32_t reserved;
 } tnx_mix_job; /* 48 bytes */
 
@@ -341,6 +342,16 @@ typedef struct tnx_dot_job {
   int32_t ldx, ldy;
 } tnx_dot_job; /* 32 bytes */
 int tnx_trace_dot(const tnx_dot_job* jobs, int njob, double* out, tnx_stream_t stream);
+
+/* Batched in-place normalisation: x_i /= ||x_i||_2 over n_i f64 elements
+ * (a zero vector is left as it is); one workgroup per job.  Replaces the
+ * normalize=True of orthogonalize_mps / norm_mps on the centre tensor in the
+ * time-evolution drivers (MPSClass.py:1421-1447, TEBDalgo.py:51-52).        */
+typedef struct tnx_norm_job {
+  uint64_t x;
+  int64_t n;
+} tnx_norm_job; /* 16 bytes */
+int tnx_normalize_f64(const tnx_norm_job* jobs, int njob, tnx_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -20,15 +20,91 @@ void tnx_set_error(const char* fmt, ...) {
 
 namespace {
 
-constexpr int MIX_PPT = 2;                // (a, b) pairs per lane
-constexpr int MIX_PAIRS = 256 * MIX_PPT;  // (a, b) index pairs per workgroup
-constexpr int MIX_DMAX = 4;               // largest physical dimension (the operator table holds 4 x 4)
+constexpr int MIX_DMAX = 4;   // largest physical dimension (the operator table holds 4 x 4)
 
-// One lane per (a, b) pair, MIX_PPT pairs per lane: the d source values
+// (a, b) pairs per lane and per workgroup of a job of physical dimension d:
+// jobs with d <= 2 (spin-1/2 sites) take 4 pairs per lane, larger d 2 (the
+// registers of a lane hold PPT x d sources and PPT x d sums either way).
+__host__ __device__ constexpr int mix_ppt(int d) { return d <= 2 ? 4 : 2; }
+
+// One lane per (a, b) pair, PPT pairs per lane: the d source values
 // src[a, :, b] of a term are loaded once and mixed by the d x d operator in
 // registers, so each output element costs one load per term.  Terms are the
 // outer loop: a term's record and operator are read once per lane, and the
 // loads of the lane's pairs are issued together.
+template <int DM>
+__device__ __forceinline__ void mix_body(const tnx_mix_job& J, int blk,
+                                         const tnx_mix_term* __restrict__ terms,
+                                         const double* __restrict__ ops) {
+  constexpr int PPT = mix_ppt(DM);
+  const int d = J.d, Dr = J.Dr;
+  const int npairs = J.Dl * Dr;
+  const int base = blk * 256 * PPT;
+  int64_t e0[PPT];  // element (a, 0, b) of each pair
+  bool ok[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    const int p = base + q * 256 + threadIdx.x;
+    ok[q] = p < npairs;
+    const int a = ok[q] ? p / Dr : 0;
+    e0[q] = ok[q] ? (int64_t)a * d * Dr + (p - a * Dr) : 0;
+  }
+  double acc[PPT][DM];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q)
+#pragma unroll
+    for (int s = 0; s < DM; ++s) acc[q][s] = 0.0;
+  for (int ti = J.term_begin; ti < J.term_begin + J.term_count; ++ti) {
+    const tnx_mix_term T = terms[ti];
+    double x[PPT][DM];
+#pragma unroll
+    for (int q = 0; q < PPT; ++q)
+#pragma unroll
+      for (int t = 0; t < DM; ++t)
+        x[q][t] = (ok[q] && t < d) ? tnx_load((const void*)T.src, T.src_dtype,
+                                              e0[q] + (int64_t)t * Dr) : 0.0;
+    if (T.op < 0) {
+#pragma unroll
+      for (int q = 0; q < PPT; ++q)
+#pragma unroll
+        for (int s = 0; s < DM; ++s) acc[q][s] += T.coef * x[q][s];
+    } else {
+      double O[DM * DM];
+      const double* Op = ops + (int64_t)T.op * d * d;
+#pragma unroll
+      for (int s = 0; s < DM; ++s)
+#pragma unroll
+        for (int t = 0; t < DM; ++t)
+          O[s * DM + t] = (s < d && t < d) ? Op[s * d + t] : 0.0;
+#pragma unroll
+      for (int q = 0; q < PPT; ++q)
+#pragma unroll
+        for (int s = 0; s < DM; ++s) {
+          if (s < d) {
+            double v = 0.0;
+#pragma unroll
+            for (int t = 0; t < DM; ++t)
+              if (t < d) v += O[s * DM + t] * x[q][t];
+            acc[q][s] += T.coef * v;
+          }
+        }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    if (!ok[q]) continue;
+#pragma unroll
+    for (int s = 0; s < DM; ++s) {
+      if (s < d) {
+        const int64_t e = e0[q] + (int64_t)s * Dr;
+        double v = acc[q][s];
+        if (J.accumulate) v += tnx_load((const void*)J.dst, J.dst_dtype, e);
+        tnx_store((void*)J.dst, J.dst_dtype, e, v);
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void mix_kernel(const tnx_mix_job* __restrict__ jobs, int njob,
                                                   const tnx_mix_term* __restrict__ terms,
                                                   const double* __restrict__ ops,
@@ -38,72 +114,10 @@ __global__ __launch_bounds__(256) void mix_kernel(const tnx_mix_job* __restrict_
                                 njob, bid);
   const tnx_mix_job J = jobs[ji];
   if (J.active >= 0 && active != nullptr && active[J.active] == 0) return;
-  const int d = J.d, Dr = J.Dr;
-  const int npairs = J.Dl * Dr;
-  const int base = (bid - J.block_begin) * MIX_PAIRS;
-  int64_t e0[MIX_PPT];  // element (a, 0, b) of each pair
-  bool ok[MIX_PPT];
-#pragma unroll
-  for (int q = 0; q < MIX_PPT; ++q) {
-    const int p = base + q * 256 + threadIdx.x;
-    ok[q] = p < npairs;
-    const int a = ok[q] ? p / Dr : 0;
-    e0[q] = ok[q] ? (int64_t)a * d * Dr + (p - a * Dr) : 0;
-  }
-  double acc[MIX_PPT][MIX_DMAX];
-#pragma unroll
-  for (int q = 0; q < MIX_PPT; ++q)
-#pragma unroll
-    for (int s = 0; s < MIX_DMAX; ++s) acc[q][s] = 0.0;
-  for (int ti = J.term_begin; ti < J.term_begin + J.term_count; ++ti) {
-    const tnx_mix_term T = terms[ti];
-    double x[MIX_PPT][MIX_DMAX];
-#pragma unroll
-    for (int q = 0; q < MIX_PPT; ++q)
-#pragma unroll
-      for (int t = 0; t < MIX_DMAX; ++t)
-        x[q][t] = (ok[q] && t < d) ? tnx_load((const void*)T.src, T.src_dtype,
-                                              e0[q] + (int64_t)t * Dr) : 0.0;
-    if (T.op < 0) {
-#pragma unroll
-      for (int q = 0; q < MIX_PPT; ++q)
-#pragma unroll
-        for (int s = 0; s < MIX_DMAX; ++s) acc[q][s] += T.coef * x[q][s];
-    } else {
-      double O[MIX_DMAX * MIX_DMAX];
-      const double* Op = ops + (int64_t)T.op * d * d;
-#pragma unroll
-      for (int s = 0; s < MIX_DMAX; ++s)
-#pragma unroll
-        for (int t = 0; t < MIX_DMAX; ++t)
-          O[s * MIX_DMAX + t] = (s < d && t < d) ? Op[s * d + t] : 0.0;
-#pragma unroll
-      for (int q = 0; q < MIX_PPT; ++q)
-#pragma unroll
-        for (int s = 0; s < MIX_DMAX; ++s) {
-          if (s < d) {
-            double v = 0.0;
-#pragma unroll
-            for (int t = 0; t < MIX_DMAX; ++t)
-              if (t < d) v += O[s * MIX_DMAX + t] * x[q][t];
-            acc[q][s] += T.coef * v;
-          }
-        }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < MIX_PPT; ++q) {
-    if (!ok[q]) continue;
-#pragma unroll
-    for (int s = 0; s < MIX_DMAX; ++s) {
-      if (s < d) {
-        const int64_t e = e0[q] + (int64_t)s * Dr;
-        double v = acc[q][s];
-        if (J.accumulate) v += tnx_load((const void*)J.dst, J.dst_dtype, e);
-        tnx_store((void*)J.dst, J.dst_dtype, e, v);
-      }
-    }
-  }
+  if (J.d <= 2)
+    mix_body<2>(J, bid - J.block_begin, terms, ops);
+  else
+    mix_body<MIX_DMAX>(J, bid - J.block_begin, terms, ops);
 }
 
 __global__ __launch_bounds__(256) void trace_dot_kernel(const tnx_dot_job* __restrict__ jobs,
@@ -129,7 +143,40 @@ __global__ __launch_bounds__(256) void trace_dot_kernel(const tnx_dot_job* __res
   if (threadIdx.x == 0) out[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// x /= ||x|| for one job per workgroup: the sum of squares in a fixed order
+// (lane-strided partials, then the wave and workgroup sums), then the scale.
+__global__ __launch_bounds__(1024) void normalize_kernel(const tnx_norm_job* __restrict__ jobs) {
+  const tnx_norm_job J = jobs[blockIdx.x];
+  double* x = (double*)J.x;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < J.n; i += 1024) acc += x[i] * x[i];
+  __shared__ double red[16];
+  __shared__ double scale;
+  acc = tnx_wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < 16; ++w) t += red[w];
+    scale = t > 0.0 ? 1.0 / sqrt(t) : 1.0;
+  }
+  __syncthreads();
+  const double sc = scale;
+  for (int64_t i = threadIdx.x; i < J.n; i += 1024) x[i] *= sc;
+}
+
 }  // namespace
+
+extern "C" int tnx_normalize_f64(const tnx_norm_job* jobs, int njob, tnx_stream_t stream) {
+  if (njob == 0) return TNX_OK;
+  if (jobs == nullptr || njob < 0) {
+    tnx_set_error("tnx_normalize_f64: bad arguments (njob=%d)", njob);
+    return TNX_EINVAL;
+  }
+  hipLaunchKernelGGL(normalize_kernel, dim3(njob), dim3(1024), 0, (hipStream_t)stream, jobs);
+  TNX_CHECK_LAUNCH();
+  return TNX_OK;
+}
 
 extern "C" int tnx_version(void) { return TNX_VERSION; }
 
@@ -142,7 +189,7 @@ extern "C" int tnx_abi_sizes(int32_t* sizes, int n) {
                          (int32_t)sizeof(tnx_lanczos_result), (int32_t)sizeof(tnx_qr_job),
                          (int32_t)sizeof(tnx_svd_job),       (int32_t)sizeof(tnx_dot_job),
                          (int32_t)sizeof(tnx_chol_job),      (int32_t)sizeof(tnx_svdj_job),
-                         (int32_t)sizeof(tnx_prof_stat)};
+                         (int32_t)sizeof(tnx_prof_stat),     (int32_t)sizeof(tnx_norm_job)};
   const int na = (int)(sizeof(all) / sizeof(all[0]));
   if (sizes == nullptr) return na;
   for (int i = 0; i < n && i < na; ++i) sizes[i] = all[i];

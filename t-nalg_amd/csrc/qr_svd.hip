@@ -320,7 +320,8 @@ __device__ __forceinline__ void chol_steps(double (&s)[8][8], double* rowbuf, do
       d0 = 1e-300;
       bad = 1;
     }
-    const double sq0 = sqrt(d0), ri0 = 1.0 / sq0;
+    // one reciprocal square root on the serial chain instead of sqrt + divide
+    const double ri0 = rsqrt(d0), sq0 = d0 * ri0;
     if (threadIdx.x == 0) rdiag[k] = ri0;
     double ci0[8], cj0[8];
 #pragma unroll
@@ -341,8 +342,8 @@ __device__ __forceinline__ void chol_steps(double (&s)[8][8], double* rowbuf, do
         d1 = 1e-300;
         bad = 1;
       }
-      sq1 = sqrt(d1);
-      ri1 = 1.0 / sq1;
+      ri1 = rsqrt(d1);
+      sq1 = d1 * ri1;
       if (threadIdx.x == 0) rdiag[k + 1] = ri1;
       double ci1[8], cj1[8];
 #pragma unroll

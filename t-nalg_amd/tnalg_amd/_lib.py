@@ -39,6 +39,7 @@ QR_JOB = np.dtype([('src', '<u8'), ('q', '<u8'), ('r', '<u8'), ('work', '<u8'),
 SVD_JOB = np.dtype([('src', '<u8'), ('u', '<u8'), ('s', '<u8'), ('svt', '<u8'), ('work', '<u8'),
                     ('m', '<i4'), ('n', '<i4'), ('ld', '<i4'), ('trans', '<i4'),
                     ('u_trans', '<i4'), ('reserved', '<i4')])
+NORM_JOB = np.dtype([('x', '<u8'), ('n', '<i8')])
 DOT_JOB = np.dtype([('x', '<u8'), ('y', '<u8'), ('rows', '<i4'), ('cols', '<i4'),
                     ('ldx', '<i4'), ('ldy', '<i4')])
 CHOL_JOB = np.dtype([('g', '<u8'), ('r', '<u8'), ('rinv', '<u8'), ('status', '<u8'),
@@ -68,7 +69,16 @@ def svdj_blocking(n, values_only=False):
 CHOL_NMAX = 128
 
 GEMM_TILE = 64
-MIX_PAIRS = 512    # (a, b) pairs per mixer workgroup
+MIX_PAIRS = 512    # (a, b) pairs per mixer workgroup for d > 2 (1024 for d <= 2)
+
+
+def mix_pairs(d):
+    """(a, b) pairs per mixer workgroup of a job of physical dimension ``d``
+    (scalar or array): 4 pairs per lane for d <= 2, 2 otherwise."""
+    import numpy as _np
+    return _np.where(_np.asarray(d) <= 2, 2 * MIX_PAIRS, MIX_PAIRS)
+
+
 MIX_DMAX = 4       # largest physical dimension of the mixer
 LZ_CHUNK = 1024
 LZ_NCV_MAX = 32
@@ -121,6 +131,7 @@ SIGNATURES = {
     'tnx_chol_inv_f64': (_i, [_vp, _i, _vp]),
     'tnx_svd_block_jacobi': (_i, [_vp, _i, _i, _i, _i, _vp, _i, _vp]),
     'tnx_trace_dot': (_i, [_vp, _i, _vp, _vp]),
+    'tnx_normalize_f64': (_i, [_vp, _i, _vp]),
     'tnx_profile_begin': (_i, [_i, _i]),
     'tnx_profile_end': (_i, [ctypes.POINTER(ProfStat), _i]),
 }
@@ -151,7 +162,7 @@ def load(path=None):
     want = [GEMM_SEG.itemsize, GEMM_PROBLEM.itemsize, MIX_JOB.itemsize, MIX_TERM.itemsize,
             ctypes.sizeof(MatvecPlan), LANCZOS_SOLVE.itemsize, LANCZOS_RESULT.itemsize,
             QR_JOB.itemsize, SVD_JOB.itemsize, DOT_JOB.itemsize, CHOL_JOB.itemsize,
-            SVDJ_JOB.itemsize, ctypes.sizeof(ProfStat)]
+            SVDJ_JOB.itemsize, ctypes.sizeof(ProfStat), NORM_JOB.itemsize]
     got = list(sizes[:n])
     if got != want:
         raise RuntimeError('ABI size mismatch: library %s, binding %s' % (got, want))

@@ -51,6 +51,36 @@ def _p(t, off_elems=0, itemsize=8):
     return t.data_ptr() + off_elems * itemsize
 
 
+class _ArenaViews:
+    """Per-index buffers of one kind (site tensors, environments) inside a
+    solve's arena.  Descriptors need only addresses (``ptr``); a tensor view
+    is made on first indexing and kept (a solve has hundreds of buffers and
+    its host code touches few of them as tensors)."""
+    __slots__ = ('arena', 'base', 'ent', 'cache')
+
+    def __init__(self, arena, n):
+        self.arena = arena
+        self.base = arena.data_ptr()
+        self.ent = [None] * n      # (offset, bytes, f64 view?)
+        self.cache = [None] * n
+
+    def __len__(self):
+        return len(self.ent)
+
+    def __getitem__(self, i):
+        v = self.cache[i]
+        if v is None:
+            off, nb, f64 = self.ent[i]
+            v = self.arena[off:off + nb]
+            if f64:
+                v = v.view(torch.float64)
+            self.cache[i] = v
+        return v
+
+    def ptr(self, i, off_elems=0, itemsize=8):
+        return self.base + self.ent[i][0] + off_elems * itemsize
+
+
 class _Stager:
     """Pinned host staging for uploads into persistent device buffers: a
     ring of pinned buffers that only grow, each reused once the copy out of
@@ -344,16 +374,23 @@ def random_mps_host(n, d, chi, seed):
 
 class Step:
     """One schedule entry of a solve: kind in {'move', 'solve', 'svdmove',
-    'qmove', 'spectra', 'resume', 'solve2', 'split2', 'resume2'}, site,
-    direction (+1: centre moves right, -1: left).  'resume' continues the
-    local eigensolve of a 'solve' step from its Ritz vector (the MPS site
-    tensor) when one Krylov cycle did not converge; 'qmove' is a right QR
-    move that keeps its R factor; 'spectra' (site = first bond, direction =
-    last bond) takes the singular values of the kept factors."""
-    __slots__ = ('kind', 'site', 'dir', 'sweep')
+    'qmove', 'spectra', 'resume', 'solve2', 'split2', 'resume2', 'tmove',
+    'gate2', 'tsplit', 'tnorm'}, site, direction (+1: centre moves right,
+    -1: left).  'resume' continues the local eigensolve of a 'solve' step
+    from its Ritz vector (the MPS site tensor) when one Krylov cycle did not
+    converge; 'qmove' is a right QR move that keeps its R factor; 'spectra'
+    (site = first bond, direction = last bond) takes the singular values of
+    the kept factors.  The time-evolution steps touch no environment:
+    'tmove' is a QR centre move; 'gate2' (site p) splits the previous pair
+    p - dir (when dir != 0) after normalising it, forms theta = A_p A_{p+1}
+    and applies the two-site gate (merged operator ``gate``); 'tsplit'
+    normalises and splits the last pair; 'tnorm' normalises the centre
+    tensor at the site."""
+    __slots__ = ('kind', 'site', 'dir', 'sweep', 'gate')
 
-    def __init__(self, kind, site, direction, sweep=-1):
+    def __init__(self, kind, site, direction, sweep=-1, gate=0):
         self.kind, self.site, self.dir, self.sweep = kind, site, direction, sweep
+        self.gate = gate
 
     def __repr__(self):
         return 'Step(%s,%d,%+d,sw=%d)' % (self.kind, self.site, self.dir, self.sweep)
@@ -362,10 +399,11 @@ class Step:
 class Solve:
     """Device state of one ground-state solve."""
 
-    def __init__(self, engine, para, seed=0, mps=None, tag=None, stage=None):
+    def __init__(self, engine, para, seed=0, mps=None, tag=None, stage=None, gates=0):
         """``mps``: initial site tensors (default: the seeded random MPS);
         ``stage``: its prepared pinned image (``make_stage``), when the
-        caller built it already."""
+        caller built it already; ``gates``: merged two-site operators the
+        solve reserves after its Hamiltonian's (``set_gate``)."""
         self.engine = engine
         self.dev = engine.device
         self.tag = tag
@@ -400,23 +438,29 @@ class Solve:
         # one arena per solve; every buffer is a view into it
         self.arena = engine.arena_get(lay['bytes'])
         self.base = self.arena.data_ptr()
-        views = {'mps': [None] * n, 'Lenv': [None] * (n + 1), 'Renv': [None] * (n + 1),
-                 'Lenv_lp': [None] * (n + 1), 'Renv_lp': [None] * (n + 1)}
+        views = {'mps': _ArenaViews(self.arena, n), 'Lenv': _ArenaViews(self.arena, n + 1),
+                 'Renv': _ArenaViews(self.arena, n + 1), 'Lenv_lp': _ArenaViews(self.arena, n + 1),
+                 'Renv_lp': _ArenaViews(self.arena, n + 1)}
         for name, idx, off, nb, f64 in lay['arena']:
-            v = self.arena[off:off + nb]
-            if f64:
-                v = v.view(torch.float64)
             if idx is None:
-                setattr(self, name, v)
+                v = self.arena[off:off + nb]
+                setattr(self, name, v.view(torch.float64) if f64 else v)
             else:
-                views[name][idx] = v
+                views[name].ent[idx] = (off, nb, f64)
         self.mps, self.Lenv, self.Renv = views['mps'], views['Lenv'], views['Renv']
         if self.lp_bytes:
             self.Lenv_lp, self.Renv_lp = views['Lenv_lp'], views['Renv_lp']
         # staged prefix (initial MPS, CholeskyQR constants) from the pinned
         # image; the rest of the arena starts at zero
         engine.arena_fill(self.arena, stage)
-        self.ops_slot = engine.alloc_ops(T.ops, T.ops2 if T.pairs else None)
+        ops2 = T.ops2 if T.pairs else None
+        self.gate_base = 0 if ops2 is None else len(ops2)
+        if gates:
+            if ops2 is None:
+                raise ValueError('gates need the two-site tables (dmrg_sites = 2)')
+            ops2 = np.concatenate([ops2, np.zeros((gates,) + ops2.shape[1:])])
+        self.ngates = int(gates)
+        self.ops_slot = engine.alloc_ops(T.ops, ops2)
         # relocation of a shared plan to this solve: arena base, operator
         # slot, coupling-coefficient values
         self.reloc = (self.base, self.ops_slot, T.coef_values)
@@ -467,6 +511,18 @@ class Solve:
             self._lm_pending = {}
         return self._lm
 
+    def set_gate(self, k, gate):
+        """Write gate ``k`` (a d^2 x d^2 matrix on the pair's merged index
+        s_1 d + s_2) into the solve's slot of the operator table."""
+        if not 0 <= k < self.ngates:
+            raise ValueError('gate %d not reserved (the solve has %d)' % (k, self.ngates))
+        g = np.ascontiguousarray(gate, dtype=np.float64).reshape(-1)
+        d4 = self.d2 * self.d2
+        if g.size != d4:
+            raise ValueError('a gate has %d elements (d^4), got %d' % (d4, g.size))
+        o = self.op_index2(self.gate_base + k) * d4
+        self.engine.ops[o:o + d4].copy_(torch.from_numpy(g).to(self.engine.device))
+
     # ----------------------------------------------------------- addresses
     def op_index(self, k):
         return self.ops_slot + k
@@ -478,20 +534,20 @@ class Solve:
         return self.ops_slot + self.engine.ops_off2 + k
 
     def L(self, b, m):
-        return _p(self.Lenv[b], m * self.dmax[b] ** 2)
+        return self.Lenv.ptr(b, m * self.dmax[b] ** 2)
 
     def Rr(self, b, m):
-        return _p(self.Renv[b], m * self.dmax[b] ** 2)
+        return self.Renv.ptr(b, m * self.dmax[b] ** 2)
 
     def L_lp(self, b, m, ti):
         if ti == L.F64:
             return self.L(b, m)
-        return self.Lenv_lp[b].data_ptr() + m * self.dmax[b] ** 2 * ITEMSIZE[ti]
+        return self.Lenv_lp.ptr(b, m * self.dmax[b] ** 2, ITEMSIZE[ti])
 
     def R_lp(self, b, m, ti):
         if ti == L.F64:
             return self.Rr(b, m)
-        return self.Renv_lp[b].data_ptr() + m * self.dmax[b] ** 2 * ITEMSIZE[ti]
+        return self.Renv_lp.ptr(b, m * self.dmax[b] ** 2, ITEMSIZE[ti])
 
     def free(self):
         if self.arena is None:
@@ -593,7 +649,7 @@ class Solve:
         are relative to the solve's arena and operator indices to its slot of
         the operator table (``Solve.part`` moves a phase to a solve)."""
         dims = self.dims if type(self.dims) is tuple else tuple(self.dims)
-        key = (self.sig, st.kind, st.site, st.dir, dims, prec)
+        key = (self.sig, st.kind, st.site, st.dir, dims, prec, st.gate)
         cache = self.engine.plan_cache
         hit = cache.get(key)
         if hit is not None:
@@ -684,6 +740,21 @@ class Solve:
             self._plan_save_r(ph, st.site)
         elif st.kind == 'spectra':
             self._plan_spectra(ph, st.site, st.dir)
+        elif st.kind == 'tmove':
+            self._plan_gauge(ph, st.site, st.dir, copy_back=True)
+        elif st.kind in ('gate2', 'tsplit'):
+            p = st.site - st.dir if st.kind == 'gate2' else st.site
+            if st.dir:
+                d2 = self.d * self.d
+                ph['norm'] = np.array([(self.th.data_ptr(),
+                                        self.dims[p] * d2 * self.dims[p + 2])], dtype=L.NORM_JOB)
+                self._plan_split(ph, p, st.dir)
+            if st.kind == 'gate2':
+                self._plan_gate(ph, st.site, st.gate)
+        elif st.kind == 'tnorm':
+            p = st.site
+            ph['norm'] = np.array([(self.mps.ptr(p), self.dims[p] * self.d * self.dims[p + 1])],
+                                  dtype=L.NORM_JOB)
         elif st.kind == 'resume':
             # start vector <- the Ritz vector the last cycle wrote to the site
             # (the same vector, and so the same normalised V[0], as a restart
@@ -691,7 +762,7 @@ class Solve:
             p = st.site
             mx = R.MixList()
             mx.add(self.init.data_ptr(), L.F64, 1, 1, self.dims[p] * self.d * self.dims[p + 1],
-                   [(self.mps[p].data_ptr(), L.F64, 1.0, -1)])
+                   [(self.mps.ptr(p), L.F64, 1.0, -1)])
             ph['copy'] = mx.arrays()
             self._plan_local(ph, p, prec)
         elif st.kind == 'resume2':   # the same for a pair: theta holds the Ritz vector
@@ -712,7 +783,7 @@ class Solve:
         a right move.  Stages (each one launch over all solves):
         G=M^T M | chol+inv (shifted) | Q1=M Ri | G=Q1^T Q1 | chol+inv |
         Q2=Q1 Ri, R21=R2 R1 | G=Q2^T Q2 | chol+inv | Q=Q2 Ri, R=R3 R21."""
-        A = self.mps[q].data_ptr()
+        A = self.mps.ptr(q)
         dsq = self.dsq_max
         Q1 = _p(self.qrw, 0)
         Q2 = _p(self.qrw, self.psi_max)
@@ -764,7 +835,7 @@ class Solve:
         B = L.CHOL_NMAX
         nb = (n + B - 1) // B
         dsq = self.dsq_max
-        A = self.mps[q].data_ptr()
+        A = self.mps.ptr(q)
         X = _p(self.qrw, 0) if right else A          # row-major m x n panel
         Wt = _p(self.qrw, self.psi_max)              # m x B scratch
         G = _p(self.qrw, 2 * self.psi_max)           # n x n
@@ -838,16 +909,16 @@ class Solve:
             elif self._use_cholqr_blocked(m, nn):
                 self._plan_cholqr_blocked(ph, q, m, nn, right=False)
             else:
-                ph['qr'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(),
+                ph['qr'] = np.array([(self.mps.ptr(q), self.mps.ptr(q),
                                       self.Rq.data_ptr(), self.qrw.data_ptr(), m, nn, nn, 0, 0, 0)],
                                     dtype=L.QR_JOB)
             g = R.GemmList()
             g.add(self.init.data_ptr(), k, d * Dn, d * Dn,
-                  [(self.Rq.data_ptr(), nn, 0, self.mps[q + 1].data_ptr(), d * Dn, 0, Dm, 1.0)])
+                  [(self.Rq.data_ptr(), nn, 0, self.mps.ptr(q + 1), d * Dn, 0, Dm, 1.0)])
             ph['absorb'] = g.arrays()
             if copy_back:
                 mx = R.MixList()
-                mx.add(self.mps[q + 1].data_ptr(), L.F64, 1, 1, k * d * Dn,
+                mx.add(self.mps.ptr(q + 1), L.F64, 1, 1, k * d * Dn,
                        [(self.init.data_ptr(), L.F64, 1.0, -1)])
                 ph['copy'] = mx.arrays()
             self.dims[q + 1] = k
@@ -862,16 +933,16 @@ class Solve:
             elif self._use_cholqr_blocked(m, nn):
                 self._plan_cholqr_blocked(ph, q, m, nn, right=True)
             else:
-                ph['qr'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(),
+                ph['qr'] = np.array([(self.mps.ptr(q), self.mps.ptr(q),
                                       self.Rq.data_ptr(), self.qrw.data_ptr(), m, nn, m, 1, 1, 0)],
                                     dtype=L.QR_JOB)
             g = R.GemmList()
             g.add(self.init.data_ptr(), Dp * d, k, k,
-                  [(self.mps[q - 1].data_ptr(), Dm, 0, self.Rq.data_ptr(), Dm, 1, Dm, 1.0)])
+                  [(self.mps.ptr(q - 1), Dm, 0, self.Rq.data_ptr(), Dm, 1, Dm, 1.0)])
             ph['absorb'] = g.arrays()
             if copy_back:
                 mx = R.MixList()
-                mx.add(self.mps[q - 1].data_ptr(), L.F64, 1, 1, Dp * d * k,
+                mx.add(self.mps.ptr(q - 1), L.F64, 1, 1, Dp * d * k,
                        [(self.init.data_ptr(), L.F64, 1.0, -1)])
                 ph['copy'] = mx.arrays()
             self.dims[q] = k
@@ -883,8 +954,21 @@ class Solve:
         Dl, D, Dr = dims[p], dims[p + 1], dims[p + 2]
         g = R.GemmList()
         g.add(self.init.data_ptr(), Dl * d, d * Dr, d * Dr,
-              [(self.mps[p].data_ptr(), D, 0, self.mps[p + 1].data_ptr(), d * Dr, 0, D, 1.0)])
+              [(self.mps.ptr(p), D, 0, self.mps.ptr(p + 1), d * Dr, 0, D, 1.0)])
         ph['absorb'] = g.arrays()
+
+    def _plan_gate(self, ph, p, k):
+        """theta = A_p A_{p+1} into the start-vector buffer, then the pair
+        tensor ``th`` = G theta with G the solve's gate k acting on the
+        merged physical index (``evolve_gate_tebd``, ``MPSClass.py:1416-1429``,
+        for a nearest-neighbour pair: the two gate halves contracted over
+        their bond are the gate itself)."""
+        self._plan_theta(ph, p)
+        d2, dims = self.d * self.d, self.dims
+        mx = R.MixList()
+        mx.add(self.th.data_ptr(), L.F64, dims[p], d2, dims[p + 2],
+               [(self.init.data_ptr(), L.F64, 1.0, self.op_index2(self.gate_base + k))])
+        ph['copy'] = mx.arrays()
 
     def _plan_split(self, ph, p, dirn):
         """Split the pair tensor theta (``th``, M = row-major m x n, m = Dl d,
@@ -910,7 +994,7 @@ class Solve:
         k = min(chi, m, n)
         N = max(m, n)
         th = self.th.data_ptr()
-        A, B = self.mps[p].data_ptr(), self.mps[p + 1].data_ptr()
+        A, B = self.mps.ptr(p), self.mps.ptr(p + 1)
         dsq = self.dsq_max
         I, LDI = _p(self.cholb, 5 * dsq), self.dmax_max
         if N > L.SVDJ_NMAX:
@@ -974,7 +1058,7 @@ class Solve:
         """Block environment through the (new) isometry at site q."""
         T, d, dims = self.T, self.d, self.dims
         site = T.sites[q]
-        Qp = self.mps[q].data_ptr()
+        Qp = self.mps.ptr(q)
         lc, g1, mx, gT = R.MixList(), R.GemmList(), R.MixList(), R.GemmList()
         psi = self.psi_max
         Y = lambda t: _p(self.Yb, t * psi)
@@ -1101,9 +1185,9 @@ class Solve:
         ti, tc, mode1, mode2, floor_rel = PREC[prec]
         T, dims = self.T, self.dims
         if pair:
-            site, d, rb, opi, out = T.pairs[p], self.d * self.d, p + 2, self.op_index2, self.th
+            site, d, rb, opi, out = T.pairs[p], self.d * self.d, p + 2, self.op_index2, self.th.data_ptr()
         else:
-            site, d, rb, opi, out = T.sites[p], self.d, p + 1, self.op_index, self.mps[p]
+            site, d, rb, opi, out = T.sites[p], self.d, p + 1, self.op_index, self.mps.ptr(p)
         Dl, Dr = dims[p], dims[rb]
         n_el = Dl * d * Dr
         psi, dsq = self.psi_max, self.dsq_max
@@ -1169,7 +1253,7 @@ class Solve:
         ph['mv_g2'] = g2.arrays()
         tol = self.eigs_tol if self.eigs_tol > 0 else 2.220446049250313e-16
         ph['lz'] = np.array([(self.V.data_ptr(), self.w.data_ptr(), Xp, self.init.data_ptr(),
-                              out.data_ptr(), n_el, psi, min(self.engine.ncv, n_el), 0,
+                              out, n_el, psi, min(self.engine.ncv, n_el), 0,
                               tol / self.tau, self.tau, floor_rel, ti, tc)],
                             dtype=L.LANCZOS_SOLVE)
         ph['prec'] = prec
@@ -1206,25 +1290,25 @@ class Solve:
             return
         if dirn > 0:
             Dl, Dm, Dn = dims[q], dims[q + 1], dims[q + 2]
-            ph['svd'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(), lm_ptr,
+            ph['svd'] = np.array([(self.mps.ptr(q), self.mps.ptr(q), lm_ptr,
                                    _p(self.Rq, nn), self.qrw.data_ptr(), m, nn, nn, 0, 0, 0)],
                                  dtype=L.SVD_JOB)
             g = R.GemmList()
             g.add(self.init.data_ptr(), nn, d * Dn, d * Dn,
-                  [(_p(self.Rq, nn), nn, 0, self.mps[q + 1].data_ptr(), d * Dn, 0, Dm, 1.0)])
+                  [(_p(self.Rq, nn), nn, 0, self.mps.ptr(q + 1), d * Dn, 0, Dm, 1.0)])
             mx = R.MixList()
-            mx.add(self.mps[q + 1].data_ptr(), L.F64, 1, 1, nn * d * Dn,
+            mx.add(self.mps.ptr(q + 1), L.F64, 1, 1, nn * d * Dn,
                    [(self.init.data_ptr(), L.F64, 1.0, -1)])
         else:
             Dp, Dm, Dr = dims[q - 1], dims[q], dims[q + 1]
-            ph['svd'] = np.array([(self.mps[q].data_ptr(), self.mps[q].data_ptr(), lm_ptr,
+            ph['svd'] = np.array([(self.mps.ptr(q), self.mps.ptr(q), lm_ptr,
                                    _p(self.Rq, nn), self.qrw.data_ptr(), m, nn, m, 1, 1, 0)],
                                  dtype=L.SVD_JOB)
             g = R.GemmList()
             g.add(self.init.data_ptr(), Dp * d, nn, nn,
-                  [(self.mps[q - 1].data_ptr(), Dm, 0, _p(self.Rq, nn), nn, 1, Dm, 1.0)])
+                  [(self.mps.ptr(q - 1), Dm, 0, _p(self.Rq, nn), nn, 1, Dm, 1.0)])
             mx = R.MixList()
-            mx.add(self.mps[q - 1].data_ptr(), L.F64, 1, 1, Dp * d * nn,
+            mx.add(self.mps.ptr(q - 1), L.F64, 1, 1, Dp * d * nn,
                    [(self.init.data_ptr(), L.F64, 1.0, -1)])
         ph['absorb'] = g.arrays()
         ph['copy'] = mx.arrays()
@@ -1277,19 +1361,19 @@ class Solve:
         bw, nblk = L.svdj_blocking(nn)
         ph['svdj'] = np.array([(self.Rq.data_ptr(), _p(self.cholb, 0), U, V, lm_ptr, nn, nn, bw,
                                 nblk, 0, 0)], dtype=L.SVDJ_JOB)
-        A = self.mps[q].data_ptr()
+        A = self.mps.ptr(q)
         T1, T2 = self.qrw.data_ptr(), self.init.data_ptr()
         g, mx = R.GemmList(), R.MixList()
         if dirn > 0:   # A_q (m x n) holds Q: A_q <- Q u^T, A_{q+1} <- v A_{q+1}
             Dn = dims[q + 2]
-            nb = self.mps[q + 1].data_ptr()
+            nb = self.mps.ptr(q + 1)
             g.add(T1, m, nn, nn, [(A, nn, 0, U, nn, 1, nn, 1.0)])
             g.add(T2, nn, d * Dn, d * Dn, [(V, nn, 0, nb, d * Dn, 0, nn, 1.0)])
             mx.add(A, L.F64, 1, 1, m * nn, [(T1, L.F64, 1.0, -1)])
             mx.add(nb, L.F64, 1, 1, nn * d * Dn, [(T2, L.F64, 1.0, -1)])
         else:          # A_q (n x m) holds Q^T: A_q <- u Q^T, A_{q-1} <- A_{q-1} v^T
             Dp = dims[q - 1]
-            nb = self.mps[q - 1].data_ptr()
+            nb = self.mps.ptr(q - 1)
             g.add(T1, nn, m, m, [(U, nn, 0, A, m, 0, nn, 1.0)])
             g.add(T2, Dp * d, nn, nn, [(nb, nn, 0, V, nn, 1, nn, 1.0)])
             mx.add(A, L.F64, 1, 1, nn * m, [(T1, L.F64, 1.0, -1)])
@@ -1318,8 +1402,9 @@ def assemble_pre(pairs):
     """Descriptor arrays (one launch per phase over the batch) and launch
     metadata of the pre-Lanczos phases of one step of every ``(solve, plan)``.
     Array order: qr, svd, 3 x (gemm problems, segments), 4 x (mix jobs,
-    terms), the CholeskyQR stages, then the SVD-move records.  (The fifth mix
-    phase saves R factors for the spectrum, after the CholeskyQR stages.)"""
+    terms), the CholeskyQR stages, the SVD-move records, then the
+    normalisations.  (The fifth mix phase saves R factors for the spectrum,
+    after the CholeskyQR stages; the normalisations run first.)"""
     qr = R.concat_plain(_parts(pairs, 'qr'), L.QR_JOB)
     svd = R.concat_plain(_parts(pairs, 'svd'), L.SVD_JOB)
     gemms = [R.concat_gemm(_parts(pairs, k)) for k in ('absorb', 'env_g1', 'env_gT')]
@@ -1363,9 +1448,14 @@ def assemble_pre(pairs):
         pj, pt, pnb = R.concat_mix(_parts(pairs, 'svd_post_m'))
         svdj_meta += ((len(pg), pnt, len(arrs), gemm_kmax(pg, ps)), (len(pj), pnb, len(arrs) + 2))
         arrs += [pg, ps, pj, pt]
+    norm = R.concat_plain(_parts(pairs, 'norm'), L.NORM_JOB)
+    norm_meta = (len(norm), len(arrs)) if len(norm) else None
+    if len(norm):
+        arrs.append(norm)
     meta = {'nqr': len(qr), 'nsvd': len(svd),
             'gemm': [(len(p), nt, gemm_kmax(p, sg)) for p, sg, nt in gemms],
-            'mix': [(len(j), nb) for j, _, nb in mixes], 'cq': cq_meta, 'svdj': svdj_meta}
+            'mix': [(len(j), nb) for j, _, nb in mixes], 'cq': cq_meta, 'svdj': svdj_meta,
+            'norm': norm_meta}
     return arrs, meta
 
 
@@ -1641,10 +1731,11 @@ class Engine:
         return ent
 
     # ------------------------------------------------------------- solves
-    def add_solve(self, para, seed=0, mps=None, tag=None, stage=None):
+    def add_solve(self, para, seed=0, mps=None, tag=None, stage=None, gates=0):
         """A solve of ``para``; ``stage``: its initial image from
-        :meth:`prepare` (default: drawn and staged here)."""
-        s = Solve(self, para, seed=seed, mps=mps, tag=tag, stage=stage)
+        :meth:`prepare` (default: drawn and staged here); ``gates``: merged
+        two-site operators reserved for time evolution."""
+        s = Solve(self, para, seed=seed, mps=mps, tag=tag, stage=stage, gates=gates)
         self.solves.append(s)
         return s
 
@@ -1928,6 +2019,9 @@ class Engine:
                 L.check(lib.tnx_mix(ctypes_void(ptr[8 + 2 * i]), n_j, ctypes_void(ptr[9 + 2 * i]),
                                     nb, opsp, None, st_), 'mix')
 
+        if meta.get('norm'):
+            L.check(lib.tnx_normalize_f64(ctypes_void(ptr[meta['norm'][1]]), meta['norm'][0], st_),
+                    'tnx_normalize_f64')
         if meta['nqr']:
             L.check(lib.tnx_qr_f64(ctypes_void(ptr[0]), meta['nqr'], st_), 'tnx_qr_f64')
         if meta['nsvd']:
@@ -1997,11 +2091,11 @@ class Engine:
                 s.resume = False
                 kind = 'resume2' if st.kind in ('solve2', 'resume2') else 'resume'
                 s.queue.insert(0, Step(kind, st.site, st.dir, st.sweep))
-            if st.kind in ('solve', 'resume', 'solve2', 'resume2'):
+            if st.kind in ('solve', 'resume', 'solve2', 'resume2', 'gate2'):
                 s.center = st.site
-            elif st.kind in ('move', 'svdmove', 'qmove'):
+            elif st.kind in ('move', 'svdmove', 'qmove', 'tmove'):
                 s.center = st.site + st.dir
-            elif st.kind == 'split2':
+            elif st.kind in ('split2', 'tsplit'):
                 s.center = st.site + (st.dir > 0)
             if 'lm_bond' in ph:   # spectrum written to lmv on the device; read lazily
                 s._lm_pending[ph['lm_bond']] = ph['svd_n']

@@ -212,7 +212,7 @@ class _MeasureProgram:
                 if b >= s.n:
                     continue
                 d, Dl, Dr = s.d, s.dims[b], s.dims[b + 1]
-                A = s.mps[b].data_ptr()
+                A = s.mps.ptr(b)
                 g1.add(_p(P.Z), Dl * d, Dr, Dr, [(A, Dr, 0, P.F_at(b + 1), Dr, 1, Dr, 1.0)])
                 g2.add(P.F_at(b), Dl, Dl, Dl, [(A, d * Dr, 0, _p(P.Z), d * Dr, 1, d * Dr, 1.0)])
             add_gemm(g1)
@@ -230,7 +230,7 @@ class _MeasureProgram:
                 if site >= s.n:
                     continue
                 d, Dl, Dr = s.d, s.dims[site], s.dims[site + 1]
-                A = s.mps[site].data_ptr()
+                A = s.mps.ptr(site)
                 cur, nxt = P.X[site % 2], P.X[(site + 1) % 2]
                 psi = P.psi
                 Zp = lambda t: _p(P.Z, t * psi)

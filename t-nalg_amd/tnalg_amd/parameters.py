@@ -387,3 +387,40 @@ def make_consistent_parameter_dmrg(para):
     para['op'] = [op['id'], op['sx'], op['sy'], op['sz'], op['su'], op['sd'],
                   -para['hx'] * op['sx'] - para['hz'] * op['sz']]
     return para
+
+
+# ---------------------------------------------------------------- TEBD
+def generate_parameters_standard_tebd(lattice='chain'):
+    """``library/Parameters.py:334-357``: imaginary-time TEBD of the XXZ
+    chain (``tau = tau0 dtau^nt`` for ``nt < taut``, ``iterate_time`` sweeps
+    of gates per tau, an energy test every ``dt_ob`` sweeps)."""
+    para = {'spin': 'half', 'jxy': 1, 'jz': 1, 'hx': 0, 'hz': 0, 'l': 12, 'chi': 32,
+            'bound_cond': 'open', 'tau0': 1e-1, 'dtau': 0.1, 'taut': 3, 'dt_ob': 10,
+            'iterate_time': 5000, 'lattice': lattice, 'save_mode': 'final', 'if_break': True,
+            'break_tol': 1e-7}
+    para['data_path'] = os.path.join(os.getcwd(), 'data_tebd')
+    return make_para_consistent_tebd(para)
+
+
+def make_para_consistent_tebd(para):
+    """``library/Parameters.py:360-368``."""
+    para['positions_h2'] = positions_nearest_neighbor_1d(para['l'], para['bound_cond'])
+    para['num_h2'] = para['positions_h2'].shape[0]
+    para['d'] = physical_dim_from_spin(para['spin'])
+    op = spin_operators(para['spin'])
+    para['op'] = [op['id'], op['sx'], op['sy'], op['sz'], op['su'], op['sd'],
+                  -para['hx'] * op['sx'] - para['hz'] * op['sz']]
+    para['ob_time'] = int(para['iterate_time'] / para['dt_ob'])
+    return para
+
+
+def hamiltonian_heisenberg(spin, jx, jy, jz, hx, hz):
+    """Two-site bond Hamiltonian (``library/HamiltonianModule.py:141-147``):
+    ``jx SxSx + jy Re(SySy) + jz SzSz - hx (1 Sx + Sx 1) - hz (1 Sz + Sz 1)``,
+    a d^2 x d^2 matrix on the merged index ``s_1 d + s_2``."""
+    op = spin_operators(spin)
+    h = jx * np.kron(op['sx'], op['sx']) + jy * np.kron(op['sy'], op['sy']).real + \
+        jz * np.kron(op['sz'], op['sz'])
+    h = h - hx * (np.kron(op['id'], op['sx']) + np.kron(op['sx'], op['id']))
+    h = h - hz * (np.kron(op['id'], op['sz']) + np.kron(op['sz'], op['id']))
+    return np.real(h)

@@ -64,7 +64,7 @@ class MixList:
             self.terms.append((src, coef_field(coef), op, sdt))
         self.jobs.append((dst, Dl, d, Dr, dst_dtype, start, len(self.terms) - start, accumulate,
                           active, self.blocks, 0))
-        self.blocks += -(-(Dl * Dr) // L.MIX_PAIRS)
+        self.blocks += -(-(Dl * Dr) // int(L.mix_pairs(d)))
 
     def arrays(self):
         j = np.array(self.jobs, dtype=MIX_JOB) if self.jobs else EMPTY_JOB.copy()
@@ -115,8 +115,8 @@ def mix_blocks(j):
     if len(j) == 0:
         return 0
     last = j[-1]
-    return int(last['block_begin'] + (int(last['Dl']) * int(last['Dr'])
-                                      + L.MIX_PAIRS - 1) // L.MIX_PAIRS)
+    pairs = int(L.mix_pairs(int(last['d'])))
+    return int(last['block_begin'] + (int(last['Dl']) * int(last['Dr']) + pairs - 1) // pairs)
 
 
 # ------------------------------------------------------------ relocation
@@ -534,8 +534,9 @@ def _gemm_end(last):
 
 
 def _mix_end(last):
+    pairs = L.mix_pairs(last['d']).astype(np.int64)
     return last['block_begin'].astype(np.int64) + \
-        (last['Dl'].astype(np.int64) * last['Dr'] + L.MIX_PAIRS - 1) // L.MIX_PAIRS
+        (last['Dl'].astype(np.int64) * last['Dr'] + pairs - 1) // pairs
 
 
 def concat_gemm(parts, actives=None, relocs=None):
