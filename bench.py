@@ -63,7 +63,7 @@ def engine_for(precision, ncv=20):
     if key not in _ENGINES:
         _ENGINES[key] = Engine(precision_schedule=_schedule_from(precision), ncv=ncv,
                                lowp=precision not in (None, 'fp64'), plan_ahead=_PLAN_AHEAD[0],
-                               lanczos_streams=_LZ_STREAMS[0] or 2)
+                               lanczos_streams=_LZ_STREAMS[0] or 2, init_rng='device')
     return _ENGINES[key]
 
 
@@ -141,7 +141,7 @@ def _run_pipelined(para, seeds, precision, ncv):
         for st in streams:
             with torch.cuda.stream(st):
                 engs.append(Engine(precision_schedule=_schedule_from(precision), ncv=ncv,
-                                   lowp=precision not in (None, 'fp64'),
+                                   init_rng='device', lowp=precision not in (None, 'fp64'),
                                    plan_ahead=_PLAN_AHEAD[0], lanczos_streams=1))
         _ENGINES[key] = (engs, streams)
     engs, streams = _ENGINES[key]
@@ -724,7 +724,8 @@ def main():
            'warmup': args.warmup, 'ms_per_step': 1e3 * total / args.steps,
            'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
            'dtype': 'f64' if args.precision == 'fp64' else 'bf16/f32 matvec, f64 Lanczos+QR',
-           'data': 'synthetic (seeded random initial MPS, uniform XXZ couplings)',
+           'data': 'synthetic (seeded random initial MPS drawn on the device, uniform XXZ '
+                   'couplings)',
            'config': {'workload': 'XXZ chain N=%d open, Jxy=1 Jz=1 h=0, chi=%d, ncv=20, tol=1e-10, '
                                   '%d sweeps' % (args.n, args.chi, args.sweeps),
                       'batch': args.batch, 'precision': args.precision,

@@ -353,6 +353,21 @@ typedef struct tnx_norm_job {
 } tnx_norm_job; /* 16 bytes */
 int tnx_normalize_f64(const tnx_norm_job* jobs, int njob, tnx_stream_t stream);
 
+/* Batched standard-normal fill: dst[i] = N(0, 1) number offset + i of
+ * the stream of `seed` (i < n), f64, one workgroup per job.  The stream is
+ * counter-based (Philox4x32-10 of the pair index (offset + i) / 2 under
+ * the 64-bit key `seed`; Box-Muller of its two 53-bit uniforms gives the
+ * pair), so a job's values do not depend on how a stream is cut into jobs.
+ * Replaces the host np.random.randn draws of the initial random MPS
+ * (TensorBasicModule.py:179-211) when the draw is made on the device.     */
+typedef struct tnx_fill_job {
+  uint64_t dst;
+  int64_t n;
+  uint64_t seed;
+  int64_t offset;
+} tnx_fill_job; /* 32 bytes */
+int tnx_fill_normal_f64(const tnx_fill_job* jobs, int njob, tnx_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,7 @@ constexpr int NCV_MAX = 32;
 constexpr int CHUNK = 1024;  // 256 lanes x 4 elements
 constexpr int NT = 256;
 constexpr int LZ_KEEP = 4;   // vectors kept by an implicit restart (at most)
+constexpr double LZ_REORTH_SKIP = 1e-12;   // second projection left out below this (relative)
 
 struct LzState {
   double alpha[NCV_MAX];
@@ -83,12 +84,14 @@ struct ChunkInfo {
   int i0, i1;   // element range
 };
 
-__device__ __forceinline__ ChunkInfo chunk_of(const tnx_lanczos_solve* solves, int nsolve) {
+__device__ __forceinline__ ChunkInfo chunk_of(const tnx_lanczos_solve* solves, int nsolve,
+                                           int b = -1) {
+  if (b < 0) b = blockIdx.x;   // chunk index (the block index unless offset)
   ChunkInfo ci;
   ci.s = tnx_find_owner(solves, sizeof(tnx_lanczos_solve), offsetof(tnx_lanczos_solve, chunk_begin),
-                        nsolve, blockIdx.x);
+                        nsolve, b);
   const int n = solves[ci.s].n;
-  ci.c = blockIdx.x - solves[ci.s].chunk_begin;
+  ci.c = b - solves[ci.s].chunk_begin;
   ci.nch = (n + CHUNK - 1) / CHUNK;
   ci.i0 = ci.c * CHUNK;
   ci.i1 = min(n, ci.i0 + CHUNK);
@@ -433,37 +436,49 @@ __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __r
 // beta_j = sqrt(||r||^2 - ||h2||^2), the norm of r - V h2 (V^T r = h2 and
 // V is orthonormal; ||h2|| is at rounding level next to ||r|| except at a
 // Lanczos breakdown, which the test below catches).
-// Blocks [0, nchunks): V[j+1] = (r - V h2) / beta_j and x = cast(V[j+1])
-// over their chunk, when another Krylov vector follows.
-// Blocks [nchunks, nchunks + nsolve), one per solve: alpha_j = h1[j] + h2[j]
-// and beta_j (the same reduction, so the same bits), the tridiagonal
-// eigenproblem and the convergence test (clearing act[s]; the solve's chunk
-// blocks may still write V[j+1], which is then unused).  The eigensolve runs
-// beside the chunks' pass over V instead of after it.
+// Chunk blocks: V[j+1] = (r - V h2) / beta_j and x = cast(V[j+1]) over
+// their chunk, when another Krylov vector follows.  When the second
+// projection is below LZ_REORTH_SKIP relative to beta_j (the usual case:
+// r is already orthogonal to V to that level after the first pass), the
+// correction is left out and V is not read: V[j+1] = r / beta_j, whose
+// overlap with every earlier basis vector is then at most LZ_REORTH_SKIP,
+// far below the semi-orthogonality (sqrt of the unit roundoff) that keeps
+// the Ritz values of the tridiagonal matrix accurate to rounding.  Every
+// block of a solve takes the same decision from the same sums.
+// Blocks [0, nsolve), one per solve: alpha_j = h1[j] + h2[j] and beta_j (the
+// same reduction, so the same bits), the tridiagonal eigenproblem and the
+// convergence test (clearing act[s]; the solve's chunk blocks, which read
+// act[s] when they start, then skip V[j+1] and x, unused after
+// convergence).  They come first in the grid, so the serial eigensolve
+// starts with the chunks' pass (blocks [nsolve, nsolve + nchunks)) instead
+// of after it.
 __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __restrict__ S, int nsolve,
                                                     void* state, int nchunks, int j, int x_dtype,
                                                     int normalize, int min_iters) {
   LzLayout L = lz_layout(state, nsolve, nchunks);
-  const bool solver = (int)blockIdx.x >= nchunks;
+  const bool solver = (int)blockIdx.x < nsolve;
   ChunkInfo ci;
   int s;
   if (solver) {
-    s = (int)blockIdx.x - nchunks;
+    s = (int)blockIdx.x;
   } else {
     if (!normalize) return;
-    ci = chunk_of(S, nsolve);
+    ci = chunk_of(S, nsolve, (int)blockIdx.x - nsolve);
     s = ci.s;
   }
   if (!solve_active(L.act, s)) return;
   const tnx_lanczos_solve D = S[s];
   __shared__ double h[NCV_MAX + 1];
   __shared__ double beta_s;
+  __shared__ int skip_s;
   reduce_partials(L.partB, D.chunk_begin, (D.n + CHUNK - 1) / CHUNK, j + 2, h);
   __syncthreads();
   if (threadIdx.x == 0) {
     double hh = 0.0;
     for (int k = 0; k <= j; ++k) hh += h[k] * h[k];
-    beta_s = sqrt(fmax(h[j + 1] - hh, 0.0));
+    const double bb = fmax(h[j + 1] - hh, 0.0);
+    beta_s = sqrt(bb);
+    skip_s = hh <= LZ_REORTH_SKIP * LZ_REORTH_SKIP * bb;
   }
   __syncthreads();
   const double beta_j = beta_s;
@@ -476,7 +491,7 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
       int i = ci.i0 + q * NT + threadIdx.x;
       r[q] = (i < ci.i1) ? Vn[i] : 0.0;
     }
-    chunk_axpy(V, D.ldv, j, ci.i0, ci.i1, h, r);
+    if (!skip_s) chunk_axpy(V, D.ldv, j, ci.i0, ci.i1, h, r);
     const double inv = 1.0 / beta_j;
     const int xd = dtype_of(x_dtype, D.x_dtype);
 #pragma unroll
@@ -894,7 +909,9 @@ int issue_step(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve*
   mk = prof ? tnx_prof_mark(TNX_PC_UPDATE_NORM, s) : -1;
   hipLaunchKernelGGL(k_update_norm, dim3(nchunks + nsolve), block, 0, s, solves, nsolve, state,
                      nchunks, j, x_dtype, 1, min_iters);
-  tnx_prof_done(mk, s, E * (vrows + 16.0 + xs));
+  // (r read, V[j+1] and x written; the basis rows of the second correction,
+  // read only when it is not negligible, are not counted)
+  tnx_prof_done(mk, s, E * (16.0 + xs));
   TNX_CHECK_LAUNCH();
   return TNX_OK;
 }

@@ -165,7 +165,60 @@ __global__ __launch_bounds__(1024) void normalize_kernel(const tnx_norm_job* __r
   for (int64_t i = threadIdx.x; i < J.n; i += 1024) x[i] *= sc;
 }
 
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC 2011): ten rounds of two 32 x 32 -> 64 multiplies on
+// a 128-bit counter under a 64-bit key bumped by the Weyl constants.
+__device__ __forceinline__ void philox4x32_10(uint32_t (&c)[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = (uint32_t)p1;
+    c[2] = n2;
+    c[3] = (uint32_t)p0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// dst[i] = normal number (offset + i) of the stream of `seed`: pair q =
+// (offset + i) / 2 comes from Philox of counter (q, 0) -> two 53-bit
+// uniforms in (0, 1) -> Box-Muller (cos, sin).
+__global__ __launch_bounds__(256) void fill_normal_kernel(const tnx_fill_job* __restrict__ jobs) {
+  const tnx_fill_job J = jobs[blockIdx.x];
+  double* x = (double*)J.dst;
+  const uint32_t k0 = (uint32_t)J.seed, k1 = (uint32_t)(J.seed >> 32);
+  for (int64_t i = threadIdx.x; i < J.n; i += 256) {
+    const uint64_t g = (uint64_t)(J.offset + i);
+    const uint64_t q = g >> 1;
+    uint32_t c[4] = {(uint32_t)q, (uint32_t)(q >> 32), 0u, 0u};
+    philox4x32_10(c, k0, k1);
+    const uint64_t a = ((uint64_t)c[0] << 32) | c[1];
+    const uint64_t b = ((uint64_t)c[2] << 32) | c[3];
+    const double u1 = ((double)(a >> 11) + 0.5) * 1.1102230246251565e-16;   // 2^-53
+    const double u2 = ((double)(b >> 11) + 0.5) * 1.1102230246251565e-16;
+    const double r = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincospi(2.0 * u2, &sn, &cs);
+    x[i] = (g & 1) ? r * sn : r * cs;
+  }
+}
+
 }  // namespace
+
+extern "C" int tnx_fill_normal_f64(const tnx_fill_job* jobs, int njob, tnx_stream_t stream) {
+  if (njob == 0) return TNX_OK;
+  if (jobs == nullptr || njob < 0) {
+    tnx_set_error("tnx_fill_normal_f64: bad arguments (njob=%d)", njob);
+    return TNX_EINVAL;
+  }
+  hipLaunchKernelGGL(fill_normal_kernel, dim3(njob), dim3(256), 0, (hipStream_t)stream, jobs);
+  TNX_CHECK_LAUNCH();
+  return TNX_OK;
+}
 
 extern "C" int tnx_normalize_f64(const tnx_norm_job* jobs, int njob, tnx_stream_t stream) {
   if (njob == 0) return TNX_OK;
@@ -189,7 +242,8 @@ extern "C" int tnx_abi_sizes(int32_t* sizes, int n) {
                          (int32_t)sizeof(tnx_lanczos_result), (int32_t)sizeof(tnx_qr_job),
                          (int32_t)sizeof(tnx_svd_job),       (int32_t)sizeof(tnx_dot_job),
                          (int32_t)sizeof(tnx_chol_job),      (int32_t)sizeof(tnx_svdj_job),
-                         (int32_t)sizeof(tnx_prof_stat),     (int32_t)sizeof(tnx_norm_job)};
+                         (int32_t)sizeof(tnx_prof_stat),     (int32_t)sizeof(tnx_norm_job),
+                         (int32_t)sizeof(tnx_fill_job)};
   const int na = (int)(sizeof(all) / sizeof(all[0]));
   if (sizes == nullptr) return na;
   for (int i = 0; i < n && i < na; ++i) sizes[i] = all[i];

@@ -40,6 +40,7 @@ SVD_JOB = np.dtype([('src', '<u8'), ('u', '<u8'), ('s', '<u8'), ('svt', '<u8'), 
                     ('m', '<i4'), ('n', '<i4'), ('ld', '<i4'), ('trans', '<i4'),
                     ('u_trans', '<i4'), ('reserved', '<i4')])
 NORM_JOB = np.dtype([('x', '<u8'), ('n', '<i8')])
+FILL_JOB = np.dtype([('dst', '<u8'), ('n', '<i8'), ('seed', '<u8'), ('offset', '<i8')])
 DOT_JOB = np.dtype([('x', '<u8'), ('y', '<u8'), ('rows', '<i4'), ('cols', '<i4'),
                     ('ldx', '<i4'), ('ldy', '<i4')])
 CHOL_JOB = np.dtype([('g', '<u8'), ('r', '<u8'), ('rinv', '<u8'), ('status', '<u8'),
@@ -132,6 +133,7 @@ SIGNATURES = {
     'tnx_svd_block_jacobi': (_i, [_vp, _i, _i, _i, _i, _vp, _i, _vp]),
     'tnx_trace_dot': (_i, [_vp, _i, _vp, _vp]),
     'tnx_normalize_f64': (_i, [_vp, _i, _vp]),
+    'tnx_fill_normal_f64': (_i, [_vp, _i, _vp]),
     'tnx_profile_begin': (_i, [_i, _i]),
     'tnx_profile_end': (_i, [ctypes.POINTER(ProfStat), _i]),
 }
@@ -162,7 +164,7 @@ def load(path=None):
     want = [GEMM_SEG.itemsize, GEMM_PROBLEM.itemsize, MIX_JOB.itemsize, MIX_TERM.itemsize,
             ctypes.sizeof(MatvecPlan), LANCZOS_SOLVE.itemsize, LANCZOS_RESULT.itemsize,
             QR_JOB.itemsize, SVD_JOB.itemsize, DOT_JOB.itemsize, CHOL_JOB.itemsize,
-            SVDJ_JOB.itemsize, ctypes.sizeof(ProfStat), NORM_JOB.itemsize]
+            SVDJ_JOB.itemsize, ctypes.sizeof(ProfStat), NORM_JOB.itemsize, FILL_JOB.itemsize]
     got = list(sizes[:n])
     if got != want:
         raise RuntimeError('ABI size mismatch: library %s, binding %s' % (got, want))

@@ -26,11 +26,20 @@ from .parameters import generate_parameters_dmrg, positions_fully_connected
 def get_bond_energies(eb_full, positions, index2):
     """Sum the per-row energies per bond (``algorithms/DMRG_anyH.py:338-346``)."""
     eb_full = np.asarray(eb_full).reshape(-1)
-    positions = np.asarray(positions)
+    positions = np.asarray(positions).reshape(-1, 2)
+    index2 = np.asarray(index2)
     eb = np.zeros((positions.shape[0], 1))
-    for i in range(eb_full.size):
-        hit = (index2[i, 0] == positions[:, 0]) & (index2[i, 1] == positions[:, 1])
-        eb[np.nonzero(hit)] += eb_full[i]
+    if eb_full.size == 0 or positions.shape[0] == 0:
+        return eb
+    # every row adds to every position with its (i, j) (equal positions each get it)
+    big = int(max(positions.max(), index2[:, :2].max())) + 1
+    pk = positions[:, 0].astype(np.int64) * big + positions[:, 1]
+    rk = index2[:eb_full.size, 0].astype(np.int64) * big + index2[:eb_full.size, 1]
+    order = np.argsort(pk, kind='stable')
+    lo = np.searchsorted(pk[order], rk, side='left')
+    hi = np.searchsorted(pk[order], rk, side='right')
+    for r in np.nonzero(hi - lo)[0]:
+        eb[order[lo[r]:hi[r]], 0] += eb_full[r]
     return eb
 
 
@@ -40,6 +49,27 @@ def entanglement_entropy(lm, tol=1e-20):
     lm = lm[lm > tol]
     lm = lm / np.linalg.norm(lm)
     return float(-2.0 * np.dot(lm ** 2, np.log(lm)))
+
+
+def entanglement_entropies(lms, tol=1e-20):
+    """``entanglement_entropy`` of every spectrum in ``lms`` (-1 for an
+    empty one), as an ``(len(lms), 1)`` array, in one vectorised pass."""
+    out = np.full((len(lms), 1), -1.0)
+    full = [k for k, x in enumerate(lms) if np.size(x)]
+    if not full:
+        return out
+    width = max(np.size(lms[k]) for k in full)
+    pad = np.zeros((len(full), width))
+    for r, k in enumerate(full):
+        x = np.asarray(lms[k], dtype=float).reshape(-1)
+        pad[r, :x.size] = x
+    pad = np.where(pad > tol, pad, 0.0)
+    nrm = np.sqrt(np.sum(pad ** 2, axis=1, keepdims=True))
+    p = pad / np.where(nrm > 0, nrm, 1.0)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        terms = np.where(p > 0, p ** 2 * np.log(np.where(p > 0, p, 1.0)), 0.0)
+    out[full, 0] = -2.0 * terms.sum(axis=1)
+    return out
 
 
 def mixed_schedule(low='bf16', mid='fp32', n_fp64=1, n_mid=2):
@@ -261,14 +291,27 @@ def observe_energy(engine, solve, para):
 
 def observe_energies(engine, solves, paras):
     """``observe_energy`` of several solves in one batched measurement."""
-    from .observe import measure
+    return observe_energies_end(observe_energies_begin(engine, solves, paras))
+
+
+def observe_energies_begin(engine, solves, paras):
+    """The launches of :func:`observe_energies` (values read by
+    :func:`observe_energies_end`)."""
+    from .observe import measure_begin
     reqs = []
     for s, para in zip(solves, paras):
         idx2 = np.asarray(para['index2'])
         one = [(i, 1) for i in range(s.n)] + [(i, 3) for i in range(s.n)]
         reqs.append((one, [tuple(r) for r in idx2]))
+    return solves, paras, measure_begin(engine, solves, reqs,
+                                        [('energies', s.sig, s.n) for s in solves])
+
+
+def observe_energies_end(handle):
+    from .observe import measure_end
+    solves, paras, h = handle
     out = []
-    for s, para, (vone, vtwo, _) in zip(solves, paras, measure(engine, solves, reqs)):
+    for s, para, (vone, vtwo, _) in zip(solves, paras, measure_end(h)):
         n = s.n
         ob = {}
         ob['eb_full'] = (np.asarray(para['coeff2'], dtype=float).reshape(-1) * vtwo).reshape(-1, 1)
@@ -383,28 +426,50 @@ def finish_outputs(engine, solves, paras, obs=None):
     middle (one batched measurement).  Fills ``obs`` (the keys of the
     reference's ``ob``) and returns ``(obs, spectra)`` with
     ``spectra[k] = (lm, ent)`` of solve k (``A.lm`` and ``A.ent``)."""
-    from .observe import measure
-    if obs is None:
-        obs = observe_energies(engine, solves, paras)
-    for s in solves:
-        schedule_spectrum(s)
-    engine.run_queues(solves)
+    from .observe import measure_begin, measure_end
+    # both measurements are queued first, then the spectrum moves (which
+    # change the gauge of the tensors, not the state) run behind them on the
+    # stream while the host prepares them
+    h_e = None if obs is not None else observe_energies_begin(engine, solves, paras)
     reqs, npairs = [], []
     for s in solves:
         pairs = correlator_pairs(s.n)
         npairs.append(len(pairs))
         reqs.append(([], [(a, b, 1, 1) for a, b in pairs] + [(a, b, 3, 3) for a, b in pairs]))
+    h_c = measure_begin(engine, solves, reqs, [('corr', s.n) for s in solves])
+    for s in solves:
+        schedule_spectrum(s)
+    engine.run_queues(solves)
+    if h_e is not None:
+        obs = observe_energies_end(h_e)
+    fetch_spectra(solves)
     spectra = []
-    for s, para, ob, k, (_, v, _) in zip(solves, paras, obs, npairs,
-                                         measure(engine, solves, reqs)):
+    for s, para, ob, k, (_, v, _) in zip(solves, paras, obs, npairs, measure_end(h_c)):
         ob['eb'] = get_bond_energies(ob['eb_full'], para['positions_h2'],
                                      np.asarray(para['index2']))
         ob['corr_x'] = np.asarray(v[:k])
         ob['corr_z'] = np.asarray(v[k:])
         lm = [np.asarray(x) for x in s.lm]
-        spectra.append((lm, np.array([[-1.0 if x.size == 0 else entanglement_entropy(x)]
-                                      for x in lm])))
+        spectra.append((lm, entanglement_entropies(lm)))
     return obs, spectra
+
+
+def fetch_spectra(solves):
+    """Copy the spectra written on the device of every solve to the host
+    in one transfer (what reading ``solve.lm`` does for one solve)."""
+    import torch
+    pend = [s for s in solves if s._lm_pending]
+    if len(pend) < 2:
+        return
+    host = torch.cat([s.lmv for s in pend]).cpu().numpy()
+    o = 0
+    for s in pend:
+        m = s.lmv.numel()
+        rows = host[o:o + m].reshape(s.n - 1, s.dmax_max)
+        o += m
+        for b, k in s._lm_pending.items():
+            s._lm[b] = rows[b, :k].copy()
+        s._lm_pending = {}
 
 
 def all_pair_spin_correlations(A):

@@ -421,7 +421,8 @@ class Solve:
             raise ValueError('the chain needs at least 2 sites')
         if not 0 <= self.ob < n:
             raise ValueError('ob_position must be in [0, l)')
-        if stage is None:
+        on_device = stage is None and mps is None and engine.init_rng == 'device'
+        if stage is None and not on_device:
             host = (random_mps_host(n, d, initial_chi(para), seed) if mps is None
                     else [np.asarray(a, dtype=np.float64) for a in mps])
             stage = make_stage(lay, host, pin=engine.device.type != 'cpu')
@@ -451,8 +452,11 @@ class Solve:
         if self.lp_bytes:
             self.Lenv_lp, self.Renv_lp = views['Lenv_lp'], views['Renv_lp']
         # staged prefix (initial MPS, CholeskyQR constants) from the pinned
-        # image; the rest of the arena starts at zero
-        engine.arena_fill(self.arena, stage)
+        # image, or drawn on the device; the rest of the arena starts at zero
+        if on_device:
+            engine.arena_fill_device(self.arena, lay, n, d, dims, seed)
+        else:
+            engine.arena_fill(self.arena, stage)
         ops2 = T.ops2 if T.pairs else None
         self.gate_base = 0 if ops2 is None else len(ops2)
         if gates:
@@ -493,8 +497,9 @@ class Solve:
         # switches a solve that converged at low precision to f64)
         self.prec_override = None
         # shared plan parts relocated to this solve, by (plan uid, phase);
-        # a solve revisits the same plans every sweep
-        self.moved = {}
+        # a solve revisits the same plans every sweep, and a later solve on
+        # the same arena, operator slot and couplings takes them over
+        self.moved = engine._moved_pool.pop(self.reloc_key, None) or {}
         self._next = None   # (step, precision, plan) planned ahead
         self.cycles = 0       # unconverged Krylov cycles of the current local solve
         self.resume = False   # the last cycle did not converge: resume next step
@@ -561,6 +566,11 @@ class Solve:
             setattr(self, name, None)
         self.engine.arena_release(self.arena)
         self.arena = None
+        if self.moved:
+            pool = self.engine._moved_pool
+            pool[self.reloc_key] = self.moved
+            while len(pool) > self.engine.moved_pool_limit:
+                pool.pop(next(iter(pool)))
         self.moved = {}
         self._next = None
 
@@ -1529,8 +1539,20 @@ class Engine:
 
     def __init__(self, precision_schedule=None, ncv=20, max_restarts=20, min_iters=1,
                  device='cuda:0', lowp=True, ops_capacity=4096, plan_ahead=True,
-                 cycle_steps=False, autotune=True, lanczos_streams=2, cycle_restarts=2):
+                 cycle_steps=False, autotune=True, lanczos_streams=2, cycle_restarts=2,
+                 init_rng='numpy'):
         require_gpu()
+        # initial random MPS: 'numpy' draws RandomState(seed) on the host in
+        # the reference's order (TensorBasicModule.py:179-211); 'device' draws
+        # the same shapes of standard normals on the GPU (a Philox stream
+        # seeded by the solve's seed), with no host RNG or staging
+        if init_rng not in ('numpy', 'device'):
+            raise ValueError("init_rng must be 'numpy' or 'device'")
+        self.init_rng = init_rng
+        self._moved_pool = {}        # reloc key -> relocated plan parts of a freed solve
+        self.moved_pool_limit = 256
+        self._stage_consts = {}      # layout id -> device image of the CholeskyQR constants
+        self._pending_fill = []      # initial-MPS fill jobs not yet launched
         self.lib = L.load()
         self.device = torch.device(device)
         self.ncv = int(ncv)
@@ -1711,6 +1733,44 @@ class Engine:
         arena[:n].copy_(stage, non_blocking=True)
         arena[n:].zero_()
 
+    def arena_fill_device(self, arena, lay, n, d, dims, seed):
+        """The arena of a solve whose initial MPS is drawn on the device:
+        zero, the CholeskyQR constants of the staged prefix, then every site
+        (dims[s], d, dims[s+1]) filled with standard normals of the
+        counter-based stream of ``seed`` (sites in the reference's draw
+        order: 0, n-1, then 1..n-2), queued as jobs of one batched fill
+        launch (``flush_fill``)."""
+        arena.zero_()
+        key = id(lay)
+        got = self._stage_consts.get(key)
+        if got is None or got[0] is not lay:
+            o = lay['offsets'][('cholb', None)]
+            img = make_stage(lay, [], pin=False)[o:lay['stage_bytes']]
+            order = [0, n - 1] + list(range(1, n - 1)) if n > 1 else [0]
+            offs = np.array([lay['offsets'][('mps', s)] for s in order], dtype=np.int64)
+            got = (lay, o, img.to(self.device), np.array(order), offs)
+            self._stage_consts[key] = got
+        _, o, img, order, offs = got
+        arena[o:o + img.numel()].copy_(img, non_blocking=True)
+        dims = np.asarray(dims, dtype=np.int64)
+        cnt = dims[order] * d * dims[order + 1]
+        jobs = np.zeros(len(order), dtype=L.FILL_JOB)
+        jobs['dst'] = arena.data_ptr() + offs
+        jobs['n'] = cnt
+        jobs['seed'] = np.uint64(int(seed) & ((1 << 64) - 1))
+        jobs['offset'] = np.cumsum(cnt) - cnt
+        self._pending_fill.append(jobs)
+
+    def flush_fill(self):
+        """Launch the queued initial-MPS fills (one launch for all)."""
+        if not self._pending_fill:
+            return
+        jobs = np.concatenate(self._pending_fill)
+        self._pending_fill = []
+        (dj,) = self.up.upload([jobs])
+        L.check(self.lib.tnx_fill_normal_f64(ctypes_void(dj), len(jobs), self.stream()),
+                'tnx_fill_normal_f64')
+
     def arena_release(self, arena):
         self._arena_pool.setdefault(arena.numel(), []).append(arena)
 
@@ -1736,6 +1796,7 @@ class Engine:
         :meth:`prepare` (default: drawn and staged here); ``gates``: merged
         two-site operators reserved for time evolution."""
         s = Solve(self, para, seed=seed, mps=mps, tag=tag, stage=stage, gates=gates)
+        self.flush_fill()
         self.solves.append(s)
         return s
 
@@ -1743,14 +1804,26 @@ class Engine:
         """The host work of ``add_solve(para, seed)`` that needs no device
         state: term tables and layout, the seeded initial MPS drawn (numpy's
         generator runs without the GIL) and staged in pinned memory.  Safe on
-        a worker thread; the result goes to ``add_solve(..., stage=...)``."""
+        a worker thread; the result goes to ``add_solve(..., stage=...)``.
+        With ``init_rng='device'`` there is nothing to stage (None)."""
+        if self.init_rng == 'device':
+            return None
         T, _, _, lay = solve_geometry(para, self.ncv, self.uses_lowp)
         return make_stage(lay, random_mps_host(T.n, T.d, initial_chi(para), seed),
                           pin=self.device.type != 'cpu')
 
     def add_solves(self, items, threads=16):
         """Solves of ``items = [(para, seed), ...]``; the seeded initial MPS
-        draws (host RNG, which runs without the GIL) go to a thread pool."""
+        draws (host RNG, which runs without the GIL) go to a thread pool, or
+        run on the GPU with ``init_rng='device'``."""
+        if self.init_rng == 'device':   # drawn on the GPU, one fill launch for the batch
+            out = []
+            for para, seed in items:
+                s = Solve(self, para, seed=seed)
+                self.solves.append(s)
+                out.append(s)
+            self.flush_fill()
+            return out
         geoms = [solve_geometry(para, self.ncv, self.uses_lowp) for para, _ in items]
 
         def draw(k):

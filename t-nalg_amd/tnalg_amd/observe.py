@@ -139,31 +139,62 @@ def real_rows(ops, a, b, tol=1e-12):
     return rows
 
 
-def measure(engine, solves, requests):
+def measure(engine, solves, requests, tags=None):
     """``requests[k] = (one, two)`` for ``solves[k]``: ``one`` = [(site, op)],
     ``two`` = [(i, j, op_i, op_j)].  Returns, per solve, ``(one_vals, two_vals,
     norm)`` as numpy arrays: the real parts of the expectation values
     (``one``: ``Re <op>``, which for a real state is that of the operator's
     real part; ``two``: see :func:`real_rows`)."""
-    expanded, maps = [], []
-    for s, (one, two) in zip(solves, requests):
-        ops = s.para['op']
-        rows, where = [], []
-        cache = {}
-        for (i, j, a, b) in two:
-            key = (int(a), int(b))
-            if key not in cache:
-                cache[key] = real_rows(ops, *key)
-            lst = []
-            for (ka, kb, c) in cache[key]:
-                lst.append((len(rows), c))
-                rows.append((i, j, ka, kb))
-            where.append(lst)
-        expanded.append((one, rows))
-        maps.append(where)
+    return measure_end(measure_begin(engine, solves, requests, tags))
+
+
+def measure_begin(engine, solves, requests, tags=None):
+    """The launches of :func:`measure` (on the engine's stream, reading the
+    solves' current tensors); :func:`measure_end` waits for and returns the
+    values.  Later work on the stream may change the tensors meanwhile.
+    ``tags[k]`` (optional, hashable) names request k: equal tags promise
+    equal requests on solves with equal operator lists, so their expansion
+    and program lookup are keyed by the tag instead of the rows."""
+    expanded, maps, keys = [], [], []
+    memo = engine.__dict__.setdefault('_measure_req_memo', {})
+    for k, (s, (one, two)) in enumerate(zip(solves, requests)):
+        tag = None if tags is None else tags[k]
+        got = memo.get(tag) if tag is not None else None
+        if got is None:
+            ops = s.para['op']
+            rows, where = [], []
+            cache = {}
+            for (i, j, a, b) in two:
+                key = (int(a), int(b))
+                if key not in cache:
+                    cache[key] = real_rows(ops, *key)
+                lst = []
+                for (ka, kb, c) in cache[key]:
+                    lst.append((len(rows), c))
+                    rows.append((i, j, ka, kb))
+                where.append(lst)
+            simple = all(len(lst) == 1 and lst[0][1] == 1.0 for lst in where)
+            got = (one, rows, where, simple)
+            if tag is not None:
+                if len(memo) > 4096:
+                    memo.clear()
+                memo[tag] = got
+        one_, rows, where, simple = got
+        expanded.append((one_, rows))
+        maps.append((where, simple))
+        keys.append(None if tag is None else (_solve_key(s), tag))
+    return _measure_real_begin(engine, solves, expanded, keys), maps
+
+
+def measure_end(handle):
+    real, maps = handle
     out = []
-    for (vone, vtwo, norm), where in zip(_measure_real(engine, solves, expanded), maps):
-        two = np.array([sum(c * vtwo[t] for t, c in lst) for lst in where], dtype=np.float64)
+    for (vone, vtwo, norm), (where, simple) in zip(_measure_real_end(real), maps):
+        if simple:   # one row per request row, coefficient 1: the values themselves
+            two = np.asarray(vtwo, dtype=np.float64).copy()
+        else:
+            two = np.array([sum(c * vtwo[t] for t, c in lst) for lst in where],
+                           dtype=np.float64)
         out.append((vone, two, norm))
     return out
 
@@ -295,6 +326,12 @@ class _MeasureProgram:
     def run(self, engine):
         """Launch the measurement on the engine's stream; returns per plan
         ``(one_vals, two_vals, norm)``."""
+        self.launch(engine)
+        return self.collect()
+
+    def launch(self, engine):
+        """Queue the measurement's kernels on the engine's stream (they read
+        the solves' tensors as they are at this point of the stream)."""
         lib, stream, ptr = engine.lib, engine.stream(), self.ptrs
         opsp = ctypes_void(engine.ops.data_ptr())
         out = self.dot_out.data_ptr()
@@ -308,6 +345,9 @@ class _MeasureProgram:
             else:
                 L.check(lib.tnx_trace_dot(ctypes_void(ptr[i]), n, ctypes_void(out + 8 * x),
                                           stream), 'trace_dot')
+
+    def collect(self):
+        """The values of the last launch (waits for the stream up to here)."""
         vals = self.dot_out.cpu().numpy()
         for P in self.plans:
             P.values[:] = 0.0
@@ -327,11 +367,16 @@ MEASURE_CACHE_BYTES = 8 << 30
 
 def _measure_real(engine, solves, requests):
     """:func:`measure` for operators whose real parts are meant."""
+    return _measure_real_end(_measure_real_begin(engine, solves, requests))
+
+
+def _measure_real_begin(engine, solves, requests, keys=None):
     engine.settle()   # solve state is final once the engine's host work is done
     # a program serves any batch of the same (solve, request) pairs, in any
     # order: its results are permuted back to the caller's order
-    keys = [(_solve_key(s), tuple(map(tuple, one)), tuple(map(tuple, two)))
-            for s, (one, two) in zip(solves, requests)]
+    keys = [k if keys is not None and keys[t] is not None else
+            (_solve_key(s), tuple(map(tuple, one)), tuple(map(tuple, two)))
+            for t, (s, (one, two), k) in enumerate(zip(solves, requests, keys or [None] * len(solves)))]
     order = sorted(range(len(keys)), key=lambda k: keys[k])
     key = tuple(keys[k] for k in order)
     cache = getattr(engine, '_measure_cache', None)
@@ -346,8 +391,14 @@ def _measure_real(engine, solves, requests):
             cache.popitem(last=False)
     else:
         cache.move_to_end(key)
-    res = prog.run(engine)
-    out = [None] * len(solves)
+    prog.launch(engine)
+    return prog, order
+
+
+def _measure_real_end(handle):
+    prog, order = handle
+    res = prog.collect()
+    out = [None] * len(order)
     for t, k in enumerate(order):
         out[k] = res[t]
     return out

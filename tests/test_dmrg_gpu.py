@@ -171,3 +171,34 @@ def test_saved_result_reloads(gpu, tmp_path):
     A.clean_to_save()
     assert A._solve is None
     assert np.allclose(A.observe_magnetization(3), mz, atol=1e-12)
+
+
+def test_device_initial_draw(gpu):
+    """init_rng='device': the initial MPS of a seed is drawn on the GPU with
+    the shapes of the host draw, the same bits alone and in a batch, standard
+    normal entries; the solve then runs to the same ground state as one
+    started from the host draw."""
+    from tnalg_amd.dmrg import _schedule_from
+    from tnalg_amd.engine import Engine
+    para = _para(model.chain_couplings(12, 1.0, 1.0, hz=0.1), 16, 4)
+    print('device initial draw', flush=True)
+    e1 = Engine(precision_schedule=_schedule_from('fp64'), lowp=False, init_rng='device')
+    a = e1.add_solves([(para, 5), (para, 6)])
+    e2 = Engine(precision_schedule=_schedule_from('fp64'), lowp=False, init_rng='device')
+    b = e2.add_solves([(para, 5)])
+    ha, hb = a[0].host_mps(), b[0].host_mps()
+    hn = Engine(precision_schedule=_schedule_from('fp64'), lowp=False).add_solves(
+        [(para, 5)])[0].host_mps()
+    assert [x.shape for x in ha] == [x.shape for x in hn]
+    assert all(np.array_equal(x, y) for x, y in zip(ha, hb))
+    assert not np.array_equal(ha[1], a[1].host_mps()[1])
+    v = np.concatenate([x.ravel() for x in ha])
+    assert abs(v.mean()) < 0.05 and abs(v.std() - 1) < 0.05
+    energies = []
+    for eng, s in ((e1, a[0]), (e2, b[0])):
+        s.schedule_canonicalise()
+        for t in range(4):
+            s.schedule_sweep(t)
+        eng.run_queues([s])
+        energies.append(s.last_theta)
+    assert energies[0] == energies[1]

@@ -163,25 +163,29 @@ def test_mix_matches_numpy(gpu):
     from tnalg_amd import records as R
     lib = L.load()
     rng = np.random.RandomState(3)
-    ops = rng.randn(5, 2, 2)
-    opsd = _dev(ops.reshape(-1))
+    flat = rng.randn(200)   # operator k of a job of dimension d: flat[k d^2 : (k + 1) d^2]
+    opsd = _dev(flat)
     mx = R.MixList()
     keep, expect, outs = [], [], []
-    for (Dl, Dr, acc) in [(1, 1, 0), (3, 5, 1), (40, 33, 0), (128, 64, 1)]:
-        srcs = [rng.randn(Dl, 2, Dr) for _ in range(3)]
+    # jobs of every physical dimension in one launch (d <= 2 and d > 2 jobs
+    # cover different numbers of (a, b) pairs per workgroup)
+    for (Dl, d, Dr, acc) in [(1, 2, 1, 0), (3, 2, 5, 1), (40, 3, 33, 0), (128, 2, 64, 1),
+                             (70, 4, 50, 1), (9, 1, 300, 0), (64, 3, 64, 1)]:
+        op = lambda k: flat[k * d * d:(k + 1) * d * d].reshape(d, d)
+        srcs = [rng.randn(Dl, d, Dr) for _ in range(3)]
         sd = [_dev(s) for s in srcs]
         keep += sd
-        dst0 = rng.randn(Dl, 2, Dr)
+        dst0 = rng.randn(Dl, d, Dr)
         dst = _dev(dst0)
         outs.append(dst)
         terms = [(sd[0].data_ptr(), L.F64, 0.5, -1), (sd[1].data_ptr(), L.F64, -1.5, 2),
                  (sd[2].data_ptr(), L.F64, 2.0, 4)]
-        ref = 0.5 * srcs[0] - 1.5 * np.einsum('st,atb->asb', ops[2], srcs[1]) + \
-            2.0 * np.einsum('st,atb->asb', ops[4], srcs[2])
+        ref = 0.5 * srcs[0] - 1.5 * np.einsum('st,atb->asb', op(2), srcs[1]) + \
+            2.0 * np.einsum('st,atb->asb', op(4), srcs[2])
         if acc:
             ref = ref + dst0
         expect.append(ref)
-        mx.add(dst.data_ptr(), L.F64, Dl, 2, Dr, terms, accumulate=acc)
+        mx.add(dst.data_ptr(), L.F64, Dl, d, Dr, terms, accumulate=acc)
     j, t = mx.arrays()
     dj, dt = _upload([j, t])
     print('mix jobs', len(j), flush=True)
@@ -402,6 +406,58 @@ def test_lanczos_dense_symmetric(gpu):
         assert np.linalg.norm(H @ out - e0 * out) < 1e-6 * max(1, abs(e0))
 
 
+def test_lanczos_basis_stays_orthonormal(gpu):
+    """A full cycle past convergence (every step forced, no restart) of
+    matrices whose lowest eigenvalue is well separated, the case in which a
+    Lanczos basis loses orthogonality first: the ncv + 1 basis vectors stay
+    orthonormal to 1e-10 (the second projection is applied whenever it
+    exceeds 1e-12 of the new coupling, and left out otherwise), and the
+    tridiagonal's lowest eigenvalue matches numpy.linalg.eigh."""
+    import torch
+    from tnalg_amd import _lib as L
+    from tnalg_amd import records as R
+    lib = L.load()
+    rng = np.random.RandomState(5)
+    n, ncv, ns = 3000, 20, 3
+    keep, solves, refs = [], [], []
+    g2 = R.GemmList()
+    for k in range(ns):
+        diag = np.linspace(0.0, 10.0, n)
+        diag[0] = -5.0 - k
+        A = rng.randn(n, n)
+        H = 0.01 * (A + A.T) / 2 + np.diag(diag)
+        refs.append(np.linalg.eigvalsh(H)[0])
+        Hd = _dev(H)
+        V = torch.zeros(ncv + 1, n, dtype=torch.float64, device='cuda')
+        w, x, out = (torch.zeros(n, dtype=torch.float64, device='cuda') for _ in range(3))
+        init = _dev(rng.randn(n))
+        keep += [Hd, V, w, x, init, out]
+        g2.add(w.data_ptr(), n, 1, 1, [(Hd.data_ptr(), n, 0, x.data_ptr(), 1, 0, n, 1.0)], active=k)
+        solves.append((V.data_ptr(), w.data_ptr(), x.data_ptr(), init.data_ptr(), out.data_ptr(), n,
+                       n, 0, 0, 1e-30, 1e-4, 0.0, 0, 0))
+    lz = np.array(solves, dtype=L.LANCZOS_SOLVE)
+    ch = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
+    lz['chunk_begin'] = np.concatenate([[0], np.cumsum(ch)[:-1]])
+    nch = int(ch.sum())
+    p, s = g2.arrays()
+    dp, ds, dlz = _upload([p, s, lz])
+    plan = L.MatvecPlan(0, 0, 0, 0, 0, 0, 0, 0, dp.data_ptr(), ds.data_ptr(), len(p),
+                        R.gemm_tiles(p), 0, 0, 0)
+    state = torch.zeros(int(lib.tnx_lanczos_state_bytes(ns, nch, ncv)), dtype=torch.uint8,
+                        device='cuda')
+    res = np.zeros(ns, dtype=L.LANCZOS_RESULT)
+    print('lanczos basis orthogonality n', n, 'ncv', ncv, flush=True)
+    rc = lib.tnx_lanczos_run_ex(plan, _vp(dlz.data_ptr()), ns, nch, ncv, 0, ncv, 0, 0,
+                                _vp(state.data_ptr()), _vp(res.ctypes.data), _stream())
+    assert rc == L.ENOCONV, (rc, lib.tnx_last_error())
+    for k in range(ns):
+        assert res[k]['iters'] == ncv, res[k]
+        Vh = keep[6 * k + 1].cpu().numpy()
+        G = Vh @ Vh.T
+        assert np.abs(G - np.eye(ncv + 1)).max() < 1e-10, np.abs(G - np.eye(ncv + 1)).max()
+        assert abs(res[k]['theta'] - refs[k]) < 1e-10 * abs(refs[k]), (res[k], refs[k])
+
+
 def test_profile_times_the_first_krylov_step(gpu):
     """Sampled kernel timing around one Lanczos call (every call sampled):
     each matvec GEMM, k_dots, k_update_dots and k_update_norm is timed once
@@ -447,7 +503,7 @@ def test_profile_times_the_first_krylov_step(gpu):
     assert g.work == flops and g.seconds > 0
     assert by['gemm f64 matvec T = F x'].samples == 1     # an empty GEMM 1, still timed
     for k, words in (('k_dots', 8 + 8), ('k_update_dots', 8 + 8 + 8),
-                     ('k_update_norm', 8 + 16 + 8)):
+                     ('k_update_norm', 16 + 8)):
         st = by[k]
         assert st.samples == 1 and st.launches == g.launches and st.unit == 1, k
         assert st.work == nch * L.LZ_CHUNK * words, (k, st.work)
@@ -610,3 +666,44 @@ def test_lanczos_implicit_restart_clustered(gpu):
         else:
             assert 1 - (c @ c) < 1e-6, (n, gap, c)
         assert res[k]['iters'] < 1200, (n, gap, res[k])
+
+
+def test_fill_normal_stream(gpu):
+    """The counter-based normal fill: the oracle's restatement of the
+    stream (Philox4x32-10, pinned by its known-answer vectors, and
+    Box-Muller) at the start and past 2^32 pairs, standard normal moments,
+    the same values however the stream is cut into jobs (pair boundaries
+    included), a different stream for another seed."""
+    import torch
+    from tnalg_amd import _lib as L
+    lib = L.load()
+    n = 1 << 20
+    a = torch.zeros(n, dtype=torch.float64, device='cuda')
+    b = torch.zeros(n, dtype=torch.float64, device='cuda')
+    c = torch.zeros(n, dtype=torch.float64, device='cuda')
+    cuts = [0, 1, 1000, 1001, 77777, n]
+    jobs = [np.array([(a.data_ptr(), n, 12345, 0)], dtype=L.FILL_JOB),
+            np.array([(b.data_ptr() + 8 * lo, hi - lo, 12345, lo)
+                      for lo, hi in zip(cuts[:-1], cuts[1:])], dtype=L.FILL_JOB),
+            np.array([(c.data_ptr(), n, 12346, 0)], dtype=L.FILL_JOB)]
+    print('fill normal n', n, flush=True)
+    for j in jobs:
+        (dj,) = _upload([j])
+        L.check(lib.tnx_fill_normal_f64(_vp(dj.data_ptr()), len(j), _stream()), 'fill')
+        torch.cuda.synchronize()
+    x, y, z = a.cpu().numpy(), b.cpu().numpy(), c.cpu().numpy()
+    assert np.array_equal(x, y)
+    assert not np.array_equal(x, z)
+    assert abs(x.mean()) < 5e-3 and abs(x.std() - 1) < 5e-3
+    assert abs(np.mean(x ** 4) - 3) < 0.05             # Gaussian kurtosis
+    assert abs(np.corrcoef(x[0::2], x[1::2])[0, 1]) < 5e-3   # the two halves of a pair
+    from oracle.rng_ref import normal
+    far = (1 << 33) + 5
+    d = torch.zeros(64, dtype=torch.float64, device='cuda')
+    (dj,) = _upload([np.array([(d.data_ptr(), 64, (1 << 40) + 99, far)], dtype=L.FILL_JOB)])
+    L.check(lib.tnx_fill_normal_f64(_vp(dj.data_ptr()), 1, _stream()), 'fill')
+    torch.cuda.synchronize()
+    ref = np.array([normal(12345, g) for g in range(256)])
+    assert np.allclose(x[:256], ref, rtol=1e-14, atol=1e-14)
+    ref = np.array([normal((1 << 40) + 99, far + g) for g in range(64)])
+    assert np.allclose(d.cpu().numpy(), ref, rtol=1e-14, atol=1e-14)

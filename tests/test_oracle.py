@@ -298,3 +298,16 @@ def test_oracle_reproduces_saved_run_chi24():
     e0, _ = ed.ground_state(12, c['op'], c['index1'], c['coeff1'], c['index2'], c['coeff2'])
     assert 0 < e - e0 < 1e-8
     assert A.virtual_dim == [1, 2, 4, 8, 16, 24, 24, 24, 16, 8, 4, 2, 1]
+
+
+def test_philox_known_answers():
+    """Philox4x32-10 of the oracle's normal stream against the algorithm's
+    published known-answer vectors (Random123 kat_vectors)."""
+    from oracle.rng_ref import normal, philox4x32_10
+    assert philox4x32_10((0, 0, 0, 0), (0, 0)) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert philox4x32_10((0xffffffff,) * 4, (0xffffffff, 0xffffffff)) == \
+        [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert philox4x32_10((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344),
+                         (0xa4093822, 0x299f31d0)) == [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+    v = np.array([normal(7, g) for g in range(20000)])
+    assert abs(v.mean()) < 0.03 and abs(v.std() - 1) < 0.03

@@ -16,11 +16,14 @@ class CpuEngine:
     """The parts of Engine that Solve uses, on host memory."""
     device = torch.device('cpu')
     ncv = 20
+    init_rng = 'numpy'
 
     def __init__(self, lowp):
         self.uses_lowp = lowp
         self.plan_cache = {}
         self._next_slot = 0
+        self._moved_pool = {}
+        self.moved_pool_limit = 0
 
     def arena_get(self, nbytes):
         return torch.empty(nbytes, dtype=torch.uint8)
