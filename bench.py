@@ -107,6 +107,17 @@ def _solve_batch(eng, para, seeds, keep_mps=False):
     return obs
 
 
+def _freeze_caches():
+    """A solver service keeps its step plans, descriptor sets and term
+    tables for its lifetime: after a warm-up, move every object alive into
+    the interpreter's permanent generation, so that a full cyclic
+    collection (which would scan those caches and stall the host for about
+    0.1 s at a random step) does not run over them."""
+    import gc
+    gc.collect()
+    gc.freeze()
+
+
 def run_batch(para, seeds, precision, ncv=20, keep_mps=False):
     """One batched solve: every chain of ``seeds`` from its seeded random MPS
     through all sweeps in lockstep, then all its outputs (``_solve_batch``);
@@ -261,6 +272,16 @@ def matvec_roofline(para, batch, precs=('fp64', 'bf16'), reps=20):
         for _ in range(3):
             best['g1'](), mix(), best['g2']()
         torch.cuda.synchronize()
+        # a spin kernel marks the start of each timed loop in a counter
+        # profile (--pmc-summary): 3 warm-up launches then `reps` timed ones
+        torch.cuda._sleep(1000)
+        for _ in range(3):
+            mix()
+        res['mx'] = {'t': timed([mix]), 'bytes': R.mix_bytes(mxj, mxt), 'blocks': int(mxb)}
+        torch.cuda.synchronize()
+        torch.cuda._sleep(1000)
+        for _ in range(3):
+            best['g1'](), mix(), best['g2']()
         res['t_matvec'] = timed([best['g1'], mix, best['g2']])
         out[prec] = res
     for s in list(solves):
@@ -300,13 +321,14 @@ def live_kernels(stats):
                     'frac': ach / peak, 'avg_launch_s': avg,
                     ('flops_per_launch' if st.unit == 0 else 'bytes_per_launch'): work,
                     'timed_launches': int(st.samples), 'launches': int(st.launches),
+                    'work_all_launches': st.work_all,
                     'est_s': avg * st.launches})
     return sorted(out, key=lambda k: -k['est_s'])
 
 
 KERNEL_OF_CLASS = {'gemm f64': 'gemm_grouped_kernel<0>/<4>', 'gemm f32': 'gemm_grouped_kernel<1>',
                    'gemm bf16 matvec T': 'gemm_grouped_kernel<3>',
-                   'gemm bf16 matvec w': 'gemm_grouped_kernel<2>', 'mixer': 'mix_kernel',
+                   'gemm bf16 matvec w': 'gemm_grouped_kernel<2>', 'mixer': 'mix_kernel<1>',
                    'k_dots': 'k_dots', 'k_update_dots': 'k_update_dots',
                    'k_update_norm': 'k_update_norm'}
 
@@ -327,9 +349,9 @@ def roofline_object(kern, note):
 
 def output_differences(obs, ref):
     """Largest absolute difference, over the solves, of every output against
-    the f64 solve of the same seed (row N2 of the precision statement):
-    energy (relative), mx, mz, eb_full, eb, corr_x, corr_z, the spectra and
-    entropies."""
+    the f64 solve of the same seed (the precision statement of every
+    output): energy (relative), mx, mz, eb_full, eb, corr_x, corr_z, the
+    spectra and entropies."""
     out = {'E_rel': max(abs(a['E'] - b['E']) / abs(b['E']) for a, b in zip(obs, ref))}
     for key in ('mx', 'mz', 'eb_full', 'eb', 'corr_x', 'corr_z', 'ent'):
         out[key] = max(float(np.abs(np.ravel(a[key]) - np.ravel(b[key])).max())
@@ -358,6 +380,37 @@ def copy_calibration(mib=256):
     torch.mul(x, 2.0, out=y)
     torch.cuda.synchronize()
     return mib << 20
+
+
+def pmc_summary(paths):
+    """Per counter file (rocprofv3 --pmc ... --output-format csv of a
+    --roofline-only run): the dispatches after each spin-kernel marker form
+    a segment (per precision: the mixer alone, then the matvec GEMMs and
+    mixer; last: the calibration copy); per segment and kernel, the number
+    of dispatches and the mean value of each counter (KiB per dispatch for
+    FETCH_SIZE / WRITE_SIZE)."""
+    import csv
+    out = {}
+    for path in paths:
+        rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r['Dispatch_Id']))
+        segs, cur = [], None
+        seen = set()
+        for r in rows:
+            name = r['Kernel_Name']
+            if 'spin_kernel' in name:
+                if r['Dispatch_Id'] not in seen:
+                    seen.add(r['Dispatch_Id'])
+                    cur = {}
+                    segs.append(cur)
+                continue
+            if cur is None:
+                continue
+            short = name.replace('void ', '').replace('(anonymous namespace)::', '').split('(')[0]
+            ent = cur.setdefault(short, {})
+            ent.setdefault(r['Counter_Name'], []).append(float(r['Counter_Value']))
+        out[path] = [{k: {c: {'n': len(v), 'mean': float(np.mean(v))} for c, v in d.items()}
+                      for k, d in seg.items()} for seg in segs]
+    return out
 
 
 def serving_run(n_req, rate, max_active=64, seed=0):
@@ -519,7 +572,7 @@ def _sweep_seconds(sites, per_step, n):
 def cpu_baseline(seconds=15.0, converged=None, n=64, chi=128, sweeps=8):
     """The oracle's restatement of the reference algorithm (per-term effective
     operators re-propagated as ``MPSClass.py:585-611`` does, ``opt_env``
-    matvec, ARPACK ``eigsh`` ncv=20, QR gauge moves) on the C2 chain, timed
+    matvec, ARPACK ``eigsh`` ncv=20, QR gauge moves) on the N=64, chi=128 XXZ chain, timed
     on local steps sampled across the chain and extrapolated to one full
     solve; the block-environment restatement (``oracle/block_dmrg``, O(1)
     transfers per step) is timed the same way as a second number.
@@ -638,6 +691,10 @@ def main():
                     help='c2: the batched N=64 chi=128 XXZ chain (default); c3 / c5: one large '
                          'snake-mapped square-lattice solve (chi 512 / 1024); c4: the serving '
                          'stream (--serving-requests requests at --serving-rate per second)')
+    ap.add_argument('--pmc-summary', nargs='+', metavar='CSV',
+                    help='summarise rocprofv3 --pmc counter_collection.csv files of a '
+                         '--roofline-only run: per marked loop, the mean counter value per '
+                         'dispatch of every kernel, and the copy calibration')
     ap.add_argument('--roofline-only', action='store_true',
                     help='only the matvec-GEMM roofline measurement and a copy kernel of known '
                          'size (for rocprofv3 kernel-trace / counter runs)')
@@ -680,8 +737,12 @@ def main():
         # around each variant's launches), and the copy kernel the HBM
         # counters are scaled by
         precs = ('fp64',) if args.precision == 'fp64' else ('fp64', 'fp32', 'bf16')
-        print(json.dumps({'matvec_variants': matvec_roofline(para, args.batch, precs),
-                          'copy_calibration_bytes': copy_calibration()}))
+        mv = matvec_roofline(para, args.batch, precs)
+        torch.cuda._sleep(1000)   # marks the calibration copy for --pmc-summary
+        print(json.dumps({'matvec_variants': mv, 'copy_calibration_bytes': copy_calibration()}))
+        return
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(args.pmc_summary), indent=1))
         return
     seeds = list(range(args.batch))
     cold = None
@@ -691,6 +752,7 @@ def main():
         dt, _, _, _ = run_batch(para, [1000 + s for s in seeds], args.precision)
         cold = dt if cold is None else cold
         print('warmup %d: %.3f s' % (w, dt), file=sys.stderr, flush=True)
+    _freeze_caches()
     times, lat = [], []
     obs = None
     counters = None
@@ -745,6 +807,7 @@ def main():
         # batch of the same shape (other seeds), like the timed engine; and the
         # plain f64 path (one solve at a time, all-f64, warmed likewise)
         run_batch(para, [1000 + s for s in seeds], 'fp64')
+        _freeze_caches()
         dt_b, ob64, _, _ = run_batch(para, seeds, 'fp64')
         e_b = np.array([o['E'] for o in ob64])
         rel = np.abs(energies - e_b) / np.abs(e_b)
@@ -756,6 +819,7 @@ def main():
         out['max_abs_diff_vs_fp64'] = output_differences(obs, ob64)
         _release_engine('fp64')
         run_batch(para, [1000], 'fp64')     # warm the one-solve engine like the timed one
+        _freeze_caches()
         dt_p, ob_p, _, _ = run_batch(para, seeds[:1], 'fp64', keep_mps=True)
         converged = ob_p[0]['mps']
         out['plain_fp64'] = {'solves_per_s': 1.0 / dt_p, 'latency_s': dt_p,

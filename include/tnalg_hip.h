@@ -327,9 +327,10 @@ typedef struct tnx_prof_stat {
   int64_t samples;     /* launches timed                                    */
   double seconds;      /* their summed duration                             */
   double work;         /* their summed algorithmic work                     */
+  double work_all;     /* the algorithmic work of every launch counted      */
   int32_t unit;        /* 0: flops, 1: bytes                                */
   int32_t reserved;
-} tnx_prof_stat; /* 88 bytes */
+} tnx_prof_stat; /* 96 bytes */
 int tnx_profile_begin(int every, int max_launches);
 int tnx_profile_end(tnx_prof_stat* out, int cap);
 

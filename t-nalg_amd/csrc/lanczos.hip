@@ -894,9 +894,9 @@ int issue_step(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve*
   const double xs = m1 == TNX_GEMM_F64 ? 8.0 : m1 == TNX_GEMM_F32 ? 4.0 : 2.0;
   const double ws = m2 == TNX_GEMM_F64 ? 8.0 : m2 == TNX_GEMM_BF16_BF16 ? 2.0 : 4.0;
   const double vrows = 8.0 * (j + 1);
-  tnx_prof_count(TNX_PC_DOTS);
-  tnx_prof_count(TNX_PC_UPDATE_DOTS);
-  tnx_prof_count(TNX_PC_UPDATE_NORM);
+  tnx_prof_count(TNX_PC_DOTS, E * (vrows + ws));
+  tnx_prof_count(TNX_PC_UPDATE_DOTS, E * (vrows + ws + 8.0));
+  tnx_prof_count(TNX_PC_UPDATE_NORM, E * (16.0 + xs));
   int mk = prof ? tnx_prof_mark(TNX_PC_DOTS, s) : -1;
   hipLaunchKernelGGL(k_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
   tnx_prof_done(mk, s, E * (vrows + ws));

@@ -105,6 +105,143 @@ __device__ __forceinline__ void mix_body(const tnx_mix_job& J, int blk,
   }
 }
 
+// Four consecutive elements [e, e + 4) of a buffer of runtime dtype, as
+// double; the caller guarantees that e is a multiple of 4 and the buffer
+// 4-element aligned, so the load is one 8-, 16- or two 16-byte accesses.
+// The conversions are tnx_load's.
+__device__ __forceinline__ void tnx_load4(const void* p, int dtype, int64_t e, double (&v)[4]) {
+  if (dtype == TNX_F64) {
+    const double2 a = *(const double2*)((const double*)p + e);
+    const double2 b = *(const double2*)((const double*)p + e + 2);
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+  } else if (dtype == TNX_F32) {
+    const float4 a = *(const float4*)((const float*)p + e);
+    v[0] = (double)a.x; v[1] = (double)a.y; v[2] = (double)a.z; v[3] = (double)a.w;
+  } else {
+    typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+    const bf4 a = *(const bf4*)((const __bf16*)p + e);
+    v[0] = (double)(float)a[0]; v[1] = (double)(float)a[1];
+    v[2] = (double)(float)a[2]; v[3] = (double)(float)a[3];
+  }
+}
+
+__device__ __forceinline__ void tnx_store4(void* p, int dtype, int64_t e, const double (&v)[4]) {
+  if (dtype == TNX_F64) {
+    *(double2*)((double*)p + e) = make_double2(v[0], v[1]);
+    *(double2*)((double*)p + e + 2) = make_double2(v[2], v[3]);
+  } else if (dtype == TNX_F32) {
+    *(float4*)((float*)p + e) = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+  } else {
+    typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+    bf4 a;
+    a[0] = (__bf16)(float)v[0]; a[1] = (__bf16)(float)v[1];
+    a[2] = (__bf16)(float)v[2]; a[3] = (__bf16)(float)v[3];
+    *(bf4*)((__bf16*)p + e) = a;
+  }
+}
+
+__device__ __forceinline__ bool aligned4(const void* p, int dtype) {
+  const int isz = dtype == TNX_F64 ? 8 : dtype == TNX_F32 ? 4 : 2;
+  return ((uintptr_t)p & (uintptr_t)(4 * isz - 1)) == 0;
+}
+
+// d <= 2 jobs whose rows hold a multiple of 4 pairs (Dr % 4 == 0): lane t
+// takes the 4 consecutive pairs [base + 4t, base + 4t + 4), all in one row,
+// so each source row of a term is one vector load per lane and a wave reads
+// 256 consecutive elements.  Same arithmetic per element as mix_body (terms
+// in order, the operator applied to the d sources of the pair), so the same
+// bits.  A term or destination that is not 4-element aligned takes scalar
+// accesses.
+__device__ __forceinline__ void mix_body_vec2(const tnx_mix_job& J, int blk,
+                                              const tnx_mix_term* __restrict__ terms,
+                                              const double* __restrict__ ops) {
+  const int d = J.d, Dr = J.Dr;
+  const int npairs = J.Dl * Dr;
+  const int p0 = blk * 256 * 4 + 4 * threadIdx.x;
+  if (p0 >= npairs) return;
+  const int a = p0 / Dr;
+  const int64_t e0 = (int64_t)a * d * Dr + (p0 - a * Dr);
+  double acc[2][4];
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[s][q] = 0.0;
+  for (int ti = J.term_begin; ti < J.term_begin + J.term_count; ++ti) {
+    const tnx_mix_term T = terms[ti];
+    double x[2][4];
+    const bool vec = aligned4((const void*)T.src, T.src_dtype);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      if (t < d) {
+        if (vec) {
+          tnx_load4((const void*)T.src, T.src_dtype, e0 + (int64_t)t * Dr, x[t]);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            x[t][q] = tnx_load((const void*)T.src, T.src_dtype, e0 + (int64_t)t * Dr + q);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[t][q] = 0.0;
+      }
+    }
+    if (T.op < 0) {
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[s][q] += T.coef * x[s][q];
+    } else {
+      double O[4];
+      const double* Op = ops + (int64_t)T.op * d * d;
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) O[s * 2 + t] = (s < d && t < d) ? Op[s * d + t] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          if (s < d) {
+            double v = 0.0;
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+              if (t < d) v += O[s * 2 + t] * x[t][q];
+            acc[s][q] += T.coef * v;
+          }
+        }
+    }
+  }
+  const bool vec = aligned4((const void*)J.dst, J.dst_dtype);
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    if (s < d) {
+      const int64_t e = e0 + (int64_t)s * Dr;
+      double v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = acc[s][q];
+      if (vec) {
+        if (J.accumulate) {
+          double o[4];
+          tnx_load4((const void*)J.dst, J.dst_dtype, e, o);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) v[q] += o[q];
+        }
+        tnx_store4((void*)J.dst, J.dst_dtype, e, v);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          double w = v[q];
+          if (J.accumulate) w += tnx_load((const void*)J.dst, J.dst_dtype, e + q);
+          tnx_store((void*)J.dst, J.dst_dtype, e + q, w);
+        }
+      }
+    }
+  }
+}
+
+// TAG only names the instance: 1 for the matvec's mixer (so profiles can tell
+// its launches from the environment, cast and copy mixes), 0 otherwise.
+template <int TAG>
 __global__ __launch_bounds__(256) void mix_kernel(const tnx_mix_job* __restrict__ jobs, int njob,
                                                   const tnx_mix_term* __restrict__ terms,
                                                   const double* __restrict__ ops,
@@ -114,7 +251,9 @@ __global__ __launch_bounds__(256) void mix_kernel(const tnx_mix_job* __restrict_
                                 njob, bid);
   const tnx_mix_job J = jobs[ji];
   if (J.active >= 0 && active != nullptr && active[J.active] == 0) return;
-  if (J.d <= 2)
+  if (J.d <= 2 && J.Dr % 4 == 0)
+    mix_body_vec2(J, bid - J.block_begin, terms, ops);
+  else if (J.d <= 2)
     mix_body<2>(J, bid - J.block_begin, terms, ops);
   else
     mix_body<MIX_DMAX>(J, bid - J.block_begin, terms, ops);
@@ -250,17 +389,26 @@ extern "C" int tnx_abi_sizes(int32_t* sizes, int n) {
   return na;
 }
 
-extern "C" int tnx_mix(const tnx_mix_job* job, int njob, const tnx_mix_term* term, int nblocks,
-                       const double* ops, const int32_t* active, tnx_stream_t stream) {
+static int launch_mix(const tnx_mix_job* job, int njob, const tnx_mix_term* term, int nblocks,
+                      const double* ops, const int32_t* active, tnx_stream_t stream, bool matvec) {
   if (nblocks == 0 || njob == 0) return TNX_OK;
   if (job == nullptr || term == nullptr || nblocks < 0 || njob < 0) {
     tnx_set_error("tnx_mix: bad arguments (njob=%d nblocks=%d)", njob, nblocks);
     return TNX_EINVAL;
   }
-  hipLaunchKernelGGL(mix_kernel, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, job, njob, term,
-                     ops, active);
+  if (matvec)
+    hipLaunchKernelGGL(mix_kernel<1>, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, job, njob,
+                       term, ops, active);
+  else
+    hipLaunchKernelGGL(mix_kernel<0>, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, job, njob,
+                       term, ops, active);
   TNX_CHECK_LAUNCH();
   return TNX_OK;
+}
+
+extern "C" int tnx_mix(const tnx_mix_job* job, int njob, const tnx_mix_term* term, int nblocks,
+                       const double* ops, const int32_t* active, tnx_stream_t stream) {
+  return launch_mix(job, njob, term, nblocks, ops, active, stream, false);
 }
 
 extern "C" int tnx_trace_dot(const tnx_dot_job* jobs, int njob, double* out, tnx_stream_t stream) {
@@ -291,17 +439,17 @@ extern "C" int tnx_heff_matvec(const tnx_matvec_plan* plan, const int32_t* activ
   }
   hipStream_t s = (hipStream_t)stream;
   const bool prof = tnx_prof_on();
-  tnx_prof_count(gemm_class(plan->mode, 0));
-  if (plan->mix_nblocks > 0) tnx_prof_count(TNX_PC_MIX);
-  tnx_prof_count(gemm_class(plan->mode2, 1));
+  tnx_prof_count(gemm_class(plan->mode, 0), plan->work[0]);
+  if (plan->mix_nblocks > 0) tnx_prof_count(TNX_PC_MIX, plan->work[1]);
+  tnx_prof_count(gemm_class(plan->mode2, 1), plan->work[2]);
   int mk = prof ? tnx_prof_mark(gemm_class(plan->mode, 0), s) : -1;
   int rc = tnx_gemm_grouped(plan->mode, plan->g1_prob, plan->g1_nprob, plan->g1_seg,
                             plan->g1_ntiles, active, stream);
   if (rc) return rc;
   tnx_prof_done(mk, s, plan->work[0]);
   mk = (prof && plan->mix_nblocks > 0) ? tnx_prof_mark(TNX_PC_MIX, s) : -1;
-  rc = tnx_mix(plan->mix_job, plan->mix_njob, plan->mix_term, plan->mix_nblocks, plan->ops, active,
-               stream);
+  rc = launch_mix(plan->mix_job, plan->mix_njob, plan->mix_term, plan->mix_nblocks, plan->ops, active,
+                  stream, true);
   if (rc) return rc;
   tnx_prof_done(mk, s, plan->work[1]);
   mk = prof ? tnx_prof_mark(gemm_class(plan->mode2, 1), s) : -1;

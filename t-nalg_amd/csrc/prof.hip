@@ -6,9 +6,9 @@
 // step (every solve still active, so the algorithmic work of each launch is
 // the full work its descriptors describe) are each bracketed by two HIP
 // events on the launch's stream.  tnx_profile_end resolves the events and
-// returns, per kernel class, the launches made while armed, the number of
-// timed launches, their summed duration and their summed algorithmic work
-// (flops or bytes).  Nothing is
+// returns, per kernel class, the launches made while armed with their summed
+// algorithmic work (flops or bytes), and the number of timed launches, their
+// summed duration and their summed algorithmic work.  Nothing is
 // timed while sampling is not armed; the cost when armed is two event
 // records per timed launch.
 #include <string.h>
@@ -39,6 +39,7 @@ const int kClassUnit[TNX_PROF_NCLASS] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1};
 std::mutex g_mu;
 std::atomic<int> g_armed{0};    // launches are counted while armed
 std::atomic<long long> g_count[TNX_PROF_NCLASS];
+std::atomic<double> g_work_all[TNX_PROF_NCLASS];   // algorithmic work of every counted launch
 int g_every = 0;                // 0: not armed
 long long g_calls = 0;          // Lanczos calls seen while armed
 std::vector<Sample> g_samples;  // launches timed
@@ -58,8 +59,12 @@ void tnx_prof_set(bool on) { t_on = on; }
 
 bool tnx_prof_on() { return t_on; }
 
-void tnx_prof_count(int cls) {
-  if (g_armed.load(std::memory_order_relaxed)) g_count[cls].fetch_add(1, std::memory_order_relaxed);
+void tnx_prof_count(int cls, double work) {
+  if (!g_armed.load(std::memory_order_relaxed)) return;
+  g_count[cls].fetch_add(1, std::memory_order_relaxed);
+  double cur = g_work_all[cls].load(std::memory_order_relaxed);
+  while (!g_work_all[cls].compare_exchange_weak(cur, cur + work, std::memory_order_relaxed)) {
+  }
 }
 
 int tnx_prof_mark(int cls, hipStream_t s) {
@@ -104,7 +109,10 @@ extern "C" int tnx_profile_begin(int every, int max_launches) {
   g_samples.clear();
   g_calls = 0;
   g_every = every;
-  for (int c = 0; c < TNX_PROF_NCLASS; ++c) g_count[c].store(0);
+  for (int c = 0; c < TNX_PROF_NCLASS; ++c) {
+    g_count[c].store(0);
+    g_work_all[c].store(0.0);
+  }
   g_armed.store(1);
   return TNX_OK;
 }
@@ -130,6 +138,7 @@ extern "C" int tnx_profile_end(tnx_prof_stat* out, int cap) {
     snprintf(st[c].name, sizeof(st[c].name), "%s", kClassName[c]);
     st[c].unit = kClassUnit[c];
     st[c].launches = g_count[c].load();
+    st[c].work_all = g_work_all[c].load();
   }
   int rc = TNX_OK;
   for (const Sample& smp : samples) {

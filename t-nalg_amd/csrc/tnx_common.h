@@ -26,7 +26,7 @@ void tnx_prof_set(bool on);      // timing on / off for the calling thread
 bool tnx_prof_on();
 int tnx_prof_mark(int cls, hipStream_t s);                // -1: not timed
 void tnx_prof_done(int mark, hipStream_t s, double work);
-void tnx_prof_count(int cls);   // one launch of the class (counted while armed)
+void tnx_prof_count(int cls, double work);   // one launch of the class and its work (while armed)
 
 #define TNX_CHECK_HIP(call)                                                      \
   do {                                                                           \

@@ -102,6 +102,7 @@ class ProfStat(ctypes.Structure):
     _fields_ = [('name', ctypes.c_char * 48), ('launches', ctypes.c_int64),
                 ('samples', ctypes.c_int64),
                 ('seconds', ctypes.c_double), ('work', ctypes.c_double),
+                ('work_all', ctypes.c_double),
                 ('unit', ctypes.c_int32), ('reserved', ctypes.c_int32)]
 
 

@@ -196,6 +196,56 @@ def test_mix_matches_numpy(gpu):
         assert np.allclose(o.cpu().numpy(), r, rtol=1e-13, atol=1e-13)
 
 
+@pytest.mark.parametrize('sdt,ddt', [('float32', 'float32'), ('bfloat16', 'float32'),
+                                     ('bfloat16', 'bfloat16'), ('float32', 'float64')])
+def test_mix_dtypes_and_alignment(gpu, sdt, ddt):
+    """Low-precision sources and destinations, through the vector path
+    (rows of 4k pairs) and the scalar one (a source or destination that is
+    not 4-element aligned, rows of other lengths): numpy on the rounded
+    inputs, to the destination's rounding."""
+    import torch
+    from tnalg_amd import _lib as L
+    from tnalg_amd import records as R
+    lib = L.load()
+    code = {'float64': L.F64, 'float32': L.F32, 'bfloat16': L.BF16}
+    st, dtt = getattr(torch, sdt), getattr(torch, ddt)
+    rng = np.random.RandomState(9)
+    ops = rng.randn(3, 2, 2)
+    opsd = _dev(ops.reshape(-1))
+    mx = R.MixList()
+    keep, expect, outs = [], [], []
+    for (Dl, Dr, acc, shift) in [(16, 64, 0, 0), (33, 128, 1, 0), (8, 64, 0, 1), (5, 7, 1, 0),
+                                 (128, 128, 1, 0)]:
+        srcs = []
+        for _ in range(2):
+            buf = torch.from_numpy(rng.randn(Dl * 2 * Dr + shift)).to(st).cuda()
+            keep.append(buf)
+            srcs.append(buf[shift:])
+        ref_src = [x.to(torch.float64).cpu().numpy().reshape(Dl, 2, Dr) for x in srcs]
+        dbuf = torch.from_numpy(rng.randn(Dl * 2 * Dr + shift)).to(dtt).cuda()
+        keep.append(dbuf)
+        dst = dbuf[shift:]
+        d0 = dst.to(torch.float64).cpu().numpy().reshape(Dl, 2, Dr)
+        ref = 0.75 * ref_src[0] + np.einsum('st,atb->asb', ops[1], ref_src[1]) * (-2.0)
+        if acc:
+            ref = ref + d0
+        expect.append(ref)
+        outs.append(dst)
+        mx.add(dst.data_ptr(), code[ddt], Dl, 2, Dr,
+               [(srcs[0].data_ptr(), code[sdt], 0.75, -1), (srcs[1].data_ptr(), code[sdt], -2.0, 1)],
+               accumulate=acc)
+    j, t = mx.arrays()
+    dj, dtd = _upload([j, t])
+    print('mix dtypes', sdt, ddt, flush=True)
+    L.check(lib.tnx_mix(_vp(dj.data_ptr()), len(j), _vp(dtd.data_ptr()), R.mix_blocks(j),
+                        _vp(opsd.data_ptr()), None, _stream()), 'mix')
+    torch.cuda.synchronize()
+    tol = {'float64': 1e-13, 'float32': 1e-6, 'bfloat16': 1e-2}[ddt]
+    for o, r in zip(outs, expect):
+        got = o.to(torch.float64).cpu().numpy().reshape(r.shape)
+        assert np.allclose(got, r, rtol=tol, atol=tol * np.abs(r).max()), (sdt, ddt, r.shape)
+
+
 @pytest.mark.parametrize('m,n,trans', [(4, 2, 0), (2, 16, 0), (256, 128, 0), (256, 128, 1),
                                        (100, 37, 1), (64, 64, 0), (8, 8, 1)])
 def test_qr_f64(gpu, m, n, trans):
@@ -501,6 +551,7 @@ def test_profile_times_the_first_krylov_step(gpu):
     g = by['gemm f64 matvec w = U R^T']
     assert g.samples == 1 and steps <= g.launches <= steps + 1
     assert g.work == flops and g.seconds > 0
+    assert g.work_all == flops * g.launches      # every launch counted with its work
     assert by['gemm f64 matvec T = F x'].samples == 1     # an empty GEMM 1, still timed
     for k, words in (('k_dots', 8 + 8), ('k_update_dots', 8 + 8 + 8),
                      ('k_update_norm', 16 + 8)):
