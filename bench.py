@@ -8,10 +8,13 @@ engine; the plain-f64 path and the CPU baseline are reported beside it, with
 the relative energy difference of every mixed solve against the f64 solve of
 the same seed.
 
-Extra JSON objects: ``roofline`` (dominant kernel = the f64 grouped GEMM,
-measured on its matvec launches with HIP events on the engine's stream; the
-bf16 matvec GEMMs are listed beside it) and ``cpu_baseline`` (the oracle's
-restatement of the reference algorithm timed on a bounded sample).
+Extra JSON objects: ``roofline`` (the kernel class of the Lanczos loop with
+the most estimated time in the timed batch, from the library's sampled timing
+of that batch: every launch of the first Krylov step of every 4th Lanczos
+call timed by its own blocks with the device wall clock; the other classes
+under ``kernels``; ``traffic`` from counter profiles, MATVEC_PMC_TRAFFIC) and
+``cpu_baseline`` (the oracle's restatement of the reference algorithm timed on
+a bounded sample across the chain).
 """
 import argparse
 import json
@@ -326,6 +329,24 @@ def live_kernels(stats):
     return sorted(out, key=lambda k: -k['est_s'])
 
 
+# HBM traffic per launch (bytes) of the matvec kernels at the mid-chain step
+# of the 64-solve N=64, chi=128 batch, with the algorithmic bytes of the same
+# launch: two rocprofv3 counter runs of `bench.py --roofline-only`
+# (--pmc FETCH_SIZE, then --pmc WRITE_SIZE; KiB per dispatch averaged over
+# the 23 launches after each loop marker, --pmc-summary), scaled by what the
+# counters read for the calibration copy of 256 MiB (FETCH_SIZE x 2.00,
+# WRITE_SIZE x 1.00).  The GEMMs are priced in flops, so only the mixer has
+# an algorithmic byte count.
+MATVEC_PMC_TRAFFIC = {
+    'mixer matvec': {'bf16': (42.5e6, 50.3e6), 'fp32': (76.1e6, 92.3e6),
+                     'fp64': (152.1e6, 184.5e6)},
+    'gemm f64 matvec T = F x': {'fp64': (117.7e6, None)},
+    'gemm f64 matvec w = U R^T': {'fp64': (134.4e6, None)},
+    'gemm bf16 matvec T = F x': {'bf16': (29.7e6, None)},
+    'gemm bf16 matvec w = U R^T': {'bf16': (42.7e6, None)},
+}
+
+
 KERNEL_OF_CLASS = {'gemm f64': 'gemm_grouped_kernel<0>/<4>', 'gemm f32': 'gemm_grouped_kernel<1>',
                    'gemm bf16 matvec T': 'gemm_grouped_kernel<3>',
                    'gemm bf16 matvec w': 'gemm_grouped_kernel<2>', 'mixer': 'mix_kernel<1>',
@@ -341,8 +362,21 @@ def roofline_object(kern, note):
         return None
     dom = kern[0]
     hip = next((v for k, v in KERNEL_OF_CLASS.items() if dom['kernel'].startswith(k)), '?')
+    pmc = MATVEC_PMC_TRAFFIC.get(dom['kernel'])
+    traffic, detail = None, None
+    if pmc:
+        # the precision of most matvec launches of the mixed schedule (bf16
+        # in 5 of the 8 sweeps), else the class's own precision
+        prec = 'bf16' if 'bf16' in pmc else next(iter(pmc))
+        traffic = pmc[prec][0]
+        detail = {'precision': prec, 'launch': 'mid-chain step, 64 solves',
+                  'by_precision': {k: {'hbm_bytes': v[0], 'algorithmic_bytes': v[1]}
+                                   for k, v in pmc.items()},
+                  'source': 'rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --roofline-only, '
+                            'scaled by the 256 MiB calibration copy'}
     return {'bound': dom['bound'], 'achieved': dom['achieved'], 'peak': dom['peak'],
-            'unit': dom['unit'], 'frac': dom['frac'], 'traffic': None,
+            'unit': dom['unit'], 'frac': dom['frac'], 'traffic': traffic,
+            'traffic_detail': detail,
             'kernel': '%s (%s)' % (hip, dom['kernel']), 'avg_launch_s': dom['avg_launch_s'],
             'measured': note, 'kernels': kern}
 
@@ -411,6 +445,73 @@ def pmc_summary(paths):
         out[path] = [{k: {c: {'n': len(v), 'mean': float(np.mean(v))} for c, v in d.items()}
                       for k, d in seg.items()} for seg in segs]
     return out
+
+
+def trace_summary(path, t0=None, t1=None):
+    """Busy and idle time of a rocprofv3 --kernel-trace CSV between host
+    monotonic clocks t0, t1 (ns, as the timed steps print them): span,
+    busy fraction, idle by gap size and by the kernel pair around each gap,
+    kernel time per kernel, and the average duration of the launches of
+    the first Krylov step of every Lanczos call (the launches that follow a
+    call's k_init_scale up to its first k_update_norm), which is what the
+    live sampled timing measures."""
+    import collections
+    import csv
+    import re
+
+    def short(n):
+        n = n.replace('(anonymous namespace)::', '').replace('void ', '')
+        return re.sub(r'\(.*', '', n)
+
+    rows = sorted((int(r['Start_Timestamp']), int(r['End_Timestamp']), short(r['Kernel_Name']),
+                   r.get('Stream_Id') or r.get('Queue_Id') or '0')
+                  for r in csv.DictReader(open(path)))
+    if t0 is not None:
+        rows = [r for r in rows if r[0] >= t0 and r[1] <= t1]
+    if not rows:
+        return {'kernels': 0}
+    a, b = rows[0][0], max(r[1] for r in rows)
+    busy, cs, ce = 0, rows[0][0], rows[0][1]
+    gaps = collections.defaultdict(lambda: [0, 0])
+    hist = collections.Counter()
+    prev = rows[0]
+    for r in rows[1:]:
+        if r[0] > ce:
+            busy += ce - cs
+            g = r[0] - ce
+            gaps[(prev[2], r[2])][0] += g
+            gaps[(prev[2], r[2])][1] += 1
+            k = 0
+            while (1 << k) * 1000 < g:
+                k += 1
+            hist[1 << k] += g
+            cs, ce, prev = r[0], r[1], r
+        else:
+            if r[1] >= ce:
+                prev = r
+            ce = max(ce, r[1])
+    busy += ce - cs
+    tot = collections.Counter()
+    for r in rows:
+        tot[r[2]] += r[1] - r[0]
+    first = collections.defaultdict(list)
+    inside = collections.defaultdict(bool)   # per stream: inside a first Krylov step
+    for r in rows:
+        q = r[3]
+        if r[2] == 'k_init_scale':
+            inside[q] = True
+        elif inside[q]:
+            first[r[2]].append(r[1] - r[0])
+            if r[2] == 'k_update_norm':
+                inside[q] = False
+    return {'span_s': (b - a) / 1e9, 'busy_s': busy / 1e9, 'busy_frac': busy / (b - a),
+            'kernels': len(rows), 'summed_kernel_s': sum(r[1] - r[0] for r in rows) / 1e9,
+            'idle_by_gap_us_le': {str(k): v / 1e9 for k, v in sorted(hist.items())},
+            'idle_by_transition': [{'prev': k[0], 'next': k[1], 'idle_s': g / 1e9, 'gaps': n}
+                                   for k, (g, n) in sorted(gaps.items(), key=lambda x: -x[1][0])[:20]],
+            'kernel_s': {k: v / 1e9 for k, v in tot.most_common(24)},
+            'first_krylov_step_avg_us': {k: {'launches': len(v), 'avg_us': float(np.mean(v)) / 1e3}
+                                         for k, v in first.items()}}
 
 
 def serving_run(n_req, rate, max_active=64, seed=0):
@@ -695,6 +796,9 @@ def main():
                     help='summarise rocprofv3 --pmc counter_collection.csv files of a '
                          '--roofline-only run: per marked loop, the mean counter value per '
                          'dispatch of every kernel, and the copy calibration')
+    ap.add_argument('--trace-summary', nargs='+', metavar='ARG',
+                    help='summarise a rocprofv3 --kernel-trace CSV: CSV [t0_ns t1_ns] (the '
+                         'monotonic clocks a timed step prints)')
     ap.add_argument('--roofline-only', action='store_true',
                     help='only the matvec-GEMM roofline measurement and a copy kernel of known '
                          'size (for rocprofv3 kernel-trace / counter runs)')
@@ -743,6 +847,10 @@ def main():
         return
     if args.pmc_summary:
         print(json.dumps(pmc_summary(args.pmc_summary), indent=1))
+        return
+    if args.trace_summary:
+        t = args.trace_summary
+        print(json.dumps(trace_summary(t[0], *(int(x) for x in t[1:3])), indent=1))
         return
     seeds = list(range(args.batch))
     cold = None
@@ -829,9 +937,10 @@ def main():
               % (len(seeds) / dt_b, 1.0 / dt_p, rel.max()), file=sys.stderr, flush=True)
     out['roofline'] = roofline_object(
         live_kernels(list(prof)),
-        'HIP events around the first Krylov step of every 4th Lanczos call of the first timed '
-        'batch (%d solves), on the stream each call runs on; algorithmic work from the '
-        'launches\' descriptors' % args.batch)
+        'every launch of the first Krylov step of every 4th Lanczos call of the first timed '
+        'batch (%d solves) timed by its own blocks (device wall-clock stamps of the first '
+        'block start and last block end, the kernel span a profiler trace shows); '
+        'algorithmic work from the launches\' descriptors' % args.batch)
     if args.serving_requests > 0:
         out['serving'] = serving_run(args.serving_requests, args.serving_rate)
         print('serving: %s' % out['serving'], file=sys.stderr, flush=True)

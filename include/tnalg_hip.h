@@ -314,8 +314,11 @@ int tnx_svd_block_jacobi(const tnx_svdj_job* jobs, int njob, int npairs, int max
  * running engine for its roofline).  tnx_profile_begin(every, max_launches)
  * arms it: in every `every`-th Lanczos call made afterwards, on any host
  * thread, the launches of the first Krylov step (every solve active) are
- * each timed with two HIP events on their stream, up to max_launches
- * launches.  tnx_profile_end waits for the device, fills out[0..cap) with
+ * each timed by their own blocks, which stamp their start and end with the
+ * device's constant wall clock (a launch's time is first start to last end,
+ * the duration a profiler's kernel trace shows), up to max_launches
+ * launches; it allocates the stamp slots (1 KiB per launch) on the current
+ * device.  tnx_profile_end waits for the device, fills out[0..cap) with
  * one entry per kernel class (TNX_PROF_NCLASS of them) and returns that
  * count.  The work of a timed launch is its algorithmic flops (GEMMs, from
  * the plan's `work`) or bytes (mixer; Lanczos passes: basis rows read and

@@ -206,7 +206,9 @@ template <int CFG>
 __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem* __restrict__ probs,
                                                           int nprob,
                                                           const tnx_gemm_seg* __restrict__ segs,
-                                                          const int32_t* __restrict__ active) {
+                                                          const int32_t* __restrict__ active,
+                                                          unsigned long long* ts) {
+  tnx_stamp_begin(ts);
   typedef Cfg<CFG> C;
   constexpr int MODE = C::MODE;
   typedef typename C::ACC ACC;
@@ -349,6 +351,7 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
           Cp[(int64_t)gi * P.ldc + gj] = from_d<TC>(v);
         }
       }
+  tnx_stamp_end(ts);
 }
 
 }  // namespace
@@ -368,6 +371,12 @@ extern "C" int tnx_gemm_variants(int mode) {
 extern "C" int tnx_gemm_grouped(int mode, const tnx_gemm_problem* prob, int nprob,
                                 const tnx_gemm_seg* seg, int ntiles, const int32_t* active,
                                 tnx_stream_t stream) {
+  return tnx_gemm_grouped_ts(mode, prob, nprob, seg, ntiles, active, stream, nullptr);
+}
+
+int tnx_gemm_grouped_ts(int mode, const tnx_gemm_problem* prob, int nprob,
+                        const tnx_gemm_seg* seg, int ntiles, const int32_t* active,
+                        tnx_stream_t stream, unsigned long long* ts) {
   if (nprob == 0 && ntiles == 0) return TNX_OK;
   if (nprob < 0 || ntiles < 0 || prob == nullptr || seg == nullptr) {
     tnx_set_error("tnx_gemm_grouped: bad arguments (nprob=%d ntiles=%d)", nprob, ntiles);
@@ -385,18 +394,18 @@ extern "C" int tnx_gemm_grouped(int mode, const tnx_gemm_problem* prob, int npro
   switch (base) {
     case TNX_GEMM_F64:
       if (var == 0)
-        hipLaunchKernelGGL(gemm_grouped_kernel<0>, grid, block, 0, s, prob, nprob, seg, active);
+        hipLaunchKernelGGL(gemm_grouped_kernel<0>, grid, block, 0, s, prob, nprob, seg, active, ts);
       else
-        hipLaunchKernelGGL(gemm_grouped_kernel<4>, grid, block, 0, s, prob, nprob, seg, active);
+        hipLaunchKernelGGL(gemm_grouped_kernel<4>, grid, block, 0, s, prob, nprob, seg, active, ts);
       break;
     case TNX_GEMM_F32:
-      hipLaunchKernelGGL(gemm_grouped_kernel<1>, grid, block, 0, s, prob, nprob, seg, active);
+      hipLaunchKernelGGL(gemm_grouped_kernel<1>, grid, block, 0, s, prob, nprob, seg, active, ts);
       break;
     case TNX_GEMM_BF16_F32:
-      hipLaunchKernelGGL(gemm_grouped_kernel<2>, grid, block, 0, s, prob, nprob, seg, active);
+      hipLaunchKernelGGL(gemm_grouped_kernel<2>, grid, block, 0, s, prob, nprob, seg, active, ts);
       break;
     default:
-      hipLaunchKernelGGL(gemm_grouped_kernel<3>, grid, block, 0, s, prob, nprob, seg, active);
+      hipLaunchKernelGGL(gemm_grouped_kernel<3>, grid, block, 0, s, prob, nprob, seg, active, ts);
       break;
   }
   TNX_CHECK_LAUNCH();

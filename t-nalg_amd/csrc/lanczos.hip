@@ -373,7 +373,9 @@ __global__ __launch_bounds__(NT) void k_init_scale(const tnx_lanczos_solve* __re
 
 // pass 1: chunk partials of V[k] . w, k <= j -> partA
 __global__ __launch_bounds__(NT) void k_dots(const tnx_lanczos_solve* __restrict__ S, int nsolve,
-                                             void* state, int nchunks, int j, int w_dtype) {
+                                             void* state, int nchunks, int j, int w_dtype,
+                                             unsigned long long* ts) {
+  tnx_stamp_begin(ts);
   LzLayout L = lz_layout(state, nsolve, nchunks);
   ChunkInfo ci = chunk_of(S, nsolve);
   if (L.act[ci.s] == 0) return;
@@ -391,12 +393,15 @@ __global__ __launch_bounds__(NT) void k_dots(const tnx_lanczos_solve* __restrict
   __syncthreads();
   for (int k = threadIdx.x; k <= j; k += NT)
     L.partA[(int64_t)blockIdx.x * (NCV_MAX + 1) + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+  tnx_stamp_end(ts);
 }
 
 // h1 = sum of the pass-1 partials; r = w - sum h1[k] V[k] -> V[j+1];
 // pass 2: chunk partials of V[k] . r (k <= j) and of ||r||^2 (slot j+1) -> partB
 __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __restrict__ S, int nsolve,
-                                                    void* state, int nchunks, int j, int w_dtype) {
+                                                    void* state, int nchunks, int j, int w_dtype,
+                                                    unsigned long long* ts) {
+  tnx_stamp_begin(ts);
   LzLayout L = lz_layout(state, nsolve, nchunks);
   ChunkInfo ci = chunk_of(S, nsolve);
   if (L.act[ci.s] == 0) return;
@@ -430,6 +435,7 @@ __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __r
   __syncthreads();
   for (int k = threadIdx.x; k <= j + 1; k += NT)
     L.partB[(int64_t)blockIdx.x * (NCV_MAX + 1) + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+  tnx_stamp_end(ts);
 }
 
 // h2 = sum of the pass-2 partials (slots 0..j), ||r||^2 = slot j+1;
@@ -454,7 +460,9 @@ __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __r
 // of after it.
 __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __restrict__ S, int nsolve,
                                                     void* state, int nchunks, int j, int x_dtype,
-                                                    int normalize, int min_iters) {
+                                                    int normalize, int min_iters,
+                                                    unsigned long long* ts) {
+  tnx_stamp_begin(ts);
   LzLayout L = lz_layout(state, nsolve, nchunks);
   const bool solver = (int)blockIdx.x < nsolve;
   ChunkInfo ci;
@@ -503,6 +511,7 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
         tnx_store((void*)D.x, xd, i, v);
       }
     }
+    tnx_stamp_end(ts);
     return;
   }
   LzState& st = L.st[s];
@@ -545,6 +554,7 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
       atomicSub(L.nact, 1);
     }
   }
+  tnx_stamp_end(ts);
 }
 
 // Sturm count (eigenvalues below x) of the m x m tridiagonal held in
@@ -898,17 +908,19 @@ int issue_step(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve*
   tnx_prof_count(TNX_PC_UPDATE_DOTS, E * (vrows + ws + 8.0));
   tnx_prof_count(TNX_PC_UPDATE_NORM, E * (16.0 + xs));
   int mk = prof ? tnx_prof_mark(TNX_PC_DOTS, s) : -1;
-  hipLaunchKernelGGL(k_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
+  hipLaunchKernelGGL(k_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype,
+                     tnx_prof_ts(mk));
   tnx_prof_done(mk, s, E * (vrows + ws));
   mk = prof ? tnx_prof_mark(TNX_PC_UPDATE_DOTS, s) : -1;
-  hipLaunchKernelGGL(k_update_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype);
+  hipLaunchKernelGGL(k_update_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype,
+                     tnx_prof_ts(mk));
   tnx_prof_done(mk, s, E * (vrows + ws + 8.0));
   // the chunk blocks normalise V[j+1] at every step, the last of a cycle
   // included (an implicit restart combines V[ncv] into the new basis)
   (void)ncv;
   mk = prof ? tnx_prof_mark(TNX_PC_UPDATE_NORM, s) : -1;
   hipLaunchKernelGGL(k_update_norm, dim3(nchunks + nsolve), block, 0, s, solves, nsolve, state,
-                     nchunks, j, x_dtype, 1, min_iters);
+                     nchunks, j, x_dtype, 1, min_iters, tnx_prof_ts(mk));
   // (r read, V[j+1] and x written; the basis rows of the second correction,
   // read only when it is not negligible, are not counted)
   tnx_prof_done(mk, s, E * (16.0 + xs));

@@ -245,7 +245,9 @@ template <int TAG>
 __global__ __launch_bounds__(256) void mix_kernel(const tnx_mix_job* __restrict__ jobs, int njob,
                                                   const tnx_mix_term* __restrict__ terms,
                                                   const double* __restrict__ ops,
-                                                  const int32_t* __restrict__ active) {
+                                                  const int32_t* __restrict__ active,
+                                                  unsigned long long* ts) {
+  tnx_stamp_begin(ts);
   const int bid = blockIdx.x;
   const int ji = tnx_find_owner(jobs, sizeof(tnx_mix_job), offsetof(tnx_mix_job, block_begin),
                                 njob, bid);
@@ -257,6 +259,7 @@ __global__ __launch_bounds__(256) void mix_kernel(const tnx_mix_job* __restrict_
     mix_body<2>(J, bid - J.block_begin, terms, ops);
   else
     mix_body<MIX_DMAX>(J, bid - J.block_begin, terms, ops);
+  tnx_stamp_end(ts);
 }
 
 __global__ __launch_bounds__(256) void trace_dot_kernel(const tnx_dot_job* __restrict__ jobs,
@@ -390,7 +393,8 @@ extern "C" int tnx_abi_sizes(int32_t* sizes, int n) {
 }
 
 static int launch_mix(const tnx_mix_job* job, int njob, const tnx_mix_term* term, int nblocks,
-                      const double* ops, const int32_t* active, tnx_stream_t stream, bool matvec) {
+                      const double* ops, const int32_t* active, tnx_stream_t stream, bool matvec,
+                      unsigned long long* ts = nullptr) {
   if (nblocks == 0 || njob == 0) return TNX_OK;
   if (job == nullptr || term == nullptr || nblocks < 0 || njob < 0) {
     tnx_set_error("tnx_mix: bad arguments (njob=%d nblocks=%d)", njob, nblocks);
@@ -398,10 +402,10 @@ static int launch_mix(const tnx_mix_job* job, int njob, const tnx_mix_term* term
   }
   if (matvec)
     hipLaunchKernelGGL(mix_kernel<1>, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, job, njob,
-                       term, ops, active);
+                       term, ops, active, ts);
   else
     hipLaunchKernelGGL(mix_kernel<0>, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, job, njob,
-                       term, ops, active);
+                       term, ops, active, ts);
   TNX_CHECK_LAUNCH();
   return TNX_OK;
 }
@@ -443,18 +447,18 @@ extern "C" int tnx_heff_matvec(const tnx_matvec_plan* plan, const int32_t* activ
   if (plan->mix_nblocks > 0) tnx_prof_count(TNX_PC_MIX, plan->work[1]);
   tnx_prof_count(gemm_class(plan->mode2, 1), plan->work[2]);
   int mk = prof ? tnx_prof_mark(gemm_class(plan->mode, 0), s) : -1;
-  int rc = tnx_gemm_grouped(plan->mode, plan->g1_prob, plan->g1_nprob, plan->g1_seg,
-                            plan->g1_ntiles, active, stream);
+  int rc = tnx_gemm_grouped_ts(plan->mode, plan->g1_prob, plan->g1_nprob, plan->g1_seg,
+                               plan->g1_ntiles, active, stream, tnx_prof_ts(mk));
   if (rc) return rc;
   tnx_prof_done(mk, s, plan->work[0]);
   mk = (prof && plan->mix_nblocks > 0) ? tnx_prof_mark(TNX_PC_MIX, s) : -1;
   rc = launch_mix(plan->mix_job, plan->mix_njob, plan->mix_term, plan->mix_nblocks, plan->ops, active,
-                  stream, true);
+                  stream, true, tnx_prof_ts(mk));
   if (rc) return rc;
   tnx_prof_done(mk, s, plan->work[1]);
   mk = prof ? tnx_prof_mark(gemm_class(plan->mode2, 1), s) : -1;
-  rc = tnx_gemm_grouped(plan->mode2, plan->g2_prob, plan->g2_nprob, plan->g2_seg,
-                        plan->g2_ntiles, active, stream);
+  rc = tnx_gemm_grouped_ts(plan->mode2, plan->g2_prob, plan->g2_nprob, plan->g2_seg,
+                           plan->g2_ntiles, active, stream, tnx_prof_ts(mk));
   if (rc) return rc;
   tnx_prof_done(mk, s, plan->work[2]);
   return TNX_OK;

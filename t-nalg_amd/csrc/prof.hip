@@ -4,13 +4,16 @@
 // tnx_profile_begin(every) arms sampling: in every `every`-th Lanczos call
 // made afterwards (on any host thread), the launches of the first Krylov
 // step (every solve still active, so the algorithmic work of each launch is
-// the full work its descriptors describe) are each bracketed by two HIP
-// events on the launch's stream.  tnx_profile_end resolves the events and
-// returns, per kernel class, the launches made while armed with their summed
-// algorithmic work (flops or bytes), and the number of timed launches, their
-// summed duration and their summed algorithmic work.  Nothing is
-// timed while sampling is not armed; the cost when armed is two event
-// records per timed launch.
+// the full work its descriptors describe) are timed by the kernels
+// themselves: each block stamps its start and end with the device's
+// constant wall clock into the launch's slot (tnx_stamp_begin / _end), so
+// a launch's time is the span from its first block's start to its last
+// block's end, the kernel duration a profiler's trace reports.
+// tnx_profile_end returns, per kernel class, the launches made while armed
+// with their summed algorithmic work (flops or bytes), and the number of
+// timed launches, their summed duration and their summed algorithmic work.
+// Nothing is timed while sampling is not armed; when armed, a timed launch
+// costs two atomics per block.
 #include <string.h>
 
 #include <atomic>
@@ -23,7 +26,6 @@ namespace {
 
 struct Sample {
   int cls;
-  hipEvent_t a, b;
   double work;
 };
 
@@ -35,6 +37,7 @@ const char* const kClassName[TNX_PROF_NCLASS] = {
     "k_update_dots",                "k_update_norm",
 };
 const int kClassUnit[TNX_PROF_NCLASS] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1};
+constexpr int kSlotWords = 2 * TNX_TS_PAIRS;
 
 std::mutex g_mu;
 std::atomic<int> g_armed{0};    // launches are counted while armed
@@ -43,7 +46,8 @@ std::atomic<double> g_work_all[TNX_PROF_NCLASS];   // algorithmic work of every 
 int g_every = 0;                // 0: not armed
 long long g_calls = 0;          // Lanczos calls seen while armed
 std::vector<Sample> g_samples;  // launches timed
-std::vector<hipEvent_t> g_pool; // events not yet used
+int g_cap = 0;                  // stamp slots
+unsigned long long* g_ts = nullptr;   // device: g_cap slots of kSlotWords
 thread_local bool t_on = false; // the current thread's launches are sampled
 
 }  // namespace
@@ -68,26 +72,25 @@ void tnx_prof_count(int cls, double work) {
 }
 
 int tnx_prof_mark(int cls, hipStream_t s) {
+  (void)s;
   if (!t_on) return -1;
   std::lock_guard<std::mutex> lock(g_mu);
-  if (g_pool.size() < 2) return -1;   // pool spent: stop timing
-  Sample smp;
-  smp.cls = cls;
-  smp.b = g_pool.back();
-  g_pool.pop_back();
-  smp.a = g_pool.back();
-  g_pool.pop_back();
-  smp.work = 0.0;
-  if (hipEventRecord(smp.a, s) != hipSuccess) return -1;
-  g_samples.push_back(smp);
+  if (g_ts == nullptr || (int)g_samples.size() >= g_cap) return -1;   // slots spent
+  g_samples.push_back(Sample{cls, 0.0});
   return (int)g_samples.size() - 1;
 }
 
+unsigned long long* tnx_prof_ts(int mark) {
+  if (mark < 0) return nullptr;
+  std::lock_guard<std::mutex> lock(g_mu);
+  return g_ts == nullptr ? nullptr : g_ts + (size_t)mark * kSlotWords;
+}
+
 void tnx_prof_done(int mark, hipStream_t s, double work) {
+  (void)s;
   if (mark < 0) return;
   std::lock_guard<std::mutex> lock(g_mu);
   g_samples[mark].work = work;
-  (void)hipEventRecord(g_samples[mark].b, s);
 }
 
 extern "C" int tnx_profile_begin(int every, int max_launches) {
@@ -101,11 +104,21 @@ extern "C" int tnx_profile_begin(int every, int max_launches) {
     tnx_set_error("tnx_profile_begin: already armed");
     return TNX_EINVAL;
   }
-  for (int i = 0; i < 2 * max_launches; ++i) {
-    hipEvent_t e;
-    TNX_CHECK_HIP(hipEventCreate(&e));
-    g_pool.push_back(e);
+  // stamp slots: starts at the largest clock value, ends at zero
+  std::vector<unsigned long long> init((size_t)max_launches * kSlotWords);
+  for (size_t i = 0; i < init.size(); i += 2) {
+    init[i] = ~0ull;
+    init[i + 1] = 0ull;
   }
+  TNX_CHECK_HIP(hipMalloc((void**)&g_ts, init.size() * sizeof(unsigned long long)));
+  if (hipMemcpy(g_ts, init.data(), init.size() * sizeof(unsigned long long),
+                hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(g_ts);
+    g_ts = nullptr;
+    tnx_set_error("tnx_profile_begin: slot upload failed");
+    return TNX_EHIP;
+  }
+  g_cap = max_launches;
   g_samples.clear();
   g_calls = 0;
   g_every = every;
@@ -119,7 +132,7 @@ extern "C" int tnx_profile_begin(int every, int max_launches) {
 
 extern "C" int tnx_profile_end(tnx_prof_stat* out, int cap) {
   std::vector<Sample> samples;
-  std::vector<hipEvent_t> pool;
+  unsigned long long* ts = nullptr;
   {
     std::lock_guard<std::mutex> lock(g_mu);
     if (g_every <= 0) {
@@ -129,9 +142,19 @@ extern "C" int tnx_profile_end(tnx_prof_stat* out, int cap) {
     g_every = 0;
     g_armed.store(0);
     samples.swap(g_samples);
-    pool.swap(g_pool);
+    ts = g_ts;
+    g_ts = nullptr;
+    g_cap = 0;
   }
   TNX_CHECK_HIP(hipDeviceSynchronize());
+  std::vector<unsigned long long> stamps(samples.size() * kSlotWords);
+  int dev = 0, khz = 0;
+  const bool ok = hipMemcpy(stamps.data(), ts, stamps.size() * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost) == hipSuccess &&
+                  hipGetDevice(&dev) == hipSuccess &&
+                  hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess &&
+                  khz > 0;
+  (void)hipFree(ts);
   tnx_prof_stat st[TNX_PROF_NCLASS];
   for (int c = 0; c < TNX_PROF_NCLASS; ++c) {
     memset(&st[c], 0, sizeof(st[c]));
@@ -140,26 +163,25 @@ extern "C" int tnx_profile_end(tnx_prof_stat* out, int cap) {
     st[c].launches = g_count[c].load();
     st[c].work_all = g_work_all[c].load();
   }
-  int rc = TNX_OK;
-  for (const Sample& smp : samples) {
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, smp.a, smp.b) == hipSuccess) {
-      st[smp.cls].samples += 1;
-      st[smp.cls].seconds += 1e-3 * (double)ms;
-      st[smp.cls].work += smp.work;
-    } else {
-      rc = TNX_EHIP;
-    }
-    (void)hipEventDestroy(smp.a);
-    (void)hipEventDestroy(smp.b);
+  if (!ok) {
+    tnx_set_error("tnx_profile_end: stamps not read back");
+    return TNX_EHIP;
   }
-  for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  for (size_t k = 0; k < samples.size(); ++k) {
+    const unsigned long long* w = stamps.data() + k * kSlotWords;
+    unsigned long long a = ~0ull, b = 0ull;
+    for (int p = 0; p < TNX_TS_PAIRS; ++p) {
+      if (w[2 * p] < a) a = w[2 * p];
+      if (w[2 * p + 1] > b) b = w[2 * p + 1];
+    }
+    if (a == ~0ull || b <= a) continue;   // no block stamped (every solve masked)
+    const Sample& smp = samples[k];
+    st[smp.cls].samples += 1;
+    st[smp.cls].seconds += (double)(b - a) / (1e3 * (double)khz);
+    st[smp.cls].work += smp.work;
+  }
   const int n = cap < TNX_PROF_NCLASS ? cap : TNX_PROF_NCLASS;
   if (out != nullptr)
     for (int c = 0; c < n; ++c) out[c] = st[c];
-  if (rc) {
-    tnx_set_error("tnx_profile_end: an event was not resolved");
-    return rc;
-  }
   return TNX_PROF_NCLASS;
 }

@@ -25,8 +25,29 @@ bool tnx_prof_call_begin();      // this call is sampled (timing stays off)
 void tnx_prof_set(bool on);      // timing on / off for the calling thread
 bool tnx_prof_on();
 int tnx_prof_mark(int cls, hipStream_t s);                // -1: not timed
+unsigned long long* tnx_prof_ts(int mark);                // the launch's stamp slot (or null)
 void tnx_prof_done(int mark, hipStream_t s, double work);
 void tnx_prof_count(int cls, double work);   // one launch of the class and its work (while armed)
+
+// Launch-span stamps of a timed launch: the slot holds TNX_TS_PAIRS pairs
+// (earliest block start, latest block end) of the device's constant wall
+// clock; block b updates pair b % TNX_TS_PAIRS with vector atomics (spread
+// over pairs, so no address takes more than a few dozen), and the host
+// takes the minimum start and maximum end.  A null slot: not timed.
+constexpr int TNX_TS_PAIRS = 64;
+__device__ __forceinline__ void tnx_stamp_begin(unsigned long long* ts) {
+  if (ts != nullptr && threadIdx.x == 0)
+    atomicMin(ts + 2 * (blockIdx.x % TNX_TS_PAIRS), (unsigned long long)wall_clock64());
+}
+__device__ __forceinline__ void tnx_stamp_end(unsigned long long* ts) {
+  if (ts != nullptr && threadIdx.x == 0)
+    atomicMax(ts + 2 * (blockIdx.x % TNX_TS_PAIRS) + 1, (unsigned long long)wall_clock64());
+}
+// Grouped GEMM launch with a stamp slot (gemm.hip); tnx_gemm_grouped is this
+// with a null slot.
+int tnx_gemm_grouped_ts(int mode, const tnx_gemm_problem* prob, int nprob,
+                        const tnx_gemm_seg* seg, int ntiles, const int32_t* active,
+                        tnx_stream_t stream, unsigned long long* ts);
 
 #define TNX_CHECK_HIP(call)                                                      \
   do {                                                                           \
