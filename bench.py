@@ -364,14 +364,32 @@ def roofline_object(kern, note):
     hip = next((v for k, v in KERNEL_OF_CLASS.items() if dom['kernel'].startswith(k)), '?')
     pmc = MATVEC_PMC_TRAFFIC.get(dom['kernel'])
     traffic, detail = None, None
-    if pmc:
-        # the precision of most matvec launches of the mixed schedule (bf16
-        # in 5 of the 8 sweeps), else the class's own precision
-        prec = 'bf16' if 'bf16' in pmc else next(iter(pmc))
+    if pmc and dom['unit'] == 'GB/s' and all(v[1] for v in pmc.values()):
+        # the mixer serves every precision: weight the measured traffic per
+        # algorithmic byte of each precision by that precision's share of
+        # the timed matvec launches (its GEMM class's timed launches), and
+        # apply it to the live average launch
+        share = {}
+        for k in kern:
+            if k['kernel'].startswith('gemm') and 'T = F x' in k['kernel']:
+                share['fp64' if 'f64' in k['kernel'] else 'fp32' if 'f32' in k['kernel']
+                      else 'bf16'] = k['timed_launches']
+        tot = sum(share.get(p, 0) for p in pmc) or 1
+        hbm = sum(share.get(p, 0) * v[0] for p, v in pmc.items()) / tot
+        alg = sum(share.get(p, 0) * v[1] for p, v in pmc.items()) / tot
+        ratio = hbm / alg if alg else None
+        traffic = dom['bytes_per_launch'] * ratio if ratio else None
+        detail = {'hbm_per_algorithmic_byte': ratio,
+                  'launch_shares': {p: share.get(p, 0) / tot for p in pmc},
+                  'mid_chain_by_precision': {k: {'hbm_bytes': v[0], 'algorithmic_bytes': v[1]}
+                                             for k, v in pmc.items()},
+                  'source': 'rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --roofline-only '
+                            '(mid-chain step, 64 solves), scaled by the 256 MiB calibration copy; '
+                            'traffic = the live average launch x the share-weighted ratio'}
+    elif pmc:
+        prec = next(iter(pmc))
         traffic = pmc[prec][0]
         detail = {'precision': prec, 'launch': 'mid-chain step, 64 solves',
-                  'by_precision': {k: {'hbm_bytes': v[0], 'algorithmic_bytes': v[1]}
-                                   for k, v in pmc.items()},
                   'source': 'rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --roofline-only, '
                             'scaled by the 256 MiB calibration copy'}
     return {'bound': dom['bound'], 'achieved': dom['achieved'], 'peak': dom['peak'],
