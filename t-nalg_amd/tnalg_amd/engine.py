@@ -167,10 +167,16 @@ class GemmTuner:
         self.table[key] = best
 
 
+# the current raw stream of a device (torch's own accessor of it)
+_RAW_STREAM = getattr(torch._C, '_cuda_getCurrentRawStream', None)
+
+
 class _Resident:
     """A set of descriptor arrays uploaded once and kept in device memory
-    (steps that repeat every sweep reuse them without host work)."""
-    __slots__ = ('buf', 'ptrs', 'meta')
+    (steps that repeat every sweep reuse them without host work).  ``host``:
+    host objects built for it once (a Lanczos call's plan array and result
+    array)."""
+    __slots__ = ('buf', 'ptrs', 'meta', 'host')
 
     def __init__(self, device, arrays, meta, stager):
         offs, total = [], 0
@@ -182,6 +188,7 @@ class _Resident:
         base = self.buf.data_ptr()
         self.ptrs = [base + o for o in offs]
         self.meta = meta
+        self.host = None
 
 
 def max_dims(dims, d, ob):
@@ -1555,6 +1562,8 @@ class Engine:
         self._pending_fill = []      # initial-MPS fill jobs not yet launched
         self.lib = L.load()
         self.device = torch.device(device)
+        self._dev_index = self.device.index if self.device.index is not None else \
+            (torch.cuda.current_device() if self.device.type == 'cuda' else 0)
         self.ncv = int(ncv)
         if not 2 <= self.ncv <= L.LZ_NCV_MAX:
             raise ValueError('ncv must be in [2, %d]' % L.LZ_NCV_MAX)
@@ -1716,6 +1725,11 @@ class Engine:
         self.ops_free = merged
 
     def stream(self):
+        """Torch's current stream on the engine's device, as a HIP stream
+        handle (the raw handle is read directly: building a Stream object
+        per launch costs more than the launch)."""
+        if _RAW_STREAM is not None:
+            return ctypes_void(_RAW_STREAM(self._dev_index))
         return ctypes_void(torch.cuda.current_stream().cuda_stream)
 
     # ------------------------------------------------------------- arenas
@@ -1919,7 +1933,7 @@ class Engine:
         """Encoded mode of a launch whose shape class may have been tuned."""
         return mode if self.tuner is None else self.tuner.mode_for(mode, ntiles, kmax)
 
-    def _lanczos(self, groups, spec=None, ahead=None):
+    def _lanczos(self, groups, spec=None, ahead=None, parts=None):
         """One batched Lanczos solve over ``groups``: lists of (solve, plan),
         one list per matvec precision (each Krylov step runs every group's
         matvec, then the shared f64 kernels).  ``spec``: descriptor sets
@@ -1931,10 +1945,14 @@ class Engine:
         streams, each driven by its own host thread: one half's matvec
         kernels (latency bound) overlap the other half's passes over its
         Krylov basis (bandwidth bound).  A solve's arithmetic does not depend
-        on the other solves of its call, so the results are the same bits."""
-        parts = _split_groups(groups, self.lanczos_streams, self.split_min)
+        on the other solves of its call, so the results are the same bits.
+        ``parts``: the split already made, as ``(groups, key)`` pairs."""
+        if parts is None:
+            parts = [(part, _lanczos_key(part))
+                     for part in _split_groups(groups, self.lanczos_streams, self.split_min)]
         got = spec.get('lz') if spec else None
-        preps = [self._lanczos_prepare(k, part, got) for k, part in enumerate(parts)]
+        preps = [self._lanczos_prepare(k, part, got, rkey)
+                 for k, (part, rkey) in enumerate(parts)]
         if ahead:
             self._start_ahead(ahead)
         restarts = self.cycle_restarts if self.cycle_steps else self.max_restarts
@@ -1989,12 +2007,11 @@ class Engine:
                                       '(residual %.3g after %d matvecs)'
                                       % (s._lz_site, float(res['resid'][k]), it))
 
-    def _lanczos_prepare(self, k, groups, got):
-        """Resident descriptors, the linear-combination mix (on the engine's
-        stream), matvec plans with tuned GEMM variants, the state buffer and
-        the result array of Lanczos call ``k``."""
+    def _lanczos_prepare(self, k, groups, got, rkey):
+        """Resident descriptors (key ``rkey``), the linear-combination mix
+        (on the engine's stream), matvec plans with tuned GEMM variants, the
+        state buffer and the result array of Lanczos call ``k``."""
         flat = [sp for g in groups for sp in g]
-        rkey = _lanczos_key(groups)
         ent = self._resident_get(rkey)
         if ent is None:
             ahead = dict(got).get(rkey) if got else None
@@ -2005,21 +2022,26 @@ class Engine:
             L.check(self.lib.tnx_mix(ctypes_void(ptrs[0]), meta['lc'][0], ctypes_void(ptrs[1]),
                                      meta['lc'][1], ctypes_void(self.ops.data_ptr()), None,
                                      self.stream()), 'tnx_mix')
-        plans = (L.MatvecPlan * len(groups))()
-        for q, (mode1, mode2, g1, mx, g2, work) in enumerate(meta['groups']):
-            b = 3 + 6 * q
-            m1, m2 = self._tuned_gemm(mode1, ptrs[b], ptrs[b + 1], g1), \
-                self._tuned_gemm(mode2, ptrs[b + 4], ptrs[b + 5], g2)
-            plans[q] = L.MatvecPlan(ptrs[b], ptrs[b + 1], g1[0], g1[1], ptrs[b + 2], ptrs[b + 3],
-                                    mx[0], mx[1], ptrs[b + 4], ptrs[b + 5], g2[0], g2[1],
-                                    self.ops.data_ptr(), m1, m2, (ctypes.c_double * 3)(*work))
-        need = int(self.lib.tnx_lanczos_state_bytes(len(flat), meta['nchunks'], self.ncv))
+        if ent.host is None:
+            # the plan array (GEMM variants tuned on the first build) and the
+            # result array, reused every time this descriptor set runs
+            plans = (L.MatvecPlan * len(groups))()
+            for q, (mode1, mode2, g1, mx, g2, work) in enumerate(meta['groups']):
+                b = 3 + 6 * q
+                m1, m2 = self._tuned_gemm(mode1, ptrs[b], ptrs[b + 1], g1), \
+                    self._tuned_gemm(mode2, ptrs[b + 4], ptrs[b + 5], g2)
+                plans[q] = L.MatvecPlan(ptrs[b], ptrs[b + 1], g1[0], g1[1], ptrs[b + 2],
+                                        ptrs[b + 3], mx[0], mx[1], ptrs[b + 4], ptrs[b + 5],
+                                        g2[0], g2[1], self.ops.data_ptr(), m1, m2,
+                                        (ctypes.c_double * 3)(*work))
+            need = int(self.lib.tnx_lanczos_state_bytes(len(flat), meta['nchunks'], self.ncv))
+            ent.host = (plans, np.zeros(len(flat), dtype=L.LANCZOS_RESULT), need)
+        plans, res, need = ent.host
         if self.lz_states[k] is None or self.lz_states[k].numel() < need:
             self.lz_states[k] = torch.empty(max(need, 1 << 20), dtype=torch.uint8,
                                             device=self.device)
         return {'flat': flat, 'plans': plans, 'ngroups': len(groups), 'ptrs': ptrs,
-                'nchunks': meta['nchunks'], 'state': self.lz_states[k],
-                'res': np.zeros(len(flat), dtype=L.LANCZOS_RESULT)}
+                'nchunks': meta['nchunks'], 'state': self.lz_states[k], 'res': res}
 
     def _lanczos_call(self, prep, stream, restarts):
         """The C Lanczos loop of one prepared call on ``stream`` (holds no
@@ -2044,6 +2066,10 @@ class Engine:
         t_a = time.perf_counter()
         plans = []
         casts = []   # low-precision copies of every block, for solves changing precision
+        # solves in lockstep share their step's plan: memo of (precision,
+        # plan) per signature, step and bond-dimension object (the value
+        # keeps that object alive, so an identical id is the same object)
+        memo = {}
         for s in solves:
             st = s.queue.pop(0)
             s._lz_site = st.site
@@ -2052,12 +2078,21 @@ class Engine:
                 prec, ph = nx[1], nx[2]
                 self.counters['ahead_used'] += 1
             else:
-                prec, ph = self._prec_of(s, st), None
+                d = s.dims
+                mk = (s.sig, st.kind, st.site, st.dir, st.gate, st.sweep, id(d),
+                      s.prec_override, s.para.get('sweep_time', 1))
+                hit = memo.get(mk)
+                if hit is not None and hit[0] is d:
+                    prec, ph = hit[1], hit[2]
+                else:
+                    prec, ph = self._prec_of(s, st), None
+            # the casts read the bond dimensions before the step
             if self.uses_lowp and prec != 'fp64' and s.lowp_dtype != PREC[prec][0]:
                 casts.append(s.cast_all_records(PREC[prec][0]))
                 s.lowp_dtype = PREC[prec][0]
             if ph is None:
                 ph = s.plan(st, prec)
+                memo[mk] = (d, prec, ph)
             else:
                 s.dims = ph['dims_after_t']
             plans.append((s, st, ph))
@@ -2147,13 +2182,13 @@ class Engine:
             # the next step's host work overlaps this Lanczos call, when this
             # step needed host work (a batch whose descriptor sets are all
             # resident repeats a known sequence and has none to overlap)
-            host_miss = host_miss or any(
-                _lanczos_key(part) not in self._resident
-                for part in _split_groups(groups, self.lanczos_streams, self.split_min))
+            parts = [(part, _lanczos_key(part))
+                     for part in _split_groups(groups, self.lanczos_streams, self.split_min)]
+            host_miss = host_miss or any(key not in self._resident for _, key in parts)
             ahead = None
             if self.plan_ahead and host_miss:
                 ahead = [s for s, _, _ in plans if s.queue]
-            self._lanczos(groups, spec, ahead)
+            self._lanczos(groups, spec, ahead, parts)
         t_d = time.perf_counter()
         self.counters['t_plan'] += t_b - t_a
         self.counters['t_pre'] += t_c - t_b
