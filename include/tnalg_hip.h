@@ -372,6 +372,41 @@ typedef struct tnx_fill_job {
 } tnx_fill_job; /* 32 bytes */
 int tnx_fill_normal_f64(const tnx_fill_job* jobs, int njob, tnx_stream_t stream);
 
+/* Batched Kronecker sums: dst[(i,k),(j,l)] = sum_t coef_t A_t[i,j] B_t[k,l]
+ * over the job's terms (row-major (p q) x (p q) dst with leading dimension
+ * ldd; A_t p x p with lda, B_t q x q with ldb; a null factor is the
+ * identity), KRON = 256 dst elements per workgroup.  Builds the blocks of
+ * the infinite-size driver's effective Hamiltonian with one cell absorbed
+ * into each environment: env_L (x) T (the cont of env, T, T, env in
+ * effective_hamiltonian, MPSClass.py:1831-1845) and bath (x) 1 +
+ * sum j eff (x) op (the np.kron sums of effective_hamilt_from_op,
+ * MPSClass.py:1709-1727).                                                   */
+typedef struct tnx_kron_term {
+  uint64_t a, b;       /* f64 factors (0: identity)                          */
+  int32_t lda, ldb;
+  double coef;
+} tnx_kron_term; /* 32 bytes */
+typedef struct tnx_kron_job {
+  uint64_t dst;
+  int32_t p, q, ldd;
+  int32_t term_begin, term_count;
+  int32_t block_begin; /* prefix sum of ceil((p q)^2 / 256) over jobs        */
+} tnx_kron_job; /* 32 bytes */
+int tnx_kron_f64(const tnx_kron_job* jobs, int njob, const tnx_kron_term* terms, int nblocks,
+                 tnx_stream_t stream);
+/* Batched in-place symmetrisation X <- (X + X^T) / 2 of `count` n x n
+ * blocks (leading dimension ld, block m at x + m * stride elements), 256
+ * elements of the blocks per workgroup.  Replaces the symmetrisations of
+ * the environments and the bath of the infinite-size driver
+ * (MPSClass.py:1652-1664,1763).                                             */
+typedef struct tnx_sym_job {
+  uint64_t x;
+  int32_t n, ld, count, stride;
+  int32_t block_begin; /* prefix sum of ceil(count n^2 / 256) over jobs      */
+  int32_t reserved;
+} tnx_sym_job; /* 32 bytes */
+int tnx_symmetrize_f64(const tnx_sym_job* jobs, int njob, int nblocks, tnx_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

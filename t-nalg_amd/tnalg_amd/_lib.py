@@ -41,6 +41,13 @@ SVD_JOB = np.dtype([('src', '<u8'), ('u', '<u8'), ('s', '<u8'), ('svt', '<u8'), 
                     ('u_trans', '<i4'), ('reserved', '<i4')])
 NORM_JOB = np.dtype([('x', '<u8'), ('n', '<i8')])
 FILL_JOB = np.dtype([('dst', '<u8'), ('n', '<i8'), ('seed', '<u8'), ('offset', '<i8')])
+KRON_TERM = np.dtype([('a', '<u8'), ('b', '<u8'), ('lda', '<i4'), ('ldb', '<i4'),
+                      ('coef', '<f8')])
+KRON_JOB = np.dtype([('dst', '<u8'), ('p', '<i4'), ('q', '<i4'), ('ldd', '<i4'),
+                     ('term_begin', '<i4'), ('term_count', '<i4'), ('block_begin', '<i4')])
+SYM_JOB = np.dtype([('x', '<u8'), ('n', '<i4'), ('ld', '<i4'), ('count', '<i4'),
+                    ('stride', '<i4'), ('block_begin', '<i4'), ('reserved', '<i4')])
+KRON_BLOCK = 256   # dst elements per workgroup of tnx_kron_f64 / tnx_symmetrize_f64
 DOT_JOB = np.dtype([('x', '<u8'), ('y', '<u8'), ('rows', '<i4'), ('cols', '<i4'),
                     ('ldx', '<i4'), ('ldy', '<i4')])
 CHOL_JOB = np.dtype([('g', '<u8'), ('r', '<u8'), ('rinv', '<u8'), ('status', '<u8'),
@@ -135,6 +142,8 @@ SIGNATURES = {
     'tnx_trace_dot': (_i, [_vp, _i, _vp, _vp]),
     'tnx_normalize_f64': (_i, [_vp, _i, _vp]),
     'tnx_fill_normal_f64': (_i, [_vp, _i, _vp]),
+    'tnx_kron_f64': (_i, [_vp, _i, _vp, _i, _vp]),
+    'tnx_symmetrize_f64': (_i, [_vp, _i, _i, _vp]),
     'tnx_profile_begin': (_i, [_i, _i]),
     'tnx_profile_end': (_i, [ctypes.POINTER(ProfStat), _i]),
 }
@@ -160,12 +169,13 @@ def load(path=None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    sizes = (ctypes.c_int32 * 16)()
-    n = lib.tnx_abi_sizes(sizes, 16)
+    sizes = (ctypes.c_int32 * 32)()
+    n = lib.tnx_abi_sizes(sizes, 32)
     want = [GEMM_SEG.itemsize, GEMM_PROBLEM.itemsize, MIX_JOB.itemsize, MIX_TERM.itemsize,
             ctypes.sizeof(MatvecPlan), LANCZOS_SOLVE.itemsize, LANCZOS_RESULT.itemsize,
             QR_JOB.itemsize, SVD_JOB.itemsize, DOT_JOB.itemsize, CHOL_JOB.itemsize,
-            SVDJ_JOB.itemsize, ctypes.sizeof(ProfStat), NORM_JOB.itemsize, FILL_JOB.itemsize]
+            SVDJ_JOB.itemsize, ctypes.sizeof(ProfStat), NORM_JOB.itemsize, FILL_JOB.itemsize,
+            KRON_TERM.itemsize, KRON_JOB.itemsize, SYM_JOB.itemsize]
     got = list(sizes[:n])
     if got != want:
         raise RuntimeError('ABI size mismatch: library %s, binding %s' % (got, want))

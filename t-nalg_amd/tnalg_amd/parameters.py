@@ -424,3 +424,76 @@ def hamiltonian_heisenberg(spin, jx, jy, jz, hx, hz):
     h = h - hx * (np.kron(op['id'], op['sx']) + np.kron(op['sx'], op['id']))
     h = h - hz * (np.kron(op['id'], op['sz']) + np.kron(op['sz'], op['id']))
     return np.real(h)
+
+
+# ---------------------------------------------------------------- iDMRG
+def generate_parameters_infinite_dmrg():
+    """``library/Parameters.py:430-452``: two-site infinite-size DMRG of the
+    XXZ chain in a field, 'mpo' type (cell tensor of the shift gate
+    I - tau h), chi = 16, up to 1000 iterations with an energy test every
+    ``dt_ob``."""
+    para = {'dmrg_type': 'mpo', 'spin': 'half', 'jxy': 0, 'jz': 1, 'hx': 0.5, 'hz': 0,
+            'n_site': 2, 'chi': 16, 'sweep_time': 1000, 'tau': 1e-4, 'eigs_tol': 1e-15,
+            'break_tol': 2e-10, 'is_symme_env': False, 'is_real': True, 'form': 'center_ort',
+            'dt_ob': 10}
+    para['data_path'] = os.path.join(os.getcwd(), 'data_idmrg')
+    return make_para_consistent_idmrg(para)
+
+
+def make_para_consistent_idmrg(para):
+    """``library/Parameters.py:455-467``: 'white' forces symmetric
+    environments; ``hamilt_index`` rows (op1, op2, coefficient) of the
+    Heisenberg bond with the field split over the bond's two sites."""
+    if para['dmrg_type'] not in ('mpo', 'white'):
+        print("Bad para['dmrg_type']. Set to 'white'")
+        para['dmrg_type'] = 'white'
+    if para['dmrg_type'] == 'white':
+        print("Warning: dmrg_type=='white' only suits for nearest-neighbor chains")
+        print('In this mode, self.is_symme_env is set as True')
+        para['is_symme_env'] = True
+    para['model'] = 'heisenberg'
+    para['hamilt_index'] = hamiltonian_indexes(
+        para['model'], (para['jxy'], para['jz'], -para['hx'] / 2, -para['hz'] / 2))
+    para['d'] = physical_dim_from_spin(para['spin'])
+    return para
+
+
+def hamiltonian_indexes(model, parameters):
+    """``library/HamiltonianModule.py:97-138``: rows (op1, op2, coefficient)
+    over the operators [id, sx, sy, sz, su, sd]; 'heisenberg' takes
+    (jud, jzz, hx, hz), 'q-ising' (jzz, hx)."""
+    if model == 'heisenberg':
+        jud, jzz, hx, hz = parameters
+        rows = [(4, 5, jud), (5, 4, jud), (3, 3, jzz), (0, 1, hx), (1, 0, hx), (0, 3, hz),
+                (3, 0, hz)]
+    elif model == 'q-ising':
+        jzz, hx = parameters
+        rows = [(3, 3, jzz), (0, 1, hx), (1, 0, hx)]
+    else:
+        raise ValueError('unknown model %r' % (model,))
+    return np.array(rows, dtype=float)
+
+
+def hamiltonian2cell_tensor(h, tau, way='shift'):
+    """Cell tensor of a two-site Hamiltonian (``library/HamiltonianModule.py:177-200``):
+    the gate g = I - tau h ('shift') or expm(-tau h) is split by an SVD over
+    (s1 s1') | (s2 s2') into vl (s1, s1', k) and vr (k, s2, s2'), each with
+    the square roots of the singular values, and
+    ``T[k, (x y), (c e), q] = sum_ab vr[k, x, a] g[(a b), (c e)] vl[y, b, q]``:
+    one cell of two spins with the intra-cell gate inside and the halves of
+    the gates to the neighbouring cells on its MPO bonds k (left) and q
+    (right).  Shape (d^2, d^2, d^2, d^2)."""
+    dd = h.shape[0]
+    d = int(round(dd ** 0.5))
+    if way == 'shift':
+        g = np.eye(dd) - tau * h
+    else:
+        from scipy.linalg import expm
+        g = expm(-tau * h)
+    g4 = g.reshape(d, d, d, d)
+    u, s, vt = np.linalg.svd(g4.transpose(0, 2, 1, 3).reshape(dd, dd))
+    root = np.sqrt(s)
+    vl = (u * root[None, :]).reshape(d, d, dd)
+    vr = (root[:, None] * vt).reshape(dd, d, d)
+    t = np.einsum('kxa,abce,ybq->kxyceq', vr, g4, vl)
+    return t.reshape(dd, dd, dd, dd)

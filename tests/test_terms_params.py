@@ -145,9 +145,10 @@ def test_library_exports_header_symbols():
 def test_library_abi_struct_sizes():
     from tnalg_amd import _lib
     lib = _lib.load()
-    sizes = (ctypes.c_int32 * 16)()
-    n = lib.tnx_abi_sizes(sizes, 16)
-    assert list(sizes[:n]) == [48, 56, 48, 24, 112, 88, 24, 56, 64, 32, 56, 64, 96, 16, 32]
+    sizes = (ctypes.c_int32 * 32)()
+    n = lib.tnx_abi_sizes(sizes, 32)
+    assert list(sizes[:n]) == [48, 56, 48, 24, 112, 88, 24, 56, 64, 32, 56, 64, 96, 16, 32,
+                               32, 32, 32]
 
 
 def test_easystart_coefficient_convention():
