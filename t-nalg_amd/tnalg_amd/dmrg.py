@@ -85,14 +85,18 @@ def mixed_schedule(low='bf16', mid='fp32', n_fp64=1, n_mid=2):
 
 
 def default_mixed_schedule(sweep, nsweeps):
-    """bf16 for the first quarter of the sweeps (from the random start the
-    local problems are far from converged), f32 up to the last two, f64 for
-    the last two (one sweep when there are fewer than four).  Measured on the
-    GPU: every 8-sweep schedule of this shape ends within 1e-11 of the f64
-    energy on 16- and 32-site chains (bf16 for six of eight sweeps does not:
+    """bf16 for the first sweep (from the random start the local problems
+    are far from converged and bf16 matvecs are the cheapest), f32 up to the
+    last two, f64 for the last two (one sweep when there are fewer than
+    four).  A second bf16 sweep buys little: its eigensolves stop at the
+    bf16 residual floor, and the first f32 sweep then does the same work.
+    Measured on MI355X, N=64, chi=128, 8 sweeps, 64 solves: one bf16 sweep
+    27.7 solves/s at |dE/E| <= 1.5e-13 against f64, two bf16 sweeps 26.1;
+    every 8-sweep schedule of this shape ends within 1e-11 of the f64 energy
+    on 16- and 32-site chains (bf16 for six of eight sweeps does not:
     1.6e-5)."""
     n_fp64 = 2 if nsweeps >= 4 else 1
-    n_low = max(0, nsweeps // 4) if nsweeps >= 4 else 0
+    n_low = 1 if nsweeps >= 4 else 0
     if sweep >= nsweeps - n_fp64:
         return 'fp64'
     if sweep < n_low:

@@ -1977,9 +1977,25 @@ class Engine:
             if rc not in (L.OK, L.ENOCONV):
                 raise L.HipError('tnx_lanczos_run_multi failed (rc=%d): %s' % (rc, msg))
         self.counters['lanczos_calls'] += 1
+        for prep in preps:
+            # the sites of the eigensolves (a queued next step moves _lz_site)
+            prep['sites'] = [s._lz_site for s, _ in prep['flat']]
+        return preps
+
+    def _resume_possible(self, preps):
+        """Whether booking these results may re-queue an eigensolve (then the
+        next step's composition is not known before the booking)."""
+        return self.cycle_steps and any(bool((p['res']['converged'] == 0).any())
+                                        for p in preps)
+
+    def _lanczos_book(self, preps):
+        """Per-solve booking of Lanczos results: statistics, Ritz values,
+        warnings and the re-queue flag of a solve that did not converge
+        within its cycle."""
         cnt = self.counters
         for prep in preps:
             res = prep['res']
+            sites = prep['sites']
             iters = res['iters'].tolist()
             conv = res['converged'].tolist()
             theta = res['theta'].tolist()
@@ -2005,7 +2021,7 @@ class Engine:
                     s.stats['unconverged'] += 1
                     s.warnings.append('local eigensolve at site %d did not converge '
                                       '(residual %.3g after %d matvecs)'
-                                      % (s._lz_site, float(res['resid'][k]), it))
+                                      % (sites[k], float(res['resid'][k]), it))
 
     def _lanczos_prepare(self, k, groups, got, rkey):
         """Resident descriptors (key ``rkey``), the linear-combination mix
@@ -2058,11 +2074,80 @@ class Engine:
 
     def step(self, solves=None):
         """Advance every solve with a queued entry by one step."""
+        return self._step(solves, None, False)[0]
+
+    def run_queues(self, solves=None):
+        """Run steps until no queue of ``solves`` holds an entry.  Once a
+        step's Lanczos call has returned and no eigensolve has to be resumed,
+        the next step's pre-Lanczos phases are planned and queued before the
+        host books the results, so the device does not wait for that host
+        work (the queued kernels follow the eigensolve on the stream)."""
+        pre = None
+        while True:
+            n, pre = self._step(solves, pre, True)
+            if not n:
+                return
+
+    def _step(self, solves, pre, lookahead):
+        """One step: its pre-Lanczos phases (``pre``: already queued by the
+        previous step), the Lanczos call, then (``lookahead``) the next
+        step's pre-Lanczos phases, then the booking.  Returns
+        ``(solves stepped, the next step's queued phases or None)``."""
+        if pre is None:
+            pre = self._step_pre(solves)
+        if pre is None:
+            return 0, None
+        plans, spec, host_miss, t_a, t_b, t_c = pre
+        groups = _lanczos_groups([(s, ph) for s, _, ph in plans])
+        preps = []
+        if groups:
+            # the next step's host work overlaps this Lanczos call, when this
+            # step needed host work (a batch whose descriptor sets are all
+            # resident repeats a known sequence and has none to overlap)
+            parts = [(part, _lanczos_key(part))
+                     for part in _split_groups(groups, self.lanczos_streams, self.split_min)]
+            host_miss = host_miss or any(key not in self._resident for _, key in parts)
+            ahead = None
+            if self.plan_ahead and host_miss:
+                ahead = [s for s, _, _ in plans if s.queue]
+            preps = self._lanczos(groups, spec, ahead, parts)
+        t_d = time.perf_counter()
+        nxt = None
+        if lookahead and not self._resume_possible(preps):
+            nxt = self._step_pre(solves)
+        self._lanczos_book(preps)
+        self.counters['t_plan'] += t_b - t_a
+        self.counters['t_pre'] += t_c - t_b
+        self.counters['t_lz_total'] += t_d - t_c
+        for s, st, ph in plans:
+            s.stats['steps'] += 1
+            if s.resume:
+                s.resume = False
+                kind = 'resume2' if st.kind in ('solve2', 'resume2') else 'resume'
+                s.queue.insert(0, Step(kind, st.site, st.dir, st.sweep))
+            if st.kind in ('solve', 'resume', 'solve2', 'resume2', 'gate2'):
+                s.center = st.site
+            elif st.kind in ('move', 'svdmove', 'qmove', 'tmove'):
+                s.center = st.site + st.dir
+            elif st.kind in ('split2', 'tsplit'):
+                s.center = st.site + (st.dir > 0)
+            if 'lm_bond' in ph:   # spectrum written to lmv on the device; read lazily
+                s._lm_pending[ph['lm_bond']] = ph['svd_n']
+            if 'lm_bonds' in ph:
+                for b, k in ph['lm_bonds']:
+                    s._lm_pending[b] = k
+        self.counters['steps'] += 1
+        return len(plans), nxt
+
+    def _step_pre(self, solves):
+        """Plan the next queued entry of every solve of ``solves`` with one
+        and queue the step's pre-Lanczos phases (gauge moves, environment
+        update, casts).  Returns the step context, or None without work."""
         self.settle()
         spec, self._spec = self._spec, None
         solves = [s for s in (solves or self.solves) if s.queue and s.failed is None]
         if not solves:
-            return 0
+            return None
         t_a = time.perf_counter()
         plans = []
         casts = []   # low-precision copies of every block, for solves changing precision
@@ -2177,42 +2262,4 @@ class Engine:
         run_gemm(2)      # Q^T . W
         run_mix(3)       # low-precision copies of the new block
         t_c = time.perf_counter()
-        groups = _lanczos_groups([(s, ph) for s, _, ph in plans])
-        if groups:
-            # the next step's host work overlaps this Lanczos call, when this
-            # step needed host work (a batch whose descriptor sets are all
-            # resident repeats a known sequence and has none to overlap)
-            parts = [(part, _lanczos_key(part))
-                     for part in _split_groups(groups, self.lanczos_streams, self.split_min)]
-            host_miss = host_miss or any(key not in self._resident for _, key in parts)
-            ahead = None
-            if self.plan_ahead and host_miss:
-                ahead = [s for s, _, _ in plans if s.queue]
-            self._lanczos(groups, spec, ahead, parts)
-        t_d = time.perf_counter()
-        self.counters['t_plan'] += t_b - t_a
-        self.counters['t_pre'] += t_c - t_b
-        self.counters['t_lz_total'] += t_d - t_c
-        for s, st, ph in plans:
-            s.stats['steps'] += 1
-            if s.resume:
-                s.resume = False
-                kind = 'resume2' if st.kind in ('solve2', 'resume2') else 'resume'
-                s.queue.insert(0, Step(kind, st.site, st.dir, st.sweep))
-            if st.kind in ('solve', 'resume', 'solve2', 'resume2', 'gate2'):
-                s.center = st.site
-            elif st.kind in ('move', 'svdmove', 'qmove', 'tmove'):
-                s.center = st.site + st.dir
-            elif st.kind in ('split2', 'tsplit'):
-                s.center = st.site + (st.dir > 0)
-            if 'lm_bond' in ph:   # spectrum written to lmv on the device; read lazily
-                s._lm_pending[ph['lm_bond']] = ph['svd_n']
-            if 'lm_bonds' in ph:
-                for b, k in ph['lm_bonds']:
-                    s._lm_pending[b] = k
-        self.counters['steps'] += 1
-        return len(plans)
-
-    def run_queues(self, solves=None):
-        while self.step(solves):
-            pass
+        return plans, spec, host_miss, t_a, t_b, t_c
