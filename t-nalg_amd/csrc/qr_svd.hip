@@ -286,6 +286,7 @@ __global__ __launch_bounds__(NT) void svd_kernel(const tnx_svd_job* __restrict__
 // 1 / R[k,k] (the 1 / sqrt it divides by) in LDS, so the inverse divides
 // nothing.
 constexpr int CHOL_NMAX = 128;
+constexpr double CHOL_NEAR_ID = 1e-10;  // largest |S - I| entry of the first-order path
 constexpr int CHOL_LD = CHOL_NMAX + 1;  // LDS row stride of R: column reads hit distinct banks
 constexpr size_t CHOL_LDS = sizeof(double) * (CHOL_NMAX * CHOL_LD + 5 * CHOL_NMAX);
 
@@ -471,6 +472,45 @@ __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __rest
     shift = J.shift_scale * t;
   }
   __syncthreads();
+  // Near-identity Gram matrix (the last pass of CholeskyQR3, whose panel is
+  // already orthonormal to working accuracy): with S = I + E, ||E||_max <=
+  // CHOL_NEAR_ID and no shift, R = I + U (U = strict upper part of E plus
+  // half its diagonal) and R^{-1} = I - U hold up to O(E^2) <= 1e-20, below
+  // the rounding of the sequential factorisation, and need no sequential
+  // steps.
+  {
+    double emax = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const int i = tr + 16 * a, j = tc + 16 * b;
+        if (i < n && j < n) emax = fmax(emax, fabs(s[a][b] - (i == j ? 1.0 : 0.0)));
+      }
+    for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o, 64));
+    __shared__ double redm[4];
+    if (lane == 0) redm[wave] = emax;
+    __syncthreads();
+    const double em = fmax(fmax(redm[0], redm[1]), fmax(redm[2], redm[3]));
+    if (shift == 0.0 && em <= CHOL_NEAR_ID) {
+      double* R = (double*)J.r;
+      double* Ri = (double*)J.rinv;
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          const int i = tr + 16 * a, j = tc + 16 * b;
+          if (i < n && j < n) {
+            const double u = j > i ? s[a][b] : (j == i ? 0.5 * (s[a][b] - 1.0) : 0.0);
+            const double id = (i == j) ? 1.0 : 0.0;
+            R[(int64_t)i * ld + j] = id + u;
+            Ri[(int64_t)i * ld + j] = id - u;
+          }
+        }
+      if (threadIdx.x == 0 && J.status != 0) *(int32_t*)J.status = 0;
+      return;
+    }
+  }
   {
     const double sh = shift;
 #pragma unroll

@@ -333,6 +333,37 @@ def test_chol_inv_f64(gpu, n):
     assert int(st.cpu()[0]) == 0
 
 
+@pytest.mark.parametrize('eps', [1e-15, 3e-11, 1e-8])
+def test_chol_inv_near_identity(gpu, eps):
+    """A Gram matrix I + E with a tiny symmetric E (the last CholeskyQR3
+    pass) takes the first-order path (|E| <= 1e-10) or the factorisation;
+    both give R^T R = S and R R^{-1} = I to rounding and numpy's factor."""
+    import torch
+    from tnalg_amd import _lib as L
+    lib = L.load()
+    n = 100
+    rng = np.random.RandomState(17)
+    E = rng.randn(n, n)
+    E = eps * (E + E.T) / 2
+    S = np.eye(n) + E
+    Gd = _dev(S)
+    r = torch.zeros(n * n, dtype=torch.float64, device='cuda')
+    ri = torch.zeros(n * n, dtype=torch.float64, device='cuda')
+    jobs = np.array([(Gd.data_ptr(), r.data_ptr(), ri.data_ptr(), 0, 0, n, n, 0.0)],
+                    dtype=L.CHOL_JOB)
+    (dj,) = _upload([jobs])
+    print('chol_inv near identity eps', eps, flush=True)
+    L.check(lib.tnx_chol_inv_f64(_vp(dj.data_ptr()), 1, _stream()), 'chol')
+    torch.cuda.synchronize()
+    Rm = r.cpu().numpy().reshape(n, n)
+    Ri = ri.cpu().numpy().reshape(n, n)
+    tol = 2e-15 if eps <= 1e-10 else 1e-14
+    assert np.array_equal(np.tril(Rm, -1), np.zeros((n, n)))
+    assert np.allclose(Rm.T @ Rm, S, rtol=0, atol=tol)
+    assert np.allclose(Rm @ Ri, np.eye(n), rtol=0, atol=tol)
+    assert np.allclose(Rm, np.linalg.cholesky(S).T, rtol=0, atol=tol)
+
+
 def test_svd_block_jacobi(gpu):
     """Batched block-Jacobi SVD of triangular factors of mixed orders (one
     launch per round over all jobs): R = u^T v, u orthogonal, s the singular
