@@ -119,7 +119,7 @@ typedef struct tnx_matvec_plan {
 
 /* One eigenproblem of a batched Lanczos solve.                              */
 typedef struct tnx_lanczos_solve {
-  uint64_t V;          /* f64 Krylov basis, ncv rows of stride ldv          */
+  uint64_t V;          /* Krylov basis (v_dtype), ncv + 1 rows of stride ldv */
   uint64_t w;          /* matvec output (f64 or f32, see w_dtype)           */
   uint64_t x;          /* matvec input copy (tnx_dtype x_dtype)             */
   uint64_t init;       /* f64 start vector (read at start)                  */
@@ -134,7 +134,14 @@ typedef struct tnx_lanczos_solve {
   double floor_rel;    /*   or |r| <= floor_rel*|theta| (precision floor)   */
   int32_t x_dtype;     /* tnx_dtype of x and of w, read when the call passes */
   int32_t w_dtype;     /*   TNX_DTYPE_PER_SOLVE                             */
-} tnx_lanczos_solve; /* 88 bytes */
+  int32_t v_dtype;     /* storage of the basis: TNX_F64, or TNX_F32 for a
+                          solve whose matvec runs in f32 or bf16 (half the
+                          basis traffic; dots, norms and the tridiagonal
+                          problem stay f64, and the second projection is
+                          left out below 1e-6 instead of 1e-12 of the new
+                          coupling, the basis's own rounding level)        */
+  int32_t reserved;
+} tnx_lanczos_solve; /* 96 bytes */
 
 /* Per-solve results of tnx_lanczos_run (host-visible after the call).      */
 typedef struct tnx_lanczos_result {

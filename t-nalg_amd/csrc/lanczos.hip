@@ -11,8 +11,10 @@
 //   matvec (3 launches) + dots + update/dots/norm
 //   + update/normalise (plus one block per solve: tridiagonal solve,
 //     convergence).
-// The Krylov basis and all reductions are f64 (two-pass classical
-// Gram-Schmidt against the full basis); the matvec may run in f32 or bf16.
+// All reductions are f64 (two-pass classical Gram-Schmidt against the full
+// basis); the basis is stored in f64, or in f32 for a solve whose matvec
+// runs in f32 or bf16 (v_dtype: the basis passes then move half the bytes).
+// The matvec may run in f32 or bf16.
 // The (j+1)x(j+1) tridiagonal eigenproblem is solved on the device by 257-way
 // multisection of the Sturm count (one workgroup) followed by inverse
 // iteration with partial pivoting.
@@ -33,6 +35,7 @@ constexpr int CHUNK = 1024;  // 256 lanes x 4 elements
 constexpr int NT = 256;
 constexpr int LZ_KEEP = 4;   // vectors kept by an implicit restart (at most)
 constexpr double LZ_REORTH_SKIP = 1e-12;   // second projection left out below this (relative)
+constexpr double LZ_REORTH_SKIP_F32 = 1e-6;  // the same for an f32 basis (its rounding level)
 
 struct LzState {
   double alpha[NCV_MAX];
@@ -108,7 +111,8 @@ __device__ __forceinline__ void wave_to_lds(double s, int k, double (*red)[NCV_M
 // Partial dots V[k] . x over this chunk for k = 0..j, wave sums into
 // red[wave][k].  Rows are taken four at a time so their loads are in flight
 // together (the loop is latency bound otherwise).
-__device__ __forceinline__ void chunk_dots(const double* __restrict__ V, int ldv, int j, int i0,
+template <typename VT>
+__device__ __forceinline__ void chunk_dots(const VT* __restrict__ V, int ldv, int j, int i0,
                                            int i1, const double (&x)[CHUNK / NT],
                                            double (*red)[NCV_MAX + 1]) {
   for (int k0 = 0; k0 <= j; k0 += 4) {
@@ -117,11 +121,11 @@ __device__ __forceinline__ void chunk_dots(const double* __restrict__ V, int ldv
     for (int u = 0; u < 4; ++u) {
       const int k = k0 + u;
       if (k <= j) {
-        const double* row = V + (int64_t)k * ldv;
+        const VT* row = V + (int64_t)k * ldv;
 #pragma unroll
         for (int q = 0; q < CHUNK / NT; ++q) {
           const int i = i0 + q * NT + threadIdx.x;
-          if (i < i1) s[u] += row[i] * x[q];
+          if (i < i1) s[u] += (double)row[i] * x[q];
         }
       }
     }
@@ -132,18 +136,19 @@ __device__ __forceinline__ void chunk_dots(const double* __restrict__ V, int ldv
 }
 
 // x -= sum_{k<=j} h[k] V[k] over this chunk (rows four at a time).
-__device__ __forceinline__ void chunk_axpy(const double* __restrict__ V, int ldv, int j, int i0,
+template <typename VT>
+__device__ __forceinline__ void chunk_axpy(const VT* __restrict__ V, int ldv, int j, int i0,
                                            int i1, const double* h, double (&x)[CHUNK / NT]) {
   for (int k0 = 0; k0 <= j; k0 += 4) {
     double v[4][CHUNK / NT];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int k = k0 + u;
-      const double* row = V + (int64_t)(k <= j ? k : j) * ldv;
+      const VT* row = V + (int64_t)(k <= j ? k : j) * ldv;
 #pragma unroll
       for (int q = 0; q < CHUNK / NT; ++q) {
         const int i = i0 + q * NT + threadIdx.x;
-        v[u][q] = (k <= j && i < i1) ? row[i] : 0.0;
+        v[u][q] = (k <= j && i < i1) ? (double)row[i] : 0.0;
       }
     }
 #pragma unroll
@@ -361,12 +366,19 @@ __global__ __launch_bounds__(NT) void k_init_scale(const tnx_lanczos_solve* __re
     L.act[ci.s] = 1;
   }
   const double* v = (const double*)D.init;
-  double* V0 = (double*)D.V;
   const double flat = 1.0 / sqrt((double)D.n);
   const int xd = dtype_of(x_dtype, D.x_dtype);
+  const bool vf = D.v_dtype == TNX_F32;
   for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
     double val = sc > 0.0 ? v[i] * sc : flat;
-    V0[i] = val;
+    if (vf) {
+      // the basis row as stored: the start vector rounded to f32
+      const float vv = (float)val;
+      ((float*)D.V)[i] = vv;
+      val = (double)vv;
+    } else {
+      ((double*)D.V)[i] = val;
+    }
     tnx_store((void*)D.x, xd, i, val);
   }
 }
@@ -387,9 +399,9 @@ __global__ __launch_bounds__(NT) void k_dots(const tnx_lanczos_solve* __restrict
     int i = ci.i0 + q * NT + threadIdx.x;
     w[q] = (i < ci.i1) ? tnx_load((const void*)D.w, wd, i) : 0.0;
   }
-  const double* V = (const double*)D.V;
   __shared__ double red[4][NCV_MAX + 1];
-  chunk_dots(V, D.ldv, j, ci.i0, ci.i1, w, red);
+  if (D.v_dtype == TNX_F32) chunk_dots((const float*)D.V, D.ldv, j, ci.i0, ci.i1, w, red);
+  else chunk_dots((const double*)D.V, D.ldv, j, ci.i0, ci.i1, w, red);
   __syncthreads();
   for (int k = threadIdx.x; k <= j; k += NT)
     L.partA[(int64_t)blockIdx.x * (NCV_MAX + 1) + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
@@ -411,7 +423,6 @@ __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __r
   __syncthreads();
   if (ci.c == 0)
     for (int k = threadIdx.x; k <= j; k += NT) L.st[ci.s].h1[k] = h[k];
-  const double* V = (const double*)D.V;
   const int wd = dtype_of(w_dtype, D.w_dtype);
   double r[CHUNK / NT];
 #pragma unroll
@@ -419,15 +430,30 @@ __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __r
     int i = ci.i0 + q * NT + threadIdx.x;
     r[q] = (i < ci.i1) ? tnx_load((const void*)D.w, wd, i) : 0.0;
   }
-  chunk_axpy(V, D.ldv, j, ci.i0, ci.i1, h, r);
-  double* Vn = (double*)D.V + (int64_t)(j + 1) * D.ldv;
-#pragma unroll
-  for (int q = 0; q < CHUNK / NT; ++q) {
-    int i = ci.i0 + q * NT + threadIdx.x;
-    if (i < ci.i1) Vn[i] = r[q];
-  }
   __shared__ double red[4][NCV_MAX + 1];
-  chunk_dots(V, D.ldv, j, ci.i0, ci.i1, r, red);
+  if (D.v_dtype == TNX_F32) {
+    // r (kept in f64 registers for its dots and norm) is parked in row j+1
+    // rounded to f32; the next pass normalises the parked row
+    const float* V = (const float*)D.V;
+    chunk_axpy(V, D.ldv, j, ci.i0, ci.i1, h, r);
+    float* Vn = (float*)D.V + (int64_t)(j + 1) * D.ldv;
+#pragma unroll
+    for (int q = 0; q < CHUNK / NT; ++q) {
+      int i = ci.i0 + q * NT + threadIdx.x;
+      if (i < ci.i1) Vn[i] = (float)r[q];
+    }
+    chunk_dots(V, D.ldv, j, ci.i0, ci.i1, r, red);
+  } else {
+    const double* V = (const double*)D.V;
+    chunk_axpy(V, D.ldv, j, ci.i0, ci.i1, h, r);
+    double* Vn = (double*)D.V + (int64_t)(j + 1) * D.ldv;
+#pragma unroll
+    for (int q = 0; q < CHUNK / NT; ++q) {
+      int i = ci.i0 + q * NT + threadIdx.x;
+      if (i < ci.i1) Vn[i] = r[q];
+    }
+    chunk_dots(V, D.ldv, j, ci.i0, ci.i1, r, red);
+  }
   double nn = 0.0;  // ||r||^2 (r is zero past the chunk end)
 #pragma unroll
   for (int q = 0; q < CHUNK / NT; ++q) nn += r[q] * r[q];
@@ -486,29 +512,51 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
     for (int k = 0; k <= j; ++k) hh += h[k] * h[k];
     const double bb = fmax(h[j + 1] - hh, 0.0);
     beta_s = sqrt(bb);
-    skip_s = hh <= LZ_REORTH_SKIP * LZ_REORTH_SKIP * bb;
+    const double sk = D.v_dtype == TNX_F32 ? LZ_REORTH_SKIP_F32 : LZ_REORTH_SKIP;
+    skip_s = hh <= sk * sk * bb;
   }
   __syncthreads();
   const double beta_j = beta_s;
   if (!solver) {
-    const double* V = (const double*)D.V;
-    double* Vn = (double*)D.V + (int64_t)(j + 1) * D.ldv;
-    double r[CHUNK / NT];
-#pragma unroll
-    for (int q = 0; q < CHUNK / NT; ++q) {
-      int i = ci.i0 + q * NT + threadIdx.x;
-      r[q] = (i < ci.i1) ? Vn[i] : 0.0;
-    }
-    if (!skip_s) chunk_axpy(V, D.ldv, j, ci.i0, ci.i1, h, r);
     const double inv = 1.0 / beta_j;
     const int xd = dtype_of(x_dtype, D.x_dtype);
+    double r[CHUNK / NT];
+    if (D.v_dtype == TNX_F32) {
+      const float* V = (const float*)D.V;
+      float* Vn = (float*)D.V + (int64_t)(j + 1) * D.ldv;
 #pragma unroll
-    for (int q = 0; q < CHUNK / NT; ++q) {
-      int i = ci.i0 + q * NT + threadIdx.x;
-      if (i < ci.i1) {
-        const double v = r[q] * inv;
-        Vn[i] = v;
-        tnx_store((void*)D.x, xd, i, v);
+      for (int q = 0; q < CHUNK / NT; ++q) {
+        int i = ci.i0 + q * NT + threadIdx.x;
+        r[q] = (i < ci.i1) ? (double)Vn[i] : 0.0;
+      }
+      if (!skip_s) chunk_axpy(V, D.ldv, j, ci.i0, ci.i1, h, r);
+#pragma unroll
+      for (int q = 0; q < CHUNK / NT; ++q) {
+        int i = ci.i0 + q * NT + threadIdx.x;
+        if (i < ci.i1) {
+          // x is the basis row as stored, so the next matvec acts on it
+          const float v = (float)(r[q] * inv);
+          Vn[i] = v;
+          tnx_store((void*)D.x, xd, i, (double)v);
+        }
+      }
+    } else {
+      const double* V = (const double*)D.V;
+      double* Vn = (double*)D.V + (int64_t)(j + 1) * D.ldv;
+#pragma unroll
+      for (int q = 0; q < CHUNK / NT; ++q) {
+        int i = ci.i0 + q * NT + threadIdx.x;
+        r[q] = (i < ci.i1) ? Vn[i] : 0.0;
+      }
+      if (!skip_s) chunk_axpy(V, D.ldv, j, ci.i0, ci.i1, h, r);
+#pragma unroll
+      for (int q = 0; q < CHUNK / NT; ++q) {
+        int i = ci.i0 + q * NT + threadIdx.x;
+        if (i < ci.i1) {
+          const double v = r[q] * inv;
+          Vn[i] = v;
+          tnx_store((void*)D.x, xd, i, v);
+        }
       }
     }
     tnx_stamp_end(ts);
@@ -743,6 +791,34 @@ __global__ __launch_bounds__(NT) void k_irl_solve(const tnx_lanczos_solve* __res
   }
 }
 
+// Rows V'[u] = sum_t C[t][u] V[t] (u <= k, t <= m) of this chunk in place,
+// x = cast(V'[k]) as stored.
+template <typename VT>
+__device__ __forceinline__ void irl_apply_rows(VT* V, const tnx_lanczos_solve& D,
+                                               const double (*C)[LZ_KEEP + 1],
+                                               const ChunkInfo& ci, int m, int k, int xd) {
+  for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
+    double acc[LZ_KEEP + 1];
+#pragma unroll
+    for (int u = 0; u <= LZ_KEEP; ++u) acc[u] = 0.0;
+    for (int t = 0; t <= m; ++t) {
+      const double v = (double)V[(int64_t)t * D.ldv + i];
+#pragma unroll
+      for (int u = 0; u <= LZ_KEEP; ++u)
+        if (u <= k) acc[u] += C[t][u] * v;
+    }
+    double xk = 0.0;
+#pragma unroll
+    for (int u = 0; u <= LZ_KEEP; ++u)
+      if (u <= k) {
+        const VT st = (VT)acc[u];
+        V[(int64_t)u * D.ldv + i] = st;
+        if (u == k) xk = (double)st;
+      }
+    tnx_store((void*)D.x, xd, i, xk);
+  }
+}
+
 // Implicit restart, basis part (chunk workgroups of the active solves): the
 // new rows V'[u] = sum_t C[t][u] V[t] (u <= k, t <= m) in place (every
 // element is read in full before it is written), x = cast(V'[k]).
@@ -758,27 +834,9 @@ __global__ __launch_bounds__(NT) void k_irl_apply(const tnx_lanczos_solve* __res
   for (int e = threadIdx.x; e < (m + 1) * (LZ_KEEP + 1); e += NT)
     C[e / (LZ_KEEP + 1)][e % (LZ_KEEP + 1)] = st.C[e / (LZ_KEEP + 1)][e % (LZ_KEEP + 1)];
   __syncthreads();
-  double* V = (double*)D.V;
   const int xd = dtype_of(x_dtype, D.x_dtype);
-  for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
-    double acc[LZ_KEEP + 1];
-#pragma unroll
-    for (int u = 0; u <= LZ_KEEP; ++u) acc[u] = 0.0;
-    for (int t = 0; t <= m; ++t) {
-      const double v = V[(int64_t)t * D.ldv + i];
-#pragma unroll
-      for (int u = 0; u <= LZ_KEEP; ++u)
-        if (u <= k) acc[u] += C[t][u] * v;
-    }
-#pragma unroll
-    for (int u = 0; u <= LZ_KEEP; ++u)
-      if (u <= k) V[(int64_t)u * D.ldv + i] = acc[u];
-    double xk = 0.0;
-#pragma unroll
-    for (int u = 0; u <= LZ_KEEP; ++u)
-      if (u == k) xk = acc[u];
-    tnx_store((void*)D.x, xd, i, xk);
-  }
+  if (D.v_dtype == TNX_F32) irl_apply_rows((float*)D.V, D, C, ci, m, k, xd);
+  else irl_apply_rows((double*)D.V, D, C, ci, m, k, xd);
 }
 
 // out = sum_k y[k] V[k] for every solve
@@ -789,12 +847,21 @@ __global__ __launch_bounds__(NT) void k_final(const tnx_lanczos_solve* __restric
   const tnx_lanczos_solve D = S[ci.s];
   const LzState& st = L.st[ci.s];
   const int kk = st.jlast + 1;
-  const double* V = (const double*)D.V;
   double* out = (double*)D.out;
-  for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
-    double s = 0.0;
-    for (int k = 0; k < kk; ++k) s += st.y[k] * V[(int64_t)k * D.ldv + i];
-    out[i] = s;
+  if (D.v_dtype == TNX_F32) {
+    const float* V = (const float*)D.V;
+    for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
+      double s = 0.0;
+      for (int k = 0; k < kk; ++k) s += st.y[k] * (double)V[(int64_t)k * D.ldv + i];
+      out[i] = s;
+    }
+  } else {
+    const double* V = (const double*)D.V;
+    for (int i = ci.i0 + threadIdx.x; i < ci.i1; i += NT) {
+      double s = 0.0;
+      for (int k = 0; k < kk; ++k) s += st.y[k] * V[(int64_t)k * D.ldv + i];
+      out[i] = s;
+    }
   }
 }
 

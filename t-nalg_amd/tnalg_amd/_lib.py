@@ -30,7 +30,15 @@ MIX_TERM = np.dtype([('src', '<u8'), ('coef', '<f8'), ('op', '<i4'), ('src_dtype
 LANCZOS_SOLVE = np.dtype([('V', '<u8'), ('w', '<u8'), ('x', '<u8'), ('init', '<u8'),
                           ('out', '<u8'), ('n', '<i4'), ('ldv', '<i4'), ('m', '<i4'),
                           ('chunk_begin', '<i4'), ('tol_scale', '<f8'), ('tau', '<f8'),
-                          ('floor_rel', '<f8'), ('x_dtype', '<i4'), ('w_dtype', '<i4')])
+                          ('floor_rel', '<f8'), ('x_dtype', '<i4'), ('w_dtype', '<i4'),
+                          ('v_dtype', '<i4'), ('reserved', '<i4')])
+
+
+def lanczos_records(rows):
+    """LANCZOS_SOLVE records from tuples of the first 14 fields (V ... w_dtype)
+    or of all of them; the basis is f64 unless a row gives v_dtype."""
+    full = len(LANCZOS_SOLVE.names)
+    return np.array([tuple(r) + (0,) * (full - len(r)) for r in rows], dtype=LANCZOS_SOLVE)
 LANCZOS_RESULT = np.dtype([('theta', '<f8'), ('resid', '<f8'), ('iters', '<i4'),
                            ('converged', '<i4')])
 QR_JOB = np.dtype([('src', '<u8'), ('q', '<u8'), ('r', '<u8'), ('work', '<u8'),

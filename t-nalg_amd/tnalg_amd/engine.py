@@ -1269,9 +1269,12 @@ class Solve:
         ph['mv_mix'] = mx.arrays()
         ph['mv_g2'] = g2.arrays()
         tol = self.eigs_tol if self.eigs_tol > 0 else 2.220446049250313e-16
+        # the basis of a low-precision matvec is stored in f32 (half the
+        # traffic of the Krylov passes; the reductions stay f64)
+        vdt = L.F64 if prec == 'fp64' else L.F32
         ph['lz'] = np.array([(self.V.data_ptr(), self.w.data_ptr(), Xp, self.init.data_ptr(),
                               out, n_el, psi, min(self.engine.ncv, n_el), 0,
-                              tol / self.tau, self.tau, floor_rel, ti, tc)],
+                              tol / self.tau, self.tau, floor_rel, ti, tc, vdt, 0)],
                             dtype=L.LANCZOS_SOLVE)
         ph['prec'] = prec
         ph['mv_flops'] = self._matvec_flops(p, pair)

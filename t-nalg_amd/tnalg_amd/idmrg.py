@@ -423,7 +423,7 @@ class InfiniteRun:
         kj_, kt_ = kick.arrays()
         dkj, dkt = self._upload([kj_, kt_])
         st['kick'] = (dkj, len(kj_), dkt, kick.blocks)
-        lz = np.array(lz, dtype=L.LANCZOS_SOLVE)
+        lz = L.lanczos_records(lz)
         ch = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
         lz['chunk_begin'] = np.concatenate([[0], np.cumsum(ch)[:-1]])
         st['nchunks'] = int(ch.sum())

@@ -464,7 +464,7 @@ def test_lanczos_dense_symmetric(gpu):
         g2.add(w.data_ptr(), n, 1, 1, [(Hd.data_ptr(), n, 0, x.data_ptr(), 1, 0, n, 1.0)], active=k)
         solves.append((V.data_ptr(), w.data_ptr(), x.data_ptr(), init.data_ptr(), out.data_ptr(), n,
                        n, min(ncv, n), 0, 1e-12 / 1e-4, 1e-4, 0.0, 0, 0))
-    lz = np.array(solves, dtype=L.LANCZOS_SOLVE)
+    lz = L.lanczos_records(solves)
     ch = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
     lz['chunk_begin'] = np.concatenate([[0], np.cumsum(ch)[:-1]])
     nch = int(ch.sum())
@@ -516,7 +516,7 @@ def test_lanczos_basis_stays_orthonormal(gpu):
         g2.add(w.data_ptr(), n, 1, 1, [(Hd.data_ptr(), n, 0, x.data_ptr(), 1, 0, n, 1.0)], active=k)
         solves.append((V.data_ptr(), w.data_ptr(), x.data_ptr(), init.data_ptr(), out.data_ptr(), n,
                        n, 0, 0, 1e-30, 1e-4, 0.0, 0, 0))
-    lz = np.array(solves, dtype=L.LANCZOS_SOLVE)
+    lz = L.lanczos_records(solves)
     ch = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
     lz['chunk_begin'] = np.concatenate([[0], np.cumsum(ch)[:-1]])
     nch = int(ch.sum())
@@ -537,6 +537,73 @@ def test_lanczos_basis_stays_orthonormal(gpu):
         G = Vh @ Vh.T
         assert np.abs(G - np.eye(ncv + 1)).max() < 1e-10, np.abs(G - np.eye(ncv + 1)).max()
         assert abs(res[k]['theta'] - refs[k]) < 1e-10 * abs(refs[k]), (res[k], refs[k])
+
+
+def test_lanczos_f32_basis(gpu):
+    """Solves whose basis is stored in f32 (v_dtype): a full forced cycle
+    keeps the stored rows orthonormal to f32 level, and a converging solve
+    gives the lowest eigenpair to f32 accuracy, beside the same problem with
+    an f64 basis in the same call."""
+    import torch
+    from tnalg_amd import _lib as L
+    from tnalg_amd import records as R
+    lib = L.load()
+    rng = np.random.RandomState(8)
+    n, ncv = 3000, 20
+    diag = np.linspace(0.0, 10.0, n)
+    diag[0] = -4.0
+    A = rng.randn(n, n)
+    H = 0.01 * (A + A.T) / 2 + np.diag(diag)
+    wref, vref = np.linalg.eigh(H)
+    Hd = _dev(H)
+    init0 = rng.randn(n)
+    keep, solves = [], []
+    g2 = R.GemmList()
+    # (f32 basis, forced full cycle), (f32 basis, converging), (f64 basis, converging)
+    for k, (vdt, tol) in enumerate([(L.F32, 1e-30), (L.F32, 1e-9), (L.F64, 1e-9)]):
+        V = torch.zeros((ncv + 1) * n, dtype=torch.float64, device='cuda')
+        w, x, out = (torch.zeros(n, dtype=torch.float64, device='cuda') for _ in range(3))
+        init = _dev(init0)
+        keep.append((V, out))
+        keep += [w, x, init]
+        g2.add(w.data_ptr(), n, 1, 1, [(Hd.data_ptr(), n, 0, x.data_ptr(), 1, 0, n, 1.0)], active=k)
+        solves.append((V.data_ptr(), w.data_ptr(), x.data_ptr(), init.data_ptr(), out.data_ptr(), n,
+                       n, 0, 0, tol / 1e-4, 1e-4, 0.0, 0, 0, vdt, 0))
+    lz = L.lanczos_records(solves)
+    ch = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
+    lz['chunk_begin'] = np.concatenate([[0], np.cumsum(ch)[:-1]])
+    nch = int(ch.sum())
+    p, s_ = g2.arrays()
+    dp, ds, dlz = _upload([p, s_, lz])
+    plan = L.MatvecPlan(0, 0, 0, 0, 0, 0, 0, 0, dp.data_ptr(), ds.data_ptr(), len(p),
+                        R.gemm_tiles(p), 0, 0, 0)
+    state = torch.zeros(int(lib.tnx_lanczos_state_bytes(3, nch, ncv)), dtype=torch.uint8,
+                        device='cuda')
+    res = np.zeros(3, dtype=L.LANCZOS_RESULT)
+    print('lanczos f32 basis n', n, flush=True)
+    rc = lib.tnx_lanczos_run_ex(plan, _vp(dlz.data_ptr()), 3, nch, ncv, 30, 1, 0, 0,
+                                _vp(state.data_ptr()), _vp(res.ctypes.data), _stream())
+    assert rc in (L.OK, L.ENOCONV), (rc, lib.tnx_last_error())
+    torch.cuda.synchronize()
+    # the forced solve: ncv + 1 stored f32 rows, orthonormal to f32 level
+    Vf = keep[0][0].cpu().numpy().view(np.float32)[:(ncv + 1) * n].reshape(ncv + 1, n)
+    Vf = Vf.astype(np.float64)
+    if res[0]['iters'] == ncv:   # no restart happened: every row is a Lanczos vector
+        G = Vf @ Vf.T
+        assert np.abs(G - np.eye(ncv + 1)).max() < 2e-6, np.abs(G - np.eye(ncv + 1)).max()
+    # both converging solves stop at |r| <= 1e-5 (the same test); the f32
+    # basis leaves the eigenvector's direction as accurate as that test asks
+    # and its norm within the f32 rounding of the basis rows
+    for k in (1, 2):
+        assert res[k]['converged'] == 1, res[k]
+        v = keep[4 * k][1].cpu().numpy()
+        nv = np.linalg.norm(v)
+        assert abs(nv - 1) < (1e-6 if k == 1 else 1e-12), (k, nv)
+        v = v / nv
+        th = res[k]['theta']
+        assert abs(th - wref[0]) < 1e-7 * np.abs(wref).max(), (k, th, wref[0])
+        assert abs(abs(v @ vref[:, 0]) - 1) < 1e-10, (k, abs(v @ vref[:, 0]) - 1)
+        assert np.linalg.norm(H @ v - th * v) < 2e-5, (k, np.linalg.norm(H @ v - th * v))
 
 
 def test_profile_times_the_first_krylov_step(gpu):
@@ -560,8 +627,8 @@ def test_profile_times_the_first_krylov_step(gpu):
     init, out = _dev(rng.randn(n)), torch.zeros(n, dtype=torch.float64, device='cuda')
     g2 = R.GemmList()
     g2.add(w.data_ptr(), n, 1, 1, [(Hd.data_ptr(), n, 0, x.data_ptr(), 1, 0, n, 1.0)], active=0)
-    lz = np.array([(V.data_ptr(), w.data_ptr(), x.data_ptr(), init.data_ptr(), out.data_ptr(), n,
-                    n, 0, 0, 1e-10 / 1e-4, 1e-4, 0.0, 0, 0)], dtype=L.LANCZOS_SOLVE)
+    lz = L.lanczos_records([(V.data_ptr(), w.data_ptr(), x.data_ptr(), init.data_ptr(),
+                            out.data_ptr(), n, n, 0, 0, 1e-10 / 1e-4, 1e-4, 0.0, 0, 0)])
     nch = -(-n // L.LZ_CHUNK)
     p, s = g2.arrays()
     dp, ds, dlz = _upload([p, s, lz])
@@ -648,7 +715,7 @@ def test_lanczos_multi_precision(gpu):
             recs.append((b['V'].data_ptr(), b['w'].data_ptr(), b['x'].data_ptr(),
                          b['init'].data_ptr(), b['out'].data_ptr(), n, n, min(ncv, n), 0,
                          1e-12 / 1e-4, 1e-4, floor, dts[k], dts[k]))
-        lz = np.array(recs, dtype=L.LANCZOS_SOLVE)
+        lz = L.lanczos_records(recs)
         ch = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
         lz['chunk_begin'] = np.concatenate([[0], np.cumsum(ch)[:-1]])
         nch = int(ch.sum())
@@ -720,7 +787,7 @@ def test_lanczos_implicit_restart_clustered(gpu):
         g2.add(w.data_ptr(), n, 1, 1, [(Hd.data_ptr(), n, 0, x.data_ptr(), 1, 0, n, 1.0)], active=k)
         solves.append((V.data_ptr(), w.data_ptr(), x.data_ptr(), init.data_ptr(), out.data_ptr(), n,
                        n, 0, 0, 1e-10 / 1e-4, 1e-4, 0.0, 0, 0))
-    lz = np.array(solves, dtype=L.LANCZOS_SOLVE)
+    lz = L.lanczos_records(solves)
     ch = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
     lz['chunk_begin'] = np.concatenate([[0], np.cumsum(ch)[:-1]])
     nch = int(ch.sum())

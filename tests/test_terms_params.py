@@ -147,7 +147,7 @@ def test_library_abi_struct_sizes():
     lib = _lib.load()
     sizes = (ctypes.c_int32 * 32)()
     n = lib.tnx_abi_sizes(sizes, 32)
-    assert list(sizes[:n]) == [48, 56, 48, 24, 112, 88, 24, 56, 64, 32, 56, 64, 96, 16, 32,
+    assert list(sizes[:n]) == [48, 56, 48, 24, 112, 96, 24, 56, 64, 32, 56, 64, 96, 16, 32,
                                32, 32, 32]
 
 

@@ -67,7 +67,7 @@ def _descriptors(bufs):
     recs = [(b['V'].data_ptr(), b['w'].data_ptr(), b['x'].data_ptr(), b['init'].data_ptr(),
              b['out'].data_ptr(), b['n'], b['n'], min(NCV, b['n']), 0, 1e-12 / 1e-4, 1e-4,
              2e-6 if b['dt'] == L.F32 else 0.0, b['dt'], b['dt']) for b in bufs]
-    lz = np.array(recs, dtype=L.LANCZOS_SOLVE)
+    lz = L.lanczos_records(recs)
     ch = (lz['n'] + L.LZ_CHUNK - 1) // L.LZ_CHUNK
     lz['chunk_begin'] = np.concatenate([[0], np.cumsum(ch)[:-1]])
     dlz = up(lz)
