@@ -329,7 +329,9 @@ int tnx_svd_block_jacobi(const tnx_svdj_job* jobs, int njob, int npairs, int max
  * one entry per kernel class (TNX_PROF_NCLASS of them) and returns that
  * count.  The work of a timed launch is its algorithmic flops (GEMMs, from
  * the plan's `work`) or bytes (mixer; Lanczos passes: basis rows read and
- * written over nchunks * 1024 elements, only for calls with one plan).    */
+ * written over nchunks * 1024 elements, only for calls with one plan; the
+ * rows count 8 bytes for an f64 matvec plan and 4 otherwise, the f32 basis
+ * of a low-precision matvec).                                             */
 enum { TNX_PROF_NCLASS = 10 };
 typedef struct tnx_prof_stat {
   char name[48];       /* kernel class                                      */
@@ -413,6 +415,32 @@ typedef struct tnx_sym_job {
   int32_t reserved;
 } tnx_sym_job; /* 32 bytes */
 int tnx_symmetrize_f64(const tnx_sym_job* jobs, int njob, int nblocks, tnx_stream_t stream);
+
+/* Recorded launch sequence (no counterpart in the reference: the engine
+ * records the launches of a step's gauge move and environment update once
+ * per resident descriptor set, the calls above that dmrg_finite_size's
+ * local step makes through MPSClass.py:860-891, and replays them with one
+ * call per step).  The ops run in order on `stream`, each as the entry
+ * point its kind names with the op's fields as arguments (null active
+ * flags).  Returns the first failing op's status (the rest are not
+ * issued); a block Jacobi SVD that reaches its sweep limit does not stop
+ * the sequence, and the call then returns TNX_ENOCONV.                     */
+enum tnx_op_kind {
+  TNX_OP_GEMM = 1,   /* tnx_gemm_grouped(mode, a0, n0, a1, n1, 0)          */
+  TNX_OP_MIX = 2,    /* tnx_mix(a0, n0, a1, n1, a2 (operator table), 0)     */
+  TNX_OP_QR = 3,     /* tnx_qr_f64(a0, n0)                                  */
+  TNX_OP_SVD = 4,    /* tnx_svd_jacobi_f64(a0, n0, mode = max sweeps)       */
+  TNX_OP_CHOL = 5,   /* tnx_chol_inv_f64(a0, n0)                            */
+  TNX_OP_SVDJ = 6,   /* tnx_svd_block_jacobi(a0, n0, n1, n2, n3, a1 (flag),
+                        mode = max sweeps)                                  */
+  TNX_OP_NORM = 7,   /* tnx_normalize_f64(a0, n0)                           */
+};
+typedef struct tnx_op {
+  int32_t kind, mode;
+  int32_t n0, n1, n2, n3;
+  uint64_t a0, a1, a2;
+} tnx_op; /* 48 bytes */
+int tnx_run_ops(const tnx_op* ops /* host */, int nops, tnx_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -962,18 +962,20 @@ int issue_step(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve*
   for (int p = 0; p < nplan; ++p)
     if (int rc = tnx_heff_matvec(plans + p, L.act, stream)) return rc;
   const dim3 grid(nchunks), block(NT);
-  // sampled timing (one plan: its modes give the x and w element sizes);
-  // algorithmic bytes over the nchunks * CHUNK elements: basis rows 0..j
-  // read once, w / r / x read or written once
+  // sampled timing (one plan: its modes give the x and w element sizes, and
+  // the basis element size under the policy that a low-precision matvec
+  // has an f32 basis); algorithmic bytes over the nchunks * CHUNK
+  // elements: basis rows 0..j read once, w / r / x read or written once
   const bool prof = tnx_prof_on() && nplan == 1;
   const double E = (double)nchunks * CHUNK;
   const int m1 = plans[0].mode & 0xff, m2 = plans[0].mode2 & 0xff;
   const double xs = m1 == TNX_GEMM_F64 ? 8.0 : m1 == TNX_GEMM_F32 ? 4.0 : 2.0;
   const double ws = m2 == TNX_GEMM_F64 ? 8.0 : m2 == TNX_GEMM_BF16_BF16 ? 2.0 : 4.0;
-  const double vrows = 8.0 * (j + 1);
+  const double vs = m1 == TNX_GEMM_F64 ? 8.0 : 4.0;
+  const double vrows = vs * (j + 1);
   tnx_prof_count(TNX_PC_DOTS, E * (vrows + ws));
-  tnx_prof_count(TNX_PC_UPDATE_DOTS, E * (vrows + ws + 8.0));
-  tnx_prof_count(TNX_PC_UPDATE_NORM, E * (16.0 + xs));
+  tnx_prof_count(TNX_PC_UPDATE_DOTS, E * (vrows + ws + vs));
+  tnx_prof_count(TNX_PC_UPDATE_NORM, E * (2.0 * vs + xs));
   int mk = prof ? tnx_prof_mark(TNX_PC_DOTS, s) : -1;
   hipLaunchKernelGGL(k_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype,
                      tnx_prof_ts(mk));
@@ -981,7 +983,7 @@ int issue_step(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve*
   mk = prof ? tnx_prof_mark(TNX_PC_UPDATE_DOTS, s) : -1;
   hipLaunchKernelGGL(k_update_dots, grid, block, 0, s, solves, nsolve, state, nchunks, j, w_dtype,
                      tnx_prof_ts(mk));
-  tnx_prof_done(mk, s, E * (vrows + ws + 8.0));
+  tnx_prof_done(mk, s, E * (vrows + ws + vs));
   // the chunk blocks normalise V[j+1] at every step, the last of a cycle
   // included (an implicit restart combines V[ncv] into the new basis)
   (void)ncv;
@@ -990,7 +992,7 @@ int issue_step(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve*
                      nchunks, j, x_dtype, 1, min_iters, tnx_prof_ts(mk));
   // (r read, V[j+1] and x written; the basis rows of the second correction,
   // read only when it is not negligible, are not counted)
-  tnx_prof_done(mk, s, E * (16.0 + xs));
+  tnx_prof_done(mk, s, E * (2.0 * vs + xs));
   TNX_CHECK_LAUNCH();
   return TNX_OK;
 }

@@ -386,7 +386,8 @@ extern "C" int tnx_abi_sizes(int32_t* sizes, int n) {
                          (int32_t)sizeof(tnx_chol_job),      (int32_t)sizeof(tnx_svdj_job),
                          (int32_t)sizeof(tnx_prof_stat),     (int32_t)sizeof(tnx_norm_job),
                          (int32_t)sizeof(tnx_fill_job),      (int32_t)sizeof(tnx_kron_term),
-                         (int32_t)sizeof(tnx_kron_job),      (int32_t)sizeof(tnx_sym_job)};
+                         (int32_t)sizeof(tnx_kron_job),      (int32_t)sizeof(tnx_sym_job),
+                         (int32_t)sizeof(tnx_op)};
   const int na = (int)(sizeof(all) / sizeof(all[0]));
   if (sizes == nullptr) return na;
   for (int i = 0; i < n && i < na; ++i) sizes[i] = all[i];

@@ -56,6 +56,9 @@ KRON_JOB = np.dtype([('dst', '<u8'), ('p', '<i4'), ('q', '<i4'), ('ldd', '<i4'),
 SYM_JOB = np.dtype([('x', '<u8'), ('n', '<i4'), ('ld', '<i4'), ('count', '<i4'),
                     ('stride', '<i4'), ('block_begin', '<i4'), ('reserved', '<i4')])
 KRON_BLOCK = 256   # dst elements per workgroup of tnx_kron_f64 / tnx_symmetrize_f64
+OP_REC = np.dtype([('kind', '<i4'), ('mode', '<i4'), ('n0', '<i4'), ('n1', '<i4'), ('n2', '<i4'),
+                   ('n3', '<i4'), ('a0', '<u8'), ('a1', '<u8'), ('a2', '<u8')])
+OP_GEMM, OP_MIX, OP_QR, OP_SVD, OP_CHOL, OP_SVDJ, OP_NORM = 1, 2, 3, 4, 5, 6, 7
 DOT_JOB = np.dtype([('x', '<u8'), ('y', '<u8'), ('rows', '<i4'), ('cols', '<i4'),
                     ('ldx', '<i4'), ('ldy', '<i4')])
 CHOL_JOB = np.dtype([('g', '<u8'), ('r', '<u8'), ('rinv', '<u8'), ('status', '<u8'),
@@ -152,6 +155,7 @@ SIGNATURES = {
     'tnx_fill_normal_f64': (_i, [_vp, _i, _vp]),
     'tnx_kron_f64': (_i, [_vp, _i, _vp, _i, _vp]),
     'tnx_symmetrize_f64': (_i, [_vp, _i, _i, _vp]),
+    'tnx_run_ops': (_i, [_vp, _i, _vp]),
     'tnx_profile_begin': (_i, [_i, _i]),
     'tnx_profile_end': (_i, [ctypes.POINTER(ProfStat), _i]),
 }
@@ -183,7 +187,7 @@ def load(path=None):
             ctypes.sizeof(MatvecPlan), LANCZOS_SOLVE.itemsize, LANCZOS_RESULT.itemsize,
             QR_JOB.itemsize, SVD_JOB.itemsize, DOT_JOB.itemsize, CHOL_JOB.itemsize,
             SVDJ_JOB.itemsize, ctypes.sizeof(ProfStat), NORM_JOB.itemsize, FILL_JOB.itemsize,
-            KRON_TERM.itemsize, KRON_JOB.itemsize, SYM_JOB.itemsize]
+            KRON_TERM.itemsize, KRON_JOB.itemsize, SYM_JOB.itemsize, OP_REC.itemsize]
     got = list(sizes[:n])
     if got != want:
         raise RuntimeError('ABI size mismatch: library %s, binding %s' % (got, want))

@@ -135,6 +135,7 @@ class GemmTuner:
                                                                   L.GEMM_BF16_F32,
                                                                   L.GEMM_BF16_BF16)}
         self.table = {}
+        self.version = 0   # bumped whenever the table changes
 
     @staticmethod
     def shape_class(mode, ntiles, kmax):
@@ -165,6 +166,7 @@ class GemmTuner:
             if best_t is None or t < best_t:
                 best, best_t = v, t
         self.table[key] = best
+        self.version += 1
 
 
 # the current raw stream of a device (torch's own accessor of it)
@@ -175,7 +177,7 @@ class _Resident:
     """A set of descriptor arrays uploaded once and kept in device memory
     (steps that repeat every sweep reuse them without host work).  ``host``:
     host objects built for it once (a Lanczos call's plan array and result
-    array)."""
+    array; a step's recorded pre-Lanczos launches)."""
     __slots__ = ('buf', 'ptrs', 'meta', 'host')
 
     def __init__(self, device, arrays, meta, stager):
@@ -1792,6 +1794,64 @@ class Engine:
         self._arena_pool.setdefault(arena.numel(), []).append(arena)
 
     # ------------------------------------------------- resident descriptors
+    def _pre_program(self, ent):
+        """The launches of a step's pre-Lanczos phases over resident
+        descriptor set ``ent``, in order, as tnx_run_ops records: the
+        normalisations, QR / SVD gauge moves, CholeskyQR3 stages, kept R
+        factors, block Jacobi SVD with its products, the absorption of R,
+        the copy-back and the environment update (GEMM, mixer, GEMM), the
+        low-precision copies.  GEMM variants are the tuner's at build time
+        (the caller rebuilds when the tuner's table changes)."""
+        ptr, meta = ent.ptrs, ent.meta
+        opsp = int(self.ops.data_ptr())
+        recs = []
+
+        def gemm(a, n_p, b, nt, kmax):
+            if nt:
+                recs.append((L.OP_GEMM, self._gemm_mode(L.GEMM_F64, nt, kmax), n_p, nt, 0, 0,
+                             a, b, 0))
+
+        def mix(a, n_j, b, nb):
+            if nb:
+                recs.append((L.OP_MIX, 0, n_j, nb, 0, 0, a, b, opsp))
+
+        def run_gemm(i):
+            n_p, nt, kmax = meta['gemm'][i]
+            gemm(ptr[2 + 2 * i], n_p, ptr[3 + 2 * i], nt, kmax)
+
+        def run_mix(i):
+            n_j, nb = meta['mix'][i]
+            mix(ptr[8 + 2 * i], n_j, ptr[9 + 2 * i], nb)
+
+        if meta.get('norm'):
+            recs.append((L.OP_NORM, 0, meta['norm'][0], 0, 0, 0, ptr[meta['norm'][1]], 0, 0))
+        if meta['nqr']:
+            recs.append((L.OP_QR, 0, meta['nqr'], 0, 0, 0, ptr[0], 0, 0))
+        if meta['nsvd']:
+            recs.append((L.OP_SVD, 60, meta['nsvd'], 0, 0, 0, ptr[1], 0, 0))
+        for (kind, cnt, nt, ai, kmax) in meta['cq']:   # CholeskyQR3 stages
+            if kind == 'g':
+                gemm(ptr[ai], cnt, ptr[ai + 1], nt, kmax)
+            elif kind == 'm':
+                mix(ptr[ai], cnt, ptr[ai + 1], nt)
+            elif cnt:
+                recs.append((L.OP_CHOL, 0, cnt, 0, 0, 0, ptr[ai], 0, 0))
+        run_mix(4)       # R factors kept for the spectrum (qmove steps)
+        if meta['svdj'] is not None:   # SVD of the CholeskyQR factors, then apply u and v
+            nj, npairs, max_nblk, max_n, ai, (gp, gnt, gi, gk), (mj, mnb, mi) = meta['svdj']
+            recs.append((L.OP_SVDJ, 60, nj, npairs, max_nblk, max_n, ptr[ai],
+                         int(self.svd_flag.data_ptr()), 0))
+            gemm(ptr[gi], gp, ptr[gi + 1], gnt, gk)
+            mix(ptr[mi], mj, ptr[mi + 1], mnb)
+        run_gemm(0)      # absorb R into the neighbour
+        run_mix(0)       # copy back (moves)
+        run_mix(1)       # linear combinations for the environment update
+        run_gemm(1)      # X . Q
+        run_mix(2)       # physical mixing
+        run_gemm(2)      # Q^T . W
+        run_mix(3)       # low-precision copies of the new block
+        return np.array(recs, dtype=L.OP_REC) if recs else np.zeros(0, dtype=L.OP_REC)
+
     def _resident_get(self, key):
         ent = self._resident.get(key)
         if ent is not None:
@@ -2197,72 +2257,22 @@ class Engine:
             got = spec.get('pre') if spec else None
             ent = self._resident_put(rkey, *(got[1] if got and got[0] == rkey else
                                              assemble_pre([(s, ph) for s, _, ph in plans])))
-        ptr, meta = ent.ptrs, ent.meta
-        st_ = self.stream()
-        lib = self.lib
-        opsp = ctypes_void(self.ops.data_ptr())
-
-        def run_gemm(i):
-            n_p, nt, kmax = meta['gemm'][i]
-            if nt:
-                L.check(lib.tnx_gemm_grouped(self._gemm_mode(L.GEMM_F64, nt, kmax),
-                                             ctypes_void(ptr[2 + 2 * i]), n_p,
-                                             ctypes_void(ptr[3 + 2 * i]), nt, None, st_), 'gemm')
-
-        def run_mix(i):
-            n_j, nb = meta['mix'][i]
-            if nb:
-                L.check(lib.tnx_mix(ctypes_void(ptr[8 + 2 * i]), n_j, ctypes_void(ptr[9 + 2 * i]),
-                                    nb, opsp, None, st_), 'mix')
-
-        if meta.get('norm'):
-            L.check(lib.tnx_normalize_f64(ctypes_void(ptr[meta['norm'][1]]), meta['norm'][0], st_),
-                    'tnx_normalize_f64')
-        if meta['nqr']:
-            L.check(lib.tnx_qr_f64(ctypes_void(ptr[0]), meta['nqr'], st_), 'tnx_qr_f64')
-        if meta['nsvd']:
-            L.check(lib.tnx_svd_jacobi_f64(ctypes_void(ptr[1]), meta['nsvd'], 60, st_), 'tnx_svd')
-        for (kind, cnt, nt, ai, kmax) in meta['cq']:   # CholeskyQR3 stages
-            if kind == 'g':
-                if nt:
-                    L.check(lib.tnx_gemm_grouped(self._gemm_mode(L.GEMM_F64, nt, kmax),
-                                                 ctypes_void(ptr[ai]), cnt,
-                                                 ctypes_void(ptr[ai + 1]), nt, None, st_), 'cq gemm')
-            elif kind == 'm':
-                if nt:
-                    L.check(lib.tnx_mix(ctypes_void(ptr[ai]), cnt, ctypes_void(ptr[ai + 1]), nt,
-                                        opsp, None, st_), 'cq mix')
-            elif cnt:
-                L.check(lib.tnx_chol_inv_f64(ctypes_void(ptr[ai]), cnt, st_), 'tnx_chol_inv_f64')
-        run_mix(4)       # R factors kept for the spectrum (qmove steps)
-        if meta['svdj'] is not None:   # SVD of the CholeskyQR factors, then apply u and v
-            nj, npairs, max_nblk, max_n, ai, (gp, gnt, gi, gk), (mj, mnb, mi) = meta['svdj']
-            rc = lib.tnx_svd_block_jacobi(ctypes_void(ptr[ai]), nj, npairs, max_nblk, max_n,
-                                          ctypes_void(self.svd_flag.data_ptr()), 60, st_)
-            if rc == L.ENOCONV:
-                # the rotation flag is shared by the launch: every solve with
-                # an SVD job in it is flagged
-                self.counters['svd_unconverged'] = self.counters.get('svd_unconverged', 0) + 1
-                for s, st, ph in plans:
-                    if 'svdj' in ph:
-                        s.stats['svd_unconverged'] = s.stats.get('svd_unconverged', 0) + 1
-                        s.warnings.append('block Jacobi SVD of step %r reached 60 sweeps '
-                                          '(%d SVDs in the launch)' % (st, nj))
-            else:
-                L.check(rc, 'tnx_svd_block_jacobi')
-            if gnt:
-                L.check(lib.tnx_gemm_grouped(self._gemm_mode(L.GEMM_F64, gnt, gk),
-                                             ctypes_void(ptr[gi]), gp,
-                                             ctypes_void(ptr[gi + 1]), gnt, None, st_), 'svd gemm')
-            if mnb:
-                L.check(lib.tnx_mix(ctypes_void(ptr[mi]), mj, ctypes_void(ptr[mi + 1]), mnb, opsp,
-                                    None, st_), 'svd copy')
-        run_gemm(0)      # absorb R into the neighbour
-        run_mix(0)       # copy back (moves)
-        run_mix(1)       # linear combinations for the environment update
-        run_gemm(1)      # X . Q
-        run_mix(2)       # physical mixing
-        run_gemm(2)      # Q^T . W
-        run_mix(3)       # low-precision copies of the new block
+        ver = self.tuner.version if self.tuner is not None else 0
+        if ent.host is None or ent.host[1] != ver:
+            ent.host = (self._pre_program(ent), ver)
+        prog = ent.host[0]
+        rc = self.lib.tnx_run_ops(ctypes_void(prog.ctypes.data), len(prog), self.stream())
+        if rc == L.ENOCONV:
+            # a block Jacobi SVD reached its sweep limit; the rotation flag is
+            # shared by the launch: every solve with an SVD job in it is flagged
+            nj = ent.meta['svdj'][0]
+            self.counters['svd_unconverged'] = self.counters.get('svd_unconverged', 0) + 1
+            for s, st, ph in plans:
+                if 'svdj' in ph:
+                    s.stats['svd_unconverged'] = s.stats.get('svd_unconverged', 0) + 1
+                    s.warnings.append('block Jacobi SVD of step %r reached 60 sweeps '
+                                      '(%d SVDs in the launch)' % (st, nj))
+        else:
+            L.check(rc, 'tnx_run_ops (pre-Lanczos phases)')
         t_c = time.perf_counter()
         return plans, spec, host_miss, t_a, t_b, t_c

@@ -148,7 +148,7 @@ def test_library_abi_struct_sizes():
     sizes = (ctypes.c_int32 * 32)()
     n = lib.tnx_abi_sizes(sizes, 32)
     assert list(sizes[:n]) == [48, 56, 48, 24, 112, 96, 24, 56, 64, 32, 56, 64, 96, 16, 32,
-                               32, 32, 32]
+                               32, 32, 32, 48]
 
 
 def test_easystart_coefficient_convention():
