@@ -3,7 +3,7 @@
 Workload (default): XXZ chain, N=64, open BC, Jxy=1, Jz=1, h=0, chi=128,
 20 Krylov vectors, tol 1e-10 (tau=1e-4), 8 sweeps, seeded random initial
 MPS; a *step* is one batched solve of ``--batch`` such chains (all 8 sweeps,
-observables at the end), advanced as ``--pipelines`` (default 3) sub-batches,
+observables at the end), advanced as ``--pipelines`` (default 4) sub-batches,
 each on its own engine, stream and host thread, so that one sub-batch's
 latency-bound gauge moves overlap the others' Lanczos calls.  ``value`` = solves/s of the batched mixed-precision
 engine; the plain-f64 path and the CPU baseline are reported beside it, with
@@ -52,7 +52,7 @@ def c2_para(n=64, chi=128, sweeps=8, jz=1.0, hz=0.0, tol=1e-10, sites=1, two_sit
 _ENGINES = {}
 _PLAN_AHEAD = [True]   # --no-plan-ahead: next-step host work in line (for comparison)
 _CYCLE_STEPS = [True]  # serving: one Krylov cycle per Lanczos call (--no-cycle-steps: all)
-_PIPELINES = [3]       # --pipelines k: the batch as k sub-batches on k engines, streams, threads
+_PIPELINES = [4]       # --pipelines k: the batch as k sub-batches on k engines, streams, threads
 _LZ_STREAMS = [None]   # Lanczos calls of a step over this many streams (None: 2 for the
                        # fixed batch, 1 for serving, where one Krylov cycle per call and
                        # small mixed groups measured faster on one stream)
@@ -785,7 +785,7 @@ def main():
     ap.add_argument('--chi', type=int, default=128)
     ap.add_argument('--sweeps', type=int, default=8)
     ap.add_argument('--no-plain', action='store_true')
-    ap.add_argument('--pipelines', type=int, default=3,
+    ap.add_argument('--pipelines', type=int, default=4,
                     help='advance the batch as this many sub-batches, each on its own engine, '
                          'stream and host thread')
     ap.add_argument('--dmrg-sites', type=int, default=1, choices=[1, 2],
