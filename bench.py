@@ -337,16 +337,21 @@ def live_kernels(stats):
 # (--pmc FETCH_SIZE, then --pmc WRITE_SIZE; KiB per dispatch averaged over
 # the 23 launches after each loop marker, --pmc-summary), scaled by what the
 # counters read for the calibration copy of 256 MiB (FETCH_SIZE x 2.00,
-# WRITE_SIZE x 1.00).  The GEMMs are priced in flops, so only the mixer has
-# an algorithmic byte count.
+# WRITE_SIZE x 1.00).  The GEMMs are priced in flops (2.15e9 per launch at
+# that step), so only the mixer has an algorithmic byte count; a GEMM
+# class's traffic in the bench line is its live average launch's flops
+# times the measured bytes per flop.
 MATVEC_PMC_TRAFFIC = {
     'mixer matvec': {'bf16': (42.5e6, 50.3e6), 'fp32': (76.1e6, 92.3e6),
                      'fp64': (152.1e6, 184.5e6)},
     'gemm f64 matvec T = F x': {'fp64': (117.7e6, None)},
     'gemm f64 matvec w = U R^T': {'fp64': (134.4e6, None)},
+    'gemm f32 matvec T = F x': {'fp32': (63.5e6, None)},
+    'gemm f32 matvec w = U R^T': {'fp32': (63.5e6, None)},
     'gemm bf16 matvec T = F x': {'bf16': (29.7e6, None)},
     'gemm bf16 matvec w = U R^T': {'bf16': (42.7e6, None)},
 }
+MATVEC_PMC_FLOPS = 2.15e9   # flops of each GEMM launch of those counter runs
 
 
 KERNEL_OF_CLASS = {'gemm f64': 'gemm_grouped_kernel<0>/<4>', 'gemm f32': 'gemm_grouped_kernel<1>',
@@ -390,10 +395,14 @@ def roofline_object(kern, note):
                             'traffic = the live average launch x the share-weighted ratio'}
     elif pmc:
         prec = next(iter(pmc))
-        traffic = pmc[prec][0]
-        detail = {'precision': prec, 'launch': 'mid-chain step, 64 solves',
-                  'source': 'rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --roofline-only, '
-                            'scaled by the 256 MiB calibration copy'}
+        per_flop = pmc[prec][0] / MATVEC_PMC_FLOPS
+        traffic = dom.get('flops_per_launch', 0.0) * per_flop
+        detail = {'precision': prec, 'hbm_bytes_per_flop': per_flop,
+                  'mid_chain_hbm_bytes': pmc[prec][0], 'mid_chain_flops': MATVEC_PMC_FLOPS,
+                  'source': 'rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --roofline-only '
+                            '(mid-chain step, 64 solves), scaled by the 256 MiB calibration copy; '
+                            'traffic = the live average launch\'s flops x the measured bytes per '
+                            'flop'}
     return {'bound': dom['bound'], 'achieved': dom['achieved'], 'peak': dom['peak'],
             'unit': dom['unit'], 'frac': dom['frac'], 'traffic': traffic,
             'traffic_detail': detail,

@@ -40,31 +40,49 @@ namespace {
 //   measured slower (f32 GEMM 2: 38 -> 42 us; bf16 GEMM 1: 16 -> 22 us).
 // All variants of a mode issue the same MFMAs in the same k order, so they
 // give the same bits.
+// NT lanes per workgroup as WR x WC wavefronts, each owning a
+// (64 / WR) x (64 / WC) part of the tile as 16x16 MFMA blocks.  The 8-wave
+// bf16 variants (16x32 per wave) double the waves per workgroup for
+// launches of few tiles: mid-chain GEMM 2 (K 512), 16 solves 12.7 -> 11.0
+// us, 64 solves 19.9 -> 17.8 us; each output block still sees the same
+// MFMAs in the same k order.  Measured and not kept: the same 8-wave split
+// for f32 (25.0 -> 25.2 us), 128-deep f32 chunks (25.0 -> 27.7 us) and
+// 64-deep f64 chunks with one workgroup per CU (43.1 -> 45.0 us).
 template <int CFG> struct Cfg;
 template <> struct Cfg<0> {
   typedef double TI; typedef double TC; typedef double ACC; typedef double CT;
-  static constexpr int MODE = TNX_GEMM_F64, KB = 32, NE = 8;
+  static constexpr int MODE = TNX_GEMM_F64, KB = 32, NE = 8, NT = 256, WR = 2, WC = 2;
   static constexpr bool DB = true;
 };
 template <> struct Cfg<1> {
   typedef float TI; typedef float TC; typedef float ACC; typedef float CT;
-  static constexpr int MODE = TNX_GEMM_F32, KB = 64, NE = 16;
+  static constexpr int MODE = TNX_GEMM_F32, KB = 64, NE = 16, NT = 256, WR = 2, WC = 2;
   static constexpr bool DB = false;
 };
 template <> struct Cfg<2> {
   typedef __bf16 TI; typedef float TC; typedef float ACC; typedef __bf16 CT;
-  static constexpr int MODE = TNX_GEMM_BF16_F32, KB = 128, NE = 32;
+  static constexpr int MODE = TNX_GEMM_BF16_F32, KB = 128, NE = 32, NT = 256, WR = 2, WC = 2;
   static constexpr bool DB = false;
 };
 template <> struct Cfg<3> {
   typedef __bf16 TI; typedef __bf16 TC; typedef float ACC; typedef __bf16 CT;
-  static constexpr int MODE = TNX_GEMM_BF16_BF16, KB = 128, NE = 32;
+  static constexpr int MODE = TNX_GEMM_BF16_BF16, KB = 128, NE = 32, NT = 256, WR = 2, WC = 2;
   static constexpr bool DB = false;
 };
 template <> struct Cfg<4> {   // f64, variant 1
   typedef double TI; typedef double TC; typedef double ACC; typedef double CT;
-  static constexpr int MODE = TNX_GEMM_F64, KB = 16, NE = 4;
+  static constexpr int MODE = TNX_GEMM_F64, KB = 16, NE = 4, NT = 256, WR = 2, WC = 2;
   static constexpr bool DB = true;
+};
+template <> struct Cfg<6> {   // bf16 -> f32, variant 1: 8 waves
+  typedef __bf16 TI; typedef float TC; typedef float ACC; typedef __bf16 CT;
+  static constexpr int MODE = TNX_GEMM_BF16_F32, KB = 128, NE = 16, NT = 512, WR = 4, WC = 2;
+  static constexpr bool DB = false;
+};
+template <> struct Cfg<7> {   // bf16 -> bf16, variant 1: 8 waves
+  typedef __bf16 TI; typedef __bf16 TC; typedef float ACC; typedef __bf16 CT;
+  static constexpr int MODE = TNX_GEMM_BF16_BF16, KB = 128, NE = 16, NT = 512, WR = 4, WC = 2;
+  static constexpr bool DB = false;
 };
 
 template <typename T> __device__ __forceinline__ double to_d(T v) { return (double)v; }
@@ -72,7 +90,7 @@ template <> __device__ __forceinline__ double to_d<__bf16>(__bf16 v) { return (d
 template <typename T> __device__ __forceinline__ T from_d(double v) { return (T)v; }
 template <> __device__ __forceinline__ __bf16 from_d<__bf16>(double v) { return (__bf16)(float)v; }
 
-constexpr int TM = 64, TN = 64, NT = 256;
+constexpr int TM = 64, TN = 64;
 
 // NE contiguous elements starting at p, of which the first `valid` are in
 // bounds; out-of-bounds ones read as zero.  Loaded as 16-byte vectors when
@@ -104,7 +122,7 @@ template <int CFG> struct Frag {
   int ta, tb;
 };
 
-// Thread t's share of a 64 x KB A tile / KB x 64 B tile (256 threads x NE =
+// Thread t's share of a 64 x KB A tile / KB x 64 B tile (NT threads x NE =
 // 64 x KB).
 //   transA = 0 (k contiguous): row t/(KB/NE), k = (t%(KB/NE))*NE .. +NE
 //   transA = 1 (i contiguous): k  t/(64/NE), i = (t%(64/NE))*NE .. +NE
@@ -203,7 +221,7 @@ __device__ __forceinline__ void commit_bf16(const Frag<CFG>& f, __bf16 (*As)[Cfg
 }
 
 template <int CFG>
-__global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem* __restrict__ probs,
+__global__ __launch_bounds__(Cfg<CFG>::NT) void gemm_grouped_kernel(const tnx_gemm_problem* __restrict__ probs,
                                                           int nprob,
                                                           const tnx_gemm_seg* __restrict__ segs,
                                                           const int32_t* __restrict__ active,
@@ -224,15 +242,17 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
   const int row0 = (t_local / tiles_n) * TM, col0 = (t_local % tiles_n) * TN;
   if (row0 >= P.M) return;  // cannot happen with exact prefix sums
 
+  static_assert(C::NT * C::NE == TM * KB && C::WR * C::WC * 64 == C::NT, "tile split");
+  constexpr int MB = TM / C::WR / 16, NB = TN / C::WC / 16;   // MFMA blocks per wave
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int wr = (wave / C::WC) * (TM / C::WR), wc = (wave % C::WC) * (TN / C::WC);
 
   typedef typename std::conditional<sizeof(ACC) == 8, tnx_d4, tnx_f4>::type V4;
-  V4 acc[2][2];
+  V4 acc[MB][NB];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < MB; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = V4{0, 0, 0, 0};
+    for (int b = 0; b < NB; ++b) acc[a][b] = V4{0, 0, 0, 0};
 
   typedef typename std::conditional<(MODE == TNX_GEMM_F64 || MODE == TNX_GEMM_F32),
                                     typename C::CT, __bf16>::type CT;
@@ -284,34 +304,35 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
     if constexpr (K4) {
 #pragma unroll
       for (int kk = 0; kk < KB; kk += 4) {
-        CT a0 = As[stage][kk + (lane >> 4)][wr + (lane & 15)];
-        CT a1 = As[stage][kk + (lane >> 4)][wr + 16 + (lane & 15)];
-        CT b0 = Bs[stage][kk + (lane >> 4)][wc + (lane & 15)];
-        CT b1 = Bs[stage][kk + (lane >> 4)][wc + 16 + (lane & 15)];
-        if constexpr (MODE == TNX_GEMM_F64) {
-          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-        } else {
-          acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
-          acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
-          acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
-          acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
-        }
+        CT a[MB], b[NB];
+#pragma unroll
+        for (int mi = 0; mi < MB; ++mi) a[mi] = As[stage][kk + (lane >> 4)][wr + 16 * mi + (lane & 15)];
+#pragma unroll
+        for (int nj = 0; nj < NB; ++nj) b[nj] = Bs[stage][kk + (lane >> 4)][wc + 16 * nj + (lane & 15)];
+#pragma unroll
+        for (int mi = 0; mi < MB; ++mi)
+#pragma unroll
+          for (int nj = 0; nj < NB; ++nj) {
+            if constexpr (MODE == TNX_GEMM_F64)
+              acc[mi][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mi], b[nj], acc[mi][nj], 0, 0, 0);
+            else
+              acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mi], b[nj], acc[mi][nj], 0, 0, 0);
+          }
       }
     } else {
 #pragma unroll
       for (int kk = 0; kk < KB; kk += 32) {
         const int kq = kk + (lane >> 4) * 8;
-        tnx_bf8 a0 = *(const tnx_bf8*)&As[stage][wr + (lane & 15)][kq];
-        tnx_bf8 a1 = *(const tnx_bf8*)&As[stage][wr + 16 + (lane & 15)][kq];
-        tnx_bf8 b0 = *(const tnx_bf8*)&Bs[stage][wc + (lane & 15)][kq];
-        tnx_bf8 b1 = *(const tnx_bf8*)&Bs[stage][wc + 16 + (lane & 15)][kq];
-        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
+        tnx_bf8 a[MB], b[NB];
+#pragma unroll
+        for (int mi = 0; mi < MB; ++mi) a[mi] = *(const tnx_bf8*)&As[stage][wr + 16 * mi + (lane & 15)][kq];
+#pragma unroll
+        for (int nj = 0; nj < NB; ++nj) b[nj] = *(const tnx_bf8*)&Bs[stage][wc + 16 * nj + (lane & 15)][kq];
+#pragma unroll
+        for (int mi = 0; mi < MB; ++mi)
+#pragma unroll
+          for (int nj = 0; nj < NB; ++nj)
+            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mi], b[nj], acc[mi][nj], 0, 0, 0);
       }
     }
     staged = held;
@@ -337,9 +358,9 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
   TC* Cp = (TC*)P.c;
   const TC* C0 = (const TC*)P.c0;
 #pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
+  for (int mi = 0; mi < MB; ++mi)
 #pragma unroll
-    for (int nj = 0; nj < 2; ++nj)
+    for (int nj = 0; nj < NB; ++nj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int rr = (MODE == TNX_GEMM_F64) ? ((lane >> 4) + 4 * r) : ((lane >> 4) * 4 + r);
@@ -359,9 +380,9 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const tnx_gemm_problem
 extern "C" int tnx_gemm_variants(int mode) {
   switch (mode) {
     case TNX_GEMM_F64: return 2;
-    case TNX_GEMM_F32:
+    case TNX_GEMM_F32: return 1;
     case TNX_GEMM_BF16_F32:
-    case TNX_GEMM_BF16_BF16: return 1;
+    case TNX_GEMM_BF16_BF16: return 2;
     default:
       tnx_set_error("tnx_gemm_variants: unknown mode %d", mode);
       return TNX_EINVAL;
@@ -390,24 +411,28 @@ int tnx_gemm_grouped_ts(int mode, const tnx_gemm_problem* prob, int nprob,
   }
   if (ntiles == 0) return TNX_OK;
   hipStream_t s = (hipStream_t)stream;
-  dim3 grid(ntiles), block(NT);
+  const dim3 grid(ntiles);
+#define TNX_GEMM_LAUNCH(CFG)                                                       \
+  hipLaunchKernelGGL(gemm_grouped_kernel<CFG>, grid, dim3(Cfg<CFG>::NT), 0, s, prob, nprob, \
+                     seg, active, ts)
   switch (base) {
     case TNX_GEMM_F64:
-      if (var == 0)
-        hipLaunchKernelGGL(gemm_grouped_kernel<0>, grid, block, 0, s, prob, nprob, seg, active, ts);
-      else
-        hipLaunchKernelGGL(gemm_grouped_kernel<4>, grid, block, 0, s, prob, nprob, seg, active, ts);
+      if (var == 0) TNX_GEMM_LAUNCH(0);
+      else TNX_GEMM_LAUNCH(4);
       break;
     case TNX_GEMM_F32:
-      hipLaunchKernelGGL(gemm_grouped_kernel<1>, grid, block, 0, s, prob, nprob, seg, active, ts);
+      TNX_GEMM_LAUNCH(1);
       break;
     case TNX_GEMM_BF16_F32:
-      hipLaunchKernelGGL(gemm_grouped_kernel<2>, grid, block, 0, s, prob, nprob, seg, active, ts);
+      if (var == 0) TNX_GEMM_LAUNCH(2);
+      else TNX_GEMM_LAUNCH(6);
       break;
     default:
-      hipLaunchKernelGGL(gemm_grouped_kernel<3>, grid, block, 0, s, prob, nprob, seg, active, ts);
+      if (var == 0) TNX_GEMM_LAUNCH(3);
+      else TNX_GEMM_LAUNCH(7);
       break;
   }
+#undef TNX_GEMM_LAUNCH
   TNX_CHECK_LAUNCH();
   return TNX_OK;
 }

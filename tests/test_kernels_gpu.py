@@ -96,25 +96,31 @@ def test_gemm_grouped_matches_numpy(gpu, mode):
         assert err < tol, (mode, sh, err)
 
 
-def test_gemm_f64_variants_agree(gpu):
-    """Every kernel variant of the f64 GEMM gives the same bits (they issue
-    the same MFMAs in the same k order); an unknown variant is refused."""
+@pytest.mark.parametrize('mode_name', ['f64', 'f32', 'bf16_f32', 'bf16_bf16'])
+def test_gemm_variants_agree(gpu, mode_name):
+    """Every kernel variant of a GEMM mode gives the same bits (they issue
+    the same MFMAs in the same k order for every output block); an unknown
+    variant is refused."""
     import torch
     from tnalg_amd import _lib as L
     from tnalg_amd import records as R
     lib = L.load()
-    nvar = lib.tnx_gemm_variants(L.GEMM_F64)
-    assert nvar >= 2 and lib.tnx_gemm_variants(L.GEMM_BF16_F32) == 1
+    mode, tin, tout = {'f64': (L.GEMM_F64, torch.float64, torch.float64),
+                       'f32': (L.GEMM_F32, torch.float32, torch.float32),
+                       'bf16_f32': (L.GEMM_BF16_F32, torch.bfloat16, torch.float32),
+                       'bf16_bf16': (L.GEMM_BF16_BF16, torch.bfloat16, torch.bfloat16)}[mode_name]
+    nvar = lib.tnx_gemm_variants(mode)
+    assert nvar >= (1 if mode == L.GEMM_F32 else 2)
     rng = np.random.RandomState(12)
     g = R.GemmList()
     keep, outs = [], []
     for (M, N, K, ta, tb) in [(64, 64, 128, 0, 0), (130, 70, 512, 1, 0), (33, 200, 37, 0, 1),
                               (256, 128, 300, 1, 1)]:
-        A = _dev(rng.randn(K, M) if ta else rng.randn(M, K))
-        B = _dev(rng.randn(N, K) if tb else rng.randn(K, N))
-        C0 = _dev(rng.randn(M, N))
+        A = _dev(rng.randn(K, M) if ta else rng.randn(M, K), tin)
+        B = _dev(rng.randn(N, K) if tb else rng.randn(K, N), tin)
+        C0 = _dev(rng.randn(M, N), tout)
         keep += [A, B, C0]
-        C = torch.zeros((M, N), dtype=torch.float64, device='cuda')
+        C = torch.zeros((M, N), dtype=tout, device='cuda')
         outs.append(C)
         g.add(C.data_ptr(), M, N, N, [(A.data_ptr(), A.shape[1], ta, B.data_ptr(), B.shape[1], tb,
                                        K, 0.75)], c0=C0.data_ptr(), ldc0=N, beta=-0.5)
@@ -122,17 +128,17 @@ def test_gemm_f64_variants_agree(gpu):
     dp, ds = _upload([p, s])
     got = []
     for v in range(nvar):
-        print('gemm f64 variant', v, flush=True)
+        print('gemm', mode_name, 'variant', v, flush=True)
         for C in outs:
             C.zero_()
-        L.check(lib.tnx_gemm_grouped(L.GEMM_F64 | (v << 8), _vp(dp.data_ptr()), len(p),
+        L.check(lib.tnx_gemm_grouped(mode | (v << 8), _vp(dp.data_ptr()), len(p),
                                      _vp(ds.data_ptr()), R.gemm_tiles(p), None, _stream()), 'gemm')
         torch.cuda.synchronize()
-        got.append([C.cpu().numpy().copy() for C in outs])
+        got.append([C.float().cpu().numpy().copy() for C in outs])
     for v in range(1, nvar):
         for a, b in zip(got[0], got[v]):
             assert np.array_equal(a, b), v
-    rc = lib.tnx_gemm_grouped(L.GEMM_F64 | (nvar << 8), _vp(dp.data_ptr()), len(p),
+    rc = lib.tnx_gemm_grouped(mode | (nvar << 8), _vp(dp.data_ptr()), len(p),
                               _vp(ds.data_ptr()), R.gemm_tiles(p), None, _stream())
     assert rc == L.EINVAL
 
