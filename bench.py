@@ -533,7 +533,32 @@ def trace_summary(path, t0=None, t1=None):
             first[r[2]].append(r[1] - r[0])
             if r[2] == 'k_update_norm':
                 inside[q] = False
+    # per stream (one per sub-batch): its kernel time, the time between its
+    # consecutive kernels, and its kernel time inside and outside Lanczos
+    # calls (from a call's k_zero_words to its k_results)
+    per_stream = {}
+    for q in sorted({r[3] for r in rows}):
+        rs = [r for r in rows if r[3] == q]
+        kt = collections.Counter()
+        lz = [0, 0]
+        in_lz = False
+        gap = 0
+        for i, r in enumerate(rs):
+            kt[r[2]] += r[1] - r[0]
+            if r[2] == 'k_zero_words':
+                in_lz = True
+            lz[in_lz] += r[1] - r[0]
+            if r[2] == 'k_results':
+                in_lz = False
+            if i:
+                gap += max(0, r[0] - rs[i - 1][1])
+        per_stream[str(q)] = {
+            'kernels': len(rs), 'span_s': (rs[-1][1] - rs[0][0]) / 1e9,
+            'kernel_s': sum(r[1] - r[0] for r in rs) / 1e9, 'gap_s': gap / 1e9,
+            'kernel_s_in_lanczos': lz[1] / 1e9, 'kernel_s_outside_lanczos': lz[0] / 1e9,
+            'top': {k: v / 1e9 for k, v in kt.most_common(10)}}
     return {'span_s': (b - a) / 1e9, 'busy_s': busy / 1e9, 'busy_frac': busy / (b - a),
+            'per_stream': per_stream,
             'kernels': len(rows), 'summed_kernel_s': sum(r[1] - r[0] for r in rows) / 1e9,
             'idle_by_gap_us_le': {str(k): v / 1e9 for k, v in sorted(hist.items())},
             'idle_by_transition': [{'prev': k[0], 'next': k[1], 'idle_s': g / 1e9, 'gaps': n}

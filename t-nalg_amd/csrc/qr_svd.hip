@@ -271,304 +271,259 @@ __global__ __launch_bounds__(NT) void svd_kernel(const tnx_svd_job* __restrict__
   }
 }
 
-// Cholesky factor + inverse of one n x n SPD matrix (n <= 128), 256 lanes.
-// Lane (tr, tc) = (tid / 16, tid % 16) holds the 8 x 8 register tile of rows
-// {tr + 16a} and columns {tc + 16b}; the cyclic split keeps the shrinking
-// trailing matrix spread over every lane.  Right-looking Cholesky of
-// S = G + shift I = R^T R: steps k and k+1 broadcast rows k and k+1 through
-// LDS (row pairs double-buffered, so one barrier per two steps) and each lane
-// applies the two rank-1 updates to its tile.  R is then parked in LDS and
-// R^{-1} is formed from the bottom row up, rows k and k-1 per barrier:
-// X[k,:] /= R[k,k], X[i,:] -= R[i,k] X[k,:] for i < k.  Every lane forms the
-// second row of a pair from the broadcast rows with the same operations, in
-// the same order, as the step that would have followed a barrier, so the
-// results are those of one step per barrier.  The factorisation keeps
-// 1 / R[k,k] (the 1 / sqrt it divides by) in LDS, so the inverse divides
-// nothing.
+// Cholesky factor + inverse of one n x n SPD matrix (n <= 128), 256 lanes,
+// in 16 x 16 blocks (nb = ceil(n / 16) block rows; the padding is the
+// identity).  Right-looking, with the inverse carried along: eliminating
+// [S | I] = R^T [R | F] gives R and F = R^{-T}, so R^{-1} = F^T comes out of
+// the same steps.  Per panel p:
+//   1. the owner wave of the diagonal block factors S_pp = R_pp^T R_pp in
+//      registers (the block in the v_mfma_f64_16x16x4_f64 output layout,
+//      row k read by lane shuffles, one reciprocal square root per step),
+//      F_pp = R_pp^{-T} from the same steps;
+//   2. row panel p on MFMA: R_pb = F_pp S_pb (b > p), F_pe = F_pp E_pe (e < p);
+//   3. trailing update on MFMA: S_ab -= R_pa^T R_pb (p < a <= b) and
+//      E_ae -= R_pa^T F_pe (a > p, e <= p).
+// Wave w owns block columns w and 7 - w of the upper part (S -> R) and of
+// the strictly lower part (E -> F): 16 blocks of 4 accumulator registers per
+// lane, about the same work per wave in every phase.  The owner of the next
+// diagonal block updates it first and factors it while the other waves
+// finish the trailing update; two barriers per panel.  LDS holds the row
+// panels as they are formed (R above the diagonal, F below it, F_pp in its
+// own buffer), which are the MFMA operands of the later phases and the
+// output at the end.
 constexpr int CHOL_NMAX = 128;
 constexpr double CHOL_NEAR_ID = 1e-10;  // largest |S - I| entry of the first-order path
-constexpr int CHOL_LD = CHOL_NMAX + 1;  // LDS row stride of R: column reads hit distinct banks
-constexpr size_t CHOL_LDS = sizeof(double) * (CHOL_NMAX * CHOL_LD + 5 * CHOL_NMAX);
+constexpr int CB_LD = CHOL_NMAX + 1;    // LDS row stride: row and column reads hit distinct banks
+constexpr int CB_XLD = 17;              // row stride of the F_pp blocks
+constexpr size_t CHOL_LDS = sizeof(double) * (CHOL_NMAX * CB_LD + 8 * 16 * CB_XLD);
 
-// Cholesky steps k in [16 KA, 16 KA + 16) (k < n), in pairs (k, k+1) that
-// stay inside the block (16 KA is even).  With the tile-row block KA of
-// column k a compile-time constant, every register index is static, and the
-// blocks a < KA / b < KA (rows and columns already factored, whose update
-// factors are zero) drop out of the rank-1 updates at compile time, as do
-// the blocks a > b: they lie strictly below the diagonal (row tr + 16a >
-// column tc + 16b), an entry no broadcast or output reads.  `it` counts
-// barriers (buffer parity).
-template <int KA>
-__device__ __forceinline__ void chol_steps(double (&s)[8][8], double* rowbuf, double* rdiag, int n,
-                                           int tr, int tc, int& bad, int& it) {
-  const int kend = min(16 * KA + 16, n);
-  for (int k = 16 * KA; k < kend; k += 2) {
-    double* r0 = rowbuf + (it & 1) * 2 * CHOL_NMAX;
-    double* r1 = r0 + CHOL_NMAX;
-    ++it;
-    const bool two = k + 1 < kend;
-    if (tr == (k & 15)) {
-#pragma unroll
-      for (int b = 0; b < 8; ++b) r0[tc + 16 * b] = s[KA][b];
-    }
-    if (two && tr == ((k + 1) & 15)) {
-#pragma unroll
-      for (int b = 0; b < 8; ++b) r1[tc + 16 * b] = s[KA][b];
-    }
-    __syncthreads();
-    double d0 = r0[k];
-    if (!(d0 > 0.0)) {
-      d0 = 1e-300;
-      bad = 1;
-    }
-    // one reciprocal square root on the serial chain instead of sqrt + divide
-    const double ri0 = rsqrt(d0), sq0 = d0 * ri0;
-    if (threadIdx.x == 0) rdiag[k] = ri0;
-    double ci0[8], cj0[8];
-#pragma unroll
-    for (int a = KA; a < 8; ++a) ci0[a] = (a > KA || tr + 16 * a > k) ? r0[tr + 16 * a] * ri0 : 0.0;
-#pragma unroll
-    for (int b = KA; b < 8; ++b) cj0[b] = (b > KA || tc + 16 * b > k) ? r0[tc + 16 * b] * ri0 : 0.0;
-#pragma unroll
-    for (int a = KA; a < 8; ++a)
-#pragma unroll
-      for (int b = a; b < 8; ++b) s[a][b] = fma(-ci0[a], cj0[b], s[a][b]);
-    double sq1 = 0.0, ri1 = 0.0;
-    if (two) {
-      // row k+1 after step k: fma(-c0[k+1], c0[x], S[k+1][x]), as its owner
-      // would have broadcast it
-      const double c0k1 = r0[k + 1] * ri0;
-      double d1 = fma(-c0k1, c0k1, r1[k + 1]);
-      if (!(d1 > 0.0)) {
-        d1 = 1e-300;
-        bad = 1;
-      }
-      ri1 = rsqrt(d1);
-      sq1 = d1 * ri1;
-      if (threadIdx.x == 0) rdiag[k + 1] = ri1;
-      double ci1[8], cj1[8];
-#pragma unroll
-      for (int a = KA; a < 8; ++a) {
-        const int i = tr + 16 * a;
-        ci1[a] = (a > KA || i > k + 1) ? fma(-c0k1, r0[i] * ri0, r1[i]) * ri1 : 0.0;
-      }
-#pragma unroll
-      for (int b = KA; b < 8; ++b) {
-        const int j = tc + 16 * b;
-        cj1[b] = (b > KA || j > k + 1) ? fma(-c0k1, r0[j] * ri0, r1[j]) * ri1 : 0.0;
-      }
-#pragma unroll
-      for (int a = KA; a < 8; ++a)
-#pragma unroll
-        for (int b = a; b < 8; ++b) s[a][b] = fma(-ci1[a], cj1[b], s[a][b]);
-    }
-    if (tr == (k & 15)) {
-#pragma unroll
-      for (int b = KA; b < 8; ++b) {
-        const int j = tc + 16 * b;
-        s[KA][b] = (j > k) ? s[KA][b] * ri0 : (j == k ? sq0 : s[KA][b]);
-      }
-    }
-    if (two && tr == ((k + 1) & 15)) {
-#pragma unroll
-      for (int b = KA; b < 8; ++b) {
-        const int j = tc + 16 * b;
-        s[KA][b] = (j > k + 1) ? s[KA][b] * ri1 : (j == k + 1 ? sq1 : s[KA][b]);
-      }
-    }
+// block (a, b) of accumulator slot s of wave w: slots 0, 1 the diagonal
+// blocks (w, w), (7-w, 7-w); 2..8 the blocks above them in those columns;
+// 9..15 the blocks below them (the F part)
+__device__ __forceinline__ void chol_slot(int w, int s, int& a, int& b) {
+  const int c0 = w, c1 = 7 - w;
+  if (s == 0) {
+    a = c0; b = c0;
+  } else if (s == 1) {
+    a = c1; b = c1;
+  } else if (s <= 8) {
+    const int t = s - 2;
+    if (t < w) { a = t; b = c0; } else { a = t - w; b = c1; }
+  } else {
+    const int t = s - 9;
+    if (t < 7 - w) { a = c0 + 1 + t; b = c0; } else { a = 8 - w + (t - (7 - w)); b = c1; }
   }
 }
 
-// Inverse steps k in [16 KA, 16 KA + 16) from the top down (k < n), in pairs
-// (k, k-1) inside the block: rows of blocks a > KA lie below k (no update),
-// columns of blocks b < KA lie left of k, where row k of X = R^{-1} is zero.
-template <int KA>
-__device__ __forceinline__ void inv_steps(double (&s)[8][8], const double* Rl, const double* rdiag,
-                                          double* rowbuf, int n, int tr, int tc, int& it) {
-  const int khi = min(16 * KA + 15, n - 1);
-  for (int k = khi; k >= 16 * KA; k -= 2) {
-    double* x0 = rowbuf + (it & 1) * 2 * CHOL_NMAX;
-    double* x1 = x0 + CHOL_NMAX;
-    ++it;
-    const bool two = k - 1 >= 16 * KA;
-    if (tr == (k & 15)) {
-      const double rkk = rdiag[k];
+// 16 x 16 block in the MFMA output layout: row (lane / 16) + 4 r, column lane % 16
+__device__ __forceinline__ tnx_d4 chol_ld(const double* p, int ld, int lane) {
+  tnx_d4 v;
 #pragma unroll
-      for (int b = KA; b < 8; ++b) {
-        s[KA][b] *= rkk;
-        x0[tc + 16 * b] = s[KA][b];
+  for (int r = 0; r < 4; ++r) v[r] = p[((lane >> 4) + 4 * r) * ld + (lane & 15)];
+  return v;
+}
+__device__ __forceinline__ void chol_st(double* p, int ld, int lane, tnx_d4 v) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) p[((lane >> 4) + 4 * r) * ld + (lane & 15)] = v[r];
+}
+
+// acc += sgn * A B over K = 16, A[m][k] = pa[m * sam + k * sak], B[k][n] = pb[k * sbk + n]
+__device__ __forceinline__ tnx_d4 chol_mma(tnx_d4 acc, const double* pa, int sam, int sak,
+                                           const double* pb, int sbk, int lane, double sgn) {
+  const int mn = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int kk = 0; kk < 16; kk += 4) {
+    const double av = sgn * pa[mn * sam + (kk + kq) * sak];
+    const double bv = pb[(kk + kq) * sbk + mn];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ double chol_readlane(double v, int l) {
+  const long long x = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)x, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(x >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// d (the diagonal block, output layout) -> R_pp with zeros below the
+// diagonal; f -> F_pp = R_pp^{-T}.  Step k: R[k][j] = D[k][j] / sqrt(D[k][k])
+// (j > k), F[k][j] = E[k][j] / sqrt(D[k][k]); rows i > k lose R[k][i] times
+// row k of R and of F.  Row k sits in lanes 16 (k % 4) .. +15, register k / 4.
+__device__ __forceinline__ void chol_diag(tnx_d4& d, tnx_d4& f, int lane, int& bad) {
+  const int j = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) f[r] = (g + 4 * r == j) ? 1.0 : 0.0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int kr = k >> 2, src = 16 * (k & 3);
+    double piv = chol_readlane(d[kr], src + k);
+    const double dkj = __shfl(d[kr], src + j, 64);
+    const double ekj = __shfl(f[kr], src + j, 64);
+    double dki[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) dki[r] = __shfl(d[kr], src + g + 4 * r, 64);
+    if (!(piv > 0.0)) {
+      piv = 1e-300;
+      bad = 1;
+    }
+    // one reciprocal square root on the serial chain instead of sqrt + divide
+    const double rk = rsqrt(piv), sq = piv * rk;
+    const double rkj = (j > k) ? dkj * rk : (j == k ? sq : 0.0);
+    const double fkj = ekj * rk;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (g + 4 * r > k) {
+        const double rki = dki[r] * rk;
+        d[r] = fma(-rki, rkj, d[r]);
+        f[r] = fma(-rki, fkj, f[r]);
       }
     }
-    if (two && tr == ((k - 1) & 15)) {   // row k-1 before step k
-#pragma unroll
-      for (int b = KA; b < 8; ++b) x1[tc + 16 * b] = s[KA][b];
-    }
-    __syncthreads();
-    double ra[8], xv[8];
-#pragma unroll
-    for (int a = 0; a <= KA; ++a)
-      ra[a] = (a < KA || tr + 16 * a < k) ? Rl[(tr + 16 * a) * CHOL_LD + k] : 0.0;
-#pragma unroll
-    for (int b = KA; b < 8; ++b) xv[b] = x0[tc + 16 * b];
-#pragma unroll
-    for (int a = 0; a <= KA; ++a)
-#pragma unroll
-      for (int b = KA; b < 8; ++b) s[a][b] = fma(-ra[a], xv[b], s[a][b]);
-    if (two) {
-      const int k1 = k - 1;
-      const double rk1 = rdiag[k1];
-      const double rk1k = Rl[k1 * CHOL_LD + k];
-      if (tr == (k1 & 15)) {
-#pragma unroll
-        for (int b = KA; b < 8; ++b) s[KA][b] *= rk1;
-      }
-      // X[k-1,:] = (row k-1 after step k) / R[k-1,k-1], as its owner holds it
-      double xw[8];
-#pragma unroll
-      for (int b = KA; b < 8; ++b) xw[b] = fma(-rk1k, xv[b], x1[tc + 16 * b]) * rk1;
-#pragma unroll
-      for (int a = 0; a <= KA; ++a)
-        ra[a] = (a < KA || tr + 16 * a < k1) ? Rl[(tr + 16 * a) * CHOL_LD + k1] : 0.0;
-#pragma unroll
-      for (int a = 0; a <= KA; ++a)
-#pragma unroll
-        for (int b = KA; b < 8; ++b) s[a][b] = fma(-ra[a], xw[b], s[a][b]);
+    if (g == (k & 3)) {
+      d[kr] = rkj;
+      f[kr] = fkj;
     }
   }
 }
 
 __global__ __launch_bounds__(NT) void chol_inv_kernel(const tnx_chol_job* __restrict__ jobs) {
   extern __shared__ double lds[];
-  double* Rl = lds;                         // R, row stride CHOL_LD
-  double* rowbuf = lds + CHOL_NMAX * CHOL_LD;  // two broadcast row pairs
-  double* rdiag = rowbuf + 4 * CHOL_NMAX;      // 1 / R[k,k]
+  double* Sl = lds;                          // CHOL_NMAX rows of CB_LD
+  double* Xd = lds + CHOL_NMAX * CB_LD;      // F_pp, 16 rows of CB_XLD per panel
   const tnx_chol_job J = jobs[blockIdx.x];
   const int n = J.n;
   if (n <= 0 || n > CHOL_NMAX) return;  // rejected on the host
   const double* G = (const double*)J.g;
   const int ld = J.ld > 0 ? J.ld : n;
-  __shared__ double red[4];
-  __shared__ double shift;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
-  double s[8][8];
-  double trace = 0.0;
-#pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    const int i = tr + 16 * a;
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const int j = tc + 16 * b;
-      s[a][b] = (i < n && j < n) ? G[(int64_t)i * ld + j] : 0.0;
-      if (i == j) trace += s[a][b];
+  const int nb = (n + 15) >> 4, np = 16 * nb;
+  __shared__ double red[8];
+  __shared__ int bad_s;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // G padded with the identity; trace (shift) and largest |G - I| entry
+  double trace = 0.0, emax = 0.0;
+  for (int i = wave; i < np; i += 4)
+    for (int j = lane; j < np; j += 64) {
+      double v = (i == j) ? 1.0 : 0.0;
+      if (i < n && j < n) {
+        const double gv = G[(int64_t)i * ld + j];
+        if (i == j) trace += gv;
+        emax = fmax(emax, fabs(gv - v));
+        v = gv;
+      }
+      Sl[i * CB_LD + j] = v;
     }
-  }
   trace = tnx_wave_sum(trace);
-  if (lane == 0) red[wave] = trace;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const double t = J.shift_ref ? *(const double*)J.shift_ref
-                                 : ((red[0] + red[1]) + (red[2] + red[3]));
-    shift = J.shift_scale * t;
+  for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o, 64));
+  if (lane == 0) {
+    red[wave] = trace;
+    red[4 + wave] = emax;
   }
+  if (threadIdx.x == 0) bad_s = 0;
   __syncthreads();
+  const double shift = J.shift_scale * (J.shift_ref ? *(const double*)J.shift_ref
+                                                    : ((red[0] + red[1]) + (red[2] + red[3])));
+  const double em = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
+  double* R = (double*)J.r;
+  double* Ri = (double*)J.rinv;
   // Near-identity Gram matrix (the last pass of CholeskyQR3, whose panel is
   // already orthonormal to working accuracy): with S = I + E, ||E||_max <=
   // CHOL_NEAR_ID and no shift, R = I + U (U = strict upper part of E plus
   // half its diagonal) and R^{-1} = I - U hold up to O(E^2) <= 1e-20, below
-  // the rounding of the sequential factorisation, and need no sequential
-  // steps.
-  {
-    double emax = 0.0;
-#pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        const int i = tr + 16 * a, j = tc + 16 * b;
-        if (i < n && j < n) emax = fmax(emax, fabs(s[a][b] - (i == j ? 1.0 : 0.0)));
+  // the rounding of the factorisation, and need no sequential steps.
+  if (shift == 0.0 && em <= CHOL_NEAR_ID) {
+    for (int i = wave; i < n; i += 4)
+      for (int j = lane; j < n; j += 64) {
+        const double sv = Sl[i * CB_LD + j];
+        const double u = j > i ? sv : (j == i ? 0.5 * (sv - 1.0) : 0.0);
+        const double id = (i == j) ? 1.0 : 0.0;
+        R[(int64_t)i * ld + j] = id + u;
+        Ri[(int64_t)i * ld + j] = id - u;
       }
-    for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o, 64));
-    __shared__ double redm[4];
-    if (lane == 0) redm[wave] = emax;
-    __syncthreads();
-    const double em = fmax(fmax(redm[0], redm[1]), fmax(redm[2], redm[3]));
-    if (shift == 0.0 && em <= CHOL_NEAR_ID) {
-      double* R = (double*)J.r;
-      double* Ri = (double*)J.rinv;
+    if (threadIdx.x == 0 && J.status != 0) *(int32_t*)J.status = 0;
+    return;
+  }
+  int sa[16], sb[16];
+  tnx_d4 acc[16];
 #pragma unroll
-      for (int a = 0; a < 8; ++a)
+  for (int s = 0; s < 16; ++s) {
+    chol_slot(wave, s, sa[s], sb[s]);
+    acc[s] = tnx_d4{0.0, 0.0, 0.0, 0.0};
+    if (s <= 8 && sb[s] < nb) {
+      acc[s] = chol_ld(Sl + 16 * sa[s] * CB_LD + 16 * sb[s], CB_LD, lane);
+      if (sa[s] == sb[s]) {   // the shift on the diagonal (not on the padding)
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-          const int i = tr + 16 * a, j = tc + 16 * b;
-          if (i < n && j < n) {
-            const double u = j > i ? s[a][b] : (j == i ? 0.5 * (s[a][b] - 1.0) : 0.0);
-            const double id = (i == j) ? 1.0 : 0.0;
-            R[(int64_t)i * ld + j] = id + u;
-            Ri[(int64_t)i * ld + j] = id - u;
-          }
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * sa[s] + (lane >> 4) + 4 * r;
+          if ((lane >> 4) + 4 * r == (lane & 15) && i < n) acc[s][r] += shift;
         }
-      if (threadIdx.x == 0 && J.status != 0) *(int32_t*)J.status = 0;
-      return;
+      }
     }
   }
-  {
-    const double sh = shift;
+  int bad = 0;
+  for (int p = 0; p < nb; ++p) {
+    const double* Pp = Sl + 16 * p * CB_LD;       // row panel p
+    if (p > 0) {   // trailing update of panel p-1, diagonal blocks first
+      const double* Pq = Pp - 16 * CB_LD;
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-      for (int b = 0; b < 8; ++b)
-        if (tr + 16 * a == tc + 16 * b && tr + 16 * a < n) s[a][b] += sh;
-  }
-  int bad = 0, it = 0;
-  // Cholesky: row k of the tiles becomes row k of R at step k.  The trailing
-  // block is updated on and above the diagonal blocks, so the part of row k
-  // right of column k is current when its owners broadcast it.
-  chol_steps<0>(s, rowbuf, rdiag, n, tr, tc, bad, it);
-  chol_steps<1>(s, rowbuf, rdiag, n, tr, tc, bad, it);
-  chol_steps<2>(s, rowbuf, rdiag, n, tr, tc, bad, it);
-  chol_steps<3>(s, rowbuf, rdiag, n, tr, tc, bad, it);
-  chol_steps<4>(s, rowbuf, rdiag, n, tr, tc, bad, it);
-  chol_steps<5>(s, rowbuf, rdiag, n, tr, tc, bad, it);
-  chol_steps<6>(s, rowbuf, rdiag, n, tr, tc, bad, it);
-  chol_steps<7>(s, rowbuf, rdiag, n, tr, tc, bad, it);
-  double* R = (double*)J.r;
-#pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    const int i = tr + 16 * a;
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const int j = tc + 16 * b;
-      if (i < n && j < n) R[(int64_t)i * ld + j] = (j >= i) ? s[a][b] : 0.0;
-      Rl[i * CHOL_LD + j] = s[a][b];
+      for (int s = 0; s < 2; ++s)
+        if (sa[s] > p - 1 && sa[s] < nb)
+          acc[s] = chol_mma(acc[s], Pq + 16 * sa[s], 1, CB_LD, Pq + 16 * sb[s], CB_LD, lane, -1.0);
     }
+    if (min(p, 7 - p) == wave) {   // factor the diagonal block of panel p
+      tnx_d4 dd = (p == wave) ? acc[0] : acc[1], ff;
+      chol_diag(dd, ff, lane, bad);
+      chol_st(Sl + 16 * p * CB_LD + 16 * p, CB_LD, lane, dd);
+      chol_st(Xd + 16 * p * CB_XLD, CB_XLD, lane, ff);
+    }
+    if (p > 0) {
+      const double* Pq = Pp - 16 * CB_LD;
+      const double* Fq = Xd + 16 * (p - 1) * CB_XLD;
+#pragma unroll
+      for (int s = 2; s < 9; ++s)
+        if (sa[s] > p - 1 && sb[s] < nb)
+          acc[s] = chol_mma(acc[s], Pq + 16 * sa[s], 1, CB_LD, Pq + 16 * sb[s], CB_LD, lane, -1.0);
+#pragma unroll
+      for (int s = 9; s < 16; ++s)
+        if (sa[s] > p - 1 && sa[s] < nb && sb[s] <= p - 1) {
+          const bool diag = sb[s] == p - 1;
+          acc[s] = chol_mma(acc[s], Pq + 16 * sa[s], 1, CB_LD, diag ? Fq : Pq + 16 * sb[s],
+                            diag ? CB_XLD : CB_LD, lane, -1.0);
+        }
+    }
+    // row panel p of S (above the diagonal) and E (below it) as updated
+#pragma unroll
+    for (int s = 2; s < 16; ++s)
+      if (sa[s] == p && sb[s] < nb) chol_st(Sl + 16 * p * CB_LD + 16 * sb[s], CB_LD, lane, acc[s]);
+    __syncthreads();
+    // R_pb = F_pp S_pb, F_pe = F_pp E_pe
+    const double* Fp = Xd + 16 * p * CB_XLD;
+#pragma unroll
+    for (int s = 2; s < 16; ++s)
+      if (sa[s] == p && sb[s] < nb) {
+        const tnx_d4 c = chol_mma(tnx_d4{0.0, 0.0, 0.0, 0.0}, Fp, CB_XLD, 1, Pp + 16 * sb[s], CB_LD,
+                                  lane, 1.0);
+        chol_st(Sl + 16 * p * CB_LD + 16 * sb[s], CB_LD, lane, c);
+      }
+    __syncthreads();
   }
+  if (bad && lane == 0) bad_s = 1;
   __syncthreads();
-  // X = R^{-1}: the tiles restart as the identity; row k is final once the
-  // rows below it have been subtracted.
-#pragma unroll
-  for (int a = 0; a < 8; ++a)
-#pragma unroll
-    for (int b = 0; b < 8; ++b)
-      s[a][b] = (tr + 16 * a == tc + 16 * b && tr + 16 * a < n) ? 1.0 : 0.0;
-  inv_steps<7>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
-  inv_steps<6>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
-  inv_steps<5>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
-  inv_steps<4>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
-  inv_steps<3>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
-  inv_steps<2>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
-  inv_steps<1>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
-  inv_steps<0>(s, Rl, rdiag, rowbuf, n, tr, tc, it);
-  double* Ri = (double*)J.rinv;
-#pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    const int i = tr + 16 * a;
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const int j = tc + 16 * b;
-      if (i < n && j < n) Ri[(int64_t)i * ld + j] = (j >= i) ? s[a][b] : 0.0;
+  // R (upper) and R^{-1} = F^T: F_pp from its buffer, the blocks below the
+  // diagonal of F from the lower part of Sl
+  for (int i = wave; i < n; i += 4)
+    for (int j = lane; j < n; j += 64) {
+      double rv = 0.0, xv = 0.0;
+      if (j >= i) {
+        rv = Sl[i * CB_LD + j];
+        xv = ((i >> 4) == (j >> 4)) ? Xd[16 * (j >> 4) * CB_XLD + (j & 15) * CB_XLD + (i & 15)]
+                                    : Sl[j * CB_LD + i];
+      }
+      R[(int64_t)i * ld + j] = rv;
+      Ri[(int64_t)i * ld + j] = xv;
     }
-  }
-  if (threadIdx.x == 0 && J.status != 0) *(int32_t*)J.status = bad;
+  if (threadIdx.x == 0 && J.status != 0) *(int32_t*)J.status = bad_s;
 }
 
 }  // namespace
