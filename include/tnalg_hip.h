@@ -441,6 +441,22 @@ typedef struct tnx_op {
   uint64_t a0, a1, a2;
 } tnx_op; /* 48 bytes */
 int tnx_run_ops(const tnx_op* ops /* host */, int nops, tnx_stream_t stream);
+/* No counterpart in the reference (its eigensolve returns before the next
+ * step's tensor moves start, MPSClass.py:883-889 then TBM:416-526): as
+ * tnx_lanczos_run_multi, with the recorded launches `next` (host array,
+ * nnext entries, may be 0) queued on the stream behind the Lanczos kernels
+ * and result copies.  The call returns once the results are on the host;
+ * the queued launches may still be running (later work on the stream
+ * follows them).  *next_rc (may be null) receives tnx_run_ops's code:
+ * TNX_ENOCONV there is a block Jacobi SVD of `next` at its sweep limit; a
+ * launch failure is also the call's return code.  A host that books one
+ * step's eigensolves while the device already runs the next step's gauge
+ * move and environment update uses this.                                   */
+int tnx_lanczos_run_multi_then(const tnx_matvec_plan* plans, int nplan,
+                               const tnx_lanczos_solve* solves, int nsolve, int nchunks, int ncv,
+                               int max_restarts, int min_iters, void* state,
+                               tnx_lanczos_result* results, const tnx_op* next, int nnext,
+                               int32_t* next_rc, tnx_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -1024,9 +1024,14 @@ int issue_finish(const tnx_lanczos_solve* solves, int nsolve, void* state, int n
 // nplan matvec plans in turn (one per precision of the batch), then the
 // shared orthogonalisation kernels over all solves.  The host polls the
 // active count one step behind the device, so one step stays queued.
+// With `next` (nnext recorded launches), those launches are queued behind
+// the result copies and the call returns once the results are on the host,
+// while the queued launches may still run; *next_rc receives their
+// tnx_run_ops code.
 int lanczos_run(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve* solves,
                 int nsolve, int nchunks, int ncv, int max_restarts, int min_iters, int x_dtype,
-                int w_dtype, void* state, tnx_lanczos_result* results, tnx_stream_t stream) {
+                int w_dtype, void* state, tnx_lanczos_result* results, tnx_stream_t stream,
+                const tnx_op* next = nullptr, int nnext = 0, int32_t* next_rc = nullptr) {
   if (int rc = check_args("tnx_lanczos_run", plans, nplan, solves, nsolve, nchunks, ncv,
                           max_restarts, x_dtype, w_dtype))
     return rc;
@@ -1082,12 +1087,28 @@ int lanczos_run(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve
   (void)hipEventDestroy(ev[1]);
   if (rc) return rc;
   if ((rc = issue_finish(solves, nsolve, state, nchunks, g_pin.res, s))) return rc;
-  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  int orc = TNX_OK;
+  if (nnext > 0) {
+    hipEvent_t fin;
+    TNX_CHECK_HIP(hipEventCreateWithFlags(&fin, hipEventDisableTiming));
+    TNX_CHECK_HIP(hipEventRecord(fin, s));
+    orc = tnx_run_ops(next, nnext, stream);   // sets the message of a failure
+    if (next_rc) *next_rc = orc;
+    const hipError_t e = hipEventSynchronize(fin);
+    (void)hipEventDestroy(fin);
+    if (e != hipSuccess) {
+      tnx_set_error("tnx_lanczos_run_multi_then: %s", hipGetErrorString(e));
+      return TNX_EHIP;
+    }
+  } else {
+    TNX_CHECK_HIP(hipStreamSynchronize(s));
+  }
   int nconv = 0;
   for (int i = 0; i < nsolve; ++i) {
     if (results) results[i] = g_pin.res[i];
     nconv += g_pin.res[i].converged;
   }
+  if (orc != TNX_OK && orc != TNX_ENOCONV) return orc;
   return nconv == nsolve ? TNX_OK : TNX_ENOCONV;
 }
 
@@ -1107,6 +1128,22 @@ extern "C" int tnx_lanczos_run_multi(const tnx_matvec_plan* plans, int nplan,
                                      tnx_lanczos_result* results, tnx_stream_t stream) {
   return lanczos_run(plans, nplan, solves, nsolve, nchunks, ncv, max_restarts, min_iters,
                      TNX_DTYPE_PER_SOLVE, TNX_DTYPE_PER_SOLVE, state, results, stream);
+}
+
+extern "C" int tnx_lanczos_run_multi_then(const tnx_matvec_plan* plans, int nplan,
+                                          const tnx_lanczos_solve* solves, int nsolve,
+                                          int nchunks, int ncv, int max_restarts, int min_iters,
+                                          void* state, tnx_lanczos_result* results,
+                                          const tnx_op* next, int nnext, int32_t* next_rc,
+                                          tnx_stream_t stream) {
+  if (next_rc) *next_rc = TNX_OK;
+  if (nnext < 0 || (nnext > 0 && next == nullptr)) {
+    tnx_set_error("tnx_lanczos_run_multi_then: bad launch list (nnext=%d)", nnext);
+    return TNX_EINVAL;
+  }
+  return lanczos_run(plans, nplan, solves, nsolve, nchunks, ncv, max_restarts, min_iters,
+                     TNX_DTYPE_PER_SOLVE, TNX_DTYPE_PER_SOLVE, state, results, stream, next,
+                     nnext, next_rc);
 }
 
 extern "C" int tnx_lanczos_run(const tnx_matvec_plan* plan, const tnx_lanczos_solve* solves,

@@ -144,6 +144,8 @@ SIGNATURES = {
                                 _vp, _vp]),
     'tnx_lanczos_run_multi': (_i, [ctypes.POINTER(MatvecPlan), _i, _vp, _i, _i, _i, _i, _i, _vp,
                                    _vp, _vp]),
+    'tnx_lanczos_run_multi_then': (_i, [ctypes.POINTER(MatvecPlan), _i, _vp, _i, _i, _i, _i, _i,
+                                        _vp, _vp, _vp, _i, _vp, _vp]),
     'tnx_lanczos_run_multi_static': (_i, [ctypes.POINTER(MatvecPlan), _i, _vp, _i, _i, _i, _i,
                                           _i, _vp, _vp]),
     'tnx_qr_f64': (_i, [_vp, _i, _vp]),

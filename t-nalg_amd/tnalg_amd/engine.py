@@ -1592,6 +1592,9 @@ class Engine:
         if self.lanczos_streams not in (1, 2):
             raise ValueError('lanczos_streams must be 1 or 2')
         self.split_min = 16   # smallest batch split over two streams
+        # run_queues: a step whose eigensolves cannot be re-queued has the
+        # next step's recorded pre-Lanczos launches queued by its Lanczos call
+        self.chain_next = True
         self._lz_stream = None
         self._lz_pool = None
         self._resident = collections.OrderedDict()
@@ -1996,7 +1999,7 @@ class Engine:
         """Encoded mode of a launch whose shape class may have been tuned."""
         return mode if self.tuner is None else self.tuner.mode_for(mode, ntiles, kmax)
 
-    def _lanczos(self, groups, spec=None, ahead=None, parts=None):
+    def _lanczos(self, groups, spec=None, ahead=None, parts=None, chained=None, sites=None):
         """One batched Lanczos solve over ``groups``: lists of (solve, plan),
         one list per matvec precision (each Krylov step runs every group's
         matvec, then the shared f64 kernels).  ``spec``: descriptor sets
@@ -2009,7 +2012,10 @@ class Engine:
         kernels (latency bound) overlap the other half's passes over its
         Krylov basis (bandwidth bound).  A solve's arithmetic does not depend
         on the other solves of its call, so the results are the same bits.
-        ``parts``: the split already made, as ``(groups, key)`` pairs."""
+        ``parts``: the split already made, as ``(groups, key)`` pairs.
+        ``chained``: recorded launches (the next step's pre-Lanczos phases)
+        that a single call queues behind its own kernels.  ``sites``: the
+        eigensolve site of every solve (by ``id``), for the warnings."""
         if parts is None:
             parts = [(part, _lanczos_key(part))
                      for part in _split_groups(groups, self.lanczos_streams, self.split_min)]
@@ -2021,7 +2027,7 @@ class Engine:
         restarts = self.cycle_restarts if self.cycle_steps else self.max_restarts
         t0 = time.perf_counter()
         if len(preps) == 1:
-            rcs = [self._lanczos_call(preps[0], self.stream(), restarts)]
+            rcs = [self._lanczos_call(preps[0], self.stream(), restarts, chained)]
         else:
             # the second stream starts after everything queued so far
             if self._lz_stream is None:
@@ -2041,8 +2047,10 @@ class Engine:
                 raise L.HipError('tnx_lanczos_run_multi failed (rc=%d): %s' % (rc, msg))
         self.counters['lanczos_calls'] += 1
         for prep in preps:
-            # the sites of the eigensolves (a queued next step moves _lz_site)
-            prep['sites'] = [s._lz_site for s, _ in prep['flat']]
+            # the sites of the eigensolves (a next step planned already has
+            # moved _lz_site)
+            prep['sites'] = ([sites[id(s)] for s, _ in prep['flat']] if sites is not None
+                             else [s._lz_site for s, _ in prep['flat']])
         return preps
 
     def _resume_possible(self, preps):
@@ -2122,16 +2130,28 @@ class Engine:
         return {'flat': flat, 'plans': plans, 'ngroups': len(groups), 'ptrs': ptrs,
                 'nchunks': meta['nchunks'], 'state': self.lz_states[k], 'res': res}
 
-    def _lanczos_call(self, prep, stream, restarts):
+    def _lanczos_call(self, prep, stream, restarts, chained=None):
         """The C Lanczos loop of one prepared call on ``stream`` (holds no
-        GIL; returns when the stream's work is done).  Returns ``(rc, msg)``:
+        GIL; returns when the stream's work is done, or with ``chained``
+        launches queued behind it, when its results are on the host; their
+        return code goes to ``prep['next_rc']``).  Returns ``(rc, msg)``:
         the library keeps its last error per thread, so the message is read
         on the thread that made the call."""
-        rc = self.lib.tnx_lanczos_run_multi(prep['plans'], prep['ngroups'],
-                                            ctypes_void(prep['ptrs'][2]), len(prep['flat']),
-                                            prep['nchunks'], self.ncv, restarts, self.min_iters,
-                                            ctypes_void(prep['state'].data_ptr()),
-                                            ctypes_void(prep['res'].ctypes.data), stream)
+        if chained is None:
+            rc = self.lib.tnx_lanczos_run_multi(prep['plans'], prep['ngroups'],
+                                                ctypes_void(prep['ptrs'][2]), len(prep['flat']),
+                                                prep['nchunks'], self.ncv, restarts,
+                                                self.min_iters,
+                                                ctypes_void(prep['state'].data_ptr()),
+                                                ctypes_void(prep['res'].ctypes.data), stream)
+        else:
+            nrc = ctypes.c_int32(0)
+            rc = self.lib.tnx_lanczos_run_multi_then(
+                prep['plans'], prep['ngroups'], ctypes_void(prep['ptrs'][2]), len(prep['flat']),
+                prep['nchunks'], self.ncv, restarts, self.min_iters,
+                ctypes_void(prep['state'].data_ptr()), ctypes_void(prep['res'].ctypes.data),
+                ctypes_void(chained.ctypes.data), len(chained), ctypes.byref(nrc), stream)
+            prep['next_rc'] = nrc.value
         msg = '' if rc in (L.OK, L.ENOCONV) else self.lib.tnx_last_error().decode(errors='replace')
         return rc, msg
 
@@ -2144,7 +2164,11 @@ class Engine:
         step's Lanczos call has returned and no eigensolve has to be resumed,
         the next step's pre-Lanczos phases are planned and queued before the
         host books the results, so the device does not wait for that host
-        work (the queued kernels follow the eigensolve on the stream)."""
+        work (the queued kernels follow the eigensolve on the stream).  When
+        no eigensolve can be re-queued at all (``cycle_steps`` off) and the
+        next step's descriptor set is resident, its launches are queued by
+        the Lanczos call itself, right behind its last kernel
+        (``chain_next``)."""
         pre = None
         while True:
             n, pre = self._step(solves, pre, True)
@@ -2163,6 +2187,7 @@ class Engine:
         plans, spec, host_miss, t_a, t_b, t_c = pre
         groups = _lanczos_groups([(s, ph) for s, _, ph in plans])
         preps = []
+        early = chained = None
         if groups:
             # the next step's host work overlaps this Lanczos call, when this
             # step needed host work (a batch whose descriptor sets are all
@@ -2173,10 +2198,23 @@ class Engine:
             ahead = None
             if self.plan_ahead and host_miss:
                 ahead = [s for s, _, _ in plans if s.queue]
-            preps = self._lanczos(groups, spec, ahead, parts)
+            sites = None
+            if (lookahead and not host_miss and not self.cycle_steps and len(parts) == 1
+                    and self.chain_next):
+                sites = {id(s): st.site for s, st, _ in plans}
+                # no eigensolve can be re-queued: the next step is known now.
+                # A resident next step without casts has its launches queued
+                # by the Lanczos call behind its own kernels, so the device
+                # runs them while the host books this step.
+                early = self._pre_plan(solves)
+                if early is not None and early['prog'] is not None and not early['casts']:
+                    chained = early['prog']
+            preps = self._lanczos(groups, spec, ahead, parts, chained, sites)
         t_d = time.perf_counter()
         nxt = None
-        if lookahead and not self._resume_possible(preps):
+        if early is not None:
+            nxt = self._pre_issue(early, preps[0]['next_rc'] if chained is not None else None)
+        elif lookahead and not self._resume_possible(preps):
             nxt = self._step_pre(solves)
         self._lanczos_book(preps)
         self.counters['t_plan'] += t_b - t_a
@@ -2206,6 +2244,14 @@ class Engine:
         """Plan the next queued entry of every solve of ``solves`` with one
         and queue the step's pre-Lanczos phases (gauge moves, environment
         update, casts).  Returns the step context, or None without work."""
+        ctx = self._pre_plan(solves)
+        return None if ctx is None else self._pre_issue(ctx)
+
+    def _pre_plan(self, solves):
+        """The host part of ``_step_pre``: pops and plans the next queued
+        entry of every solve, finds the step's resident descriptor set and
+        its recorded launches (``ctx['prog']``, None before the set is
+        resident).  Queues nothing on the device."""
         self.settle()
         spec, self._spec = self._spec, None
         solves = [s for s in (solves or self.solves) if s.queue and s.failed is None]
@@ -2244,24 +2290,40 @@ class Engine:
             else:
                 s.dims = ph['dims_after_t']
             plans.append((s, st, ph))
-        if casts:
-            self._mix(casts)   # one launch for all of them
-
         t_b = time.perf_counter()
         # every descriptor of the step's pre-Lanczos phases, resident on the
         # device per combination of (plan, solve) (steps repeat every sweep)
         rkey = ('pre',) + tuple((ph['uid'],) + s.reloc_key for s, _, ph in plans)
         ent = self._resident_get(rkey)
+        prog = None
+        if ent is not None:
+            ver = self.tuner.version if self.tuner is not None else 0
+            if ent.host is None or ent.host[1] != ver:
+                ent.host = (self._pre_program(ent), ver)
+            prog = ent.host[0]
+        return {'plans': plans, 'casts': casts, 'spec': spec, 'rkey': rkey, 'ent': ent,
+                'prog': prog, 't_a': t_a, 't_b': t_b}
+
+    def _pre_issue(self, ctx, issued_rc=None):
+        """Queue the casts and the recorded pre-Lanczos launches of a planned
+        step (``issued_rc``: the launches were already queued behind the
+        previous Lanczos call, with this return code).  Returns the step
+        context of ``_step_pre``."""
+        plans, spec, rkey, ent = ctx['plans'], ctx['spec'], ctx['rkey'], ctx['ent']
         host_miss = ent is None   # a new batch composition: host work worth overlapping
-        if ent is None:
-            got = spec.get('pre') if spec else None
-            ent = self._resident_put(rkey, *(got[1] if got and got[0] == rkey else
-                                             assemble_pre([(s, ph) for s, _, ph in plans])))
-        ver = self.tuner.version if self.tuner is not None else 0
-        if ent.host is None or ent.host[1] != ver:
-            ent.host = (self._pre_program(ent), ver)
-        prog = ent.host[0]
-        rc = self.lib.tnx_run_ops(ctypes_void(prog.ctypes.data), len(prog), self.stream())
+        if issued_rc is None:
+            if ctx['casts']:
+                self._mix(ctx['casts'])   # one launch for all of them
+            if ent is None:
+                got = spec.get('pre') if spec else None
+                ent = self._resident_put(rkey, *(got[1] if got and got[0] == rkey else
+                                                 assemble_pre([(s, ph) for s, _, ph in plans])))
+                ver = self.tuner.version if self.tuner is not None else 0
+                ent.host = (self._pre_program(ent), ver)
+            prog = ent.host[0]
+            rc = self.lib.tnx_run_ops(ctypes_void(prog.ctypes.data), len(prog), self.stream())
+        else:
+            rc = issued_rc
         if rc == L.ENOCONV:
             # a block Jacobi SVD reached its sweep limit; the rotation flag is
             # shared by the launch: every solve with an SVD job in it is flagged
@@ -2275,4 +2337,4 @@ class Engine:
         else:
             L.check(rc, 'tnx_run_ops (pre-Lanczos phases)')
         t_c = time.perf_counter()
-        return plans, spec, host_miss, t_a, t_b, t_c
+        return plans, spec, host_miss, ctx['t_a'], ctx['t_b'], t_c

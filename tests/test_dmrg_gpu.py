@@ -202,3 +202,42 @@ def test_device_initial_draw(gpu):
         eng.run_queues([s])
         energies.append(s.last_theta)
     assert energies[0] == energies[1]
+
+
+def test_chained_next_step_same_bits(gpu):
+    """A step's Lanczos call that queues the next step's recorded launches
+    behind its own kernels (tnx_lanczos_run_multi_then) gives the same bits
+    as queuing them after the call returns: energies, Ritz values of every
+    local solve and the final MPS tensors, mixed precision schedule."""
+    import torch
+    from tnalg_amd import parameters as P
+    from tnalg_amd.dmrg import _schedule_from
+    from tnalg_amd.engine import Engine
+    para = P.generate_parameters_dmrg('chain')
+    para.update(l=24, chi=32, jxy=1.0, jz=0.8, hz=0.1, sweep_time=4, eigs_tol=1e-10, tau=1e-4,
+                dt_ob=4, break_tol=0.0)
+    para = P.make_consistent_parameter_dmrg(para)
+    out = []
+    for chain in (True, False):
+        eng = Engine(precision_schedule=_schedule_from('mixed'), lowp=True)
+        eng.chain_next = chain
+        solves = eng.add_solves([(para, s) for s in (3, 4, 5)])
+        for s in solves:
+            s.schedule_canonicalise()
+        eng.run_queues(solves)
+        for t in range(para['sweep_time']):
+            for s in solves:
+                s.schedule_sweep(t)
+            eng.run_queues(solves)
+        print('chain_next', chain, 'steps', eng.counters['steps'], flush=True)
+        e = eng.finalize(solves)
+        torch.cuda.synchronize()
+        out.append((np.array(e), [list(s.thetas) for s in solves],
+                    [[s.mps[i][:s.dims[i] * s.d * s.dims[i + 1]].cpu().numpy().copy()
+                      for i in range(para['l'])] for s in solves]))
+    (e1, th1, m1), (e2, th2, m2) = out
+    assert np.array_equal(e1, e2)
+    assert th1 == th2
+    for a, b in zip(m1, m2):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
