@@ -484,10 +484,30 @@ __global__ __launch_bounds__(NT) void k_update_dots(const tnx_lanczos_solve* __r
 // convergence).  They come first in the grid, so the serial eigensolve
 // starts with the chunks' pass (blocks [nsolve, nsolve + nchunks)) instead
 // of after it.
+// The active count after a step, written to the host by the last eigensolve
+// block of k_update_norm to finish (every block has made its convergence
+// decision by then): each block counts itself in nact[1] after a fence; the
+// one that completes the count reads nact, writes it to `host` (pinned
+// memory the host reads after the launch's event) and clears the count for
+// the next launch.  No block waits for another.
+__device__ __forceinline__ void publish_nact(const LzLayout& L, int nsolve, int32_t* host) {
+  if (threadIdx.x != 0) return;
+  __threadfence();
+  if (atomicAdd(L.nact + 1, 1) == nsolve - 1) {
+    __threadfence();
+    const int32_t v = __hip_atomic_load(L.nact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (host != nullptr) {
+      *(volatile int32_t*)host = v;
+      __threadfence_system();
+    }
+    __hip_atomic_store(L.nact + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __restrict__ S, int nsolve,
                                                     void* state, int nchunks, int j, int x_dtype,
                                                     int normalize, int min_iters,
-                                                    unsigned long long* ts) {
+                                                    int32_t* nact_host, unsigned long long* ts) {
   tnx_stamp_begin(ts);
   LzLayout L = lz_layout(state, nsolve, nchunks);
   const bool solver = (int)blockIdx.x < nsolve;
@@ -500,7 +520,10 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
     ci = chunk_of(S, nsolve, (int)blockIdx.x - nsolve);
     s = ci.s;
   }
-  if (!solve_active(L.act, s)) return;
+  if (!solve_active(L.act, s)) {
+    if (solver) publish_nact(L, nsolve, nact_host);
+    return;
+  }
   const tnx_lanczos_solve D = S[s];
   __shared__ double h[NCV_MAX + 1];
   __shared__ double beta_s;
@@ -602,6 +625,7 @@ __global__ __launch_bounds__(NT) void k_update_norm(const tnx_lanczos_solve* __r
       atomicSub(L.nact, 1);
     }
   }
+  publish_nact(L, nsolve, nact_host);
   tnx_stamp_end(ts);
 }
 
@@ -867,7 +891,10 @@ __global__ __launch_bounds__(NT) void k_final(const tnx_lanczos_solve* __restric
 
 __global__ void k_set_nact(void* state, int nsolve, int nchunks) {
   LzLayout L = lz_layout(state, nsolve, nchunks);
-  if (threadIdx.x == 0 && blockIdx.x == 0) *L.nact = nsolve;
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    L.nact[0] = nsolve;
+    L.nact[1] = 0;   // blocks of the running k_update_norm that have finished
+  }
 }
 
 __global__ void k_results(void* state, int nsolve, int nchunks, tnx_lanczos_result* res) {
@@ -956,7 +983,7 @@ int issue_start(const tnx_lanczos_solve* solves, int nsolve, void* state, int nc
 // tridiagonal solve and the convergence test.
 int issue_step(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve* solves,
                int nsolve, void* state, int nchunks, int j, int ncv, int min_iters, int x_dtype,
-               int w_dtype, tnx_stream_t stream) {
+               int w_dtype, tnx_stream_t stream, int32_t* nact_host = nullptr) {
   hipStream_t s = (hipStream_t)stream;
   LzLayout L = lz_layout(state, nsolve, nchunks);
   for (int p = 0; p < nplan; ++p)
@@ -989,7 +1016,7 @@ int issue_step(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve*
   (void)ncv;
   mk = prof ? tnx_prof_mark(TNX_PC_UPDATE_NORM, s) : -1;
   hipLaunchKernelGGL(k_update_norm, dim3(nchunks + nsolve), block, 0, s, solves, nsolve, state,
-                     nchunks, j, x_dtype, 1, min_iters, tnx_prof_ts(mk));
+                     nchunks, j, x_dtype, 1, min_iters, nact_host, tnx_prof_ts(mk));
   // (r read, V[j+1] and x written; the basis rows of the second correction,
   // read only when it is not negligible, are not counted)
   tnx_prof_done(mk, s, E * (2.0 * vs + xs));
@@ -1058,15 +1085,13 @@ int lanczos_run(const tnx_matvec_plan* plans, int nplan, const tnx_lanczos_solve
     if (r > 0 && (rc = issue_restart(solves, nsolve, state, nchunks, ncv, x_dtype, s))) break;
     for (int j = (r > 0 ? lz_keep(ncv) : 0); j < ncv && !done; ++j) {
       tnx_prof_set(sampled && r == 0 && j == 0);
+      // lagged poll: the step writes its active count to pin[slot], then
+      // wait for the previous step (the device keeps one step queued)
+      const int slot = issued & 1;
       rc = issue_step(plans, nplan, solves, nsolve, state, nchunks, j, ncv, min_iters, x_dtype,
-                      w_dtype, stream);
+                      w_dtype, stream, (int32_t*)(pin + slot));
       tnx_prof_set(false);
       if (rc) break;
-      // lagged poll: copy the active count of this step, then wait for the
-      // previous step's copy (the device keeps one step of work queued)
-      const int slot = issued & 1;
-      TNX_CHECK_HIP(hipMemcpyAsync((void*)(pin + slot), L.nact, sizeof(int32_t),
-                                   hipMemcpyDeviceToHost, s));
       TNX_CHECK_HIP(hipEventRecord(ev[slot], s));
       ++issued;
       if (issued >= 2) {
