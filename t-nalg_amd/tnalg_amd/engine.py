@@ -2310,6 +2310,7 @@ class Engine:
         previous Lanczos call, with this return code).  Returns the step
         context of ``_step_pre``."""
         plans, spec, rkey, ent = ctx['plans'], ctx['spec'], ctx['rkey'], ctx['ent']
+        t_b = ctx['t_b'] if issued_rc is None else time.perf_counter()
         host_miss = ent is None   # a new batch composition: host work worth overlapping
         if issued_rc is None:
             if ctx['casts']:
@@ -2337,4 +2338,6 @@ class Engine:
         else:
             L.check(rc, 'tnx_run_ops (pre-Lanczos phases)')
         t_c = time.perf_counter()
-        return plans, spec, host_miss, ctx['t_a'], ctx['t_b'], t_c
+        # t_a..t_b: the planning; t_b..t_c: the queuing (measured after the
+        # previous Lanczos call when the launches were queued by that call)
+        return plans, spec, host_miss, t_b - (ctx['t_b'] - ctx['t_a']), t_b, t_c
