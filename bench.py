@@ -535,7 +535,7 @@ def trace_summary(path, t0=None, t1=None):
                 inside[q] = False
     # per stream (one per sub-batch): its kernel time, the time between its
     # consecutive kernels, and its kernel time inside and outside Lanczos
-    # calls (from a call's k_zero_words to its k_results)
+    # calls (from a call's k_zero_words to its k_final)
     per_stream = {}
     for q in sorted({r[3] for r in rows}):
         rs = [r for r in rows if r[3] == q]
@@ -548,7 +548,7 @@ def trace_summary(path, t0=None, t1=None):
             if r[2] == 'k_zero_words':
                 in_lz = True
             lz[in_lz] += r[1] - r[0]
-            if r[2] == 'k_results':
+            if r[2] == 'k_final':
                 in_lz = False
             if i:
                 gap += max(0, r[0] - rs[i - 1][1])

@@ -863,13 +863,21 @@ __global__ __launch_bounds__(NT) void k_irl_apply(const tnx_lanczos_solve* __res
   else irl_apply_rows((double*)D.V, D, C, ci, m, k, xd);
 }
 
-// out = sum_k y[k] V[k] for every solve
+// out = sum_k y[k] V[k] for every solve; the first chunk's block of each
+// solve also writes its result record to `res`
 __global__ __launch_bounds__(NT) void k_final(const tnx_lanczos_solve* __restrict__ S, int nsolve,
-                                              void* state, int nchunks) {
+                                              void* state, int nchunks,
+                                              tnx_lanczos_result* res) {
   LzLayout L = lz_layout(state, nsolve, nchunks);
   ChunkInfo ci = chunk_of(S, nsolve);
   const tnx_lanczos_solve D = S[ci.s];
   const LzState& st = L.st[ci.s];
+  if (ci.c == 0 && threadIdx.x == 0) {
+    res[ci.s].theta = st.theta;
+    res[ci.s].resid = st.resid;
+    res[ci.s].iters = st.iters;
+    res[ci.s].converged = st.converged;
+  }
   const int kk = st.jlast + 1;
   double* out = (double*)D.out;
   if (D.v_dtype == TNX_F32) {
@@ -889,24 +897,6 @@ __global__ __launch_bounds__(NT) void k_final(const tnx_lanczos_solve* __restric
   }
 }
 
-__global__ void k_set_nact(void* state, int nsolve, int nchunks) {
-  LzLayout L = lz_layout(state, nsolve, nchunks);
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    L.nact[0] = nsolve;
-    L.nact[1] = 0;   // blocks of the running k_update_norm that have finished
-  }
-}
-
-__global__ void k_results(void* state, int nsolve, int nchunks, tnx_lanczos_result* res) {
-  LzLayout L = lz_layout(state, nsolve, nchunks);
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < nsolve) {
-    res[s].theta = L.st[s].theta;
-    res[s].resid = L.st[s].resid;
-    res[s].iters = L.st[s].iters;
-    res[s].converged = L.st[s].converged;
-  }
-}
 
 // pinned staging for the polled active count and the results
 struct Pinned {
@@ -941,8 +931,14 @@ extern "C" size_t tnx_lanczos_state_bytes(int nsolve, int nchunks, int ncv) {
 namespace {
 
 // Zeroes n 8-byte words (the solves' state at the start of a call; a kernel
-// rather than a memset node, so the launch sequence is plain kernels).
-__global__ void k_zero_words(uint64_t* p, size_t n) {
+// rather than a memset node, so the launch sequence is plain kernels) and
+// sets the active count.
+__global__ void k_zero_words(uint64_t* p, size_t n, int32_t* nact, int nsolve) {
+  // every solve active; no block of a k_update_norm launch counted done yet
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    nact[0] = nsolve;
+    nact[1] = 0;
+  }
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
     p[i] = 0;
@@ -969,8 +965,7 @@ int issue_start(const tnx_lanczos_solve* solves, int nsolve, void* state, int nc
   const size_t words = sizeof(LzState) * (size_t)nsolve / sizeof(uint64_t);
   static_assert(sizeof(LzState) % sizeof(uint64_t) == 0, "LzState is whole words");
   hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((words + 255) / 256 < 1024 ? (words + 255) / 256 : 1024)),
-                     dim3(256), 0, s, (uint64_t*)L.st, words);
-  hipLaunchKernelGGL(k_set_nact, dim3(1), dim3(64), 0, s, state, nsolve, nchunks);
+                     dim3(256), 0, s, (uint64_t*)L.st, words, L.nact, nsolve);
   hipLaunchKernelGGL(k_init_norm, dim3(nchunks), dim3(NT), 0, s, solves, nsolve, state, nchunks);
   hipLaunchKernelGGL(k_init_scale, dim3(nchunks), dim3(NT), 0, s, solves, nsolve, state, nchunks,
                      x_dtype);
@@ -1040,9 +1035,7 @@ int issue_restart(const tnx_lanczos_solve* solves, int nsolve, void* state, int 
 // Eigenvectors into every solve's `out`, results into `res` (device-visible).
 int issue_finish(const tnx_lanczos_solve* solves, int nsolve, void* state, int nchunks,
                  tnx_lanczos_result* res, hipStream_t s) {
-  hipLaunchKernelGGL(k_final, dim3(nchunks), dim3(NT), 0, s, solves, nsolve, state, nchunks);
-  hipLaunchKernelGGL(k_results, dim3((nsolve + 63) / 64), dim3(64), 0, s, state, nsolve, nchunks,
-                     res);
+  hipLaunchKernelGGL(k_final, dim3(nchunks), dim3(NT), 0, s, solves, nsolve, state, nchunks, res);
   TNX_CHECK_LAUNCH();
   return TNX_OK;
 }
