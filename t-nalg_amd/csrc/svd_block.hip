@@ -126,6 +126,9 @@ __global__ __launch_bounds__(NT) void k_svdj_round(const tnx_svdj_job* __restric
     double* x = Bs + u * n;
     double* y = Bs + v * n;
     double a = 0.0, c = 0.0, g = 0.0;
+    // loops unrolled so that several LDS loads are in flight (same operations
+    // in the same order per element)
+#pragma unroll 4
     for (int i = l; i < n; i += T) {
       const double xi = x[i], yi = y[i];
       a += xi * xi;
@@ -143,6 +146,7 @@ __global__ __launch_bounds__(NT) void k_svdj_round(const tnx_svdj_job* __restric
       const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
       double* wx = Ws + u * n;
       double* wy = Ws + v * n;
+#pragma unroll 4
       for (int i = l; i < n; i += T) {
         const double xi = x[i], yi = y[i];
         x[i] = cs * xi - sn * yi;
